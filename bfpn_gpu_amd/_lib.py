@@ -1,0 +1,253 @@
+"""ctypes binding of libbfpn_hip.so (include/bfpn.h)."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .abi import Case, Deltas, Grid, Phys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+_LIB_FILE = os.path.join(PKG, "libbfpn_hip.so")
+
+FP32 = 0
+FP64 = 1
+
+_lib = None
+
+
+class BfpnError(RuntimeError):
+    """A bfpn_* call returned a non-zero status."""
+
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"bfpn status {status}: {msg}")
+        self.status = status
+
+
+def lib_path() -> str:
+    return _LIB_FILE
+
+
+def lib():
+    """Load libbfpn_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_FILE):
+        raise ImportError(
+            f"{_LIB_FILE} is missing: build it with `python -m bfpn_gpu_amd.build` "
+            "(there is no CPU fallback)")
+    L = C.CDLL(_LIB_FILE)
+    vp, i, d, sz = C.c_void_p, C.c_int, C.c_double, C.c_size_t
+    L.bfpn_last_error.restype = C.c_char_p
+    L.bfpn_version.restype = C.c_char_p
+    L.bfpn_device_count.argtypes = [C.POINTER(C.c_int)]
+    L.bfpn_device_info.argtypes = [i, C.c_char_p, sz, C.POINTER(sz), C.POINTER(C.c_int)]
+    L.bfpn_malloc.argtypes = [C.POINTER(vp), sz]
+    L.bfpn_free.argtypes = [vp]
+    L.bfpn_memcpy_h2d.argtypes = [vp, vp, sz]
+    L.bfpn_memcpy_d2h.argtypes = [vp, vp, sz]
+    L.bfpn_memset.argtypes = [vp, i, sz]
+    L.bfpn_mem_info.argtypes = [C.POINTER(sz), C.POINTER(sz)]
+    L.bfpn_workspace_create.argtypes = [C.POINTER(vp)]
+    L.bfpn_workspace_destroy.argtypes = [vp]
+    for suffix in ("f32", "f64"):
+        getattr(L, f"bfpn_energy_half_{suffix}").argtypes = [vp, vp, vp, vp, vp, vp, d, d, d, i, i, i,
+                                                             vp, vp, i, vp]
+        getattr(L, f"bfpn_secondary_source_{suffix}").argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                                                  i, i, i, d, vp]
+        getattr(L, f"bfpn_transport_half_{suffix}").argtypes = [vp, vp, vp, i, i, i, i, d, d, d, i, vp]
+        getattr(L, f"bfpn_angle_full_{suffix}").argtypes = [vp, vp, vp, i, i, i, i, d, d, d, d, vp]
+        getattr(L, f"bfpn_idd_{suffix}").argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i, i, i, i, d, d,
+                                                     d, d, d, d, i, i, C.POINTER(d), vp]
+    L.bfpn_create.argtypes = [C.POINTER(Grid), C.POINTER(Phys), C.c_char_p, i, C.POINTER(vp)]
+    L.bfpn_destroy.argtypes = [vp]
+    L.bfpn_get_deltas.argtypes = [vp, C.POINTER(Deltas)]
+    L.bfpn_cells.argtypes = [vp]
+    L.bfpn_cells.restype = sz
+    L.bfpn_device_bytes.argtypes = [vp]
+    L.bfpn_device_bytes.restype = sz
+    L.bfpn_num_steps.argtypes = [vp, d]
+    L.bfpn_get_table.argtypes = [vp, i, vp, sz]
+    L.bfpn_advance.argtypes = [vp, i]
+    L.bfpn_history.argtypes = [vp, vp, vp, i, C.POINTER(C.c_int)]
+    L.bfpn_run.argtypes = [vp, d, vp, vp, i, C.POINTER(C.c_int)]
+    L.bfpn_read_state.argtypes = [vp, i, vp, sz]
+    L.bfpn_write_state.argtypes = [vp, i, vp, sz]
+    L.bfpn_stream.argtypes = [vp]
+    L.bfpn_stream.restype = vp
+    L.bfpn_set_profiling.argtypes = [vp, i]
+    L.bfpn_phase_stats.argtypes = [vp, vp, vp, vp]
+    _lib = L
+    return L
+
+
+def check(status: int) -> None:
+    if status != 0:
+        raise BfpnError(status, lib().bfpn_last_error().decode())
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    try:
+        check(lib().bfpn_device_count(C.byref(n)))
+    except BfpnError:
+        return 0
+    return n.value
+
+
+def _addr(a: np.ndarray):
+    return C.c_void_p(a.ctypes.data)
+
+
+class DeviceArray:
+    """A device allocation holding a copy of a numpy array (dtype kept)."""
+
+    def __init__(self, host: np.ndarray | None = None, *, shape=None, dtype=np.float64):
+        if host is not None:
+            host = np.ascontiguousarray(host)
+            shape, dtype = host.shape, host.dtype
+        self.shape = tuple(np.atleast_1d(shape)) if shape is not None else (0,)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p(0)
+        check(lib().bfpn_malloc(C.byref(p), max(self.nbytes, 1)))
+        self.ptr = p
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, host: np.ndarray) -> None:
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        assert host.nbytes == self.nbytes, "size mismatch"
+        check(lib().bfpn_memcpy_h2d(self.ptr, _addr(host), self.nbytes))
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        check(lib().bfpn_memcpy_d2h(_addr(out), self.ptr, self.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr and self.ptr.value:
+            lib().bfpn_free(self.ptr)
+            self.ptr = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Workspace:
+    """Scratch owner for operator-level calls (bfpn_workspace)."""
+
+    def __init__(self):
+        p = C.c_void_p(0)
+        check(lib().bfpn_workspace_create(C.byref(p)))
+        self.ptr = p
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().bfpn_workspace_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+TABLES = ("S", "sigma", "T", "sig_tr", "en", "ker_e1", "ker_e2", "ker_v", "omend", "mu",
+          "ker_beg", "ker_end")
+
+
+class Solver:
+    """One beam: tables, state and depth march on the GPU (bfpn_ctx)."""
+
+    def __init__(self, case: Case, precision: int = FP32):
+        self.case = case
+        self.precision = precision
+        g, p = case.grid(), case.phys()
+        h = C.c_void_p(0)
+        check(lib().bfpn_create(C.byref(g), C.byref(p), case.data.encode(), precision, C.byref(h)))
+        self._h = h
+        d = Deltas()
+        check(lib().bfpn_get_deltas(self._h, C.byref(d)))
+        self.deltas = d
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().bfpn_destroy(self._h)
+            self._h = C.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def cells(self) -> int:
+        return lib().bfpn_cells(self._h)
+
+    @property
+    def device_bytes(self) -> int:
+        return lib().bfpn_device_bytes(self._h)
+
+    def num_steps(self, t_final=None) -> int:
+        return lib().bfpn_num_steps(self._h, self.case.time if t_final is None else t_final)
+
+    def table(self, which) -> np.ndarray:
+        if isinstance(which, str):
+            which = TABLES.index(which)
+        c = self.case
+        sizes = [c.Ng + 1, c.Ng, c.Ng, c.Ng, c.Ng + 1, c.Ng * c.Ng, c.Ng * c.Ng,
+                 c.Ng * c.nang * c.nang, 2 * c.nang, 3 * c.Ng, c.Ng, c.Ng]
+        out = np.zeros(sizes[which])
+        check(lib().bfpn_get_table(self._h, which, _addr(out), out.size))
+        return out
+
+    def advance(self, n: int = 1) -> None:
+        check(lib().bfpn_advance(self._h, n))
+
+    def history(self):
+        n = C.c_int(0)
+        check(lib().bfpn_history(self._h, None, None, 0, C.byref(n)))
+        idd = np.zeros(max(n.value, 1))
+        px = np.zeros(max(n.value, 1))
+        check(lib().bfpn_history(self._h, _addr(idd), _addr(px), n.value, C.byref(n)))
+        return idd[: n.value], px[: n.value]
+
+    def run(self, t_final=None):
+        self.advance(self.num_steps(t_final))
+        return self.history()
+
+    def state(self, which: int) -> np.ndarray:
+        out = np.zeros(self.cells)
+        check(lib().bfpn_read_state(self._h, which, _addr(out), out.size))
+        return out
+
+    def set_state(self, which: int, values: np.ndarray) -> None:
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        check(lib().bfpn_write_state(self._h, which, _addr(v), v.size))
+
+    def set_profiling(self, on: bool = True) -> None:
+        check(lib().bfpn_set_profiling(self._h, int(on)))
+
+    PHASES = ("energy", "transport", "angle", "source", "dose")
+
+    def phase_stats(self) -> dict:
+        """{phase: (seconds, calls, algorithmic bytes)} since profiling was turned on."""
+        sec = np.zeros(5)
+        calls = np.zeros(5, np.int32)
+        byt = np.zeros(5)
+        check(lib().bfpn_phase_stats(self._h, _addr(sec), _addr(calls), _addr(byt)))
+        return {p: (float(sec[k]), int(calls[k]), float(byt[k])) for k, p in enumerate(self.PHASES)}
+
+    @property
+    def stream(self):
+        return lib().bfpn_stream(self._h)

@@ -1,0 +1,295 @@
+// bfp_solver — drop-in command line front end of the MI355X BFPn solver.
+//
+// Reference: src/main.cu:44-356.  Same executable name, flags, defaults,
+// silent ignoring of unknown flags, positional Nx anywhere on the line,
+// material presets that `--rho` later overrides, `Error: <what>` + rc 1 on
+// failure, and the same output files in the working directory:
+//   idd_output.txt   "# x_cm value" then (i+1)*dt and the IDD, 17 digits
+//   dose_output.txt  same format, dg * sum psi^2 (primary)
+// (BFPnSolver::saveResults, src/bfpm_solver.cu:3808-3815).
+//
+// Opt-in additions: --precision fp32|fp64, --lg <MeV span>.
+// The out-of-core streaming flags are accepted and run in core (the whole
+// phase space stays resident in HBM).
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "bfpn.h"
+
+namespace {
+
+void print_devices() {
+    int n = 0;
+    if (bfpn_device_count(&n) != BFPN_OK) {
+        std::cout << "No HIP device: " << bfpn_last_error() << std::endl;
+        return;
+    }
+    std::cout << "Found " << n << " HIP device(s)" << std::endl;
+    for (int i = 0; i < n; ++i) {
+        char name[256] = {0};
+        size_t total = 0;
+        int cus = 0;
+        if (bfpn_device_info(i, name, sizeof(name), &total, &cus) != BFPN_OK) continue;
+        std::cout << "\nDevice " << i << ": " << name << std::endl;
+        std::cout << "  Total Memory: " << total / (1024 * 1024) << " MB" << std::endl;
+        std::cout << "  Compute Units: " << cus << std::endl;
+    }
+    std::cout << std::endl;
+}
+
+void save_series(const std::vector<double>& v, double dt, const std::string& path) {
+    std::ofstream out(path);
+    out << std::setprecision(17);
+    out << "# x_cm value\n";
+    for (size_t i = 0; i < v.size(); ++i) out << (i + 1) * dt << " " << v[i] << "\n";
+}
+
+void usage(const char* argv0) {
+    std::cout << "Usage: " << argv0 << " [options]\n"
+              << "Options:\n"
+              << "  <Nx>             Optional x grid points (default: 4000)\n"
+              << "  --data <path>    Data file directory (default: ./data)\n"
+              << "  --time <value>   Final simulation time (default: 40.0)\n"
+              << "  --energy <MeV>   Incoming beam energy E0 (default: 230.0)\n"
+              << "  --sigma-e <MeV>  Initial energy Gaussian sigma (default: 1.0)\n"
+              << "  --material <name> Material density preset: water, bone, air\n"
+              << "  --rho <value>    Override material density in g/cm^3\n"
+              << "  --ny/--nz <n>    Y/Z grid cells (nodes are n+1)\n"
+              << "  --ly/--lz <cm>   Y/Z domain width (default: 1)\n"
+              << "  --lu/--lv <v>    U/V angular domain width (default: 1)\n"
+              << "  --ng <n>         Energy groups (default: 500)\n"
+              << "  --nmu/--nom <n>  Angular grid points (default: 11)\n"
+              << "  --primary-only   Skip secondary proton evolution/source\n"
+              << "  --energy-model eq15   Energy update model (eq15 only)\n"
+              << "  --eq15-straggling     Enable Eq.15 straggling terms\n"
+              << "  --energy-only    Only advance the primary energy subsystem\n"
+              << "  --no-transport   Skip the spatial transport subsystem (primary)\n"
+              << "  --no-angle       Skip the angular diffusion subsystem (primary)\n"
+              << "  --no-spatial-clipping Disable positivity clipping of primary psi^1\n"
+              << "  --profile-steps  Print energy/transport/angle step timing summary\n"
+              << "  --discrete-angle-delta Quadrature-normalized mono-directional beam\n"
+              << "  --normalize-initial-mass Normalize the discrete beam mass\n"
+              << "  --trapezoidal-yz Trapezoidal y/z weights in normalization and IDD\n"
+              << "  --dose-catastrophic-loss Include the Eq.24 E*sigma_c*psi term (default)\n"
+              << "  --stopping-power-dose-only Omit the Eq.24 catastrophic-loss term\n"
+              << "  --no-cross-section-calibration Use fitted sigma_c (default)\n"
+              << "  --legacy-cross-section-calibration Apply the legacy sigma_c correction\n"
+              << "  --sigma-yz <cm>  Initial transverse Gaussian sigma (default: 0.1)\n"
+              << "  --streaming-full Accepted; the state always stays resident in HBM\n"
+              << "  --precision <fp32|fp64> State precision (default: fp32)\n"
+              << "  --lg <MeV>       Energy span of the group grid [1, 1+Lg] (default: 259)\n"
+              << "  --test           Print device info and exit\n"
+              << "  --help           Show this help\n";
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    std::string data_path = "./data";
+    bool data_set = false;
+    std::string material;
+    double t_final = 40.0;
+    bool test_mode = false;
+    int nx = 4000, ny = 10, nz = 10, ng = 500, nmu = 11, nom = 11;
+    double ly = 1.0, lz = 1.0, lu = 1.0, lv = 1.0, lg = 259.0;
+    double energy = 230.0, sigma_e = 1.0, sigma_yz = 0.1, rho = 1.0, rho_e = 1.0;
+    bool lite = false, streaming = false, legacy_energy = false, profile = false;
+    int precision = BFPN_FP32;
+    bfpn_phys phys{};
+    phys.dose_catastrophic_loss = 1;
+
+    try {
+        for (int i = 1; i < argc; ++i) {
+            const std::string a = argv[i];
+            const bool has = i + 1 < argc;
+            if (a == "--data" && has) { data_path = argv[++i]; data_set = true; }
+            else if (a == "--time" && has) t_final = std::stod(argv[++i]);
+            else if (a == "--energy" && has) energy = std::stod(argv[++i]);
+            else if (a == "--sigma-e" && has) sigma_e = std::stod(argv[++i]);
+            else if (a == "--rho" && has) { rho = std::stod(argv[++i]); rho_e = rho; }
+            else if (a == "--ny" && has) ny = std::stoi(argv[++i]);
+            else if (a == "--nz" && has) nz = std::stoi(argv[++i]);
+            else if (a == "--ly" && has) ly = std::stod(argv[++i]);
+            else if (a == "--lz" && has) lz = std::stod(argv[++i]);
+            else if (a == "--lu" && has) lu = std::stod(argv[++i]);
+            else if (a == "--lv" && has) lv = std::stod(argv[++i]);
+            else if (a == "--ng" && has) ng = std::stoi(argv[++i]);
+            else if (a == "--nmu" && has) nmu = std::stoi(argv[++i]);
+            else if (a == "--nom" && has) nom = std::stoi(argv[++i]);
+            else if (a == "--lg" && has) lg = std::stod(argv[++i]);
+            else if (a == "--material" && has) {
+                material = argv[++i];
+                if (material == "water") rho = rho_e = 1.0;
+                else if (material == "bone") rho = rho_e = 1.757;
+                else if (material == "air") rho = rho_e = 0.001205;
+                else {
+                    std::cerr << "Unknown material: " << material << std::endl;
+                    return 1;
+                }
+            } else if (a == "--test") test_mode = true;
+            else if (a == "--lite-memory") lite = true;
+            else if (a == "--streaming-full") streaming = true;
+            else if (a == "--primary-only") phys.primary_only = 1;
+            else if (a == "--energy-model" && has) {
+                const std::string m = argv[++i];
+                if (m == "legacy") legacy_energy = true;
+                else if (m == "eq15") legacy_energy = false;
+                else {
+                    std::cerr << "Unknown energy model: " << m << std::endl;
+                    return 1;
+                }
+            } else if (a == "--eq15-straggling") phys.eq15_straggling = 1;
+            else if (a == "--save-energy-moments") phys.save_energy_moments = 1;
+            else if (a == "--energy-only") phys.energy_only = 1;
+            else if (a == "--no-transport") phys.no_transport = 1;
+            else if (a == "--no-angle") phys.no_angle = 1;
+            else if (a == "--no-spatial-clipping") phys.no_spatial_clipping = 1;
+            else if (a == "--profile-steps") profile = true;
+            else if (a == "--discrete-angle-delta") phys.discrete_angle_delta = 1;
+            else if (a == "--normalize-initial-mass") phys.normalize_initial_mass = 1;
+            else if (a == "--trapezoidal-yz") phys.trapezoidal_yz = 1;
+            else if (a == "--dose-catastrophic-loss") phys.dose_catastrophic_loss = 1;
+            else if (a == "--stopping-power-dose-only") phys.dose_catastrophic_loss = 0;
+            else if (a == "--no-cross-section-calibration") phys.calibrate_cross_sections = 0;
+            else if (a == "--legacy-cross-section-calibration") phys.calibrate_cross_sections = 1;
+            else if (a == "--sigma-yz" && has) sigma_yz = std::stod(argv[++i]);
+            else if (a == "--precision" && has) {
+                const std::string p = argv[++i];
+                if (p == "fp32") precision = BFPN_FP32;
+                else if (p == "fp64") precision = BFPN_FP64;
+                else {
+                    std::cerr << "Unknown precision: " << p << std::endl;
+                    return 1;
+                }
+            } else if ((a == "--lane-chunk" || a == "--energy-chunk" || a == "--stream-dir" ||
+                        a == "--idd-stride" || a == "--spot-depths" || a == "--spot-prefix") && has)
+                ++i;  // streaming / diagnostics knobs: accepted, no effect in core
+            else if (a == "--help") {
+                usage(argv[0]);
+                return 0;
+            } else if (!a.empty() && a[0] != '-')
+                nx = std::stoi(a);
+        }
+    } catch (const std::exception& e) {
+        std::cerr << "Error: " << e.what() << std::endl;
+        return 1;
+    }
+
+    if (!data_set && !material.empty()) data_path = "BFPn_CPU_Solver/" + material;
+    if (lite && streaming) {
+        std::cerr << "--lite-memory and --streaming-full are mutually exclusive" << std::endl;
+        return 1;
+    }
+    print_devices();
+    if (test_mode) {
+        std::cout << "Running in test mode..." << std::endl;
+        return 0;
+    }
+    if (lite) {
+        std::cerr << "Error: --lite-memory (primary-only energy toy) is not provided; "
+                     "the full eq15 state fits in HBM"
+                  << std::endl;
+        return 1;
+    }
+    if (legacy_energy) {
+        std::cerr << "Error: --energy-model legacy is not provided; use eq15" << std::endl;
+        return 1;
+    }
+
+    bfpn_grid grid{nx, ny, nz, ng, nmu, nom, 40.0, ly, lz, lu, lv, lg};
+    phys.sig_E = sigma_e;
+    phys.a_1 = 1.0 / 2.0 / sigma_yz / sigma_yz;
+    phys.a_2 = 1.0 / 2.0 / 1e-6 / 1e-6;
+    phys.beam_energy = energy;
+    phys.density = rho;
+    phys.energy_density = rho_e;
+    std::snprintf(phys.material_name, sizeof(phys.material_name), "%s", material.c_str());
+
+    std::cout << "Domain widths: Lx=" << grid.Lx << ", Ly=" << ly << ", Lz=" << lz << ", Lu=" << lu
+              << ", Lv=" << lv << std::endl;
+    std::cout << "Initial beam sigmas: yz=" << sigma_yz << " cm, E=" << sigma_e
+              << " MeV, angular=1e-6" << std::endl;
+    std::cout << "Initial quadrature: angle="
+              << (phys.discrete_angle_delta ? "discrete-delta" : "sampled-gaussian")
+              << ", mass normalization=" << (phys.normalize_initial_mass ? "on" : "off")
+              << ", y/z trapezoidal weights=" << (phys.trapezoidal_yz ? "on" : "off") << std::endl;
+    std::cout << "Dose catastrophic-loss term: "
+              << (phys.dose_catastrophic_loss ? "enabled" : "disabled") << std::endl;
+    std::cout << "Cross-section calibration: "
+              << (phys.calibrate_cross_sections ? "legacy empirical correction" : "disabled")
+              << std::endl;
+    std::cout << "Spatial positivity clipping: "
+              << (phys.no_spatial_clipping ? "disabled" : "enabled") << std::endl;
+    std::cout << "State precision: " << (precision == BFPN_FP64 ? "fp64" : "fp32")
+              << (streaming ? " (streaming flags ignored: in-core)" : "") << std::endl;
+
+    bfpn_ctx* ctx = nullptr;
+    if (bfpn_create(&grid, &phys, data_path.c_str(), precision, &ctx) != BFPN_OK) {
+        std::cerr << "Error: " << bfpn_last_error() << std::endl;
+        return 1;
+    }
+    bfpn_deltas d{};
+    bfpn_get_deltas(ctx, &d);
+    const int steps = bfpn_num_steps(ctx, t_final);
+    std::cout << "Memory: " << std::fixed << std::setprecision(2)
+              << bfpn_device_bytes(ctx) / (1024.0 * 1024.0 * 1024.0) << " GiB resident"
+              << std::defaultfloat << std::setprecision(6) << std::endl;
+    std::cout << "Starting BFPn solver..." << std::endl;
+    std::cout << "Grid: " << nx << "x" << ny << "x" << nz << ", Energy: " << ng << ", Angle: " << nmu
+              << "x" << nom << std::endl;
+    if (profile) bfpn_set_profiling(ctx, 1);
+
+    std::vector<double> idd(steps), proxy(steps);
+    double t = 0.0;
+    int done = 0;
+    while (done < steps) {
+        const int chunk = std::min(100 - done % 100, steps - done);
+        if (bfpn_advance(ctx, chunk) != BFPN_OK) {
+            std::cerr << "Error: " << bfpn_last_error() << std::endl;
+            bfpn_destroy(ctx);
+            return 1;
+        }
+        for (int k = 0; k < chunk; ++k) t += d.dt;
+        done += chunk;
+        if (done % 100 == 0) {
+            int n = 0;
+            if (bfpn_history(ctx, idd.data(), proxy.data(), steps, &n) != BFPN_OK) {
+                std::cerr << "Error: " << bfpn_last_error() << std::endl;
+                bfpn_destroy(ctx);
+                return 1;
+            }
+            std::cout << "Step " << done << ", t = " << t << "/" << t_final
+                      << ", IDD = " << idd[done - 1] << std::endl;
+        }
+    }
+    int n = 0;
+    if (bfpn_history(ctx, idd.data(), proxy.data(), steps, &n) != BFPN_OK) {
+        std::cerr << "Error: " << bfpn_last_error() << std::endl;
+        bfpn_destroy(ctx);
+        return 1;
+    }
+    save_series(idd, d.dt, "idd_output.txt");
+    save_series(proxy, d.dt, "dose_output.txt");
+    std::cout << "Solver completed. Paper-style IDD saved to idd_output.txt" << std::endl;
+    std::cout << "Diagnostic scalar proxy saved to dose_output.txt" << std::endl;
+    if (profile) {
+        double ph[5];
+        bfpn_phase_stats(ctx, ph, nullptr, nullptr);
+        const double tot = ph[0] + ph[1] + ph[2] + ph[3] + ph[4];
+        const char* names[5] = {"energy", "transport", "angle", "source", "diagnostics"};
+        std::cout << "Step profile (" << steps << " steps, device time):" << std::endl;
+        for (int k = 0; k < 5; ++k)
+            std::cout << "  " << names[k] << ": " << ph[k] << " s ("
+                      << (tot > 0 ? 100.0 * ph[k] / tot : 0.0) << "%)" << std::endl;
+    }
+    bfpn_destroy(ctx);
+    return 0;
+}

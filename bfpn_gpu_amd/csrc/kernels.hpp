@@ -1,0 +1,62 @@
+// kernels.hpp — launch wrappers of the HIP kernels (all stream-ordered).
+#pragma once
+
+#include "common.hpp"
+
+namespace bfpn {
+
+// Eq.15 energy half step, in place.  coef: Ng device EnergyCoef.  Q1 may be
+// null.  When dose != null each block writes FP64 partials:
+//   dose[block]        = sum w_yz(s) (F*w1[g] + f*w2[g])  over its lanes
+//   dose[nblk + block] = sum f                           (proxy), if proxy
+// The lanes handled by a block are [block*256, block*256+256).
+template <typename T>
+void launch_energy_half(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng, long long lanes,
+                        bool straggling, const DoseCoef* dose_coef, const double* yz_weight,
+                        int nang, double* dose_partials, bool proxy, hipStream_t s);
+int energy_blocks(long long lanes);
+
+// Secondary source: A1/A2 angular contraction then the banded energy sum.
+template <typename T>
+void launch_source_angular(const T* Fp, const T* fp, const T* kv, T* A1, T* A2, int Ng, int nyz,
+                           int nang, hipStream_t s);
+template <typename T>
+void launch_source_energy(const T* A1, const T* A2, const double* e1, const double* e2,
+                          const int* beg, const int* end, T* Q, int Ng, long long lanes, double dg,
+                          hipStream_t s);
+
+// MUSCL transport half step: out = PC(in).  pred is scratch; out != in.
+template <typename T>
+void launch_transport_half(const T* in, T* pred, T* out, const T* omend, int Ny, int Nz, int Ng,
+                           int nang, double dy, double dz, double h, bool preserve_sign,
+                           hipStream_t s);
+
+// CN angular diffusion on the DST-I basis, in place.  factors: Ng*nang (T),
+// su: Nom*Nom, sv: Nmu*Nmu (T).
+template <typename T>
+void launch_angle_full(T* F, const T* factors, const T* su, const T* sv, int Ng, int nyz, int Nmu,
+                       int Nom, hipStream_t s);
+
+// Eq.24 dose of (F+F1, f+f1) into per-block partials (count = dose_blocks()).
+template <typename T>
+void launch_dose(const T* F, const T* f, const T* F1, const T* f1, const DoseCoef* w,
+                 const double* yz_weight, int Ng, long long lanes, int nang, double* partials,
+                 hipStream_t s);
+int dose_blocks(int Ng, long long lanes);
+
+// out[slot] = scale * sum(a[0:na]) + scale * sum(b[0:nb])  (b may be null), one block.
+void launch_finish_sum(const double* a, int na, const double* b, int nb, double scale, double* out,
+                       hipStream_t s);
+
+// F[g][s][m] = space_angle[s*nang+m] * energy[g]
+template <typename T>
+void launch_beam(T* F, const double* space_angle, const double* energy, int Ng, long long lanes,
+                 hipStream_t s);
+
+}  // namespace bfpn
+
+namespace bfpn {
+// dst[k] = (T) src[k], k < n (device to device)
+template <typename T>
+void launch_convert(const double* src, T* dst, long long n, hipStream_t s);
+}  // namespace bfpn

@@ -1,0 +1,145 @@
+// misc.hip — dose reduction, deterministic final sums and beam initialisation.
+//
+// Reference: integratedDepthDoseKernel + reduceSumKernel + the 8-byte D2H per
+// step (src/bfpm_solver.cu:199-309, 2132-2164) and the host-built Eq.23 beam
+// copied H2D at init (src/bfpm_solver.cu:1978-2020).
+//
+// Here the per-step IDD normally comes out of the fused energy kernel; this
+// stand-alone dose kernel serves the operator-level API.  Final sums run in
+// one block in a fixed order, so results never depend on scheduling, and the
+// IDD history stays on the device until the caller asks for it.
+
+#include "kernels.hpp"
+
+namespace bfpn {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ inline double block_sum256(double v, double* lds) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kBlock / 64; ++w) t += lds[w];
+    return t;
+}
+
+// grid: (ceil(lanes/256)); each thread walks its lane through all groups.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void dose_kernel(const T* __restrict__ F,
+                                                      const T* __restrict__ f,
+                                                      const T* __restrict__ F1,
+                                                      const T* __restrict__ f1,
+                                                      const DoseCoef* __restrict__ w,
+                                                      const double* __restrict__ yzw, int Ng,
+                                                      long long lanes, int nang,
+                                                      double* __restrict__ partials) {
+    __shared__ double lds[kBlock / 64];
+    const long long lane = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    double acc = 0.0;
+    if (lane < lanes) {
+        for (int g = 0; g < Ng; ++g) {
+            const long long idx = g * lanes + lane;
+            double p1 = static_cast<double>(F[idx]);
+            double p2 = f ? static_cast<double>(f[idx]) : 0.0;
+            if (F1) p1 += static_cast<double>(F1[idx]);
+            if (f1) p2 += static_cast<double>(f1[idx]);
+            acc += p1 * w[g].w1 + p2 * w[g].w2;
+        }
+        acc *= yzw[lane / nang];
+    }
+    const double t = block_sum256(acc, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(1024) void finish_sum_kernel(const double* __restrict__ a, int na,
+                                                          const double* __restrict__ b, int nb,
+                                                          double scale, double* __restrict__ out) {
+    __shared__ double lds[1024];
+    double s = 0.0;
+    for (int k = threadIdx.x; k < na; k += 1024) s += a[k];
+    if (b)
+        for (int k = threadIdx.x; k < nb; k += 1024) s += b[k];
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = lds[0] * scale;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void beam_kernel(T* __restrict__ F,
+                                                      const double* __restrict__ space_angle,
+                                                      const double* __restrict__ energy,
+                                                      long long lanes) {
+    const long long lane = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (lane >= lanes) return;
+    const double sa = space_angle[lane];
+    const double e = energy[blockIdx.y];
+    F[static_cast<long long>(blockIdx.y) * lanes + lane] = static_cast<T>(sa * e);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void convert_kernel(const double* __restrict__ src,
+                                                         T* __restrict__ dst, long long n) {
+    const long long k = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (k < n) dst[k] = static_cast<T>(src[k]);
+}
+
+}  // namespace
+
+template <typename T>
+void launch_convert(const double* src, T* dst, long long n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(convert_kernel<T>, dim3(static_cast<unsigned>((n + kBlock - 1) / kBlock)),
+                       dim3(kBlock), 0, s, src, dst, n);
+    BFPN_HIP(hipGetLastError());
+}
+template void launch_convert<float>(const double*, float*, long long, hipStream_t);
+template void launch_convert<double>(const double*, double*, long long, hipStream_t);
+
+int dose_blocks(int Ng, long long lanes) {
+    (void)Ng;
+    return static_cast<int>((lanes + kBlock - 1) / kBlock);
+}
+
+template <typename T>
+void launch_dose(const T* F, const T* f, const T* F1, const T* f1, const DoseCoef* w,
+                 const double* yz_weight, int Ng, long long lanes, int nang, double* partials,
+                 hipStream_t s) {
+    if (lanes <= 0) return;
+    hipLaunchKernelGGL(dose_kernel<T>, dim3(dose_blocks(Ng, lanes)), dim3(kBlock), 0, s, F, f, F1,
+                       f1, w, yz_weight, Ng, lanes, nang, partials);
+    BFPN_HIP(hipGetLastError());
+}
+
+void launch_finish_sum(const double* a, int na, const double* b, int nb, double scale, double* out,
+                       hipStream_t s) {
+    hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(1024), 0, s, a, na, b, nb, scale, out);
+    BFPN_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_beam(T* F, const double* space_angle, const double* energy, int Ng, long long lanes,
+                 hipStream_t s) {
+    if (lanes <= 0 || Ng <= 0) return;
+    dim3 grid(static_cast<unsigned>((lanes + kBlock - 1) / kBlock), Ng);
+    hipLaunchKernelGGL(beam_kernel<T>, grid, dim3(kBlock), 0, s, F, space_angle, energy, lanes);
+    BFPN_HIP(hipGetLastError());
+}
+
+template void launch_dose<float>(const float*, const float*, const float*, const float*,
+                                 const DoseCoef*, const double*, int, long long, int, double*,
+                                 hipStream_t);
+template void launch_dose<double>(const double*, const double*, const double*, const double*,
+                                  const DoseCoef*, const double*, int, long long, int, double*,
+                                  hipStream_t);
+template void launch_beam<float>(float*, const double*, const double*, int, long long, hipStream_t);
+template void launch_beam<double>(double*, const double*, const double*, int, long long,
+                                  hipStream_t);
+
+}  // namespace bfpn

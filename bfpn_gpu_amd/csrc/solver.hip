@@ -1,0 +1,774 @@
+// solver.hip — per-beam context, the Strang depth march and the C ABI.
+//
+// Reference orchestration: BFPnSolver (src/bfpm_solver.cu:1171-1568,
+// 2166-2267, 3238-3502).  Its loop body per depth step is
+//   stepPrimary:   E(dt/2) -> T(dt/2) -> A(dt) -> T(dt/2) -> E(dt/2)     on (F, f)
+//   stepSecondary: the same order on (F1, f1) with the source from the primary
+//                  (only when step > 0 and the secondary is enabled)
+//   IDD (Eq.24) of F+F1, f+f1 and the proxy dg*sum(f), recorded at x=(k+1)dt.
+// with a host sync after every operator and two scalar D2H copies per step.
+//
+// Here the whole step is stream-ordered on one HIP stream with no host sync:
+// transport ping-pongs between the state buffer and a spare, the IDD and
+// proxy are accumulated inside the final energy half steps and reduced into
+// a device history array, and the single source evaluation per step feeds
+// both secondary energy half steps (the primary is unchanged in between, so
+// the reference's two evaluations are identical).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "tables.hpp"
+
+namespace bfpn {
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+// ------------------------------------------------------------ device memory
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { release(); }
+    void release() {
+        if (ptr_) (void)hipFree(ptr_);
+        ptr_ = nullptr;
+        bytes_ = 0;
+    }
+    void ensure(size_t bytes) {
+        if (bytes <= bytes_) return;
+        release();
+        if (bytes == 0) return;
+        hipError_t e = hipMalloc(&ptr_, bytes);
+        if (e != hipSuccess) {
+            ptr_ = nullptr;
+            (void)hipGetLastError();
+            throw Error(BFPN_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                                          hipGetErrorString(e));
+        }
+        bytes_ = bytes;
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(ptr_);
+    }
+    size_t bytes() const { return bytes_; }
+    void swap(DeviceBuffer& o) {
+        std::swap(ptr_, o.ptr_);
+        std::swap(bytes_, o.bytes_);
+    }
+
+private:
+    void* ptr_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+// Host -> device copy of a (usually temporary) host vector.  The copy is
+// complete when this returns; callers order it against `s` explicitly.
+template <typename T>
+void upload(DeviceBuffer& b, const std::vector<T>& host, hipStream_t s) {
+    (void)s;
+    b.ensure(host.size() * sizeof(T));
+    if (!host.empty())
+        BFPN_HIP(hipMemcpy(b.as<void>(), host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+template <typename T>
+std::vector<T> cast_vector(const std::vector<double>& v) {
+    return std::vector<T>(v.begin(), v.end());
+}
+
+std::vector<double> yz_weights(int Ny, int Nz, bool trapezoidal) {
+    std::vector<double> w(static_cast<size_t>(Ny + 1) * (Nz + 1), 1.0);
+    if (trapezoidal)
+        for (int iz = 0; iz <= Nz; ++iz)
+            for (int iy = 0; iy <= Ny; ++iy) {
+                double v = 1.0;
+                if (iy == 0 || iy == Ny) v *= 0.5;
+                if (iz == 0 || iz == Nz) v *= 0.5;
+                w[static_cast<size_t>(iz) * (Ny + 1) + iy] = v;
+            }
+    return w;
+}
+
+}  // namespace bfpn
+
+using namespace bfpn;
+
+// ============================================================== context ===
+struct bfpn_ctx {
+    virtual ~bfpn_ctx() = default;
+    virtual void advance(int nsteps) = 0;
+    virtual void history(double* idd, double* proxy, int cap, int* n) = 0;
+    virtual void read_state(int which, double* host, size_t count) = 0;
+    virtual void write_state(int which, const double* host, size_t count) = 0;
+    virtual size_t device_bytes() const = 0;
+
+    bfpn_grid grid{};
+    bfpn_phys phys{};
+    bfpn_deltas d{};
+    int nyz = 0, nang = 0;
+    long long lanes = 0;
+    size_t cells = 0;
+    hipStream_t stream = nullptr;
+    bool profiling = false;
+    double phase_seconds[5] = {0, 0, 0, 0, 0};
+    double phase_bytes[5] = {0, 0, 0, 0, 0};
+    int phase_calls[5] = {0, 0, 0, 0, 0};
+    // host copies of the tables (FP64)
+    MaterialTables mat;
+    std::vector<double> S, Tc, sig_tr, en, om;
+    SourceKernels ker;
+};
+
+namespace {
+
+enum Phase { kEnergy = 0, kTransport = 1, kAngle = 2, kSource = 3, kDose = 4 };
+
+template <typename T>
+class Context final : public bfpn_ctx {
+public:
+    Context(const bfpn_grid& g, const bfpn_phys& p, const std::string& data_path) {
+        grid = g;
+        phys = p;
+        validate();
+        d = compute_deltas(g);
+        nyz = (g.Ny + 1) * (g.Nz + 1);
+        nang = g.Nmu * g.Nom;
+        lanes = static_cast<long long>(nyz) * nang;
+        cells = static_cast<size_t>(g.Ng) * lanes;
+        secondary_ = !p.primary_only && !p.energy_only;
+
+        // ---- host tables (initialize, src/bfpm_solver.cu:1435-1568)
+        mat = load_material_tables(data_path, g.Ng, p.calibrate_cross_sections != 0);
+        const std::string material = p.material_name;
+        S = stopping_power_table(material, g.Ng, g.Lg, d.dg);
+        Tc = straggling_table(g.Ng);
+        sig_tr = transport_xs_table(g.Ng, d.dg);
+        en = energy_grid(g.Ng, d.dg);
+        om = direction_table(g, d);
+        ker = build_source_kernels(g.Ng, d.dg, nang, mat.mu, mat.sigma, om);
+        const BeamFactors beam = beam_factors(g, p, d, om);
+
+        preflight();
+        BFPN_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+
+        const size_t state_bytes = cells * sizeof(T);
+        for (auto& b : state_) b.ensure(state_bytes);
+        spare_.ensure(state_bytes);
+        pred_.ensure(state_bytes);
+        if (secondary_) {
+            a1_.ensure(state_bytes);
+            a2_.ensure(state_bytes);
+            q_.ensure(state_bytes);
+        }
+        for (int k = 0; k < 4; ++k) cur_[k] = state_[k].as<T>();
+        spare_ptr_ = spare_.as<T>();
+
+        upload(coef_, energy_coefs(S, mat.sigma, Tc.data(), p.eq15_straggling != 0, 0.5 * d.dt, d.dg,
+                                   p.energy_density, g.Ng),
+               stream);
+        upload(dose_w_, dose_coefs(S, mat.sigma, en, d.dg, p.dose_catastrophic_loss != 0, g.Ng),
+               stream);
+        upload(yzw_, yz_weights(g.Ny, g.Nz, p.trapezoidal_yz != 0), stream);
+        upload(om_, cast_vector<T>(om), stream);
+        upload(factors_, cast_vector<T>(angle_factors(sig_tr.data(), g.Ng, g.Nmu, g.Nom, d.du, d.dv,
+                                                      d.dt, p.density)),
+               stream);
+        upload(su_, cast_vector<T>(sine_basis(g.Nom)), stream);
+        upload(sv_, cast_vector<T>(sine_basis(g.Nmu)), stream);
+        if (secondary_) {
+            upload(e1_, ker.e1, stream);
+            upload(e2_, ker.e2, stream);
+            upload(beg_, ker.beg, stream);
+            upload(end_, ker.end, stream);
+            upload(kv_, cast_vector<T>(ker.v), stream);
+        }
+        const int eb = energy_blocks(lanes);
+        partial_p_.ensure(2 * eb * sizeof(double));
+        partial_s_.ensure(eb * sizeof(double));
+        grow_history(std::max(16, g.Nx));
+
+        // Eq.23 beam on the device (the reference builds a full host array)
+        DeviceBuffer sa, ee;
+        upload(sa, beam.space_angle, stream);
+        upload(ee, beam.energy, stream);
+        launch_beam<T>(cur_[0], sa.as<double>(), ee.as<double>(), g.Ng, lanes, stream);
+        for (int k = 1; k < 4; ++k) BFPN_HIP(hipMemsetAsync(cur_[k], 0, state_bytes, stream));
+        BFPN_HIP(hipStreamSynchronize(stream));
+    }
+
+    ~Context() override {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
+
+    size_t device_bytes() const override {
+        size_t b = spare_.bytes() + pred_.bytes() + a1_.bytes() + a2_.bytes() + q_.bytes() +
+                   kv_.bytes() + e1_.bytes() + e2_.bytes() + hist_.bytes();
+        for (const auto& s : state_) b += s.bytes();
+        return b;
+    }
+
+    void advance(int nsteps) override {
+        BFPN_REQUIRE(nsteps >= 0, "nsteps must be >= 0");
+        if (nsteps == 0) return;
+        if (step_ + nsteps > hist_cap_) grow_history(std::max(2 * hist_cap_, step_ + nsteps));
+        events_.clear();
+        for (int k = 0; k < nsteps; ++k) step();
+        if (profiling) collect_profile();
+    }
+
+    void history(double* idd, double* proxy, int cap, int* n) override {
+        BFPN_HIP(hipStreamSynchronize(stream));
+        const int cnt = std::min(cap, step_);
+        if (cnt > 0) {
+            std::vector<double> h(2 * static_cast<size_t>(hist_cap_));
+            BFPN_HIP(hipMemcpy(h.data(), hist_.as<double>(), h.size() * sizeof(double),
+                               hipMemcpyDeviceToHost));
+            for (int k = 0; k < cnt; ++k) {
+                if (idd) idd[k] = h[k];
+                if (proxy) proxy[k] = h[hist_cap_ + k];
+            }
+        }
+        if (n) *n = step_;
+    }
+
+    void read_state(int which, double* host, size_t count) override {
+        BFPN_REQUIRE(which >= 0 && which < 4, "state index must be 0..3");
+        BFPN_REQUIRE(count == cells, "state count mismatch");
+        BFPN_HIP(hipStreamSynchronize(stream));
+        std::vector<T> tmp(cells);
+        BFPN_HIP(hipMemcpy(tmp.data(), cur_[which], cells * sizeof(T), hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < cells; ++k) host[k] = static_cast<double>(tmp[k]);
+    }
+
+    void write_state(int which, const double* host, size_t count) override {
+        BFPN_REQUIRE(which >= 0 && which < 4, "state index must be 0..3");
+        BFPN_REQUIRE(count == cells, "state count mismatch");
+        std::vector<T> tmp(host, host + cells);
+        BFPN_HIP(hipStreamSynchronize(stream));
+        BFPN_HIP(hipMemcpy(cur_[which], tmp.data(), cells * sizeof(T), hipMemcpyHostToDevice));
+    }
+
+private:
+    void validate() const {
+        const auto& g = grid;
+        BFPN_REQUIRE(g.Nx > 0 && g.Ny > 0 && g.Nz > 0 && g.Ng > 1 && g.Nmu > 0 && g.Nom > 0,
+                     "grid sizes must be positive (Ng > 1)");
+        BFPN_REQUIRE(g.Nmu * g.Nom <= 1024, "Nmu*Nom must be <= 1024");
+        BFPN_REQUIRE(g.Lx > 0 && g.Ly > 0 && g.Lz > 0 && g.Lu > 0 && g.Lv > 0 && g.Lg > 0,
+                     "domain lengths must be positive");
+        BFPN_REQUIRE(phys.density > 0 && phys.energy_density > 0, "density must be positive");
+        BFPN_REQUIRE(phys.sig_E > 0, "sigma-e must be positive");
+    }
+
+    // Memory preflight (preflightMemory, src/bfpm_solver.cu:1321-1433), for
+    // this layout: 6 state-sized arrays (+3 with the secondary source) and Kv.
+    void preflight() const {
+        size_t free_b = 0, total_b = 0;
+        BFPN_HIP(hipMemGetInfo(&free_b, &total_b));
+        const size_t arrays = secondary_ ? 9 : 6;
+        const size_t need = arrays * cells * sizeof(T) +
+                            (secondary_ ? static_cast<size_t>(grid.Ng) * nang * nang * sizeof(T) : 0) +
+                            (64ull << 20);
+        if (need > free_b)
+            throw Error(BFPN_E_NOMEM, "Requested grid needs about " +
+                                          std::to_string(need >> 20) + " MiB of device memory, but only " +
+                                          std::to_string(free_b >> 20) + " MiB is free (Ny=" +
+                                          std::to_string(grid.Ny) + ", Nz=" + std::to_string(grid.Nz) +
+                                          ", Ng=" + std::to_string(grid.Ng) + ", Nmu=" +
+                                          std::to_string(grid.Nmu) + ", Nom=" + std::to_string(grid.Nom) + ")");
+    }
+
+    void grow_history(int cap) {
+        DeviceBuffer nb;
+        nb.ensure(2 * static_cast<size_t>(cap) * sizeof(double));
+        BFPN_HIP(hipMemsetAsync(nb.as<void>(), 0, nb.bytes(), stream));
+        if (hist_cap_ > 0 && step_ > 0) {
+            BFPN_HIP(hipMemcpyAsync(nb.as<double>(), hist_.as<double>(), step_ * sizeof(double),
+                                    hipMemcpyDeviceToDevice, stream));
+            BFPN_HIP(hipMemcpyAsync(nb.as<double>() + cap, hist_.as<double>() + hist_cap_,
+                                    step_ * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        }
+        BFPN_HIP(hipStreamSynchronize(stream));
+        hist_.swap(nb);
+        hist_cap_ = cap;
+    }
+
+    // Event after an operator call; the interval since the previous event is
+    // charged to phase p together with the call's algorithmic bytes.
+    void mark(Phase p, double bytes = 0.0) {
+        if (!profiling) return;
+        hipEvent_t e;
+        BFPN_HIP(hipEventCreate(&e));
+        BFPN_HIP(hipEventRecord(e, stream));
+        events_.push_back({e, p});
+        phase_bytes[p] += bytes;
+        phase_calls[p] += bytes > 0.0 ? 1 : 0;
+    }
+
+    void collect_profile() {
+        BFPN_HIP(hipStreamSynchronize(stream));
+        for (size_t k = 1; k < events_.size(); ++k) {
+            float ms = 0.f;
+            BFPN_HIP(hipEventElapsedTime(&ms, events_[k - 1].first, events_[k].first));
+            phase_seconds[events_[k].second] += ms * 1e-3;
+        }
+        for (auto& e : events_) (void)hipEventDestroy(e.first);
+        events_.clear();
+    }
+
+    double array_bytes() const { return static_cast<double>(cells) * sizeof(T); }
+
+    void energy(int species, bool with_source, double* partials, bool proxy) {
+        T* F = cur_[2 * species];
+        T* f = cur_[2 * species + 1];
+        launch_energy_half<T>(F, f, coef_.as<EnergyCoef>(), with_source ? q_.as<T>() : nullptr,
+                              grid.Ng, lanes, phys.eq15_straggling != 0, dose_w_.as<DoseCoef>(),
+                              yzw_.as<double>(), nang, partials, proxy, stream);
+        mark(kEnergy, array_bytes() * (with_source ? 5 : 4));
+    }
+
+    void transport(int which, bool preserve_sign) {
+        launch_transport_half<T>(cur_[which], pred_.as<T>(), spare_ptr_, om_.as<T>(), grid.Ny,
+                                 grid.Nz, grid.Ng, nang, d.dy, d.dz, 0.5 * d.dt, preserve_sign,
+                                 stream);
+        std::swap(cur_[which], spare_ptr_);
+        mark(kTransport, array_bytes() * 2);
+    }
+
+    void angle(int which) {
+        launch_angle_full<T>(cur_[which], factors_.as<T>(), su_.as<T>(), sv_.as<T>(), grid.Ng, nyz,
+                             grid.Nmu, grid.Nom, stream);
+        mark(kAngle, array_bytes() * 2);
+    }
+
+    // One depth step (BFPnSolver::solve loop body, src/bfpm_solver.cu:2213-2242).
+    void step() {
+        const bool primary_clip_preserve = phys.no_spatial_clipping != 0;
+        double* pp = partial_p_.as<double>();
+        double* ps = partial_s_.as<double>();
+        const int eb = energy_blocks(lanes);
+        mark(kDose);  // opens the first interval of the step
+        if (phys.energy_only) {  // stepPrimaryEnergyOnly, src/bfpm_solver.cu:3313-3333
+            energy(0, false, nullptr, false);
+            energy(0, false, pp, true);
+        } else {  // stepPrimary, src/bfpm_solver.cu:3238-3311
+            energy(0, false, nullptr, false);
+            if (!phys.no_transport) {
+                transport(0, primary_clip_preserve);
+                transport(1, true);
+            }
+            if (!phys.no_angle) {
+                angle(0);
+                angle(1);
+            }
+            if (!phys.no_transport) {
+                transport(0, primary_clip_preserve);
+                transport(1, true);
+            }
+            energy(0, false, pp, true);
+        }
+        const bool sec = secondary_ && step_ > 0;
+        if (sec) {  // stepSecondary, src/bfpm_solver.cu:3335-3394
+            launch_source_angular<T>(cur_[0], cur_[1], kv_.as<T>(), a1_.as<T>(), a2_.as<T>(),
+                                     grid.Ng, nyz, nang, stream);
+            launch_source_energy<T>(a1_.as<T>(), a2_.as<T>(), e1_.as<double>(), e2_.as<double>(),
+                                    beg_.as<int>(), end_.as<int>(), q_.as<T>(), grid.Ng, lanes,
+                                    d.dg, stream);
+            mark(kSource, array_bytes() * 7);  // 2 in, 2 out; 2 in, 1 out
+            energy(1, true, nullptr, false);
+            transport(2, false);  // secondary psi^1 always clipped
+            transport(3, true);
+            angle(2);
+            angle(3);
+            transport(2, false);
+            transport(3, true);
+            energy(1, true, ps, false);
+        }
+        const double scale = d.du * d.dv * d.dy * d.dz / phys.density;
+        double* hist = hist_.as<double>();
+        launch_finish_sum(pp, eb, sec ? ps : nullptr, eb, scale, hist + step_, stream);
+        launch_finish_sum(pp + eb, eb, nullptr, 0, d.dg, hist + hist_cap_ + step_, stream);
+        mark(kDose);
+        ++step_;
+    }
+
+    bool secondary_ = true;
+    DeviceBuffer state_[4], spare_, pred_, a1_, a2_, q_;
+    DeviceBuffer coef_, dose_w_, yzw_, om_, factors_, su_, sv_, e1_, e2_, beg_, end_, kv_;
+    DeviceBuffer partial_p_, partial_s_, hist_;
+    T* cur_[4] = {nullptr, nullptr, nullptr, nullptr};
+    T* spare_ptr_ = nullptr;
+    int step_ = 0;
+    int hist_cap_ = 0;
+    std::vector<std::pair<hipEvent_t, Phase>> events_;
+};
+
+}  // namespace
+
+// ============================================================ workspace ===
+struct bfpn_workspace {
+    DeviceBuffer coef, partial, pred, out, a1, a2, conv, factors, su, sv, om, dose_w, yzw, kvT;
+};
+
+namespace {
+
+template <typename T>
+std::vector<double> to_host(const T* dptr, size_t n) {
+    std::vector<T> t(n);
+    if (n) BFPN_HIP(hipMemcpy(t.data(), dptr, n * sizeof(T), hipMemcpyDeviceToHost));
+    return std::vector<double>(t.begin(), t.end());
+}
+
+template <typename T>
+void op_energy(bfpn_workspace* ws, T* F, T* f, const double* S, const double* sigma,
+               const double* Tc, double h, double dg, double rho, int Ng, int nyz, int nang,
+               const T* Q1, const T* Q2, int straggling, hipStream_t s) {
+    BFPN_REQUIRE(ws && F && f && S && sigma, "null argument");
+    BFPN_REQUIRE(Ng > 0 && nyz > 0 && nang > 0 && h > 0, "invalid sizes");
+    if (Q2) throw Error(BFPN_E_UNSUPPORTED, "a psi^2 source (Q2) is not supported");
+    const auto hS = to_host(S, Ng + 1);
+    const auto hsig = to_host(sigma, Ng);
+    std::vector<double> hT;
+    if (Tc) hT = to_host(Tc, Ng);
+    upload(ws->coef, energy_coefs(hS, hsig, Tc ? hT.data() : nullptr, straggling != 0, h, dg, rho, Ng), s);
+    launch_energy_half<T>(F, f, ws->coef.as<EnergyCoef>(), Q1, Ng,
+                          static_cast<long long>(nyz) * nang, straggling != 0 && Tc, nullptr,
+                          nullptr, nang, nullptr, false, s);
+    BFPN_HIP(hipStreamSynchronize(s));  // the coefficient upload came from host memory
+}
+
+template <typename T>
+void op_source(bfpn_workspace* ws, const T* Fp, const T* fp, T* Q1, const double* e1,
+               const double* e2, const double* kv, const int* beg, const int* end, int Ng, int nyz,
+               int nang, double dg, hipStream_t s) {
+    BFPN_REQUIRE(ws && Fp && fp && Q1 && e1 && e2 && kv && beg && end, "null argument");
+    BFPN_REQUIRE(Ng > 0 && nyz > 0 && nang > 0, "invalid sizes");
+    const size_t n = static_cast<size_t>(Ng) * nyz * nang;
+    const long long nk = static_cast<long long>(Ng) * nang * nang;
+    ws->a1.ensure(n * sizeof(T));
+    ws->a2.ensure(n * sizeof(T));
+    ws->kvT.ensure(nk * sizeof(T));
+    launch_convert<T>(kv, ws->kvT.as<T>(), nk, s);
+    launch_source_angular<T>(Fp, fp, ws->kvT.as<T>(), ws->a1.as<T>(), ws->a2.as<T>(), Ng, nyz, nang, s);
+    launch_source_energy<T>(ws->a1.as<T>(), ws->a2.as<T>(), e1, e2, beg, end, Q1, Ng,
+                            static_cast<long long>(nyz) * nang, dg, s);
+}
+
+template <typename T>
+void op_transport(bfpn_workspace* ws, T* F, const double* omend, int Ny, int Nz, int Ng, int nang,
+                  double dy, double dz, double h, int preserve, hipStream_t s) {
+    BFPN_REQUIRE(ws && F && omend, "null argument");
+    BFPN_REQUIRE(Ny > 0 && Nz > 0 && Ng > 0 && nang > 0, "invalid sizes");
+    const size_t n = static_cast<size_t>(Ng) * (Ny + 1) * (Nz + 1) * nang;
+    ws->pred.ensure(n * sizeof(T));
+    ws->out.ensure(n * sizeof(T));
+    ws->om.ensure(2 * nang * sizeof(T));
+    launch_convert<T>(omend, ws->om.as<T>(), 2 * nang, s);
+    launch_transport_half<T>(F, ws->pred.as<T>(), ws->out.as<T>(), ws->om.as<T>(), Ny, Nz, Ng, nang,
+                             dy, dz, h, preserve != 0, s);
+    BFPN_HIP(hipMemcpyAsync(F, ws->out.as<T>(), n * sizeof(T), hipMemcpyDeviceToDevice, s));
+}
+
+template <typename T>
+void op_angle(bfpn_workspace* ws, T* F, const double* sig_tr_host, int nyz, int Ng, int Nmu,
+              int Nom, double du, double dv, double dt, double rho, hipStream_t s) {
+    BFPN_REQUIRE(ws && F && sig_tr_host, "null argument");
+    BFPN_REQUIRE(nyz > 0 && Ng > 0 && Nmu > 0 && Nom > 0 && Nmu * Nom <= 1024, "invalid sizes");
+    upload(ws->factors, cast_vector<T>(angle_factors(sig_tr_host, Ng, Nmu, Nom, du, dv, dt, rho)), s);
+    upload(ws->su, cast_vector<T>(sine_basis(Nom)), s);
+    upload(ws->sv, cast_vector<T>(sine_basis(Nmu)), s);
+    launch_angle_full<T>(F, ws->factors.as<T>(), ws->su.as<T>(), ws->sv.as<T>(), Ng, nyz, Nmu, Nom, s);
+    BFPN_HIP(hipStreamSynchronize(s));
+}
+
+template <typename T>
+void op_idd(bfpn_workspace* ws, const T* F, const T* f, const T* F1, const T* f1, const double* S,
+            const double* sigma, const double* en, int Ng, int Ny, int Nz, int nang, double dg,
+            double du, double dv, double dy, double dz, double rho, int trap, int cat, double* out,
+            hipStream_t s) {
+    BFPN_REQUIRE(ws && F && S && sigma && en && out, "null argument");
+    BFPN_REQUIRE(Ng > 0 && Ny > 0 && Nz > 0 && nang > 0 && rho > 0, "invalid sizes");
+    const auto hS = to_host(S, Ng + 1);
+    const auto hsig = to_host(sigma, Ng);
+    const auto hen = to_host(en, Ng + 1);
+    upload(ws->dose_w, dose_coefs(hS, hsig, hen, dg, cat != 0, Ng), s);
+    upload(ws->yzw, yz_weights(Ny, Nz, trap != 0), s);
+    const long long lanes = static_cast<long long>(Ny + 1) * (Nz + 1) * nang;
+    const int nb = dose_blocks(Ng, lanes);
+    ws->partial.ensure((nb + 1) * sizeof(double));
+    double* part = ws->partial.as<double>();
+    launch_dose<T>(F, f, F1, f1, ws->dose_w.as<DoseCoef>(), ws->yzw.as<double>(), Ng, lanes, nang, part, s);
+    launch_finish_sum(part, nb, nullptr, 0, du * dv * dy * dz / rho, part + nb, s);
+    BFPN_HIP(hipMemcpyAsync(out, part + nb, sizeof(double), hipMemcpyDeviceToHost, s));
+    BFPN_HIP(hipStreamSynchronize(s));
+}
+
+hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+}  // namespace
+
+// ================================================================ C ABI ===
+extern "C" {
+
+BFPN_API const char* bfpn_last_error(void) { return g_last_error.c_str(); }
+BFPN_API const char* bfpn_version(void) { return "bfpn-mi355x 0.1 (gfx950)"; }
+
+BFPN_API bfpn_status bfpn_device_count(int* count) {
+    return guarded([&] {
+        BFPN_REQUIRE(count, "null argument");
+        BFPN_HIP(hipGetDeviceCount(count));
+    });
+}
+
+BFPN_API bfpn_status bfpn_device_info(int device, char* name, size_t name_len, size_t* total,
+                                      int* cus) {
+    return guarded([&] {
+        hipDeviceProp_t prop;
+        BFPN_HIP(hipGetDeviceProperties(&prop, device));
+        if (name && name_len) {
+            std::strncpy(name, prop.name, name_len - 1);
+            name[name_len - 1] = 0;
+        }
+        if (total) *total = prop.totalGlobalMem;
+        if (cus) *cus = prop.multiProcessorCount;
+    });
+}
+
+BFPN_API bfpn_status bfpn_malloc(void** p, size_t bytes) {
+    return guarded([&] {
+        BFPN_REQUIRE(p, "null argument");
+        hipError_t e = hipMalloc(p, bytes);
+        if (e != hipSuccess) {
+            *p = nullptr;
+            (void)hipGetLastError();
+            throw Error(BFPN_E_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        }
+    });
+}
+BFPN_API bfpn_status bfpn_free(void* p) { return guarded([&] { BFPN_HIP(hipFree(p)); }); }
+BFPN_API bfpn_status bfpn_memcpy_h2d(void* dst, const void* src, size_t bytes) {
+    return guarded([&] { BFPN_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+BFPN_API bfpn_status bfpn_memcpy_d2h(void* dst, const void* src, size_t bytes) {
+    return guarded([&] { BFPN_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+BFPN_API bfpn_status bfpn_memset(void* p, int v, size_t bytes) {
+    return guarded([&] {
+        BFPN_HIP(hipMemset(p, v, bytes));
+        BFPN_HIP(hipDeviceSynchronize());
+    });
+}
+BFPN_API bfpn_status bfpn_synchronize(void) {
+    return guarded([&] { BFPN_HIP(hipDeviceSynchronize()); });
+}
+BFPN_API bfpn_status bfpn_mem_info(size_t* free_b, size_t* total_b) {
+    return guarded([&] { BFPN_HIP(hipMemGetInfo(free_b, total_b)); });
+}
+
+BFPN_API bfpn_status bfpn_workspace_create(bfpn_workspace** ws) {
+    return guarded([&] {
+        BFPN_REQUIRE(ws, "null argument");
+        *ws = new bfpn_workspace();
+    });
+}
+BFPN_API void bfpn_workspace_destroy(bfpn_workspace* ws) { delete ws; }
+
+BFPN_API bfpn_status bfpn_energy_half_f32(bfpn_workspace* ws, float* F, float* f, const double* S,
+                                          const double* sigma, const double* T, double h, double dg,
+                                          double rho, int Ng, int nyz, int nang, const float* Q1,
+                                          const float* Q2, int straggling, void* stream) {
+    return guarded([&] { op_energy<float>(ws, F, f, S, sigma, T, h, dg, rho, Ng, nyz, nang, Q1, Q2, straggling, as_stream(stream)); });
+}
+BFPN_API bfpn_status bfpn_energy_half_f64(bfpn_workspace* ws, double* F, double* f, const double* S,
+                                          const double* sigma, const double* T, double h, double dg,
+                                          double rho, int Ng, int nyz, int nang, const double* Q1,
+                                          const double* Q2, int straggling, void* stream) {
+    return guarded([&] { op_energy<double>(ws, F, f, S, sigma, T, h, dg, rho, Ng, nyz, nang, Q1, Q2, straggling, as_stream(stream)); });
+}
+
+BFPN_API bfpn_status bfpn_secondary_source_f32(bfpn_workspace* ws, const float* Fp, const float* fp,
+                                               float* Q1, const double* e1, const double* e2,
+                                               const double* kv, const int* beg, const int* end,
+                                               int Ng, int nyz, int nang, double dg, void* stream) {
+    return guarded([&] { op_source<float>(ws, Fp, fp, Q1, e1, e2, kv, beg, end, Ng, nyz, nang, dg, as_stream(stream)); });
+}
+BFPN_API bfpn_status bfpn_secondary_source_f64(bfpn_workspace* ws, const double* Fp, const double* fp,
+                                               double* Q1, const double* e1, const double* e2,
+                                               const double* kv, const int* beg, const int* end,
+                                               int Ng, int nyz, int nang, double dg, void* stream) {
+    return guarded([&] { op_source<double>(ws, Fp, fp, Q1, e1, e2, kv, beg, end, Ng, nyz, nang, dg, as_stream(stream)); });
+}
+
+BFPN_API bfpn_status bfpn_transport_half_f32(bfpn_workspace* ws, float* F, const double* omend,
+                                             int Ny, int Nz, int Ng, int nang, double dy, double dz,
+                                             double h, int preserve_sign, void* stream) {
+    return guarded([&] { op_transport<float>(ws, F, omend, Ny, Nz, Ng, nang, dy, dz, h, preserve_sign, as_stream(stream)); });
+}
+BFPN_API bfpn_status bfpn_transport_half_f64(bfpn_workspace* ws, double* F, const double* omend,
+                                             int Ny, int Nz, int Ng, int nang, double dy, double dz,
+                                             double h, int preserve_sign, void* stream) {
+    return guarded([&] { op_transport<double>(ws, F, omend, Ny, Nz, Ng, nang, dy, dz, h, preserve_sign, as_stream(stream)); });
+}
+
+BFPN_API bfpn_status bfpn_angle_full_f32(bfpn_workspace* ws, float* F, const double* sig_tr,
+                                         int nyz, int Ng, int Nmu, int Nom, double du, double dv,
+                                         double dt, double rho, void* stream) {
+    return guarded([&] { op_angle<float>(ws, F, sig_tr, nyz, Ng, Nmu, Nom, du, dv, dt, rho, as_stream(stream)); });
+}
+BFPN_API bfpn_status bfpn_angle_full_f64(bfpn_workspace* ws, double* F, const double* sig_tr,
+                                         int nyz, int Ng, int Nmu, int Nom, double du, double dv,
+                                         double dt, double rho, void* stream) {
+    return guarded([&] { op_angle<double>(ws, F, sig_tr, nyz, Ng, Nmu, Nom, du, dv, dt, rho, as_stream(stream)); });
+}
+
+BFPN_API bfpn_status bfpn_idd_f32(bfpn_workspace* ws, const float* F, const float* f, const float* F1,
+                                  const float* f1, const double* S, const double* sigma,
+                                  const double* en, int Ng, int Ny, int Nz, int nang, double dg,
+                                  double du, double dv, double dy, double dz, double rho, int trap,
+                                  int cat, double* idd, void* stream) {
+    return guarded([&] { op_idd<float>(ws, F, f, F1, f1, S, sigma, en, Ng, Ny, Nz, nang, dg, du, dv, dy, dz, rho, trap, cat, idd, as_stream(stream)); });
+}
+BFPN_API bfpn_status bfpn_idd_f64(bfpn_workspace* ws, const double* F, const double* f,
+                                  const double* F1, const double* f1, const double* S,
+                                  const double* sigma, const double* en, int Ng, int Ny, int Nz,
+                                  int nang, double dg, double du, double dv, double dy, double dz,
+                                  double rho, int trap, int cat, double* idd, void* stream) {
+    return guarded([&] { op_idd<double>(ws, F, f, F1, f1, S, sigma, en, Ng, Ny, Nz, nang, dg, du, dv, dy, dz, rho, trap, cat, idd, as_stream(stream)); });
+}
+
+BFPN_API bfpn_status bfpn_create(const bfpn_grid* grid, const bfpn_phys* phys, const char* data_path,
+                                 int precision, bfpn_ctx** out) {
+    return guarded([&] {
+        BFPN_REQUIRE(grid && phys && out, "null argument");
+        *out = nullptr;
+        const std::string path = data_path ? data_path : "./data";
+        if (precision == BFPN_FP32)
+            *out = new Context<float>(*grid, *phys, path);
+        else if (precision == BFPN_FP64)
+            *out = new Context<double>(*grid, *phys, path);
+        else
+            throw Error(BFPN_E_INVALID, "precision must be BFPN_FP32 or BFPN_FP64");
+    });
+}
+
+BFPN_API void bfpn_destroy(bfpn_ctx* ctx) { delete ctx; }
+
+BFPN_API bfpn_status bfpn_get_deltas(const bfpn_ctx* ctx, bfpn_deltas* out) {
+    return guarded([&] {
+        BFPN_REQUIRE(ctx && out, "null argument");
+        *out = ctx->d;
+    });
+}
+
+BFPN_API size_t bfpn_cells(const bfpn_ctx* ctx) { return ctx ? ctx->cells : 0; }
+BFPN_API size_t bfpn_device_bytes(const bfpn_ctx* ctx) { return ctx ? ctx->device_bytes() : 0; }
+BFPN_API int bfpn_num_steps(const bfpn_ctx* ctx, double t_final) {
+    return ctx ? static_cast<int>(std::ceil(t_final / ctx->d.dt - 1e-12)) : -1;
+}
+
+BFPN_API bfpn_status bfpn_get_table(const bfpn_ctx* ctx, int which, double* out, size_t count) {
+    return guarded([&] {
+        BFPN_REQUIRE(ctx && out, "null argument");
+        std::vector<double> tmp;
+        const std::vector<double>* v = nullptr;
+        switch (which) {
+            case 0: v = &ctx->S; break;
+            case 1: v = &ctx->mat.sigma; break;
+            case 2: v = &ctx->Tc; break;
+            case 3: v = &ctx->sig_tr; break;
+            case 4: v = &ctx->en; break;
+            case 5: v = &ctx->ker.e1; break;
+            case 6: v = &ctx->ker.e2; break;
+            case 7: v = &ctx->ker.v; break;
+            case 8: v = &ctx->om; break;
+            case 9: v = &ctx->mat.mu; break;
+            case 10: tmp.assign(ctx->ker.beg.begin(), ctx->ker.beg.end()); v = &tmp; break;
+            case 11: tmp.assign(ctx->ker.end.begin(), ctx->ker.end.end()); v = &tmp; break;
+            default: throw Error(BFPN_E_INVALID, "unknown table index");
+        }
+        BFPN_REQUIRE(count == v->size(), "table size mismatch: expected " + std::to_string(v->size()));
+        std::memcpy(out, v->data(), count * sizeof(double));
+    });
+}
+
+BFPN_API bfpn_status bfpn_advance(bfpn_ctx* ctx, int nsteps) {
+    return guarded([&] {
+        BFPN_REQUIRE(ctx, "null context");
+        ctx->advance(nsteps);
+    });
+}
+
+BFPN_API bfpn_status bfpn_history(bfpn_ctx* ctx, double* idd, double* proxy, int cap, int* n) {
+    return guarded([&] {
+        BFPN_REQUIRE(ctx, "null context");
+        ctx->history(idd, proxy, cap, n);
+    });
+}
+
+BFPN_API bfpn_status bfpn_run(bfpn_ctx* ctx, double t_final, double* idd, double* proxy, int cap,
+                              int* n) {
+    return guarded([&] {
+        BFPN_REQUIRE(ctx, "null context");
+        ctx->advance(static_cast<int>(std::ceil(t_final / ctx->d.dt - 1e-12)));
+        ctx->history(idd, proxy, cap, n);
+    });
+}
+
+BFPN_API bfpn_status bfpn_read_state(bfpn_ctx* ctx, int which, double* host, size_t count) {
+    return guarded([&] {
+        BFPN_REQUIRE(ctx && host, "null argument");
+        ctx->read_state(which, host, count);
+    });
+}
+
+BFPN_API bfpn_status bfpn_write_state(bfpn_ctx* ctx, int which, const double* host, size_t count) {
+    return guarded([&] {
+        BFPN_REQUIRE(ctx && host, "null argument");
+        ctx->write_state(which, host, count);
+    });
+}
+
+BFPN_API void* bfpn_stream(bfpn_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+BFPN_API bfpn_status bfpn_set_profiling(bfpn_ctx* ctx, int on) {
+    return guarded([&] {
+        BFPN_REQUIRE(ctx, "null context");
+        ctx->profiling = on != 0;
+        for (int k = 0; k < 5; ++k) {
+            ctx->phase_seconds[k] = 0.0;
+            ctx->phase_bytes[k] = 0.0;
+            ctx->phase_calls[k] = 0;
+        }
+    });
+}
+
+BFPN_API bfpn_status bfpn_phase_stats(bfpn_ctx* ctx, double* seconds5, int* calls5,
+                                      double* bytes5) {
+    return guarded([&] {
+        BFPN_REQUIRE(ctx, "null context");
+        for (int k = 0; k < 5; ++k) {
+            if (seconds5) seconds5[k] = ctx->phase_seconds[k];
+            if (calls5) calls5[k] = ctx->phase_calls[k];
+            if (bytes5) bytes5[k] = ctx->phase_bytes[k];
+        }
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,267 @@
+/*
+ * bfpn.h — C ABI of the MI355X-native in-core eq15 BFPn solver.
+ *
+ * The reference solver (BFPnSolver + four operator classes) is a C++/CUDA
+ * API with raw device pointers, host-synchronous calls and CUDA_CHECK ->
+ * exit(1) error handling.  This header replaces that seam with plain
+ * extern "C" entry points: POD parameter structs, opaque handles, status
+ * codes (never exit()), stream-ordered calls without hidden host syncs, and
+ * no global state shared between contexts.
+ *
+ * Every entry point cites the reference interface it replaces.  Reference
+ * paths are relative to the root of xiaojin1423764/BFPn_GPU.
+ *
+ * Layout of every phase-space array (unchanged from the reference,
+ * src/bfpm_solver.cu:2013):
+ *     idx = (g * nyz + s) * nang + m,  s = iz*(Ny+1) + iy,  m = iv*Nom + iu
+ * with nyz = (Ny+1)*(Nz+1) transverse nodes and nang = Nmu*Nom directions.
+ *
+ * State precision: every operator exists for FP32 state ("_f32") and FP64
+ * state ("_f64").  The energy recurrence and all dose reductions always
+ * accumulate in FP64 whatever the state type.
+ */
+#ifndef BFPN_H
+#define BFPN_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BFPN_API __attribute__((visibility("default")))
+
+/* ---- status codes (replace CUDA_CHECK -> exit(1), src/utils.cuh:13-21) ---- */
+typedef enum bfpn_status {
+    BFPN_OK = 0,
+    BFPN_E_INVALID = 1,     /* bad argument / inconsistent sizes            */
+    BFPN_E_NOMEM = 2,       /* device memory preflight or allocation failed */
+    BFPN_E_HIP = 3,         /* HIP runtime error                            */
+    BFPN_E_IO = 4,          /* table file missing or short                  */
+    BFPN_E_UNSUPPORTED = 5  /* option not available in this build           */
+} bfpn_status;
+
+typedef enum bfpn_precision {
+    BFPN_FP32 = 0,          /* FP32 state, FP64 energy recurrence and IDD   */
+    BFPN_FP64 = 1           /* FP64 everywhere (oracle-grade instantiation) */
+} bfpn_precision;
+
+/* GridParams (src/bfpm_solver.cuh:21-27); deltas are derived by the library
+ * exactly as GridParams::computeDeltas (src/bfpm_solver.cu:1116-1126). */
+typedef struct bfpn_grid {
+    int Nx, Ny, Nz, Ng, Nmu, Nom;
+    double Lx, Ly, Lz, Lu, Lv, Lg;
+} bfpn_grid;
+
+/* PhysicsParams (src/bfpm_solver.cuh:30-62), in-core fields only.  The
+ * streaming/lite knobs of the reference are CLI-level aliases here. */
+typedef struct bfpn_phys {
+    double sig_E;
+    double a_1, a_2;
+    double beam_energy;
+    double density;
+    double energy_density;
+    char material_name[32];          /* "", "water", "bone", "air"          */
+    int primary_only;
+    int eq15_straggling;
+    int energy_only;
+    int no_transport;
+    int no_angle;
+    int no_spatial_clipping;
+    int discrete_angle_delta;
+    int normalize_initial_mass;
+    int trapezoidal_yz;
+    int dose_catastrophic_loss;
+    int calibrate_cross_sections;
+    int save_energy_moments;
+} bfpn_phys;
+
+/* Derived deltas (GridParams::computeDeltas, src/bfpm_solver.cu:1116-1126). */
+typedef struct bfpn_deltas {
+    double dt, dy, dz, dg, du, dv;
+} bfpn_deltas;
+
+typedef struct bfpn_ctx bfpn_ctx;         /* one beam, all state in HBM */
+typedef struct bfpn_workspace bfpn_workspace; /* scratch for operator calls */
+
+/* Last error message of the calling thread ("" if none). */
+BFPN_API const char* bfpn_last_error(void);
+BFPN_API const char* bfpn_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Device memory helpers.  Callers that hold no other HIP runtime (ctypes,
+ * the CLI) use these to stage operator inputs.  Replaces DeviceArray<T>
+ * (src/utils.cuh:55-140).
+ * ------------------------------------------------------------------------- */
+BFPN_API bfpn_status bfpn_device_count(int* count);
+BFPN_API bfpn_status bfpn_device_info(int device, char* name, size_t name_len,
+                                      size_t* total_bytes, int* cu_count);
+BFPN_API bfpn_status bfpn_malloc(void** dptr, size_t bytes);
+BFPN_API bfpn_status bfpn_free(void* dptr);
+BFPN_API bfpn_status bfpn_memcpy_h2d(void* dst, const void* src, size_t bytes);
+BFPN_API bfpn_status bfpn_memcpy_d2h(void* dst, const void* src, size_t bytes);
+BFPN_API bfpn_status bfpn_memset(void* dptr, int value, size_t bytes);
+BFPN_API bfpn_status bfpn_synchronize(void);
+BFPN_API bfpn_status bfpn_mem_info(size_t* free_bytes, size_t* total_bytes);
+
+/* ---------------------------------------------------------------------------
+ * Operator-level API (the seam of the reference's sub-solver classes).
+ * All pointers are device pointers; `stream` is a hipStream_t (NULL = the
+ * default stream).  Calls are stream-ordered and never synchronize the host
+ * unless they return a host value.  Scratch comes from a workspace that grows
+ * on demand, like the sub-solvers' lazily grown buffers
+ * (src/energy_solver.cu:461-464, src/spatial_transport.cu:211-215,
+ *  src/angle_diffusion.cu:429-440).
+ * ------------------------------------------------------------------------- */
+BFPN_API bfpn_status bfpn_workspace_create(bfpn_workspace** ws);
+BFPN_API void        bfpn_workspace_destroy(bfpn_workspace* ws);
+
+/* Eq.15 DG/CN energy half step, in place on (F, f).
+ * Replaces EnergyDepositionSolver::solve with the eq15 model
+ * (src/energy_solver.cuh:65-77, src/energy_solver.cu:41-161, 444-506).
+ * S has Ng+1 entries, sigma and T have Ng.  Q1/Q2 may be NULL (primary);
+ * T may be NULL or straggling = 0 to disable straggling. */
+BFPN_API bfpn_status bfpn_energy_half_f32(
+    bfpn_workspace* ws, float* F, float* f,
+    const double* S, const double* sigma, const double* T,
+    double h, double dg, double density, int Ng, int nyz, int nang,
+    const float* Q1, const float* Q2, int straggling, void* stream);
+BFPN_API bfpn_status bfpn_energy_half_f64(
+    bfpn_workspace* ws, double* F, double* f,
+    const double* S, const double* sigma, const double* T,
+    double h, double dg, double density, int Ng, int nyz, int nang,
+    const double* Q1, const double* Q2, int straggling, void* stream);
+
+/* Catastrophic secondary source Q1 from the primary state.
+ * Replaces secondarySourceFromPrimaryKernel (src/energy_solver.cu:287-333):
+ *   Q1[g,s,m] = dg * sum_{g' in [beg_g,end_g)} sum_m'
+ *               (K1[g,g'] max(Fp[g',s,m'],0) + K2[g,g'] fp[g',s,m']/3) Kv[g',m',m]
+ * ker_e1/ker_e2 are Ng*Ng row-major FP64, ker_v is Ng*nang*nang FP64 [g'][in][out],
+ * ker_beg/ker_end are the per-row nonzero column spans (Ng ints). */
+BFPN_API bfpn_status bfpn_secondary_source_f32(
+    bfpn_workspace* ws, const float* Fp, const float* fp, float* Q1,
+    const double* ker_e1, const double* ker_e2, const double* ker_v,
+    const int* ker_beg, const int* ker_end,
+    int Ng, int nyz, int nang, double dg, void* stream);
+BFPN_API bfpn_status bfpn_secondary_source_f64(
+    bfpn_workspace* ws, const double* Fp, const double* fp, double* Q1,
+    const double* ker_e1, const double* ker_e2, const double* ker_v,
+    const int* ker_beg, const int* ker_end,
+    int Ng, int nyz, int nang, double dg, void* stream);
+
+/* Explicit MUSCL predictor-corrector transport over h, in place.
+ * Replaces SpatialTransportSolver::solveEq18
+ * (src/spatial_transport.cuh:41-44, src/spatial_transport.cu:23-161, 202-262).
+ * omend holds nang (omega_y, omega_z) pairs.  preserve_sign = 0 clips to >= 0. */
+BFPN_API bfpn_status bfpn_transport_half_f32(
+    bfpn_workspace* ws, float* F, const double* omend,
+    int Ny, int Nz, int Ng, int nang, double dy, double dz, double h,
+    int preserve_sign, void* stream);
+BFPN_API bfpn_status bfpn_transport_half_f64(
+    bfpn_workspace* ws, double* F, const double* omend,
+    int Ny, int Nz, int Ng, int nang, double dy, double dz, double h,
+    int preserve_sign, void* stream);
+
+/* Crank-Nicolson angular diffusion over dt on the zero-boundary DST-I basis,
+ * in place.  Replaces AngleDiffusionSolver::solveEq19
+ * (src/angle_diffusion.cuh:52-54, src/angle_diffusion.cu:7-34, 254-483).
+ * sig_tr has Ng entries (host pointer: the CN factors are built on the host). */
+BFPN_API bfpn_status bfpn_angle_full_f32(
+    bfpn_workspace* ws, float* F, const double* sig_tr_host,
+    int nyz, int Ng, int Nmu, int Nom, double du, double dv, double dt,
+    double density, void* stream);
+BFPN_API bfpn_status bfpn_angle_full_f64(
+    bfpn_workspace* ws, double* F, const double* sig_tr_host,
+    int nyz, int Ng, int Nmu, int Nom, double du, double dv, double dt,
+    double density, void* stream);
+
+/* Eq.24 integrated depth dose of (F+F1, f+f1).  Synchronizes and returns the
+ * value in *idd.  Replaces computeIntegratedDepthDose + integratedDepthDoseKernel
+ * + reduceDevicePartials (src/bfpm_solver.cu:229-309, 2132-2164, 3473-3502).
+ * S/sigma/en are device pointers (Ng+1, Ng, Ng+1). F1/f1 may be NULL. */
+BFPN_API bfpn_status bfpn_idd_f32(
+    bfpn_workspace* ws, const float* F, const float* f, const float* F1, const float* f1,
+    const double* S, const double* sigma, const double* en,
+    int Ng, int Ny, int Nz, int nang, double dg, double du, double dv,
+    double dy, double dz, double density, int trapezoidal_yz,
+    int dose_catastrophic_loss, double* idd, void* stream);
+BFPN_API bfpn_status bfpn_idd_f64(
+    bfpn_workspace* ws, const double* F, const double* f, const double* F1, const double* f1,
+    const double* S, const double* sigma, const double* en,
+    int Ng, int Ny, int Nz, int nang, double dg, double du, double dv,
+    double dy, double dz, double density, int trapezoidal_yz,
+    int dose_catastrophic_loss, double* idd, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Context-level API: one beam, state resident in HBM.
+ * Replaces BFPnSolver(GridParams, PhysicsParams), initialize(path) and
+ * solve(tFinal) (src/bfpm_solver.cuh:166-178, src/bfpm_solver.cu:1171-1202,
+ * 1435-1568, 2166-2267).
+ * ------------------------------------------------------------------------- */
+
+/* Builds tables on the host (FP64), allocates the state on the current
+ * device, and initializes the Eq.23 beam on the device.  Fails with
+ * BFPN_E_NOMEM when the estimated footprint exceeds free memory
+ * (preflightMemory, src/bfpm_solver.cu:1321-1433). */
+BFPN_API bfpn_status bfpn_create(const bfpn_grid* grid, const bfpn_phys* phys,
+                                 const char* data_path, int precision,
+                                 bfpn_ctx** out);
+BFPN_API void bfpn_destroy(bfpn_ctx* ctx);
+
+/* Derived deltas and array sizes. */
+BFPN_API bfpn_status bfpn_get_deltas(const bfpn_ctx* ctx, bfpn_deltas* out);
+BFPN_API size_t bfpn_cells(const bfpn_ctx* ctx);          /* Ng*nyz*nang */
+BFPN_API size_t bfpn_device_bytes(const bfpn_ctx* ctx);   /* bytes held  */
+/* Number of depth steps of a run to t_final: ceil(t_final/dt - 1e-12)
+ * (src/bfpm_solver.cu:2178). */
+BFPN_API int bfpn_num_steps(const bfpn_ctx* ctx, double t_final);
+
+/* Host tables the context was built with (FP64 copies), for parity tests.
+ * which: 0 S_s (Ng+1), 1 sigma_c (Ng), 2 T_c (Ng), 3 sig_trg (Ng), 4 en (Ng+1),
+ * 5 ker_e1 (Ng*Ng), 6 ker_e2 (Ng*Ng), 7 ker_v (Ng*nang*nang), 8 omend (2*nang),
+ * 9 mu (3*Ng), 10 ker_beg (Ng, as doubles), 11 ker_end (Ng, as doubles). */
+BFPN_API bfpn_status bfpn_get_table(const bfpn_ctx* ctx, int which,
+                                    double* out, size_t count);
+
+/* Advance `nsteps` Strang depth steps (stepPrimary, stepSecondary when
+ * step > 0, IDD and proxy; src/bfpm_solver.cu:2189-2255, 3238-3394).
+ * Stream-ordered on the context's stream; returns without synchronizing. */
+BFPN_API bfpn_status bfpn_advance(bfpn_ctx* ctx, int nsteps);
+
+/* Waits for the context's stream, then copies the IDD and proxy history
+ * (one value per completed step, x = (i+1)*dt) to the host. */
+BFPN_API bfpn_status bfpn_history(bfpn_ctx* ctx, double* idd, double* proxy,
+                                  int capacity, int* nsteps);
+
+/* Runs a whole solve to t_final and writes the history (BFPnSolver::solve). */
+BFPN_API bfpn_status bfpn_run(bfpn_ctx* ctx, double t_final, double* idd,
+                              double* proxy, int capacity, int* nsteps);
+
+/* Copies one state array to/from the host as FP64.
+ * which: 0 = psi1 primary (F), 1 = psi2 primary (f_F), 2 = psi1 secondary (F1),
+ * 3 = psi2 secondary (f_F1). */
+BFPN_API bfpn_status bfpn_read_state(bfpn_ctx* ctx, int which, double* host,
+                                     size_t count);
+BFPN_API bfpn_status bfpn_write_state(bfpn_ctx* ctx, int which,
+                                      const double* host, size_t count);
+
+/* The context's HIP stream (hipStream_t) so callers can order their own work. */
+BFPN_API void* bfpn_stream(bfpn_ctx* ctx);
+
+/* Per-phase device timing of the bfpn_advance calls made while profiling is
+ * on (replaces StepProfile, src/bfpm_solver.cuh:64-75).  Each operator launch
+ * is bracketed by HIP events on the context's stream.  Phases: 0 energy,
+ * 1 transport, 2 angle, 3 source, 4 dose/history.  For each phase: summed
+ * seconds, number of operator calls, and the algorithmic HBM bytes those
+ * calls must move (each state array read once and written once per call,
+ * plus the source's intermediate arrays).  Turning profiling on resets them. */
+BFPN_API bfpn_status bfpn_set_profiling(bfpn_ctx* ctx, int on);
+BFPN_API bfpn_status bfpn_phase_stats(bfpn_ctx* ctx, double* seconds5, int* calls5,
+                                      double* bytes5);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BFPN_H */

@@ -1,0 +1,992 @@
+// bfpn_oracle.cpp — CPU FP64 restatement of the reference's in-core eq15 BFPn path.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker that the HIP product is
+// compared against.  Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may load it; no product code path calls it.
+//
+// Every function follows the reference file:line it restates (paths relative
+// to the root of xiaojin1423764/BFPn_GPU).  Operation order inside each
+// expression is kept as in the reference so that an FP64 run reproduces the
+// reference's numbers up to FMA contraction and summation order.  Built with
+// -ffp-contract=off so that host results are reproducible bit for bit.
+//
+// Parallelism: OpenMP over independent lanes / planes / cells only; every
+// reduction is done in a fixed order, so results do not depend on the thread
+// count.
+
+#include "../include/bfpn.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <string>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;      // include/config.hpp:9
+constexpr double kMe = 9.10956e-31;                 // include/config.hpp:10
+constexpr double kMp = 1.673e-27;                   // include/config.hpp:11
+constexpr double kAlpha = 1.0 / 137.0;              // include/config.hpp:12
+constexpr double kC = 2.998e8;                      // include/config.hpp:13
+constexpr double kHbarc = 197.3;                    // include/config.hpp:14
+constexpr double kMevToJ = 1.6e-13;                 // include/config.hpp:17
+constexpr double kEps = 1e-10;                      // include/config.hpp:39
+constexpr double kKernelThreshold = 1e-3;           // include/config.hpp:40
+constexpr double kSparseThreshold = 1e-5;           // include/config.hpp:41
+
+thread_local std::string g_err;
+bool g_direct_source = false;   // full solve: factorized source unless set
+
+// ---------------------------------------------------------------- tables ---
+
+struct Elem { int Z; double w; };
+
+// meanExcitationEV, src/bfpm_solver.cu:150-154
+double mean_excitation_ev(int Z) {
+    if (Z == 1) return 19.0;
+    if (Z <= 13) return 11.2 + 11.7 * Z;
+    return 52.8 + 8.71 * Z;
+}
+
+// atomicMassNumber, src/bfpm_solver.cu:156-170
+double atomic_mass(int Z) {
+    switch (Z) {
+        case 1: return 1.0;
+        case 6: return 12.0;
+        case 7: return 14.0;
+        case 8: return 16.0;
+        case 11: return 23.0;
+        case 12: return 24.0;
+        case 15: return 31.0;
+        case 16: return 32.0;
+        case 18: return 40.0;
+        case 20: return 40.0;
+        default: return 2.0 * Z;
+    }
+}
+
+// betheStoppingElement, src/bfpm_solver.cu:172-182
+double bethe_element(double E, int Z) {
+    const double kappa = 0.307;
+    const double me_c2 = 0.51099895e6;
+    double beta2 = E * (E + 2.0 * 938.3) / ((E + 938.3) * (E + 938.3));
+    beta2 = std::max(beta2, kEps);
+    double gamma2 = 1.0 / std::max(1.0 - beta2, kEps);
+    double I = mean_excitation_ev(Z);
+    double log_arg = std::max(2.0 * me_c2 * beta2 * gamma2 / I, 1.0 + kEps);
+    double bracket = std::log(log_arg) - beta2;
+    return kappa * (static_cast<double>(Z) / atomic_mass(Z)) / beta2 * bracket;
+}
+
+// materialComposition, src/bfpm_solver.cu:184-196
+std::vector<Elem> composition(const std::string& m) {
+    if (m == "bone")
+        return {{1, 0.04200}, {6, 0.1940}, {7, 0.04000}, {8, 0.4250}, {11, 0.001000},
+                {12, 0.002000}, {15, 0.09200}, {16, 0.003000}, {20, 0.2010}};
+    if (m == "air") return {{6, 0.0001248}, {7, 0.7553}, {8, 0.2318}, {18, 0.01283}};
+    return {{1, 0.1111}, {8, 0.8889}};
+}
+
+// GridParams::computeDeltas, src/bfpm_solver.cu:1116-1126
+bfpn_deltas deltas_of(const bfpn_grid& g) {
+    bfpn_deltas d;
+    d.dt = g.Lx / g.Nx;
+    d.dy = g.Ly / g.Ny;
+    d.dz = g.Lz / g.Nz;
+    d.dg = g.Lg / g.Ng;
+    d.du = g.Lu / (g.Nom + 1);
+    d.dv = g.Lv / (g.Nmu + 1);
+    return d;
+}
+
+// computeStoppingPower, src/bfpm_solver.cu:1570-1597
+void stopping_power(const std::string& material, int Ng, double Lg, double dg, double* S) {
+    const auto comp = composition(material);
+    const auto water = composition("water");
+    for (int i = 0; i <= Ng; i++) {
+        double E = (i == 0) ? 1.0 : (i == Ng) ? Lg + 1.0 : (i - 0.5) * dg + 1.0;
+        double weighted = 0.0;
+        for (const auto& e : comp) weighted += e.w * bethe_element(E, e.Z);
+        double water_weighted = 0.0;
+        for (const auto& e : water) water_weighted += e.w * bethe_element(E, e.Z);
+        double beta_2 = E * (E + 2.0 * 938.3) / std::pow(E + 938.3, 2);
+        double F_beta = std::log(1.02e6 * beta_2 / (1.0 - beta_2)) - beta_2 - 4.31;
+        double water_legacy = 0.170 / beta_2 * F_beta + 0.02;
+        double scale = (water_weighted > kEps) ? weighted / water_weighted : 1.0;
+        S[i] = water_legacy * scale;
+    }
+}
+
+// computeStraggling, src/bfpm_solver.cu:1599-1615 (constant in E)
+void straggling(int Ng, double* T) {
+    for (int i = 0; i < Ng; i++) {
+        double eta = 8.99e18 * 4.0 * kPi * std::pow(1.6e-19, 4) / std::pow(1.6e-13, 2) / 100.0;
+        T[i] = eta * 3.34e29 * 1.2;
+    }
+}
+
+// computeTransportCrossSection, src/bfpm_solver.cu:1617-1636
+void transport_xs(int Ng, double dg, double* sig) {
+    for (int i = 0; i < Ng; i++) {
+        double E = (i + 0.5) * dg + 1.0;
+        double v = std::sqrt(2.0 * E * kMevToJ / kMp);
+        double beta = v / (3.0 * kC);
+        double Zt = 3;
+        double eta = std::pow(Zt, 2.0 / 3.0) * std::pow(kAlpha, 2) * std::pow(kMe / kMp, 2) /
+                     (beta * beta);
+        double mid = std::log((eta + 1.0) / eta) - 1.0 / (eta + 1.0);
+        sig[i] = 2.0 * kPi * 3.34 * std::pow(kHbarc, 2) * std::pow(kAlpha, 2) / 4.0 * Zt * Zt /
+                 (E * E) * mid / 1000.0;
+    }
+}
+
+// Table loading, src/bfpm_solver.cu:1436-1493
+bool load_tables(const std::string& data_path, int Ng, bool calibrate, std::vector<double>& mu,
+                 std::vector<double>& sigma, int* mu_rows_out) {
+    mu.assign(static_cast<size_t>(Ng) * 3, 0.0);
+    sigma.assign(Ng, 0.0);
+    std::string mu_path = data_path + "/data_cross.txt";
+    std::ifstream mf(mu_path);
+    if (!mf) {
+        const size_t slash = data_path.find_last_of("/\\");
+        const std::string mat = data_path.substr(slash == std::string::npos ? 0 : slash + 1);
+        if (mat == "bone" || mat == "air") {
+            mu_path = data_path + "/data_" + mat.substr(0, 1) + ".txt";
+            mf.clear();
+            mf.open(mu_path);
+        }
+    }
+    if (!mf) {
+        g_err = "Cannot open scattering data file in " + data_path;
+        return false;
+    }
+    int rows = 0;
+    while (rows < Ng && (mf >> mu[rows * 3] >> mu[rows * 3 + 1] >> mu[rows * 3 + 2])) ++rows;
+    if (rows < Ng - 1) {
+        g_err = "Scattering data " + mu_path + " has only " + std::to_string(rows) +
+                " rows; need at least " + std::to_string(Ng - 1);
+        return false;
+    }
+    // A failed extraction may have zeroed the partially read row; restore zeros.
+    for (int r = rows; r < Ng; ++r) mu[r * 3] = mu[r * 3 + 1] = mu[r * 3 + 2] = 0.0;
+    const std::string sp = data_path + "/cross_total.txt";
+    std::ifstream sf(sp);
+    if (!sf) {
+        g_err = "Cannot open total cross-section file " + sp;
+        return false;
+    }
+    int srows = 0;
+    while (srows < Ng && (sf >> sigma[srows])) ++srows;
+    if (srows != Ng) {
+        g_err = "Total cross-section data " + sp + " has " + std::to_string(srows) +
+                " rows; need " + std::to_string(Ng);
+        return false;
+    }
+    if (calibrate) {
+        for (auto& s : sigma) s *= 0.9;
+        for (int i = 401; i < 444 && i < Ng; i++) sigma[i] -= 0.0002 * (0.5 + 0.5 * (i - 401));
+    }
+    if (mu_rows_out) *mu_rows_out = rows;
+    return true;
+}
+
+// Direction table Omend, src/bfpm_solver.cu:1516-1533
+void omend_table(const bfpn_grid& g, const bfpn_deltas& d, double* om) {
+    std::vector<double> u(g.Nom), v(g.Nmu);
+    for (int i = 0; i < g.Nom; i++) u[i] = -0.5 * g.Lu + (i + 1) * d.du;
+    for (int j = 0; j < g.Nmu; j++) v[j] = -0.5 * g.Lv + (j + 1) * d.dv;
+    for (int j = 0; j < g.Nmu; j++)
+        for (int i = 0; i < g.Nom; i++) {
+            int idx = j * g.Nom + i;
+            om[idx * 2] = -v[g.Nmu - 1 - j];
+            om[idx * 2 + 1] = -u[i];
+        }
+}
+
+// initializeKernels, src/bfpm_solver.cu:1638-1813 (dense tables only)
+void build_kernels(int Ng, double dg, int nang, const double* mu, const double* sigma,
+                   const double* om, double* ke1, double* ke2, int* beg, int* end, double* kv) {
+    const size_t NG = Ng;
+    std::vector<double> init(NG * NG, 0.0), shifted(NG * NG, 0.0), ke(NG * NG, 0.0);
+    for (int j = 0; j < Ng - 1; j++)
+        for (int i = 0; i < Ng - 1; i++) {
+            if (mu[j * 3 + 2] == 0) init[i * NG + j] = 0.0;
+            else if (i < j) init[i * NG + j] = 0.0;
+            else init[i * NG + j] = dg / mu[j * 3 + 2] * std::exp(-(i - j + 0.5) * dg / mu[j * 3 + 2]);
+        }
+    for (int j = 0; j < Ng - 1; ++j)
+        for (int i = 0; i < Ng; ++i) {
+            if (i == j) shifted[i * NG + j] = 0.0;
+            else if (i < j) shifted[i * NG + j] = init[i * NG + j];
+            else shifted[i * NG + j] = init[(i - 1) * NG + j];
+        }
+    for (int r = 0; r < Ng; ++r)
+        for (int c = 0; c < Ng; ++c) {
+            double v = shifted[(NG - 1 - r) * NG + (NG - 1 - c)];
+            if (std::abs(v) < kSparseThreshold) v = 0.0;
+            ke[r * NG + c] = v * sigma[c];
+        }
+    for (int r = 0; r < Ng; ++r)
+        for (int c = 0; c < Ng; ++c) {
+            const double cur = ke[r * NG + c];
+            const double next = (r + 1 < Ng) ? ke[(r + 1) * NG + c] : 0.0;
+            ke1[r * NG + c] = 0.5 * (cur + next);
+            ke2[r * NG + c] = 0.5 * (-cur + next);
+        }
+    for (int r = 0; r < Ng; ++r) {
+        int b = Ng, e = 0;
+        for (int c = 0; c < Ng; ++c)
+            if (ke1[r * NG + c] != 0.0 || ke2[r * NG + c] != 0.0) {
+                b = std::min(b, c);
+                e = std::max(e, c + 1);
+            }
+        if (b == Ng) { b = 0; e = 0; }
+        beg[r] = b;
+        end[r] = e;
+    }
+    const size_t NA = nang;
+    std::vector<double> angles(NA * NA, 0.0);
+    for (int out = 0; out < nang; ++out) {
+        const double oy = om[out * 2], oz = om[out * 2 + 1];
+        const double on = std::sqrt(1.0 + oy * oy + oz * oz);
+        for (int in = 0; in < nang; ++in) {
+            const double iy = om[in * 2], iz = om[in * 2 + 1];
+            const double inn = std::sqrt(1.0 + iy * iy + iz * iz);
+            double ct = (1.0 + iy * oy + iz * oz) / std::max(inn * on, kEps);
+            ct = std::max(-1.0, std::min(1.0, ct));
+            angles[in * NA + out] = std::abs(std::acos(ct) * 180.0 / kPi);
+        }
+    }
+    std::fill(kv, kv + NG * NA * NA, 0.0);
+    for (int ek = 0; ek < Ng - 1; ++ek) {
+        if (mu[ek * 3 + 0] == 0.0 || mu[ek * 3 + 1] == 0.0) continue;
+        double* K = kv + static_cast<size_t>(ek) * NA * NA;
+        for (int out = 0; out < nang; ++out) {
+            double col = 0.0;
+            for (int in = 0; in < nang; ++in) {
+                const double th = angles[in * NA + out];
+                const double iy = om[in * 2], iz = om[in * 2 + 1];
+                double v = std::pow(th, mu[ek * 3 + 0] - 1.0) * std::exp(-th / mu[ek * 3 + 1]) *
+                           std::pow(1.0 + iy * iy + iz * iz, -1.5);
+                if (!std::isfinite(v)) v = 0.0;
+                K[in * NA + out] = v;
+                col += v;
+            }
+            if (col > kEps)
+                for (int in = 0; in < nang; ++in) {
+                    double& v = K[in * NA + out];
+                    v /= col;
+                    if (v < kKernelThreshold) v = 0.0;
+                }
+        }
+    }
+}
+
+// initializeDistribution (in-core branch), src/bfpm_solver.cu:1815-1884, 1978-2025
+bool init_beam(const bfpn_grid& g, const bfpn_phys& p, const bfpn_deltas& d, const double* om,
+               double* F, double* masses) {
+    const size_t nyz = static_cast<size_t>(g.Ny + 1) * (g.Nz + 1);
+    const size_t nang = static_cast<size_t>(g.Nmu) * g.Nom;
+    double c1 = std::numeric_limits<double>::max(), c2 = std::numeric_limits<double>::max();
+    for (int j = 0; j < g.Nmu; ++j) c1 = std::min(c1, std::abs(-0.5 * g.Lv + (j + 1) * d.dv));
+    for (int i = 0; i < g.Nom; ++i) c2 = std::min(c2, std::abs(-0.5 * g.Lu + (i + 1) * d.du));
+    int n1 = 0, n2 = 0;
+    double amass = 0.0;
+    for (int j = 0; j < g.Nmu; ++j) {
+        double v = -0.5 * g.Lv + (j + 1) * d.dv;
+        double delta = std::abs(v) - c1;
+        if (std::abs(delta) < 0.25 * d.dv) ++n1;
+        for (int i = 0; i < g.Nom; ++i) {
+            double u = -0.5 * g.Lu + (i + 1) * d.du;
+            double du_ = std::abs(u) - c2;
+            amass += std::exp(-p.a_2 * (delta * delta + du_ * du_)) * d.du * d.dv;
+        }
+    }
+    for (int i = 0; i < g.Nom; ++i) {
+        double u = -0.5 * g.Lu + (i + 1) * d.du;
+        if (std::abs(std::abs(u) - c2) < 0.25 * d.du) ++n2;
+    }
+    if (p.discrete_angle_delta) amass = 1.0;
+    double smass = 0.0;
+    for (int j = 0; j <= g.Nz; ++j) {
+        double z = -0.5 * g.Lz + j * d.dz;
+        double wz = p.trapezoidal_yz && (j == 0 || j == g.Nz) ? 0.5 : 1.0;
+        for (int i = 0; i <= g.Ny; ++i) {
+            double y = -0.5 * g.Ly + i * d.dy;
+            double wy = p.trapezoidal_yz && (i == 0 || i == g.Ny) ? 0.5 : 1.0;
+            smass += wy * wz * std::exp(-p.a_1 * (y * y + z * z));
+        }
+    }
+    smass *= d.dy * d.dz;
+    double emass = 0.0;
+    for (int ek = 0; ek < g.Ng; ++ek) {
+        double E = (ek + 0.5) * d.dg + 1.0;
+        emass += 1.0 / std::sqrt(2.0 * kPi) / p.sig_E *
+                 std::exp(-std::pow((E - p.beam_energy) / p.sig_E / std::sqrt(2.0), 2));
+    }
+    emass *= d.dg;
+    double mass = smass * amass * emass;
+    if (!(mass > 0.0) || !std::isfinite(mass)) {
+        g_err = "Initial beam has invalid discrete phase-space mass";
+        return false;
+    }
+    double scale = p.normalize_initial_mass ? 1.0 / mass : 1.0;
+    if (masses) { masses[0] = smass; masses[1] = amass; masses[2] = emass; masses[3] = scale; }
+    for (size_t k = 0; k < nang; k++) {
+        double o1 = om[k * 2], o2 = om[k * 2 + 1];
+        for (int j = 0; j <= g.Nz; j++)
+            for (int i = 0; i <= g.Ny; i++) {
+                double y = -0.5 * g.Ly + i * d.dy;
+                double z = -0.5 * g.Lz + j * d.dz;
+                double d1 = std::abs(o1) - c1;
+                double d2 = std::abs(o2) - c2;
+                double ang = std::exp(-p.a_2 * (d1 * d1 + d2 * d2));
+                if (p.discrete_angle_delta) {
+                    bool sel = std::abs(d1) < 0.25 * d.dv && std::abs(d2) < 0.25 * d.du;
+                    ang = sel ? 1.0 / (n1 * n2 * d.du * d.dv) : 0.0;
+                }
+                double f1 = scale * std::exp(-p.a_1 * (y * y + z * z)) * ang;
+                for (int ek = 0; ek < g.Ng; ek++) {
+                    double E = (ek + 0.5) * d.dg + 1.0;
+                    double f2 = 1.0 / std::sqrt(2.0 * kPi) / p.sig_E *
+                                std::exp(-std::pow((E - p.beam_energy) / p.sig_E / std::sqrt(2.0), 2));
+                    size_t idx = (ek * nyz + j * (g.Ny + 1) + i) * nang + k;
+                    F[idx] = f1 * f2;
+                }
+            }
+    }
+    return true;
+}
+
+// ------------------------------------------------------------- operators ---
+
+// energyDgCnKernel, eq15 branch, src/energy_solver.cu:41-87, 108-160, plus the
+// copy-back of EnergyDepositionSolver::solve (src/energy_solver.cu:497-505).
+void energy_half(double* F, double* f, const double* S_s, const double* sigma_c,
+                 const double* T_c, double dt, double dg, double density, int Ng, long long nyz,
+                 int nang, const double* Q1, const double* Q2, bool straggling) {
+    const long long n_per_E = nyz * nang;
+    const size_t total = static_cast<size_t>(Ng) * n_per_E;
+    std::vector<double> Fn(total), fn(total);
+    const double dg2 = std::max(dg * dg, kEps);
+#pragma omp parallel for schedule(static)
+    for (long long lane = 0; lane < n_per_E; ++lane) {
+        auto load = [&](const double* a, int ek) -> double {
+            if (!a || ek < 0 || ek >= Ng) return 0.0;
+            return a[static_cast<long long>(ek) * n_per_E + lane];
+        };
+        double next_F = 0.0, next_f = 0.0;
+        for (int ek = Ng - 1; ek >= 0; --ek) {
+            long long idx = static_cast<long long>(ek) * n_per_E + lane;
+            double Fg = F[idx];
+            double fg = f ? f[idx] : 0.0;
+            double sig = std::max(sigma_c[ek], 0.0);
+            double S_lo = std::max(S_s[ek], 0.0);
+            double S_hi = std::max(S_s[ek + 1], 0.0);
+            double source = Q1 ? Q1[idx] : 0.0;
+            double source2 = Q2 ? Q2[idx] : 0.0;
+            double T_g = (straggling && T_c) ? std::max(T_c[ek], 0.0) : 0.0;
+            double T_hi = (straggling && T_c && ek + 1 < Ng) ? std::max(T_c[ek + 1], 0.0) : 0.0;
+            double T_lo = (straggling && T_c && ek > 0) ? std::max(T_c[ek - 1], 0.0) : 0.0;
+            const double p1_hi = load(F, ek + 1);
+            const double p2_hi = load(f, ek + 1);
+            const double p1_lo = load(F, ek - 1);
+            const double up = p1_hi - p2_hi;
+            const double cur = Fg - fg;
+            const double den_F = std::max(1.0 / dt + 0.5 * density * S_lo / dg + 0.5 * sig, kEps);
+            const double inv_F = 1.0 / den_F;
+            const double A = (1.5 * density * S_hi / dg) * inv_F;
+            const double F_from_f = (0.5 * density * S_lo / dg) * inv_F;
+            const double F_from_hi = (0.5 * density * S_hi / dg) * inv_F;
+            const double strag_rhs = 0.5 * density * T_hi / dg2 * p1_hi +
+                                     0.5 * density * T_lo / dg2 * p1_lo - density * T_g / dg2 * Fg;
+            const double rhs4 = inv_F * (0.5 * density * S_hi / dg * up -
+                                         0.5 * density * S_lo / dg * cur - 0.5 * sig * Fg +
+                                         Fg / dt + strag_rhs + source);
+            const double rhs1e = fg / dt - 0.5 * sig * fg;
+            const double rhs2e = 1.5 * density * S_hi / dg * up + 1.5 * density * S_lo / dg * cur -
+                                 1.5 * density * (S_lo + S_hi) / dg * Fg -
+                                 0.5 * density * (S_hi - S_lo) / dg * fg;
+            const double rhs3e = A * (0.5 * density * S_hi / dg * up -
+                                      0.5 * density * S_lo / dg * cur - 0.5 * sig * Fg + Fg / dt +
+                                      strag_rhs + source);
+            const double high2 = 1.5 * density * S_hi / dg;
+            const double hi_delta = next_F - next_f;
+            const double rhs = rhs1e + rhs2e - rhs3e + source2 +
+                               (high2 - A * 0.5 * density * S_hi / dg) * hi_delta;
+            const double coe = 1.0 / dt + 0.5 * sig + 1.5 * density * S_lo / dg +
+                               0.5 * density * (S_hi - S_lo) / dg + A * 0.5 * density * S_lo / dg;
+            double f_val = rhs / std::max(coe, kEps);
+            double F_val = F_from_f * f_val + rhs4 + F_from_hi * hi_delta;
+            Fn[idx] = F_val;
+            fn[idx] = f_val;
+            next_F = F_val;
+            next_f = f_val;
+        }
+    }
+    std::memcpy(F, Fn.data(), total * sizeof(double));
+    if (f) std::memcpy(f, fn.data(), total * sizeof(double));
+}
+
+// secondarySourceFromPrimaryKernel, src/energy_solver.cu:287-333.  The dense
+// ker_v is walked through its nonzeros per output column in increasing input
+// order, which is the reference's order with its zero skips.
+void secondary_source(const double* Fp, const double* fp, double* Q, const double* ke1,
+                      const double* ke2, const double* kv, const int* beg, const int* end, int Ng,
+                      long long nyz, int nang, double dg) {
+    const size_t NA = nang;
+    std::vector<std::vector<int>> col_start(Ng);
+    std::vector<std::vector<int>> col_in(Ng);
+    std::vector<std::vector<double>> col_w(Ng);
+    for (int e = 0; e < Ng; ++e) {
+        col_start[e].assign(nang + 1, 0);
+        const double* K = kv + static_cast<size_t>(e) * NA * NA;
+        for (int out = 0; out < nang; ++out) {
+            col_start[e][out] = static_cast<int>(col_in[e].size());
+            for (int in = 0; in < nang; ++in) {
+                double w = K[in * NA + out];
+                if (w != 0.0) { col_in[e].push_back(in); col_w[e].push_back(w); }
+            }
+        }
+        col_start[e][nang] = static_cast<int>(col_in[e].size());
+    }
+#pragma omp parallel
+    {
+        std::vector<double> q1(nang), q2(nang);
+#pragma omp for schedule(dynamic, 4) collapse(2)
+        for (int ek = 0; ek < Ng; ++ek)
+            for (long long cell = 0; cell < nyz; ++cell) {
+                std::fill(q1.begin(), q1.end(), 0.0);
+                std::fill(q2.begin(), q2.end(), 0.0);
+                const int sb = std::max(0, beg[ek]);
+                const int se = std::min(Ng, end[ek]);
+                for (int src = sb; src < se; ++src) {
+                    const double e1 = ke1[static_cast<size_t>(ek) * Ng + src];
+                    const double e2 = ke2[static_cast<size_t>(ek) * Ng + src];
+                    if (e1 == 0.0 && e2 == 0.0) continue;
+                    const long long base = (static_cast<long long>(src) * nyz + cell) * nang;
+                    const auto& cs = col_start[src];
+                    const auto& ci = col_in[src];
+                    const auto& cw = col_w[src];
+                    for (int out = 0; out < nang; ++out)
+                        for (int k = cs[out]; k < cs[out + 1]; ++k) {
+                            const double w = cw[k];
+                            const double p1 = std::max(Fp[base + ci[k]], 0.0);
+                            const double p2 = fp ? fp[base + ci[k]] : 0.0;
+                            q1[out] += p1 * e1 * w;
+                            q2[out] += p2 * e2 * w;
+                        }
+                }
+                double* qo = Q + (static_cast<long long>(ek) * nyz + cell) * nang;
+                for (int out = 0; out < nang; ++out) qo[out] = dg * (q1[out] + q2[out] / 3.0);
+            }
+    }
+}
+
+// The same source with the angular contraction done first:
+//   A1[g',s,m] = sum_m' Kv[g',m',m] max(Fp[g',s,m'],0),  A2 likewise with fp,
+//   Q[g,s,m]   = dg * (sum_g' K1[g,g'] A1 + (sum_g' K2[g,g'] A2) / 3).
+// Mathematically identical to secondary_source(); only the summation order
+// differs (checked against it to 1e-13 relative in tests).  The oracle's full
+// solve uses this form because it is ~nang/2 times cheaper.
+void secondary_source_factorized(const double* Fp, const double* fp, double* Q, const double* ke1,
+                                 const double* ke2, const double* kv, const int* beg,
+                                 const int* end, int Ng, long long nyz, int nang, double dg) {
+    const size_t NA = nang;
+    const size_t n_per_E = static_cast<size_t>(nyz) * NA;
+    std::vector<double> A1(static_cast<size_t>(Ng) * n_per_E), A2(A1.size());
+#pragma omp parallel for schedule(static) collapse(2)
+    for (int e = 0; e < Ng; ++e)
+        for (long long cell = 0; cell < nyz; ++cell) {
+            const double* K = kv + static_cast<size_t>(e) * NA * NA;
+            const size_t base = static_cast<size_t>(e) * n_per_E + static_cast<size_t>(cell) * NA;
+            double* a1 = A1.data() + base;
+            double* a2 = A2.data() + base;
+            for (size_t out = 0; out < NA; ++out) { a1[out] = 0.0; a2[out] = 0.0; }
+            for (size_t in = 0; in < NA; ++in) {
+                const double p1 = std::max(Fp[base + in], 0.0);
+                const double p2 = fp ? fp[base + in] : 0.0;
+                const double* Kr = K + in * NA;
+                for (size_t out = 0; out < NA; ++out) {
+                    a1[out] += Kr[out] * p1;
+                    a2[out] += Kr[out] * p2;
+                }
+            }
+        }
+#pragma omp parallel for schedule(static)
+    for (int ek = 0; ek < Ng; ++ek) {
+        const int sb = std::max(0, beg[ek]);
+        const int se = std::min(Ng, end[ek]);
+        double* qo = Q + static_cast<size_t>(ek) * n_per_E;
+        std::vector<double> q1(n_per_E, 0.0), q2(n_per_E, 0.0);
+        for (int src = sb; src < se; ++src) {
+            const double e1 = ke1[static_cast<size_t>(ek) * Ng + src];
+            const double e2 = ke2[static_cast<size_t>(ek) * Ng + src];
+            if (e1 == 0.0 && e2 == 0.0) continue;
+            const double* a1 = A1.data() + static_cast<size_t>(src) * n_per_E;
+            const double* a2 = A2.data() + static_cast<size_t>(src) * n_per_E;
+            for (size_t k = 0; k < n_per_E; ++k) {
+                q1[k] += e1 * a1[k];
+                q2[k] += e2 * a2[k];
+            }
+        }
+        for (size_t k = 0; k < n_per_E; ++k) qo[k] = dg * (q1[k] + q2[k] / 3.0);
+    }
+}
+
+// muscl_paper_slope, src/spatial_transport.cu:23-29
+inline double muscl_slope(double left, double center, double right) {
+    double forward = right - center;
+    if (std::fabs(forward) < kEps) return 0.0;
+    double theta = (center - left) / forward;
+    double eta = std::max(0.0, std::max(std::min(2.0 * theta, 1.0), std::min(theta, 2.0)));
+    return forward * eta;
+}
+
+// spatial_flux_divergence_value, src/spatial_transport.cu:31-102
+double flux_div(const double* Fin, const double* om, int ek, int ang, int i, int j, int Ny,
+                int Nz, int nang, double dy, double dz) {
+    const int Ny1 = Ny + 1;
+    const long long nyz = static_cast<long long>(Ny1) * (Nz + 1);
+    const long long s = static_cast<long long>(j) * Ny1 + i;
+    const long long base = (static_cast<long long>(ek) * nyz + s) * nang + ang;
+    const double oy = om[2 * ang], oz = om[2 * ang + 1];
+    const double c = Fin[base];
+    const int im2 = std::max(0, i - 2), im1 = std::max(0, i - 1);
+    const int ip1 = std::min(Ny, i + 1), ip2 = std::min(Ny, i + 2);
+    const long long pb = static_cast<long long>(ek) * nyz * nang + ang;
+    const long long yr = static_cast<long long>(j) * Ny1;
+    const double ym2 = Fin[pb + (yr + im2) * nang], ym1 = Fin[pb + (yr + im1) * nang];
+    const double yp1 = Fin[pb + (yr + ip1) * nang], yp2 = Fin[pb + (yr + ip2) * nang];
+    const double sym1 = muscl_slope(ym2, ym1, c), sy0 = muscl_slope(ym1, c, yp1),
+                 syp1 = muscl_slope(c, yp1, yp2);
+    double lfy, rfy;
+    if (oy >= 0.0) { lfy = oy * (ym1 + 0.5 * sym1); rfy = oy * (c + 0.5 * sy0); }
+    else { lfy = oy * (c - 0.5 * sy0); rfy = oy * (yp1 - 0.5 * syp1); }
+    const int jm2 = std::max(0, j - 2), jm1 = std::max(0, j - 1);
+    const int jp1 = std::min(Nz, j + 1), jp2 = std::min(Nz, j + 2);
+    const double zm2 = Fin[pb + (static_cast<long long>(jm2) * Ny1 + i) * nang];
+    const double zm1 = Fin[pb + (static_cast<long long>(jm1) * Ny1 + i) * nang];
+    const double zp1 = Fin[pb + (static_cast<long long>(jp1) * Ny1 + i) * nang];
+    const double zp2 = Fin[pb + (static_cast<long long>(jp2) * Ny1 + i) * nang];
+    const double szm1 = muscl_slope(zm2, zm1, c), sz0 = muscl_slope(zm1, c, zp1),
+                 szp1 = muscl_slope(c, zp1, zp2);
+    double lfz, rfz;
+    if (oz >= 0.0) { lfz = oz * (zm1 + 0.5 * szm1); rfz = oz * (c + 0.5 * sz0); }
+    else { lfz = oz * (c - 0.5 * sz0); rfz = oz * (zp1 - 0.5 * szp1); }
+    return (rfy - lfy) / dy + (rfz - lfz) / dz;
+}
+
+// SpatialTransportSolver::solveEq18 with its flux/predictor/corrector kernels,
+// src/spatial_transport.cu:104-161, 202-262
+void transport_half(double* F, const double* om, int Ny, int Nz, int Ng, int nang, double dy,
+                    double dz, double h, bool preserve_sign) {
+    const long long nyz = static_cast<long long>(Ny + 1) * (Nz + 1);
+    const size_t total = static_cast<size_t>(Ng) * nyz * nang;
+    std::vector<double> fo(total), pred(total), fp(total);
+    auto flux_all = [&](const double* in, double* out) {
+#pragma omp parallel for schedule(static) collapse(2)
+        for (int ek = 0; ek < Ng; ++ek)
+            for (long long s = 0; s < nyz; ++s) {
+                const int j = static_cast<int>(s / (Ny + 1));
+                const int i = static_cast<int>(s - static_cast<long long>(j) * (Ny + 1));
+                for (int a = 0; a < nang; ++a)
+                    out[(static_cast<long long>(ek) * nyz + s) * nang + a] =
+                        flux_div(in, om, ek, a, i, j, Ny, Nz, nang, dy, dz);
+            }
+    };
+    flux_all(F, fo.data());
+#pragma omp parallel for schedule(static)
+    for (long long k = 0; k < static_cast<long long>(total); ++k) {
+        double v = F[k] - h * fo[k];
+        pred[k] = preserve_sign ? v : std::max(v, 0.0);
+    }
+    flux_all(pred.data(), fp.data());
+#pragma omp parallel for schedule(static)
+    for (long long k = 0; k < static_cast<long long>(total); ++k) {
+        double v = F[k] - 0.5 * h * (fo[k] + fp[k]);
+        F[k] = preserve_sign ? v : std::max(v, 0.0);
+    }
+}
+
+// AngleDiffusionSolver::solveEq19 with buildDiffusionFactorKernel,
+// complexToRealKernel and the four separable DST-I kernels,
+// src/angle_diffusion.cu:7-34, 75-80, 254-362, 395-408, 420-483
+void angle_full(double* F, const double* sig_trg, long long nyz, int Ng, int Nmu, int Nom,
+                double du, double dv, double dt, double density) {
+    const int na = Nmu * Nom;
+    std::vector<double> su(static_cast<size_t>(Nom) * Nom), sv(static_cast<size_t>(Nmu) * Nmu);
+    for (int mu = 0; mu < Nom; ++mu)
+        for (int i = 0; i < Nom; ++i) su[mu * Nom + i] = std::sin(kPi * (mu + 1) * (i + 1) / (Nom + 1));
+    for (int mv = 0; mv < Nmu; ++mv)
+        for (int j = 0; j < Nmu; ++j) sv[mv * Nmu + j] = std::sin(kPi * (mv + 1) * (j + 1) / (Nmu + 1));
+    std::vector<double> coeff(static_cast<size_t>(Ng) * na);
+    const double norm = 4.0 / ((Nom + 1.0) * (Nmu + 1.0));
+    for (int ek = 0; ek < Ng; ++ek)
+        for (int a = 0; a < na; ++a) {
+            int i = a % Nom, j = a / Nom;
+            double lu = (2.0 - 2.0 * std::cos(kPi * (i + 1) / (Nom + 1))) / (du * du);
+            double lv = (2.0 - 2.0 * std::cos(kPi * (j + 1) / (Nmu + 1))) / (dv * dv);
+            double D = 0.5 * density * sig_trg[ek];
+            double lam = D * (lu + lv);
+            double ratio = (2.0 / dt - lam) / (2.0 / dt + lam);
+            coeff[static_cast<size_t>(ek) * na + a] = ratio * norm;
+        }
+    const long long batch = static_cast<long long>(Ng) * nyz;
+#pragma omp parallel
+    {
+        std::vector<double> Y(na), Z(na), W(na);
+#pragma omp for schedule(static)
+        for (long long b = 0; b < batch; ++b) {
+            double* in = F + b * na;
+            const int ek = static_cast<int>(b / nyz);
+            for (int v = 0; v < Nmu; ++v)
+                for (int mu = 0; mu < Nom; ++mu) {
+                    double sum = 0.0;
+                    for (int a = 0; a < Nom; ++a) sum += in[v * Nom + a] * su[mu * Nom + a];
+                    Y[v * Nom + mu] = sum;
+                }
+            for (int nu = 0; nu < Nmu; ++nu)
+                for (int mu = 0; mu < Nom; ++mu) {
+                    double sum = 0.0;
+                    for (int v = 0; v < Nmu; ++v) sum += sv[nu * Nmu + v] * Y[v * Nom + mu];
+                    Z[nu * Nom + mu] = sum * coeff[static_cast<size_t>(ek) * na + nu * Nom + mu];
+                }
+            for (int v = 0; v < Nmu; ++v)
+                for (int mu = 0; mu < Nom; ++mu) {
+                    double sum = 0.0;
+                    for (int nu = 0; nu < Nmu; ++nu) sum += sv[nu * Nmu + v] * Z[nu * Nom + mu];
+                    W[v * Nom + mu] = sum;
+                }
+            for (int v = 0; v < Nmu; ++v)
+                for (int a = 0; a < Nom; ++a) {
+                    double sum = 0.0;
+                    for (int mu = 0; mu < Nom; ++mu) sum += W[v * Nom + mu] * su[mu * Nom + a];
+                    in[v * Nom + a] = sum;
+                }
+        }
+    }
+}
+
+// integratedDepthDoseKernel, src/bfpm_solver.cu:229-309, reduced per energy
+// group in a fixed order.
+double idd_value(const double* F, const double* f, const double* F1, const double* f1,
+                 const double* S_s, const double* sigma_c, const double* en, int Ng, int Ny, int Nz,
+                 int nang, double dg, double du, double dv, double dy, double dz, double density,
+                 bool trapezoidal, bool catastrophic) {
+    const long long nyz = static_cast<long long>(Ny + 1) * (Nz + 1);
+    std::vector<double> part(Ng, 0.0);
+#pragma omp parallel for schedule(static)
+    for (int ek = 0; ek < Ng; ++ek) {
+        double acc = 0.0;
+        const double S_mid = 0.5 * (S_s[ek] + S_s[ek + 1]);
+        const double slope_w = (S_s[ek + 1] - S_s[ek]) / 6.0;
+        const double sigma = std::max(sigma_c[ek], 0.0);
+        const double E_mid = 0.5 * (en[ek] + en[ek + 1]);
+        for (long long s = 0; s < nyz; ++s) {
+            const int iy = static_cast<int>(s % (Ny + 1));
+            const int iz = static_cast<int>(s / (Ny + 1));
+            double w = 1.0;
+            if (trapezoidal) {
+                if (iy == 0 || iy == Ny) w *= 0.5;
+                if (iz == 0 || iz == Nz) w *= 0.5;
+            }
+            const long long base = (static_cast<long long>(ek) * nyz + s) * nang;
+            for (int a = 0; a < nang; ++a) {
+                const long long c = base + a;
+                double psi1 = F[c] + (F1 ? F1[c] : 0.0);
+                double psi2 = (f ? f[c] : 0.0) + (f1 ? f1[c] : 0.0);
+                double dose = psi1 * S_mid * dg + psi2 * slope_w * dg;
+                if (catastrophic)
+                    dose += psi1 * E_mid * sigma * dg + psi2 * (en[ek + 1] - en[ek]) * sigma * dg / 6.0;
+                acc += w * dose / density;
+            }
+        }
+        part[ek] = acc;
+    }
+    double sum = 0.0;
+    for (int ek = 0; ek < Ng; ++ek) sum += part[ek];
+    return sum * du * dv * dy * dz;
+}
+
+// ------------------------------------------------------------------ solver ---
+
+struct Solver {
+    bfpn_grid g{};
+    bfpn_phys p{};
+    bfpn_deltas d{};
+    long long nyz = 0;
+    int nang = 0;
+    size_t N = 0;
+    int mu_rows = 0;
+    double masses[4] = {0, 0, 0, 0};
+    std::vector<double> mu, sigma, S, T, sig_tr, en, om, ke1, ke2, kv, kbeg, kend;
+    std::vector<int> beg, end;
+    std::vector<double> F, f, F1, f1, Q;
+    std::vector<double> idd, proxy;
+    int step = 0;
+};
+
+bool solver_init(Solver& s, const bfpn_grid& g, const bfpn_phys& p, const std::string& path) {
+    s.g = g;
+    s.p = p;
+    s.d = deltas_of(g);
+    s.nyz = static_cast<long long>(g.Ny + 1) * (g.Nz + 1);
+    s.nang = g.Nmu * g.Nom;
+    s.N = static_cast<size_t>(g.Ng) * s.nyz * s.nang;
+    if (!load_tables(path, g.Ng, p.calibrate_cross_sections != 0, s.mu, s.sigma, &s.mu_rows))
+        return false;
+    const int Ng = g.Ng;
+    s.S.assign(Ng + 1, 0.0);
+    s.T.assign(Ng, 0.0);
+    s.sig_tr.assign(Ng, 0.0);
+    s.en.assign(Ng + 1, 0.0);
+    for (int i = 0; i <= Ng; i++) s.en[i] = i * s.d.dg + 1.0;
+    stopping_power(p.material_name, Ng, g.Lg, s.d.dg, s.S.data());
+    straggling(Ng, s.T.data());
+    transport_xs(Ng, s.d.dg, s.sig_tr.data());
+    s.om.assign(2 * static_cast<size_t>(s.nang), 0.0);
+    omend_table(g, s.d, s.om.data());
+    const size_t NG = Ng;
+    s.ke1.assign(NG * NG, 0.0);
+    s.ke2.assign(NG * NG, 0.0);
+    s.kv.assign(NG * s.nang * s.nang, 0.0);
+    s.beg.assign(Ng, 0);
+    s.end.assign(Ng, 0);
+    build_kernels(Ng, s.d.dg, s.nang, s.mu.data(), s.sigma.data(), s.om.data(), s.ke1.data(),
+                  s.ke2.data(), s.beg.data(), s.end.data(), s.kv.data());
+    s.kbeg.assign(s.beg.begin(), s.beg.end());
+    s.kend.assign(s.end.begin(), s.end.end());
+    s.F.assign(s.N, 0.0);
+    s.f.assign(s.N, 0.0);
+    s.F1.assign(s.N, 0.0);
+    s.f1.assign(s.N, 0.0);
+    if (!init_beam(g, p, s.d, s.om.data(), s.F.data(), s.masses)) return false;
+    return true;
+}
+
+// One depth step: BFPnSolver::solve loop body (src/bfpm_solver.cu:2189-2255)
+// with stepPrimary (3238-3311), stepPrimaryEnergyOnly (3313-3333),
+// stepSecondary (3335-3394), computeIntegratedDepthDose (3473-3502) and
+// computeScalarDoseProxy (3418-3444).
+void solver_step(Solver& s) {
+    const auto& g = s.g;
+    const auto& p = s.p;
+    const auto& d = s.d;
+    const double half = 0.5 * d.dt;
+    const bool strag = p.eq15_straggling != 0;
+    auto E = [&](std::vector<double>& a, std::vector<double>& b, const double* Q) {
+        energy_half(a.data(), b.data(), s.S.data(), s.sigma.data(), s.T.data(), half, d.dg,
+                    p.energy_density, g.Ng, s.nyz, s.nang, Q, nullptr, strag);
+    };
+    auto Tr = [&](std::vector<double>& a, bool preserve) {
+        transport_half(a.data(), s.om.data(), g.Ny, g.Nz, g.Ng, s.nang, d.dy, d.dz, half, preserve);
+    };
+    auto A = [&](std::vector<double>& a) {
+        angle_full(a.data(), s.sig_tr.data(), s.nyz, g.Ng, g.Nmu, g.Nom, d.du, d.dv, d.dt, p.density);
+    };
+    if (p.energy_only) {
+        E(s.F, s.f, nullptr);
+        E(s.F, s.f, nullptr);
+    } else {
+        E(s.F, s.f, nullptr);
+        if (!p.no_transport) { Tr(s.F, p.no_spatial_clipping != 0); Tr(s.f, true); }
+        if (!p.no_angle) { A(s.F); A(s.f); }
+        if (!p.no_transport) { Tr(s.F, p.no_spatial_clipping != 0); Tr(s.f, true); }
+        E(s.F, s.f, nullptr);
+    }
+    if (!p.energy_only && !p.primary_only && s.step > 0) {
+        // Both secondary energy half steps of the reference rebuild the source
+        // from the same primary state, so one evaluation serves both.
+        s.Q.assign(s.N, 0.0);
+        auto src = g_direct_source ? secondary_source : secondary_source_factorized;
+        src(s.F.data(), s.f.data(), s.Q.data(), s.ke1.data(), s.ke2.data(),
+                         s.kv.data(), s.beg.data(), s.end.data(), g.Ng, s.nyz, s.nang, d.dg);
+        E(s.F1, s.f1, s.Q.data());
+        Tr(s.F1, false);
+        Tr(s.f1, true);
+        A(s.F1);
+        A(s.f1);
+        Tr(s.F1, false);
+        Tr(s.f1, true);
+        E(s.F1, s.f1, s.Q.data());
+    }
+    double v = idd_value(s.F.data(), s.f.data(), s.F1.data(), s.f1.data(), s.S.data(),
+                         s.sigma.data(), s.en.data(), g.Ng, g.Ny, g.Nz, s.nang, d.dg, d.du, d.dv,
+                         d.dy, d.dz, p.density, p.trapezoidal_yz != 0, p.dose_catastrophic_loss != 0);
+    double px = 0.0;
+    for (size_t k = 0; k < s.N; ++k) px += s.f[k];
+    s.idd.push_back(v);
+    s.proxy.push_back(px * d.dg);
+    s.step++;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------- C API ---
+extern "C" {
+
+BFPN_API const char* orc_last_error(void) { return g_err.c_str(); }
+
+BFPN_API int orc_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+BFPN_API void orc_deltas(const bfpn_grid* g, bfpn_deltas* out) { *out = deltas_of(*g); }
+
+BFPN_API void orc_stopping_power(const char* material, int Ng, double Lg, double* S) {
+    stopping_power(material ? material : "", Ng, Lg, Lg / Ng, S);
+}
+BFPN_API void orc_straggling(int Ng, double* T) { straggling(Ng, T); }
+BFPN_API void orc_transport_xs(int Ng, double dg, double* sig) { transport_xs(Ng, dg, sig); }
+
+BFPN_API int orc_load_tables(const char* path, int Ng, int calibrate, double* mu, double* sigma,
+                             int* mu_rows) {
+    std::vector<double> m, s;
+    if (!load_tables(path, Ng, calibrate != 0, m, s, mu_rows)) return 1;
+    std::memcpy(mu, m.data(), m.size() * sizeof(double));
+    std::memcpy(sigma, s.data(), s.size() * sizeof(double));
+    return 0;
+}
+
+BFPN_API void orc_omend(const bfpn_grid* g, double* om) { omend_table(*g, deltas_of(*g), om); }
+
+BFPN_API void orc_build_kernels(int Ng, double dg, int nang, const double* mu,
+                                const double* sigma, const double* om, double* ke1, double* ke2,
+                                int* beg, int* end, double* kv) {
+    build_kernels(Ng, dg, nang, mu, sigma, om, ke1, ke2, beg, end, kv);
+}
+
+BFPN_API int orc_init_beam(const bfpn_grid* g, const bfpn_phys* p, double* F, double* masses) {
+    bfpn_deltas d = deltas_of(*g);
+    std::vector<double> om(2 * static_cast<size_t>(g->Nmu) * g->Nom);
+    omend_table(*g, d, om.data());
+    return init_beam(*g, *p, d, om.data(), F, masses) ? 0 : 1;
+}
+
+BFPN_API void orc_energy_half(double* F, double* f, const double* S, const double* sigma,
+                              const double* T, double h, double dg, double density, int Ng,
+                              long long nyz, int nang, const double* Q1, const double* Q2,
+                              int straggling) {
+    energy_half(F, f, S, sigma, T, h, dg, density, Ng, nyz, nang, Q1, Q2, straggling != 0);
+}
+
+BFPN_API void orc_secondary_source(const double* Fp, const double* fp, double* Q,
+                                   const double* ke1, const double* ke2, const double* kv,
+                                   const int* beg, const int* end, int Ng, long long nyz,
+                                   int nang, double dg) {
+    secondary_source(Fp, fp, Q, ke1, ke2, kv, beg, end, Ng, nyz, nang, dg);
+}
+
+BFPN_API void orc_secondary_source_factorized(const double* Fp, const double* fp, double* Q,
+                                              const double* ke1, const double* ke2,
+                                              const double* kv, const int* beg, const int* end,
+                                              int Ng, long long nyz, int nang, double dg) {
+    secondary_source_factorized(Fp, fp, Q, ke1, ke2, kv, beg, end, Ng, nyz, nang, dg);
+}
+
+BFPN_API void orc_set_direct_source(int on) { g_direct_source = on != 0; }
+
+BFPN_API void orc_transport_half(double* F, const double* om, int Ny, int Nz, int Ng, int nang,
+                                 double dy, double dz, double h, int preserve_sign) {
+    transport_half(F, om, Ny, Nz, Ng, nang, dy, dz, h, preserve_sign != 0);
+}
+
+BFPN_API void orc_angle_full(double* F, const double* sig_tr, long long nyz, int Ng, int Nmu,
+                             int Nom, double du, double dv, double dt, double density) {
+    angle_full(F, sig_tr, nyz, Ng, Nmu, Nom, du, dv, dt, density);
+}
+
+BFPN_API double orc_idd(const double* F, const double* f, const double* F1, const double* f1,
+                        const double* S, const double* sigma, const double* en, int Ng, int Ny,
+                        int Nz, int nang, double dg, double du, double dv, double dy, double dz,
+                        double density, int trapezoidal, int catastrophic) {
+    return idd_value(F, f, F1, f1, S, sigma, en, Ng, Ny, Nz, nang, dg, du, dv, dy, dz, density,
+                     trapezoidal != 0, catastrophic != 0);
+}
+
+BFPN_API void* orc_create(const bfpn_grid* g, const bfpn_phys* p, const char* path) {
+    auto* s = new Solver();
+    if (!solver_init(*s, *g, *p, path ? path : "./data")) {
+        delete s;
+        return nullptr;
+    }
+    return s;
+}
+
+BFPN_API void orc_destroy(void* h) { delete static_cast<Solver*>(h); }
+
+BFPN_API int orc_num_steps(void* h, double t_final) {
+    auto* s = static_cast<Solver*>(h);
+    return static_cast<int>(std::ceil(t_final / s->d.dt - 1e-12));
+}
+
+BFPN_API void orc_advance(void* h, int nsteps) {
+    auto* s = static_cast<Solver*>(h);
+    for (int k = 0; k < nsteps; ++k) solver_step(*s);
+}
+
+BFPN_API int orc_history(void* h, double* idd, double* proxy, int cap) {
+    auto* s = static_cast<Solver*>(h);
+    int n = std::min<int>(cap, static_cast<int>(s->idd.size()));
+    for (int k = 0; k < n; ++k) {
+        if (idd) idd[k] = s->idd[k];
+        if (proxy) proxy[k] = s->proxy[k];
+    }
+    return static_cast<int>(s->idd.size());
+}
+
+BFPN_API size_t orc_cells(void* h) { return static_cast<Solver*>(h)->N; }
+BFPN_API int orc_mu_rows(void* h) { return static_cast<Solver*>(h)->mu_rows; }
+BFPN_API void orc_masses(void* h, double* out4) {
+    auto* s = static_cast<Solver*>(h);
+    for (int k = 0; k < 4; ++k) out4[k] = s->masses[k];
+}
+
+// which: 0 F, 1 f, 2 F1, 3 f1 (writable views)
+BFPN_API double* orc_state(void* h, int which) {
+    auto* s = static_cast<Solver*>(h);
+    switch (which) {
+        case 0: return s->F.data();
+        case 1: return s->f.data();
+        case 2: return s->F1.data();
+        case 3: return s->f1.data();
+        default: return nullptr;
+    }
+}
+
+// Same table numbering as bfpn_get_table.
+BFPN_API const double* orc_table(void* h, int which, size_t* count) {
+    auto* s = static_cast<Solver*>(h);
+    const std::vector<double>* v = nullptr;
+    switch (which) {
+        case 0: v = &s->S; break;
+        case 1: v = &s->sigma; break;
+        case 2: v = &s->T; break;
+        case 3: v = &s->sig_tr; break;
+        case 4: v = &s->en; break;
+        case 5: v = &s->ke1; break;
+        case 6: v = &s->ke2; break;
+        case 7: v = &s->kv; break;
+        case 8: v = &s->om; break;
+        case 9: v = &s->mu; break;
+        case 10: v = &s->kbeg; break;
+        case 11: v = &s->kend; break;
+        default: *count = 0; return nullptr;
+    }
+    *count = v->size();
+    return v->data();
+}
+
+}  // extern "C"

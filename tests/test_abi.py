@@ -1,0 +1,78 @@
+"""The C-ABI library loads and exports every entry point include/bfpn.h declares;
+the CLI front end parses like the reference's main (src/main.cu:91-273).
+These run without a GPU (no compute call is made)."""
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+HEADER = os.path.join(ROOT, "include", "bfpn.h")
+CLI = os.path.join(ROOT, "bfpn_gpu_amd", "bin", "bfp_solver")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"BFPN_API\s+[\w\s\*]+?\b(bfpn_\w+)\s*\(", text)))
+
+
+def test_header_declares_the_operator_and_context_api():
+    names = declared_symbols()
+    for required in ("bfpn_create", "bfpn_advance", "bfpn_history", "bfpn_run", "bfpn_destroy",
+                     "bfpn_energy_half_f32", "bfpn_transport_half_f64", "bfpn_angle_full_f32",
+                     "bfpn_secondary_source_f32", "bfpn_idd_f64", "bfpn_last_error"):
+        assert required in names
+    assert len(names) >= 35
+
+
+def test_library_exports_every_declared_symbol():
+    import bfpn_gpu_amd as bg
+
+    L = bg.lib()
+    missing = [n for n in declared_symbols() if not hasattr(L, n)]
+    assert not missing, missing
+    assert L.bfpn_version().decode().startswith("bfpn-mi355x")
+    out = subprocess.run(["nm", "-D", "--defined-only", bg.lib_path()], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    stray = sorted(n for n in exported if not n.startswith("bfpn_") and not n.startswith("_"))
+    assert not stray, f"unexpected exports: {stray[:10]}"
+
+
+def test_errors_are_status_codes_not_exits():
+    import ctypes as C
+
+    import bfpn_gpu_amd as bg
+    from bfpn_gpu_amd.abi import Case
+
+    L = bg.lib()
+    h = C.c_void_p(0)
+    c = Case(Ny=0)  # invalid grid is rejected before any device call
+    g, p = c.grid(), c.phys()
+    st = L.bfpn_create(C.byref(g), C.byref(p), b"./data", 0, C.byref(h))
+    assert st == 1 and not h.value
+    assert b"positive" in L.bfpn_last_error()
+    assert L.bfpn_advance(None, 1) == 1
+
+
+def _cli(*args, cwd=None):
+    return subprocess.run([CLI, *args], capture_output=True, text=True, cwd=cwd, timeout=60)
+
+
+def test_cli_help_and_test_mode():
+    r = _cli("--help")
+    assert r.returncode == 0 and "--eq15-straggling" in r.stdout and "<Nx>" in r.stdout
+    r = _cli("--test", "--bogus-flag", "123")
+    assert r.returncode == 0 and "Running in test mode" in r.stdout
+
+
+@pytest.mark.parametrize("args,msg", [
+    (("--material", "steel"), "Unknown material: steel"),
+    (("--energy-model", "foo"), "Unknown energy model: foo"),
+    (("--lite-memory", "--streaming-full"), "mutually exclusive"),
+])
+def test_cli_rejects_like_reference(args, msg):
+    r = _cli(*args)
+    assert r.returncode == 1 and msg in r.stderr

@@ -1,0 +1,143 @@
+"""Operator-level parity: HIP kernels through the C ABI vs the CPU FP64 oracle.
+
+Inputs are a physical state (the oracle's small beam after a few depth steps,
+so tails, zeros and both signs of psi^2 occur) plus seeded random fields.
+Tolerances, relative to the largest magnitude of the reference output:
+  FP64 state: 1e-12 (same algebra; only FMA contraction and summation order differ)
+  FP32 state: energy 2e-6, source 1e-5, angle 1e-5, transport 5e-5, IDD 1e-6
+              (transport: a single-precision limiter can take the other branch
+              of the superbee kink in isolated cells).
+"""
+import numpy as np
+import pytest
+
+from gpu_cases import small_case
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(7)
+TOL64 = 1e-12
+TOL32 = {"energy": 2e-6, "source": 1e-5, "angle": 1e-5, "transport": 5e-5}
+
+
+@pytest.fixture(scope="module")
+def physical(oracle_mod):
+    c = small_case()
+    s = oracle_mod.Solver(c)
+    s.advance(3)
+    st = [s.state(k).copy() for k in range(4)]
+    tabs = {k: s.table(k) for k in ("S", "sigma", "T", "sig_tr", "en", "ker_e1", "ker_e2",
+                                    "ker_v", "omend")}
+    tabs["beg"] = s.table("ker_beg").astype(np.int32)
+    tabs["end"] = s.table("ker_end").astype(np.int32)
+    return c, oracle_mod.deltas(c), st, tabs
+
+
+def _close(gpu_out, ref, tol):
+    scale = max(np.abs(ref).max(), 1e-300)
+    err = np.abs(gpu_out - ref).max() / scale
+    assert err <= tol, f"max error {err:.3e} > {tol:.1e} (scale {scale:.3e})"
+
+
+def _dev(bg, a, dtype):
+    return bg.DeviceArray(np.asarray(a, dtype=dtype))
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("straggling,with_source", [(False, False), (True, False), (False, True),
+                                                     (True, True)])
+def test_energy_half(gpu, oracle_mod, physical, dtype, straggling, with_source):
+    c, d, st, t = physical
+    F, f = st[0].copy(), st[1].copy()
+    Q = RNG.random(c.cells) * np.abs(F).max() * 0.1 if with_source else None
+    if with_source and dtype == np.float32:
+        Q = Q.astype(np.float32).astype(np.float64)
+    F = F.astype(dtype).astype(np.float64)
+    f = f.astype(dtype).astype(np.float64)
+    dF, df = _dev(gpu, F, dtype), _dev(gpu, f, dtype)
+    S, sig, T = (_dev(gpu, t[k], np.float64) for k in ("S", "sigma", "T"))
+    dQ = _dev(gpu, Q, dtype) if with_source else None
+    h = 0.5 * d.dt
+    gpu.ops.energy_half(dF, df, S, sig, T, h, d.dg, 1.0, c.Ng, c.nyz, c.nang, Q1=dQ,
+                        straggling=straggling)
+    oracle_mod.energy_half(F, f, t["S"], t["sigma"], t["T"], h, d.dg, 1.0, c.Ng, c.nyz, c.nang,
+                           Q1=Q, straggling=straggling)
+    tol = TOL64 if dtype == np.float64 else TOL32["energy"]
+    _close(dF.numpy().astype(np.float64), F, tol)
+    _close(df.numpy().astype(np.float64), f, tol)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("which,preserve", [(0, False), (1, True), (0, True)])
+def test_transport_half(gpu, oracle_mod, physical, dtype, which, preserve):
+    c, d, st, t = physical
+    F = st[which].astype(dtype).astype(np.float64)
+    dF = _dev(gpu, F, dtype)
+    om = _dev(gpu, t["omend"], np.float64)
+    gpu.ops.transport_half(dF, om, c.Ny, c.Nz, c.Ng, c.nang, d.dy, d.dz, 0.5 * d.dt, preserve)
+    oracle_mod.transport_half(F, t["omend"], c.Ny, c.Nz, c.Ng, c.nang, d.dy, d.dz, 0.5 * d.dt,
+                              preserve)
+    _close(dF.numpy().astype(np.float64), F, TOL64 if dtype == np.float64 else TOL32["transport"])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_transport_random_field_large_velocity(gpu, oracle_mod, dtype):
+    Ny, Nz, Ng, nang = 9, 6, 3, 7
+    F = RNG.standard_normal(Ng * (Ny + 1) * (Nz + 1) * nang).astype(dtype).astype(np.float64)
+    om = RNG.uniform(-0.9, 0.9, 2 * nang)
+    dF = _dev(gpu, F, dtype)
+    gpu.ops.transport_half(dF, _dev(gpu, om, np.float64), Ny, Nz, Ng, nang, 0.2, 0.3, 0.05, False)
+    oracle_mod.transport_half(F, om, Ny, Nz, Ng, nang, 0.2, 0.3, 0.05, False)
+    _close(dF.numpy().astype(np.float64), F, TOL64 if dtype == np.float64 else TOL32["transport"])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("which", [0, 1])
+def test_angle_full(gpu, oracle_mod, physical, dtype, which):
+    c, d, st, t = physical
+    F = st[which].astype(dtype).astype(np.float64)
+    dF = _dev(gpu, F, dtype)
+    gpu.ops.angle_full(dF, t["sig_tr"], c.nyz, c.Ng, c.Nmu, c.Nom, d.du, d.dv, d.dt, 1.0)
+    oracle_mod.angle_full(F, t["sig_tr"], c.nyz, c.Ng, c.Nmu, c.Nom, d.du, d.dv, d.dt, 1.0)
+    _close(dF.numpy().astype(np.float64), F, TOL64 if dtype == np.float64 else TOL32["angle"])
+
+
+@pytest.mark.parametrize("Nmu,Nom", [(20, 20), (7, 3), (1, 9)])
+def test_angle_shapes(gpu, oracle_mod, Nmu, Nom):
+    Ng, nyz = 4, 5
+    F = RNG.standard_normal(Ng * nyz * Nmu * Nom)
+    sig = RNG.random(Ng) * 1e-3
+    dF = _dev(gpu, F, np.float64)
+    gpu.ops.angle_full(dF, sig, nyz, Ng, Nmu, Nom, 0.1, 0.07, 0.02, 1.85)
+    oracle_mod.angle_full(F, sig, nyz, Ng, Nmu, Nom, 0.1, 0.07, 0.02, 1.85)
+    _close(dF.numpy(), F, TOL64)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_secondary_source(gpu, oracle_mod, physical, dtype):
+    c, d, st, t = physical
+    Fp = st[0].astype(dtype).astype(np.float64)
+    fp = st[1].astype(dtype).astype(np.float64)
+    Q = oracle_mod.secondary_source(Fp, fp, t["ker_e1"], t["ker_e2"], t["ker_v"], t["beg"],
+                                    t["end"], c.Ng, c.nyz, c.nang, d.dg)
+    assert np.abs(Q).max() > 0
+    dQ = gpu.DeviceArray(shape=c.cells, dtype=dtype)
+    gpu.ops.secondary_source(_dev(gpu, Fp, dtype), _dev(gpu, fp, dtype), dQ,
+                             _dev(gpu, t["ker_e1"], np.float64), _dev(gpu, t["ker_e2"], np.float64),
+                             _dev(gpu, t["ker_v"], np.float64), _dev(gpu, t["beg"], np.int32),
+                             _dev(gpu, t["end"], np.int32), c.Ng, c.nyz, c.nang, d.dg)
+    _close(dQ.numpy().astype(np.float64), Q, TOL64 if dtype == np.float64 else TOL32["source"])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("trap,cat", [(True, True), (False, False)])
+def test_idd(gpu, oracle_mod, physical, dtype, trap, cat):
+    c, d, st, t = physical
+    arrs = [a.astype(dtype).astype(np.float64) for a in st]
+    ref = oracle_mod.idd(*arrs, t["S"], t["sigma"], t["en"], c.Ng, c.Ny, c.Nz, c.nang, d.dg,
+                         d.du, d.dv, d.dy, d.dz, 1.3, trapezoidal=trap, catastrophic=cat)
+    dev = [_dev(gpu, a, dtype) for a in arrs]
+    v = gpu.ops.idd(*dev, *(_dev(gpu, t[k], np.float64) for k in ("S", "sigma", "en")), c.Ng, c.Ny,
+                    c.Nz, c.nang, d.dg, d.du, d.dv, d.dy, d.dz, 1.3, trapezoidal=trap,
+                    catastrophic=cat)
+    assert v == pytest.approx(ref, rel=1e-13 if dtype == np.float64 else 1e-6)

@@ -1,0 +1,161 @@
+"""Whole depth march on the GPU through the C ABI vs the CPU FP64 oracle,
+and against the reference's own golden IDD files.
+
+Tolerances:
+  FP64 state vs oracle (same steps):  states and IDD within 1e-10 relative to
+      the array maximum (the only differences are FMA contraction and
+      summation order, compounded over the steps);
+  FP64 full golden run (1000 steps):  IDD within 1e-8 relative pointwise where
+      IDD >= 1e-6 of its max, metrics within 1e-4 cm;
+  FP32 state (north_star gate):       IDD within 0.5 % pointwise where
+      IDD >= 10 % of max; BP, D90, D20 within 0.01 cm (0.1 mm).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from bfpn_gpu_amd import metrics
+from conftest import GOLDEN, ROOT
+from gpu_cases import golden_case, small_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def _run_both(gpu, oracle_mod, case, steps, precision):
+    o = oracle_mod.Solver(case)
+    o.advance(steps)
+    g = gpu.Solver(case, precision)
+    g.advance(steps)
+    return o, g
+
+
+@pytest.mark.parametrize("flags", [
+    {},
+    {"eq15_straggling": 1},
+    {"primary_only": 1},
+    {"energy_only": 1},
+    {"no_transport": 1, "no_angle": 1},
+    {"no_spatial_clipping": 1, "dose_catastrophic_loss": 0},
+    {"calibrate_cross_sections": 1, "trapezoidal_yz": 0, "discrete_angle_delta": 0,
+     "normalize_initial_mass": 0},
+], ids=lambda f: "-".join(sorted(f)) or "default")
+def test_fp64_solver_matches_oracle(gpu, oracle_mod, flags):
+    case = small_case(flags=flags)
+    o, g = _run_both(gpu, oracle_mod, case, 4, gpu.FP64)
+    for k in range(4):
+        assert _rel(g.state(k), o.state(k)) < 1e-10 or np.abs(o.state(k)).max() == 0, k
+    gi, gp = g.history()
+    oi, op = o.history()
+    assert len(gi) == 4
+    np.testing.assert_allclose(gi, oi, rtol=1e-11)
+    np.testing.assert_allclose(gp, op, rtol=1e-10, atol=1e-12 * np.abs(op).max())
+
+
+@pytest.mark.parametrize("material,rho", [("bone", 1.85), ("air", None)])
+def test_fp64_materials(gpu, oracle_mod, material, rho):
+    from conftest import TABLES
+
+    case = small_case(material=material, density=rho, data=os.path.join(TABLES, material),
+                      energy=150.0, Ny=3, Nz=3)
+    o, g = _run_both(gpu, oracle_mod, case, 3, gpu.FP64)
+    np.testing.assert_allclose(g.history()[0], o.history()[0], rtol=1e-11)
+    for k in range(4):
+        assert _rel(g.state(k), o.state(k)) < 1e-10 or np.abs(o.state(k)).max() == 0
+
+
+def test_tables_bit_identical_to_oracle(gpu, oracle_mod):
+    case = small_case()
+    o = oracle_mod.Solver(case)
+    g = gpu.Solver(case, gpu.FP32)
+    for name in ("S", "sigma", "T", "sig_tr", "en", "ker_e1", "ker_e2", "ker_v", "omend", "mu",
+                 "ker_beg", "ker_end"):
+        np.testing.assert_array_equal(g.table(name), o.table(name), err_msg=name)
+    F = g.state(0)
+    np.testing.assert_allclose(F, o.state(0).astype(np.float32), rtol=0, atol=0)
+
+
+def test_fp32_solver_tracks_oracle(gpu, oracle_mod):
+    case = small_case()
+    o, g = _run_both(gpu, oracle_mod, case, 6, gpu.FP32)
+    np.testing.assert_allclose(g.history()[0], o.history()[0], rtol=2e-5)
+    for k in (0, 2):
+        assert _rel(g.state(k), o.state(k)) < 1e-4
+
+
+def test_history_and_restart_are_consistent(gpu):
+    """advance(a)+advance(b) equals advance(a+b); write_state round-trips."""
+    case = small_case(Ny=3, Nz=3)
+    a = gpu.Solver(case, gpu.FP64)
+    a.advance(2)
+    a.advance(3)
+    b = gpu.Solver(case, gpu.FP64)
+    b.advance(5)
+    np.testing.assert_array_equal(a.history()[0], b.history()[0])
+    st = b.state(2)
+    b.set_state(2, st)
+    np.testing.assert_array_equal(b.state(2), st)
+
+
+def test_invalid_data_path_is_an_error(gpu):
+    case = small_case(data="/nonexistent")
+    with pytest.raises(gpu.BfpnError, match="Cannot open scattering data"):
+        gpu.Solver(case, gpu.FP32)
+
+
+# ---------------------------------------------------------------- golden ---
+@pytest.fixture(scope="module")
+def golden():
+    return np.loadtxt(os.path.join(GOLDEN, "stage6_water230", "8_7_nx1000", "idd_output.txt"))
+
+
+def test_fp64_full_run_reproduces_reference_golden(gpu, golden):
+    g = gpu.Solver(golden_case("8_7"), gpu.FP64)
+    idd, _ = g.run()
+    assert len(idd) == 1000
+    ref = golden[:, 1]
+    m = ref >= 1e-6 * ref.max()
+    rel = np.abs(idd[m] - ref[m]) / ref[m]
+    assert rel.max() < 1e-8, rel.max()
+    mg, mr = metrics.analyze(golden[:, 0], idd), metrics.analyze(golden[:, 0], ref)
+    for k in ("BP", "P90", "D90", "D20"):
+        assert abs(mg[k] - mr[k]) < 1e-4
+
+
+def test_fp32_full_run_meets_accuracy_gate(gpu, golden):
+    g = gpu.Solver(golden_case("8_7"), gpu.FP32)
+    idd, _ = g.run()
+    ref = golden[:, 1]
+    cmp = metrics.compare(idd, ref, band=0.10)
+    assert cmp["max_rel"] <= 5e-3, cmp
+    mg, mr = metrics.analyze(golden[:, 0], idd), metrics.analyze(golden[:, 0], ref)
+    for k in ("BP", "D90", "D20"):
+        assert abs(mg[k] - mr[k]) <= 0.01, (k, mg[k], mr[k])
+
+
+def test_cli_fp64_writes_the_reference_file(tmp_path, golden, gpu):
+    """bfp_solver with the fixture's flags writes idd_output.txt in the
+    reference's format (17 digits, x = (i+1) dt) with the golden values."""
+    cli = os.path.join(ROOT, "bfpn_gpu_amd", "bin", "bfp_solver")
+    data = os.path.join(ROOT, "bfpn_gpu_amd", "tables", "water")
+    cmd = [cli, "1000", "--data", data, "--material", "water", "--energy", "230", "--ny", "8",
+           "--nz", "8", "--nmu", "7", "--nom", "7", "--ly", "8", "--lz", "8", "--lu", "2", "--lv", "2",
+           "--sigma-yz", "0.3", "--discrete-angle-delta", "--normalize-initial-mass",
+           "--trapezoidal-yz", "--time", "40", "--precision", "fp64"]
+    r = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "Step 1000, t = 40" in r.stdout
+    lines = open(tmp_path / "idd_output.txt").read().splitlines()
+    assert lines[0] == "# x_cm value" and len(lines) == 1001
+    ref_lines = open(os.path.join(GOLDEN, "stage6_water230", "8_7_nx1000",
+                                  "idd_output.txt")).read().splitlines()
+    assert [ln.split()[0] for ln in lines] == [ln.split()[0] for ln in ref_lines]
+    out = np.loadtxt(tmp_path / "idd_output.txt")
+    m = golden[:, 1] >= 1e-6 * golden[:, 1].max()
+    assert np.max(np.abs(out[m, 1] - golden[m, 1]) / golden[m, 1]) < 1e-8
+    assert os.path.exists(tmp_path / "dose_output.txt")

@@ -1,0 +1,276 @@
+"""Known-answer checks of each oracle operator, independent of its code.
+
+Each check is chosen so that a plausible transcription mistake (dropped term,
+wrong sign, swapped index, transposed operand) fails it:
+
+energy (src/energy_solver.cu:108-160)
+  * S = sigma = T = 0 is the exact identity on (psi1, psi2);
+  * S = T = 0, sigma > 0 is the exact CN decay (1/h - s/2)/(1/h + s/2);
+  * S = sigma = 0, T > 0 is one explicit Euler step of energy diffusion
+    psi1 += h rho T/2 (psi1[g+1] - 2 psi1[g] + psi1[g-1]) / dg^2, psi2 fixed;
+  * the full operator (S > 0) is linear: E(a x + b y) = a E(x) + b E(y).
+transport (src/spatial_transport.cu:23-262)
+  * a constant field is unchanged; omega = 0 is the identity;
+  * away from the boundary the fluxes telescope: without clipping the sum of
+    psi over the plane is conserved exactly;
+  * a linear profile is advected exactly by the MUSCL predictor-corrector
+    (superbee limiter gives eta(1) = 1), psi(y) -> psi(y - w h).
+angle (src/angle_diffusion.cu:7-34, 254-483)
+  * sigma_tr = 0 (ratio 1): the DST-I sandwich is the identity (pins the
+    4/((Nom+1)(Nmu+1)) normalisation and the basis orientation);
+  * the result equals the dense Crank-Nicolson step
+    (I + hD L/2)^-1 (I - hD L/2) psi with the 5-point zero-Dirichlet
+    Laplacian on the (v, u) grid, solved with numpy.linalg (pins the
+    u <-> Nom, v <-> Nmu pairing of the eigenvalues).
+source (src/energy_solver.cu:287-333)
+  * equals a dense numpy einsum of its defining formula on a tiny grid;
+  * the factorized form agrees with the reference-order loop;
+  * kernel tables: K_omega entries in [0,1] with column sums <= 1, K_E
+    non-zero only for source groups above the output group.
+IDD (src/bfpm_solver.cu:229-309)
+  * one cell, one group: closed form with trapezoid weights.
+beam (src/bfpm_solver.cu:1815-2025)
+  * --normalize-initial-mass gives unit discrete mass.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from bfpn_gpu_amd.abi import Case
+from conftest import TABLES
+
+RNG = np.random.default_rng(20261202)
+
+
+def _tables(Ng, S=None, sigma=None, T=None):
+    z = np.zeros
+    return (z(Ng + 1) if S is None else S, z(Ng) if sigma is None else sigma,
+            z(Ng) if T is None else T)
+
+
+# ------------------------------------------------------------------ energy --
+def test_energy_identity_without_physics(oracle_mod):
+    Ng, nyz, nang = 7, 3, 4
+    F = RNG.random(Ng * nyz * nang)
+    f = RNG.standard_normal(Ng * nyz * nang)
+    F0, f0 = F.copy(), f.copy()
+    S, sig, T = _tables(Ng)
+    oracle_mod.energy_half(F, f, S, sig, T, 0.01, 0.5, 1.0, Ng, nyz, nang, straggling=True)
+    np.testing.assert_allclose(F, F0, rtol=1e-15, atol=0)
+    np.testing.assert_allclose(f, f0, rtol=1e-15, atol=0)
+
+
+def test_energy_cn_decay(oracle_mod):
+    Ng, nyz, nang = 6, 2, 3
+    h = 0.02
+    sig = RNG.random(Ng) * 5
+    F = RNG.random(Ng * nyz * nang)
+    f = RNG.standard_normal(Ng * nyz * nang)
+    F0, f0 = F.copy(), f.copy()
+    S, _, T = _tables(Ng)
+    oracle_mod.energy_half(F, f, S, sig, T, h, 0.5, 1.0, Ng, nyz, nang)
+    fac = np.repeat((1 / h - sig / 2) / (1 / h + sig / 2), nyz * nang)
+    np.testing.assert_allclose(F, F0 * fac, rtol=1e-13)
+    np.testing.assert_allclose(f, f0 * fac, rtol=1e-13)
+
+
+def test_energy_straggling_is_explicit_diffusion(oracle_mod):
+    Ng, lanes = 9, 5
+    h, dg, rho, Tv = 0.01, 0.7, 1.3, 0.4
+    F = RNG.random(Ng * lanes)
+    f = RNG.standard_normal(Ng * lanes)
+    F0, f0 = F.reshape(Ng, lanes).copy(), f.copy()
+    S, sig, _ = _tables(Ng)
+    oracle_mod.energy_half(F, f, S, sig, np.full(Ng, Tv), h, dg, rho, Ng, 1, lanes, straggling=True)
+    pad = np.vstack([np.zeros(lanes), F0, np.zeros(lanes)])
+    lap = pad[2:] - 2 * pad[1:-1] + pad[:-2]
+    np.testing.assert_allclose(F.reshape(Ng, lanes), F0 + h * rho * Tv / 2 * lap / dg**2, rtol=1e-12)
+    np.testing.assert_allclose(f, f0, rtol=1e-15)
+
+
+def test_energy_is_linear(oracle_mod):
+    Ng, nyz, nang = 12, 2, 3
+    S = 1 + RNG.random(Ng + 1)
+    sig = RNG.random(Ng) * 0.1
+    T = np.full(Ng, 0.01)
+    n = Ng * nyz * nang
+    x1, x2, y1, y2 = (RNG.standard_normal(n) for _ in range(4))
+    a, b = 0.7, -1.9
+
+    def run(F, f):
+        F, f = F.copy(), f.copy()
+        oracle_mod.energy_half(F, f, S, sig, T, 0.01, 0.5, 1.1, Ng, nyz, nang, straggling=True)
+        return F, f
+
+    Fx, fx = run(x1, x2)
+    Fy, fy = run(y1, y2)
+    Fz, fz = run(a * x1 + b * y1, a * x2 + b * y2)
+    np.testing.assert_allclose(Fz, a * Fx + b * Fy, atol=1e-10 * np.abs(Fz).max())
+    np.testing.assert_allclose(fz, a * fx + b * fy, atol=1e-10 * np.abs(fz).max())
+
+
+# --------------------------------------------------------------- transport --
+def _om(nang, scale=0.3):
+    return RNG.uniform(-scale, scale, 2 * nang)
+
+
+def test_transport_constant_and_zero_velocity(oracle_mod):
+    Ny, Nz, Ng, nang = 5, 4, 2, 3
+    n = Ng * (Ny + 1) * (Nz + 1) * nang
+    F = np.full(n, 2.5)
+    oracle_mod.transport_half(F, _om(nang), Ny, Nz, Ng, nang, 0.1, 0.2, 0.01, False)
+    np.testing.assert_allclose(F, 2.5, rtol=1e-15)
+    G = RNG.standard_normal(n)
+    G0 = G.copy()
+    oracle_mod.transport_half(G, np.zeros(2 * nang), Ny, Nz, Ng, nang, 0.1, 0.2, 0.01, True)
+    np.testing.assert_array_equal(G, G0)
+
+
+def test_transport_conserves_interior_mass(oracle_mod):
+    Ny = Nz = 12
+    Ng, nang = 2, 4
+    F = np.zeros((Ng, Nz + 1, Ny + 1, nang))
+    F[:, 4:9, 4:9, :] = RNG.random((Ng, 5, 5, nang))
+    flat = F.ravel().copy()
+    oracle_mod.transport_half(flat, _om(nang), Ny, Nz, Ng, nang, 0.1, 0.1, 0.02, True)
+    out = flat.reshape(F.shape)
+    np.testing.assert_allclose(out.sum(axis=(1, 2)), F.sum(axis=(1, 2)), rtol=1e-13)
+    assert np.abs(out - F).max() > 1e-3  # something moved
+
+
+@pytest.mark.parametrize("wy,wz", [(0.4, 0.0), (-0.3, 0.0), (0.0, 0.25), (0.2, -0.35)])
+def test_transport_advects_linear_profile_exactly(oracle_mod, wy, wz):
+    Ny = Nz = 10
+    dy, dz, h = 0.1, 0.1, 0.05
+    y = np.arange(Ny + 1) * dy
+    z = np.arange(Nz + 1) * dz
+    a, by, bz = 3.0, 1.5, -0.8
+    F = (a + by * y[None, :] + bz * z[:, None])[None, :, :, None].copy()
+    flat = F.ravel().copy()
+    oracle_mod.transport_half(flat, np.array([wy, wz]), Ny, Nz, 1, 1, dy, dz, h, True)
+    out = flat.reshape(F.shape)[0, :, :, 0]
+    exact = a + by * (y[None, :] - wy * h) + bz * (z[:, None] - wz * h)
+    inner = (slice(4, Nz - 3), slice(4, Ny - 3))  # both PC stages exact here
+    np.testing.assert_allclose(out[inner], exact[inner], rtol=1e-13)
+
+
+# ------------------------------------------------------------------- angle --
+def test_angle_identity_without_scattering(oracle_mod):
+    Nmu, Nom, Ng, nyz = 5, 7, 3, 2
+    F = RNG.standard_normal(Ng * nyz * Nmu * Nom)
+    F0 = F.copy()
+    oracle_mod.angle_full(F, np.zeros(Ng), nyz, Ng, Nmu, Nom, 0.1, 0.2, 0.02, 1.0)
+    np.testing.assert_allclose(F, F0, atol=1e-13)
+
+
+def _laplacian_1d(n, h):
+    return (np.diag(np.full(n, -2.0)) + np.diag(np.ones(n - 1), 1) + np.diag(np.ones(n - 1), -1)) / h**2
+
+
+def test_angle_matches_dense_crank_nicolson(oracle_mod):
+    Nmu, Nom, Ng, nyz = 6, 4, 2, 3
+    du, dv, dt, rho = 0.21, 0.13, 0.05, 1.2
+    sig_tr = np.array([3.0, 7.5])
+    F = RNG.standard_normal((Ng, nyz, Nmu, Nom))
+    flat = F.ravel().copy()
+    oracle_mod.angle_full(flat, sig_tr, nyz, Ng, Nmu, Nom, du, dv, dt, rho)
+    out = flat.reshape(F.shape)
+    # plane index (v row, u column), row-major over (v, u)
+    L = np.kron(_laplacian_1d(Nmu, dv), np.eye(Nom)) + np.kron(np.eye(Nmu), _laplacian_1d(Nom, du))
+    eye = np.eye(Nmu * Nom)
+    for g in range(Ng):
+        D = 0.5 * rho * sig_tr[g]
+        step = np.linalg.solve(eye - 0.5 * dt * D * L, eye + 0.5 * dt * D * L)
+        for s in range(nyz):
+            np.testing.assert_allclose(out[g, s].ravel(), step @ F[g, s].ravel(), atol=1e-12)
+
+
+# ------------------------------------------------------------------ source --
+def _source_case():
+    c = Case(Nx=100, Ny=2, Nz=1, Ng=40, Nmu=3, Nom=4, Ly=1, Lz=1, Lu=2, Lv=2,
+             material="water", data=os.path.join(TABLES, "synthetic"), Lg=259.0 * 40 / 500)
+    return c
+
+
+def test_source_matches_dense_definition(oracle_mod):
+    c = _source_case()
+    mu, sig, _ = oracle_mod.load_tables(c.data, c.Ng)
+    d = oracle_mod.deltas(c)
+    om = oracle_mod.omend(c)
+    ke1, ke2, beg, end, kv = oracle_mod.build_kernels(c.Ng, d.dg, c.nang, mu, sig, om)
+    assert np.count_nonzero(ke1) > 0 and np.count_nonzero(kv) > 0
+    Fp = RNG.standard_normal(c.cells)
+    fp = RNG.standard_normal(c.cells)
+    Q = oracle_mod.secondary_source(Fp, fp, ke1, ke2, kv, beg, end, c.Ng, c.nyz, c.nang, d.dg)
+    P1 = np.maximum(Fp, 0).reshape(c.Ng, c.nyz, c.nang)
+    P2 = fp.reshape(c.Ng, c.nyz, c.nang)
+    dense = d.dg * (np.einsum("gh,hsi,hio->gso", ke1, P1, kv)
+                    + np.einsum("gh,hsi,hio->gso", ke2, P2, kv) / 3.0)
+    np.testing.assert_allclose(Q.reshape(dense.shape), dense, atol=1e-12 * np.abs(dense).max())
+    Qf = oracle_mod.secondary_source_factorized(Fp, fp, ke1, ke2, kv, beg, end, c.Ng, c.nyz, c.nang,
+                                                d.dg)
+    np.testing.assert_allclose(Qf, Q, atol=1e-13 * np.abs(Q).max())
+
+
+def test_kernel_table_invariants(oracle_mod):
+    c = Case(Nx=100, Ny=2, Nz=2, Ng=500, Nmu=5, Nom=5, Lu=2, Lv=2, material="water",
+             data=os.path.join(TABLES, "water"))
+    mu, sig, _ = oracle_mod.load_tables(c.data, c.Ng)
+    d = oracle_mod.deltas(c)
+    ke1, ke2, beg, end, kv = oracle_mod.build_kernels(c.Ng, d.dg, c.nang, mu, sig,
+                                                      oracle_mod.omend(c))
+    assert kv.min() >= 0 and kv.max() <= 1
+    assert np.all(kv.sum(axis=1) <= 1 + 1e-12)
+    assert np.all(kv[-1] == 0)  # last group has no angular kernel
+    rows, cols = np.nonzero((ke1 != 0) | (ke2 != 0))
+    assert np.all(cols > rows)  # energy never increases in a collision
+    for r in range(c.Ng):
+        nz = np.nonzero((ke1[r] != 0) | (ke2[r] != 0))[0]
+        if nz.size:
+            assert beg[r] == nz.min() and end[r] == nz.max() + 1
+        else:
+            assert beg[r] == 0 and end[r] == 0
+
+
+# --------------------------------------------------------------------- IDD --
+def test_idd_single_cell_closed_form(oracle_mod):
+    Ng, Ny, Nz, nang = 3, 1, 1, 1
+    S = np.array([4.0, 3.0, 2.5, 2.0])
+    sig = np.array([0.1, -0.2, 0.3])
+    en = np.array([1.0, 2.0, 3.0, 4.0])
+    F = np.zeros(Ng * 4)
+    f = np.zeros(Ng * 4)
+    g, s = 1, 3  # group 1, corner node (iy=1, iz=1)
+    F[g * 4 + s] = 2.0
+    f[g * 4 + s] = 0.5
+    dg, du, dv, dy, dz, rho = 1.0, 0.1, 0.2, 0.3, 0.4, 1.7
+    v = oracle_mod.idd(F, f, None, None, S, sig, en, Ng, Ny, Nz, nang, dg, du, dv, dy, dz, rho,
+                       trapezoidal=True, catastrophic=True)
+    dose = 2.0 * 0.5 * (S[1] + S[2]) * dg + 0.5 * (S[2] - S[1]) / 6 * dg  # sigma clamps to 0
+    assert v == pytest.approx(0.25 * dose / rho * du * dv * dy * dz, rel=1e-14)
+    f2 = f.copy()
+    F[:] = 0
+    F[2 * 4 + 0] = 1.0
+    v2 = oracle_mod.idd(F, f2 * 0, None, None, S, sig, en, Ng, Ny, Nz, nang, dg, du, dv, dy, dz, rho,
+                        trapezoidal=False, catastrophic=True)
+    dose2 = 0.5 * (S[2] + S[3]) * dg + 0.5 * (en[2] + en[3]) * 0.3 * dg
+    assert v2 == pytest.approx(dose2 / rho * du * dv * dy * dz, rel=1e-14)
+
+
+def test_beam_mass_normalisation(oracle_mod):
+    c = Case(Nx=100, Ny=6, Nz=4, Ng=60, Nmu=4, Nom=6, Ly=2, Lz=2, Lu=2, Lv=2, energy=50,
+             sigma_yz=0.3, Lg=100, flags=dict(normalize_initial_mass=1, trapezoidal_yz=1,
+                                                discrete_angle_delta=1))
+    F, m = oracle_mod.init_beam(c)
+    d = oracle_mod.deltas(c)
+    wy = np.ones(c.Ny + 1)
+    wy[[0, -1]] = 0.5
+    wz = np.ones(c.Nz + 1)
+    wz[[0, -1]] = 0.5
+    w = np.outer(wz, wy).ravel()
+    mass = (F.reshape(c.Ng, c.nyz, c.nang) * w[None, :, None]).sum() * d.du * d.dv * d.dy * d.dz * d.dg
+    assert mass == pytest.approx(1.0, rel=1e-12)
+    # the discrete delta puts all angular mass on the 4 central directions
+    nz = np.nonzero(F.reshape(c.Ng, c.nyz, c.nang).sum(axis=(0, 1)))[0]
+    assert len(nz) == 4
