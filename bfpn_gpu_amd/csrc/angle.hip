@@ -14,9 +14,105 @@
 // are built once on the host per (g, direction).
 
 #include "kernels.hpp"
+#include "mfma.hpp"
 
 namespace bfpn {
 namespace {
+
+// ---------------------------------------------------------- FP32 / MFMA ----
+// One wave per (g, s) plane (grid-stride over planes).  The plane, padded to
+// 32 x 32 with zeros, lives in a per-wave LDS tile; each stage is a 32x32x32
+// product on the matrix cores (split-BF16, FP32 accumulation), and the LDS
+// tile converts the D layout of one stage into the A/B layout of the next.
+// The DST-I bases are symmetric (S[k][j] = S[j][k]), so Y = X Su^T uses Su
+// as the B operand and W = Sv^T Z uses Sv as the A operand, like Sv Y.
+__global__ __launch_bounds__(256) void angle_mfma_kernel(
+    float* __restrict__ F, const float* __restrict__ factors, const __bf16* __restrict__ su_h,
+    const __bf16* __restrict__ su_l, const __bf16* __restrict__ sv_h,
+    const __bf16* __restrict__ sv_l, long long planes, int nyz, int Nmu, int Nom) {
+    __shared__ float tile[4][32][33];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = lane & 15, q = lane >> 4;
+    float(*B)[33] = tile[wave];
+    const int na = Nmu * Nom;
+    const int mt_n = Nmu > 16 ? 2 : 1;  // tiles along v / nu
+    const int nt_n = Nom > 16 ? 2 : 1;  // tiles along a / mu
+    SplitFrag su[2], sv[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int off = (t * 16 + i) * 32 + 8 * q;
+        su[t].hi = *reinterpret_cast<const bf16x8*>(su_h + off);
+        su[t].lo = *reinterpret_cast<const bf16x8*>(su_l + off);
+        sv[t].hi = *reinterpret_cast<const bf16x8*>(sv_h + off);
+        sv[t].lo = *reinterpret_cast<const bf16x8*>(sv_l + off);
+    }
+    for (int e = lane; e < 32 * 32; e += 64) B[e >> 5][e & 31] = 0.f;
+
+    auto row_frag = [&](int row) {  // A operand: B[row][8q .. 8q+7]
+        float v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = B[row][8 * q + e];
+        return split8(v);
+    };
+    auto col_frag = [&](int col) {  // B operand: B[8q .. 8q+7][col]
+        float v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = B[8 * q + e][col];
+        return split8(v);
+    };
+    auto store = [&](const f32x4 (&d)[2][2]) {
+        for (int mt = 0; mt < mt_n; ++mt)
+            for (int nt = 0; nt < nt_n; ++nt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) B[mt * 16 + 4 * q + r][nt * 16 + i] = d[mt][nt][r];
+    };
+
+    for (long long p = static_cast<long long>(blockIdx.x) * 4 + wave; p < planes;
+         p += static_cast<long long>(gridDim.x) * 4) {
+        float* X = F + p * na;
+        const float* C = factors + (p / nyz) * na;
+        for (int e = lane; e < na; e += 64) {
+            const int v = e / Nom;
+            B[v][e - v * Nom] = X[e];
+        }
+        f32x4 d[2][2];
+        // stage 1: Y[v][mu] = sum_a X[v][a] Su[mu][a]
+        SplitFrag a[2], b[2];
+        for (int mt = 0; mt < mt_n; ++mt) a[mt] = row_frag(mt * 16 + i);
+        for (int mt = 0; mt < mt_n; ++mt)
+            for (int nt = 0; nt < nt_n; ++nt) d[mt][nt] = mma_split(a[mt], su[nt], zero4());
+        store(d);
+        // stage 2: Z[nu][mu] = C[nu][mu] sum_v Sv[nu][v] Y[v][mu]
+        for (int nt = 0; nt < nt_n; ++nt) b[nt] = col_frag(nt * 16 + i);
+        for (int mt = 0; mt < mt_n; ++mt)
+            for (int nt = 0; nt < nt_n; ++nt) {
+                d[mt][nt] = mma_split(sv[mt], b[nt], zero4());
+                const int mu = nt * 16 + i;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int nu = mt * 16 + 4 * q + r;
+                    d[mt][nt][r] *= (nu < Nmu && mu < Nom) ? C[nu * Nom + mu] : 0.f;
+                }
+            }
+        store(d);
+        // stage 3: W[v][mu] = sum_nu Sv[nu][v] Z[nu][mu]
+        for (int nt = 0; nt < nt_n; ++nt) b[nt] = col_frag(nt * 16 + i);
+        for (int mt = 0; mt < mt_n; ++mt)
+            for (int nt = 0; nt < nt_n; ++nt) d[mt][nt] = mma_split(sv[mt], b[nt], zero4());
+        store(d);
+        // stage 4: X'[v][a] = sum_mu W[v][mu] Su[mu][a]
+        for (int mt = 0; mt < mt_n; ++mt) a[mt] = row_frag(mt * 16 + i);
+        for (int mt = 0; mt < mt_n; ++mt)
+            for (int nt = 0; nt < nt_n; ++nt) d[mt][nt] = mma_split(a[mt], su[nt], zero4());
+        store(d);
+        for (int e = lane; e < na; e += 64) {
+            const int v = e / Nom;
+            X[e] = B[v][e - v * Nom];
+        }
+    }
+}
+
+// --------------------------------------------------- vector (FP64, wide) ---
 
 constexpr int kThreads = 256;
 constexpr int kMaxAngles = 1024;  // Nmu*Nom per plane held in LDS
@@ -83,6 +179,17 @@ __global__ __launch_bounds__(kThreads) void angle_kernel(T* __restrict__ F,
 }
 
 }  // namespace
+
+void launch_angle_mfma(float* F, const float* factors, const AnglePlan& plan, int Ng, int nyz,
+                       int Nmu, int Nom, hipStream_t s) {
+    BFPN_REQUIRE(Nmu >= 1 && Nom >= 1 && Nmu <= 32 && Nom <= 32, "MFMA DST needs Nmu, Nom <= 32");
+    const long long planes = static_cast<long long>(Ng) * nyz;
+    if (planes == 0) return;
+    const long long blocks = std::min<long long>((planes + 3) / 4, 4096);
+    hipLaunchKernelGGL(angle_mfma_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, F,
+                       factors, plan.su_h, plan.su_l, plan.sv_h, plan.sv_l, planes, nyz, Nmu, Nom);
+    BFPN_HIP(hipGetLastError());
+}
 
 template <typename T>
 void launch_angle_full(T* F, const T* factors, const T* su, const T* sv, int Ng, int nyz, int Nmu,

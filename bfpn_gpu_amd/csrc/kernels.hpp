@@ -16,7 +16,27 @@ void launch_energy_half(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng,
                         int nang, double* dose_partials, bool proxy, hipStream_t s);
 int energy_blocks(long long lanes);
 
-// Secondary source: A1/A2 angular contraction then the banded energy sum.
+// Split-BF16 operands of the FP32 source (matrix cores), built once per beam.
+struct SourcePlan {
+    const __bf16* kth = nullptr;  // Kv transposed [g][out][in], nap x nap, hi part
+    const __bf16* ktl = nullptr;  //                                      lo part
+    const __bf16* e1h = nullptr;  // dg*K1  [erows][ngp], hi / lo
+    const __bf16* e1l = nullptr;
+    const __bf16* e2h = nullptr;  // dg*K2/3
+    const __bf16* e2l = nullptr;
+    const int2* tile_band = nullptr;  // [min beg, max end) per 16-group tile
+};
+int source_nap(int nang);    // nang padded to 32
+int source_ngp(int Ng);      // Ng padded to 32 (E columns)
+int source_erows(int Ng);    // Ng padded to 64 (E rows); tile_band has erows/16 entries
+void launch_split_transpose(const float* kv, __bf16* hi, __bf16* lo, int Ng, int nang,
+                            hipStream_t s);
+void launch_split(const float* src, __bf16* hi, __bf16* lo, long long n, hipStream_t s);
+// FP32 source on the matrix cores: A1/A2 scratch, Q output.
+void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, float* Q,
+                        const SourcePlan& plan, int Ng, int nyz, int nang, hipStream_t s);
+
+// Secondary source (FP64 vector path): A1/A2 angular contraction, banded sum.
 template <typename T>
 void launch_source_angular(const T* Fp, const T* fp, const T* kv, T* A1, T* A2, int Ng, int nyz,
                            int nang, hipStream_t s);
@@ -25,17 +45,27 @@ void launch_source_energy(const T* A1, const T* A2, const double* e1, const doub
                           const int* beg, const int* end, T* Q, int Ng, long long lanes, double dg,
                           hipStream_t s);
 
-// MUSCL transport half step: out = PC(in).  pred is scratch; out != in.
+// MUSCL predictor-corrector transport half step, in place (no scratch).
 template <typename T>
-void launch_transport_half(const T* in, T* pred, T* out, const T* omend, int Ny, int Nz, int Ng,
-                           int nang, double dy, double dz, double h, bool preserve_sign,
-                           hipStream_t s);
+void launch_transport_half(T* F, const T* omend, int Ny, int Nz, int Ng, int nang, double dy,
+                           double dz, double h, bool preserve_sign, hipStream_t s);
 
 // CN angular diffusion on the DST-I basis, in place.  factors: Ng*nang (T),
 // su: Nom*Nom, sv: Nmu*Nmu (T).
 template <typename T>
 void launch_angle_full(T* F, const T* factors, const T* su, const T* sv, int Ng, int nyz, int Nmu,
                        int Nom, hipStream_t s);
+
+// Split-BF16 sine bases padded to 32x32 (row-major), for the MFMA DST.
+struct AnglePlan {
+    const __bf16* su_h = nullptr;
+    const __bf16* su_l = nullptr;
+    const __bf16* sv_h = nullptr;
+    const __bf16* sv_l = nullptr;
+};
+// FP32 DST on the matrix cores (Nmu, Nom <= 32), in place.
+void launch_angle_mfma(float* F, const float* factors, const AnglePlan& plan, int Ng, int nyz,
+                       int Nmu, int Nom, hipStream_t s);
 
 // Eq.24 dose of (F+F1, f+f1) into per-block partials (count = dose_blocks()).
 template <typename T>

@@ -1,0 +1,55 @@
+// mfma.hpp — split-BF16 matrix-core helpers for CDNA4 (gfx950).
+//
+// A product x*y of FP32 values is evaluated on the BF16 MFMA pipe as
+//     x*y ~= xh*yh + xh*yl + xl*yh,   xh = bf16(x), xl = bf16(x - xh)
+// with FP32 accumulation (v_mfma_f32_16x16x32_bf16, three issues per tile).
+// xh + xl carries 16 significand bits, so each product is accurate to about
+// 2^-16 relative; the dropped xl*yl term is ~2^-18.
+//
+// Operand layouts of v_mfma_f32_16x16x32_bf16 (one wave64, q = lane/16):
+//   A (16 x 32, row-major):  lane holds A[lane%16][8q .. 8q+7]
+//   B (32 x 16):             lane holds B[8q .. 8q+7][lane%16]
+//   D (16 x 16):             lane holds D[4q + r][lane%16], r = 0..3
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace bfpn {
+
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8;
+typedef __attribute__((ext_vector_type(4))) float f32x4;
+
+struct SplitFrag {
+    bf16x8 hi;
+    bf16x8 lo;
+};
+
+__device__ inline void split_bf16(float x, __bf16& hi, __bf16& lo) {
+    hi = static_cast<__bf16>(x);
+    lo = static_cast<__bf16>(x - static_cast<float>(hi));
+}
+
+// Splits 8 FP32 values into a hi/lo BF16 fragment.
+__device__ inline SplitFrag split8(const float (&v)[8]) {
+    SplitFrag f;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        __bf16 h, l;
+        split_bf16(v[e], h, l);
+        f.hi[e] = h;
+        f.lo[e] = l;
+    }
+    return f;
+}
+
+// acc += A*B with both operands split (three MFMA issues).
+__device__ inline f32x4 mma_split(const SplitFrag& a, const SplitFrag& b, f32x4 acc) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.lo, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.lo, b.hi, acc, 0, 0, 0);
+    return acc;
+}
+
+__device__ inline f32x4 zero4() { return f32x4{0.f, 0.f, 0.f, 0.f}; }
+
+}  // namespace bfpn

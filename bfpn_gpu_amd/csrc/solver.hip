@@ -9,7 +9,7 @@
 // with a host sync after every operator and two scalar D2H copies per step.
 //
 // Here the whole step is stream-ordered on one HIP stream with no host sync:
-// transport ping-pongs between the state buffer and a spare, the IDD and
+// transport updates each state array in place, the IDD and
 // proxy are accumulated inside the final energy half steps and reduced into
 // a device history array, and the single source evaluation per step feeds
 // both secondary energy half steps (the primary is unchanged in between, so
@@ -22,6 +22,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "kernels.hpp"
@@ -103,6 +104,86 @@ std::vector<double> yz_weights(int Ny, int Nz, bool trapezoidal) {
     return w;
 }
 
+// Device storage of a SourcePlan (FP32 source on the matrix cores).
+struct SourcePlanBuffers {
+    DeviceBuffer kth, ktl, e1h, e1l, e2h, e2l, band, scratch;
+    SourcePlan plan;
+
+    // kv_f32: device FP32 Kv [Ng][nang][nang]; e1/e2/beg/end: host tables.
+    void build(const float* kv_f32, const std::vector<double>& e1, const std::vector<double>& e2,
+               const std::vector<int>& beg, const std::vector<int>& end, int Ng, int nang,
+               double dg, hipStream_t s) {
+        const int nap = source_nap(nang), ngp = source_ngp(Ng), erows = source_erows(Ng);
+        const size_t kt = static_cast<size_t>(Ng) * nap * nap;
+        kth.ensure(kt * sizeof(__bf16));
+        ktl.ensure(kt * sizeof(__bf16));
+        launch_split_transpose(kv_f32, kth.as<__bf16>(), ktl.as<__bf16>(), Ng, nang, s);
+        const size_t ne = static_cast<size_t>(erows) * ngp;
+        std::vector<float> E1(ne, 0.f), E2(ne, 0.f);
+        for (int r = 0; r < Ng; ++r)
+            for (int c = 0; c < Ng; ++c) {
+                E1[static_cast<size_t>(r) * ngp + c] = static_cast<float>(dg * e1[static_cast<size_t>(r) * Ng + c]);
+                E2[static_cast<size_t>(r) * ngp + c] =
+                    static_cast<float>(dg * e2[static_cast<size_t>(r) * Ng + c] / 3.0);
+            }
+        for (DeviceBuffer* b : {&e1h, &e1l, &e2h, &e2l}) b->ensure(ne * sizeof(__bf16));
+        upload(scratch, E1, s);
+        launch_split(scratch.as<float>(), e1h.as<__bf16>(), e1l.as<__bf16>(), static_cast<long long>(ne), s);
+        BFPN_HIP(hipStreamSynchronize(s));
+        upload(scratch, E2, s);
+        launch_split(scratch.as<float>(), e2h.as<__bf16>(), e2l.as<__bf16>(), static_cast<long long>(ne), s);
+        std::vector<int2> tiles(erows / 16);
+        for (size_t t = 0; t < tiles.size(); ++t) {
+            int lo = 1 << 30, hi = 0;
+            for (int r = static_cast<int>(t) * 16; r < static_cast<int>(t) * 16 + 16 && r < Ng; ++r)
+                if (end[r] > beg[r]) {
+                    lo = std::min(lo, beg[r]);
+                    hi = std::max(hi, end[r]);
+                }
+            tiles[t] = hi > 0 ? make_int2(lo, hi) : make_int2(0, 0);
+        }
+        upload(band, tiles, s);
+        BFPN_HIP(hipStreamSynchronize(s));
+        scratch.release();
+        plan.kth = kth.as<__bf16>();
+        plan.ktl = ktl.as<__bf16>();
+        plan.e1h = e1h.as<__bf16>();
+        plan.e1l = e1l.as<__bf16>();
+        plan.e2h = e2h.as<__bf16>();
+        plan.e2l = e2l.as<__bf16>();
+        plan.tile_band = band.as<int2>();
+    }
+    size_t bytes() const {
+        return kth.bytes() + ktl.bytes() + e1h.bytes() + e1l.bytes() + e2h.bytes() + e2l.bytes() +
+               band.bytes();
+    }
+};
+
+// Split-BF16 sine bases for the MFMA DST (FP32 state, Nmu, Nom <= 32).
+struct AnglePlanBuffers {
+    DeviceBuffer h, l, scratch;
+    AnglePlan plan;
+    static bool usable(int Nmu, int Nom) { return Nmu <= 32 && Nom <= 32; }
+    void build(int Nmu, int Nom, hipStream_t s) {
+        std::vector<float> pad(2 * 32 * 32, 0.f);  // Su then Sv, zero padded
+        const auto su = sine_basis(Nom), sv = sine_basis(Nmu);
+        for (int r = 0; r < Nom; ++r)
+            for (int c = 0; c < Nom; ++c) pad[r * 32 + c] = static_cast<float>(su[r * Nom + c]);
+        for (int r = 0; r < Nmu; ++r)
+            for (int c = 0; c < Nmu; ++c) pad[1024 + r * 32 + c] = static_cast<float>(sv[r * Nmu + c]);
+        h.ensure(pad.size() * sizeof(__bf16));
+        l.ensure(pad.size() * sizeof(__bf16));
+        upload(scratch, pad, s);
+        launch_split(scratch.as<float>(), h.as<__bf16>(), l.as<__bf16>(), static_cast<long long>(pad.size()), s);
+        BFPN_HIP(hipStreamSynchronize(s));
+        scratch.release();
+        plan.su_h = h.as<__bf16>();
+        plan.su_l = l.as<__bf16>();
+        plan.sv_h = h.as<__bf16>() + 1024;
+        plan.sv_l = l.as<__bf16>() + 1024;
+    }
+};
+
 }  // namespace bfpn
 
 using namespace bfpn;
@@ -167,15 +248,12 @@ public:
 
         const size_t state_bytes = cells * sizeof(T);
         for (auto& b : state_) b.ensure(state_bytes);
-        spare_.ensure(state_bytes);
-        pred_.ensure(state_bytes);
         if (secondary_) {
             a1_.ensure(state_bytes);
             a2_.ensure(state_bytes);
             q_.ensure(state_bytes);
         }
         for (int k = 0; k < 4; ++k) cur_[k] = state_[k].as<T>();
-        spare_ptr_ = spare_.as<T>();
 
         upload(coef_, energy_coefs(S, mat.sigma, Tc.data(), p.eq15_straggling != 0, 0.5 * d.dt, d.dg,
                                    p.energy_density, g.Ng),
@@ -189,12 +267,20 @@ public:
                stream);
         upload(su_, cast_vector<T>(sine_basis(g.Nom)), stream);
         upload(sv_, cast_vector<T>(sine_basis(g.Nmu)), stream);
+        if (std::is_same_v<T, float> && AnglePlanBuffers::usable(g.Nmu, g.Nom))
+            angle_plan_.build(g.Nmu, g.Nom, stream);
         if (secondary_) {
-            upload(e1_, ker.e1, stream);
-            upload(e2_, ker.e2, stream);
-            upload(beg_, ker.beg, stream);
-            upload(end_, ker.end, stream);
             upload(kv_, cast_vector<T>(ker.v), stream);
+            if constexpr (std::is_same_v<T, float>) {
+                plan_.build(kv_.as<float>(), ker.e1, ker.e2, ker.beg, ker.end, g.Ng, nang, d.dg,
+                            stream);
+                kv_.release();
+            } else {
+                upload(e1_, ker.e1, stream);
+                upload(e2_, ker.e2, stream);
+                upload(beg_, ker.beg, stream);
+                upload(end_, ker.end, stream);
+            }
         }
         const int eb = energy_blocks(lanes);
         partial_p_.ensure(2 * eb * sizeof(double));
@@ -218,8 +304,8 @@ public:
     }
 
     size_t device_bytes() const override {
-        size_t b = spare_.bytes() + pred_.bytes() + a1_.bytes() + a2_.bytes() + q_.bytes() +
-                   kv_.bytes() + e1_.bytes() + e2_.bytes() + hist_.bytes();
+        size_t b = a1_.bytes() + a2_.bytes() + q_.bytes() + kv_.bytes() + e1_.bytes() +
+                   e2_.bytes() + hist_.bytes() + plan_.bytes();
         for (const auto& s : state_) b += s.bytes();
         return b;
     }
@@ -278,11 +364,11 @@ private:
     }
 
     // Memory preflight (preflightMemory, src/bfpm_solver.cu:1321-1433), for
-    // this layout: 6 state-sized arrays (+3 with the secondary source) and Kv.
+    // this layout: 4 state arrays (+3 source arrays with the secondary) and Kv.
     void preflight() const {
         size_t free_b = 0, total_b = 0;
         BFPN_HIP(hipMemGetInfo(&free_b, &total_b));
-        const size_t arrays = secondary_ ? 9 : 6;
+        const size_t arrays = secondary_ ? 7 : 4;
         const size_t need = arrays * cells * sizeof(T) +
                             (secondary_ ? static_cast<size_t>(grid.Ng) * nang * nang * sizeof(T) : 0) +
                             (64ull << 20);
@@ -345,14 +431,20 @@ private:
     }
 
     void transport(int which, bool preserve_sign) {
-        launch_transport_half<T>(cur_[which], pred_.as<T>(), spare_ptr_, om_.as<T>(), grid.Ny,
-                                 grid.Nz, grid.Ng, nang, d.dy, d.dz, 0.5 * d.dt, preserve_sign,
-                                 stream);
-        std::swap(cur_[which], spare_ptr_);
+        launch_transport_half<T>(cur_[which], om_.as<T>(), grid.Ny, grid.Nz, grid.Ng, nang, d.dy,
+                                 d.dz, 0.5 * d.dt, preserve_sign, stream);
         mark(kTransport, array_bytes() * 2);
     }
 
     void angle(int which) {
+        if constexpr (std::is_same_v<T, float>) {
+            if (angle_plan_.plan.su_h) {
+                launch_angle_mfma(cur_[which], factors_.as<float>(), angle_plan_.plan, grid.Ng, nyz,
+                                  grid.Nmu, grid.Nom, stream);
+                mark(kAngle, array_bytes() * 2);
+                return;
+            }
+        }
         launch_angle_full<T>(cur_[which], factors_.as<T>(), su_.as<T>(), sv_.as<T>(), grid.Ng, nyz,
                              grid.Nmu, grid.Nom, stream);
         mark(kAngle, array_bytes() * 2);
@@ -386,11 +478,16 @@ private:
         }
         const bool sec = secondary_ && step_ > 0;
         if (sec) {  // stepSecondary, src/bfpm_solver.cu:3335-3394
-            launch_source_angular<T>(cur_[0], cur_[1], kv_.as<T>(), a1_.as<T>(), a2_.as<T>(),
-                                     grid.Ng, nyz, nang, stream);
-            launch_source_energy<T>(a1_.as<T>(), a2_.as<T>(), e1_.as<double>(), e2_.as<double>(),
-                                    beg_.as<int>(), end_.as<int>(), q_.as<T>(), grid.Ng, lanes,
-                                    d.dg, stream);
+            if constexpr (std::is_same_v<T, float>) {
+                launch_source_mfma(cur_[0], cur_[1], a1_.as<float>(), a2_.as<float>(),
+                                   q_.as<float>(), plan_.plan, grid.Ng, nyz, nang, stream);
+            } else {
+                launch_source_angular<T>(cur_[0], cur_[1], kv_.as<T>(), a1_.as<T>(), a2_.as<T>(),
+                                         grid.Ng, nyz, nang, stream);
+                launch_source_energy<T>(a1_.as<T>(), a2_.as<T>(), e1_.as<double>(),
+                                        e2_.as<double>(), beg_.as<int>(), end_.as<int>(),
+                                        q_.as<T>(), grid.Ng, lanes, d.dg, stream);
+            }
             mark(kSource, array_bytes() * 7);  // 2 in, 2 out; 2 in, 1 out
             energy(1, true, nullptr, false);
             transport(2, false);  // secondary psi^1 always clipped
@@ -410,11 +507,12 @@ private:
     }
 
     bool secondary_ = true;
-    DeviceBuffer state_[4], spare_, pred_, a1_, a2_, q_;
+    DeviceBuffer state_[4], a1_, a2_, q_;
     DeviceBuffer coef_, dose_w_, yzw_, om_, factors_, su_, sv_, e1_, e2_, beg_, end_, kv_;
+    SourcePlanBuffers plan_;
+    AnglePlanBuffers angle_plan_;
     DeviceBuffer partial_p_, partial_s_, hist_;
     T* cur_[4] = {nullptr, nullptr, nullptr, nullptr};
-    T* spare_ptr_ = nullptr;
     int step_ = 0;
     int hist_cap_ = 0;
     std::vector<std::pair<hipEvent_t, Phase>> events_;
@@ -424,7 +522,9 @@ private:
 
 // ============================================================ workspace ===
 struct bfpn_workspace {
-    DeviceBuffer coef, partial, pred, out, a1, a2, conv, factors, su, sv, om, dose_w, yzw, kvT;
+    DeviceBuffer coef, partial, a1, a2, factors, su, sv, om, dose_w, yzw, kvT;
+    SourcePlanBuffers plan;
+    AnglePlanBuffers angle_plan;
 };
 
 namespace {
@@ -466,9 +566,23 @@ void op_source(bfpn_workspace* ws, const T* Fp, const T* fp, T* Q1, const double
     ws->a2.ensure(n * sizeof(T));
     ws->kvT.ensure(nk * sizeof(T));
     launch_convert<T>(kv, ws->kvT.as<T>(), nk, s);
-    launch_source_angular<T>(Fp, fp, ws->kvT.as<T>(), ws->a1.as<T>(), ws->a2.as<T>(), Ng, nyz, nang, s);
-    launch_source_energy<T>(ws->a1.as<T>(), ws->a2.as<T>(), e1, e2, beg, end, Q1, Ng,
-                            static_cast<long long>(nyz) * nang, dg, s);
+    if constexpr (std::is_same_v<T, float>) {
+        std::vector<double> h1 = to_host(e1, static_cast<size_t>(Ng) * Ng);
+        std::vector<double> h2 = to_host(e2, static_cast<size_t>(Ng) * Ng);
+        std::vector<int> hb(Ng), he(Ng);
+        BFPN_HIP(hipMemcpy(hb.data(), beg, Ng * sizeof(int), hipMemcpyDeviceToHost));
+        BFPN_HIP(hipMemcpy(he.data(), end, Ng * sizeof(int), hipMemcpyDeviceToHost));
+        BFPN_HIP(hipStreamSynchronize(s));
+        ws->plan.build(ws->kvT.as<float>(), h1, h2, hb, he, Ng, nang, dg, s);
+        launch_source_mfma(Fp, fp, ws->a1.as<float>(), ws->a2.as<float>(), Q1, ws->plan.plan, Ng,
+                           nyz, nang, s);
+        BFPN_HIP(hipStreamSynchronize(s));
+    } else {
+        launch_source_angular<T>(Fp, fp, ws->kvT.as<T>(), ws->a1.as<T>(), ws->a2.as<T>(), Ng, nyz,
+                                 nang, s);
+        launch_source_energy<T>(ws->a1.as<T>(), ws->a2.as<T>(), e1, e2, beg, end, Q1, Ng,
+                                static_cast<long long>(nyz) * nang, dg, s);
+    }
 }
 
 template <typename T>
@@ -476,14 +590,9 @@ void op_transport(bfpn_workspace* ws, T* F, const double* omend, int Ny, int Nz,
                   double dy, double dz, double h, int preserve, hipStream_t s) {
     BFPN_REQUIRE(ws && F && omend, "null argument");
     BFPN_REQUIRE(Ny > 0 && Nz > 0 && Ng > 0 && nang > 0, "invalid sizes");
-    const size_t n = static_cast<size_t>(Ng) * (Ny + 1) * (Nz + 1) * nang;
-    ws->pred.ensure(n * sizeof(T));
-    ws->out.ensure(n * sizeof(T));
     ws->om.ensure(2 * nang * sizeof(T));
     launch_convert<T>(omend, ws->om.as<T>(), 2 * nang, s);
-    launch_transport_half<T>(F, ws->pred.as<T>(), ws->out.as<T>(), ws->om.as<T>(), Ny, Nz, Ng, nang,
-                             dy, dz, h, preserve != 0, s);
-    BFPN_HIP(hipMemcpyAsync(F, ws->out.as<T>(), n * sizeof(T), hipMemcpyDeviceToDevice, s));
+    launch_transport_half<T>(F, ws->om.as<T>(), Ny, Nz, Ng, nang, dy, dz, h, preserve != 0, s);
 }
 
 template <typename T>
@@ -492,9 +601,16 @@ void op_angle(bfpn_workspace* ws, T* F, const double* sig_tr_host, int nyz, int 
     BFPN_REQUIRE(ws && F && sig_tr_host, "null argument");
     BFPN_REQUIRE(nyz > 0 && Ng > 0 && Nmu > 0 && Nom > 0 && Nmu * Nom <= 1024, "invalid sizes");
     upload(ws->factors, cast_vector<T>(angle_factors(sig_tr_host, Ng, Nmu, Nom, du, dv, dt, rho)), s);
-    upload(ws->su, cast_vector<T>(sine_basis(Nom)), s);
-    upload(ws->sv, cast_vector<T>(sine_basis(Nmu)), s);
-    launch_angle_full<T>(F, ws->factors.as<T>(), ws->su.as<T>(), ws->sv.as<T>(), Ng, nyz, Nmu, Nom, s);
+    if (std::is_same_v<T, float> && AnglePlanBuffers::usable(Nmu, Nom)) {
+        ws->angle_plan.build(Nmu, Nom, s);
+        launch_angle_mfma(reinterpret_cast<float*>(F), ws->factors.as<float>(), ws->angle_plan.plan,
+                          Ng, nyz, Nmu, Nom, s);
+    } else {
+        upload(ws->su, cast_vector<T>(sine_basis(Nom)), s);
+        upload(ws->sv, cast_vector<T>(sine_basis(Nmu)), s);
+        launch_angle_full<T>(F, ws->factors.as<T>(), ws->su.as<T>(), ws->sv.as<T>(), Ng, nyz, Nmu,
+                             Nom, s);
+    }
     BFPN_HIP(hipStreamSynchronize(s));
 }
 

@@ -4,24 +4,215 @@
 //   Q1[g,s,m] = dg * sum_{g'} sum_{m'} (K1[g,g'] max(Fp,0) + K2[g,g'] fp/3)[g',s,m'] Kv[g',m',m]
 // evaluated per output cell as a (g', m') double loop (1.6k-4k terms/cell).
 //
-// MI355X design: the sum is factorised into
-//   1. an angular contraction per source group, a dense GEMM
-//        A1[g'] = max(Fp[g'],0) . Kv[g'],   A2[g'] = fp[g'] . Kv[g']
-//      with rows = transverse nodes (nyz) and K = N = nang, tiled through LDS
-//      (both products share the Kv tile); and
-//   2. a banded energy sum Q = dg (sum K1 A1 + (sum K2 A2)/3) over the band
-//      [beg_g, end_g) of non-zero source groups, coalesced over lanes.
-// This is exactly the reference's sum regrouped; only rounding order differs.
+// MI355X design: the sum is factorised into two GEMMs,
+//   1. angular, one per source group g':  A1[g'] = max(Fp[g'],0) . Kv[g'],
+//      A2[g'] = fp[g'] . Kv[g']   (rows = transverse nodes, K = N = nang);
+//   2. banded energy:  Q = E1 . A1 + E2 . A2  with E1 = dg K1 and E2 = dg K2/3,
+//      where each 16-group output tile only visits the source groups of its
+//      band [min beg, max end).
+// FP32 state runs both on the matrix cores in split-BF16 with FP32
+// accumulation (three MFMA issues per tile, ~2^-16 relative per product);
+// the Kv and E operands are split once at set-up.  FP64 state keeps exact
+// FP64 vector kernels (oracle-grade instantiation).  This is the reference's
+// sum regrouped; only rounding order differs.
 
 #include "kernels.hpp"
+#include "mfma.hpp"
 
 namespace bfpn {
 namespace {
 
-constexpr int kTile = 64;   // output tile: 64 nodes x 64 directions
-constexpr int kDepth = 16;  // contraction depth per LDS stage
+// ------------------------------------------------------------ FP32 / MFMA --
+// grid: (ceil(nang/64), ceil(nyz/64), Ng); 4 waves, each a 32x32 output block.
+__global__ __launch_bounds__(256) void source_angular_mfma(const float* __restrict__ Fp,
+                                                           const float* __restrict__ fp,
+                                                           const __bf16* __restrict__ kth,
+                                                           const __bf16* __restrict__ ktl,
+                                                           float* __restrict__ A1,
+                                                           float* __restrict__ A2, int nyz,
+                                                           int nang, int nap) {
+    const int g = blockIdx.z;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = lane & 15, q = lane >> 4;
+    const int s_base = blockIdx.y * 64 + (wave & 1) * 32;
+    const int m_base = blockIdx.x * 64 + (wave >> 1) * 32;
+    const long long plane = static_cast<long long>(g) * nyz * nang;
+    const long long kt_plane = static_cast<long long>(g) * nap * nap;
+    f32x4 acc1[2][2], acc2[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc1[a][b] = acc2[a][b] = zero4();
 
-// grid: (ceil(nang/64), ceil(nyz/64), Ng); 256 threads, each 4x4 outputs x2.
+    for (int k0 = 0; k0 < nap; k0 += 32) {
+        SplitFrag x1[2], x2[2], kf[2];
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) {
+            const int s = s_base + mt * 16 + i;
+            float v1[8], v2[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int k = k0 + 8 * q + e;
+                const bool in = s < nyz && k < nang;
+                const long long idx = plane + static_cast<long long>(s) * nang + k;
+                const float a = in ? Fp[idx] : 0.f;
+                v1[e] = a > 0.f ? a : 0.f;
+                v2[e] = in ? fp[idx] : 0.f;
+            }
+            x1[mt] = split8(v1);
+            x2[mt] = split8(v2);
+        }
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const int m = m_base + nt * 16 + i;
+            if (m < nap) {
+                const long long off = kt_plane + static_cast<long long>(m) * nap + k0 + 8 * q;
+                kf[nt].hi = *reinterpret_cast<const bf16x8*>(kth + off);
+                kf[nt].lo = *reinterpret_cast<const bf16x8*>(ktl + off);
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) kf[nt].hi[e] = kf[nt].lo[e] = static_cast<__bf16>(0.f);
+            }
+        }
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) {
+                acc1[mt][nt] = mma_split(x1[mt], kf[nt], acc1[mt][nt]);
+                acc2[mt][nt] = mma_split(x2[mt], kf[nt], acc2[mt][nt]);
+            }
+    }
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const int m = m_base + nt * 16 + i;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int s = s_base + mt * 16 + 4 * q + r;
+                if (s < nyz && m < nang) {
+                    const long long idx = plane + static_cast<long long>(s) * nang + m;
+                    A1[idx] = acc1[mt][nt][r];
+                    A2[idx] = acc2[mt][nt][r];
+                }
+            }
+        }
+}
+
+constexpr int kBandLanes = 128;  // lanes per work-group
+constexpr int kBandRows = 64;    // output groups per work-group
+constexpr int kPitch = kBandLanes + 2;
+
+// grid: (ceil(lanes/128), ceil(Ng/64)); Q[g][lane] = E1.A1 + E2.A2 over the band.
+__global__ __launch_bounds__(256) void source_band_mfma(
+    const float* __restrict__ A1, const float* __restrict__ A2, const __bf16* __restrict__ e1h,
+    const __bf16* __restrict__ e1l, const __bf16* __restrict__ e2h, const __bf16* __restrict__ e2l,
+    const int2* __restrict__ tile_band, float* __restrict__ Q, int Ng, int ngp, long long lanes) {
+    __shared__ float b1s[32][kPitch];
+    __shared__ float b2s[32][kPitch];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = lane & 15, q = lane >> 4;
+    const int g_base = blockIdx.y * kBandRows;
+    const long long n_base = static_cast<long long>(blockIdx.x) * kBandLanes;
+    const int t0 = g_base / 16;
+    int2 band[4];
+    int kb = 1 << 30, ke = 0;
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        band[mt] = tile_band[t0 + mt];
+        if (band[mt].y > band[mt].x) {
+            kb = min(kb, band[mt].x);
+            ke = max(ke, band[mt].y);
+        }
+    }
+    f32x4 acc[4][2];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) acc[mt][0] = acc[mt][1] = zero4();
+
+    for (int k0 = (kb / 32) * 32; k0 < ke; k0 += 32) {
+        // stage A1/A2 rows [k0, k0+32) x lanes [n_base, n_base+128)
+        for (int e = threadIdx.x; e < 32 * kBandLanes; e += 256) {
+            const int r = e / kBandLanes, c = e - r * kBandLanes;
+            const int gs = k0 + r;
+            const long long ln = n_base + c;
+            const bool in = gs < Ng && ln < lanes;
+            const long long idx = static_cast<long long>(gs) * lanes + ln;
+            b1s[r][c] = in ? A1[idx] : 0.f;
+            b2s[r][c] = in ? A2[idx] : 0.f;
+        }
+        __syncthreads();
+        SplitFrag bf1[2], bf2[2];
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const int c = wave * 32 + nt * 16 + i;
+            float v1[8], v2[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                v1[e] = b1s[8 * q + e][c];
+                v2[e] = b2s[8 * q + e][c];
+            }
+            bf1[nt] = split8(v1);
+            bf2[nt] = split8(v2);
+        }
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            if (band[mt].y <= band[mt].x || k0 + 32 <= band[mt].x || k0 >= band[mt].y) continue;
+            const long long off = static_cast<long long>(g_base + mt * 16 + i) * ngp + k0 + 8 * q;
+            SplitFrag a1, a2;
+            a1.hi = *reinterpret_cast<const bf16x8*>(e1h + off);
+            a1.lo = *reinterpret_cast<const bf16x8*>(e1l + off);
+            a2.hi = *reinterpret_cast<const bf16x8*>(e2h + off);
+            a2.lo = *reinterpret_cast<const bf16x8*>(e2l + off);
+#pragma unroll
+            for (int nt = 0; nt < 2; ++nt) {
+                acc[mt][nt] = mma_split(a1, bf1[nt], acc[mt][nt]);
+                acc[mt][nt] = mma_split(a2, bf2[nt], acc[mt][nt]);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+        for (int nt = 0; nt < 2; ++nt) {
+            const long long ln = n_base + wave * 32 + nt * 16 + i;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int g = g_base + mt * 16 + 4 * q + r;
+                if (g < Ng && ln < lanes) Q[static_cast<long long>(g) * lanes + ln] = acc[mt][nt][r];
+            }
+        }
+}
+
+// Kv[g][in][out] (FP32) -> transposed split copies KT[g][out][in] (BF16 hi/lo),
+// zero padded to nap x nap.  grid: (ceil(nap*nap/256), Ng)
+__global__ void split_transpose_kernel(const float* __restrict__ kv, __bf16* __restrict__ hi,
+                                       __bf16* __restrict__ lo, int nang, int nap) {
+    const int e = blockIdx.x * 256 + threadIdx.x;
+    if (e >= nap * nap) return;
+    const int out = e / nap, in = e - out * nap;
+    const long long g = blockIdx.y;
+    const float v = (out < nang && in < nang) ? kv[(g * nang + in) * nang + out] : 0.f;
+    __bf16 h, l;
+    split_bf16(v, h, l);
+    hi[g * nap * nap + e] = h;
+    lo[g * nap * nap + e] = l;
+}
+
+__global__ void split_kernel(const float* __restrict__ src, __bf16* __restrict__ hi,
+                             __bf16* __restrict__ lo, long long n) {
+    const long long e = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+    if (e >= n) return;
+    __bf16 h, l;
+    split_bf16(src[e], h, l);
+    hi[e] = h;
+    lo[e] = l;
+}
+
+// ------------------------------------------------------------ FP64 vector --
+constexpr int kTile = 64;
+constexpr int kDepth = 16;
+
 template <typename T>
 __global__ __launch_bounds__(256) void source_angular_kernel(const T* __restrict__ Fp,
                                                              const T* __restrict__ fp,
@@ -36,16 +227,14 @@ __global__ __launch_bounds__(256) void source_angular_kernel(const T* __restrict
     const int s0 = blockIdx.y * kTile;
     const long long plane = static_cast<long long>(g) * nyz * nang;
     const T* K = kv + static_cast<long long>(g) * nang * nang;
-    const int tx = threadIdx.x & 15;  // output directions m0 + tx + 16*j
-    const int ty = threadIdx.x >> 4;  // output nodes     s0 + ty + 16*i
+    const int tx = threadIdx.x & 15;
+    const int ty = threadIdx.x >> 4;
     T acc1[4][4], acc2[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc1[i][j] = acc2[i][j] = T(0);
-
     for (int k0 = 0; k0 < nang; k0 += kDepth) {
-        // stage x tiles: 64 nodes x 16 inputs (coalesced along the input index)
         for (int e = threadIdx.x; e < kTile * kDepth; e += 256) {
             const int kk = e % kDepth, r = e / kDepth;
             const int s = s0 + r, k = k0 + kk;
@@ -58,7 +247,6 @@ __global__ __launch_bounds__(256) void source_angular_kernel(const T* __restrict
             x1s[kk][r] = v1;
             x2s[kk][r] = v2;
         }
-        // stage Kv tile: 16 inputs x 64 outputs (coalesced along outputs)
         for (int e = threadIdx.x; e < kTile * kDepth; e += 256) {
             const int c = e % kTile, kk = e / kTile;
             const int k = k0 + kk, m = m0 + c;
@@ -101,7 +289,6 @@ __global__ __launch_bounds__(256) void source_angular_kernel(const T* __restrict
     }
 }
 
-// grid: (ceil(lanes/256), Ng); Q[g][lane] over the band of row g.
 template <typename T>
 __global__ __launch_bounds__(256) void source_energy_kernel(
     const T* __restrict__ A1, const T* __restrict__ A2, const double* __restrict__ e1,
@@ -126,6 +313,39 @@ __global__ __launch_bounds__(256) void source_energy_kernel(
 }
 
 }  // namespace
+
+int source_nap(int nang) { return (nang + 31) / 32 * 32; }
+int source_ngp(int Ng) { return (Ng + 31) / 32 * 32; }
+int source_erows(int Ng) { return (Ng + kBandRows - 1) / kBandRows * kBandRows; }
+
+void launch_split_transpose(const float* kv, __bf16* hi, __bf16* lo, int Ng, int nang,
+                            hipStream_t s) {
+    const int nap = source_nap(nang);
+    dim3 grid((nap * nap + 255) / 256, Ng);
+    hipLaunchKernelGGL(split_transpose_kernel, grid, dim3(256), 0, s, kv, hi, lo, nang, nap);
+    BFPN_HIP(hipGetLastError());
+}
+
+void launch_split(const float* src, __bf16* hi, __bf16* lo, long long n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(split_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, s,
+                       src, hi, lo, n);
+    BFPN_HIP(hipGetLastError());
+}
+
+void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, float* Q,
+                        const SourcePlan& plan, int Ng, int nyz, int nang, hipStream_t s) {
+    const int nap = source_nap(nang);
+    dim3 g1((nang + 63) / 64, (nyz + 63) / 64, Ng);
+    hipLaunchKernelGGL(source_angular_mfma, g1, dim3(256), 0, s, Fp, fp, plan.kth, plan.ktl, A1, A2,
+                       nyz, nang, nap);
+    const long long lanes = static_cast<long long>(nyz) * nang;
+    dim3 g2(static_cast<unsigned>((lanes + kBandLanes - 1) / kBandLanes),
+            (Ng + kBandRows - 1) / kBandRows);
+    hipLaunchKernelGGL(source_band_mfma, g2, dim3(256), 0, s, A1, A2, plan.e1h, plan.e1l, plan.e2h,
+                       plan.e2l, plan.tile_band, Q, Ng, source_ngp(Ng), lanes);
+    BFPN_HIP(hipGetLastError());
+}
 
 template <typename T>
 void launch_source_angular(const T* Fp, const T* fp, const T* kv, T* A1, T* A2, int Ng, int nyz,

@@ -3,113 +3,172 @@
 // Reference: muscl_paper_slope, spatial_flux_divergence_value and the
 // flux / predictor / corrector kernels of SpatialTransportSolver::solveEq18
 // (src/spatial_transport.cu:23-161, 202-262): four launches and three
-// scratch arrays per call.
+// scratch arrays per call, a division per slope.
 //
-// MI355X design: two launches.  The predictor writes psi* = clip(psi - h L psi);
-// the corrector recomputes L psi from the input instead of storing it and
-// writes psi_new = clip(psi - h/2 (L psi + L psi*)) to a separate buffer, so no
-// flux array ever touches HBM.  Threads run along the direction index m,
-// which is the contiguous axis of the [g][s][m] layout, so every stencil tap
-// (+-1, +-2 nodes in y or z) is a coalesced row read served mostly from L2.
+// MI355X design: one launch, in place, each value read once and written once.
+//  * A work-group owns one energy group g and a chunk of MW consecutive
+//    directions m (the contiguous axis) for the whole (y, z) plane, so no
+//    other work-group touches its data and the update can be written in place.
+//  * It marches along z with rolling windows in LDS: 5 rows of psi, 5 rows of
+//    the predictor psi* and 3 rows of L(psi).  At step t it loads psi row t+4,
+//    forms L(psi) and psi* = clip(psi - h L psi) on row t+2, and writes row t:
+//    clip(psi - h/2 (L psi + L psi*)).  Rows outside [0, Nz] are clamped,
+//    exactly like the reference's clamped stencil taps.
+//  * The superbee slope is evaluated without a division:
+//      a = R - C, b = C - L, ab > 0:
+//      slope = sign(a) max(min(2|b|, |a|), min(|b|, 2|a|)),   else 0,
+//    which equals forward * max(0, min(2 theta,1), min(theta,2)) with
+//    theta = b/a, up to the rounding of that quotient; |a| < 1e-10 -> 0 as in
+//    the reference.
+//  * MW adapts to the plane width (64 / 32 / 16 directions for <= 24 / 48 /
+//    more y nodes) so every z step gives the 256 threads ~1000 cells.
 
 #include "kernels.hpp"
 
 namespace bfpn {
 namespace {
 
-template <typename T>
-__device__ inline T limited_slope(T left, T center, T right) {
-    const T forward = right - center;
-    if (fabs(forward) < T(1e-10)) return T(0);
-    const T theta = (center - left) / forward;
-    const T eta = fmax(T(0), fmax(fmin(T(2) * theta, T(1)), fmin(theta, T(2))));
-    return forward * eta;
-}
-
-// Upwind MUSCL face-flux difference along one axis for velocity w, given the
-// five taps (clamped at the domain edge) around the centre c.
-template <typename T>
-__device__ inline T axis_flux_difference(T w, T m2, T m1, T c, T p1, T p2) {
-    if (w >= T(0)) {
-        const T left = w * (m1 + T(0.5) * limited_slope(m2, m1, c));
-        const T right = w * (c + T(0.5) * limited_slope(m1, c, p1));
-        return right - left;
-    }
-    const T left = w * (c - T(0.5) * limited_slope(m1, c, p1));
-    const T right = w * (p1 - T(0.5) * limited_slope(c, p1, p2));
-    return right - left;
-}
+constexpr int kThreads = 256;
+constexpr int kPsiRows = 5, kPredRows = 5, kLRows = 3;
 
 template <typename T>
-__device__ inline T flux_divergence(const T* __restrict__ in, long long idx, int i, int j, int Ny,
-                                    int Nz, long long sy, long long sz, T wy, T wz, T dy,
-                                    T dz) {
-    const T c = in[idx];
-    const T ym2 = in[idx + (max(0, i - 2) - i) * sy];
-    const T ym1 = in[idx + (max(0, i - 1) - i) * sy];
-    const T yp1 = in[idx + (min(Ny, i + 1) - i) * sy];
-    const T yp2 = in[idx + (min(Ny, i + 2) - i) * sy];
-    const T zm2 = in[idx + (max(0, j - 2) - j) * sz];
-    const T zm1 = in[idx + (max(0, j - 1) - j) * sz];
-    const T zp1 = in[idx + (min(Nz, j + 1) - j) * sz];
-    const T zp2 = in[idx + (min(Nz, j + 2) - j) * sz];
-    return axis_flux_difference(wy, ym2, ym1, c, yp1, yp2) / dy +
-           axis_flux_difference(wz, zm2, zm1, c, zp1, zp2) / dz;
+__device__ inline T superbee_slope(T left, T center, T right) {
+    const T a = right - center;
+    const T b = center - left;
+    if (fabs(a) < T(1e-10) || !(a * b > T(0))) return T(0);
+    const T aa = fabs(a), ab = fabs(b);
+    const T mag = fmax(fmin(T(2) * ab, aa), fmin(ab, T(2) * aa));
+    return a > T(0) ? mag : -mag;
 }
 
-// grid: (ceil(nyz*nang/256), Ng)
-template <typename T, bool CORRECT>
-__global__ __launch_bounds__(256) void transport_kernel(const T* __restrict__ in,
-                                                        const T* __restrict__ pred,
-                                                        T* __restrict__ out,
-                                                        const T* __restrict__ omend, int Ny,
-                                                        int Nz, int nang, T dy, T dz, T h,
-                                                        int preserve_sign) {
+// Upwind MUSCL face-flux difference along one axis (w >= 0: faces use the
+// left cells' reconstructions; w < 0: the right cells').
+template <typename T>
+__device__ inline T flux_difference(T w, T m2, T m1, T c, T p1, T p2) {
+    const bool pos = w >= T(0);
+    // left face value and right face value of the reconstruction
+    const T sl = pos ? superbee_slope(m2, m1, c) : superbee_slope(m1, c, p1);
+    const T sr = pos ? superbee_slope(m1, c, p1) : superbee_slope(c, p1, p2);
+    const T left = pos ? m1 + T(0.5) * sl : c - T(0.5) * sl;
+    const T right = pos ? c + T(0.5) * sr : p1 - T(0.5) * sr;
+    return w * right - w * left;
+}
+
+// L(row r) for the cell (iy, mm) from a ring of rows in LDS.
+template <typename T, int MW>
+__device__ inline T divergence(const T* __restrict__ rows, int ring, int r, int Nz, int iy, int Ny,
+                               int mm, T wy, T wz, T inv_dy, T inv_dz, int row_stride) {
+    auto at = [&](int rr, int yy) -> T {
+        rr = min(max(rr, 0), Nz);
+        return rows[(rr % ring) * row_stride + yy * MW + mm];
+    };
+    const T c = at(r, iy);
+    const T ym2 = at(r, max(iy - 2, 0)), ym1 = at(r, max(iy - 1, 0));
+    const T yp1 = at(r, min(iy + 1, Ny)), yp2 = at(r, min(iy + 2, Ny));
+    const T zm2 = at(r - 2, iy), zm1 = at(r - 1, iy), zp1 = at(r + 1, iy), zp2 = at(r + 2, iy);
+    return flux_difference(wy, ym2, ym1, c, yp1, yp2) * inv_dy +
+           flux_difference(wz, zm2, zm1, c, zp1, zp2) * inv_dz;
+}
+
+// grid: (ceil(nang/MW), Ng); block: 256 threads; dynamic LDS (13 rows).
+template <typename T, int MW>
+__global__ __launch_bounds__(kThreads) void transport_march_kernel(
+    T* __restrict__ F, const T* __restrict__ omend, int Ny, int Nz, int nang, T inv_dy, T inv_dz,
+    T h, int preserve_sign) {
+    extern __shared__ unsigned char lds_raw[];
     const int Ny1 = Ny + 1;
-    const int plane = Ny1 * (Nz + 1) * nang;
-    const int r = blockIdx.x * 256 + threadIdx.x;
-    if (r >= plane) return;
-    const int m = r % nang;
-    const int s = r / nang;
-    const int i = s % Ny1;
-    const int j = s / Ny1;
-    const long long idx = static_cast<long long>(blockIdx.y) * plane + r;
-    const long long sy = nang;
-    const long long sz = static_cast<long long>(Ny1) * nang;
+    const int row_stride = Ny1 * MW;
+    T* psi = reinterpret_cast<T*>(lds_raw);
+    T* pred = psi + kPsiRows * row_stride;
+    T* Lrow = pred + kPredRows * row_stride;
+    const int m0 = blockIdx.x * MW;
+    const long long plane = static_cast<long long>(Ny1) * (Nz + 1) * nang;
+    T* Fg = F + static_cast<long long>(blockIdx.y) * plane;
+    const long long zstride = static_cast<long long>(Ny1) * nang;
+
+    // MW divides the block size, so every thread keeps one direction mm and
+    // walks the y nodes iy0, iy0 + kThreads/MW, ...
+    static_assert(kThreads % MW == 0, "MW must divide the block size");
+    constexpr int kYStep = kThreads / MW;
+    const int mm = threadIdx.x % MW;
+    const int iy0 = threadIdx.x / MW;
+    const bool live = m0 + mm < nang;
+    const int m = live ? m0 + mm : nang - 1;
     const T wy = omend[2 * m], wz = omend[2 * m + 1];
-    const T L0 = flux_divergence(in, idx, i, j, Ny, Nz, sy, sz, wy, wz, dy, dz);
-    T v;
-    if (CORRECT) {
-        const T L1 = flux_divergence(pred, idx, i, j, Ny, Nz, sy, sz, wy, wz, dy, dz);
-        v = in[idx] - T(0.5) * h * (L0 + L1);
-    } else {
-        v = in[idx] - h * L0;
+
+    auto load_row = [&](int r) {  // psi row r (0 <= r <= Nz) into the ring
+        T* dst = psi + (r % kPsiRows) * row_stride;
+        const T* src = Fg + r * zstride + m0 + mm;
+        for (int iy = iy0; iy < Ny1; iy += kYStep)
+            dst[iy * MW + mm] = live ? src[static_cast<long long>(iy) * nang] : T(0);
+    };
+
+    for (int r = 0; r <= min(Nz, 1); ++r) load_row(r);
+    for (int t = -2; t <= Nz; ++t) {
+        if (t + 4 <= Nz && t + 4 >= 2) load_row(t + 4);
+        __syncthreads();
+        const int rp = t + 2;
+        if (rp >= 0 && rp <= Nz) {  // L(psi) and psi* on row rp
+            for (int iy = iy0; iy < Ny1; iy += kYStep) {
+                const int k = iy * MW + mm;
+                const T L0 = divergence<T, MW>(psi, kPsiRows, rp, Nz, iy, Ny, mm, wy, wz, inv_dy,
+                                               inv_dz, row_stride);
+                const T v = psi[(rp % kPsiRows) * row_stride + k] - h * L0;
+                pred[(rp % kPredRows) * row_stride + k] = preserve_sign ? v : fmax(v, T(0));
+                Lrow[(rp % kLRows) * row_stride + k] = L0;
+            }
+        }
+        __syncthreads();
+        if (t >= 0 && live) {  // corrector on row t, written in place
+            T* dst = Fg + t * zstride + m0 + mm;
+            for (int iy = iy0; iy < Ny1; iy += kYStep) {
+                const int k = iy * MW + mm;
+                const T L1 = divergence<T, MW>(pred, kPredRows, t, Nz, iy, Ny, mm, wy, wz, inv_dy,
+                                               inv_dz, row_stride);
+                const T L0 = Lrow[(t % kLRows) * row_stride + k];
+                const T v = psi[(t % kPsiRows) * row_stride + k] - T(0.5) * h * (L0 + L1);
+                dst[static_cast<long long>(iy) * nang] = preserve_sign ? v : fmax(v, T(0));
+            }
+        }
+        __syncthreads();
     }
-    out[idx] = preserve_sign ? v : fmax(v, T(0));
+}
+
+template <typename T, int MW>
+void launch_march(T* F, const T* omend, int Ny, int Nz, int Ng, int nang, double dy, double dz,
+                  double h, bool preserve_sign, hipStream_t s) {
+    const size_t lds = sizeof(T) * static_cast<size_t>(kPsiRows + kPredRows + kLRows) * (Ny + 1) * MW;
+    BFPN_REQUIRE(lds <= 160 * 1024, "transport plane too wide for the LDS window");
+    BFPN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&transport_march_kernel<T, MW>),
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    dim3 grid((nang + MW - 1) / MW, Ng);
+    hipLaunchKernelGGL((transport_march_kernel<T, MW>), grid, dim3(kThreads), lds, s, F, omend, Ny,
+                       Nz, nang, static_cast<T>(1.0 / dy), static_cast<T>(1.0 / dz),
+                       static_cast<T>(h), preserve_sign ? 1 : 0);
 }
 
 }  // namespace
 
 template <typename T>
-void launch_transport_half(const T* in, T* pred, T* out, const T* omend, int Ny, int Nz, int Ng,
-                           int nang, double dy, double dz, double h, bool preserve_sign,
-                           hipStream_t s) {
-    const long long plane = static_cast<long long>(Ny + 1) * (Nz + 1) * nang;
-    BFPN_REQUIRE(plane < (1LL << 31), "transport plane too large");
-    if (plane == 0 || Ng <= 0) return;
-    dim3 grid(static_cast<unsigned>((plane + 255) / 256), Ng);
-    const T ty = static_cast<T>(dy), tz = static_cast<T>(dz);
-    hipLaunchKernelGGL((transport_kernel<T, false>), grid, dim3(256), 0, s, in, nullptr, pred, omend,
-                       Ny, Nz, nang, ty, tz, static_cast<T>(h), preserve_sign ? 1 : 0);
-    hipLaunchKernelGGL((transport_kernel<T, true>), grid, dim3(256), 0, s, in, pred, out, omend, Ny,
-                       Nz, nang, ty, tz, static_cast<T>(h), preserve_sign ? 1 : 0);
+void launch_transport_half(T* F, const T* omend, int Ny, int Nz, int Ng, int nang, double dy,
+                           double dz, double h, bool preserve_sign, hipStream_t s) {
+    if (Ng <= 0 || nang <= 0) return;
+    BFPN_REQUIRE(Ny >= 1 && Nz >= 1, "transport needs Ny, Nz >= 1");
+    const int Ny1 = Ny + 1;
+    if (Ny1 <= 24)
+        launch_march<T, 64>(F, omend, Ny, Nz, Ng, nang, dy, dz, h, preserve_sign, s);
+    else if (Ny1 <= 48)
+        launch_march<T, 32>(F, omend, Ny, Nz, Ng, nang, dy, dz, h, preserve_sign, s);
+    else if (Ny1 <= 81)
+        launch_march<T, 16>(F, omend, Ny, Nz, Ng, nang, dy, dz, h, preserve_sign, s);
+    else
+        launch_march<T, 8>(F, omend, Ny, Nz, Ng, nang, dy, dz, h, preserve_sign, s);
     BFPN_HIP(hipGetLastError());
 }
 
-template void launch_transport_half<float>(const float*, float*, float*, const float*, int, int,
-                                           int, int, double, double, double, bool, hipStream_t);
-template void launch_transport_half<double>(const double*, double*, double*, const double*, int,
-                                            int, int, int, double, double, double, bool,
-                                            hipStream_t);
+template void launch_transport_half<float>(float*, const float*, int, int, int, int, double, double,
+                                           double, bool, hipStream_t);
+template void launch_transport_half<double>(double*, const double*, int, int, int, int, double,
+                                            double, double, bool, hipStream_t);
 
 }  // namespace bfpn

@@ -4,9 +4,11 @@ Inputs are a physical state (the oracle's small beam after a few depth steps,
 so tails, zeros and both signs of psi^2 occur) plus seeded random fields.
 Tolerances, relative to the largest magnitude of the reference output:
   FP64 state: 1e-12 (same algebra; only FMA contraction and summation order differ)
-  FP32 state: energy 2e-6, source 1e-5, angle 1e-5, transport 5e-5, IDD 1e-6
-              (transport: a single-precision limiter can take the other branch
-              of the superbee kink in isolated cells).
+  FP32 state: energy 2e-6, transport 5e-5, IDD 1e-6; source and angle 1e-4
+              (both run on the matrix cores in split-BF16 with FP32
+              accumulation, ~2^-16 relative per product and stage; a
+              single-precision limiter can take the other branch of the
+              superbee kink in isolated cells).
 """
 import numpy as np
 import pytest
@@ -17,7 +19,7 @@ pytestmark = pytest.mark.gpu
 
 RNG = np.random.default_rng(7)
 TOL64 = 1e-12
-TOL32 = {"energy": 2e-6, "source": 1e-5, "angle": 1e-5, "transport": 5e-5}
+TOL32 = {"energy": 2e-6, "source": 1e-4, "angle": 1e-4, "transport": 5e-5}
 
 
 @pytest.fixture(scope="module")
@@ -102,15 +104,17 @@ def test_angle_full(gpu, oracle_mod, physical, dtype, which):
     _close(dF.numpy().astype(np.float64), F, TOL64 if dtype == np.float64 else TOL32["angle"])
 
 
-@pytest.mark.parametrize("Nmu,Nom", [(20, 20), (7, 3), (1, 9)])
-def test_angle_shapes(gpu, oracle_mod, Nmu, Nom):
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("Nmu,Nom", [(20, 20), (7, 3), (1, 9), (17, 32), (33, 5)])
+def test_angle_shapes(gpu, oracle_mod, Nmu, Nom, dtype):
+    """Shapes around the MFMA tiling limits (16, 32) and past them (vector path)."""
     Ng, nyz = 4, 5
-    F = RNG.standard_normal(Ng * nyz * Nmu * Nom)
+    F = RNG.standard_normal(Ng * nyz * Nmu * Nom).astype(dtype).astype(np.float64)
     sig = RNG.random(Ng) * 1e-3
-    dF = _dev(gpu, F, np.float64)
+    dF = _dev(gpu, F, dtype)
     gpu.ops.angle_full(dF, sig, nyz, Ng, Nmu, Nom, 0.1, 0.07, 0.02, 1.85)
     oracle_mod.angle_full(F, sig, nyz, Ng, Nmu, Nom, 0.1, 0.07, 0.02, 1.85)
-    _close(dF.numpy(), F, TOL64)
+    _close(dF.numpy().astype(np.float64), F, TOL64 if dtype == np.float64 else TOL32["angle"])
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
