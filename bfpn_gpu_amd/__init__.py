@@ -18,6 +18,7 @@ from ._lib import (  # noqa: F401
     FP64,
     BfpnError,
     DeviceArray,
+    Server,
     Solver,
     Workspace,
     device_count,

@@ -17,6 +17,17 @@ FP64 = 1
 _lib = None
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_active", C.c_int), ("slice_steps", C.c_int),
+                ("precision", C.c_int)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("active", C.c_int), ("queued", C.c_int), ("peak_active", C.c_int),
+                ("cached_tables", C.c_int), ("bytes_in_use", C.c_size_t)]
+
+
 class BfpnError(RuntimeError):
     """A bfpn_* call returned a non-zero status."""
 
@@ -79,6 +90,16 @@ def lib():
     L.bfpn_stream.restype = vp
     L.bfpn_set_profiling.argtypes = [vp, i]
     L.bfpn_phase_stats.argtypes = [vp, vp, vp, vp]
+    L.bfpn_stopping_power.argtypes = [C.c_char_p, i, d, vp]
+    L.bfpn_server_create.argtypes = [C.POINTER(ServerOpts), C.POINTER(vp)]
+    L.bfpn_server_submit.argtypes = [vp, C.POINTER(Grid), C.POINTER(Phys), C.c_char_p, d,
+                                     C.POINTER(C.c_uint64)]
+    L.bfpn_server_poll.argtypes = [vp, C.c_uint64]
+    L.bfpn_server_wait.argtypes = [vp, C.c_uint64, d, vp, i, C.POINTER(C.c_int), vp]
+    L.bfpn_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
+    L.bfpn_server_destroy.argtypes = [vp]
+    L.bfpn_server_estimate.argtypes = [C.POINTER(Grid), C.POINTER(Phys), i, C.POINTER(sz),
+                                       C.POINTER(sz)]
     _lib = L
     return L
 
@@ -251,3 +272,65 @@ class Solver:
     @property
     def stream(self):
         return lib().bfpn_stream(self._h)
+
+
+class Server:
+    """Continuous-batching request loop (bfpn_server): submit beams, wait for IDDs."""
+
+    def __init__(self, max_active: int = 8, slice_steps: int = 8, precision: int = FP32,
+                 hbm_budget_bytes: int = 0):
+        opts = ServerOpts(hbm_budget_bytes, max_active, slice_steps, precision)
+        h = C.c_void_p(0)
+        check(lib().bfpn_server_create(C.byref(opts), C.byref(h)))
+        self._h = h
+
+    def submit(self, case: Case, t_final: float | None = None) -> int:
+        g, p = case.grid(), case.phys()
+        rid = C.c_uint64(0)
+        check(lib().bfpn_server_submit(self._h, C.byref(g), C.byref(p), case.data.encode(),
+                                       case.time if t_final is None else t_final, C.byref(rid)))
+        return rid.value
+
+    def poll(self, rid: int) -> int:
+        return lib().bfpn_server_poll(self._h, rid)
+
+    def wait(self, rid: int, timeout: float = 0.0, capacity: int = 1 << 16):
+        """Returns (idd, {"queued": s, "running": s, "total": s})."""
+        buf = np.zeros(capacity)
+        n = C.c_int(0)
+        t = np.zeros(3)
+        check(lib().bfpn_server_wait(self._h, rid, timeout, _addr(buf), capacity, C.byref(n),
+                                     _addr(t)))
+        return buf[: min(n.value, capacity)].copy(), dict(zip(("queued", "running", "total"),
+                                                              t.tolist()))
+
+    def stats(self) -> dict:
+        st = ServerStats()
+        check(lib().bfpn_server_get_stats(self._h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    @staticmethod
+    def estimate(case: Case, precision: int = FP32) -> tuple[int, int]:
+        """(beam bytes, shared physics bytes) that admission charges for `case`."""
+        g, p = case.grid(), case.phys()
+        a, b = C.c_size_t(0), C.c_size_t(0)
+        check(lib().bfpn_server_estimate(C.byref(g), C.byref(p), precision, C.byref(a),
+                                         C.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            lib().bfpn_server_destroy(self._h)
+            self._h = C.c_void_p(0)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -26,6 +26,9 @@ namespace {
 // tile converts the D layout of one stage into the A/B layout of the next.
 // The DST-I bases are symmetric (S[k][j] = S[j][k]), so Y = X Su^T uses Su
 // as the B operand and W = Sv^T Z uses Sv as the A operand, like Sv Y.
+// MT / NT: 16-row tiles along v (Nmu) and a (Nom), 1 or 2 (compile time, so
+// the fragment arrays stay in registers).
+template <int MT, int NT>
 __global__ __launch_bounds__(256) void angle_mfma_kernel(
     float* __restrict__ F, const float* __restrict__ factors, const __bf16* __restrict__ su_h,
     const __bf16* __restrict__ su_l, const __bf16* __restrict__ sv_h,
@@ -35,8 +38,8 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(
     const int i = lane & 15, q = lane >> 4;
     float(*B)[33] = tile[wave];
     const int na = Nmu * Nom;
-    const int mt_n = Nmu > 16 ? 2 : 1;  // tiles along v / nu
-    const int nt_n = Nom > 16 ? 2 : 1;  // tiles along a / mu
+    constexpr int mt_n = MT;  // tiles along v / nu
+    constexpr int nt_n = NT;  // tiles along a / mu
     SplitFrag su[2], sv[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
@@ -61,7 +64,9 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(
         return split8(v);
     };
     auto store = [&](const f32x4 (&d)[2][2]) {
+#pragma unroll
         for (int mt = 0; mt < mt_n; ++mt)
+#pragma unroll
             for (int nt = 0; nt < nt_n; ++nt)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) B[mt * 16 + 4 * q + r][nt * 16 + i] = d[mt][nt][r];
@@ -78,13 +83,19 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(
         f32x4 d[2][2];
         // stage 1: Y[v][mu] = sum_a X[v][a] Su[mu][a]
         SplitFrag a[2], b[2];
+#pragma unroll
         for (int mt = 0; mt < mt_n; ++mt) a[mt] = row_frag(mt * 16 + i);
+#pragma unroll
         for (int mt = 0; mt < mt_n; ++mt)
+#pragma unroll
             for (int nt = 0; nt < nt_n; ++nt) d[mt][nt] = mma_split(a[mt], su[nt], zero4());
         store(d);
         // stage 2: Z[nu][mu] = C[nu][mu] sum_v Sv[nu][v] Y[v][mu]
+#pragma unroll
         for (int nt = 0; nt < nt_n; ++nt) b[nt] = col_frag(nt * 16 + i);
+#pragma unroll
         for (int mt = 0; mt < mt_n; ++mt)
+#pragma unroll
             for (int nt = 0; nt < nt_n; ++nt) {
                 d[mt][nt] = mma_split(sv[mt], b[nt], zero4());
                 const int mu = nt * 16 + i;
@@ -96,13 +107,19 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(
             }
         store(d);
         // stage 3: W[v][mu] = sum_nu Sv[nu][v] Z[nu][mu]
+#pragma unroll
         for (int nt = 0; nt < nt_n; ++nt) b[nt] = col_frag(nt * 16 + i);
+#pragma unroll
         for (int mt = 0; mt < mt_n; ++mt)
+#pragma unroll
             for (int nt = 0; nt < nt_n; ++nt) d[mt][nt] = mma_split(sv[mt], b[nt], zero4());
         store(d);
         // stage 4: X'[v][a] = sum_mu W[v][mu] Su[mu][a]
+#pragma unroll
         for (int mt = 0; mt < mt_n; ++mt) a[mt] = row_frag(mt * 16 + i);
+#pragma unroll
         for (int mt = 0; mt < mt_n; ++mt)
+#pragma unroll
             for (int nt = 0; nt < nt_n; ++nt) d[mt][nt] = mma_split(a[mt], su[nt], zero4());
         store(d);
         for (int e = lane; e < na; e += 64) {
@@ -186,8 +203,16 @@ void launch_angle_mfma(float* F, const float* factors, const AnglePlan& plan, in
     const long long planes = static_cast<long long>(Ng) * nyz;
     if (planes == 0) return;
     const long long blocks = std::min<long long>((planes + 3) / 4, 4096);
-    hipLaunchKernelGGL(angle_mfma_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, s, F,
-                       factors, plan.su_h, plan.su_l, plan.sv_h, plan.sv_l, planes, nyz, Nmu, Nom);
+    const dim3 grid(static_cast<unsigned>(blocks)), block(256);
+    const int mt = Nmu > 16 ? 2 : 1, nt = Nom > 16 ? 2 : 1;
+#define BFPN_ANGLE(MT_, NT_)                                                                     \
+    hipLaunchKernelGGL((angle_mfma_kernel<MT_, NT_>), grid, block, 0, s, F, factors, plan.su_h,  \
+                       plan.su_l, plan.sv_h, plan.sv_l, planes, nyz, Nmu, Nom)
+    if (mt == 2 && nt == 2) BFPN_ANGLE(2, 2);
+    else if (mt == 2) BFPN_ANGLE(2, 1);
+    else if (nt == 2) BFPN_ANGLE(1, 2);
+    else BFPN_ANGLE(1, 1);
+#undef BFPN_ANGLE
     BFPN_HIP(hipGetLastError());
 }
 

@@ -1,0 +1,395 @@
+// server.hip — continuous-batching request loop for beamlet solves.
+//
+// The reference has no batching or serving mode (one beam per process).  This
+// runtime serves many independent beam solves on one GPU:
+//  * requests are queued by bfpn_server_submit from any thread;
+//  * a scheduler thread admits the oldest queued request as soon as a slot is
+//    free and its device memory fits the HBM budget (state + source scratch of
+//    the beam, plus its physics tables unless a cached copy matches);
+//  * physics tables (material tables, eq15 coefficients, split-BF16 source
+//    operands, CN factors) are shared between beams with equal grid, step,
+//    material and flags, and kept in an LRU cache inside the budget;
+//  * every admitted beam keeps its whole phase space resident and runs on its
+//    own HIP stream; the scheduler enqueues slices of depth steps round-robin
+//    and polls completion events, so work of different beams overlaps on the
+//    device and newly arrived requests join while others are running;
+//  * a finished beam's IDD history is copied out, its memory freed, and the
+//    slot handed to the next request.
+// Results do not depend on what else is running: beams share only read-only
+// tables, so a solve in the mix equals the same solve alone, bit for bit.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <list>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "solver.hpp"
+
+namespace bfpn {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+double seconds_between(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double>(b - a).count();
+}
+
+struct Request {
+    uint64_t id = 0;
+    bfpn_grid grid{};
+    bfpn_phys phys{};
+    std::string data;
+    int steps = 0;
+    size_t ctx_bytes = 0;
+    Clock::time_point t_submit, t_admit, t_done;
+    std::unique_ptr<bfpn_ctx> ctx;
+    std::shared_ptr<PhysicsBase> physics;
+    int issued = 0;
+    hipEvent_t slice_done = nullptr;
+    bool in_flight = false;
+    int status = 0;  // 0 queued, 1 running, 2 done, 3 failed
+    std::string error;
+    std::vector<double> idd;
+};
+
+struct CacheEntry {
+    std::shared_ptr<PhysicsBase> physics;
+    size_t bytes = 0;
+};
+
+}  // namespace
+}  // namespace bfpn
+
+using namespace bfpn;
+
+struct bfpn_server {
+    bfpn_server_opts opts{};
+    std::mutex mu;
+    std::condition_variable cv_done;
+    std::condition_variable cv_work;
+    std::deque<std::shared_ptr<Request>> pending;
+    std::vector<std::shared_ptr<Request>> active;
+    std::map<uint64_t, std::shared_ptr<Request>> finished;
+    std::map<std::string, CacheEntry> cache;
+    std::list<std::string> lru;  // front = most recent
+    size_t used = 0;             // bytes of active contexts + cached physics
+    uint64_t next_id = 1;
+    bool stopping = false;
+    std::thread worker;
+    bfpn_server_stats stats{};
+
+    void touch(const std::string& key) {
+        lru.remove(key);
+        lru.push_front(key);
+    }
+
+    // Frees cached physics tables that no active beam uses, oldest first,
+    // until `need` more bytes fit.  Called with the lock held.
+    void evict_for(size_t need) {
+        for (auto it = lru.rbegin(); it != lru.rend() && used + need > opts.hbm_budget_bytes;) {
+            auto c = cache.find(*it);
+            if (c != cache.end() && c->second.physics.use_count() == 1) {
+                used -= c->second.bytes;
+                cache.erase(c);
+                it = std::list<std::string>::reverse_iterator(lru.erase(std::next(it).base()));
+            } else {
+                ++it;
+            }
+        }
+    }
+
+    // Admits queued requests while slots and memory allow.  Lock held on entry.
+    void admit(std::unique_lock<std::mutex>& lk) {
+        while (!pending.empty() && static_cast<int>(active.size()) < opts.max_active) {
+            auto req = pending.front();
+            const std::string key = physics_key(req->grid, req->phys, req->data, opts.precision);
+            const bool cached = cache.count(key) != 0;
+            const size_t phys_need = cached ? 0 : physics_bytes(req->grid, req->phys, opts.precision);
+            const size_t need = req->ctx_bytes + phys_need;
+            if (used + need > opts.hbm_budget_bytes) evict_for(need);
+            if (used + need > opts.hbm_budget_bytes) {
+                if (!active.empty()) return;  // wait for a running beam to finish
+                pending.pop_front();
+                req->status = 3;
+                req->error = "request does not fit the HBM budget (" + std::to_string(need >> 20) +
+                             " MiB > " + std::to_string((opts.hbm_budget_bytes - used) >> 20) + " MiB free)";
+                req->t_done = Clock::now();
+                finished[req->id] = req;
+                ++stats.failed;
+                cv_done.notify_all();
+                continue;
+            }
+            pending.pop_front();
+            used += need;
+            lk.unlock();
+            std::shared_ptr<PhysicsBase> physics;
+            size_t physics_real = 0;
+            std::string err;
+            try {
+                {
+                    std::lock_guard<std::mutex> g(mu);
+                    auto c = cache.find(key);
+                    if (c != cache.end()) physics = c->second.physics;
+                }
+                if (!physics) {
+                    physics = make_physics(req->grid, req->phys, req->data, opts.precision);
+                    physics_real = physics->device_bytes();
+                }
+                req->ctx.reset(make_context(physics, req->grid, req->phys));
+                req->physics = physics;
+                BFPN_HIP(hipEventCreateWithFlags(&req->slice_done, hipEventDisableTiming));
+            } catch (const std::exception& e) {
+                err = e.what();
+            }
+            lk.lock();
+            if (!err.empty()) {
+                used -= need;
+                req->ctx.reset();
+                req->status = 3;
+                req->error = err;
+                req->t_done = Clock::now();
+                finished[req->id] = req;
+                ++stats.failed;
+                cv_done.notify_all();
+                continue;
+            }
+            if (phys_need) {
+                if (!cache.count(key)) {  // keep the larger of estimate and real size
+                    const size_t charged = std::max(phys_need, physics_real);
+                    used = used - phys_need + charged;
+                    cache[key] = CacheEntry{physics, charged};
+                } else {
+                    used -= phys_need;  // another admission cached it meanwhile
+                }
+            }
+            touch(key);
+            req->status = 1;
+            req->t_admit = Clock::now();
+            active.push_back(req);
+            stats.peak_active = std::max(stats.peak_active, static_cast<int>(active.size()));
+        }
+    }
+
+    // Enqueues the next slice of a running request; returns true if work was issued.
+    bool issue(Request& r) {
+        const int n = std::min(opts.slice_steps, r.steps - r.issued);
+        if (n <= 0) return false;
+        r.ctx->advance(n);
+        BFPN_HIP(hipEventRecord(r.slice_done, r.ctx->stream));
+        r.issued += n;
+        r.in_flight = true;
+        return true;
+    }
+
+    void finish(const std::shared_ptr<Request>& r) {
+        std::vector<double> idd(r->steps);
+        int n = 0;
+        r->ctx->history(idd.data(), nullptr, r->steps, &n);
+        idd.resize(n);
+        const size_t bytes = r->ctx_bytes;
+        (void)hipEventDestroy(r->slice_done);
+        r->slice_done = nullptr;
+        r->ctx.reset();
+        r->physics.reset();
+        std::lock_guard<std::mutex> g(mu);
+        r->idd.swap(idd);
+        r->status = 2;
+        r->t_done = Clock::now();
+        used -= bytes;
+        ++stats.completed;
+        finished[r->id] = r;
+        cv_done.notify_all();
+    }
+
+    void fail(const std::shared_ptr<Request>& r, const std::string& msg) {
+        if (r->slice_done) (void)hipEventDestroy(r->slice_done);
+        r->slice_done = nullptr;
+        r->ctx.reset();
+        r->physics.reset();
+        std::lock_guard<std::mutex> g(mu);
+        r->status = 3;
+        r->error = msg;
+        r->t_done = Clock::now();
+        used -= r->ctx_bytes;
+        ++stats.failed;
+        finished[r->id] = r;
+        cv_done.notify_all();
+    }
+
+    void run() {
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            if (pending.empty() && active.empty()) {
+                if (stopping) break;
+                cv_work.wait(lk);
+                continue;
+            }
+            admit(lk);
+            auto running = active;  // snapshot; only this thread mutates `active`
+            lk.unlock();
+            bool progressed = false;
+            std::vector<std::shared_ptr<Request>> done;
+            for (auto& r : running) {
+                try {
+                    if (r->in_flight) {
+                        const hipError_t q = hipEventQuery(r->slice_done);
+                        if (q == hipErrorNotReady) continue;
+                        BFPN_HIP(q);
+                        r->in_flight = false;
+                    }
+                    if (r->issued >= r->steps) {
+                        done.push_back(r);
+                        continue;
+                    }
+                    progressed |= issue(*r);
+                } catch (const std::exception& e) {
+                    (void)hipGetLastError();
+                    fail(r, e.what());
+                    progressed = true;
+                }
+            }
+            for (auto& r : done) {
+                try {
+                    finish(r);
+                } catch (const std::exception& e) {
+                    fail(r, e.what());
+                }
+                progressed = true;
+            }
+            lk.lock();
+            active.erase(std::remove_if(active.begin(), active.end(),
+                                        [](const std::shared_ptr<Request>& r) { return r->status >= 2; }),
+                         active.end());
+            if (!progressed) {
+                lk.unlock();
+                std::this_thread::sleep_for(std::chrono::microseconds(50));
+                lk.lock();
+            }
+        }
+    }
+};
+
+extern "C" {
+
+BFPN_API bfpn_status bfpn_server_create(const bfpn_server_opts* opts, bfpn_server** out) {
+    return guarded([&] {
+        BFPN_REQUIRE(opts && out, "null argument");
+        BFPN_REQUIRE(opts->max_active >= 1 && opts->slice_steps >= 1, "max_active and slice_steps must be >= 1");
+        BFPN_REQUIRE(opts->precision == BFPN_FP32 || opts->precision == BFPN_FP64, "bad precision");
+        auto* s = new bfpn_server();
+        s->opts = *opts;
+        if (s->opts.hbm_budget_bytes == 0) {
+            size_t free_b = 0, total_b = 0;
+            BFPN_HIP(hipMemGetInfo(&free_b, &total_b));
+            s->opts.hbm_budget_bytes = free_b - free_b / 10;
+        }
+        s->worker = std::thread([s] { s->run(); });
+        *out = s;
+    });
+}
+
+BFPN_API bfpn_status bfpn_server_submit(bfpn_server* s, const bfpn_grid* grid, const bfpn_phys* phys,
+                                        const char* data_path, double t_final, uint64_t* id) {
+    return guarded([&] {
+        BFPN_REQUIRE(s && grid && phys && id, "null argument");
+        validate_params(*grid, *phys);
+        auto r = std::make_shared<Request>();
+        r->grid = *grid;
+        r->phys = *phys;
+        r->data = data_path ? data_path : "./data";
+        const double dt = grid->Lx / grid->Nx;
+        r->steps = static_cast<int>(std::ceil(t_final / dt - 1e-12));
+        BFPN_REQUIRE(r->steps >= 1, "t_final must cover at least one depth step");
+        r->ctx_bytes = context_bytes(*grid, *phys, s->opts.precision);
+        r->t_submit = Clock::now();
+        std::lock_guard<std::mutex> g(s->mu);
+        BFPN_REQUIRE(!s->stopping, "server is shutting down");
+        r->id = s->next_id++;
+        s->pending.push_back(r);
+        ++s->stats.submitted;
+        *id = r->id;
+        s->cv_work.notify_one();
+    });
+}
+
+BFPN_API int bfpn_server_poll(bfpn_server* s, uint64_t id) {
+    if (!s) return -1;
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->finished.find(id);
+    if (it != s->finished.end()) return it->second->status;
+    for (auto& r : s->active)
+        if (r->id == id) return 1;
+    for (auto& r : s->pending)
+        if (r->id == id) return 0;
+    return -1;
+}
+
+BFPN_API bfpn_status bfpn_server_wait(bfpn_server* s, uint64_t id, double timeout_s, double* idd,
+                                      int capacity, int* nsteps, double* times3) {
+    return guarded([&] {
+        BFPN_REQUIRE(s, "null server");
+        std::unique_lock<std::mutex> lk(s->mu);
+        const auto deadline = Clock::now() + std::chrono::duration_cast<Clock::duration>(
+                                                 std::chrono::duration<double>(timeout_s > 0 ? timeout_s : 1e9));
+        while (!s->finished.count(id)) {
+            if (s->cv_done.wait_until(lk, deadline) == std::cv_status::timeout && !s->finished.count(id))
+                throw Error(BFPN_E_INVALID, "timed out waiting for request " + std::to_string(id));
+        }
+        auto r = s->finished[id];
+        s->finished.erase(id);
+        if (times3) {
+            times3[0] = r->status == 2 ? seconds_between(r->t_submit, r->t_admit) : 0.0;
+            times3[1] = r->status == 2 ? seconds_between(r->t_admit, r->t_done) : 0.0;
+            times3[2] = seconds_between(r->t_submit, r->t_done);
+        }
+        if (r->status != 2) throw Error(BFPN_E_INVALID, "request " + std::to_string(id) + " failed: " + r->error);
+        const int n = std::min<int>(capacity, static_cast<int>(r->idd.size()));
+        if (idd && n > 0) std::copy(r->idd.begin(), r->idd.begin() + n, idd);
+        if (nsteps) *nsteps = static_cast<int>(r->idd.size());
+    });
+}
+
+BFPN_API bfpn_status bfpn_server_get_stats(bfpn_server* s, bfpn_server_stats* out) {
+    return guarded([&] {
+        BFPN_REQUIRE(s && out, "null argument");
+        std::lock_guard<std::mutex> g(s->mu);
+        *out = s->stats;
+        out->active = static_cast<int>(s->active.size());
+        out->queued = static_cast<int>(s->pending.size());
+        out->bytes_in_use = s->used;
+        out->cached_tables = static_cast<int>(s->cache.size());
+    });
+}
+
+BFPN_API bfpn_status bfpn_server_estimate(const bfpn_grid* grid, const bfpn_phys* phys,
+                                          int precision, size_t* ctx_bytes, size_t* phys_bytes) {
+    return guarded([&] {
+        BFPN_REQUIRE(grid && phys && ctx_bytes && phys_bytes, "null argument");
+        BFPN_REQUIRE(precision == BFPN_FP32 || precision == BFPN_FP64, "unknown precision");
+        validate_params(*grid, *phys);
+        *ctx_bytes = context_bytes(*grid, *phys, precision);
+        *phys_bytes = physics_bytes(*grid, *phys, precision);
+    });
+}
+
+BFPN_API void bfpn_server_destroy(bfpn_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stopping = true;
+        s->cv_work.notify_one();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    delete s;
+}
+
+}  // extern "C"

@@ -1,19 +1,21 @@
-// solver.hip — per-beam context, the Strang depth march and the C ABI.
+// solver.hip — shared physics tables, per-beam contexts, the Strang depth
+// march, and the context/operator C ABI.
 //
 // Reference orchestration: BFPnSolver (src/bfpm_solver.cu:1171-1568,
 // 2166-2267, 3238-3502).  Its loop body per depth step is
 //   stepPrimary:   E(dt/2) -> T(dt/2) -> A(dt) -> T(dt/2) -> E(dt/2)     on (F, f)
 //   stepSecondary: the same order on (F1, f1) with the source from the primary
 //                  (only when step > 0 and the secondary is enabled)
-//   IDD (Eq.24) of F+F1, f+f1 and the proxy dg*sum(f), recorded at x=(k+1)dt.
+//   IDD (Eq.24) of F+F1, f+f1 and the proxy dg*sum(f), recorded at x=(k+1)dt,
 // with a host sync after every operator and two scalar D2H copies per step.
 //
-// Here the whole step is stream-ordered on one HIP stream with no host sync:
-// transport updates each state array in place, the IDD and
-// proxy are accumulated inside the final energy half steps and reduced into
-// a device history array, and the single source evaluation per step feeds
-// both secondary energy half steps (the primary is unchanged in between, so
-// the reference's two evaluations are identical).
+// Here the whole step is stream-ordered on the context's HIP stream with no
+// host sync: transport and angular diffusion update each state array in
+// place, the IDD and proxy are accumulated inside the final energy half steps
+// and reduced into a device history array, and one source evaluation per step
+// feeds both secondary energy half steps (the primary is unchanged in
+// between, so the reference's two evaluations are identical).  Tables that do
+// not depend on the beam live in a Physics object shared between contexts.
 
 #include <hip/hip_runtime.h>
 
@@ -21,11 +23,14 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <sstream>
 #include <string>
 #include <type_traits>
 #include <vector>
 
+#include "device.hpp"
 #include "kernels.hpp"
+#include "solver.hpp"
 #include "tables.hpp"
 
 namespace bfpn {
@@ -35,61 +40,6 @@ thread_local std::string g_last_error;
 }
 
 void set_last_error(const std::string& msg) { g_last_error = msg; }
-
-// ------------------------------------------------------------ device memory
-class DeviceBuffer {
-public:
-    DeviceBuffer() = default;
-    DeviceBuffer(const DeviceBuffer&) = delete;
-    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-    ~DeviceBuffer() { release(); }
-    void release() {
-        if (ptr_) (void)hipFree(ptr_);
-        ptr_ = nullptr;
-        bytes_ = 0;
-    }
-    void ensure(size_t bytes) {
-        if (bytes <= bytes_) return;
-        release();
-        if (bytes == 0) return;
-        hipError_t e = hipMalloc(&ptr_, bytes);
-        if (e != hipSuccess) {
-            ptr_ = nullptr;
-            (void)hipGetLastError();
-            throw Error(BFPN_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
-                                          hipGetErrorString(e));
-        }
-        bytes_ = bytes;
-    }
-    template <typename T>
-    T* as() const {
-        return static_cast<T*>(ptr_);
-    }
-    size_t bytes() const { return bytes_; }
-    void swap(DeviceBuffer& o) {
-        std::swap(ptr_, o.ptr_);
-        std::swap(bytes_, o.bytes_);
-    }
-
-private:
-    void* ptr_ = nullptr;
-    size_t bytes_ = 0;
-};
-
-// Host -> device copy of a (usually temporary) host vector.  The copy is
-// complete when this returns; callers order it against `s` explicitly.
-template <typename T>
-void upload(DeviceBuffer& b, const std::vector<T>& host, hipStream_t s) {
-    (void)s;
-    b.ensure(host.size() * sizeof(T));
-    if (!host.empty())
-        BFPN_HIP(hipMemcpy(b.as<void>(), host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
-}
-
-template <typename T>
-std::vector<T> cast_vector(const std::vector<double>& v) {
-    return std::vector<T>(v.begin(), v.end());
-}
 
 std::vector<double> yz_weights(int Ny, int Nz, bool trapezoidal) {
     std::vector<double> w(static_cast<size_t>(Ny + 1) * (Nz + 1), 1.0);
@@ -104,194 +54,191 @@ std::vector<double> yz_weights(int Ny, int Nz, bool trapezoidal) {
     return w;
 }
 
-// Device storage of a SourcePlan (FP32 source on the matrix cores).
-struct SourcePlanBuffers {
-    DeviceBuffer kth, ktl, e1h, e1l, e2h, e2l, band, scratch;
-    SourcePlan plan;
-
-    // kv_f32: device FP32 Kv [Ng][nang][nang]; e1/e2/beg/end: host tables.
-    void build(const float* kv_f32, const std::vector<double>& e1, const std::vector<double>& e2,
-               const std::vector<int>& beg, const std::vector<int>& end, int Ng, int nang,
-               double dg, hipStream_t s) {
-        const int nap = source_nap(nang), ngp = source_ngp(Ng), erows = source_erows(Ng);
-        const size_t kt = static_cast<size_t>(Ng) * nap * nap;
-        kth.ensure(kt * sizeof(__bf16));
-        ktl.ensure(kt * sizeof(__bf16));
-        launch_split_transpose(kv_f32, kth.as<__bf16>(), ktl.as<__bf16>(), Ng, nang, s);
-        const size_t ne = static_cast<size_t>(erows) * ngp;
-        std::vector<float> E1(ne, 0.f), E2(ne, 0.f);
-        for (int r = 0; r < Ng; ++r)
-            for (int c = 0; c < Ng; ++c) {
-                E1[static_cast<size_t>(r) * ngp + c] = static_cast<float>(dg * e1[static_cast<size_t>(r) * Ng + c]);
-                E2[static_cast<size_t>(r) * ngp + c] =
-                    static_cast<float>(dg * e2[static_cast<size_t>(r) * Ng + c] / 3.0);
-            }
-        for (DeviceBuffer* b : {&e1h, &e1l, &e2h, &e2l}) b->ensure(ne * sizeof(__bf16));
-        upload(scratch, E1, s);
-        launch_split(scratch.as<float>(), e1h.as<__bf16>(), e1l.as<__bf16>(), static_cast<long long>(ne), s);
-        BFPN_HIP(hipStreamSynchronize(s));
-        upload(scratch, E2, s);
-        launch_split(scratch.as<float>(), e2h.as<__bf16>(), e2l.as<__bf16>(), static_cast<long long>(ne), s);
-        std::vector<int2> tiles(erows / 16);
-        for (size_t t = 0; t < tiles.size(); ++t) {
-            int lo = 1 << 30, hi = 0;
-            for (int r = static_cast<int>(t) * 16; r < static_cast<int>(t) * 16 + 16 && r < Ng; ++r)
-                if (end[r] > beg[r]) {
-                    lo = std::min(lo, beg[r]);
-                    hi = std::max(hi, end[r]);
-                }
-            tiles[t] = hi > 0 ? make_int2(lo, hi) : make_int2(0, 0);
+void SourcePlanBuffers::build(const float* kv_f32, const std::vector<double>& e1,
+                              const std::vector<double>& e2, const std::vector<int>& beg,
+                              const std::vector<int>& end, int Ng, int nang, double dg,
+                              hipStream_t s) {
+    const int nap = source_nap(nang), ngp = source_ngp(Ng), erows = source_erows(Ng);
+    const size_t kt = static_cast<size_t>(Ng) * nap * nap;
+    kth.ensure(kt * sizeof(__bf16));
+    ktl.ensure(kt * sizeof(__bf16));
+    launch_split_transpose(kv_f32, kth.as<__bf16>(), ktl.as<__bf16>(), Ng, nang, s);
+    const size_t ne = static_cast<size_t>(erows) * ngp;
+    std::vector<float> E1(ne, 0.f), E2(ne, 0.f);
+    for (int r = 0; r < Ng; ++r)
+        for (int c = 0; c < Ng; ++c) {
+            const size_t src = static_cast<size_t>(r) * Ng + c, dst = static_cast<size_t>(r) * ngp + c;
+            E1[dst] = static_cast<float>(dg * e1[src]);
+            E2[dst] = static_cast<float>(dg * e2[src] / 3.0);
         }
-        upload(band, tiles, s);
-        BFPN_HIP(hipStreamSynchronize(s));
-        scratch.release();
-        plan.kth = kth.as<__bf16>();
-        plan.ktl = ktl.as<__bf16>();
-        plan.e1h = e1h.as<__bf16>();
-        plan.e1l = e1l.as<__bf16>();
-        plan.e2h = e2h.as<__bf16>();
-        plan.e2l = e2l.as<__bf16>();
-        plan.tile_band = band.as<int2>();
+    for (DeviceBuffer* b : {&e1h, &e1l, &e2h, &e2l}) b->ensure(ne * sizeof(__bf16));
+    upload(scratch, E1);
+    launch_split(scratch.as<float>(), e1h.as<__bf16>(), e1l.as<__bf16>(), static_cast<long long>(ne), s);
+    BFPN_HIP(hipStreamSynchronize(s));
+    upload(scratch, E2);
+    launch_split(scratch.as<float>(), e2h.as<__bf16>(), e2l.as<__bf16>(), static_cast<long long>(ne), s);
+    std::vector<int2> tiles(erows / 16);
+    for (size_t t = 0; t < tiles.size(); ++t) {
+        int lo = 1 << 30, hi = 0;
+        for (int r = static_cast<int>(t) * 16; r < static_cast<int>(t) * 16 + 16 && r < Ng; ++r)
+            if (end[r] > beg[r]) {
+                lo = std::min(lo, beg[r]);
+                hi = std::max(hi, end[r]);
+            }
+        tiles[t] = hi > 0 ? make_int2(lo, hi) : make_int2(0, 0);
     }
-    size_t bytes() const {
-        return kth.bytes() + ktl.bytes() + e1h.bytes() + e1l.bytes() + e2h.bytes() + e2l.bytes() +
-               band.bytes();
-    }
-};
+    upload(band, tiles);
+    BFPN_HIP(hipStreamSynchronize(s));
+    scratch.release();
+    plan.kth = kth.as<__bf16>();
+    plan.ktl = ktl.as<__bf16>();
+    plan.e1h = e1h.as<__bf16>();
+    plan.e1l = e1l.as<__bf16>();
+    plan.e2h = e2h.as<__bf16>();
+    plan.e2l = e2l.as<__bf16>();
+    plan.tile_band = band.as<int2>();
+}
 
-// Split-BF16 sine bases for the MFMA DST (FP32 state, Nmu, Nom <= 32).
-struct AnglePlanBuffers {
-    DeviceBuffer h, l, scratch;
-    AnglePlan plan;
-    static bool usable(int Nmu, int Nom) { return Nmu <= 32 && Nom <= 32; }
-    void build(int Nmu, int Nom, hipStream_t s) {
-        std::vector<float> pad(2 * 32 * 32, 0.f);  // Su then Sv, zero padded
-        const auto su = sine_basis(Nom), sv = sine_basis(Nmu);
-        for (int r = 0; r < Nom; ++r)
-            for (int c = 0; c < Nom; ++c) pad[r * 32 + c] = static_cast<float>(su[r * Nom + c]);
-        for (int r = 0; r < Nmu; ++r)
-            for (int c = 0; c < Nmu; ++c) pad[1024 + r * 32 + c] = static_cast<float>(sv[r * Nmu + c]);
-        h.ensure(pad.size() * sizeof(__bf16));
-        l.ensure(pad.size() * sizeof(__bf16));
-        upload(scratch, pad, s);
-        launch_split(scratch.as<float>(), h.as<__bf16>(), l.as<__bf16>(), static_cast<long long>(pad.size()), s);
-        BFPN_HIP(hipStreamSynchronize(s));
-        scratch.release();
-        plan.su_h = h.as<__bf16>();
-        plan.su_l = l.as<__bf16>();
-        plan.sv_h = h.as<__bf16>() + 1024;
-        plan.sv_l = l.as<__bf16>() + 1024;
-    }
-};
+void AnglePlanBuffers::build(int Nmu, int Nom, hipStream_t s) {
+    std::vector<float> pad(2 * 32 * 32, 0.f);  // Su then Sv, zero padded to 32x32
+    const auto su = sine_basis(Nom), sv = sine_basis(Nmu);
+    for (int r = 0; r < Nom; ++r)
+        for (int c = 0; c < Nom; ++c) pad[r * 32 + c] = static_cast<float>(su[r * Nom + c]);
+    for (int r = 0; r < Nmu; ++r)
+        for (int c = 0; c < Nmu; ++c) pad[1024 + r * 32 + c] = static_cast<float>(sv[r * Nmu + c]);
+    h.ensure(pad.size() * sizeof(__bf16));
+    l.ensure(pad.size() * sizeof(__bf16));
+    upload(scratch, pad);
+    launch_split(scratch.as<float>(), h.as<__bf16>(), l.as<__bf16>(), static_cast<long long>(pad.size()), s);
+    BFPN_HIP(hipStreamSynchronize(s));
+    scratch.release();
+    plan.su_h = h.as<__bf16>();
+    plan.su_l = l.as<__bf16>();
+    plan.sv_h = h.as<__bf16>() + 1024;
+    plan.sv_l = l.as<__bf16>() + 1024;
+}
 
-}  // namespace bfpn
+void validate_params(const bfpn_grid& g, const bfpn_phys& p) {
+    BFPN_REQUIRE(g.Nx > 0 && g.Ny > 0 && g.Nz > 0 && g.Ng > 1 && g.Nmu > 0 && g.Nom > 0,
+                 "grid sizes must be positive (Ng > 1)");
+    BFPN_REQUIRE(g.Nmu * g.Nom <= 1024, "Nmu*Nom must be <= 1024");
+    BFPN_REQUIRE(g.Lx > 0 && g.Ly > 0 && g.Lz > 0 && g.Lu > 0 && g.Lv > 0 && g.Lg > 0,
+                 "domain lengths must be positive");
+    BFPN_REQUIRE(p.density > 0 && p.energy_density > 0, "density must be positive");
+    BFPN_REQUIRE(p.sig_E > 0, "sigma-e must be positive");
+}
 
-using namespace bfpn;
-
-// ============================================================== context ===
-struct bfpn_ctx {
-    virtual ~bfpn_ctx() = default;
-    virtual void advance(int nsteps) = 0;
-    virtual void history(double* idd, double* proxy, int cap, int* n) = 0;
-    virtual void read_state(int which, double* host, size_t count) = 0;
-    virtual void write_state(int which, const double* host, size_t count) = 0;
-    virtual size_t device_bytes() const = 0;
-
-    bfpn_grid grid{};
-    bfpn_phys phys{};
-    bfpn_deltas d{};
-    int nyz = 0, nang = 0;
-    long long lanes = 0;
-    size_t cells = 0;
-    hipStream_t stream = nullptr;
-    bool profiling = false;
-    double phase_seconds[5] = {0, 0, 0, 0, 0};
-    double phase_bytes[5] = {0, 0, 0, 0, 0};
-    int phase_calls[5] = {0, 0, 0, 0, 0};
-    // host copies of the tables (FP64)
-    MaterialTables mat;
-    std::vector<double> S, Tc, sig_tr, en, om;
-    SourceKernels ker;
-};
+std::string physics_key(const bfpn_grid& g, const bfpn_phys& p, const std::string& data_path,
+                        int precision) {
+    std::ostringstream k;
+    k.precision(17);
+    k << precision << '|' << data_path << '|' << g.Nx << ',' << g.Ny << ',' << g.Nz << ',' << g.Ng
+      << ',' << g.Nmu << ',' << g.Nom << '|' << g.Lx << ',' << g.Ly << ',' << g.Lz << ',' << g.Lu
+      << ',' << g.Lv << ',' << g.Lg << '|' << p.density << ',' << p.energy_density << '|'
+      << p.material_name << '|' << p.primary_only << p.eq15_straggling << p.energy_only
+      << p.trapezoidal_yz << p.dose_catastrophic_loss << p.calibrate_cross_sections;
+    return k.str();
+}
 
 namespace {
 
 enum Phase { kEnergy = 0, kTransport = 1, kAngle = 2, kSource = 3, kDose = 4 };
 
 template <typename T>
-class Context final : public bfpn_ctx {
-public:
-    Context(const bfpn_grid& g, const bfpn_phys& p, const std::string& data_path) {
+struct Physics final : PhysicsBase {
+    DeviceBuffer coef, dose_w, yzw, om_dev, factors, su, sv, kv, e1, e2, beg, end;
+    SourcePlanBuffers plan;
+    AnglePlanBuffers angle_plan;
+
+    Physics(const bfpn_grid& g, const bfpn_phys& p, const std::string& data_path, int prec) {
+        precision = prec;
         grid = g;
-        phys = p;
-        validate();
         d = compute_deltas(g);
         nyz = (g.Ny + 1) * (g.Nz + 1);
         nang = g.Nmu * g.Nom;
         lanes = static_cast<long long>(nyz) * nang;
         cells = static_cast<size_t>(g.Ng) * lanes;
-        secondary_ = !p.primary_only && !p.energy_only;
+        secondary = !p.primary_only && !p.energy_only;
+        key = physics_key(g, p, data_path, prec);
 
-        // ---- host tables (initialize, src/bfpm_solver.cu:1435-1568)
+        // host tables (initialize, src/bfpm_solver.cu:1435-1568)
         mat = load_material_tables(data_path, g.Ng, p.calibrate_cross_sections != 0);
-        const std::string material = p.material_name;
-        S = stopping_power_table(material, g.Ng, g.Lg, d.dg);
+        S = stopping_power_table(p.material_name, g.Ng, g.Lg, d.dg);
         Tc = straggling_table(g.Ng);
         sig_tr = transport_xs_table(g.Ng, d.dg);
         en = energy_grid(g.Ng, d.dg);
         om = direction_table(g, d);
         ker = build_source_kernels(g.Ng, d.dg, nang, mat.mu, mat.sigma, om);
-        const BeamFactors beam = beam_factors(g, p, d, om);
 
-        preflight();
+        hipStream_t s = nullptr;
+        BFPN_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        try {
+            upload(coef, energy_coefs(S, mat.sigma, Tc.data(), p.eq15_straggling != 0, 0.5 * d.dt,
+                                      d.dg, p.energy_density, g.Ng));
+            upload(dose_w, dose_coefs(S, mat.sigma, en, d.dg, p.dose_catastrophic_loss != 0, g.Ng));
+            upload(yzw, yz_weights(g.Ny, g.Nz, p.trapezoidal_yz != 0));
+            upload(om_dev, cast_vector<T>(om));
+            upload(factors, cast_vector<T>(angle_factors(sig_tr.data(), g.Ng, g.Nmu, g.Nom, d.du,
+                                                         d.dv, d.dt, p.density)));
+            upload(su, cast_vector<T>(sine_basis(g.Nom)));
+            upload(sv, cast_vector<T>(sine_basis(g.Nmu)));
+            if (std::is_same_v<T, float> && AnglePlanBuffers::usable(g.Nmu, g.Nom))
+                angle_plan.build(g.Nmu, g.Nom, s);
+            if (secondary) {
+                upload(kv, cast_vector<T>(ker.v));
+                if constexpr (std::is_same_v<T, float>) {
+                    plan.build(kv.as<float>(), ker.e1, ker.e2, ker.beg, ker.end, g.Ng, nang, d.dg, s);
+                    kv.release();
+                } else {
+                    upload(e1, ker.e1);
+                    upload(e2, ker.e2);
+                    upload(beg, ker.beg);
+                    upload(end, ker.end);
+                }
+            }
+            BFPN_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            (void)hipStreamDestroy(s);
+            throw;
+        }
+        BFPN_HIP(hipStreamDestroy(s));
+    }
+
+    size_t device_bytes() const override {
+        return coef.bytes() + dose_w.bytes() + yzw.bytes() + om_dev.bytes() + factors.bytes() +
+               su.bytes() + sv.bytes() + kv.bytes() + e1.bytes() + e2.bytes() + beg.bytes() +
+               end.bytes() + plan.bytes() + angle_plan.h.bytes() + angle_plan.l.bytes();
+    }
+};
+
+template <typename T>
+class Context final : public bfpn_ctx {
+public:
+    Context(std::shared_ptr<Physics<T>> P, const bfpn_grid& g, const bfpn_phys& p) : P_(std::move(P)) {
+        grid = g;
+        phys = p;
+        physics = P_;
+        d = P_->d;
+        cells = P_->cells;
+        const BeamFactors beam = beam_factors(g, p, d, P_->om);
         BFPN_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-
         const size_t state_bytes = cells * sizeof(T);
         for (auto& b : state_) b.ensure(state_bytes);
-        if (secondary_) {
+        if (P_->secondary) {
             a1_.ensure(state_bytes);
             a2_.ensure(state_bytes);
             q_.ensure(state_bytes);
         }
         for (int k = 0; k < 4; ++k) cur_[k] = state_[k].as<T>();
-
-        upload(coef_, energy_coefs(S, mat.sigma, Tc.data(), p.eq15_straggling != 0, 0.5 * d.dt, d.dg,
-                                   p.energy_density, g.Ng),
-               stream);
-        upload(dose_w_, dose_coefs(S, mat.sigma, en, d.dg, p.dose_catastrophic_loss != 0, g.Ng),
-               stream);
-        upload(yzw_, yz_weights(g.Ny, g.Nz, p.trapezoidal_yz != 0), stream);
-        upload(om_, cast_vector<T>(om), stream);
-        upload(factors_, cast_vector<T>(angle_factors(sig_tr.data(), g.Ng, g.Nmu, g.Nom, d.du, d.dv,
-                                                      d.dt, p.density)),
-               stream);
-        upload(su_, cast_vector<T>(sine_basis(g.Nom)), stream);
-        upload(sv_, cast_vector<T>(sine_basis(g.Nmu)), stream);
-        if (std::is_same_v<T, float> && AnglePlanBuffers::usable(g.Nmu, g.Nom))
-            angle_plan_.build(g.Nmu, g.Nom, stream);
-        if (secondary_) {
-            upload(kv_, cast_vector<T>(ker.v), stream);
-            if constexpr (std::is_same_v<T, float>) {
-                plan_.build(kv_.as<float>(), ker.e1, ker.e2, ker.beg, ker.end, g.Ng, nang, d.dg,
-                            stream);
-                kv_.release();
-            } else {
-                upload(e1_, ker.e1, stream);
-                upload(e2_, ker.e2, stream);
-                upload(beg_, ker.beg, stream);
-                upload(end_, ker.end, stream);
-            }
-        }
-        const int eb = energy_blocks(lanes);
+        const int eb = energy_blocks(P_->lanes);
         partial_p_.ensure(2 * eb * sizeof(double));
         partial_s_.ensure(eb * sizeof(double));
         grow_history(std::max(16, g.Nx));
-
         // Eq.23 beam on the device (the reference builds a full host array)
         DeviceBuffer sa, ee;
-        upload(sa, beam.space_angle, stream);
-        upload(ee, beam.energy, stream);
-        launch_beam<T>(cur_[0], sa.as<double>(), ee.as<double>(), g.Ng, lanes, stream);
+        upload(sa, beam.space_angle);
+        upload(ee, beam.energy);
+        launch_beam<T>(cur_[0], sa.as<double>(), ee.as<double>(), g.Ng, P_->lanes, stream);
         for (int k = 1; k < 4; ++k) BFPN_HIP(hipMemsetAsync(cur_[k], 0, state_bytes, stream));
         BFPN_HIP(hipStreamSynchronize(stream));
     }
@@ -304,11 +251,13 @@ public:
     }
 
     size_t device_bytes() const override {
-        size_t b = a1_.bytes() + a2_.bytes() + q_.bytes() + kv_.bytes() + e1_.bytes() +
-                   e2_.bytes() + hist_.bytes() + plan_.bytes();
+        size_t b = a1_.bytes() + a2_.bytes() + q_.bytes() + hist_.bytes() + partial_p_.bytes() +
+                   partial_s_.bytes();
         for (const auto& s : state_) b += s.bytes();
         return b;
     }
+
+    int steps_done() const override { return step_; }
 
     void advance(int nsteps) override {
         BFPN_REQUIRE(nsteps >= 0, "nsteps must be >= 0");
@@ -323,13 +272,11 @@ public:
         BFPN_HIP(hipStreamSynchronize(stream));
         const int cnt = std::min(cap, step_);
         if (cnt > 0) {
-            std::vector<double> h(2 * static_cast<size_t>(hist_cap_));
-            BFPN_HIP(hipMemcpy(h.data(), hist_.as<double>(), h.size() * sizeof(double),
-                               hipMemcpyDeviceToHost));
-            for (int k = 0; k < cnt; ++k) {
-                if (idd) idd[k] = h[k];
-                if (proxy) proxy[k] = h[hist_cap_ + k];
-            }
+            if (idd)
+                BFPN_HIP(hipMemcpy(idd, hist_.as<double>(), cnt * sizeof(double), hipMemcpyDeviceToHost));
+            if (proxy)
+                BFPN_HIP(hipMemcpy(proxy, hist_.as<double>() + hist_cap_, cnt * sizeof(double),
+                                   hipMemcpyDeviceToHost));
         }
         if (n) *n = step_;
     }
@@ -352,35 +299,6 @@ public:
     }
 
 private:
-    void validate() const {
-        const auto& g = grid;
-        BFPN_REQUIRE(g.Nx > 0 && g.Ny > 0 && g.Nz > 0 && g.Ng > 1 && g.Nmu > 0 && g.Nom > 0,
-                     "grid sizes must be positive (Ng > 1)");
-        BFPN_REQUIRE(g.Nmu * g.Nom <= 1024, "Nmu*Nom must be <= 1024");
-        BFPN_REQUIRE(g.Lx > 0 && g.Ly > 0 && g.Lz > 0 && g.Lu > 0 && g.Lv > 0 && g.Lg > 0,
-                     "domain lengths must be positive");
-        BFPN_REQUIRE(phys.density > 0 && phys.energy_density > 0, "density must be positive");
-        BFPN_REQUIRE(phys.sig_E > 0, "sigma-e must be positive");
-    }
-
-    // Memory preflight (preflightMemory, src/bfpm_solver.cu:1321-1433), for
-    // this layout: 4 state arrays (+3 source arrays with the secondary) and Kv.
-    void preflight() const {
-        size_t free_b = 0, total_b = 0;
-        BFPN_HIP(hipMemGetInfo(&free_b, &total_b));
-        const size_t arrays = secondary_ ? 7 : 4;
-        const size_t need = arrays * cells * sizeof(T) +
-                            (secondary_ ? static_cast<size_t>(grid.Ng) * nang * nang * sizeof(T) : 0) +
-                            (64ull << 20);
-        if (need > free_b)
-            throw Error(BFPN_E_NOMEM, "Requested grid needs about " +
-                                          std::to_string(need >> 20) + " MiB of device memory, but only " +
-                                          std::to_string(free_b >> 20) + " MiB is free (Ny=" +
-                                          std::to_string(grid.Ny) + ", Nz=" + std::to_string(grid.Nz) +
-                                          ", Ng=" + std::to_string(grid.Ng) + ", Nmu=" +
-                                          std::to_string(grid.Nmu) + ", Nom=" + std::to_string(grid.Nom) + ")");
-    }
-
     void grow_history(int cap) {
         DeviceBuffer nb;
         nb.ensure(2 * static_cast<size_t>(cap) * sizeof(double));
@@ -422,40 +340,54 @@ private:
     double array_bytes() const { return static_cast<double>(cells) * sizeof(T); }
 
     void energy(int species, bool with_source, double* partials, bool proxy) {
-        T* F = cur_[2 * species];
-        T* f = cur_[2 * species + 1];
-        launch_energy_half<T>(F, f, coef_.as<EnergyCoef>(), with_source ? q_.as<T>() : nullptr,
-                              grid.Ng, lanes, phys.eq15_straggling != 0, dose_w_.as<DoseCoef>(),
-                              yzw_.as<double>(), nang, partials, proxy, stream);
+        launch_energy_half<T>(cur_[2 * species], cur_[2 * species + 1], P_->coef.template as<EnergyCoef>(),
+                              with_source ? q_.as<T>() : nullptr, grid.Ng, P_->lanes,
+                              phys.eq15_straggling != 0, P_->dose_w.template as<DoseCoef>(),
+                              P_->yzw.template as<double>(), P_->nang, partials, proxy, stream);
         mark(kEnergy, array_bytes() * (with_source ? 5 : 4));
     }
 
     void transport(int which, bool preserve_sign) {
-        launch_transport_half<T>(cur_[which], om_.as<T>(), grid.Ny, grid.Nz, grid.Ng, nang, d.dy,
-                                 d.dz, 0.5 * d.dt, preserve_sign, stream);
+        launch_transport_half<T>(cur_[which], P_->om_dev.template as<T>(), grid.Ny, grid.Nz, grid.Ng,
+                                 P_->nang, d.dy, d.dz, 0.5 * d.dt, preserve_sign, stream);
         mark(kTransport, array_bytes() * 2);
     }
 
     void angle(int which) {
         if constexpr (std::is_same_v<T, float>) {
-            if (angle_plan_.plan.su_h) {
-                launch_angle_mfma(cur_[which], factors_.as<float>(), angle_plan_.plan, grid.Ng, nyz,
-                                  grid.Nmu, grid.Nom, stream);
+            if (P_->angle_plan.plan.su_h) {
+                launch_angle_mfma(cur_[which], P_->factors.template as<float>(), P_->angle_plan.plan,
+                                  grid.Ng, P_->nyz, grid.Nmu, grid.Nom, stream);
                 mark(kAngle, array_bytes() * 2);
                 return;
             }
         }
-        launch_angle_full<T>(cur_[which], factors_.as<T>(), su_.as<T>(), sv_.as<T>(), grid.Ng, nyz,
-                             grid.Nmu, grid.Nom, stream);
+        launch_angle_full<T>(cur_[which], P_->factors.template as<T>(), P_->su.template as<T>(),
+                             P_->sv.template as<T>(), grid.Ng, P_->nyz, grid.Nmu, grid.Nom, stream);
         mark(kAngle, array_bytes() * 2);
+    }
+
+    void source() {
+        if constexpr (std::is_same_v<T, float>) {
+            launch_source_mfma(cur_[0], cur_[1], a1_.as<float>(), a2_.as<float>(), q_.as<float>(),
+                               P_->plan.plan, grid.Ng, P_->nyz, P_->nang, stream);
+        } else {
+            launch_source_angular<T>(cur_[0], cur_[1], P_->kv.template as<T>(), a1_.as<T>(),
+                                     a2_.as<T>(), grid.Ng, P_->nyz, P_->nang, stream);
+            launch_source_energy<T>(a1_.as<T>(), a2_.as<T>(), P_->e1.template as<double>(),
+                                    P_->e2.template as<double>(), P_->beg.template as<int>(),
+                                    P_->end.template as<int>(), q_.as<T>(), grid.Ng, P_->lanes, d.dg,
+                                    stream);
+        }
+        mark(kSource, array_bytes() * 7);  // 2 in, 2 out; 2 in, 1 out
     }
 
     // One depth step (BFPnSolver::solve loop body, src/bfpm_solver.cu:2213-2242).
     void step() {
-        const bool primary_clip_preserve = phys.no_spatial_clipping != 0;
+        const bool preserve_primary = phys.no_spatial_clipping != 0;
         double* pp = partial_p_.as<double>();
         double* ps = partial_s_.as<double>();
-        const int eb = energy_blocks(lanes);
+        const int eb = energy_blocks(P_->lanes);
         mark(kDose);  // opens the first interval of the step
         if (phys.energy_only) {  // stepPrimaryEnergyOnly, src/bfpm_solver.cu:3313-3333
             energy(0, false, nullptr, false);
@@ -463,7 +395,7 @@ private:
         } else {  // stepPrimary, src/bfpm_solver.cu:3238-3311
             energy(0, false, nullptr, false);
             if (!phys.no_transport) {
-                transport(0, primary_clip_preserve);
+                transport(0, preserve_primary);
                 transport(1, true);
             }
             if (!phys.no_angle) {
@@ -471,26 +403,16 @@ private:
                 angle(1);
             }
             if (!phys.no_transport) {
-                transport(0, primary_clip_preserve);
+                transport(0, preserve_primary);
                 transport(1, true);
             }
             energy(0, false, pp, true);
         }
-        const bool sec = secondary_ && step_ > 0;
+        const bool sec = P_->secondary && step_ > 0;
         if (sec) {  // stepSecondary, src/bfpm_solver.cu:3335-3394
-            if constexpr (std::is_same_v<T, float>) {
-                launch_source_mfma(cur_[0], cur_[1], a1_.as<float>(), a2_.as<float>(),
-                                   q_.as<float>(), plan_.plan, grid.Ng, nyz, nang, stream);
-            } else {
-                launch_source_angular<T>(cur_[0], cur_[1], kv_.as<T>(), a1_.as<T>(), a2_.as<T>(),
-                                         grid.Ng, nyz, nang, stream);
-                launch_source_energy<T>(a1_.as<T>(), a2_.as<T>(), e1_.as<double>(),
-                                        e2_.as<double>(), beg_.as<int>(), end_.as<int>(),
-                                        q_.as<T>(), grid.Ng, lanes, d.dg, stream);
-            }
-            mark(kSource, array_bytes() * 7);  // 2 in, 2 out; 2 in, 1 out
+            source();
             energy(1, true, nullptr, false);
-            transport(2, false);  // secondary psi^1 always clipped
+            transport(2, false);  // secondary psi^1 is always clipped
             transport(3, true);
             angle(2);
             angle(3);
@@ -506,11 +428,8 @@ private:
         ++step_;
     }
 
-    bool secondary_ = true;
+    std::shared_ptr<Physics<T>> P_;
     DeviceBuffer state_[4], a1_, a2_, q_;
-    DeviceBuffer coef_, dose_w_, yzw_, om_, factors_, su_, sv_, e1_, e2_, beg_, end_, kv_;
-    SourcePlanBuffers plan_;
-    AnglePlanBuffers angle_plan_;
     DeviceBuffer partial_p_, partial_s_, hist_;
     T* cur_[4] = {nullptr, nullptr, nullptr, nullptr};
     int step_ = 0;
@@ -519,6 +438,46 @@ private:
 };
 
 }  // namespace
+
+std::shared_ptr<PhysicsBase> make_physics(const bfpn_grid& g, const bfpn_phys& p,
+                                          const std::string& data_path, int precision) {
+    validate_params(g, p);
+    if (precision == BFPN_FP32) return std::make_shared<Physics<float>>(g, p, data_path, precision);
+    if (precision == BFPN_FP64) return std::make_shared<Physics<double>>(g, p, data_path, precision);
+    throw Error(BFPN_E_INVALID, "precision must be BFPN_FP32 or BFPN_FP64");
+}
+
+bfpn_ctx* make_context(std::shared_ptr<PhysicsBase> physics, const bfpn_grid& g,
+                       const bfpn_phys& p) {
+    validate_params(g, p);
+    if (physics->precision == BFPN_FP32)
+        return new Context<float>(std::static_pointer_cast<Physics<float>>(physics), g, p);
+    return new Context<double>(std::static_pointer_cast<Physics<double>>(physics), g, p);
+}
+
+size_t context_bytes(const bfpn_grid& g, const bfpn_phys& p, int precision) {
+    const size_t lanes = static_cast<size_t>(g.Ny + 1) * (g.Nz + 1) * g.Nmu * g.Nom;
+    const size_t cells = lanes * g.Ng;
+    const size_t word = precision == BFPN_FP64 ? 8 : 4;
+    const bool sec = !p.primary_only && !p.energy_only;
+    return cells * word * (sec ? 7 : 4) + 3 * ((lanes + 255) / 256) * 8 +
+           2 * static_cast<size_t>(std::max(16, g.Nx)) * 8;
+}
+
+size_t physics_bytes(const bfpn_grid& g, const bfpn_phys& p, int precision) {
+    const size_t nang = static_cast<size_t>(g.Nmu) * g.Nom;
+    const bool sec = !p.primary_only && !p.energy_only;
+    if (!sec) return static_cast<size_t>(g.Ng) * nang * 8 + (1 << 20);
+    if (precision == BFPN_FP64) return static_cast<size_t>(g.Ng) * nang * nang * 8 + (1 << 22);
+    const size_t nap = source_nap(static_cast<int>(nang));
+    return static_cast<size_t>(g.Ng) * nap * nap * 4 +                           // split Kv
+           4 * static_cast<size_t>(source_erows(g.Ng)) * source_ngp(g.Ng) * 2 +  // split E
+           static_cast<size_t>(g.Ng) * nang * nang * 4 + (1 << 22);              // Kv during set-up
+}
+
+}  // namespace bfpn
+
+using namespace bfpn;
 
 // ============================================================ workspace ===
 struct bfpn_workspace {
@@ -543,15 +502,15 @@ void op_energy(bfpn_workspace* ws, T* F, T* f, const double* S, const double* si
     BFPN_REQUIRE(ws && F && f && S && sigma, "null argument");
     BFPN_REQUIRE(Ng > 0 && nyz > 0 && nang > 0 && h > 0, "invalid sizes");
     if (Q2) throw Error(BFPN_E_UNSUPPORTED, "a psi^2 source (Q2) is not supported");
+    BFPN_HIP(hipStreamSynchronize(s));  // the workspace may still be in use
     const auto hS = to_host(S, Ng + 1);
     const auto hsig = to_host(sigma, Ng);
     std::vector<double> hT;
     if (Tc) hT = to_host(Tc, Ng);
-    upload(ws->coef, energy_coefs(hS, hsig, Tc ? hT.data() : nullptr, straggling != 0, h, dg, rho, Ng), s);
+    upload(ws->coef, energy_coefs(hS, hsig, Tc ? hT.data() : nullptr, straggling != 0, h, dg, rho, Ng));
     launch_energy_half<T>(F, f, ws->coef.as<EnergyCoef>(), Q1, Ng,
                           static_cast<long long>(nyz) * nang, straggling != 0 && Tc, nullptr,
                           nullptr, nang, nullptr, false, s);
-    BFPN_HIP(hipStreamSynchronize(s));  // the coefficient upload came from host memory
 }
 
 template <typename T>
@@ -560,6 +519,7 @@ void op_source(bfpn_workspace* ws, const T* Fp, const T* fp, T* Q1, const double
                int nang, double dg, hipStream_t s) {
     BFPN_REQUIRE(ws && Fp && fp && Q1 && e1 && e2 && kv && beg && end, "null argument");
     BFPN_REQUIRE(Ng > 0 && nyz > 0 && nang > 0, "invalid sizes");
+    BFPN_HIP(hipStreamSynchronize(s));
     const size_t n = static_cast<size_t>(Ng) * nyz * nang;
     const long long nk = static_cast<long long>(Ng) * nang * nang;
     ws->a1.ensure(n * sizeof(T));
@@ -576,7 +536,6 @@ void op_source(bfpn_workspace* ws, const T* Fp, const T* fp, T* Q1, const double
         ws->plan.build(ws->kvT.as<float>(), h1, h2, hb, he, Ng, nang, dg, s);
         launch_source_mfma(Fp, fp, ws->a1.as<float>(), ws->a2.as<float>(), Q1, ws->plan.plan, Ng,
                            nyz, nang, s);
-        BFPN_HIP(hipStreamSynchronize(s));
     } else {
         launch_source_angular<T>(Fp, fp, ws->kvT.as<T>(), ws->a1.as<T>(), ws->a2.as<T>(), Ng, nyz,
                                  nang, s);
@@ -590,6 +549,7 @@ void op_transport(bfpn_workspace* ws, T* F, const double* omend, int Ny, int Nz,
                   double dy, double dz, double h, int preserve, hipStream_t s) {
     BFPN_REQUIRE(ws && F && omend, "null argument");
     BFPN_REQUIRE(Ny > 0 && Nz > 0 && Ng > 0 && nang > 0, "invalid sizes");
+    BFPN_HIP(hipStreamSynchronize(s));
     ws->om.ensure(2 * nang * sizeof(T));
     launch_convert<T>(omend, ws->om.as<T>(), 2 * nang, s);
     launch_transport_half<T>(F, ws->om.as<T>(), Ny, Nz, Ng, nang, dy, dz, h, preserve != 0, s);
@@ -600,18 +560,18 @@ void op_angle(bfpn_workspace* ws, T* F, const double* sig_tr_host, int nyz, int 
               int Nom, double du, double dv, double dt, double rho, hipStream_t s) {
     BFPN_REQUIRE(ws && F && sig_tr_host, "null argument");
     BFPN_REQUIRE(nyz > 0 && Ng > 0 && Nmu > 0 && Nom > 0 && Nmu * Nom <= 1024, "invalid sizes");
-    upload(ws->factors, cast_vector<T>(angle_factors(sig_tr_host, Ng, Nmu, Nom, du, dv, dt, rho)), s);
-    if (std::is_same_v<T, float> && AnglePlanBuffers::usable(Nmu, Nom)) {
-        ws->angle_plan.build(Nmu, Nom, s);
-        launch_angle_mfma(reinterpret_cast<float*>(F), ws->factors.as<float>(), ws->angle_plan.plan,
-                          Ng, nyz, Nmu, Nom, s);
-    } else {
-        upload(ws->su, cast_vector<T>(sine_basis(Nom)), s);
-        upload(ws->sv, cast_vector<T>(sine_basis(Nmu)), s);
-        launch_angle_full<T>(F, ws->factors.as<T>(), ws->su.as<T>(), ws->sv.as<T>(), Ng, nyz, Nmu,
-                             Nom, s);
-    }
     BFPN_HIP(hipStreamSynchronize(s));
+    upload(ws->factors, cast_vector<T>(angle_factors(sig_tr_host, Ng, Nmu, Nom, du, dv, dt, rho)));
+    if constexpr (std::is_same_v<T, float>) {
+        if (AnglePlanBuffers::usable(Nmu, Nom)) {
+            ws->angle_plan.build(Nmu, Nom, s);
+            launch_angle_mfma(F, ws->factors.as<float>(), ws->angle_plan.plan, Ng, nyz, Nmu, Nom, s);
+            return;
+        }
+    }
+    upload(ws->su, cast_vector<T>(sine_basis(Nom)));
+    upload(ws->sv, cast_vector<T>(sine_basis(Nmu)));
+    launch_angle_full<T>(F, ws->factors.as<T>(), ws->su.as<T>(), ws->sv.as<T>(), Ng, nyz, Nmu, Nom, s);
 }
 
 template <typename T>
@@ -621,11 +581,12 @@ void op_idd(bfpn_workspace* ws, const T* F, const T* f, const T* F1, const T* f1
             hipStream_t s) {
     BFPN_REQUIRE(ws && F && S && sigma && en && out, "null argument");
     BFPN_REQUIRE(Ng > 0 && Ny > 0 && Nz > 0 && nang > 0 && rho > 0, "invalid sizes");
+    BFPN_HIP(hipStreamSynchronize(s));
     const auto hS = to_host(S, Ng + 1);
     const auto hsig = to_host(sigma, Ng);
     const auto hen = to_host(en, Ng + 1);
-    upload(ws->dose_w, dose_coefs(hS, hsig, hen, dg, cat != 0, Ng), s);
-    upload(ws->yzw, yz_weights(Ny, Nz, trap != 0), s);
+    upload(ws->dose_w, dose_coefs(hS, hsig, hen, dg, cat != 0, Ng));
+    upload(ws->yzw, yz_weights(Ny, Nz, trap != 0));
     const long long lanes = static_cast<long long>(Ny + 1) * (Nz + 1) * nang;
     const int nb = dose_blocks(Ng, lanes);
     ws->partial.ensure((nb + 1) * sizeof(double));
@@ -644,7 +605,7 @@ hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 extern "C" {
 
 BFPN_API const char* bfpn_last_error(void) { return g_last_error.c_str(); }
-BFPN_API const char* bfpn_version(void) { return "bfpn-mi355x 0.1 (gfx950)"; }
+BFPN_API const char* bfpn_version(void) { return "bfpn-mi355x 0.2 (gfx950)"; }
 
 BFPN_API bfpn_status bfpn_device_count(int* count) {
     return guarded([&] {
@@ -774,17 +735,33 @@ BFPN_API bfpn_status bfpn_create(const bfpn_grid* grid, const bfpn_phys* phys, c
     return guarded([&] {
         BFPN_REQUIRE(grid && phys && out, "null argument");
         *out = nullptr;
-        const std::string path = data_path ? data_path : "./data";
-        if (precision == BFPN_FP32)
-            *out = new Context<float>(*grid, *phys, path);
-        else if (precision == BFPN_FP64)
-            *out = new Context<double>(*grid, *phys, path);
-        else
-            throw Error(BFPN_E_INVALID, "precision must be BFPN_FP32 or BFPN_FP64");
+        validate_params(*grid, *phys);
+        // Memory preflight (preflightMemory, src/bfpm_solver.cu:1321-1433)
+        size_t free_b = 0, total_b = 0;
+        BFPN_HIP(hipMemGetInfo(&free_b, &total_b));
+        const size_t need = context_bytes(*grid, *phys, precision) +
+                            physics_bytes(*grid, *phys, precision) + (64ull << 20);
+        if (need > free_b)
+            throw Error(BFPN_E_NOMEM,
+                        "Requested grid needs about " + std::to_string(need >> 20) +
+                            " MiB of device memory, but only " + std::to_string(free_b >> 20) +
+                            " MiB is free (Ny=" + std::to_string(grid->Ny) + ", Nz=" +
+                            std::to_string(grid->Nz) + ", Ng=" + std::to_string(grid->Ng) +
+                            ", Nmu=" + std::to_string(grid->Nmu) + ", Nom=" + std::to_string(grid->Nom) + ")");
+        auto physics = make_physics(*grid, *phys, data_path ? data_path : "./data", precision);
+        *out = make_context(physics, *grid, *phys);
     });
 }
 
 BFPN_API void bfpn_destroy(bfpn_ctx* ctx) { delete ctx; }
+
+BFPN_API bfpn_status bfpn_stopping_power(const char* material, int Ng, double Lg, double* S) {
+    return guarded([&] {
+        BFPN_REQUIRE(S && Ng > 0 && Lg > 0, "invalid arguments");
+        const auto v = stopping_power_table(material ? material : "", Ng, Lg, Lg / Ng);
+        std::copy(v.begin(), v.end(), S);
+    });
+}
 
 BFPN_API bfpn_status bfpn_get_deltas(const bfpn_ctx* ctx, bfpn_deltas* out) {
     return guarded([&] {
@@ -794,7 +771,9 @@ BFPN_API bfpn_status bfpn_get_deltas(const bfpn_ctx* ctx, bfpn_deltas* out) {
 }
 
 BFPN_API size_t bfpn_cells(const bfpn_ctx* ctx) { return ctx ? ctx->cells : 0; }
-BFPN_API size_t bfpn_device_bytes(const bfpn_ctx* ctx) { return ctx ? ctx->device_bytes() : 0; }
+BFPN_API size_t bfpn_device_bytes(const bfpn_ctx* ctx) {
+    return ctx ? ctx->device_bytes() + ctx->physics->device_bytes() : 0;
+}
 BFPN_API int bfpn_num_steps(const bfpn_ctx* ctx, double t_final) {
     return ctx ? static_cast<int>(std::ceil(t_final / ctx->d.dt - 1e-12)) : -1;
 }
@@ -802,21 +781,22 @@ BFPN_API int bfpn_num_steps(const bfpn_ctx* ctx, double t_final) {
 BFPN_API bfpn_status bfpn_get_table(const bfpn_ctx* ctx, int which, double* out, size_t count) {
     return guarded([&] {
         BFPN_REQUIRE(ctx && out, "null argument");
+        const PhysicsBase& P = *ctx->physics;
         std::vector<double> tmp;
         const std::vector<double>* v = nullptr;
         switch (which) {
-            case 0: v = &ctx->S; break;
-            case 1: v = &ctx->mat.sigma; break;
-            case 2: v = &ctx->Tc; break;
-            case 3: v = &ctx->sig_tr; break;
-            case 4: v = &ctx->en; break;
-            case 5: v = &ctx->ker.e1; break;
-            case 6: v = &ctx->ker.e2; break;
-            case 7: v = &ctx->ker.v; break;
-            case 8: v = &ctx->om; break;
-            case 9: v = &ctx->mat.mu; break;
-            case 10: tmp.assign(ctx->ker.beg.begin(), ctx->ker.beg.end()); v = &tmp; break;
-            case 11: tmp.assign(ctx->ker.end.begin(), ctx->ker.end.end()); v = &tmp; break;
+            case 0: v = &P.S; break;
+            case 1: v = &P.mat.sigma; break;
+            case 2: v = &P.Tc; break;
+            case 3: v = &P.sig_tr; break;
+            case 4: v = &P.en; break;
+            case 5: v = &P.ker.e1; break;
+            case 6: v = &P.ker.e2; break;
+            case 7: v = &P.ker.v; break;
+            case 8: v = &P.om; break;
+            case 9: v = &P.mat.mu; break;
+            case 10: tmp.assign(P.ker.beg.begin(), P.ker.beg.end()); v = &tmp; break;
+            case 11: tmp.assign(P.ker.end.begin(), P.ker.end.end()); v = &tmp; break;
             default: throw Error(BFPN_E_INVALID, "unknown table index");
         }
         BFPN_REQUIRE(count == v->size(), "table size mismatch: expected " + std::to_string(v->size()));

@@ -1,0 +1,73 @@
+// solver.hpp — per-beam contexts over shared physics tables (internal API).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "tables.hpp"
+
+namespace bfpn {
+
+// Everything a depth march reads that does not depend on the beam itself:
+// host FP64 tables, the per-group eq15 coefficients, CN factors, dose
+// weights, direction table and the source operands.  Beams with the same
+// grid, material tables, density, step and flags share one instance.
+struct PhysicsBase {
+    virtual ~PhysicsBase() = default;
+    virtual size_t device_bytes() const = 0;
+    std::string key;
+    int precision = BFPN_FP32;
+    bfpn_grid grid{};
+    bfpn_deltas d{};
+    int nyz = 0, nang = 0;
+    long long lanes = 0;
+    size_t cells = 0;
+    bool secondary = true;
+    MaterialTables mat;
+    std::vector<double> S, Tc, sig_tr, en, om;
+    SourceKernels ker;
+};
+
+std::string physics_key(const bfpn_grid& g, const bfpn_phys& p, const std::string& data_path,
+                        int precision);
+std::shared_ptr<PhysicsBase> make_physics(const bfpn_grid& g, const bfpn_phys& p,
+                                          const std::string& data_path, int precision);
+// Device bytes of a context's own arrays (state, source scratch, partials),
+// excluding the shared physics.
+size_t context_bytes(const bfpn_grid& g, const bfpn_phys& p, int precision);
+size_t physics_bytes(const bfpn_grid& g, const bfpn_phys& p, int precision);
+
+void validate_params(const bfpn_grid& g, const bfpn_phys& p);
+
+}  // namespace bfpn
+
+// One beam: its state in HBM, history, stream; the opaque handle of the C ABI.
+struct bfpn_ctx {
+    virtual ~bfpn_ctx() = default;
+    virtual void advance(int nsteps) = 0;
+    virtual void history(double* idd, double* proxy, int cap, int* n) = 0;
+    virtual void read_state(int which, double* host, size_t count) = 0;
+    virtual void write_state(int which, const double* host, size_t count) = 0;
+    virtual size_t device_bytes() const = 0;
+    virtual int steps_done() const = 0;
+
+    bfpn_grid grid{};
+    bfpn_phys phys{};
+    bfpn_deltas d{};
+    size_t cells = 0;
+    hipStream_t stream = nullptr;
+    bool profiling = false;
+    double phase_seconds[5] = {0, 0, 0, 0, 0};
+    double phase_bytes[5] = {0, 0, 0, 0, 0};
+    int phase_calls[5] = {0, 0, 0, 0, 0};
+    std::shared_ptr<bfpn::PhysicsBase> physics;
+};
+
+namespace bfpn {
+bfpn_ctx* make_context(std::shared_ptr<PhysicsBase> physics, const bfpn_grid& g,
+                       const bfpn_phys& p);
+}  // namespace bfpn

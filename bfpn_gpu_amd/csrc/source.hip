@@ -24,6 +24,12 @@ namespace {
 
 // ------------------------------------------------------------ FP32 / MFMA --
 // grid: (ceil(nang/64), ceil(nyz/64), Ng); 4 waves, each a 32x32 output block.
+// Per 32-wide K step the work-group stages a 64(s) x 32(m') tile of both
+// inputs -- read as 128-byte row segments, split into BF16 hi/lo once -- and
+// the matching 64(m) x 32(m') tile of the pre-split transposed Kv in LDS;
+// the waves then read their MFMA fragments as 16-byte LDS loads.
+constexpr int kKPitch = 40;  // bf16 per LDS row (32 + 8 pad)
+
 __global__ __launch_bounds__(256) void source_angular_mfma(const float* __restrict__ Fp,
                                                            const float* __restrict__ fp,
                                                            const __bf16* __restrict__ kth,
@@ -31,13 +37,16 @@ __global__ __launch_bounds__(256) void source_angular_mfma(const float* __restri
                                                            float* __restrict__ A1,
                                                            float* __restrict__ A2, int nyz,
                                                            int nang, int nap) {
+    __shared__ __attribute__((aligned(16))) __bf16 xs[4][64][kKPitch];  // X1 hi/lo, X2 hi/lo
+    __shared__ __attribute__((aligned(16))) __bf16 ks[2][64][kKPitch];  // Kv^T hi/lo
     const int g = blockIdx.z;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = lane & 15, q = lane >> 4;
-    const int s_base = blockIdx.y * 64 + (wave & 1) * 32;
-    const int m_base = blockIdx.x * 64 + (wave >> 1) * 32;
+    const int s0 = blockIdx.y * 64, m0 = blockIdx.x * 64;
+    const int ws = (wave & 1) * 32, wm = (wave >> 1) * 32;  // wave's sub-block
     const long long plane = static_cast<long long>(g) * nyz * nang;
     const long long kt_plane = static_cast<long long>(g) * nap * nap;
+    const int lrow = threadIdx.x >> 2, lk = (threadIdx.x & 3) * 8;  // staging: row, 8 columns
     f32x4 acc1[2][2], acc2[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
@@ -45,34 +54,47 @@ __global__ __launch_bounds__(256) void source_angular_mfma(const float* __restri
         for (int b = 0; b < 2; ++b) acc1[a][b] = acc2[a][b] = zero4();
 
     for (int k0 = 0; k0 < nap; k0 += 32) {
-        SplitFrag x1[2], x2[2], kf[2];
-#pragma unroll
-        for (int mt = 0; mt < 2; ++mt) {
-            const int s = s_base + mt * 16 + i;
+        {  // stage inputs: rows s0+lrow, columns k0+lk .. +7
+            const int srow = s0 + lrow;
             float v1[8], v2[8];
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const int k = k0 + 8 * q + e;
-                const bool in = s < nyz && k < nang;
-                const long long idx = plane + static_cast<long long>(s) * nang + k;
+                const int k = k0 + lk + e;
+                const bool in = srow < nyz && k < nang;
+                const long long idx = plane + static_cast<long long>(srow) * nang + k;
                 const float a = in ? Fp[idx] : 0.f;
                 v1[e] = a > 0.f ? a : 0.f;
                 v2[e] = in ? fp[idx] : 0.f;
             }
-            x1[mt] = split8(v1);
-            x2[mt] = split8(v2);
-        }
-#pragma unroll
-        for (int nt = 0; nt < 2; ++nt) {
-            const int m = m_base + nt * 16 + i;
-            if (m < nap) {
-                const long long off = kt_plane + static_cast<long long>(m) * nap + k0 + 8 * q;
-                kf[nt].hi = *reinterpret_cast<const bf16x8*>(kth + off);
-                kf[nt].lo = *reinterpret_cast<const bf16x8*>(ktl + off);
+            const SplitFrag f1 = split8(v1), f2 = split8(v2);
+            *reinterpret_cast<bf16x8*>(&xs[0][lrow][lk]) = f1.hi;
+            *reinterpret_cast<bf16x8*>(&xs[1][lrow][lk]) = f1.lo;
+            *reinterpret_cast<bf16x8*>(&xs[2][lrow][lk]) = f2.hi;
+            *reinterpret_cast<bf16x8*>(&xs[3][lrow][lk]) = f2.lo;
+            const int mrow = m0 + lrow;
+            bf16x8 kh, kl;
+            if (mrow < nap) {
+                const long long off = kt_plane + static_cast<long long>(mrow) * nap + k0 + lk;
+                kh = *reinterpret_cast<const bf16x8*>(kth + off);
+                kl = *reinterpret_cast<const bf16x8*>(ktl + off);
             } else {
 #pragma unroll
-                for (int e = 0; e < 8; ++e) kf[nt].hi[e] = kf[nt].lo[e] = static_cast<__bf16>(0.f);
+                for (int e = 0; e < 8; ++e) kh[e] = kl[e] = static_cast<__bf16>(0.f);
             }
+            *reinterpret_cast<bf16x8*>(&ks[0][lrow][lk]) = kh;
+            *reinterpret_cast<bf16x8*>(&ks[1][lrow][lk]) = kl;
+        }
+        __syncthreads();
+        SplitFrag x1[2], x2[2], kf[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int r = ws + t * 16 + i, c = wm + t * 16 + i;
+            x1[t].hi = *reinterpret_cast<const bf16x8*>(&xs[0][r][8 * q]);
+            x1[t].lo = *reinterpret_cast<const bf16x8*>(&xs[1][r][8 * q]);
+            x2[t].hi = *reinterpret_cast<const bf16x8*>(&xs[2][r][8 * q]);
+            x2[t].lo = *reinterpret_cast<const bf16x8*>(&xs[3][r][8 * q]);
+            kf[t].hi = *reinterpret_cast<const bf16x8*>(&ks[0][c][8 * q]);
+            kf[t].lo = *reinterpret_cast<const bf16x8*>(&ks[1][c][8 * q]);
         }
 #pragma unroll
         for (int mt = 0; mt < 2; ++mt)
@@ -81,17 +103,18 @@ __global__ __launch_bounds__(256) void source_angular_mfma(const float* __restri
                 acc1[mt][nt] = mma_split(x1[mt], kf[nt], acc1[mt][nt]);
                 acc2[mt][nt] = mma_split(x2[mt], kf[nt], acc2[mt][nt]);
             }
+        __syncthreads();
     }
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt) {
-            const int m = m_base + nt * 16 + i;
+            const int m = m0 + wm + nt * 16 + i;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int s = s_base + mt * 16 + 4 * q + r;
-                if (s < nyz && m < nang) {
-                    const long long idx = plane + static_cast<long long>(s) * nang + m;
+                const int sidx = s0 + ws + mt * 16 + 4 * q + r;
+                if (sidx < nyz && m < nang) {
+                    const long long idx = plane + static_cast<long long>(sidx) * nang + m;
                     A1[idx] = acc1[mt][nt][r];
                     A2[idx] = acc2[mt][nt][r];
                 }

@@ -260,6 +260,58 @@ BFPN_API bfpn_status bfpn_set_profiling(bfpn_ctx* ctx, int on);
 BFPN_API bfpn_status bfpn_phase_stats(bfpn_ctx* ctx, double* seconds5, int* calls5,
                                       double* bytes5);
 
+/* Stopping power table S on the Ng+1 group edges of [1, 1+Lg] MeV for a
+ * material ("", water, bone, air) -- computeStoppingPower,
+ * src/bfpm_solver.cu:1570-1597.  Host only; used e.g. for CSDA ranges. */
+BFPN_API bfpn_status bfpn_stopping_power(const char* material, int Ng, double Lg, double* S);
+
+/* ---------------------------------------------------------------------------
+ * Serving: continuous batching of beamlet solves (no counterpart in the
+ * reference, which runs one beam per process).  A scheduler thread admits
+ * queued requests as slots and HBM budget free up, keeps each admitted beam
+ * resident on its own stream, interleaves slices of depth steps of all
+ * running beams, and shares physics tables between beams with equal grid,
+ * step, material and flags.  A beam's IDD does not depend on the other
+ * beams in flight.
+ * ------------------------------------------------------------------------- */
+typedef struct bfpn_server bfpn_server;
+
+typedef struct bfpn_server_opts {
+    size_t hbm_budget_bytes;  /* 0 = 90 % of the free device memory at creation */
+    int max_active;           /* beams resident and running at once            */
+    int slice_steps;          /* depth steps enqueued per beam per turn        */
+    int precision;            /* BFPN_FP32 or BFPN_FP64                        */
+} bfpn_server_opts;
+
+typedef struct bfpn_server_stats {
+    uint64_t submitted, completed, failed;
+    int active, queued, peak_active, cached_tables;
+    size_t bytes_in_use;
+} bfpn_server_stats;
+
+BFPN_API bfpn_status bfpn_server_create(const bfpn_server_opts* opts, bfpn_server** out);
+/* Queues one beam solve to t_final (same grid/phys/data meaning as bfpn_create);
+ * returns at once with its id. Thread safe. */
+BFPN_API bfpn_status bfpn_server_submit(bfpn_server* s, const bfpn_grid* grid,
+                                        const bfpn_phys* phys, const char* data_path,
+                                        double t_final, uint64_t* id);
+/* 0 queued, 1 running, 2 done, 3 failed, -1 unknown (or already collected). */
+BFPN_API int bfpn_server_poll(bfpn_server* s, uint64_t id);
+/* Blocks until request `id` finished (timeout_s <= 0: no limit), copies its IDD
+ * history and releases the result.  times3 = {queued s, running s, total s}. */
+BFPN_API bfpn_status bfpn_server_wait(bfpn_server* s, uint64_t id, double timeout_s,
+                                      double* idd, int capacity, int* nsteps,
+                                      double* times3);
+BFPN_API bfpn_status bfpn_server_get_stats(bfpn_server* s, bfpn_server_stats* out);
+/* The admission estimates the scheduler charges against the budget: device
+ * bytes of one beam's own arrays, and of the physics tables beams with the
+ * same grid, material and flags share.  Host only. */
+BFPN_API bfpn_status bfpn_server_estimate(const bfpn_grid* grid, const bfpn_phys* phys,
+                                          int precision, size_t* context_bytes,
+                                          size_t* physics_bytes);
+/* Finishes queued and running requests, then stops the scheduler. */
+BFPN_API void bfpn_server_destroy(bfpn_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,92 @@
+"""Continuous-batching server (bfpn_server) on the GPU.
+
+* Each beam's IDD is identical, bit for bit, alone (a plain Solver) and in a
+  mix of other beams running concurrently (beams share only read-only tables).
+* Admission respects the HBM budget and slot count; a request that can never
+  fit fails with a message and does not disturb the others.
+* Requests submitted while others are running are admitted and completed.
+"""
+import numpy as np
+import pytest
+
+from gpu_cases import small_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _mix():
+    return [
+        small_case(Ny=3, Nz=3, Nmu=4, Nom=4, energy=120.0, time=0.12, flags={"eq15_straggling": 1}),
+        small_case(Ny=5, Nz=4, Nmu=5, Nom=4, energy=200.0, time=0.16),
+        small_case(Ny=3, Nz=3, Nmu=4, Nom=4, energy=90.0, time=0.08, material="bone", density=1.85,
+                   data=small_case().data.replace("water", "bone")),
+        small_case(Ny=4, Nz=4, Nmu=4, Nom=4, energy=150.0, time=0.20, flags={"primary_only": 1}),
+    ]
+
+
+def _alone(gpu, case):
+    with gpu.Solver(case, gpu.FP32) as s:
+        idd, _ = s.run()
+    return idd
+
+
+@pytest.mark.parametrize("max_active,slice_steps", [(4, 1), (2, 3), (1, 2)])
+def test_results_are_identical_alone_and_in_the_mix(gpu, max_active, slice_steps):
+    cases = _mix()
+    ref = [_alone(gpu, c) for c in cases]
+    with gpu.Server(max_active=max_active, slice_steps=slice_steps) as srv:
+        ids = [srv.submit(c) for c in cases]
+        out = [srv.wait(i, timeout=120)[0] for i in ids]
+        st = srv.stats()
+    for a, b in zip(out, ref):
+        assert len(a) == len(b) > 0
+        np.testing.assert_array_equal(a, b)
+    assert st["completed"] == len(cases) and st["failed"] == 0
+    assert st["peak_active"] <= max_active
+    if max_active > 1:
+        assert st["peak_active"] > 1  # the beams really ran concurrently
+
+
+def test_budget_limits_admission_and_shares_tables(gpu):
+    cases = [small_case(Ny=3, Nz=3, Nmu=4, Nom=4, energy=100.0 + 10 * k, time=0.08) for k in range(3)]
+    ref = [_alone(gpu, c) for c in cases]
+    beam, shared = gpu.Server.estimate(cases[0])
+    # room for the shared tables and one and a half beams: admission runs them one at a time
+    with gpu.Server(max_active=3, slice_steps=1, hbm_budget_bytes=shared + beam * 3 // 2) as srv:
+        ids = [srv.submit(c) for c in cases]
+        out = [srv.wait(i, timeout=120)[0] for i in ids]
+        st = srv.stats()
+    for a, b in zip(out, ref):
+        np.testing.assert_array_equal(a, b)
+    assert st["peak_active"] == 1 and st["completed"] == 3 and st["cached_tables"] == 1
+
+
+def test_oversized_request_fails_cleanly(gpu):
+    small = small_case(Ny=3, Nz=3, Nmu=4, Nom=4, time=0.08)
+    huge = small_case(Ny=200, Nz=200, Nmu=20, Nom=20, time=0.08)
+    with gpu.Server(max_active=2, slice_steps=2, hbm_budget_bytes=512 * 1024 * 1024) as srv:
+        a = srv.submit(small)
+        b = srv.submit(huge)
+        c = srv.submit(small)
+        with pytest.raises(gpu.BfpnError, match="HBM budget"):
+            srv.wait(b, timeout=120)
+        ra = srv.wait(a, timeout=120)[0]
+        rc = srv.wait(c, timeout=120)[0]
+    np.testing.assert_array_equal(ra, rc)
+
+
+def test_requests_join_while_others_run(gpu):
+    import time
+
+    long_case = small_case(Ny=5, Nz=5, Nmu=5, Nom=5, energy=200.0, time=0.8)
+    short = small_case(Ny=3, Nz=3, Nmu=4, Nom=4, energy=100.0, time=0.08)
+    ref_short = _alone(gpu, short)
+    with gpu.Server(max_active=4, slice_steps=1) as srv:
+        a = srv.submit(long_case)
+        time.sleep(0.05)
+        b = srv.submit(short)
+        rb, tb = srv.wait(b, timeout=120)
+        assert srv.poll(a) in (1, 2)  # the long beam was admitted before the short one finished
+        ra, ta = srv.wait(a, timeout=120)
+    np.testing.assert_array_equal(rb, ref_short)
+    assert len(ra) == long_case.num_steps()

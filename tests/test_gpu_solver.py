@@ -81,11 +81,14 @@ def test_tables_bit_identical_to_oracle(gpu, oracle_mod):
 
 
 def test_fp32_solver_tracks_oracle(gpu, oracle_mod):
+    """FP32 state after 6 steps: IDD within 2e-5; primary psi^1 within 1e-4 of
+    its maximum; secondary psi^1 within 5e-4 (its source runs in split-BF16,
+    ~2^-16 per product, and feeds every secondary step)."""
     case = small_case()
     o, g = _run_both(gpu, oracle_mod, case, 6, gpu.FP32)
     np.testing.assert_allclose(g.history()[0], o.history()[0], rtol=2e-5)
-    for k in (0, 2):
-        assert _rel(g.state(k), o.state(k)) < 1e-4
+    assert _rel(g.state(0), o.state(0)) < 1e-4
+    assert _rel(g.state(2), o.state(2)) < 5e-4
 
 
 def test_history_and_restart_are_consistent(gpu):
