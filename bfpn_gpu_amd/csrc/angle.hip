@@ -5,13 +5,13 @@
 // InvU; src/angle_diffusion.cu:254-362), each a full pass over HBM with two
 // scratch arrays.
 //
-// MI355X design: one launch.  A work-group owns a batch of (g, s) planes
-// (Nmu x Nom values each), keeps the planes and both sine bases in LDS and
-// runs the whole sandwich
+// MI355X design: one launch, one read and one write of the state per call.
+// The whole sandwich
 //     Y = Psi Su^T,  Z = coeff_g (.) (Sv Y),  W = Sv^T Z,  Psi' = W Su
-// between LDS buffers, writing the plane back in place: one read and one
-// write of the state per call.  The CN factors (ratio * DST normalisation)
-// are built once on the host per (g, direction).
+// runs per (g, s) plane of Nmu x Nom values, in place.  The CN factors
+// (ratio * DST normalisation) are built once on the host per (g, direction).
+// FP32 state: matrix cores, stages chained in registers (below).  FP64 state
+// or planes wider than 32: vector kernel with the plane and bases in LDS.
 
 #include "kernels.hpp"
 #include "mfma.hpp"
@@ -20,111 +20,131 @@ namespace bfpn {
 namespace {
 
 // ---------------------------------------------------------- FP32 / MFMA ----
-// One wave per (g, s) plane (grid-stride over planes).  The plane, padded to
-// 32 x 32 with zeros, lives in a per-wave LDS tile; each stage is a 32x32x32
-// product on the matrix cores (split-BF16, FP32 accumulation), and the LDS
-// tile converts the D layout of one stage into the A/B layout of the next.
-// The DST-I bases are symmetric (S[k][j] = S[j][k]), so Y = X Su^T uses Su
-// as the B operand and W = Sv^T Z uses Sv as the A operand, like Sv Y.
-// MT / NT: 16-row tiles along v (Nmu) and a (Nom), 1 or 2 (compile time, so
-// the fragment arrays stay in registers).
+// One wave per (g, s) plane (each wave walks a contiguous run of planes), the
+// plane zero padded to 32 x 32, every stage a 32x32x32 product on the matrix
+// cores (split-BF16, FP32 accumulation; MT / NT = 16-row tiles along v and a).
+// The stages chain in registers, with no LDS: a stage's output tile, lane
+// holding D[4q+r][i], is the next stage's K operand when the K axis is taken
+// in the order  slot 8q + 4u + r  <->  index 16u + 4q + r  (chain_index), and
+// the constant sine bases on the other side are stored with their columns in
+// that order.  The sandwich is evaluated as
+//     Y  = X Su^T            (A = X from HBM,   B = Su)
+//     Z  = C (.) (Sv Y)      (A = Sv chained,   B = Y)
+//     W^T = Z^T Sv           (A = Z^T,          B = Sv chained)
+//     X' = W Su              (A = W,            B = Su chained)
+// using the symmetry of the DST-I bases; X' leaves the last stage in the
+// layout of the plane and is stored straight back to HBM.
 template <int MT, int NT>
-__global__ __launch_bounds__(256) void angle_mfma_kernel(
-    float* __restrict__ F, const float* __restrict__ factors, const __bf16* __restrict__ su_h,
-    const __bf16* __restrict__ su_l, const __bf16* __restrict__ sv_h,
-    const __bf16* __restrict__ sv_l, long long planes, int nyz, int Nmu, int Nom) {
-    __shared__ float tile[4][32][33];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = lane & 15, q = lane >> 4;
-    float(*B)[33] = tile[wave];
+__global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
+                                                         const float* __restrict__ factors,
+                                                         AnglePlan plan, int planes, int per_wave,
+                                                         int nyz, int Nmu, int Nom) {
+    const int lane = threadIdx.x & 63, i = lane & 15, q = lane >> 4;
+    int p = (blockIdx.x * 4 + (threadIdx.x >> 6)) * per_wave;
+    const int p_end = min(planes, p + per_wave);
+    if (p >= p_end) return;
     const int na = Nmu * Nom;
-    constexpr int mt_n = MT;  // tiles along v / nu
-    constexpr int nt_n = NT;  // tiles along a / mu
-    SplitFrag su[2], sv[2];
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
+    auto frag = [&](const __bf16* h, const __bf16* l, int t) {
         const int off = (t * 16 + i) * 32 + 8 * q;
-        su[t].hi = *reinterpret_cast<const bf16x8*>(su_h + off);
-        su[t].lo = *reinterpret_cast<const bf16x8*>(su_l + off);
-        sv[t].hi = *reinterpret_cast<const bf16x8*>(sv_h + off);
-        sv[t].lo = *reinterpret_cast<const bf16x8*>(sv_l + off);
+        return SplitFrag{*reinterpret_cast<const bf16x8*>(h + off),
+                         *reinterpret_cast<const bf16x8*>(l + off)};
+    };
+    SplitFrag su[NT], sup[NT], svp[MT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        su[t] = frag(plan.su_h, plan.su_l, t);
+        sup[t] = frag(plan.sup_h, plan.sup_l, t);
     }
-    for (int e = lane; e < 32 * 32; e += 64) B[e >> 5][e & 31] = 0.f;
+#pragma unroll
+    for (int t = 0; t < MT; ++t) svp[t] = frag(plan.svp_h, plan.svp_l, t);
 
-    auto row_frag = [&](int row) {  // A operand: B[row][8q .. 8q+7]
-        float v[8];
+    int g_cur = -1;
+    float cf[MT][NT][4];
+    for (; p < p_end; ++p) {
+        float* X = F + static_cast<long long>(p) * na;
+        const int g = p / nyz;
+        if (g != g_cur) {  // CN factors of this group, in the D layout
+            g_cur = g;
+            const float* C = factors + static_cast<long long>(g) * na;
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = B[row][8 * q + e];
-        return split8(v);
-    };
-    auto col_frag = [&](int col) {  // B operand: B[8q .. 8q+7][col]
-        float v[8];
+            for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = B[8 * q + e][col];
-        return split8(v);
-    };
-    auto store = [&](const f32x4 (&d)[2][2]) {
+                for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
-        for (int mt = 0; mt < mt_n; ++mt)
-#pragma unroll
-            for (int nt = 0; nt < nt_n; ++nt)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) B[mt * 16 + 4 * q + r][nt * 16 + i] = d[mt][nt][r];
-    };
-
-    for (long long p = static_cast<long long>(blockIdx.x) * 4 + wave; p < planes;
-         p += static_cast<long long>(gridDim.x) * 4) {
-        float* X = F + p * na;
-        const float* C = factors + (p / nyz) * na;
-        for (int e = lane; e < na; e += 64) {
-            const int v = e / Nom;
-            B[v][e - v * Nom] = X[e];
+                    for (int r = 0; r < 4; ++r) {
+                        const int nu = mt * 16 + 4 * q + r, mu = nt * 16 + i;
+                        cf[mt][nt][r] = (nu < Nmu && mu < Nom) ? C[nu * Nom + mu] : 0.f;
+                    }
         }
-        f32x4 d[2][2];
         // stage 1: Y[v][mu] = sum_a X[v][a] Su[mu][a]
-        SplitFrag a[2], b[2];
+        SplitFrag xa[MT];
 #pragma unroll
-        for (int mt = 0; mt < mt_n; ++mt) a[mt] = row_frag(mt * 16 + i);
+        for (int mt = 0; mt < MT; ++mt) {
+            const int v = mt * 16 + i;
+            float x[8];
 #pragma unroll
-        for (int mt = 0; mt < mt_n; ++mt)
+            for (int e = 0; e < 8; ++e) {
+                const int a = 8 * q + e;
+                x[e] = (v < Nmu && a < Nom) ? X[v * Nom + a] : 0.f;
+            }
+            xa[mt] = split8(x);
+        }
+        f32x4 d1[MT][NT];
 #pragma unroll
-            for (int nt = 0; nt < nt_n; ++nt) d[mt][nt] = mma_split(a[mt], su[nt], zero4());
-        store(d);
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) d1[mt][nt] = mma_split(xa[mt], su[nt], zero4());
         // stage 2: Z[nu][mu] = C[nu][mu] sum_v Sv[nu][v] Y[v][mu]
+        f32x4 d2[MT][NT];
 #pragma unroll
-        for (int nt = 0; nt < nt_n; ++nt) b[nt] = col_frag(nt * 16 + i);
+        for (int nt = 0; nt < NT; ++nt) {
+            float y[8];
 #pragma unroll
-        for (int mt = 0; mt < mt_n; ++mt)
+            for (int r = 0; r < 4; ++r) {
+                y[r] = d1[0][nt][r];
+                y[4 + r] = MT > 1 ? d1[MT - 1][nt][r] : 0.f;
+            }
+            const SplitFrag b = split8(y);
 #pragma unroll
-            for (int nt = 0; nt < nt_n; ++nt) {
-                d[mt][nt] = mma_split(sv[mt], b[nt], zero4());
-                const int mu = nt * 16 + i;
+            for (int mt = 0; mt < MT; ++mt) {
+                d2[mt][nt] = mma_split(svp[mt], b, zero4());
+#pragma unroll
+                for (int r = 0; r < 4; ++r) d2[mt][nt][r] *= cf[mt][nt][r];
+            }
+        }
+        // stage 3: W^T[mu][v] = sum_nu Z[nu][mu] Sv[nu][v]
+        f32x4 d3[NT][MT];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            float z[8];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                z[r] = d2[0][nt][r];
+                z[4 + r] = MT > 1 ? d2[MT - 1][nt][r] : 0.f;
+            }
+            const SplitFrag a = split8(z);
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt) d3[nt][mt] = mma_split(a, svp[mt], zero4());
+        }
+        // stage 4: X'[v][a] = sum_mu W[v][mu] Su[mu][a], stored in place
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+            float w[8];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                w[r] = d3[0][mt][r];
+                w[4 + r] = NT > 1 ? d3[NT - 1][mt][r] : 0.f;
+            }
+            const SplitFrag a = split8(w);
+#pragma unroll
+            for (int nt = 0; nt < NT; ++nt) {
+                const f32x4 d = mma_split(a, sup[nt], zero4());
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int nu = mt * 16 + 4 * q + r;
-                    d[mt][nt][r] *= (nu < Nmu && mu < Nom) ? C[nu * Nom + mu] : 0.f;
+                    const int v = mt * 16 + 4 * q + r, an = nt * 16 + i;
+                    if (v < Nmu && an < Nom) X[v * Nom + an] = d[r];
                 }
             }
-        store(d);
-        // stage 3: W[v][mu] = sum_nu Sv[nu][v] Z[nu][mu]
-#pragma unroll
-        for (int nt = 0; nt < nt_n; ++nt) b[nt] = col_frag(nt * 16 + i);
-#pragma unroll
-        for (int mt = 0; mt < mt_n; ++mt)
-#pragma unroll
-            for (int nt = 0; nt < nt_n; ++nt) d[mt][nt] = mma_split(sv[mt], b[nt], zero4());
-        store(d);
-        // stage 4: X'[v][a] = sum_mu W[v][mu] Su[mu][a]
-#pragma unroll
-        for (int mt = 0; mt < mt_n; ++mt) a[mt] = row_frag(mt * 16 + i);
-#pragma unroll
-        for (int mt = 0; mt < mt_n; ++mt)
-#pragma unroll
-            for (int nt = 0; nt < nt_n; ++nt) d[mt][nt] = mma_split(a[mt], su[nt], zero4());
-        store(d);
-        for (int e = lane; e < na; e += 64) {
-            const int v = e / Nom;
-            X[e] = B[v][e - v * Nom];
         }
     }
 }
@@ -202,12 +222,17 @@ void launch_angle_mfma(float* F, const float* factors, const AnglePlan& plan, in
     BFPN_REQUIRE(Nmu >= 1 && Nom >= 1 && Nmu <= 32 && Nom <= 32, "MFMA DST needs Nmu, Nom <= 32");
     const long long planes = static_cast<long long>(Ng) * nyz;
     if (planes == 0) return;
-    const long long blocks = std::min<long long>((planes + 3) / 4, 4096);
-    const dim3 grid(static_cast<unsigned>(blocks)), block(256);
+    BFPN_REQUIRE(planes * Nmu * Nom < (1LL << 31) && planes < (1LL << 30),
+                 "MFMA DST indexes planes with 32-bit offsets");
+    // 8192 waves (32 per CU) walking contiguous runs of planes
+    const int waves = static_cast<int>(std::min<long long>(planes, 8192));
+    const int per_wave = static_cast<int>((planes + waves - 1) / waves);
+    const int blocks = (static_cast<int>((planes + per_wave - 1) / per_wave) + 3) / 4;
+    const dim3 grid(blocks), block(256);
     const int mt = Nmu > 16 ? 2 : 1, nt = Nom > 16 ? 2 : 1;
-#define BFPN_ANGLE(MT_, NT_)                                                                     \
-    hipLaunchKernelGGL((angle_mfma_kernel<MT_, NT_>), grid, block, 0, s, F, factors, plan.su_h,  \
-                       plan.su_l, plan.sv_h, plan.sv_l, planes, nyz, Nmu, Nom)
+#define BFPN_ANGLE(MT_, NT_)                                                                   \
+    hipLaunchKernelGGL((angle_mfma_kernel<MT_, NT_>), grid, block, 0, s, F, factors, plan,     \
+                       static_cast<int>(planes), per_wave, nyz, Nmu, Nom)
     if (mt == 2 && nt == 2) BFPN_ANGLE(2, 2);
     else if (mt == 2) BFPN_ANGLE(2, 1);
     else if (nt == 2) BFPN_ANGLE(1, 2);

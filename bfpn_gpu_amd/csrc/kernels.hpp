@@ -57,12 +57,18 @@ void launch_angle_full(T* F, const T* factors, const T* su, const T* sv, int Ng,
                        int Nom, hipStream_t s);
 
 // Split-BF16 sine bases padded to 32x32 (row-major), for the MFMA DST.
+// Split-BF16 sine bases, 32x32 zero padded, row-major: Su as is, and Su and Sv
+// with their columns in the register-chaining order of the DST kernel.
 struct AnglePlan {
     const __bf16* su_h = nullptr;
     const __bf16* su_l = nullptr;
-    const __bf16* sv_h = nullptr;
-    const __bf16* sv_l = nullptr;
+    const __bf16* sup_h = nullptr;
+    const __bf16* sup_l = nullptr;
+    const __bf16* svp_h = nullptr;
+    const __bf16* svp_l = nullptr;
 };
+// K slot 8q + 4u + r of a chained MFMA operand <-> matrix index 16u + 4q + r.
+constexpr int chain_index(int slot) { return 16 * ((slot & 7) >> 2) + 4 * (slot >> 3) + (slot & 3); }
 // FP32 DST on the matrix cores (Nmu, Nom <= 32), in place.
 void launch_angle_mfma(float* F, const float* factors, const AnglePlan& plan, int Ng, int nyz,
                        int Nmu, int Nom, hipStream_t s);

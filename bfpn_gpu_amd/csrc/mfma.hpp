@@ -52,4 +52,14 @@ __device__ inline f32x4 mma_split(const SplitFrag& a, const SplitFrag& b, f32x4 
 
 __device__ inline f32x4 zero4() { return f32x4{0.f, 0.f, 0.f, 0.f}; }
 
+// Orders one wave's LDS writes before its later LDS reads of locations other
+// lanes wrote.  The hardware already executes a wave's LDS instructions in
+// order; the fence stops the compiler from forwarding or reusing values across
+// the exchange (its alias analysis only sees the lane's own stores).
+__device__ inline void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 }  // namespace bfpn

@@ -100,12 +100,16 @@ void SourcePlanBuffers::build(const float* kv_f32, const std::vector<double>& e1
 }
 
 void AnglePlanBuffers::build(int Nmu, int Nom, hipStream_t s) {
-    std::vector<float> pad(2 * 32 * 32, 0.f);  // Su then Sv, zero padded to 32x32
+    // Su, Su (chained columns), Sv (chained columns); each zero padded to 32x32
+    std::vector<float> pad(3 * 32 * 32, 0.f);
     const auto su = sine_basis(Nom), sv = sine_basis(Nmu);
-    for (int r = 0; r < Nom; ++r)
-        for (int c = 0; c < Nom; ++c) pad[r * 32 + c] = static_cast<float>(su[r * Nom + c]);
-    for (int r = 0; r < Nmu; ++r)
-        for (int c = 0; c < Nmu; ++c) pad[1024 + r * 32 + c] = static_cast<float>(sv[r * Nmu + c]);
+    for (int r = 0; r < 32; ++r)
+        for (int c = 0; c < 32; ++c) {
+            const int cc = chain_index(c);
+            if (r < Nom && c < Nom) pad[r * 32 + c] = static_cast<float>(su[r * Nom + c]);
+            if (r < Nom && cc < Nom) pad[1024 + r * 32 + c] = static_cast<float>(su[r * Nom + cc]);
+            if (r < Nmu && cc < Nmu) pad[2048 + r * 32 + c] = static_cast<float>(sv[r * Nmu + cc]);
+        }
     h.ensure(pad.size() * sizeof(__bf16));
     l.ensure(pad.size() * sizeof(__bf16));
     upload(scratch, pad);
@@ -114,8 +118,10 @@ void AnglePlanBuffers::build(int Nmu, int Nom, hipStream_t s) {
     scratch.release();
     plan.su_h = h.as<__bf16>();
     plan.su_l = l.as<__bf16>();
-    plan.sv_h = h.as<__bf16>() + 1024;
-    plan.sv_l = l.as<__bf16>() + 1024;
+    plan.sup_h = h.as<__bf16>() + 1024;
+    plan.sup_l = l.as<__bf16>() + 1024;
+    plan.svp_h = h.as<__bf16>() + 2048;
+    plan.svp_l = l.as<__bf16>() + 2048;
 }
 
 void validate_params(const bfpn_grid& g, const bfpn_phys& p) {

@@ -3,190 +3,246 @@
 // Reference: muscl_paper_slope, spatial_flux_divergence_value and the
 // flux / predictor / corrector kernels of SpatialTransportSolver::solveEq18
 // (src/spatial_transport.cu:23-161, 202-262): four launches and three
-// scratch arrays per call, a division per slope.
+// scratch arrays per call, three limited slopes per axis and cell, a division
+// per slope.
 //
 // MI355X design: one launch, in place, each value read once and written once.
-//  * A work-group owns one energy group g and a chunk of MW consecutive
-//    directions m (the contiguous axis) for the whole (y, z) plane, so no
-//    other work-group touches its data and the update can be written in place.
-//  * It marches along z with rolling windows in LDS: 7 rows of psi, 6 rows of
-//    the predictor psi* and 4 rows of L(psi).  At step t it commits psi row
-//    t+4 (prefetched into registers one step earlier), forms L(psi) and
-//    psi* = clip(psi - h L psi) on row t+2, and writes row t-1:
-//    clip(psi - h/2 (L psi + L psi*)).  Rows outside [0, Nz] are clamped,
-//    exactly like the reference's clamped stencil taps; one barrier per step.
-//  * Planes wider than 24 nodes are cut into y strips of <= 40 nodes that
-//    recompute a 4-node halo, so the LDS window stays ~50 KB (3 groups/CU).
+//  * A work-group owns one energy group g and MW consecutive directions m
+//    (the contiguous axis) for the whole (y, z) plane, so no other work-group
+//    reads or writes its data and the update can be written in place.
+//  * Threads own y columns (P of them each) and march along z.  The z window
+//    of psi, psi* and L(psi) stays in registers, and the limited z slopes are
+//    carried from row to row, so each row costs one new z slope per stage.
+//    y neighbours come from LDS rings of 4 rows of psi and 4 rows of psi*.
+//    Step t commits psi row t+2 (prefetched into registers one step earlier),
+//    forms L(psi) and psi* = clip(psi - h L psi) on row t, and writes row t-2:
+//    clip(psi - h/2 (L psi + L psi*)).  One barrier per z step.
+//  * The flow direction is fixed per lane, so only the upwind reconstruction
+//    is formed: per axis and stage two limited slopes in y (the cell's own
+//    and its upwind neighbour's) and one in z.
 //  * The superbee slope is evaluated without a division:
-//      a = R - C, b = C - L, ab > 0:
-//      slope = sign(a) max(min(2|b|, |a|), min(|b|, 2|a|)),   else 0,
-//    which equals forward * max(0, min(2 theta,1), min(theta,2)) with
-//    theta = b/a, up to the rounding of that quotient; |a| < 1e-10 -> 0 as in
-//    the reference.
-//  * MW = 32 directions (128-byte rows) for planes up to 24 nodes wide, 16
-//    for strips, so each z step gives the 256 threads 500-800 cells.
+//      a = R - C, b = C - L, ab > 0 and |a| >= 1e-10:
+//      slope = sign(a) min(max(|a|, |b|), 2 min(|a|, |b|)),   else 0,
+//    which equals forward * max(0, min(2 theta, 1), min(theta, 2)) with
+//    theta = b / a, up to the rounding of that quotient.
+//  * Rows and columns outside the plane are clamped, exactly like the
+//    reference's clamped stencil taps: the slope of a boundary cell is 0 and
+//    the cell outside reads the boundary value with slope 0.
 
 #include "kernels.hpp"
 
 namespace bfpn {
 namespace {
 
-constexpr int kThreads = 256;
-// Ring depths (rows): psi 7, psi* 6, L(psi) 4.  With these depths the only
-// LDS write issued before a step's barrier (the next psi row) can never alias
-// a row that a slower thread of the previous step still reads, so one barrier
-// per z step suffices.
-constexpr int kPsiRing = 7, kPredRing = 6, kLRing = 4;
-constexpr int kRows = kPsiRing + kPredRing + kLRing;
-constexpr int kMaxPerThread = 3;  // y nodes per thread and row (W*MW <= 768)
+constexpr int kMaxThreads = 1024;
 
 template <typename T>
-__device__ inline T superbee_slope(T left, T center, T right) {
-    const T a = right - center;
-    const T b = center - left;
-    if (fabs(a) < T(1e-10) || !(a * b > T(0))) return T(0);
+__device__ inline T limited_slope(T a, T b) {  // a: forward, b: backward difference
     const T aa = fabs(a), ab = fabs(b);
-    const T mag = fmax(fmin(T(2) * ab, aa), fmin(ab, T(2) * aa));
-    return a > T(0) ? mag : -mag;
+    const T mag = fmin(fmax(aa, ab), T(2) * fmin(aa, ab));
+    return (aa < T(1e-10) || !(a * b > T(0))) ? T(0) : copysign(mag, a);
 }
 
-// Upwind MUSCL face-flux difference along one axis (w >= 0: faces use the
-// left cells' reconstructions; w < 0: the right cells').
+// w (right - left) of one axis from the upwind reconstructions: for w >= 0
+// the faces take the left cells (vm, s_m) and (v0, s_0), for w < 0 the right
+// cells (v0, s_0) and (vp, s_p).  `half` is +1/2 or -1/2 with the sign of w.
 template <typename T>
-__device__ inline T flux_difference(T w, T m2, T m1, T c, T p1, T p2) {
-    const bool pos = w >= T(0);
-    const T sl = pos ? superbee_slope(m2, m1, c) : superbee_slope(m1, c, p1);
-    const T sr = pos ? superbee_slope(m1, c, p1) : superbee_slope(c, p1, p2);
-    const T left = pos ? m1 + T(0.5) * sl : c - T(0.5) * sl;
-    const T right = pos ? c + T(0.5) * sr : p1 - T(0.5) * sr;
+__device__ inline T face_difference(T w, T half, bool pos, T vm, T v0, T vp, T s_up, T s_0) {
+    const T right = fma(half, pos ? s_0 : s_up, pos ? v0 : vp);
+    const T left = fma(half, pos ? s_up : s_0, pos ? vm : v0);
     return w * right - w * left;
 }
 
-// L at one cell: z taps from five row bases, y taps at offsets within row r0.
+// Flux divergence at one cell from its z window (zm, zc, zp with the carried
+// z slopes of rows -1 and 0 and the new one of row +1) and its y neighbours
+// in one LDS row: near = the upwind neighbour (y-1 for wy >= 0, else y+1),
+// far = the one beyond it, down = the downwind neighbour.
 template <typename T>
-__device__ inline T divergence(const T* zm2, const T* zm1, const T* r0, const T* zp1,
-                               const T* zp2, int c, int ym2, int ym1, int yp1, int yp2, T wy, T wz,
+__device__ inline T divergence(const T* row, int o_near, int o_far, int o_down, T zm, T zc, T zp,
+                               T sz_m, T sz_0, T sz_p, T wy, T wz, T hy, T hz, bool py, bool pz,
                                T inv_dy, T inv_dz) {
-    const T v = r0[c];
-    return flux_difference(wy, r0[ym2], r0[ym1], v, r0[yp1], r0[yp2]) * inv_dy +
-           flux_difference(wz, zm2[c], zm1[c], v, zp1[c], zp2[c]) * inv_dz;
+    const T near = row[o_near], far = row[o_far], down = row[o_down];
+    const T ym1 = py ? near : down, yp1 = py ? down : near;
+    const T s0 = limited_slope(yp1 - zc, zc - ym1);
+    // upwind neighbour's slope: cell y-1 from (y-2, y-1, y) or cell y+1 from (y, y+1, y+2)
+    const T sn = limited_slope(py ? zc - near : far - near, py ? near - far : near - zc);
+    const T fy = face_difference(wy, hy, py, ym1, zc, yp1, sn, s0);
+    const T fz = face_difference(wz, hz, pz, zm, zc, zp, pz ? sz_m : sz_p, sz_0);
+    return fy * inv_dy + fz * inv_dz;
 }
 
-// grid: (strips * ceil(nang/MW), Ng).  A work-group owns one energy group, MW
-// consecutive directions and the y strip [y0, y0+TY) of the plane (with a
-// 4-node halo it recomputes), and marches along z: at step t it commits psi
-// row t+4 (prefetched into registers during the previous step), forms L(psi)
-// and psi* on row t+2 and writes row t-1 in place.
-template <typename T, int MW>
-__global__ __launch_bounds__(kThreads) void transport_march_kernel(
-    T* __restrict__ F, const T* __restrict__ omend, int Ny, int Nz, int nang, int TY, int mchunks,
-    T inv_dy, T inv_dz, T h, int preserve_sign) {
+// grid: (ceil(nang / MW), Ng), block: MW * R threads (R y columns per pass,
+// P passes: the thread's columns are y = ry + j R).  LDS: 8 rows of
+// (Ny+1) * MW values (two rings of 4 rows).
+template <typename T, int MW, int P>
+__global__ __launch_bounds__(kMaxThreads) void transport_march_kernel(
+    T* __restrict__ F, const T* __restrict__ omend, int Ny, int Nz, int nang, int R, T inv_dy,
+    T inv_dz, T h, int preserve_sign) {
     extern __shared__ unsigned char lds_raw[];
-    static_assert(kThreads % MW == 0, "MW must divide the block size");
-    constexpr int kYStep = kThreads / MW;
     const int Ny1 = Ny + 1;
-    const int strip = blockIdx.x / mchunks;
-    const int m0 = (blockIdx.x - strip * mchunks) * MW;
-    const int y0 = strip * TY, y1 = min(Ny1, y0 + TY);  // rows written
-    const int ylo = max(0, y0 - 4), yhi = min(Ny1, y1 + 4);  // rows held
-    const int plo = max(0, y0 - 2), phi = min(Ny1, y1 + 2);  // rows predicted
-    const int W = yhi - ylo;
-    const int rs = W * MW;  // LDS row stride
-    T* psi = reinterpret_cast<T*>(lds_raw);
-    T* pred = psi + kPsiRing * rs;
-    T* Lr = pred + kPredRing * rs;
+    const int rs = Ny1 * MW;  // LDS row stride (elements)
+    T* psi_ring = reinterpret_cast<T*>(lds_raw);  // 4 rows of psi
+    T* pred_ring = psi_ring + 4 * rs;             // 4 rows of psi*
 
-    const int mm = threadIdx.x % MW;
-    const int iy0 = threadIdx.x / MW;
+    const int mm = threadIdx.x % MW, ry = threadIdx.x / MW;
+    const int m0 = blockIdx.x * MW;
     const bool live = m0 + mm < nang;
     const int m = live ? m0 + mm : nang - 1;
     const T wy = omend[2 * m], wz = omend[2 * m + 1];
+    const bool py = wy >= T(0), pz = wz >= T(0);
+    const T hy = py ? T(0.5) : T(-0.5), hz = pz ? T(0.5) : T(-0.5);
+    const bool clip = preserve_sign == 0;
     const long long zstride = static_cast<long long>(Ny1) * nang;
     T* Fg = F + static_cast<long long>(blockIdx.y) * zstride * (Nz + 1) + m0 + mm;
 
-    T nxt[kMaxPerThread];
-    auto fetch = [&](int r) {
+    bool on[P];
+    int goff[P], o0[P], o_near[P], o_far[P], o_down[P];
+    T wpsi[P][5];   // psi rows t-2 .. t+2 (after the commit of step t)
+    T wpred[P][4];  // psi* rows t-3 .. t (after the predictor of step t)
+    T wL[P][3];     // L(psi) rows t-2 .. t
+    T sz[P][2];     // z slopes of psi, rows t-1 and t (before step t)
+    T szp[P][2];    // z slopes of psi*, rows t-3 and t-2 (before step t)
+    T nxt[P];
 #pragma unroll
-        for (int j = 0; j < kMaxPerThread; ++j) {
-            const int iy = ylo + iy0 + j * kYStep;
-            nxt[j] = (live && iy < yhi) ? Fg[r * zstride + static_cast<long long>(iy) * nang] : T(0);
+    for (int j = 0; j < P; ++j) {
+        const int y = ry + j * R;
+        on[j] = y < Ny1;
+        const int yc = min(y, Ny);
+        goff[j] = yc * nang;
+        auto lo = [&](int yy) { return (min(max(yy, 0), Ny)) * MW + mm; };
+        o0[j] = lo(yc);
+        o_near[j] = lo(py ? yc - 1 : yc + 1);
+        o_far[j] = lo(py ? yc - 2 : yc + 2);
+        o_down[j] = lo(py ? yc + 1 : yc - 1);
+        const bool ld = on[j] && live;
+        const T r0 = ld ? Fg[goff[j]] : T(0);
+        const T r1 = ld ? Fg[zstride + goff[j]] : T(0);  // Nz >= 1
+        if (on[j]) {
+            psi_ring[o0[j]] = r0;
+            psi_ring[rs + o0[j]] = r1;
         }
-    };
-    auto commit = [&](int r) {
-        T* dst = psi + (r % kPsiRing) * rs;
+        wpsi[j][0] = wpsi[j][1] = wpsi[j][2] = wpsi[j][3] = r0;
+        wpsi[j][4] = r1;
+        nxt[j] = (ld && Nz >= 2) ? Fg[2 * zstride + goff[j]] : T(0);
+        sz[j][0] = sz[j][1] = T(0);  // boundary slopes are 0 (clamped taps)
+        szp[j][0] = szp[j][1] = T(0);
 #pragma unroll
-        for (int j = 0; j < kMaxPerThread; ++j) {
-            const int ly = iy0 + j * kYStep;
-            if (ylo + ly < yhi) dst[ly * MW + mm] = nxt[j];
-        }
-    };
-    auto clampz = [&](int r) { return min(max(r, 0), Nz); };
-    auto yoff = [&](int iy) { return (min(max(iy, 0), Ny) - ylo) * MW + mm; };
-
-    for (int r = 0; r <= min(Nz, 2); ++r) {
-        fetch(r);
-        commit(r);
+        for (int k = 0; k < 4; ++k) wpred[j][k] = T(0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) wL[j][k] = T(0);
     }
-    for (int t = -2; t <= Nz + 1; ++t) {
-        if (t + 4 >= 3 && t + 4 <= Nz) commit(t + 4);
+
+    for (int t = 0; t <= Nz + 2; ++t) {
+        // commit psi row t+2 (clamped past Nz)
+        const int rc = t + 2;
+        T* crow = psi_ring + (rc & 3) * rs;
+#pragma unroll
+        for (int j = 0; j < P; ++j) {
+            const T v = rc <= Nz ? nxt[j] : wpsi[j][4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) wpsi[j][k] = wpsi[j][k + 1];
+            wpsi[j][4] = v;
+            if (rc <= Nz && on[j]) crow[o0[j]] = v;
+        }
         __syncthreads();
-        if (t + 5 >= 3 && t + 5 <= Nz) fetch(t + 5);
-        const int rp = t + 2;
-        if (rp >= 0 && rp <= Nz) {  // L(psi) and psi* on row rp
-            const T* b0 = psi + (rp % kPsiRing) * rs;
-            const T* bm2 = psi + (clampz(rp - 2) % kPsiRing) * rs;
-            const T* bm1 = psi + (clampz(rp - 1) % kPsiRing) * rs;
-            const T* bp1 = psi + (clampz(rp + 1) % kPsiRing) * rs;
-            const T* bp2 = psi + (clampz(rp + 2) % kPsiRing) * rs;
-            T* pdst = pred + (rp % kPredRing) * rs;
-            T* ldst = Lr + (rp % kLRing) * rs;
-            for (int iy = plo + iy0; iy < phi; iy += kYStep) {
-                const int c = yoff(iy);
-                const T L0 = divergence(bm2, bm1, b0, bp1, bp2, c, yoff(iy - 2), yoff(iy - 1),
-                                        yoff(iy + 1), yoff(iy + 2), wy, wz, inv_dy, inv_dz);
-                const T v = b0[c] - h * L0;
-                pdst[c] = preserve_sign ? v : fmax(v, T(0));
-                ldst[c] = L0;
+        if (t + 3 <= Nz) {
+#pragma unroll
+            for (int j = 0; j < P; ++j)
+                nxt[j] = (on[j] && live) ? Fg[(t + 3) * zstride + goff[j]] : T(0);
+        }
+        if (t <= Nz) {  // predictor on row t
+            const T* row = psi_ring + (t & 3) * rs;
+            T* prow = pred_ring + (t & 3) * rs;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const T zm = wpsi[j][1], zc = wpsi[j][2], zp = wpsi[j][3];
+                const T s_next = limited_slope(wpsi[j][4] - zp, zp - zc);
+                T v = T(0), L0 = T(0);
+                if (on[j]) {
+                    L0 = divergence(row, o_near[j], o_far[j], o_down[j], zm, zc, zp, sz[j][0],
+                                    sz[j][1], s_next, wy, wz, hy, hz, py, pz, inv_dy, inv_dz);
+                    v = zc - h * L0;
+                    v = clip ? fmax(v, T(0)) : v;
+                    prow[o0[j]] = v;
+                }
+                sz[j][0] = sz[j][1];
+                sz[j][1] = s_next;
+                if (t == 0) {
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) wpred[j][k] = v;  // rows < 0 clamp to row 0
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) wpred[j][k] = wpred[j][k + 1];
+                    wpred[j][3] = v;
+                }
+                wL[j][0] = wL[j][1];
+                wL[j][1] = wL[j][2];
+                wL[j][2] = L0;
+            }
+        } else {  // rows past Nz clamp to row Nz
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) wpred[j][k] = wpred[j][k + 1];
+                wL[j][0] = wL[j][1];
+                wL[j][1] = wL[j][2];
             }
         }
-        const int ro = t - 1;
-        if (ro >= 0 && ro <= Nz && live) {  // corrector on row ro, in place
-            const T* b0 = pred + (ro % kPredRing) * rs;
-            const T* bm2 = pred + (clampz(ro - 2) % kPredRing) * rs;
-            const T* bm1 = pred + (clampz(ro - 1) % kPredRing) * rs;
-            const T* bp1 = pred + (clampz(ro + 1) % kPredRing) * rs;
-            const T* bp2 = pred + (clampz(ro + 2) % kPredRing) * rs;
-            const T* lrow = Lr + (ro % kLRing) * rs;
-            const T* prow = psi + (ro % kPsiRing) * rs;
+        if (t >= 2) {  // corrector on row t-2, written in place
+            const int ro = t - 2;
+            const T* row = pred_ring + (ro & 3) * rs;
             T* dst = Fg + ro * zstride;
-            for (int iy = y0 + iy0; iy < y1; iy += kYStep) {
-                const int c = yoff(iy);
-                const T L1 = divergence(bm2, bm1, b0, bp1, bp2, c, yoff(iy - 2), yoff(iy - 1),
-                                        yoff(iy + 1), yoff(iy + 2), wy, wz, inv_dy, inv_dz);
-                const T v = prow[c] - T(0.5) * h * (lrow[c] + L1);
-                dst[static_cast<long long>(iy) * nang] = preserve_sign ? v : fmax(v, T(0));
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const T zm = wpred[j][0], zc = wpred[j][1], zp = wpred[j][2];
+                const T s_next = limited_slope(wpred[j][3] - zp, zp - zc);
+                if (on[j]) {
+                    const T L1 = divergence(row, o_near[j], o_far[j], o_down[j], zm, zc, zp,
+                                            szp[j][0], szp[j][1], s_next, wy, wz, hy, hz, py, pz,
+                                            inv_dy, inv_dz);
+                    T v = wpsi[j][0] - T(0.5) * h * (wL[j][0] + L1);
+                    v = clip ? fmax(v, T(0)) : v;
+                    if (live) dst[goff[j]] = v;
+                }
+                szp[j][0] = szp[j][1];
+                szp[j][1] = s_next;
             }
         }
     }
 }
 
-template <typename T, int MW>
-void launch_march(T* F, const T* omend, int Ny, int Nz, int Ng, int nang, int TY, double dy,
-                  double dz, double h, bool preserve_sign, hipStream_t s) {
-    const int Ny1 = Ny + 1;
-    const int strips = (Ny1 + TY - 1) / TY;
-    const int W = std::min(Ny1, TY + 8);
-    BFPN_REQUIRE((W + kThreads / MW - 1) / (kThreads / MW) <= kMaxPerThread,
-                 "transport strip too wide for the per-thread prefetch");
-    const size_t lds = sizeof(T) * static_cast<size_t>(kRows) * W * MW;
-    BFPN_REQUIRE(lds <= 160 * 1024, "transport strip too wide for the LDS window");
-    BFPN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&transport_march_kernel<T, MW>),
+struct MarchShape {
+    int MW, P, R, threads;
+};
+
+// Widest direction chunk (coalesced rows) whose plane fits one work-group
+// with at most 2 (MW = 32) or 4 (MW = 16, 8) columns per thread and whose two
+// LDS rings fit the 160 KB of a CU.
+MarchShape march_shape(int Ny1, size_t elem) {
+    for (int MW : {32, 16, 8}) {
+        const int rows = kMaxThreads / MW;
+        const int P = (Ny1 + rows - 1) / rows;
+        if (P > (MW == 32 ? 2 : 4)) continue;
+        if (8 * static_cast<size_t>(Ny1) * MW * elem > 160 * 1024) continue;
+        const int R0 = (Ny1 + P - 1) / P;
+        const int threads = (MW * R0 + 63) / 64 * 64;
+        return MarchShape{MW, P, threads / MW, threads};
+    }
+    return MarchShape{0, 0, 0, 0};
+}
+
+template <typename T, int MW, int P>
+void launch_march(T* F, const T* omend, int Ny, int Nz, int Ng, int nang, const MarchShape& sh,
+                  double dy, double dz, double h, bool preserve_sign, hipStream_t s) {
+    const size_t lds = sizeof(T) * 8 * static_cast<size_t>(Ny + 1) * MW;
+    BFPN_REQUIRE(lds <= 160 * 1024, "transport plane too wide for the LDS rings");
+    BFPN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&transport_march_kernel<T, MW, P>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    const int mchunks = (nang + MW - 1) / MW;
-    dim3 grid(strips * mchunks, Ng);
-    hipLaunchKernelGGL((transport_march_kernel<T, MW>), grid, dim3(kThreads), lds, s, F, omend, Ny,
-                       Nz, nang, TY, mchunks, static_cast<T>(1.0 / dy), static_cast<T>(1.0 / dz),
+    const dim3 grid((nang + MW - 1) / MW, Ng);
+    hipLaunchKernelGGL((transport_march_kernel<T, MW, P>), grid, dim3(sh.threads), lds, s, F, omend,
+                       Ny, Nz, nang, sh.R, static_cast<T>(1.0 / dy), static_cast<T>(1.0 / dz),
                        static_cast<T>(h), preserve_sign ? 1 : 0);
+    BFPN_HIP(hipGetLastError());
 }
 
 }  // namespace
@@ -196,15 +252,24 @@ void launch_transport_half(T* F, const T* omend, int Ny, int Nz, int Ng, int nan
                            double dz, double h, bool preserve_sign, hipStream_t s) {
     if (Ng <= 0 || nang <= 0) return;
     BFPN_REQUIRE(Ny >= 1 && Nz >= 1, "transport needs Ny, Nz >= 1");
-    const int Ny1 = Ny + 1;
-    if (Ny1 <= 24) {  // whole plane width, 32 directions per work-group
-        launch_march<T, 32>(F, omend, Ny, Nz, Ng, nang, Ny1, dy, dz, h, preserve_sign, s);
-    } else {  // strips of <= 40 interior nodes (+4 halo each side), 16 directions
-        const int strips = (Ny1 + 39) / 40;
-        const int TY = (Ny1 + strips - 1) / strips;
-        launch_march<T, 16>(F, omend, Ny, Nz, Ng, nang, TY, dy, dz, h, preserve_sign, s);
-    }
-    BFPN_HIP(hipGetLastError());
+    BFPN_REQUIRE(Ng <= 65535, "transport supports at most 65535 energy groups");
+    const MarchShape sh = march_shape(Ny + 1, sizeof(T));
+    BFPN_REQUIRE(sh.MW > 0, sizeof(T) == 4 ? "transport supports at most 512 y nodes in FP32"
+                                           : "transport supports at most 320 y nodes in FP64");
+#define BFPN_MARCH(MW_, P_)                                                                    \
+    if (sh.MW == MW_ && sh.P == P_)                                                            \
+        return launch_march<T, MW_, P_>(F, omend, Ny, Nz, Ng, nang, sh, dy, dz, h, preserve_sign, \
+                                        s)
+    BFPN_MARCH(32, 1);
+    BFPN_MARCH(32, 2);
+    BFPN_MARCH(16, 2);
+    BFPN_MARCH(16, 3);
+    BFPN_MARCH(16, 4);
+    BFPN_MARCH(8, 2);
+    BFPN_MARCH(8, 3);
+    BFPN_MARCH(8, 4);
+#undef BFPN_MARCH
+    throw Error(BFPN_E_INVALID, "no transport launch shape for this plane");
 }
 
 template void launch_transport_half<float>(float*, const float*, int, int, int, int, double, double,

@@ -94,6 +94,29 @@ def test_transport_random_field_large_velocity(gpu, oracle_mod, dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("Ny,Nz,nang", [(1, 1, 3), (2, 5, 40), (20, 3, 33), (31, 4, 7),
+                                        (40, 2, 17), (63, 2, 5), (64, 3, 9), (80, 2, 16),
+                                        (127, 1, 3), (128, 2, 2), (300, 1, 2)])
+def test_transport_plane_shapes(gpu, oracle_mod, Ny, Nz, nang, dtype):
+    """Every launch shape (32/16/8 directions per work-group, 1-4 y columns per
+    thread), plateaus with exact ties (flat-slope branches), both clip modes."""
+    Ng = 2
+    F = RNG.standard_normal(Ng * (Ny + 1) * (Nz + 1) * nang)
+    F[::3] = np.round(F[::3] * 2) / 2
+    F = F.astype(dtype).astype(np.float64)
+    om = RNG.uniform(-0.9, 0.9, 2 * nang)
+    om[:2] = [0.0, -0.5]
+    for preserve in (True, False):
+        ref = F.copy()
+        dF = _dev(gpu, F, dtype)
+        gpu.ops.transport_half(dF, _dev(gpu, om, np.float64), Ny, Nz, Ng, nang, 0.2, 0.3, 0.05,
+                               preserve)
+        oracle_mod.transport_half(ref, om, Ny, Nz, Ng, nang, 0.2, 0.3, 0.05, preserve)
+        _close(dF.numpy().astype(np.float64), ref,
+               TOL64 if dtype == np.float64 else TOL32["transport"])
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("which", [0, 1])
 def test_angle_full(gpu, oracle_mod, physical, dtype, which):
     c, d, st, t = physical
