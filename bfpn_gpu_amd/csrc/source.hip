@@ -23,68 +23,102 @@ namespace bfpn {
 namespace {
 
 // ------------------------------------------------------------ FP32 / MFMA --
-// grid: (ceil(nang/64), ceil(nyz/64), Ng); 4 waves, each a 32x32 output block.
-// Per 32-wide K step the work-group stages a 64(s) x 32(m') tile of both
-// inputs -- read as 128-byte row segments, split into BF16 hi/lo once -- and
-// the matching 64(m) x 32(m') tile of the pre-split transposed Kv in LDS;
-// the waves then read their MFMA fragments as 16-byte LDS loads.
+// Work-groups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Both GEMMs re-read their operands from several work-groups, so block
+// ids are remapped to put the work-groups that share operands on one XCD,
+// consecutively in dispatch order: block b runs logical tile
+//     (b mod 8) * (grid / 8) + b / 8,   grid padded to a multiple of 8.
+__device__ inline int xcd_tile(int b, int grid) { return (b & 7) * (grid >> 3) + (b >> 3); }
+inline int xcd_grid(long long tiles) { return static_cast<int>((tiles + 7) / 8 * 8); }
+
+// Angular GEMM.  Logical tile = (g, 64 transverse nodes, 64 directions); the
+// 7 x 7 tiles of one source group g share X[g] and Kv[g] through one L2.
+// 4 waves, each a 32x32 output block of both inputs.  Per 32-wide K step the
+// work-group stages a 64(s) x 32(m') tile of both inputs -- split into BF16
+// hi/lo once -- and the matching 64(m) x 32(m') tile of the pre-split
+// transposed Kv in LDS; the next step's global loads are in flight while the
+// matrix cores work on the current one.  VEC: nang % 8 == 0 (16-byte loads).
 constexpr int kKPitch = 40;  // bf16 per LDS row (32 + 8 pad)
 
+template <bool VEC>
 __global__ __launch_bounds__(256) void source_angular_mfma(const float* __restrict__ Fp,
                                                            const float* __restrict__ fp,
                                                            const __bf16* __restrict__ kth,
                                                            const __bf16* __restrict__ ktl,
                                                            float* __restrict__ A1,
                                                            float* __restrict__ A2, int nyz,
-                                                           int nang, int nap) {
+                                                           int nang, int nap, int tiles) {
     __shared__ __attribute__((aligned(16))) __bf16 xs[4][64][kKPitch];  // X1 hi/lo, X2 hi/lo
     __shared__ __attribute__((aligned(16))) __bf16 ks[2][64][kKPitch];  // Kv^T hi/lo
-    const int g = blockIdx.z;
+    const int L = xcd_tile(blockIdx.x, gridDim.x);
+    if (L >= tiles) return;
+    const int mtiles = (nang + 63) / 64, stiles = (nyz + 63) / 64;
+    const int g = L / (mtiles * stiles);
+    const int rest = L - g * mtiles * stiles;
+    const int s0 = (rest / mtiles) * 64, m0 = (rest % mtiles) * 64;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = lane & 15, q = lane >> 4;
-    const int s0 = blockIdx.y * 64, m0 = blockIdx.x * 64;
     const int ws = (wave & 1) * 32, wm = (wave >> 1) * 32;  // wave's sub-block
     const long long plane = static_cast<long long>(g) * nyz * nang;
     const long long kt_plane = static_cast<long long>(g) * nap * nap;
     const int lrow = threadIdx.x >> 2, lk = (threadIdx.x & 3) * 8;  // staging: row, 8 columns
+    const int srow = s0 + lrow, mrow = m0 + lrow;
     f32x4 acc1[2][2], acc2[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc1[a][b] = acc2[a][b] = zero4();
 
-    for (int k0 = 0; k0 < nap; k0 += 32) {
-        {  // stage inputs: rows s0+lrow, columns k0+lk .. +7
-            const int srow = s0 + lrow;
-            float v1[8], v2[8];
+    float v1[8], v2[8];  // next K step, in flight
+    bf16x8 kh, kl;
+    auto load = [&](int k0) {
+        const long long base = plane + static_cast<long long>(srow) * nang + k0 + lk;
+        if (VEC) {
+            const bool in = srow < nyz && k0 + lk < nang;
+            float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0, b0 = a0, b1 = a0;
+            if (in) {  // (a select of the loads would take the zero's address)
+                a0 = *reinterpret_cast<const float4*>(Fp + base);
+                a1 = *reinterpret_cast<const float4*>(Fp + base + 4);
+                b0 = *reinterpret_cast<const float4*>(fp + base);
+                b1 = *reinterpret_cast<const float4*>(fp + base + 4);
+            }
+            v1[0] = a0.x; v1[1] = a0.y; v1[2] = a0.z; v1[3] = a0.w;
+            v1[4] = a1.x; v1[5] = a1.y; v1[6] = a1.z; v1[7] = a1.w;
+            v2[0] = b0.x; v2[1] = b0.y; v2[2] = b0.z; v2[3] = b0.w;
+            v2[4] = b1.x; v2[5] = b1.y; v2[6] = b1.z; v2[7] = b1.w;
+        } else {
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const int k = k0 + lk + e;
-                const bool in = srow < nyz && k < nang;
-                const long long idx = plane + static_cast<long long>(srow) * nang + k;
-                const float a = in ? Fp[idx] : 0.f;
-                v1[e] = a > 0.f ? a : 0.f;
-                v2[e] = in ? fp[idx] : 0.f;
+                const bool in = srow < nyz && k0 + lk + e < nang;
+                v1[e] = in ? Fp[base + e] : 0.f;
+                v2[e] = in ? fp[base + e] : 0.f;
             }
-            const SplitFrag f1 = split8(v1), f2 = split8(v2);
+        }
+        if (mrow < nap) {
+            const long long off = kt_plane + static_cast<long long>(mrow) * nap + k0 + lk;
+            kh = *reinterpret_cast<const bf16x8*>(kth + off);
+            kl = *reinterpret_cast<const bf16x8*>(ktl + off);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) kh[e] = kl[e] = static_cast<__bf16>(0.f);
+        }
+    };
+    load(0);
+    for (int k0 = 0; k0 < nap; k0 += 32) {
+        {  // stage: rows s0+lrow / m0+lrow, columns k0+lk .. +7
+            float p1[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) p1[e] = v1[e] > 0.f ? v1[e] : 0.f;
+            const SplitFrag f1 = split8(p1), f2 = split8(v2);
             *reinterpret_cast<bf16x8*>(&xs[0][lrow][lk]) = f1.hi;
             *reinterpret_cast<bf16x8*>(&xs[1][lrow][lk]) = f1.lo;
             *reinterpret_cast<bf16x8*>(&xs[2][lrow][lk]) = f2.hi;
             *reinterpret_cast<bf16x8*>(&xs[3][lrow][lk]) = f2.lo;
-            const int mrow = m0 + lrow;
-            bf16x8 kh, kl;
-            if (mrow < nap) {
-                const long long off = kt_plane + static_cast<long long>(mrow) * nap + k0 + lk;
-                kh = *reinterpret_cast<const bf16x8*>(kth + off);
-                kl = *reinterpret_cast<const bf16x8*>(ktl + off);
-            } else {
-#pragma unroll
-                for (int e = 0; e < 8; ++e) kh[e] = kl[e] = static_cast<__bf16>(0.f);
-            }
             *reinterpret_cast<bf16x8*>(&ks[0][lrow][lk]) = kh;
             *reinterpret_cast<bf16x8*>(&ks[1][lrow][lk]) = kl;
         }
         __syncthreads();
+        if (k0 + 32 < nap) load(k0 + 32);
         SplitFrag x1[2], x2[2], kf[2];
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
@@ -126,17 +160,27 @@ constexpr int kBandLanes = 128;  // lanes per work-group
 constexpr int kBandRows = 64;    // output groups per work-group
 constexpr int kPitch = kBandLanes + 2;
 
-// grid: (ceil(lanes/128), ceil(Ng/64)); Q[g][lane] = E1.A1 + E2.A2 over the band.
+// Banded energy GEMM.  Logical tile = (128 lanes, 64 output groups), the
+// group blocks of one lane block consecutive (one XCD, one L2 for the A rows
+// their bands share).  Q[g][lane] = E1.A1 + E2.A2 over the union of the
+// 16-row tiles' bands; each 16-row tile skips K steps outside its own band.
+// The next K step's A rows are in flight during the current step's MFMAs.
+// VEC: lanes % 4 == 0 (16-byte loads).
+template <bool VEC>
 __global__ __launch_bounds__(256) void source_band_mfma(
     const float* __restrict__ A1, const float* __restrict__ A2, const __bf16* __restrict__ e1h,
     const __bf16* __restrict__ e1l, const __bf16* __restrict__ e2h, const __bf16* __restrict__ e2l,
-    const int2* __restrict__ tile_band, float* __restrict__ Q, int Ng, int ngp, long long lanes) {
+    const int2* __restrict__ tile_band, float* __restrict__ Q, int Ng, int ngp, long long lanes,
+    int tiles) {
     __shared__ float b1s[32][kPitch];
     __shared__ float b2s[32][kPitch];
+    const int L = xcd_tile(blockIdx.x, gridDim.x);
+    if (L >= tiles) return;
+    const int rblocks = (Ng + kBandRows - 1) / kBandRows;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = lane & 15, q = lane >> 4;
-    const int g_base = blockIdx.y * kBandRows;
-    const long long n_base = static_cast<long long>(blockIdx.x) * kBandLanes;
+    const int g_base = (L % rblocks) * kBandRows;
+    const long long n_base = static_cast<long long>(L / rblocks) * kBandLanes;
     const int t0 = g_base / 16;
     int2 band[4];
     int kb = 1 << 30, ke = 0;
@@ -152,18 +196,47 @@ __global__ __launch_bounds__(256) void source_band_mfma(
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) acc[mt][0] = acc[mt][1] = zero4();
 
-    for (int k0 = (kb / 32) * 32; k0 < ke; k0 += 32) {
-        // stage A1/A2 rows [k0, k0+32) x lanes [n_base, n_base+128)
-        for (int e = threadIdx.x; e < 32 * kBandLanes; e += 256) {
-            const int r = e / kBandLanes, c = e - r * kBandLanes;
+    // staging: 32 rows x 128 lanes of A1 and A2; VEC: 4 float4 per thread each
+    float4 r1[4], r2[4];
+    auto load = [&](int k0) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int e4 = threadIdx.x + j * 256;
+            const int r = e4 >> 5, c = (e4 & 31) * 4;
             const int gs = k0 + r;
             const long long ln = n_base + c;
-            const bool in = gs < Ng && ln < lanes;
             const long long idx = static_cast<long long>(gs) * lanes + ln;
-            b1s[r][c] = in ? A1[idx] : 0.f;
-            b2s[r][c] = in ? A2[idx] : 0.f;
+            if (VEC) {
+                r1[j] = r2[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (gs < Ng && ln < lanes) {
+                    r1[j] = *reinterpret_cast<const float4*>(A1 + idx);
+                    r2[j] = *reinterpret_cast<const float4*>(A2 + idx);
+                }
+            } else {
+                float a[4], b[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const bool in = gs < Ng && ln + e < lanes;
+                    a[e] = in ? A1[idx + e] : 0.f;
+                    b[e] = in ? A2[idx + e] : 0.f;
+                }
+                r1[j] = make_float4(a[0], a[1], a[2], a[3]);
+                r2[j] = make_float4(b[0], b[1], b[2], b[3]);
+            }
+        }
+    };
+    const int kstart = (kb / 32) * 32;
+    if (kstart < ke) load(kstart);
+    for (int k0 = kstart; k0 < ke; k0 += 32) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int e4 = threadIdx.x + j * 256;
+            const int r = e4 >> 5, c = (e4 & 31) * 4;
+            b1s[r][c] = r1[j].x; b1s[r][c + 1] = r1[j].y; b1s[r][c + 2] = r1[j].z; b1s[r][c + 3] = r1[j].w;
+            b2s[r][c] = r2[j].x; b2s[r][c + 1] = r2[j].y; b2s[r][c + 2] = r2[j].z; b2s[r][c + 3] = r2[j].w;
         }
         __syncthreads();
+        if (k0 + 32 < ke) load(k0 + 32);
         SplitFrag bf1[2], bf2[2];
 #pragma unroll
         for (int nt = 0; nt < 2; ++nt) {
@@ -359,14 +432,29 @@ void launch_split(const float* src, __bf16* hi, __bf16* lo, long long n, hipStre
 void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, float* Q,
                         const SourcePlan& plan, int Ng, int nyz, int nang, hipStream_t s) {
     const int nap = source_nap(nang);
-    dim3 g1((nang + 63) / 64, (nyz + 63) / 64, Ng);
-    hipLaunchKernelGGL(source_angular_mfma, g1, dim3(256), 0, s, Fp, fp, plan.kth, plan.ktl, A1, A2,
-                       nyz, nang, nap);
+    const long long t1 = static_cast<long long>(Ng) * ((nyz + 63) / 64) * ((nang + 63) / 64);
+    BFPN_REQUIRE(t1 < (1LL << 31) - 8, "source GEMM grid too large");
+    const int g1 = xcd_grid(t1);
+    if (nang % 8 == 0)
+        hipLaunchKernelGGL(source_angular_mfma<true>, dim3(g1), dim3(256), 0, s, Fp, fp, plan.kth,
+                           plan.ktl, A1, A2, nyz, nang, nap, static_cast<int>(t1));
+    else
+        hipLaunchKernelGGL(source_angular_mfma<false>, dim3(g1), dim3(256), 0, s, Fp, fp, plan.kth,
+                           plan.ktl, A1, A2, nyz, nang, nap, static_cast<int>(t1));
+    BFPN_HIP(hipGetLastError());
     const long long lanes = static_cast<long long>(nyz) * nang;
-    dim3 g2(static_cast<unsigned>((lanes + kBandLanes - 1) / kBandLanes),
-            (Ng + kBandRows - 1) / kBandRows);
-    hipLaunchKernelGGL(source_band_mfma, g2, dim3(256), 0, s, A1, A2, plan.e1h, plan.e1l, plan.e2h,
-                       plan.e2l, plan.tile_band, Q, Ng, source_ngp(Ng), lanes);
+    const long long t2 =
+        ((lanes + kBandLanes - 1) / kBandLanes) * ((Ng + kBandRows - 1) / kBandRows);
+    BFPN_REQUIRE(t2 < (1LL << 31) - 8, "source band grid too large");
+    const int g2 = xcd_grid(t2);
+    if (lanes % 4 == 0)
+        hipLaunchKernelGGL(source_band_mfma<true>, dim3(g2), dim3(256), 0, s, A1, A2, plan.e1h,
+                           plan.e1l, plan.e2h, plan.e2l, plan.tile_band, Q, Ng, source_ngp(Ng),
+                           lanes, static_cast<int>(t2));
+    else
+        hipLaunchKernelGGL(source_band_mfma<false>, dim3(g2), dim3(256), 0, s, A1, A2, plan.e1h,
+                           plan.e1l, plan.e2h, plan.e2l, plan.tile_band, Q, Ng, source_ngp(Ng),
+                           lanes, static_cast<int>(t2));
     BFPN_HIP(hipGetLastError());
 }
 
