@@ -31,10 +31,11 @@ namespace {
 //     Y  = X Su^T            (A = X from HBM,   B = Su)
 //     Z  = C (.) (Sv Y)      (A = Sv chained,   B = Y)
 //     W^T = Z^T Sv           (A = Z^T,          B = Sv chained)
-//     X' = W Su              (A = W,            B = Su chained)
-// using the symmetry of the DST-I bases; X' leaves the last stage in the
-// layout of the plane and is stored straight back to HBM.
-template <int MT, int NT>
+//     X'^T = Su^T W^T        (A = Su chained,   B = W)
+// using the symmetry of the DST-I bases; X'^T leaves the last stage with 4
+// consecutive elements of a plane row per lane and is stored straight back.
+// V4: Nom % 4 == 0 -- rows are read and written as 16-byte vectors.
+template <int MT, int NT, bool V4>
 __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                                                          const float* __restrict__ factors,
                                                          AnglePlan plan, int planes, int per_wave,
@@ -82,10 +83,24 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
         for (int mt = 0; mt < MT; ++mt) {
             const int v = mt * 16 + i;
             float x[8];
+            if (V4) {
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int a = 8 * q + e;
-                x[e] = (v < Nmu && a < Nom) ? X[v * Nom + a] : 0.f;
+                for (int h = 0; h < 2; ++h) {
+                    float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
+                    if (v < Nmu && 8 * q + 4 * h < Nom)
+                        t = *reinterpret_cast<const float4*>(X + v * Nom + 8 * q + 4 * h);
+                    x[4 * h] = t.x;
+                    x[4 * h + 1] = t.y;
+                    x[4 * h + 2] = t.z;
+                    x[4 * h + 3] = t.w;
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int a = 8 * q + e;
+                    x[e] = 0.f;
+                    if (v < Nmu && a < Nom) x[e] = X[v * Nom + a];
+                }
             }
             xa[mt] = split8(x);
         }
@@ -126,7 +141,8 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) d3[nt][mt] = mma_split(a, svp[mt], zero4());
         }
-        // stage 4: X'[v][a] = sum_mu W[v][mu] Su[mu][a], stored in place
+        // stage 4, transposed: X'^T[a][v] = sum_mu Su[mu][a] W[v][mu] (operands
+        // swapped), so each lane holds 4 consecutive a of one row v; in place
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
             float w[8];
@@ -135,14 +151,19 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                 w[r] = d3[0][mt][r];
                 w[4 + r] = NT > 1 ? d3[NT - 1][mt][r] : 0.f;
             }
-            const SplitFrag a = split8(w);
+            const SplitFrag b = split8(w);
+            const int v = mt * 16 + i;
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
-                const f32x4 d = mma_split(a, sup[nt], zero4());
+                const f32x4 d = mma_split(sup[nt], b, zero4());
+                const int a0 = nt * 16 + 4 * q;
+                if (V4) {
+                    if (v < Nmu && a0 < Nom)
+                        *reinterpret_cast<float4*>(X + v * Nom + a0) = make_float4(d[0], d[1], d[2], d[3]);
+                } else {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int v = mt * 16 + 4 * q + r, an = nt * 16 + i;
-                    if (v < Nmu && an < Nom) X[v * Nom + an] = d[r];
+                    for (int r = 0; r < 4; ++r)
+                        if (v < Nmu && a0 + r < Nom) X[v * Nom + a0 + r] = d[r];
                 }
             }
         }
@@ -230,13 +251,17 @@ void launch_angle_mfma(float* F, const float* factors, const AnglePlan& plan, in
     const int blocks = (static_cast<int>((planes + per_wave - 1) / per_wave) + 3) / 4;
     const dim3 grid(blocks), block(256);
     const int mt = Nmu > 16 ? 2 : 1, nt = Nom > 16 ? 2 : 1;
-#define BFPN_ANGLE(MT_, NT_)                                                                   \
-    hipLaunchKernelGGL((angle_mfma_kernel<MT_, NT_>), grid, block, 0, s, F, factors, plan,     \
+#define BFPN_ANGLE(MT_, NT_, V4_)                                                              \
+    hipLaunchKernelGGL((angle_mfma_kernel<MT_, NT_, V4_>), grid, block, 0, s, F, factors, plan,\
                        static_cast<int>(planes), per_wave, nyz, Nmu, Nom)
-    if (mt == 2 && nt == 2) BFPN_ANGLE(2, 2);
-    else if (mt == 2) BFPN_ANGLE(2, 1);
-    else if (nt == 2) BFPN_ANGLE(1, 2);
-    else BFPN_ANGLE(1, 1);
+#define BFPN_ANGLE_V(MT_, NT_)                                                                 \
+    if (Nom % 4 == 0) BFPN_ANGLE(MT_, NT_, true);                                              \
+    else BFPN_ANGLE(MT_, NT_, false)
+    if (mt == 2 && nt == 2) { BFPN_ANGLE_V(2, 2); }
+    else if (mt == 2) { BFPN_ANGLE_V(2, 1); }
+    else if (nt == 2) { BFPN_ANGLE_V(1, 2); }
+    else { BFPN_ANGLE_V(1, 1); }
+#undef BFPN_ANGLE_V
 #undef BFPN_ANGLE
     BFPN_HIP(hipGetLastError());
 }

@@ -56,25 +56,14 @@ bfpn_status guarded(F&& body) {
 // They depend only on (g, h, rho, dg, tables), so the host computes them once
 // in FP64 (the reference recomputes them in every thread,
 // src/energy_solver.cu:108-150).  Field names follow that code.
+// Per-group eq15 sweep coefficients.  The new (psi^2, psi^1) = (f, F) of a
+// group are linear in x = (P_hi, R_hi, P, R, P_lo, Q, Delta_hi): old F and f
+// of group g+1, old F and f of g, old F of g-1 (straggling), the source, and
+// the new F - f of g+1; f = sum af[i] x[i], F = sum aF[i] x[i].
 struct EnergyCoef {
-    double inv_h;     // 1/h
-    double half_sig;  // 0.5*sigma
-    double c_up;      // 0.5*rho*S_hi/dg
-    double c_cur;     // 0.5*rho*S_lo/dg
-    double inv_F;     // 1/den_F
-    double A;         // 1.5*rho*S_hi/dg * inv_F
-    double B;         // F_from_f  = 0.5*rho*S_lo/dg * inv_F
-    double Cc;        // F_from_hi = 0.5*rho*S_hi/dg * inv_F
-    double k_up;      // 1.5*rho*S_hi/dg   (also "high2")
-    double k_cur;     // 1.5*rho*S_lo/dg
-    double k_F;       // 1.5*rho*(S_lo+S_hi)/dg
-    double k_f;       // 0.5*rho*(S_hi-S_lo)/dg
-    double H;         // high2 - A*0.5*rho*S_hi/dg
-    double inv_coe;   // 1/max(coe, eps)
-    double a_hi;      // 0.5*rho*T_hi/dg^2   (0 without straggling)
-    double a_lo;      // 0.5*rho*T_lo/dg^2
-    double a_g;       // rho*T_g/dg^2
-    double pad;
+    double af[7];
+    double aF[7];
+    double w1, w2;  // Eq.24 dose weights of the group (when the sweep accumulates the IDD)
 };
 
 // Per-group dose weights: IDD = scale * sum w_yz(s) (psi1*w1[g] + psi2*w2[g]).

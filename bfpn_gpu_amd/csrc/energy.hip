@@ -12,8 +12,10 @@
 //  * group coefficients come from a host-built FP64 table (wave-uniform, so
 //    they are scalar loads) and the recurrence runs in FP64 for any state
 //    type;
-//  * loads are software-prefetched PF groups ahead to hide HBM latency
-//    behind the serial recurrence;
+//  * loads run 16 groups (FP64 state: 8) ahead in a register ring to hide HBM
+//    latency behind the serial recurrence (small grids leave only a few
+//    waves per SIMD); per-group coefficients, dose weights included, come
+//    through the scalar cache so they never queue behind the ring;
 //  * the final E-half of a depth step also accumulates the Eq.24 dose and the
 //    psi^2 proxy in FP64 (block partials, reduced deterministically later), so
 //    the IDD costs no extra pass over the state.
@@ -24,7 +26,6 @@ namespace bfpn {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kPrefetch = 4;
 
 __device__ inline double block_sum(double v, double* lds) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -38,80 +39,101 @@ __device__ inline double block_sum(double v, double* lds) {
     return t;
 }
 
+// Prefetch depth (groups in flight per lane): a register ring, slot k
+// refilled with group g - D right after group g consumed it.
+template <typename T>
+constexpr int prefetch_depth() { return sizeof(T) == 4 ? 16 : 8; }
+
+// Everything a group needs besides the state (recurrence coefficients and the
+// dose weights) sits in one EnergyCoef read through the scalar cache, so the
+// vector loads in flight are the state ring alone: the in-order load counter
+// then never waits on the ring to deliver a coefficient.
 template <typename T, bool SRC, bool STRAG, bool DOSE>
 __global__ __launch_bounds__(kBlock) void energy_half_kernel(
     T* __restrict__ F, T* __restrict__ f, const EnergyCoef* __restrict__ coef,
-    const T* __restrict__ Q1, int Ng, long long lanes, const DoseCoef* __restrict__ dw,
-    const double* __restrict__ yzw, int nang, double* __restrict__ partials, int proxy) {
+    const T* __restrict__ Q1, int Ng, int lanes, const double* __restrict__ yzw, int nang,
+    double* __restrict__ partials, int proxy) {
+    constexpr int D = prefetch_depth<T>();
     __shared__ double lds[kBlock / 64];
-    const long long lane = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    const int lane = blockIdx.x * kBlock + threadIdx.x;
     double acc_dose = 0.0, acc_proxy = 0.0;
     if (lane < lanes) {
-        T* Fl = F + lane;
-        T* fl = f + lane;
-        const T* Ql = SRC ? Q1 + lane : nullptr;
-        const double wyz = DOSE ? yzw[lane / nang] : 0.0;
-
-        T cF[kPrefetch], cf[kPrefetch], cQ[kPrefetch];
+        T rF[D], rf[D], rQ[D];
+        // Loads are unconditional (groups below 0 re-read group 0 and are
+        // never consumed) and full chunks run without branches, so the
+        // compiler can count the loads in flight instead of draining them.
+        // Row bases are wave-uniform (scalar), the lane a 32-bit offset.
+        auto fetch = [&](int k, int g) {
+            const long long base = static_cast<long long>(max(g, 0)) * lanes;
+            rF[k] = (F + base)[lane];
+            rf[k] = (f + base)[lane];
+            if (SRC) rQ[k] = (Q1 + base)[lane];
+        };
 #pragma unroll
-        for (int k = 0; k < kPrefetch; ++k) {
-            const int g = Ng - 1 - k;
-            cF[k] = g >= 0 ? Fl[g * lanes] : T(0);
-            cf[k] = g >= 0 ? fl[g * lanes] : T(0);
-            if (SRC) cQ[k] = g >= 0 ? Ql[g * lanes] : T(0);
-        }
+        for (int k = 0; k < D; ++k) fetch(k, Ng - 1 - k);
         double P_hi = 0.0, R_hi = 0.0, hi_delta = 0.0;
-        for (int g0 = Ng - 1; g0 >= 0; g0 -= kPrefetch) {
-            T nF[kPrefetch], nf[kPrefetch], nQ[kPrefetch];
+        // The linear forms run one group ahead through the scalar cache:
+        // group g-1's are requested before group g's arithmetic (the
+        // scheduling barrier keeps the request there), so their latency is
+        // covered.  (Dose weights are read at use: two sets of forms plus
+        // them would not fit the scalar registers.)
+        struct Forms {
+            double af[7], aF[7];
+        };
+        auto forms = [&](int g) {
+            Forms r;
 #pragma unroll
-            for (int k = 0; k < kPrefetch; ++k) {
-                const int g = g0 - kPrefetch - k;
-                nF[k] = g >= 0 ? Fl[g * lanes] : T(0);
-                nf[k] = g >= 0 ? fl[g * lanes] : T(0);
-                if (SRC) nQ[k] = g >= 0 ? Ql[g * lanes] : T(0);
+            for (int i = 0; i < 7; ++i) {
+                r.af[i] = coef[g].af[i];
+                r.aF[i] = coef[g].aF[i];
             }
+            return r;
+        };
+        Forms cn = forms(Ng - 1);
+        auto step = [&](int k, int g) {
+            const Forms c = cn;
+            cn = forms(max(g - 1, 0));
+            __builtin_amdgcn_sched_barrier(0);
+            const double P = static_cast<double>(rF[k]);
+            const double R = static_cast<double>(rf[k]);
+            // old F at g-1: next ring slot (slot 0 was refilled with g0-D)
+            const double P_lo = (STRAG && g > 0) ? static_cast<double>(rF[(k + 1) % D]) : 0.0;
+            const double src = SRC ? static_cast<double>(rQ[k]) : 0.0;
+            fetch(k, g - D);
+            double fv = c.af[0] * P_hi + c.af[1] * R_hi + c.af[2] * P + c.af[3] * R;
+            double Fv = c.aF[0] * P_hi + c.aF[1] * R_hi + c.aF[2] * P + c.aF[3] * R;
+            if (STRAG) {
+                fv += c.af[4] * P_lo;
+                Fv += c.aF[4] * P_lo;
+            }
+            if (SRC) {
+                fv += c.af[5] * src;
+                Fv += c.aF[5] * src;
+            }
+            fv += c.af[6] * hi_delta;
+            Fv += c.aF[6] * hi_delta;
+            const long long base = static_cast<long long>(g) * lanes;
+            (F + base)[lane] = static_cast<T>(Fv);
+            (f + base)[lane] = static_cast<T>(fv);
+            if (DOSE) {  // the lane's transverse weight is applied after the sweep
+                acc_dose += Fv * coef[g].w1 + fv * coef[g].w2;
+                acc_proxy += fv;
+            }
+            P_hi = P;
+            R_hi = R;
+            hi_delta = Fv - fv;
+        };
+        int g0 = Ng - 1;
+        for (; g0 >= D - 1; g0 -= D) {
 #pragma unroll
-            for (int k = 0; k < kPrefetch; ++k) {
-                const int g = g0 - k;
-                if (g >= 0) {
-                    const EnergyCoef& c = coef[g];
-                    const double P = static_cast<double>(cF[k]);
-                    const double R = static_cast<double>(cf[k]);
-                    const double up = P_hi - R_hi;
-                    const double cur = P - R;
-                    double strag = 0.0;
-                    if (STRAG) {
-                        const double P_lo =
-                            g > 0 ? static_cast<double>(k + 1 < kPrefetch ? cF[k + 1] : nF[0]) : 0.0;
-                        strag = c.a_hi * P_hi + c.a_lo * P_lo - c.a_g * P;
-                    }
-                    const double src = SRC ? static_cast<double>(cQ[k]) : 0.0;
-                    const double common =
-                        c.c_up * up - c.c_cur * cur - c.half_sig * P + P * c.inv_h + strag + src;
-                    const double rhs4 = c.inv_F * common;
-                    const double rhs1e = R * c.inv_h - c.half_sig * R;
-                    const double rhs2e = c.k_up * up + c.k_cur * cur - c.k_F * P - c.k_f * R;
-                    const double rhs = rhs1e + rhs2e - c.A * common + c.H * hi_delta;
-                    const double fv = rhs * c.inv_coe;
-                    const double Fv = c.B * fv + rhs4 + c.Cc * hi_delta;
-                    Fl[g * lanes] = static_cast<T>(Fv);
-                    fl[g * lanes] = static_cast<T>(fv);
-                    if (DOSE) {
-                        acc_dose += wyz * (Fv * dw[g].w1 + fv * dw[g].w2);
-                        acc_proxy += fv;
-                    }
-                    P_hi = P;
-                    R_hi = R;
-                    hi_delta = Fv - fv;
-                }
-            }
-#pragma unroll
-            for (int k = 0; k < kPrefetch; ++k) {
-                cF[k] = nF[k];
-                cf[k] = nf[k];
-                if (SRC) cQ[k] = nQ[k];
-            }
+            for (int k = 0; k < D; ++k) step(k, g0 - k);
         }
+#pragma unroll
+        for (int k = 0; k < D; ++k)  // tail: groups g0 .. 0 (fewer than D)
+            if (k <= g0) step(k, g0 - k);
+        // (loaded only now: a load issued before the ring would hold up its
+        // first consumer behind every state load in flight)
+        if (DOSE) acc_dose *= yzw[lane / nang];
     }
     if (DOSE) {
         const double d = block_sum(acc_dose, lds);
@@ -125,15 +147,16 @@ __global__ __launch_bounds__(kBlock) void energy_half_kernel(
 
 template <typename T, bool SRC, bool STRAG>
 void dispatch_dose(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng, long long lanes,
-                   const DoseCoef* dw, const double* yzw, int nang, double* partials, bool proxy,
-                   hipStream_t s) {
+                   const double* yzw, int nang, double* partials, bool proxy, hipStream_t s) {
     const int blocks = energy_blocks(lanes);
+    BFPN_REQUIRE(lanes < (1LL << 31), "energy sweep indexes lanes with 32-bit offsets");
     if (partials)
         hipLaunchKernelGGL((energy_half_kernel<T, SRC, STRAG, true>), dim3(blocks), dim3(kBlock), 0,
-                           s, F, f, coef, Q1, Ng, lanes, dw, yzw, nang, partials, proxy ? 1 : 0);
+                           s, F, f, coef, Q1, Ng, static_cast<int>(lanes), yzw, nang, partials,
+                           proxy ? 1 : 0);
     else
         hipLaunchKernelGGL((energy_half_kernel<T, SRC, STRAG, false>), dim3(blocks), dim3(kBlock), 0,
-                           s, F, f, coef, Q1, Ng, lanes, dw, yzw, nang, partials, 0);
+                           s, F, f, coef, Q1, Ng, static_cast<int>(lanes), yzw, nang, partials, 0);
 }
 
 }  // namespace
@@ -142,32 +165,32 @@ int energy_blocks(long long lanes) { return static_cast<int>((lanes + kBlock - 1
 
 template <typename T>
 void launch_energy_half(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng, long long lanes,
-                        bool straggling, const DoseCoef* dose_coef, const double* yz_weight,
-                        int nang, double* dose_partials, bool proxy, hipStream_t s) {
+                        bool straggling, const double* yz_weight, int nang, double* dose_partials,
+                        bool proxy, hipStream_t s) {
     if (lanes <= 0 || Ng <= 0) return;
     if (Q1) {
         if (straggling)
-            dispatch_dose<T, true, true>(F, f, coef, Q1, Ng, lanes, dose_coef, yz_weight, nang,
-                                         dose_partials, proxy, s);
+            dispatch_dose<T, true, true>(F, f, coef, Q1, Ng, lanes, yz_weight, nang, dose_partials,
+                                         proxy, s);
         else
-            dispatch_dose<T, true, false>(F, f, coef, Q1, Ng, lanes, dose_coef, yz_weight, nang,
-                                          dose_partials, proxy, s);
+            dispatch_dose<T, true, false>(F, f, coef, Q1, Ng, lanes, yz_weight, nang, dose_partials,
+                                          proxy, s);
     } else {
         if (straggling)
-            dispatch_dose<T, false, true>(F, f, coef, Q1, Ng, lanes, dose_coef, yz_weight, nang,
-                                          dose_partials, proxy, s);
+            dispatch_dose<T, false, true>(F, f, coef, Q1, Ng, lanes, yz_weight, nang, dose_partials,
+                                          proxy, s);
         else
-            dispatch_dose<T, false, false>(F, f, coef, Q1, Ng, lanes, dose_coef, yz_weight, nang,
+            dispatch_dose<T, false, false>(F, f, coef, Q1, Ng, lanes, yz_weight, nang,
                                            dose_partials, proxy, s);
     }
     BFPN_HIP(hipGetLastError());
 }
 
 template void launch_energy_half<float>(float*, float*, const EnergyCoef*, const float*, int,
-                                        long long, bool, const DoseCoef*, const double*, int,
-                                        double*, bool, hipStream_t);
+                                        long long, bool, const double*, int, double*, bool,
+                                        hipStream_t);
 template void launch_energy_half<double>(double*, double*, const EnergyCoef*, const double*, int,
-                                         long long, bool, const DoseCoef*, const double*, int,
-                                         double*, bool, hipStream_t);
+                                         long long, bool, const double*, int, double*, bool,
+                                         hipStream_t);
 
 }  // namespace bfpn

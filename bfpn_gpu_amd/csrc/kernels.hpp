@@ -12,7 +12,7 @@ namespace bfpn {
 // The lanes handled by a block are [block*256, block*256+256).
 template <typename T>
 void launch_energy_half(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng, long long lanes,
-                        bool straggling, const DoseCoef* dose_coef, const double* yz_weight,
+                        bool straggling, const double* yz_weight,
                         int nang, double* dose_partials, bool proxy, hipStream_t s);
 int energy_blocks(long long lanes);
 
@@ -25,6 +25,10 @@ struct SourcePlan {
     const __bf16* e2h = nullptr;  // dg*K2/3
     const __bf16* e2l = nullptr;
     const int2* tile_band = nullptr;  // [min beg, max end) per 16-group tile
+    // Kv block sparsity of the angular GEMM: per (g, 64-direction tile) the
+    // 32-wide K tiles holding a nonzero entry, ascending.
+    const int* kt_count = nullptr;    // [Ng][mtiles]
+    const short* kt_list = nullptr;   // [Ng][mtiles][ktiles]
 };
 int source_nap(int nang);    // nang padded to 32
 int source_ngp(int Ng);      // Ng padded to 32 (E columns)
@@ -32,6 +36,11 @@ int source_erows(int Ng);    // Ng padded to 64 (E rows); tile_band has erows/16
 void launch_split_transpose(const float* kv, __bf16* hi, __bf16* lo, int Ng, int nang,
                             hipStream_t s);
 void launch_split(const float* src, __bf16* hi, __bf16* lo, long long n, hipStream_t s);
+// flags[g][mt][kt] = 1 if the 64 x 32 block (directions mt, K tile kt) of the
+// split transposed Kv holds a nonzero entry.
+void launch_kv_block_flags(const __bf16* kth, unsigned char* flags, int Ng, int nang, hipStream_t s);
+int source_mtiles(int nang);  // 64-direction output tiles of the angular GEMM
+int source_ktiles(int nang);  // 32-wide K tiles
 // FP32 source on the matrix cores: A1/A2 scratch, Q output.
 void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, float* Q,
                         const SourcePlan& plan, int Ng, int nyz, int nang, hipStream_t s);

@@ -29,16 +29,25 @@ __device__ inline void split_bf16(float x, __bf16& hi, __bf16& lo) {
     lo = static_cast<__bf16>(x - static_cast<float>(hi));
 }
 
-// Splits 8 FP32 values into a hi/lo BF16 fragment.
+// Splits 8 FP32 values into a hi/lo BF16 fragment (the same bits as
+// split_bf16 per value).  hi is converted once, in pairs; its FP32 values are
+// read back from the packed words (low half << 16, high half & 0xffff0000).
 __device__ inline SplitFrag split8(const float (&v)[8]) {
-    SplitFrag f;
+    typedef float f32x8 __attribute__((ext_vector_type(8)));
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    f32x8 x;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        __bf16 h, l;
-        split_bf16(v[e], h, l);
-        f.hi[e] = h;
-        f.lo[e] = l;
+    for (int e = 0; e < 8; ++e) x[e] = v[e];
+    SplitFrag f;
+    f.hi = __builtin_convertvector(x, bf16x8);
+    const u32x4 w = __builtin_bit_cast(u32x4, f.hi);
+    f32x8 h;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        h[2 * k] = __uint_as_float(w[k] << 16);
+        h[2 * k + 1] = __uint_as_float(w[k] & 0xffff0000u);
     }
+    f.lo = __builtin_convertvector(x - h, bf16x8);
     return f;
 }
 

@@ -63,6 +63,22 @@ void SourcePlanBuffers::build(const float* kv_f32, const std::vector<double>& e1
     kth.ensure(kt * sizeof(__bf16));
     ktl.ensure(kt * sizeof(__bf16));
     launch_split_transpose(kv_f32, kth.as<__bf16>(), ktl.as<__bf16>(), Ng, nang, s);
+    {  // nonzero K tiles per (g, direction tile)
+        const int mtiles = source_mtiles(nang), ktiles = source_ktiles(nang);
+        const size_t nb = static_cast<size_t>(Ng) * mtiles * ktiles;
+        scratch.ensure(nb);
+        launch_kv_block_flags(kth.as<__bf16>(), scratch.as<unsigned char>(), Ng, nang, s);
+        std::vector<unsigned char> flags(nb);
+        BFPN_HIP(hipMemcpyAsync(flags.data(), scratch.as<void>(), nb, hipMemcpyDeviceToHost, s));
+        BFPN_HIP(hipStreamSynchronize(s));
+        std::vector<int> count(static_cast<size_t>(Ng) * mtiles, 0);
+        std::vector<short> list(nb, 0);
+        for (size_t b = 0; b < count.size(); ++b)
+            for (int k = 0; k < ktiles; ++k)
+                if (flags[b * ktiles + k]) list[b * ktiles + count[b]++] = static_cast<short>(k);
+        upload(kt_count, count);
+        upload(kt_list, list);
+    }
     const size_t ne = static_cast<size_t>(erows) * ngp;
     std::vector<float> E1(ne, 0.f), E2(ne, 0.f);
     for (int r = 0; r < Ng; ++r)
@@ -97,6 +113,8 @@ void SourcePlanBuffers::build(const float* kv_f32, const std::vector<double>& e1
     plan.e2h = e2h.as<__bf16>();
     plan.e2l = e2l.as<__bf16>();
     plan.tile_band = band.as<int2>();
+    plan.kt_count = kt_count.as<int>();
+    plan.kt_list = kt_list.as<short>();
 }
 
 void AnglePlanBuffers::build(int Nmu, int Nom, hipStream_t s) {
@@ -179,9 +197,15 @@ struct Physics final : PhysicsBase {
         hipStream_t s = nullptr;
         BFPN_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         try {
-            upload(coef, energy_coefs(S, mat.sigma, Tc.data(), p.eq15_straggling != 0, 0.5 * d.dt,
-                                      d.dg, p.energy_density, g.Ng));
-            upload(dose_w, dose_coefs(S, mat.sigma, en, d.dg, p.dose_catastrophic_loss != 0, g.Ng));
+            auto ec = energy_coefs(S, mat.sigma, Tc.data(), p.eq15_straggling != 0, 0.5 * d.dt, d.dg,
+                                   p.energy_density, g.Ng);
+            const auto dc = dose_coefs(S, mat.sigma, en, d.dg, p.dose_catastrophic_loss != 0, g.Ng);
+            for (int k = 0; k < g.Ng; ++k) {  // the closing E-half of a step accumulates the IDD
+                ec[k].w1 = dc[k].w1;
+                ec[k].w2 = dc[k].w2;
+            }
+            upload(coef, ec);
+            upload(dose_w, dc);
             upload(yzw, yz_weights(g.Ny, g.Nz, p.trapezoidal_yz != 0));
             upload(om_dev, cast_vector<T>(om));
             upload(factors, cast_vector<T>(angle_factors(sig_tr.data(), g.Ng, g.Nmu, g.Nom, d.du,
@@ -348,8 +372,8 @@ private:
     void energy(int species, bool with_source, double* partials, bool proxy) {
         launch_energy_half<T>(cur_[2 * species], cur_[2 * species + 1], P_->coef.template as<EnergyCoef>(),
                               with_source ? q_.as<T>() : nullptr, grid.Ng, P_->lanes,
-                              phys.eq15_straggling != 0, P_->dose_w.template as<DoseCoef>(),
-                              P_->yzw.template as<double>(), P_->nang, partials, proxy, stream);
+                              phys.eq15_straggling != 0, P_->yzw.template as<double>(), P_->nang,
+                              partials, proxy, stream);
         mark(kEnergy, array_bytes() * (with_source ? 5 : 4));
     }
 
@@ -515,8 +539,8 @@ void op_energy(bfpn_workspace* ws, T* F, T* f, const double* S, const double* si
     if (Tc) hT = to_host(Tc, Ng);
     upload(ws->coef, energy_coefs(hS, hsig, Tc ? hT.data() : nullptr, straggling != 0, h, dg, rho, Ng));
     launch_energy_half<T>(F, f, ws->coef.as<EnergyCoef>(), Q1, Ng,
-                          static_cast<long long>(nyz) * nang, straggling != 0 && Tc, nullptr,
-                          nullptr, nang, nullptr, false, s);
+                          static_cast<long long>(nyz) * nang, straggling != 0 && Tc, nullptr, nang,
+                          nullptr, false, s);
 }
 
 template <typename T>

@@ -33,6 +33,8 @@ inline int xcd_grid(long long tiles) { return static_cast<int>((tiles + 7) / 8 *
 
 // Angular GEMM.  Logical tile = (g, 64 transverse nodes, 64 directions); the
 // 7 x 7 tiles of one source group g share X[g] and Kv[g] through one L2.
+// Kv is thresholded (about half of its 64 x 32 blocks are zero at 20 x 20
+// directions), so each tile visits only its nonzero K tiles.
 // 4 waves, each a 32x32 output block of both inputs.  Per 32-wide K step the
 // work-group stages a 64(s) x 32(m') tile of both inputs -- split into BF16
 // hi/lo once -- and the matching 64(m) x 32(m') tile of the pre-split
@@ -45,6 +47,8 @@ __global__ __launch_bounds__(256) void source_angular_mfma(const float* __restri
                                                            const float* __restrict__ fp,
                                                            const __bf16* __restrict__ kth,
                                                            const __bf16* __restrict__ ktl,
+                                                           const int* __restrict__ kt_count,
+                                                           const short* __restrict__ kt_list,
                                                            float* __restrict__ A1,
                                                            float* __restrict__ A2, int nyz,
                                                            int nang, int nap, int tiles) {
@@ -103,8 +107,13 @@ __global__ __launch_bounds__(256) void source_angular_mfma(const float* __restri
             for (int e = 0; e < 8; ++e) kh[e] = kl[e] = static_cast<__bf16>(0.f);
         }
     };
-    load(0);
-    for (int k0 = 0; k0 < nap; k0 += 32) {
+    // only the K tiles where this direction tile's Kv block is nonzero (the
+    // skipped products are exact zeros)
+    const int mt_idx = m0 / 64, ktiles = nap / 32;
+    const int nk = kt_count[g * mtiles + mt_idx];
+    const short* kl_ = kt_list + static_cast<long long>(g * mtiles + mt_idx) * ktiles;
+    if (nk > 0) load(kl_[0] * 32);
+    for (int j = 0; j < nk; ++j) {
         {  // stage: rows s0+lrow / m0+lrow, columns k0+lk .. +7
             float p1[8];
 #pragma unroll
@@ -118,7 +127,7 @@ __global__ __launch_bounds__(256) void source_angular_mfma(const float* __restri
             *reinterpret_cast<bf16x8*>(&ks[1][lrow][lk]) = kl;
         }
         __syncthreads();
-        if (k0 + 32 < nap) load(k0 + 32);
+        if (j + 1 < nk) load(kl_[j + 1] * 32);
         SplitFrag x1[2], x2[2], kf[2];
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
@@ -280,6 +289,22 @@ __global__ __launch_bounds__(256) void source_band_mfma(
         }
 }
 
+// grid: (ktiles, mtiles, Ng)
+__global__ __launch_bounds__(256) void kv_block_flags_kernel(const __bf16* __restrict__ kth,
+                                                             unsigned char* __restrict__ flags,
+                                                             int nap) {
+    const int kt = blockIdx.x, mt = blockIdx.y, g = blockIdx.z;
+    const unsigned short* k = reinterpret_cast<const unsigned short*>(kth);
+    int nz = 0;
+    for (int e = threadIdx.x; e < 64 * 32; e += 256) {
+        const int r = mt * 64 + (e >> 5), c = kt * 32 + (e & 31);
+        if (r < nap) nz |= (k[(static_cast<long long>(g) * nap + r) * nap + c] & 0x7fff) != 0;
+    }
+    nz = __syncthreads_or(nz);
+    if (threadIdx.x == 0)
+        flags[(static_cast<long long>(g) * gridDim.y + mt) * gridDim.x + kt] = nz ? 1 : 0;
+}
+
 // Kv[g][in][out] (FP32) -> transposed split copies KT[g][out][in] (BF16 hi/lo),
 // zero padded to nap x nap.  grid: (ceil(nap*nap/256), Ng)
 __global__ void split_transpose_kernel(const float* __restrict__ kv, __bf16* __restrict__ hi,
@@ -411,6 +436,15 @@ __global__ __launch_bounds__(256) void source_energy_kernel(
 }  // namespace
 
 int source_nap(int nang) { return (nang + 31) / 32 * 32; }
+int source_mtiles(int nang) { return (nang + 63) / 64; }
+int source_ktiles(int nang) { return source_nap(nang) / 32; }
+
+void launch_kv_block_flags(const __bf16* kth, unsigned char* flags, int Ng, int nang,
+                           hipStream_t s) {
+    const dim3 grid(source_ktiles(nang), source_mtiles(nang), Ng);
+    hipLaunchKernelGGL(kv_block_flags_kernel, grid, dim3(256), 0, s, kth, flags, source_nap(nang));
+    BFPN_HIP(hipGetLastError());
+}
 int source_ngp(int Ng) { return (Ng + 31) / 32 * 32; }
 int source_erows(int Ng) { return (Ng + kBandRows - 1) / kBandRows * kBandRows; }
 
@@ -437,10 +471,12 @@ void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, 
     const int g1 = xcd_grid(t1);
     if (nang % 8 == 0)
         hipLaunchKernelGGL(source_angular_mfma<true>, dim3(g1), dim3(256), 0, s, Fp, fp, plan.kth,
-                           plan.ktl, A1, A2, nyz, nang, nap, static_cast<int>(t1));
+                           plan.ktl, plan.kt_count, plan.kt_list, A1, A2, nyz, nang, nap,
+                           static_cast<int>(t1));
     else
         hipLaunchKernelGGL(source_angular_mfma<false>, dim3(g1), dim3(256), 0, s, Fp, fp, plan.kth,
-                           plan.ktl, A1, A2, nyz, nang, nap, static_cast<int>(t1));
+                           plan.ktl, plan.kt_count, plan.kt_list, A1, A2, nyz, nang, nap,
+                           static_cast<int>(t1));
     BFPN_HIP(hipGetLastError());
     const long long lanes = static_cast<long long>(nyz) * nang;
     const long long t2 =

@@ -367,14 +367,19 @@ BeamFactors beam_factors(const bfpn_grid& g, const bfpn_phys& p, const bfpn_delt
 }
 
 // Host precompute of the per-group eq15 sweep coefficients
-// (src/energy_solver.cu:79-86, 114-150).
+// (src/energy_solver.cu:79-86, 114-150), folded into linear forms.  With
+//   up = P_hi - R_hi, cur = P - R, strag = a_hi P_hi + a_lo P_lo - a_g P,
+//   common = c_up up - c_cur cur - sig/2 P + P/h + strag + Q,
+//   rhs = (1/h - sig/2) R + k_up up + k_cur cur - k_F P - k_f R - A common + H Delta,
+//   f = rhs / coe,   F = B f + common / den_F + Cc Delta
+// (the reference's sequence of operations), f and F are linear in
+// x = (P_hi, R_hi, P, R, P_lo, Q, Delta); the sweep evaluates the two forms.
 std::vector<EnergyCoef> energy_coefs(const std::vector<double>& S, const std::vector<double>& sigma,
                                      const double* T, bool straggling, double h, double dg,
                                      double rho, int Ng) {
     std::vector<EnergyCoef> out(Ng);
     const double dg2 = std::max(dg * dg, kEps);
     for (int g = 0; g < Ng; ++g) {
-        EnergyCoef& c = out[g];
         const double sig = std::max(sigma[g], 0.0);
         const double Slo = std::max(S[g], 0.0);
         const double Shi = std::max(S[g + 1], 0.0);
@@ -382,26 +387,36 @@ std::vector<EnergyCoef> energy_coefs(const std::vector<double>& S, const std::ve
         const double Tg = st ? std::max(T[g], 0.0) : 0.0;
         const double Thi = st && g + 1 < Ng ? std::max(T[g + 1], 0.0) : 0.0;
         const double Tlo = st && g > 0 ? std::max(T[g - 1], 0.0) : 0.0;
-        c.inv_h = 1.0 / h;
-        c.half_sig = 0.5 * sig;
-        c.c_up = 0.5 * rho * Shi / dg;
-        c.c_cur = 0.5 * rho * Slo / dg;
-        c.inv_F = 1.0 / std::max(1.0 / h + 0.5 * rho * Slo / dg + 0.5 * sig, kEps);
-        c.A = (1.5 * rho * Shi / dg) * c.inv_F;
-        c.B = (0.5 * rho * Slo / dg) * c.inv_F;
-        c.Cc = (0.5 * rho * Shi / dg) * c.inv_F;
-        c.k_up = 1.5 * rho * Shi / dg;
-        c.k_cur = 1.5 * rho * Slo / dg;
-        c.k_F = 1.5 * rho * (Slo + Shi) / dg;
-        c.k_f = 0.5 * rho * (Shi - Slo) / dg;
-        c.H = c.k_up - c.A * 0.5 * rho * Shi / dg;
+        const double inv_h = 1.0 / h, half_sig = 0.5 * sig;
+        const double c_up = 0.5 * rho * Shi / dg, c_cur = 0.5 * rho * Slo / dg;
+        const double inv_F = 1.0 / std::max(1.0 / h + 0.5 * rho * Slo / dg + 0.5 * sig, kEps);
+        const double A = (1.5 * rho * Shi / dg) * inv_F;
+        const double B = (0.5 * rho * Slo / dg) * inv_F;
+        const double Cc = (0.5 * rho * Shi / dg) * inv_F;
+        const double k_up = 1.5 * rho * Shi / dg, k_cur = 1.5 * rho * Slo / dg;
+        const double k_F = 1.5 * rho * (Slo + Shi) / dg, k_f = 0.5 * rho * (Shi - Slo) / dg;
+        const double H = k_up - A * 0.5 * rho * Shi / dg;
         const double coe = 1.0 / h + 0.5 * sig + 1.5 * rho * Slo / dg + 0.5 * rho * (Shi - Slo) / dg +
-                           c.A * 0.5 * rho * Slo / dg;
-        c.inv_coe = 1.0 / std::max(coe, kEps);
-        c.a_hi = 0.5 * rho * Thi / dg2;
-        c.a_lo = 0.5 * rho * Tlo / dg2;
-        c.a_g = rho * Tg / dg2;
-        c.pad = 0.0;
+                           A * 0.5 * rho * Slo / dg;
+        const double inv_coe = 1.0 / std::max(coe, kEps);
+        const double a_hi = 0.5 * rho * Thi / dg2, a_lo = 0.5 * rho * Tlo / dg2, a_g = rho * Tg / dg2;
+        // common = sum cm[i] x[i] (no Delta term)
+        const double cm[6] = {c_up + a_hi, -c_up, inv_h - half_sig - c_cur - a_g, c_cur, a_lo, 1.0};
+        // rhs = sum r[i] x[i] + H Delta
+        const double r[6] = {k_up - A * cm[0],
+                             -k_up - A * cm[1],
+                             (k_cur - k_F) - A * cm[2],
+                             (inv_h - half_sig - k_cur - k_f) - A * cm[3],
+                             -A * cm[4],
+                             -A * cm[5]};
+        EnergyCoef& c = out[g];
+        for (int i = 0; i < 6; ++i) {
+            c.af[i] = inv_coe * r[i];
+            c.aF[i] = B * c.af[i] + inv_F * cm[i];
+        }
+        c.af[6] = inv_coe * H;
+        c.aF[6] = B * c.af[6] + Cc;
+        c.w1 = c.w2 = 0.0;
     }
     return out;
 }

@@ -18,8 +18,8 @@
 //    forms L(psi) and psi* = clip(psi - h L psi) on row t, and writes row t-2:
 //    clip(psi - h/2 (L psi + L psi*)).  One barrier per z step.
 //  * The flow direction is fixed per lane, so only the upwind reconstruction
-//    is formed: per axis and stage two limited slopes in y (the cell's own
-//    and its upwind neighbour's) and one in z.
+//    is formed, in the lane's upwind frame: per stage two limited slopes in y
+//    (the cell's own and its upwind neighbour's) and one in z.
 //  * The superbee slope is evaluated without a division:
 //      a = R - C, b = C - L, ab > 0 and |a| >= 1e-10:
 //      slope = sign(a) min(max(|a|, |b|), 2 min(|a|, |b|)),   else 0,
@@ -43,41 +43,45 @@ __device__ inline T limited_slope(T a, T b) {  // a: forward, b: backward differ
     return (aa < T(1e-10) || !(a * b > T(0))) ? T(0) : copysign(mag, a);
 }
 
-// w (right - left) of one axis from the upwind reconstructions: for w >= 0
-// the faces take the left cells (vm, s_m) and (v0, s_0), for w < 0 the right
-// cells (v0, s_0) and (vp, s_p).  `half` is +1/2 or -1/2 with the sign of w.
+// Limited slope from two differences of one orientation with the 1e-10
+// test on `e` (the difference that is forward in the physical axis).  With
+// a*b > 0 the sign of a is the sign of b, so the value is symmetric in (a, b).
 template <typename T>
-__device__ inline T face_difference(T w, T half, bool pos, T vm, T v0, T vp, T s_up, T s_0) {
-    const T right = fma(half, pos ? s_0 : s_up, pos ? v0 : vp);
-    const T left = fma(half, pos ? s_up : s_0, pos ? vm : v0);
-    return w * right - w * left;
+__device__ inline T upwind_slope(T a, T b, T e) {
+    const T aa = fabs(a), ab = fabs(b);
+    const T mag = fmin(fmax(aa, ab), T(2) * fmin(aa, ab));
+    return (fabs(e) < T(1e-10) || !(a * b > T(0))) ? T(0) : copysign(mag, a);
 }
 
-// Flux divergence at one cell from its z window (zm, zc, zp with the carried
-// z slopes of rows -1 and 0 and the new one of row +1) and its y neighbours
-// in one LDS row: near = the upwind neighbour (y-1 for wy >= 0, else y+1),
-// far = the one beyond it, down = the downwind neighbour.
+// Flux divergence at one cell, per axis written in the lane's upwind frame:
+//   w (F(+1/2) - F(-1/2)) = |w| (downwind face - upwind face),
+// where the downwind face is the cell's own reconstruction and the upwind
+// face the upwind neighbour's.  y: near = upwind neighbour, far = the one
+// beyond it, down = downwind neighbour (LDS row); with A = down - c,
+// B = c - near, C = near - far the two slopes are (A, B) and (B, C), the
+// 1e-10 test going to whichever is forward in +y.  z: physical window
+// (zm, zc, zp) and carried physical slopes of rows -1, 0, +1; hz = +-1/2
+// with the sign of wz.
 template <typename T>
 __device__ inline T divergence(const T* row, int o_near, int o_far, int o_down, T zm, T zc, T zp,
-                               T sz_m, T sz_0, T sz_p, T wy, T wz, T hy, T hz, bool py, bool pz,
+                               T sz_m, T sz_0, T sz_p, T awy, T awz, T hz, bool py, bool pz,
                                T inv_dy, T inv_dz) {
     const T near = row[o_near], far = row[o_far], down = row[o_down];
-    const T ym1 = py ? near : down, yp1 = py ? down : near;
-    const T s0 = limited_slope(yp1 - zc, zc - ym1);
-    // upwind neighbour's slope: cell y-1 from (y-2, y-1, y) or cell y+1 from (y, y+1, y+2)
-    const T sn = limited_slope(py ? zc - near : far - near, py ? near - far : near - zc);
-    const T fy = face_difference(wy, hy, py, ym1, zc, yp1, sn, s0);
-    const T fz = face_difference(wz, hz, pz, zm, zc, zp, pz ? sz_m : sz_p, sz_0);
+    const T A = down - zc, B = zc - near, C = near - far;
+    const T s_down = upwind_slope(A, B, py ? A : B);
+    const T s_up = upwind_slope(B, C, py ? B : C);
+    const T fy = awy * fma(T(0.5), s_down, zc) - awy * fma(T(0.5), s_up, near);
+    const T fz = awz * fma(hz, sz_0, zc) - awz * fma(hz, pz ? sz_m : sz_p, pz ? zm : zp);
     return fy * inv_dy + fz * inv_dz;
 }
 
 // grid: (ceil(nang / MW), Ng), block: MW * R threads (R y columns per pass,
 // P passes: the thread's columns are y = ry + j R).  LDS: 8 rows of
 // (Ny+1) * MW values (two rings of 4 rows).
-template <typename T, int MW, int P>
+template <typename T, int MW, int P, bool CLIP>
 __global__ __launch_bounds__(kMaxThreads) void transport_march_kernel(
     T* __restrict__ F, const T* __restrict__ omend, int Ny, int Nz, int nang, int R, T inv_dy,
-    T inv_dz, T h, int preserve_sign) {
+    T inv_dz, T h) {
     extern __shared__ unsigned char lds_raw[];
     const int Ny1 = Ny + 1;
     const int rs = Ny1 * MW;  // LDS row stride (elements)
@@ -90,8 +94,7 @@ __global__ __launch_bounds__(kMaxThreads) void transport_march_kernel(
     const int m = live ? m0 + mm : nang - 1;
     const T wy = omend[2 * m], wz = omend[2 * m + 1];
     const bool py = wy >= T(0), pz = wz >= T(0);
-    const T hy = py ? T(0.5) : T(-0.5), hz = pz ? T(0.5) : T(-0.5);
-    const bool clip = preserve_sign == 0;
+    const T awy = fabs(wy), awz = fabs(wz), hz = pz ? T(0.5) : T(-0.5);
     const long long zstride = static_cast<long long>(Ny1) * nang;
     T* Fg = F + static_cast<long long>(blockIdx.y) * zstride * (Nz + 1) + m0 + mm;
 
@@ -160,9 +163,9 @@ __global__ __launch_bounds__(kMaxThreads) void transport_march_kernel(
                 T v = T(0), L0 = T(0);
                 if (on[j]) {
                     L0 = divergence(row, o_near[j], o_far[j], o_down[j], zm, zc, zp, sz[j][0],
-                                    sz[j][1], s_next, wy, wz, hy, hz, py, pz, inv_dy, inv_dz);
+                                    sz[j][1], s_next, awy, awz, hz, py, pz, inv_dy, inv_dz);
                     v = zc - h * L0;
-                    v = clip ? fmax(v, T(0)) : v;
+                    if (CLIP) v = fmax(v, T(0));
                     prow[o0[j]] = v;
                 }
                 sz[j][0] = sz[j][1];
@@ -198,10 +201,10 @@ __global__ __launch_bounds__(kMaxThreads) void transport_march_kernel(
                 const T s_next = limited_slope(wpred[j][3] - zp, zp - zc);
                 if (on[j]) {
                     const T L1 = divergence(row, o_near[j], o_far[j], o_down[j], zm, zc, zp,
-                                            szp[j][0], szp[j][1], s_next, wy, wz, hy, hz, py, pz,
-                                            inv_dy, inv_dz);
+                                            szp[j][0], szp[j][1], s_next, awy, awz, hz, py,
+                                            pz, inv_dy, inv_dz);
                     T v = wpsi[j][0] - T(0.5) * h * (wL[j][0] + L1);
-                    v = clip ? fmax(v, T(0)) : v;
+                    if (CLIP) v = fmax(v, T(0));
                     if (live) dst[goff[j]] = v;
                 }
                 szp[j][0] = szp[j][1];
@@ -236,12 +239,18 @@ void launch_march(T* F, const T* omend, int Ny, int Nz, int Ng, int nang, const 
                   double dy, double dz, double h, bool preserve_sign, hipStream_t s) {
     const size_t lds = sizeof(T) * 8 * static_cast<size_t>(Ny + 1) * MW;
     BFPN_REQUIRE(lds <= 160 * 1024, "transport plane too wide for the LDS rings");
-    BFPN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(&transport_march_kernel<T, MW, P>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
     const dim3 grid((nang + MW - 1) / MW, Ng);
-    hipLaunchKernelGGL((transport_march_kernel<T, MW, P>), grid, dim3(sh.threads), lds, s, F, omend,
-                       Ny, Nz, nang, sh.R, static_cast<T>(1.0 / dy), static_cast<T>(1.0 / dz),
-                       static_cast<T>(h), preserve_sign ? 1 : 0);
+    auto go = [&](auto kernel) {
+        BFPN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     static_cast<int>(lds)));
+        hipLaunchKernelGGL(kernel, grid, dim3(sh.threads), lds, s, F, omend, Ny, Nz, nang, sh.R,
+                           static_cast<T>(1.0 / dy), static_cast<T>(1.0 / dz), static_cast<T>(h));
+    };
+    if (preserve_sign)
+        go(&transport_march_kernel<T, MW, P, false>);
+    else
+        go(&transport_march_kernel<T, MW, P, true>);
     BFPN_HIP(hipGetLastError());
 }
 
