@@ -1,30 +1,48 @@
 """Throughput benchmark of the in-core eq15 depth march on one MI355X.
 
-Workload (one beam, resident in HBM): water, 230 MeV Gaussian pencil
-(sigma_yz 0.3 cm), 21x21 transverse nodes over 8x8 cm, 20x20 directions,
-500 energy groups on [1, 260] MeV, dt = 0.02 cm (Nx = 2000, 40 cm),
-eq15 with straggling, primary + secondary with the catastrophic source.
-FP32 state, FP64 energy recurrence and IDD accumulation.
+Default workload ("beam", one beam resident in HBM): water, 230 MeV Gaussian
+pencil (sigma_yz 0.3 cm), 21x21 transverse nodes over 8x8 cm, 20x20
+directions, 500 energy groups on [1, 260] MeV, dt = 0.02 cm (Nx = 2000,
+40 cm), eq15 with straggling, primary + secondary with the catastrophic
+source.  FP32 state, FP64 energy recurrence and IDD accumulation.
 
 A "step" is one Strang depth step of both species (E/2, T/2, A, T/2, E/2 and
 the source), i.e. Ng*(Ny+1)*(Nz+1)*Nmu*Nom phase-space cell-updates.  The
 warmup steps include step 0 (no secondary), so every timed step carries the
 full secondary chain.
 
-Prints ONE JSON line (rank 0).  Fields beyond the common contract:
-  roofline      dominant operator: algorithmic HBM bytes per call / mean call
-                time from HIP events on the solver's stream in an
-                instrumented copy of the timed region; peak 8 TB/s
+Prints ONE JSON line.  Fields beyond the common contract:
+  roofline      dominant operator: algorithmic HBM bytes per launch / mean
+                launch time from HIP events on the solver's stream in an
+                instrumented copy of the timed region; peak 8 TB/s.  traffic:
+                HBM bytes per launch of that operator measured with rocprofv3
+                FETCH_SIZE / WRITE_SIZE (calibrated on a plain copy), read from
+                the committed summary profiles/hbm_traffic_single_beam.md
   phases        per-operator seconds/calls/GB/s of that instrumented run
-  cpu_baseline  the CPU FP64 oracle on this host, one full depth step of the
-                same workload (bounded sample)
+  cpu_baseline  the CPU FP64 oracle on this host (all cores), a bounded sample
+                of the same workload: the same physics on a 5x5-node
+                transverse slab at the same 0.4 cm spacing, two full depth
+                steps after an untimed one (cell-updates/s does not depend on
+                the slab width)
+
+Other workloads (--workload), each one JSON line:
+  paper     81x81 nodes (the paper grid), Nx = 4000, FP32 state resident
+  beamlets  a seeded sample of the 64-beam batch (E ~ U[70,230] MeV, sigma
+            ~ U[0.2,0.5] cm, 21x21 or 41x41 nodes, Ng = 300, water/bone, depth
+            1.2 x CSDA range): one-at-a-time solver vs the batching server
+  serve     seeded Poisson arrivals (rate, first N requests) into the server:
+            solves/s and arrival->IDD latency p50/p99
+  copy      the plain device copy used to calibrate the traffic counters
 """
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
+import re
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -32,83 +50,73 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "phase-space cell-updates/s; beamlet solves/s + p99 latency at IDD err ≤0.5%"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_single_beam.md")
 
 
-def workload_case():
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def beam_case(Ny=20, Nx=2000, time_cm=40.0, material="water"):
     from bfpn_gpu_amd.abi import Case
     from bfpn_gpu_amd.tables import material_dir
 
-    return Case(Nx=2000, Ny=20, Nz=20, Ng=500, Nmu=20, Nom=20, Ly=8, Lz=8, Lu=2, Lv=2,
-                energy=230.0, sigma_yz=0.3, material="water", data=material_dir("water"),
-                time=40.0, flags=dict(eq15_straggling=1))
+    return Case(Nx=Nx, Ny=Ny, Nz=Ny, Ng=500, Nmu=20, Nom=20, Ly=8, Lz=8, Lu=2, Lv=2,
+                energy=230.0, sigma_yz=0.3, material=material, data=material_dir(material),
+                time=time_cm, flags=dict(eq15_straggling=1))
 
 
 def cpu_baseline(case, threads: int | None):
-    """One full depth step (step 1: primary + secondary) of the workload with the
-    CPU FP64 oracle, after an untimed step 0."""
+    """The CPU FP64 oracle on the same physics over a 5x5-node transverse slab
+    with the workload's node spacing: one untimed step, two timed full steps."""
     if threads:
         os.environ["OMP_NUM_THREADS"] = str(threads)
     import oracle
 
     oracle.build()
-    s = oracle.Solver(case)
+    dy = case.Ly / case.Ny
+    sample = dataclasses.replace(case, Ny=4, Nz=4, Ly=4 * dy, Lz=4 * dy)
+    s = oracle.Solver(sample)
     s.advance(1)
     t0 = time.perf_counter()
-    s.advance(1)
+    s.advance(2)
     dt = time.perf_counter() - t0
     cells = s.cells
     n = oracle.threads()
     s.close()
-    return {"value": cells / dt, "unit": "cell-updates/s", "cores": n, "kind": "port",
-            "sample": f"CPU FP64 oracle (C++/OpenMP), 1 full depth step (primary+secondary) of the "
-                      f"same workload, {cells} cells, {dt:.2f} s"}
+    return {"value": 2 * cells / dt, "unit": "cell-updates/s", "cores": n, "kind": "port",
+            "sample": f"CPU FP64 oracle (C++/OpenMP, {n} threads): same physics on a 5x5-node "
+                      f"transverse slab at {dy:.2f} cm spacing ({cells} cells), 2 full depth "
+                      f"steps (primary+secondary) after an untimed one, {dt:.2f} s"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-threads", type=int, default=None)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              f"(the solver runs one beam per GPU and has no multi-GPU path)", file=sys.stderr)
-        sys.exit(2)
-    if args.warmup < 1:
-        args.warmup = 1
+def load_traffic(phase: str):
+    """HBM bytes per launch of `phase` from the committed rocprofv3 summary."""
+    try:
+        text = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"```json\s*(\{.*?\})\s*```", text, re.S)
+    if not m:
+        return None
+    data = json.loads(m.group(1))
+    return data.get("bytes_per_launch", {}).get(phase)
 
-    import numpy as np
 
-    import bfpn_gpu_amd as bg
-
-    case = workload_case()
-    prec = bg.FP32 if args.precision == "fp32" else bg.FP64
-    solver = bg.Solver(case, prec)
-    cells = solver.cells
-    print(f"[bench] workload ready: {cells} cells, {solver.device_bytes / 2**30:.2f} GiB resident",
-          file=sys.stderr, flush=True)
-
-    solver.advance(args.warmup)
+def timed_steps(solver, steps, warmup):
+    solver.advance(warmup)
     solver.history()  # synchronizes the stream
     t0 = time.perf_counter()
-    solver.advance(args.steps)
+    solver.advance(steps)
     idd, _ = solver.history()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed / args.steps * 1e3
-    value = cells * args.steps / elapsed
-    print(f"[bench] timed {args.steps} steps: {ms_per_step:.3f} ms/step", file=sys.stderr,
-          flush=True)
-    assert np.all(np.isfinite(idd)) and idd[-1] > 0
+    return time.perf_counter() - t0, idd
 
-    # instrumented copy of the timed region: per-operator HIP events
+
+def phases_of(solver, steps):
     solver.set_profiling(True)
-    solver.advance(args.steps)
+    solver.advance(steps)
     stats = solver.phase_stats()
     solver.set_profiling(False)
-    solver.close()
     phases = {}
     for name, (sec, calls, byt) in stats.items():
         if calls:
@@ -116,14 +124,33 @@ def main():
                             "ms_per_call": round(sec / calls * 1e3, 4),
                             "GB_per_call": round(byt / calls / 1e9, 4),
                             "GBps": round(byt / sec / 1e9, 1) if sec > 0 else None}
+    step_bytes = sum(v[2] for v in stats.values()) / steps
+    return phases, step_bytes
+
+
+def run_resident(args, case, workload, with_traffic):
+    import numpy as np
+
+    import bfpn_gpu_amd as bg
+
+    prec = bg.FP32 if args.precision == "fp32" else bg.FP64
+    solver = bg.Solver(case, prec)
+    cells = solver.cells
+    log(f"workload ready: {cells} cells, {solver.device_bytes / 2**30:.2f} GiB resident")
+    elapsed, idd = timed_steps(solver, args.steps, args.warmup)
+    ms_per_step = elapsed / args.steps * 1e3
+    log(f"timed {args.steps} steps: {ms_per_step:.3f} ms/step")
+    assert np.all(np.isfinite(idd)) and idd[-1] > 0
+    phases, step_bytes = phases_of(solver, args.steps)
+    solver.close()
     dom = max(phases, key=lambda k: phases[k]["seconds"])
     ach = phases[dom]["GBps"]
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None}
-    step_bytes = sum(v[2] for v in stats.values()) / args.steps
-    out = {
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": load_traffic(dom) if with_traffic else None}
+    return {
         "metric": METRIC,
-        "value": value,
+        "value": cells * args.steps / elapsed,
         "unit": "cell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -134,18 +161,177 @@ def main():
         "vs_baseline": None,
         "dtype": "f32" if prec == bg.FP32 else "f64",
         "data": "synthetic",
-        "config": {"workload": "single beam, water 230 MeV, 21x21 nodes x 20x20 directions x 500 "
-                               "groups, dt 0.02 cm, eq15 + straggling + secondary source",
-                   "cells": cells, "precision": "FP32 state; FP64 energy recurrence and IDD"
+        "config": {"workload": workload, "cells": cells,
+                   "precision": "FP32 state; FP64 energy recurrence and IDD"
                    if prec == bg.FP32 else "FP64",
-                   "step_fraction_of_hbm_roofline": round(step_bytes / (ms_per_step * 1e-3) / 1e9
-                                                          / HBM_PEAK_GBS, 4)},
+                   "step_fraction_of_hbm_roofline":
+                       round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": roofline,
         "phases": phases,
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(case, args.cpu_threads)
-    print(json.dumps(out))
+
+
+def idd_error(idd, ref, dt):
+    import numpy as np
+
+    from bfpn_gpu_amd import metrics
+
+    c = metrics.compare(idd, ref)
+    x = (np.arange(len(ref)) + 1) * dt
+    a, b = metrics.analyze(x, idd), metrics.analyze(x, ref)
+    shift = max(abs(a[k] - b[k]) for k in ("BP", "D90", "D20"))
+    return {"max_rel": c["max_rel"], "max_shift_mm": 10 * shift}
+
+
+def run_beamlets(args):
+    import numpy as np
+
+    import bfpn_gpu_amd as bg
+    from bfpn_gpu_amd.workloads import beamlet_batch
+
+    cases = beamlet_batch(64)[: args.beams]
+    log(f"{len(cases)} beams: " + ", ".join(f"{c.energy:.0f}MeV/{c.Ny}/{c.material}" for c in cases))
+    # one at a time: a fresh solver per beam (the reference's one-process-per-beam model)
+    t0 = time.perf_counter()
+    plain = []
+    for c in cases:
+        with bg.Solver(c, bg.FP32) as s:
+            plain.append(s.run()[0])
+    t_plain = time.perf_counter() - t0
+    log(f"one at a time: {t_plain:.2f} s")
+    t0 = time.perf_counter()
+    with bg.Server(max_active=args.max_active, slice_steps=args.slice_steps) as srv:
+        ids = [srv.submit(c) for c in cases]
+        served = [srv.wait(i, timeout=3600)[0] for i in ids]
+        st = srv.stats()
+    t_srv = time.perf_counter() - t0
+    log(f"server: {t_srv:.2f} s, peak active {st['peak_active']}")
+    identical = all(np.array_equal(a, b) for a, b in zip(plain, served))
+    out = {"metric": "beamlet solves/s", "value": len(cases) / t_srv, "unit": "solves/s",
+           "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"beamlets: first {len(cases)} of the seeded 64-beam batch",
+                      "max_active": args.max_active, "slice_steps": args.slice_steps},
+           "one_at_a_time": {"value": len(cases) / t_plain, "seconds": t_plain},
+           "server": {"seconds": t_srv, "peak_active": st["peak_active"],
+                      "cached_tables": st["cached_tables"]},
+           "speedup": t_plain / t_srv, "served_equals_alone": identical}
+    if args.fp64_check:
+        errs = []
+        for c, idd in zip(cases, served):
+            with bg.Solver(c, bg.FP64) as s:
+                ref = s.run()[0]
+            errs.append(idd_error(idd, ref, c.dt))
+        out["accuracy_vs_fp64"] = {"max_rel": max(e["max_rel"] for e in errs),
+                                   "max_shift_mm": max(e["max_shift_mm"] for e in errs)}
+    return out
+
+
+def run_serve(args):
+    import numpy as np
+
+    import bfpn_gpu_amd as bg
+    from bfpn_gpu_amd.workloads import serving_requests
+
+    reqs = serving_requests(args.rate, 60.0)[: args.requests]
+    log(f"{len(reqs)} requests at {args.rate}/s: " +
+        ", ".join(f"{c.energy:.0f}MeV/{c.Ny}/{c.material}" for _, c in reqs))
+    lat = [None] * len(reqs)
+    with bg.Server(max_active=args.max_active, slice_steps=args.slice_steps) as srv:
+        t0 = time.perf_counter()
+        ids = [None] * len(reqs)
+        done = threading.Event()
+
+        def submitter():
+            for k, (at, c) in enumerate(reqs):
+                dt = t0 + at - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                ids[k] = srv.submit(c)
+            done.set()
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        done.wait()
+        th.join()
+        for k, rid in enumerate(ids):
+            srv.wait(rid, timeout=3600)
+            lat[k] = time.perf_counter() - (t0 + reqs[k][0])
+        t_end = time.perf_counter()
+        st = srv.stats()
+    lat = np.array(lat)
+    return {"metric": "beamlet solves/s", "value": len(reqs) / (t_end - t0), "unit": "solves/s",
+            "n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"serve: first {len(reqs)} seeded Poisson arrivals at "
+                                   f"{args.rate}/s", "max_active": args.max_active,
+                       "slice_steps": args.slice_steps},
+            "latency_s": {"p50": float(np.percentile(lat, 50)),
+                          "p99": float(np.percentile(lat, 99)), "max": float(lat.max())},
+            "server": {"peak_active": st["peak_active"], "cached_tables": st["cached_tables"]}}
+
+
+def run_copy(args):
+    import numpy as np
+
+    import bfpn_gpu_amd as bg
+
+    n = 1 << 28  # 1 GiB of float32
+    a = bg.DeviceArray(shape=(n,), dtype=np.float32)
+    b = bg.DeviceArray(shape=(n,), dtype=np.float32)
+    bg.lib().bfpn_memset(a.ptr, 0, n * 4)
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        bg.check(bg.lib().bfpn_device_copy(b.ptr, a.ptr, n * 4))
+    dt = (time.perf_counter() - t0) / reps
+    return {"metric": "device copy", "bytes_read_per_launch": n * 4,
+            "bytes_written_per_launch": n * 4, "seconds_per_launch": dt,
+            "GBps": 2 * n * 4 / dt / 1e9, "launches": reps}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
+    ap.add_argument("--workload", choices=("beam", "paper", "beamlets", "serve", "copy"),
+                    default="beam")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=None)
+    ap.add_argument("--beams", type=int, default=8)
+    ap.add_argument("--fp64-check", action="store_true")
+    ap.add_argument("--rate", type=float, default=20.0)
+    ap.add_argument("--requests", type=int, default=8)
+    ap.add_argument("--max-active", type=int, default=8)
+    ap.add_argument("--slice-steps", type=int, default=8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              f"(the solver runs one beam per GPU and has no multi-GPU path)", file=sys.stderr)
+        sys.exit(2)
+    if args.warmup < 1:
+        args.warmup = 1
+
+    if args.workload == "beam":
+        case = beam_case()
+        out = run_resident(args, case, "single beam, water 230 MeV, 21x21 nodes x 20x20 "
+                           "directions x 500 groups, dt 0.02 cm, eq15 + straggling + secondary "
+                           "source", with_traffic=True)
+        if not args.no_cpu:
+            log("CPU baseline (oracle, bounded sample)")
+            out["cpu_baseline"] = cpu_baseline(case, args.cpu_threads)
+    elif args.workload == "paper":
+        case = beam_case(Ny=80, Nx=4000)
+        out = run_resident(args, case, "paper grid, water 230 MeV, 81x81 nodes x 20x20 "
+                           "directions x 500 groups, dt 0.01 cm, eq15 + straggling + secondary "
+                           "source", with_traffic=False)
+    elif args.workload == "beamlets":
+        out = run_beamlets(args)
+    elif args.workload == "serve":
+        out = run_serve(args)
+    else:
+        out = run_copy(args)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -21,6 +21,7 @@ from ._lib import (  # noqa: F401
     Server,
     Solver,
     Workspace,
+    check,
     device_count,
     lib,
     lib_path,

@@ -60,6 +60,7 @@ def lib():
     L.bfpn_memcpy_h2d.argtypes = [vp, vp, sz]
     L.bfpn_memcpy_d2h.argtypes = [vp, vp, sz]
     L.bfpn_memset.argtypes = [vp, i, sz]
+    L.bfpn_device_copy.argtypes = [vp, vp, sz]
     L.bfpn_mem_info.argtypes = [C.POINTER(sz), C.POINTER(sz)]
     L.bfpn_workspace_create.argtypes = [C.POINTER(vp)]
     L.bfpn_workspace_destroy.argtypes = [vp]

@@ -36,6 +36,8 @@ int source_erows(int Ng);    // Ng padded to 64 (E rows); tile_band has erows/16
 void launch_split_transpose(const float* kv, __bf16* hi, __bf16* lo, int Ng, int nang,
                             hipStream_t s);
 void launch_split(const float* src, __bf16* hi, __bf16* lo, long long n, hipStream_t s);
+// Plain streaming copy, bytes % 16 == 0.
+void launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t s);
 // flags[g][mt][kt] = 1 if the 64 x 32 block (directions mt, K tile kt) of the
 // split transposed Kv holds a nonzero entry.
 void launch_kv_block_flags(const __bf16* kth, unsigned char* flags, int Ng, int nang, hipStream_t s);

@@ -90,7 +90,24 @@ __global__ __launch_bounds__(kBlock) void convert_kernel(const double* __restric
     if (k < n) dst[k] = static_cast<T>(src[k]);
 }
 
+// grid-stride 16-byte copy (the traffic-counter calibration kernel)
+__global__ __launch_bounds__(kBlock) void copy16_kernel(const float4* __restrict__ src,
+                                                        float4* __restrict__ dst, long long n) {
+    for (long long i = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+         i += static_cast<long long>(gridDim.x) * kBlock)
+        dst[i] = src[i];
+}
+
 }  // namespace
+
+void launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t s) {
+    const long long n = static_cast<long long>(bytes / 16);
+    if (n == 0) return;
+    const long long blocks = std::min<long long>((n + kBlock - 1) / kBlock, 65536);
+    hipLaunchKernelGGL(copy16_kernel, dim3(static_cast<unsigned>(blocks)), dim3(kBlock), 0, s,
+                       static_cast<const float4*>(src), static_cast<float4*>(dst), n);
+    BFPN_HIP(hipGetLastError());
+}
 
 template <typename T>
 void launch_convert(const double* src, T* dst, long long n, hipStream_t s) {

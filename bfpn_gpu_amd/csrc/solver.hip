@@ -682,6 +682,13 @@ BFPN_API bfpn_status bfpn_memset(void* p, int v, size_t bytes) {
         BFPN_HIP(hipDeviceSynchronize());
     });
 }
+BFPN_API bfpn_status bfpn_device_copy(void* dst, const void* src, size_t bytes) {
+    return guarded([&] {
+        BFPN_REQUIRE(dst && src && bytes % 16 == 0, "device copy needs 16-byte multiples");
+        launch_copy16(src, dst, bytes, nullptr);
+        BFPN_HIP(hipDeviceSynchronize());
+    });
+}
 BFPN_API bfpn_status bfpn_synchronize(void) {
     return guarded([&] { BFPN_HIP(hipDeviceSynchronize()); });
 }

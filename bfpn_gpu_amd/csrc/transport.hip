@@ -81,22 +81,30 @@ __device__ inline T divergence(const T* row, int o_near, int o_far, int o_down, 
 template <typename T, int MW, int P, bool CLIP>
 __global__ __launch_bounds__(kMaxThreads) void transport_march_kernel(
     T* __restrict__ F, const T* __restrict__ omend, int Ny, int Nz, int nang, int R, T inv_dy,
-    T inv_dz, T h) {
+    T inv_dz, T h, int tiles) {
     extern __shared__ unsigned char lds_raw[];
+    // Blocks are dispatched round-robin over the 8 XCDs.  Direction chunks of
+    // one group share 128-byte lines (a plane row is nang floats), so the
+    // chunks of a group run consecutively on one XCD and share its L2:
+    // block b -> logical tile (b mod 8) * (grid / 8) + b / 8.
+    const int L = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    if (L >= tiles) return;
+    const int mchunks = (nang + MW - 1) / MW;
+    const int g = L / mchunks;
     const int Ny1 = Ny + 1;
     const int rs = Ny1 * MW;  // LDS row stride (elements)
     T* psi_ring = reinterpret_cast<T*>(lds_raw);  // 4 rows of psi
     T* pred_ring = psi_ring + 4 * rs;             // 4 rows of psi*
 
     const int mm = threadIdx.x % MW, ry = threadIdx.x / MW;
-    const int m0 = blockIdx.x * MW;
+    const int m0 = (L - g * mchunks) * MW;
     const bool live = m0 + mm < nang;
     const int m = live ? m0 + mm : nang - 1;
     const T wy = omend[2 * m], wz = omend[2 * m + 1];
     const bool py = wy >= T(0), pz = wz >= T(0);
     const T awy = fabs(wy), awz = fabs(wz), hz = pz ? T(0.5) : T(-0.5);
     const long long zstride = static_cast<long long>(Ny1) * nang;
-    T* Fg = F + static_cast<long long>(blockIdx.y) * zstride * (Nz + 1) + m0 + mm;
+    T* Fg = F + static_cast<long long>(g) * zstride * (Nz + 1) + m0 + mm;
 
     bool on[P];
     int goff[P], o0[P], o_near[P], o_far[P], o_down[P];
@@ -239,13 +247,16 @@ void launch_march(T* F, const T* omend, int Ny, int Nz, int Ng, int nang, const 
                   double dy, double dz, double h, bool preserve_sign, hipStream_t s) {
     const size_t lds = sizeof(T) * 8 * static_cast<size_t>(Ny + 1) * MW;
     BFPN_REQUIRE(lds <= 160 * 1024, "transport plane too wide for the LDS rings");
-    const dim3 grid((nang + MW - 1) / MW, Ng);
+    const long long tiles = static_cast<long long>((nang + MW - 1) / MW) * Ng;
+    BFPN_REQUIRE(tiles < (1LL << 31) - 8, "transport grid too large");
+    const dim3 grid(static_cast<unsigned>((tiles + 7) / 8 * 8));
     auto go = [&](auto kernel) {
         BFPN_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                      hipFuncAttributeMaxDynamicSharedMemorySize,
                                      static_cast<int>(lds)));
         hipLaunchKernelGGL(kernel, grid, dim3(sh.threads), lds, s, F, omend, Ny, Nz, nang, sh.R,
-                           static_cast<T>(1.0 / dy), static_cast<T>(1.0 / dz), static_cast<T>(h));
+                           static_cast<T>(1.0 / dy), static_cast<T>(1.0 / dz), static_cast<T>(h),
+                           static_cast<int>(tiles));
     };
     if (preserve_sign)
         go(&transport_march_kernel<T, MW, P, false>);
@@ -261,7 +272,6 @@ void launch_transport_half(T* F, const T* omend, int Ny, int Nz, int Ng, int nan
                            double dz, double h, bool preserve_sign, hipStream_t s) {
     if (Ng <= 0 || nang <= 0) return;
     BFPN_REQUIRE(Ny >= 1 && Nz >= 1, "transport needs Ny, Nz >= 1");
-    BFPN_REQUIRE(Ng <= 65535, "transport supports at most 65535 energy groups");
     const MarchShape sh = march_shape(Ny + 1, sizeof(T));
     BFPN_REQUIRE(sh.MW > 0, sizeof(T) == 4 ? "transport supports at most 512 y nodes in FP32"
                                            : "transport supports at most 320 y nodes in FP64");

@@ -102,6 +102,10 @@ BFPN_API bfpn_status bfpn_free(void* dptr);
 BFPN_API bfpn_status bfpn_memcpy_h2d(void* dst, const void* src, size_t bytes);
 BFPN_API bfpn_status bfpn_memcpy_d2h(void* dst, const void* src, size_t bytes);
 BFPN_API bfpn_status bfpn_memset(void* dptr, int value, size_t bytes);
+/* Device-to-device copy of `bytes` (a multiple of 16) by a plain streaming
+ * kernel, one read and one write of each byte; synchronous.  No counterpart
+ * in the reference: it is the calibration kernel for HBM traffic counters. */
+BFPN_API bfpn_status bfpn_device_copy(void* dst, const void* src, size_t bytes);
 BFPN_API bfpn_status bfpn_synchronize(void);
 BFPN_API bfpn_status bfpn_mem_info(size_t* free_bytes, size_t* total_bytes);
 
