@@ -32,6 +32,9 @@ Other workloads (--workload), each one JSON line:
             1.2 x CSDA range): one-at-a-time solver vs the batching server
   serve     seeded Poisson arrivals (rate, first N requests) into the server:
             solves/s and arrival->IDD latency p50/p99
+  accuracy  one case (--grid beam|paper, --material water|bone) to full depth
+            with FP32 state and with the FP64 instantiation: IDD error above 10 %
+            of the maximum and BP/D90/D20 shifts (gate 0.5 %, 0.1 mm)
   copy      the plain device copy used to calibrate the traffic counters
 """
 from __future__ import annotations
@@ -180,7 +183,7 @@ def idd_error(idd, ref, dt):
     x = (np.arange(len(ref)) + 1) * dt
     a, b = metrics.analyze(x, idd), metrics.analyze(x, ref)
     shift = max(abs(a[k] - b[k]) for k in ("BP", "D90", "D20"))
-    return {"max_rel": c["max_rel"], "max_shift_mm": 10 * shift}
+    return {"max_rel": float(c["max_rel"]), "max_shift_mm": float(10 * shift)}
 
 
 def run_beamlets(args):
@@ -269,6 +272,43 @@ def run_serve(args):
             "server": {"peak_active": st["peak_active"], "cached_tables": st["cached_tables"]}}
 
 
+def run_accuracy(args):
+    """FP32-state solver against the FP64 instantiation on the same case, full depth."""
+    import numpy as np
+
+    import bfpn_gpu_amd as bg
+
+    ny, nx = (20, 2000) if args.grid == "beam" else (80, 4000)
+    depth = 40.0 if args.material == "water" else 22.0
+    case = beam_case(Ny=ny, Nx=nx, time_cm=depth, material=args.material)
+    if args.material == "bone":
+        case = dataclasses.replace(case, density=1.85)
+    res = {}
+    for name, prec in (("fp32", bg.FP32), ("fp64", bg.FP64)):
+        t0 = time.perf_counter()
+        with bg.Solver(case, prec) as s:
+            idd, _ = s.run()
+        res[name] = (idd, time.perf_counter() - t0)
+        log(f"{name}: {len(idd)} steps in {res[name][1]:.1f} s")
+        if args.save:
+            os.makedirs(args.save, exist_ok=True)
+            np.save(os.path.join(args.save, f"idd_{args.grid}_{args.material}_{name}.npy"), idd)
+    err = idd_error(res["fp32"][0], res["fp64"][0], case.dt)
+    from bfpn_gpu_amd import metrics
+
+    x = (np.arange(len(res["fp64"][0])) + 1) * case.dt
+    return {"metric": "IDD error of the FP32-state solver vs the FP64 instantiation",
+            "config": {"workload": f"{args.grid} grid ({ny + 1}x{ny + 1} nodes), {args.material} "
+                                   f"rho {case.density or 1.0}, 230 MeV, depth {depth} cm, "
+                                   f"Nx {nx}, Ng 500, 20x20 directions, eq15 + straggling + "
+                                   f"secondary source", "cells": case.cells},
+            "max_rel_err_above_10pct": err["max_rel"], "max_shift_mm_BP_D90_D20": err["max_shift_mm"],
+            "fp64_metrics_cm": metrics.analyze(x, res["fp64"][0]),
+            "seconds": {"fp32": res["fp32"][1], "fp64": res["fp64"][1]},
+            "gate": {"max_rel_err": 0.005, "max_shift_mm": 0.1,
+                     "pass": bool(err["max_rel"] <= 0.005 and err["max_shift_mm"] <= 0.1)}}
+
+
 def run_copy(args):
     import numpy as np
 
@@ -294,8 +334,11 @@ def main():
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
-    ap.add_argument("--workload", choices=("beam", "paper", "beamlets", "serve", "copy"),
-                    default="beam")
+    ap.add_argument("--workload", choices=("beam", "paper", "beamlets", "serve", "accuracy",
+                                           "copy"), default="beam")
+    ap.add_argument("--grid", choices=("beam", "paper"), default="beam")
+    ap.add_argument("--material", choices=("water", "bone"), default="water")
+    ap.add_argument("--save", default=None, help="directory for the accuracy IDD arrays")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--beams", type=int, default=8)
@@ -329,6 +372,8 @@ def main():
         out = run_beamlets(args)
     elif args.workload == "serve":
         out = run_serve(args)
+    elif args.workload == "accuracy":
+        out = run_accuracy(args)
     else:
         out = run_copy(args)
     print(json.dumps(out), flush=True)

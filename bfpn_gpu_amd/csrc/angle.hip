@@ -22,7 +22,11 @@ namespace {
 // ---------------------------------------------------------- FP32 / MFMA ----
 // One wave per (g, s) plane (each wave walks a contiguous run of planes), the
 // plane zero padded to 32 x 32, every stage a 32x32x32 product on the matrix
-// cores (split-BF16, FP32 accumulation; MT / NT = 16-row tiles along v and a).
+// cores (both operands split in three BF16 terms, six products kept, FP32
+// accumulation; MT / NT = 16-row tiles along v and a).  The DST runs every
+// depth step on the same beam, so its error compounds: with two-term splits
+// the FP32 IDD drifts ~1 % above the FP64 one over 1600 steps, with three
+// terms the operator is as exact as FP32 arithmetic.
 // The stages chain in registers, with no LDS: a stage's output tile, lane
 // holding D[4q+r][i], is the next stage's K operand when the K axis is taken
 // in the order  slot 8q + 4u + r  <->  index 16u + 4q + r  (chain_index), and
@@ -45,19 +49,20 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
     const int p_end = min(planes, p + per_wave);
     if (p >= p_end) return;
     const int na = Nmu * Nom;
-    auto frag = [&](const __bf16* h, const __bf16* l, int t) {
+    auto frag = [&](const __bf16* h, const __bf16* m, const __bf16* l, int t) {
         const int off = (t * 16 + i) * 32 + 8 * q;
-        return SplitFrag{*reinterpret_cast<const bf16x8*>(h + off),
-                         *reinterpret_cast<const bf16x8*>(l + off)};
+        return SplitFrag3{*reinterpret_cast<const bf16x8*>(h + off),
+                          *reinterpret_cast<const bf16x8*>(m + off),
+                          *reinterpret_cast<const bf16x8*>(l + off)};
     };
-    SplitFrag su[NT], sup[NT], svp[MT];
+    SplitFrag3 su[NT], sup[NT], svp[MT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-        su[t] = frag(plan.su_h, plan.su_l, t);
-        sup[t] = frag(plan.sup_h, plan.sup_l, t);
+        su[t] = frag(plan.su_h, plan.su_m, plan.su_l, t);
+        sup[t] = frag(plan.sup_h, plan.sup_m, plan.sup_l, t);
     }
 #pragma unroll
-    for (int t = 0; t < MT; ++t) svp[t] = frag(plan.svp_h, plan.svp_l, t);
+    for (int t = 0; t < MT; ++t) svp[t] = frag(plan.svp_h, plan.svp_m, plan.svp_l, t);
 
     int g_cur = -1;
     float cf[MT][NT][4];
@@ -78,7 +83,7 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                     }
         }
         // stage 1: Y[v][mu] = sum_a X[v][a] Su[mu][a]
-        SplitFrag xa[MT];
+        SplitFrag3 xa[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
             const int v = mt * 16 + i;
@@ -102,13 +107,13 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                     if (v < Nmu && a < Nom) x[e] = X[v * Nom + a];
                 }
             }
-            xa[mt] = split8(x);
+            xa[mt] = split8_3(x);
         }
         f32x4 d1[MT][NT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) d1[mt][nt] = mma_split(xa[mt], su[nt], zero4());
+            for (int nt = 0; nt < NT; ++nt) d1[mt][nt] = mma_split3(xa[mt], su[nt], zero4());
         // stage 2: Z[nu][mu] = C[nu][mu] sum_v Sv[nu][v] Y[v][mu]
         f32x4 d2[MT][NT];
 #pragma unroll
@@ -119,10 +124,10 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                 y[r] = d1[0][nt][r];
                 y[4 + r] = MT > 1 ? d1[MT - 1][nt][r] : 0.f;
             }
-            const SplitFrag b = split8(y);
+            const SplitFrag3 b = split8_3(y);
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
-                d2[mt][nt] = mma_split(svp[mt], b, zero4());
+                d2[mt][nt] = mma_split3(svp[mt], b, zero4());
 #pragma unroll
                 for (int r = 0; r < 4; ++r) d2[mt][nt][r] *= cf[mt][nt][r];
             }
@@ -137,9 +142,9 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                 z[r] = d2[0][nt][r];
                 z[4 + r] = MT > 1 ? d2[MT - 1][nt][r] : 0.f;
             }
-            const SplitFrag a = split8(z);
+            const SplitFrag3 a = split8_3(z);
 #pragma unroll
-            for (int mt = 0; mt < MT; ++mt) d3[nt][mt] = mma_split(a, svp[mt], zero4());
+            for (int mt = 0; mt < MT; ++mt) d3[nt][mt] = mma_split3(a, svp[mt], zero4());
         }
         // stage 4, transposed: X'^T[a][v] = sum_mu Su[mu][a] W[v][mu] (operands
         // swapped), so each lane holds 4 consecutive a of one row v; in place
@@ -151,11 +156,11 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                 w[r] = d3[0][mt][r];
                 w[4 + r] = NT > 1 ? d3[NT - 1][mt][r] : 0.f;
             }
-            const SplitFrag b = split8(w);
+            const SplitFrag3 b = split8_3(w);
             const int v = mt * 16 + i;
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
-                const f32x4 d = mma_split(sup[nt], b, zero4());
+                const f32x4 d = mma_split3(sup[nt], b, zero4());
                 const int a0 = nt * 16 + 4 * q;
                 if (V4) {
                     if (v < Nmu && a0 < Nom)

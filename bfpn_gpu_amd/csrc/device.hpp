@@ -82,7 +82,7 @@ struct SourcePlanBuffers {
 
 // Split-BF16 sine bases for the MFMA DST (FP32 state, Nmu, Nom <= 32).
 struct AnglePlanBuffers {
-    DeviceBuffer h, l, scratch;
+    DeviceBuffer h, m, l;
     AnglePlan plan;
     static bool usable(int Nmu, int Nom) { return Nmu <= 32 && Nom <= 32; }
     void build(int Nmu, int Nom, hipStream_t s);

@@ -68,14 +68,18 @@ void launch_angle_full(T* F, const T* factors, const T* su, const T* sv, int Ng,
                        int Nom, hipStream_t s);
 
 // Split-BF16 sine bases padded to 32x32 (row-major), for the MFMA DST.
-// Split-BF16 sine bases, 32x32 zero padded, row-major: Su as is, and Su and Sv
-// with their columns in the register-chaining order of the DST kernel.
+// Three-term BF16 sine bases (hi + mid + lo, split from the FP64 values),
+// 32x32 zero padded, row-major: Su as is, and Su and Sv with their columns in
+// the register-chaining order of the DST kernel.
 struct AnglePlan {
     const __bf16* su_h = nullptr;
+    const __bf16* su_m = nullptr;
     const __bf16* su_l = nullptr;
     const __bf16* sup_h = nullptr;
+    const __bf16* sup_m = nullptr;
     const __bf16* sup_l = nullptr;
     const __bf16* svp_h = nullptr;
+    const __bf16* svp_m = nullptr;
     const __bf16* svp_l = nullptr;
 };
 // K slot 8q + 4u + r of a chained MFMA operand <-> matrix index 16u + 4q + r.

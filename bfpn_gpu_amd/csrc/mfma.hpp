@@ -61,6 +61,50 @@ __device__ inline f32x4 mma_split(const SplitFrag& a, const SplitFrag& b, f32x4 
 
 __device__ inline f32x4 zero4() { return f32x4{0.f, 0.f, 0.f, 0.f}; }
 
+// Three-term split, x = hi + mid + lo exactly for FP32 x (8 + 8 + 8 bits).
+// Used where an operator is applied thousands of times in a row (the DST of
+// every depth step): there a two-term split's fixed ~2^-17 representation
+// error compounds step after step, a three-term split's does not.
+struct SplitFrag3 {
+    bf16x8 hi, mid, lo;
+};
+
+__device__ inline SplitFrag3 split8_3(const float (&v)[8]) {
+    typedef float f32x8 __attribute__((ext_vector_type(8)));
+    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+    auto widen = [](bf16x8 b) {  // bf16 -> FP32 from the packed words
+        const u32x4 w = __builtin_bit_cast(u32x4, b);
+        f32x8 h;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            h[2 * k] = __uint_as_float(w[k] << 16);
+            h[2 * k + 1] = __uint_as_float(w[k] & 0xffff0000u);
+        }
+        return h;
+    };
+    f32x8 x;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) x[e] = v[e];
+    SplitFrag3 f;
+    f.hi = __builtin_convertvector(x, bf16x8);
+    const f32x8 r1 = x - widen(f.hi);
+    f.mid = __builtin_convertvector(r1, bf16x8);
+    f.lo = __builtin_convertvector(r1 - widen(f.mid), bf16x8);
+    return f;
+}
+
+// acc + A*B keeping every product term down to 2^-16 of hi*hi (six MFMA
+// issues, the smallest terms first).
+__device__ inline f32x4 mma_split3(const SplitFrag3& a, const SplitFrag3& b, f32x4 acc) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.mid, b.mid, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.lo, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.lo, b.hi, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.mid, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.mid, b.hi, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, acc, 0, 0, 0);
+    return acc;
+}
+
 // Orders one wave's LDS writes before its later LDS reads of locations other
 // lanes wrote.  The hardware already executes a wave's LDS instructions in
 // order; the fence stops the compiler from forwarding or reusing values across

@@ -117,29 +117,57 @@ void SourcePlanBuffers::build(const float* kv_f32, const std::vector<double>& e1
     plan.kt_list = kt_list.as<short>();
 }
 
-void AnglePlanBuffers::build(int Nmu, int Nom, hipStream_t s) {
-    // Su, Su (chained columns), Sv (chained columns); each zero padded to 32x32
-    std::vector<float> pad(3 * 32 * 32, 0.f);
+namespace {
+// Round to the nearest BF16 (ties to even), as a bit pattern.
+uint16_t bf16_bits(double v) {
+    const float f = static_cast<float>(v);
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+double bf16_value(uint16_t b) {
+    const uint32_t u = static_cast<uint32_t>(b) << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+}  // namespace
+
+void AnglePlanBuffers::build(int Nmu, int Nom, hipStream_t) {
+    // Su, Su (chained columns), Sv (chained columns); each zero padded to
+    // 32x32 and split into hi + mid + lo from the FP64 basis.
+    std::vector<double> pad(3 * 32 * 32, 0.0);
     const auto su = sine_basis(Nom), sv = sine_basis(Nmu);
     for (int r = 0; r < 32; ++r)
         for (int c = 0; c < 32; ++c) {
             const int cc = chain_index(c);
-            if (r < Nom && c < Nom) pad[r * 32 + c] = static_cast<float>(su[r * Nom + c]);
-            if (r < Nom && cc < Nom) pad[1024 + r * 32 + c] = static_cast<float>(su[r * Nom + cc]);
-            if (r < Nmu && cc < Nmu) pad[2048 + r * 32 + c] = static_cast<float>(sv[r * Nmu + cc]);
+            if (r < Nom && c < Nom) pad[r * 32 + c] = su[r * Nom + c];
+            if (r < Nom && cc < Nom) pad[1024 + r * 32 + c] = su[r * Nom + cc];
+            if (r < Nmu && cc < Nmu) pad[2048 + r * 32 + c] = sv[r * Nmu + cc];
         }
-    h.ensure(pad.size() * sizeof(__bf16));
-    l.ensure(pad.size() * sizeof(__bf16));
-    upload(scratch, pad);
-    launch_split(scratch.as<float>(), h.as<__bf16>(), l.as<__bf16>(), static_cast<long long>(pad.size()), s);
-    BFPN_HIP(hipStreamSynchronize(s));
-    scratch.release();
-    plan.su_h = h.as<__bf16>();
-    plan.su_l = l.as<__bf16>();
-    plan.sup_h = h.as<__bf16>() + 1024;
-    plan.sup_l = l.as<__bf16>() + 1024;
-    plan.svp_h = h.as<__bf16>() + 2048;
-    plan.svp_l = l.as<__bf16>() + 2048;
+    std::vector<uint16_t> vh(pad.size()), vm(pad.size()), vl(pad.size());
+    for (size_t k = 0; k < pad.size(); ++k) {
+        vh[k] = bf16_bits(pad[k]);
+        const double r1 = pad[k] - bf16_value(vh[k]);
+        vm[k] = bf16_bits(r1);
+        vl[k] = bf16_bits(r1 - bf16_value(vm[k]));
+    }
+    upload(h, vh);
+    upload(m, vm);
+    upload(l, vl);
+    const __bf16* H = h.as<__bf16>();
+    const __bf16* M = m.as<__bf16>();
+    const __bf16* L = l.as<__bf16>();
+    plan.su_h = H;
+    plan.su_m = M;
+    plan.su_l = L;
+    plan.sup_h = H + 1024;
+    plan.sup_m = M + 1024;
+    plan.sup_l = L + 1024;
+    plan.svp_h = H + 2048;
+    plan.svp_m = M + 2048;
+    plan.svp_l = L + 2048;
 }
 
 void validate_params(const bfpn_grid& g, const bfpn_phys& p) {
