@@ -19,7 +19,7 @@ _lib = None
 
 class ServerOpts(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_size_t), ("max_active", C.c_int), ("slice_steps", C.c_int),
-                ("precision", C.c_int)]
+                ("precision", C.c_int), ("fill_cells", C.c_size_t)]
 
 
 class ServerStats(C.Structure):
@@ -279,8 +279,8 @@ class Server:
     """Continuous-batching request loop (bfpn_server): submit beams, wait for IDDs."""
 
     def __init__(self, max_active: int = 8, slice_steps: int = 8, precision: int = FP32,
-                 hbm_budget_bytes: int = 0):
-        opts = ServerOpts(hbm_budget_bytes, max_active, slice_steps, precision)
+                 hbm_budget_bytes: int = 0, fill_cells: int = 0):
+        opts = ServerOpts(hbm_budget_bytes, max_active, slice_steps, precision, fill_cells)
         h = C.c_void_p(0)
         check(lib().bfpn_server_create(C.byref(opts), C.byref(h)))
         self._h = h

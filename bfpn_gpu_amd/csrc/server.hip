@@ -3,15 +3,20 @@
 // The reference has no batching or serving mode (one beam per process).  This
 // runtime serves many independent beam solves on one GPU:
 //  * requests are queued by bfpn_server_submit from any thread;
-//  * a scheduler thread admits the oldest queued request as soon as a slot is
-//    free and its device memory fits the HBM budget (state + source scratch of
-//    the beam, plus its physics tables unless a cached copy matches);
+//  * a preparer thread admits the queued request with the least total work
+//    (steps x cells) as soon as a slot is free and its device memory fits the
+//    HBM budget (state + source scratch of
+//    the beam, plus its physics tables unless a cached copy matches), builds
+//    the tables and the beam's context -- host work of up to seconds that
+//    overlaps the GPU work of the running beams;
 //  * physics tables (material tables, eq15 coefficients, split-BF16 source
 //    operands, CN factors) are shared between beams with equal grid, step,
 //    material and flags, and kept in an LRU cache inside the budget;
 //  * every admitted beam keeps its whole phase space resident and runs on its
-//    own HIP stream; the scheduler enqueues slices of depth steps round-robin
-//    and polls completion events, so work of different beams overlaps on the
+//    own HIP stream; the scheduler thread enqueues slices of depth steps,
+//    shortest remaining work (steps x cells) first, to as many beams as it
+//    takes to keep `fill_cells` cells in flight, and polls completion events,
+//    so small requests finish early, work of different beams overlaps on the
 //    device and newly arrived requests join while others are running;
 //  * a finished beam's IDD history is copied out, its memory freed, and the
 //    slot handed to the next request.
@@ -74,25 +79,30 @@ using namespace bfpn;
 struct bfpn_server {
     bfpn_server_opts opts{};
     std::mutex mu;
-    std::condition_variable cv_done;
-    std::condition_variable cv_work;
-    std::deque<std::shared_ptr<Request>> pending;
-    std::vector<std::shared_ptr<Request>> active;
+    std::condition_variable cv_done;  // a request finished
+    std::condition_variable cv_work;  // work for the scheduler
+    std::condition_variable cv_prep;  // work for the preparer
+    std::deque<std::shared_ptr<Request>> pending;  // submitted
+    std::deque<std::shared_ptr<Request>> ready;    // prepared, not yet scheduled
+    std::vector<std::shared_ptr<Request>> active;  // scheduled (only the scheduler mutates it)
+    int preparing = 0;
     std::map<uint64_t, std::shared_ptr<Request>> finished;
     std::map<std::string, CacheEntry> cache;
     std::list<std::string> lru;  // front = most recent
-    size_t used = 0;             // bytes of active contexts + cached physics
+    size_t used = 0;             // bytes of admitted contexts + cached physics
     uint64_t next_id = 1;
     bool stopping = false;
-    std::thread worker;
+    std::thread scheduler, preparer;
     bfpn_server_stats stats{};
+
+    int admitted() const { return preparing + static_cast<int>(ready.size() + active.size()); }
 
     void touch(const std::string& key) {
         lru.remove(key);
         lru.push_front(key);
     }
 
-    // Frees cached physics tables that no active beam uses, oldest first,
+    // Frees cached physics tables that no admitted beam uses, oldest first,
     // until `need` more bytes fit.  Called with the lock held.
     void evict_for(size_t need) {
         for (auto it = lru.rbegin(); it != lru.rend() && used + need > opts.hbm_budget_bytes;) {
@@ -107,39 +117,62 @@ struct bfpn_server {
         }
     }
 
-    // Admits queued requests while slots and memory allow.  Lock held on entry.
-    void admit(std::unique_lock<std::mutex>& lk) {
-        while (!pending.empty() && static_cast<int>(active.size()) < opts.max_active) {
-            auto req = pending.front();
+    void fail_locked(const std::shared_ptr<Request>& r, const std::string& msg) {
+        r->status = 3;
+        r->error = msg;
+        r->t_done = Clock::now();
+        finished[r->id] = r;
+        ++stats.failed;
+        cv_done.notify_all();
+    }
+
+    static double total_cells(const Request& r) {
+        const double nyz = static_cast<double>(r.grid.Ny + 1) * (r.grid.Nz + 1);
+        return static_cast<double>(r.steps) * r.grid.Ng * nyz * r.grid.Nmu * r.grid.Nom;
+    }
+
+    // Preparer thread: admission, physics tables and contexts; the queued
+    // request with the least total work first (like the scheduler's order).
+    void prepare_loop() {
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            if (pending.empty() || admitted() >= opts.max_active) {
+                if (stopping && pending.empty()) break;
+                cv_prep.wait(lk);
+                continue;
+            }
+            auto pick = std::min_element(pending.begin(), pending.end(),
+                                         [](const std::shared_ptr<Request>& a,
+                                            const std::shared_ptr<Request>& b) {
+                                             return total_cells(*a) < total_cells(*b);
+                                         });
+            auto req = *pick;
             const std::string key = physics_key(req->grid, req->phys, req->data, opts.precision);
             const bool cached = cache.count(key) != 0;
             const size_t phys_need = cached ? 0 : physics_bytes(req->grid, req->phys, opts.precision);
             const size_t need = req->ctx_bytes + phys_need;
             if (used + need > opts.hbm_budget_bytes) evict_for(need);
             if (used + need > opts.hbm_budget_bytes) {
-                if (!active.empty()) return;  // wait for a running beam to finish
-                pending.pop_front();
-                req->status = 3;
-                req->error = "request does not fit the HBM budget (" + std::to_string(need >> 20) +
-                             " MiB > " + std::to_string((opts.hbm_budget_bytes - used) >> 20) + " MiB free)";
-                req->t_done = Clock::now();
-                finished[req->id] = req;
-                ++stats.failed;
-                cv_done.notify_all();
+                if (admitted() > 0) {  // wait for a running beam to finish
+                    cv_prep.wait(lk);
+                    continue;
+                }
+                pending.erase(pick);
+                fail_locked(req, "request does not fit the HBM budget (" +
+                                     std::to_string(need >> 20) + " MiB > " +
+                                     std::to_string((opts.hbm_budget_bytes - used) >> 20) +
+                                     " MiB free)");
                 continue;
             }
-            pending.pop_front();
+            pending.erase(pick);
             used += need;
-            lk.unlock();
+            ++preparing;
             std::shared_ptr<PhysicsBase> physics;
+            if (cached) physics = cache[key].physics;
+            lk.unlock();
             size_t physics_real = 0;
             std::string err;
             try {
-                {
-                    std::lock_guard<std::mutex> g(mu);
-                    auto c = cache.find(key);
-                    if (c != cache.end()) physics = c->second.physics;
-                }
                 if (!physics) {
                     physics = make_physics(req->grid, req->phys, req->data, opts.precision);
                     physics_real = physics->device_bytes();
@@ -151,43 +184,36 @@ struct bfpn_server {
                 err = e.what();
             }
             lk.lock();
+            --preparing;
             if (!err.empty()) {
                 used -= need;
                 req->ctx.reset();
-                req->status = 3;
-                req->error = err;
-                req->t_done = Clock::now();
-                finished[req->id] = req;
-                ++stats.failed;
-                cv_done.notify_all();
+                req->physics.reset();
+                fail_locked(req, err);
+                cv_work.notify_one();
                 continue;
             }
-            if (phys_need) {
-                if (!cache.count(key)) {  // keep the larger of estimate and real size
-                    const size_t charged = std::max(phys_need, physics_real);
-                    used = used - phys_need + charged;
-                    cache[key] = CacheEntry{physics, charged};
-                } else {
-                    used -= phys_need;  // another admission cached it meanwhile
-                }
+            if (phys_need) {  // keep the larger of estimate and real size
+                const size_t charged = std::max(phys_need, physics_real);
+                used = used - phys_need + charged;
+                cache[key] = CacheEntry{physics, charged};
             }
             touch(key);
             req->status = 1;
             req->t_admit = Clock::now();
-            active.push_back(req);
-            stats.peak_active = std::max(stats.peak_active, static_cast<int>(active.size()));
+            ready.push_back(req);
+            stats.peak_active = std::max(stats.peak_active, static_cast<int>(ready.size() + active.size()));
+            cv_work.notify_one();
         }
     }
 
-    // Enqueues the next slice of a running request; returns true if work was issued.
-    bool issue(Request& r) {
+    // Enqueues the next slice of a running request.
+    void issue(Request& r) {
         const int n = std::min(opts.slice_steps, r.steps - r.issued);
-        if (n <= 0) return false;
         r.ctx->advance(n);
         BFPN_HIP(hipEventRecord(r.slice_done, r.ctx->stream));
         r.issued += n;
         r.in_flight = true;
-        return true;
     }
 
     void finish(const std::shared_ptr<Request>& r) {
@@ -208,6 +234,7 @@ struct bfpn_server {
         ++stats.completed;
         finished[r->id] = r;
         cv_done.notify_all();
+        cv_prep.notify_one();
     }
 
     void fail(const std::shared_ptr<Request>& r, const std::string& msg) {
@@ -216,44 +243,69 @@ struct bfpn_server {
         r->ctx.reset();
         r->physics.reset();
         std::lock_guard<std::mutex> g(mu);
-        r->status = 3;
-        r->error = msg;
-        r->t_done = Clock::now();
         used -= r->ctx_bytes;
-        ++stats.failed;
-        finished[r->id] = r;
-        cv_done.notify_all();
+        fail_locked(r, msg);
+        cv_prep.notify_one();
     }
 
-    void run() {
+    static double remaining_cells(const Request& r) {
+        return static_cast<double>(r.steps - r.issued) * static_cast<double>(r.ctx->cells);
+    }
+
+    // Scheduler thread: slices, shortest remaining work first.
+    void schedule_loop() {
         std::unique_lock<std::mutex> lk(mu);
         for (;;) {
-            if (pending.empty() && active.empty()) {
-                if (stopping) break;
+            while (!ready.empty()) {
+                active.push_back(ready.front());
+                ready.pop_front();
+            }
+            if (active.empty()) {
+                if (stopping && pending.empty() && preparing == 0) break;
                 cv_work.wait(lk);
                 continue;
             }
-            admit(lk);
-            auto running = active;  // snapshot; only this thread mutates `active`
             lk.unlock();
             bool progressed = false;
             std::vector<std::shared_ptr<Request>> done;
-            for (auto& r : running) {
+            double in_flight = 0.0;
+            std::vector<Request*> idle;
+            for (auto& r : active) {
                 try {
                     if (r->in_flight) {
                         const hipError_t q = hipEventQuery(r->slice_done);
-                        if (q == hipErrorNotReady) continue;
+                        if (q == hipErrorNotReady) {
+                            in_flight += static_cast<double>(r->ctx->cells);
+                            continue;
+                        }
                         BFPN_HIP(q);
                         r->in_flight = false;
+                        progressed = true;
                     }
                     if (r->issued >= r->steps) {
                         done.push_back(r);
                         continue;
                     }
-                    progressed |= issue(*r);
+                    idle.push_back(r.get());
                 } catch (const std::exception& e) {
                     (void)hipGetLastError();
                     fail(r, e.what());
+                    progressed = true;
+                }
+            }
+            std::sort(idle.begin(), idle.end(), [](const Request* a, const Request* b) {
+                return remaining_cells(*a) < remaining_cells(*b);
+            });
+            for (Request* r : idle) {
+                if (in_flight > 0.0 && in_flight >= static_cast<double>(opts.fill_cells)) break;
+                try {
+                    issue(*r);
+                    in_flight += static_cast<double>(r->ctx->cells);
+                    progressed = true;
+                } catch (const std::exception& e) {
+                    (void)hipGetLastError();
+                    for (auto& sp : active)
+                        if (sp.get() == r) fail(sp, e.what());
                     progressed = true;
                 }
             }
@@ -292,7 +344,9 @@ BFPN_API bfpn_status bfpn_server_create(const bfpn_server_opts* opts, bfpn_serve
             BFPN_HIP(hipMemGetInfo(&free_b, &total_b));
             s->opts.hbm_budget_bytes = free_b - free_b / 10;
         }
-        s->worker = std::thread([s] { s->run(); });
+        if (s->opts.fill_cells == 0) s->opts.fill_cells = 200000000;
+        s->scheduler = std::thread([s] { s->schedule_loop(); });
+        s->preparer = std::thread([s] { s->prepare_loop(); });
         *out = s;
     });
 }
@@ -317,7 +371,7 @@ BFPN_API bfpn_status bfpn_server_submit(bfpn_server* s, const bfpn_grid* grid, c
         s->pending.push_back(r);
         ++s->stats.submitted;
         *id = r->id;
-        s->cv_work.notify_one();
+        s->cv_prep.notify_one();
     });
 }
 
@@ -328,9 +382,11 @@ BFPN_API int bfpn_server_poll(bfpn_server* s, uint64_t id) {
     if (it != s->finished.end()) return it->second->status;
     for (auto& r : s->active)
         if (r->id == id) return 1;
+    for (auto& r : s->ready)
+        if (r->id == id) return 1;
     for (auto& r : s->pending)
         if (r->id == id) return 0;
-    return -1;
+    return 0;  // being prepared
 }
 
 BFPN_API bfpn_status bfpn_server_wait(bfpn_server* s, uint64_t id, double timeout_s, double* idd,
@@ -363,7 +419,7 @@ BFPN_API bfpn_status bfpn_server_get_stats(bfpn_server* s, bfpn_server_stats* ou
         BFPN_REQUIRE(s && out, "null argument");
         std::lock_guard<std::mutex> g(s->mu);
         *out = s->stats;
-        out->active = static_cast<int>(s->active.size());
+        out->active = static_cast<int>(s->active.size() + s->ready.size()) + s->preparing;
         out->queued = static_cast<int>(s->pending.size());
         out->bytes_in_use = s->used;
         out->cached_tables = static_cast<int>(s->cache.size());
@@ -387,8 +443,14 @@ BFPN_API void bfpn_server_destroy(bfpn_server* s) {
         std::lock_guard<std::mutex> g(s->mu);
         s->stopping = true;
         s->cv_work.notify_one();
+        s->cv_prep.notify_one();
     }
-    if (s->worker.joinable()) s->worker.join();
+    if (s->preparer.joinable()) s->preparer.join();
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->cv_work.notify_one();
+    }
+    if (s->scheduler.joinable()) s->scheduler.join();
     delete s;
 }
 

@@ -7,6 +7,8 @@
 
 #pragma clang fp contract(off)
 
+#include <thread>
+
 #include "tables.hpp"
 
 #include <algorithm>
@@ -268,9 +270,11 @@ SourceKernels build_source_kernels(int Ng, double dg, int nang, const std::vecto
         weight_in[i] = std::pow(1.0 + iy * iy + iz * iz, -1.5);
     }
     k.v.assign(n * na * na, 0.0);
-    for (int g = 0; g + 1 < Ng; ++g) {
+    // Groups are independent: build them on all host threads (same libm
+    // calls per entry, so the table does not depend on the thread count).
+    auto build_group = [&](int g) {
         const double shape = mu[g * 3 + 0], width = mu[g * 3 + 1];
-        if (shape == 0.0 || width == 0.0) continue;
+        if (shape == 0.0 || width == 0.0) return;
         double* Kg = k.v.data() + static_cast<size_t>(g) * na * na;
         for (size_t o = 0; o < na; ++o) {
             double total = 0.0;
@@ -287,7 +291,14 @@ SourceKernels build_source_kernels(int Ng, double dg, int nang, const std::vecto
                     Kg[i * na + o] = w < kAngularCut ? 0.0 : w;
                 }
         }
-    }
+    };
+    const int workers = std::max(1, std::min<int>(Ng - 1, static_cast<int>(std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    for (int w = 0; w < workers; ++w)
+        pool.emplace_back([&, w] {
+            for (int g = w; g + 1 < Ng; g += workers) build_group(g);
+        });
+    for (auto& t : pool) t.join();
     return k;
 }
 

@@ -285,6 +285,8 @@ typedef struct bfpn_server_opts {
     int max_active;           /* beams resident and running at once            */
     int slice_steps;          /* depth steps enqueued per beam per turn        */
     int precision;            /* BFPN_FP32 or BFPN_FP64                        */
+    size_t fill_cells;        /* phase-space cells kept in flight, shortest
+                                 remaining work first (0 = 2e8)               */
 } bfpn_server_opts;
 
 typedef struct bfpn_server_stats {

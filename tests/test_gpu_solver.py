@@ -141,6 +141,19 @@ def test_fp32_full_run_meets_accuracy_gate(gpu, golden):
         assert abs(mg[k] - mr[k]) <= 0.01, (k, mg[k], mr[k])
 
 
+def test_fp32_long_run_tracks_fp64(gpu):
+    """800 depth steps of a 230 MeV beam with 20x20 directions: the FP32-state
+    IDD stays within 1e-3 of the FP64 instantiation everywhere above 10 % of
+    its maximum (compounding operator errors would show as a steady drift)."""
+    case = small_case(Ny=4, Nz=4, Nmu=20, Nom=20, Ng=200, Nx=2000, time=16.0, Ly=4, Lz=4,
+                      flags={"eq15_straggling": 1})
+    with gpu.Solver(case, gpu.FP32) as a, gpu.Solver(case, gpu.FP64) as b:
+        ia, ib = a.run()[0], b.run()[0]
+    assert len(ia) == 800
+    cmp = metrics.compare(ia, ib, band=0.10)
+    assert cmp["max_rel"] <= 1e-3, cmp
+
+
 def test_cli_fp64_writes_the_reference_file(tmp_path, golden, gpu):
     """bfp_solver with the fixture's flags writes idd_output.txt in the
     reference's format (17 digits, x = (i+1) dt) with the golden values."""
