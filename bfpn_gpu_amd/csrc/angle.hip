@@ -44,29 +44,42 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                                                          const float* __restrict__ factors,
                                                          AnglePlan plan, int planes, int per_wave,
                                                          int nyz, int Nmu, int Nom) {
+    // The nine constant basis arrays (Su, Su chained, Sv chained; hi, mid,
+    // lo) live in LDS, shared by the work-group's four waves and read per
+    // stage: held in registers they would cost 72 VGPRs and half the
+    // occupancy.  Rows padded to 40 values (80 bytes) against bank conflicts.
+    __shared__ __attribute__((aligned(16))) __bf16 cst[9][32][40];
+    {
+        const __bf16* src[9] = {plan.su_h, plan.su_m, plan.su_l, plan.sup_h, plan.sup_m,
+                                plan.sup_l, plan.svp_h, plan.svp_m, plan.svp_l};
+        for (int e = threadIdx.x; e < 9 * 32 * 4; e += blockDim.x) {  // 16-byte chunks
+            const int a = e >> 7, r = (e >> 2) & 31, c = (e & 3) * 8;
+            *reinterpret_cast<bf16x8*>(&cst[a][r][c]) =
+                *reinterpret_cast<const bf16x8*>(src[a] + r * 32 + c);
+        }
+    }
+    __syncthreads();
     const int lane = threadIdx.x & 63, i = lane & 15, q = lane >> 4;
     int p = (blockIdx.x * 4 + (threadIdx.x >> 6)) * per_wave;
     const int p_end = min(planes, p + per_wave);
     if (p >= p_end) return;
     const int na = Nmu * Nom;
-    auto frag = [&](const __bf16* h, const __bf16* m, const __bf16* l, int t) {
-        const int off = (t * 16 + i) * 32 + 8 * q;
-        return SplitFrag3{*reinterpret_cast<const bf16x8*>(h + off),
-                          *reinterpret_cast<const bf16x8*>(m + off),
-                          *reinterpret_cast<const bf16x8*>(l + off)};
+    // Fragment of constant `a` (0 Su, 1 Su chained, 2 Sv chained), tile t,
+    // at compile-time offsets from the lane's base (row i, columns 8q..).
+    auto frag = [&](const __bf16* base, int a, int t) {
+        constexpr int kArr = 32 * 40, kTile = 16 * 40;
+        return SplitFrag3{*reinterpret_cast<const bf16x8*>(base + (3 * a) * kArr + t * kTile),
+                          *reinterpret_cast<const bf16x8*>(base + (3 * a + 1) * kArr + t * kTile),
+                          *reinterpret_cast<const bf16x8*>(base + (3 * a + 2) * kArr + t * kTile)};
     };
-    SplitFrag3 su[NT], sup[NT], svp[MT];
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        su[t] = frag(plan.su_h, plan.su_m, plan.su_l, t);
-        sup[t] = frag(plan.sup_h, plan.sup_m, plan.sup_l, t);
-    }
-#pragma unroll
-    for (int t = 0; t < MT; ++t) svp[t] = frag(plan.svp_h, plan.svp_m, plan.svp_l, t);
 
     int g_cur = -1;
     float cf[MT][NT][4];
     for (; p < p_end; ++p) {
+        // an opaque zero per plane keeps the constant reads inside the loop
+        int oz;
+        asm volatile("v_mov_b32 %0, 0" : "=v"(oz));
+        const __bf16* cb = &cst[0][i][8 * q] + oz;
         float* X = F + static_cast<long long>(p) * na;
         const int g = p / nyz;
         if (g != g_cur) {  // CN factors of this group, in the D layout
@@ -113,7 +126,7 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) d1[mt][nt] = mma_split3(xa[mt], su[nt], zero4());
+            for (int nt = 0; nt < NT; ++nt) d1[mt][nt] = mma_split3(xa[mt], frag(cb, 0, nt), zero4());
         // stage 2: Z[nu][mu] = C[nu][mu] sum_v Sv[nu][v] Y[v][mu]
         f32x4 d2[MT][NT];
 #pragma unroll
@@ -127,7 +140,7 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
             const SplitFrag3 b = split8_3(y);
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
-                d2[mt][nt] = mma_split3(svp[mt], b, zero4());
+                d2[mt][nt] = mma_split3(frag(cb, 2, mt), b, zero4());
 #pragma unroll
                 for (int r = 0; r < 4; ++r) d2[mt][nt][r] *= cf[mt][nt][r];
             }
@@ -144,7 +157,7 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
             }
             const SplitFrag3 a = split8_3(z);
 #pragma unroll
-            for (int mt = 0; mt < MT; ++mt) d3[nt][mt] = mma_split3(a, svp[mt], zero4());
+            for (int mt = 0; mt < MT; ++mt) d3[nt][mt] = mma_split3(a, frag(cb, 2, mt), zero4());
         }
         // stage 4, transposed: X'^T[a][v] = sum_mu Su[mu][a] W[v][mu] (operands
         // swapped), so each lane holds 4 consecutive a of one row v; in place
@@ -160,7 +173,7 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
             const int v = mt * 16 + i;
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
-                const f32x4 d = mma_split3(sup[nt], b, zero4());
+                const f32x4 d = mma_split3(frag(cb, 1, nt), b, zero4());
                 const int a0 = nt * 16 + 4 * q;
                 if (V4) {
                     if (v < Nmu && a0 < Nom)

@@ -207,9 +207,15 @@ struct bfpn_server {
         }
     }
 
-    // Enqueues the next slice of a running request.
+    // Enqueues the next slice of a running request: slice_steps depth steps
+    // of a beam of up to 5e7 cells, proportionally fewer (at least one) for
+    // larger beams, so a slice costs about the same whatever the beam and a
+    // newly arrived small request never waits long behind a large one.
     void issue(Request& r) {
-        const int n = std::min(opts.slice_steps, r.steps - r.issued);
+        const double scale = 5e7 / static_cast<double>(std::max<size_t>(r.ctx->cells, 1));
+        const int per = std::max(1, std::min(opts.slice_steps,
+                                             static_cast<int>(opts.slice_steps * scale)));
+        const int n = std::min(per, r.steps - r.issued);
         r.ctx->advance(n);
         BFPN_HIP(hipEventRecord(r.slice_done, r.ctx->stream));
         r.issued += n;
@@ -344,7 +350,7 @@ BFPN_API bfpn_status bfpn_server_create(const bfpn_server_opts* opts, bfpn_serve
             BFPN_HIP(hipMemGetInfo(&free_b, &total_b));
             s->opts.hbm_budget_bytes = free_b - free_b / 10;
         }
-        if (s->opts.fill_cells == 0) s->opts.fill_cells = 200000000;
+        if (s->opts.fill_cells == 0) s->opts.fill_cells = 40000000;
         s->scheduler = std::thread([s] { s->schedule_loop(); });
         s->preparer = std::thread([s] { s->prepare_loop(); });
         *out = s;

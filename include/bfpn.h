@@ -286,7 +286,8 @@ typedef struct bfpn_server_opts {
     int slice_steps;          /* depth steps enqueued per beam per turn        */
     int precision;            /* BFPN_FP32 or BFPN_FP64                        */
     size_t fill_cells;        /* phase-space cells kept in flight, shortest
-                                 remaining work first (0 = 2e8)               */
+                                 remaining work first (0 = 4e7: a beam that
+                                 fills the GPU runs alone)                    */
 } bfpn_server_opts;
 
 typedef struct bfpn_server_stats {
