@@ -289,17 +289,35 @@ def run_accuracy(args):
     if args.material == "bone":
         case = dataclasses.replace(case, density=1.85)
     res = {}
-    for name, prec in (("fp32", bg.FP32), ("fp64", bg.FP64)):
+    runs = (("fp32", bg.FP32), ("fp64", bg.FP64))
+    if args.only:
+        runs = tuple(r for r in runs if r[0] == args.only)
+    for name, prec in runs:
         t0 = time.perf_counter()
         with bg.Solver(case, prec) as s:
-            idd, _ = s.run()
+            total = case.num_steps()
+            done = 0
+            while done < total:  # in chunks, with a progress line per chunk
+                n = min(200, total - done)
+                s.advance(n)
+                done += n
+                idd, _ = s.history()  # synchronizes
+                log(f"{name}: step {done}/{total}, {time.perf_counter() - t0:.0f} s")
         res[name] = (idd, time.perf_counter() - t0)
         log(f"{name}: {len(idd)} steps in {res[name][1]:.1f} s")
         if args.save:
             os.makedirs(args.save, exist_ok=True)
             np.save(os.path.join(args.save, f"idd_{args.grid}_{args.material}_{name}.npy"), idd)
-    err = idd_error(res["fp32"][0], res["fp64"][0], case.dt)
     from bfpn_gpu_amd import metrics
+
+    if args.only:  # one precision per run: the IDD is saved for a later --compare
+        idd, sec = res[args.only]
+        x = (np.arange(len(idd)) + 1) * case.dt
+        return {"metric": f"{args.only} full-depth run (IDD saved for comparison)",
+                "config": {"workload": f"{args.grid} grid ({ny + 1}x{ny + 1} nodes), "
+                                       f"{args.material}, depth {depth} cm", "cells": case.cells},
+                "steps": len(idd), "seconds": sec, "metrics_cm": metrics.analyze(x, idd)}
+    err = idd_error(res["fp32"][0], res["fp64"][0], case.dt)
 
     x = (np.arange(len(res["fp64"][0])) + 1) * case.dt
     return {"metric": "IDD error of the FP32-state solver vs the FP64 instantiation",
@@ -310,6 +328,22 @@ def run_accuracy(args):
             "max_rel_err_above_10pct": err["max_rel"], "max_shift_mm_BP_D90_D20": err["max_shift_mm"],
             "fp64_metrics_cm": metrics.analyze(x, res["fp64"][0]),
             "seconds": {"fp32": res["fp32"][1], "fp64": res["fp64"][1]},
+            "gate": {"max_rel_err": 0.005, "max_shift_mm": 0.1,
+                     "pass": bool(err["max_rel"] <= 0.005 and err["max_shift_mm"] <= 0.1)}}
+
+
+def compare_saved(args):
+    """FP32 vs FP64 from IDDs saved by two --only runs (no GPU needed)."""
+    import numpy as np
+
+    nx = 2000 if args.grid == "beam" else 4000
+    tag = f"{args.grid}_{args.material}"
+    a = np.load(os.path.join(args.compare, f"idd_{tag}_fp32.npy"))
+    b = np.load(os.path.join(args.compare, f"idd_{tag}_fp64.npy"))
+    err = idd_error(a, b, 40.0 / nx)
+    return {"metric": "IDD error of the FP32-state solver vs the FP64 instantiation",
+            "config": {"workload": tag, "steps": len(b)},
+            "max_rel_err_above_10pct": err["max_rel"], "max_shift_mm_BP_D90_D20": err["max_shift_mm"],
             "gate": {"max_rel_err": 0.005, "max_shift_mm": 0.1,
                      "pass": bool(err["max_rel"] <= 0.005 and err["max_shift_mm"] <= 0.1)}}
 
@@ -344,6 +378,10 @@ def main():
     ap.add_argument("--grid", choices=("beam", "paper"), default="beam")
     ap.add_argument("--material", choices=("water", "bone"), default="water")
     ap.add_argument("--save", default=None, help="directory for the accuracy IDD arrays")
+    ap.add_argument("--only", choices=("fp32", "fp64"), default=None,
+                    help="accuracy: run one precision and save its IDD (needs --save)")
+    ap.add_argument("--compare", default=None,
+                    help="accuracy: compare the fp32/fp64 IDDs saved in this directory")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--beams", type=int, default=8)
@@ -377,6 +415,8 @@ def main():
         out = run_beamlets(args)
     elif args.workload == "serve":
         out = run_serve(args)
+    elif args.workload == "accuracy" and args.compare:
+        out = compare_saved(args)
     elif args.workload == "accuracy":
         out = run_accuracy(args)
     else:
