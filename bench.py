@@ -116,10 +116,14 @@ def timed_steps(solver, steps, warmup):
 
 
 def phases_of(solver, steps):
+    """Per-kernel device time of `steps` more steps, issued in order on one
+    stream (overlap off) so that each interval holds one kernel alone."""
+    solver.set_overlap(False)
     solver.set_profiling(True)
     solver.advance(steps)
     stats = solver.phase_stats()
     solver.set_profiling(False)
+    solver.set_overlap(True)
     phases = {}
     for name, (sec, calls, byt) in stats.items():
         if calls:
@@ -150,7 +154,8 @@ def run_resident(args, case, workload, with_traffic):
     ach = phases[dom]["GBps"]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": load_traffic(dom) if with_traffic else None}
+                "traffic": load_traffic(dom) if with_traffic else None,
+                "measured": "HIP events around each launch, one stream, after the timed steps"}
     return {
         "metric": METRIC,
         "value": cells * args.steps / elapsed,

@@ -90,6 +90,7 @@ def lib():
     L.bfpn_stream.argtypes = [vp]
     L.bfpn_stream.restype = vp
     L.bfpn_set_profiling.argtypes = [vp, i]
+    L.bfpn_set_overlap.argtypes = [vp, i]
     L.bfpn_phase_stats.argtypes = [vp, vp, vp, vp]
     L.bfpn_stopping_power.argtypes = [C.c_char_p, i, d, vp]
     L.bfpn_server_create.argtypes = [C.POINTER(ServerOpts), C.POINTER(vp)]
@@ -259,6 +260,10 @@ class Solver:
 
     def set_profiling(self, on: bool = True) -> None:
         check(lib().bfpn_set_profiling(self._h, int(on)))
+
+    def set_overlap(self, on: bool = True) -> None:
+        """Secondary march on its own stream, overlapping the next primary step."""
+        check(lib().bfpn_set_overlap(self._h, int(on)))
 
     PHASES = ("energy", "transport", "angle", "source", "dose")
 

@@ -8,6 +8,8 @@ Flags:
   --offload-arch=gfx950                       MI355X (CDNA4) code objects
   -Xclang -target-feature -Xclang -packed-fp32-ops
                                               no packed FP32 instructions
+  -mllvm -amdgpu-mfma-vgpr-form              MFMA results in ordinary VGPRs
+                                              (no copies out of AGPRs)
   -fvisibility=hidden                         only the BFPN_API entry points
                                               of include/bfpn.h are exported
 """
@@ -34,7 +36,7 @@ LIB_SOURCES = ["tables.cpp", "energy.hip", "source.hip", "transport.hip", "angle
 COMMON = ["-std=c++17", "-O3", "-fPIC", "-Wall", "-Wno-unused-result",
           f"-I{os.path.join(ROOT, 'include')}"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-          "-fvisibility=hidden", "-fno-gpu-rdc"]
+          "-mllvm", "-amdgpu-mfma-vgpr-form", "-fvisibility=hidden", "-fno-gpu-rdc"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -95,18 +95,20 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                         cf[mt][nt][r] = (nu < Nmu && mu < Nom) ? C[nu * Nom + mu] : 0.f;
                     }
         }
-        // stage 1: Y[v][mu] = sum_a X[v][a] Su[mu][a]
+        // stage 1: Y[v][mu] = sum_a X[v][a] Su[mu][a].  The padding rows and
+        // columns of the 32 x 32 tile load clamped copies of plane values:
+        // the bases' zero padding multiplies them by exact zeros (columns
+        // a >= Nom here, rows v >= Nmu in stage 2), so no lane is masked.
         SplitFrag3 xa[MT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
-            const int v = mt * 16 + i;
+            const float* row = X + min(mt * 16 + i, Nmu - 1) * Nom;
             float x[8];
             if (V4) {
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
-                    float4 t = make_float4(0.f, 0.f, 0.f, 0.f);
-                    if (v < Nmu && 8 * q + 4 * h < Nom)
-                        t = *reinterpret_cast<const float4*>(X + v * Nom + 8 * q + 4 * h);
+                    const float4 t =
+                        *reinterpret_cast<const float4*>(row + min(8 * q + 4 * h, Nom - 4));
                     x[4 * h] = t.x;
                     x[4 * h + 1] = t.y;
                     x[4 * h + 2] = t.z;
@@ -114,11 +116,7 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                 }
             } else {
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int a = 8 * q + e;
-                    x[e] = 0.f;
-                    if (v < Nmu && a < Nom) x[e] = X[v * Nom + a];
-                }
+                for (int e = 0; e < 8; ++e) x[e] = row[min(8 * q + e, Nom - 1)];
             }
             xa[mt] = split8_3(x);
         }

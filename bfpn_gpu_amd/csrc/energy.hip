@@ -39,39 +39,80 @@ __device__ inline double block_sum(double v, double* lds) {
     return t;
 }
 
-// Prefetch depth (groups in flight per lane): a register ring, slot k
+// LPT adjacent lanes per thread (4 with FP32 state on large lane counts):
+// 16-byte row loads and stores, four independent recurrences in flight per
+// thread, one set of coefficient loads for four lanes.
+template <typename T, int LPT>
+struct Pack {
+    T v[LPT];
+};
+
+template <typename T, int LPT>
+__device__ inline Pack<T, LPT> load_pack(const T* p) {
+    Pack<T, LPT> r;
+    if constexpr (LPT == 4) {
+        static_assert(sizeof(T) == 4, "4-lane packs are FP32");
+        const float4 q = *reinterpret_cast<const float4*>(p);
+        r.v[0] = q.x;
+        r.v[1] = q.y;
+        r.v[2] = q.z;
+        r.v[3] = q.w;
+    } else {
+        r.v[0] = *p;
+    }
+    return r;
+}
+
+template <typename T, int LPT>
+__device__ inline void store_pack(T* p, const double (&v)[LPT]) {
+    if constexpr (LPT == 4) {
+        *reinterpret_cast<float4*>(p) = make_float4(static_cast<float>(v[0]), static_cast<float>(v[1]),
+                                                    static_cast<float>(v[2]), static_cast<float>(v[3]));
+    } else {
+        *p = static_cast<T>(v[0]);
+    }
+}
+
+// Prefetch depth (groups in flight per thread): a register ring, slot k
 // refilled with group g - D right after group g consumed it.
-template <typename T>
-constexpr int prefetch_depth() { return sizeof(T) == 4 ? 16 : 8; }
+template <typename T, int LPT>
+constexpr int prefetch_depth() { return LPT == 4 ? 8 : (sizeof(T) == 4 ? 16 : 8); }
+
+// Lanes per thread for a sweep over `lanes` lanes of `elem`-byte state.
+inline int lanes_per_thread(long long lanes, size_t elem) {
+    return (elem == 4 && lanes % 4 == 0 && lanes >= (1LL << 18)) ? 4 : 1;
+}
 
 // Everything a group needs besides the state (recurrence coefficients and the
 // dose weights) sits in one EnergyCoef read through the scalar cache, so the
 // vector loads in flight are the state ring alone: the in-order load counter
 // then never waits on the ring to deliver a coefficient.
-template <typename T, bool SRC, bool STRAG, bool DOSE>
+template <typename T, int LPT, bool SRC, bool STRAG, bool DOSE>
 __global__ __launch_bounds__(kBlock) void energy_half_kernel(
     T* __restrict__ F, T* __restrict__ f, const EnergyCoef* __restrict__ coef,
     const T* __restrict__ Q1, int Ng, int lanes, const double* __restrict__ yzw, int nang,
     double* __restrict__ partials, int proxy) {
-    constexpr int D = prefetch_depth<T>();
+    constexpr int D = prefetch_depth<T, LPT>();
     __shared__ double lds[kBlock / 64];
-    const int lane = blockIdx.x * kBlock + threadIdx.x;
+    const int lane = (blockIdx.x * kBlock + threadIdx.x) * LPT;  // first of LPT lanes
     double acc_dose = 0.0, acc_proxy = 0.0;
     if (lane < lanes) {
-        T rF[D], rf[D], rQ[D];
+        Pack<T, LPT> rF[D], rf[D], rQ[D];
         // Loads are unconditional (groups below 0 re-read group 0 and are
         // never consumed) and full chunks run without branches, so the
         // compiler can count the loads in flight instead of draining them.
         // Row bases are wave-uniform (scalar), the lane a 32-bit offset.
         auto fetch = [&](int k, int g) {
             const long long base = static_cast<long long>(max(g, 0)) * lanes;
-            rF[k] = (F + base)[lane];
-            rf[k] = (f + base)[lane];
-            if (SRC) rQ[k] = (Q1 + base)[lane];
+            rF[k] = load_pack<T, LPT>(F + base + lane);
+            rf[k] = load_pack<T, LPT>(f + base + lane);
+            if (SRC) rQ[k] = load_pack<T, LPT>(Q1 + base + lane);
         };
 #pragma unroll
         for (int k = 0; k < D; ++k) fetch(k, Ng - 1 - k);
-        double P_hi = 0.0, R_hi = 0.0, hi_delta = 0.0;
+        double P_hi[LPT], R_hi[LPT], hi_delta[LPT], dose[LPT];
+#pragma unroll
+        for (int j = 0; j < LPT; ++j) P_hi[j] = R_hi[j] = hi_delta[j] = dose[j] = 0.0;
         // The linear forms run one group ahead through the scalar cache:
         // group g-1's are requested before group g's arithmetic (the
         // scheduling barrier keeps the request there), so their latency is
@@ -94,34 +135,46 @@ __global__ __launch_bounds__(kBlock) void energy_half_kernel(
             const Forms c = cn;
             cn = forms(max(g - 1, 0));
             __builtin_amdgcn_sched_barrier(0);
-            const double P = static_cast<double>(rF[k]);
-            const double R = static_cast<double>(rf[k]);
+            const Pack<T, LPT> cF = rF[k], cf = rf[k], cQ = rQ[k];
             // old F at g-1: next ring slot (slot 0 was refilled with g0-D)
-            const double P_lo = (STRAG && g > 0) ? static_cast<double>(rF[(k + 1) % D]) : 0.0;
-            const double src = SRC ? static_cast<double>(rQ[k]) : 0.0;
+            const Pack<T, LPT> lo = rF[(k + 1) % D];
             fetch(k, g - D);
-            double fv = c.af[0] * P_hi + c.af[1] * R_hi + c.af[2] * P + c.af[3] * R;
-            double Fv = c.aF[0] * P_hi + c.aF[1] * R_hi + c.aF[2] * P + c.aF[3] * R;
-            if (STRAG) {
-                fv += c.af[4] * P_lo;
-                Fv += c.aF[4] * P_lo;
+            double Fv[LPT], fv[LPT];
+#pragma unroll
+            for (int j = 0; j < LPT; ++j) {
+                const double P = static_cast<double>(cF.v[j]);
+                const double R = static_cast<double>(cf.v[j]);
+                double a = c.af[0] * P_hi[j] + c.af[1] * R_hi[j] + c.af[2] * P + c.af[3] * R;
+                double b = c.aF[0] * P_hi[j] + c.aF[1] * R_hi[j] + c.aF[2] * P + c.aF[3] * R;
+                if (STRAG) {
+                    const double P_lo = g > 0 ? static_cast<double>(lo.v[j]) : 0.0;
+                    a += c.af[4] * P_lo;
+                    b += c.aF[4] * P_lo;
+                }
+                if (SRC) {
+                    const double src = static_cast<double>(cQ.v[j]);
+                    a += c.af[5] * src;
+                    b += c.aF[5] * src;
+                }
+                a += c.af[6] * hi_delta[j];
+                b += c.aF[6] * hi_delta[j];
+                fv[j] = a;
+                Fv[j] = b;
+                P_hi[j] = P;
+                R_hi[j] = R;
+                hi_delta[j] = b - a;
             }
-            if (SRC) {
-                fv += c.af[5] * src;
-                Fv += c.aF[5] * src;
-            }
-            fv += c.af[6] * hi_delta;
-            Fv += c.aF[6] * hi_delta;
             const long long base = static_cast<long long>(g) * lanes;
-            (F + base)[lane] = static_cast<T>(Fv);
-            (f + base)[lane] = static_cast<T>(fv);
-            if (DOSE) {  // the lane's transverse weight is applied after the sweep
-                acc_dose += Fv * coef[g].w1 + fv * coef[g].w2;
-                acc_proxy += fv;
+            store_pack<T, LPT>(F + base + lane, Fv);
+            store_pack<T, LPT>(f + base + lane, fv);
+            if (DOSE) {  // the lanes' transverse weights are applied after the sweep
+                const double w1 = coef[g].w1, w2 = coef[g].w2;
+#pragma unroll
+                for (int j = 0; j < LPT; ++j) {
+                    dose[j] += Fv[j] * w1 + fv[j] * w2;
+                    acc_proxy += fv[j];
+                }
             }
-            P_hi = P;
-            R_hi = R;
-            hi_delta = Fv - fv;
         };
         int g0 = Ng - 1;
         for (; g0 >= D - 1; g0 -= D) {
@@ -133,7 +186,9 @@ __global__ __launch_bounds__(kBlock) void energy_half_kernel(
             if (k <= g0) step(k, g0 - k);
         // (loaded only now: a load issued before the ring would hold up its
         // first consumer behind every state load in flight)
-        if (DOSE) acc_dose *= yzw[lane / nang];
+        if (DOSE)
+#pragma unroll
+            for (int j = 0; j < LPT; ++j) acc_dose += dose[j] * yzw[(lane + j) / nang];
     }
     if (DOSE) {
         const double d = block_sum(acc_dose, lds);
@@ -145,23 +200,38 @@ __global__ __launch_bounds__(kBlock) void energy_half_kernel(
     }
 }
 
-template <typename T, bool SRC, bool STRAG>
+template <typename T, int LPT, bool SRC, bool STRAG>
 void dispatch_dose(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng, long long lanes,
                    const double* yzw, int nang, double* partials, bool proxy, hipStream_t s) {
-    const int blocks = energy_blocks(lanes);
+    const int blocks = energy_blocks(lanes, sizeof(T));
     BFPN_REQUIRE(lanes < (1LL << 31), "energy sweep indexes lanes with 32-bit offsets");
     if (partials)
-        hipLaunchKernelGGL((energy_half_kernel<T, SRC, STRAG, true>), dim3(blocks), dim3(kBlock), 0,
-                           s, F, f, coef, Q1, Ng, static_cast<int>(lanes), yzw, nang, partials,
-                           proxy ? 1 : 0);
+        hipLaunchKernelGGL((energy_half_kernel<T, LPT, SRC, STRAG, true>), dim3(blocks),
+                           dim3(kBlock), 0, s, F, f, coef, Q1, Ng, static_cast<int>(lanes), yzw,
+                           nang, partials, proxy ? 1 : 0);
     else
-        hipLaunchKernelGGL((energy_half_kernel<T, SRC, STRAG, false>), dim3(blocks), dim3(kBlock), 0,
-                           s, F, f, coef, Q1, Ng, static_cast<int>(lanes), yzw, nang, partials, 0);
+        hipLaunchKernelGGL((energy_half_kernel<T, LPT, SRC, STRAG, false>), dim3(blocks),
+                           dim3(kBlock), 0, s, F, f, coef, Q1, Ng, static_cast<int>(lanes), yzw,
+                           nang, partials, 0);
+}
+
+template <typename T, bool SRC, bool STRAG>
+void dispatch_lpt(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng, long long lanes,
+                  const double* yzw, int nang, double* partials, bool proxy, hipStream_t s) {
+    if constexpr (sizeof(T) == 4) {
+        if (lanes_per_thread(lanes, sizeof(T)) == 4)
+            return dispatch_dose<T, 4, SRC, STRAG>(F, f, coef, Q1, Ng, lanes, yzw, nang, partials,
+                                                   proxy, s);
+    }
+    dispatch_dose<T, 1, SRC, STRAG>(F, f, coef, Q1, Ng, lanes, yzw, nang, partials, proxy, s);
 }
 
 }  // namespace
 
-int energy_blocks(long long lanes) { return static_cast<int>((lanes + kBlock - 1) / kBlock); }
+int energy_blocks(long long lanes, size_t elem) {
+    const long long per_block = static_cast<long long>(kBlock) * lanes_per_thread(lanes, elem);
+    return static_cast<int>((lanes + per_block - 1) / per_block);
+}
 
 template <typename T>
 void launch_energy_half(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng, long long lanes,
@@ -170,17 +240,17 @@ void launch_energy_half(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng,
     if (lanes <= 0 || Ng <= 0) return;
     if (Q1) {
         if (straggling)
-            dispatch_dose<T, true, true>(F, f, coef, Q1, Ng, lanes, yz_weight, nang, dose_partials,
+            dispatch_lpt<T, true, true>(F, f, coef, Q1, Ng, lanes, yz_weight, nang, dose_partials,
                                          proxy, s);
         else
-            dispatch_dose<T, true, false>(F, f, coef, Q1, Ng, lanes, yz_weight, nang, dose_partials,
+            dispatch_lpt<T, true, false>(F, f, coef, Q1, Ng, lanes, yz_weight, nang, dose_partials,
                                           proxy, s);
     } else {
         if (straggling)
-            dispatch_dose<T, false, true>(F, f, coef, Q1, Ng, lanes, yz_weight, nang, dose_partials,
+            dispatch_lpt<T, false, true>(F, f, coef, Q1, Ng, lanes, yz_weight, nang, dose_partials,
                                           proxy, s);
         else
-            dispatch_dose<T, false, false>(F, f, coef, Q1, Ng, lanes, yz_weight, nang,
+            dispatch_lpt<T, false, false>(F, f, coef, Q1, Ng, lanes, yz_weight, nang,
                                            dose_partials, proxy, s);
     }
     BFPN_HIP(hipGetLastError());

@@ -14,7 +14,8 @@ template <typename T>
 void launch_energy_half(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng, long long lanes,
                         bool straggling, const double* yz_weight,
                         int nang, double* dose_partials, bool proxy, hipStream_t s);
-int energy_blocks(long long lanes);
+// Dose partials written by a sweep over `lanes` lanes of `elem`-byte state.
+int energy_blocks(long long lanes, size_t elem);
 
 // Split-BF16 operands of the FP32 source (matrix cores), built once per beam.
 struct SourcePlan {
@@ -44,8 +45,11 @@ void launch_kv_block_flags(const __bf16* kth, unsigned char* flags, int Ng, int 
 int source_mtiles(int nang);  // 64-direction output tiles of the angular GEMM
 int source_ktiles(int nang);  // 32-wide K tiles
 // FP32 source on the matrix cores: A1/A2 scratch, Q output.
+// `inputs_read` (optional) is recorded on s once the angular GEMM, the only
+// reader of Fp and fp, has been issued.
 void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, float* Q,
-                        const SourcePlan& plan, int Ng, int nyz, int nang, hipStream_t s);
+                        const SourcePlan& plan, int Ng, int nyz, int nang, hipStream_t s,
+                        hipEvent_t inputs_read = nullptr);
 
 // Secondary source (FP64 vector path): A1/A2 angular contraction, banded sum.
 template <typename T>

@@ -217,7 +217,7 @@ struct bfpn_server {
                                              static_cast<int>(opts.slice_steps * scale)));
         const int n = std::min(per, r.steps - r.issued);
         r.ctx->advance(n);
-        BFPN_HIP(hipEventRecord(r.slice_done, r.ctx->stream));
+        BFPN_HIP(hipEventRecord(r.slice_done, r.ctx->done_stream));
         r.issued += n;
         r.in_flight = true;
     }

@@ -9,8 +9,14 @@
 //   IDD (Eq.24) of F+F1, f+f1 and the proxy dg*sum(f), recorded at x=(k+1)dt,
 // with a host sync after every operator and two scalar D2H copies per step.
 //
-// Here the whole step is stream-ordered on the context's HIP stream with no
-// host sync: transport and angular diffusion update each state array in
+// Here a step is stream-ordered with no host sync, on two HIP streams when
+// the secondary is on: the primary march runs on the context's stream; the
+// source, the secondary march and the IDD reduction on a second one, which
+// waits for the primary of its step.  The primary of the next step waits only
+// until the source's angular GEMM (its last reader) has been issued and run,
+// so it overlaps the band GEMM and the secondary march of the previous step
+// (kernels of the two streams fill each other's tails and latency gaps).
+// Transport and angular diffusion update each state array in
 // place, the IDD and proxy are accumulated inside the final energy half steps
 // and reduced into a device history array, and one source evaluation per step
 // feeds both secondary energy half steps (the primary is unchanged in
@@ -194,7 +200,7 @@ std::string physics_key(const bfpn_grid& g, const bfpn_phys& p, const std::strin
 
 namespace {
 
-enum Phase { kEnergy = 0, kTransport = 1, kAngle = 2, kSource = 3, kDose = 4 };
+enum Phase { kEnergy = 0, kTransport = 1, kAngle = 2, kSource = 3, kDose = 4, kWait = 5 };
 
 template <typename T>
 struct Physics final : PhysicsBase {
@@ -280,6 +286,14 @@ public:
         cells = P_->cells;
         const BeamFactors beam = beam_factors(g, p, d, P_->om);
         BFPN_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        done_stream = stream;
+        if (P_->secondary) {
+            BFPN_HIP(hipStreamCreateWithFlags(&stream2_, hipStreamNonBlocking));
+            done_stream = stream2_;
+            BFPN_HIP(hipEventCreateWithFlags(&ev_prim_, hipEventDisableTiming));
+            BFPN_HIP(hipEventCreateWithFlags(&ev_src_, hipEventDisableTiming));
+            BFPN_HIP(hipEventCreateWithFlags(&ev_tail_, hipEventDisableTiming));
+        }
         const size_t state_bytes = cells * sizeof(T);
         for (auto& b : state_) b.ensure(state_bytes);
         if (P_->secondary) {
@@ -288,8 +302,8 @@ public:
             q_.ensure(state_bytes);
         }
         for (int k = 0; k < 4; ++k) cur_[k] = state_[k].as<T>();
-        const int eb = energy_blocks(P_->lanes);
-        partial_p_.ensure(2 * eb * sizeof(double));
+        const int eb = energy_blocks(P_->lanes, sizeof(T));
+        partial_p_.ensure(2 * 2 * eb * sizeof(double));  // two slots: steps alternate
         partial_s_.ensure(eb * sizeof(double));
         grow_history(std::max(16, g.Nx));
         // Eq.23 beam on the device (the reference builds a full host array)
@@ -302,6 +316,12 @@ public:
     }
 
     ~Context() override {
+        if (stream2_) {
+            (void)hipStreamSynchronize(stream2_);
+            (void)hipStreamDestroy(stream2_);
+        }
+        for (hipEvent_t e : {ev_prim_, ev_src_, ev_tail_})
+            if (e) (void)hipEventDestroy(e);
         if (stream) {
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
@@ -321,13 +341,21 @@ public:
         BFPN_REQUIRE(nsteps >= 0, "nsteps must be >= 0");
         if (nsteps == 0) return;
         if (step_ + nsteps > hist_cap_) grow_history(std::max(2 * hist_cap_, step_ + nsteps));
-        events_.clear();
+        for (auto& ev : events_) ev.clear();
+        if (stream2_) {  // work a caller ordered on done_stream precedes these steps
+            BFPN_HIP(hipEventRecord(ev_tail_, stream2_));
+            BFPN_HIP(hipStreamWaitEvent(stream, ev_tail_, 0));
+        }
         for (int k = 0; k < nsteps; ++k) step();
+        if (stream2_ && !overlap) {  // all on `stream`: done_stream still ends last
+            BFPN_HIP(hipEventRecord(ev_tail_, stream));
+            BFPN_HIP(hipStreamWaitEvent(stream2_, ev_tail_, 0));
+        }
         if (profiling) collect_profile();
     }
 
     void history(double* idd, double* proxy, int cap, int* n) override {
-        BFPN_HIP(hipStreamSynchronize(stream));
+        sync_all();
         const int cnt = std::min(cap, step_);
         if (cnt > 0) {
             if (idd)
@@ -342,7 +370,7 @@ public:
     void read_state(int which, double* host, size_t count) override {
         BFPN_REQUIRE(which >= 0 && which < 4, "state index must be 0..3");
         BFPN_REQUIRE(count == cells, "state count mismatch");
-        BFPN_HIP(hipStreamSynchronize(stream));
+        sync_all();
         std::vector<T> tmp(cells);
         BFPN_HIP(hipMemcpy(tmp.data(), cur_[which], cells * sizeof(T), hipMemcpyDeviceToHost));
         for (size_t k = 0; k < cells; ++k) host[k] = static_cast<double>(tmp[k]);
@@ -352,12 +380,18 @@ public:
         BFPN_REQUIRE(which >= 0 && which < 4, "state index must be 0..3");
         BFPN_REQUIRE(count == cells, "state count mismatch");
         std::vector<T> tmp(host, host + cells);
-        BFPN_HIP(hipStreamSynchronize(stream));
+        sync_all();
         BFPN_HIP(hipMemcpy(cur_[which], tmp.data(), cells * sizeof(T), hipMemcpyHostToDevice));
     }
 
 private:
+    void sync_all() {
+        BFPN_HIP(hipStreamSynchronize(stream));
+        if (stream2_) BFPN_HIP(hipStreamSynchronize(stream2_));
+    }
+
     void grow_history(int cap) {
+        sync_all();  // the IDD reduction writes the history on the second stream
         DeviceBuffer nb;
         nb.ensure(2 * static_cast<size_t>(cap) * sizeof(double));
         BFPN_HIP(hipMemsetAsync(nb.as<void>(), 0, nb.bytes(), stream));
@@ -372,117 +406,145 @@ private:
         hist_cap_ = cap;
     }
 
-    // Event after an operator call; the interval since the previous event is
-    // charged to phase p together with the call's algorithmic bytes.
-    void mark(Phase p, double bytes = 0.0) {
+    // Event on stream s after an operator call; the interval since the
+    // previous event on s is charged to phase p together with the call's
+    // algorithmic bytes (kWait: the interval ends a wait for the other stream
+    // and is not charged).  With both streams busy, a kernel's interval is its
+    // duration while sharing the GPU.
+    void mark(Phase p, double bytes, hipStream_t s) {
         if (!profiling) return;
         hipEvent_t e;
         BFPN_HIP(hipEventCreate(&e));
-        BFPN_HIP(hipEventRecord(e, stream));
-        events_.push_back({e, p});
+        BFPN_HIP(hipEventRecord(e, s));
+        events_[s == stream ? 0 : 1].push_back({e, p});
+        if (p == kWait) return;
         phase_bytes[p] += bytes;
         phase_calls[p] += bytes > 0.0 ? 1 : 0;
     }
 
     void collect_profile() {
-        BFPN_HIP(hipStreamSynchronize(stream));
-        for (size_t k = 1; k < events_.size(); ++k) {
-            float ms = 0.f;
-            BFPN_HIP(hipEventElapsedTime(&ms, events_[k - 1].first, events_[k].first));
-            phase_seconds[events_[k].second] += ms * 1e-3;
+        sync_all();
+        for (auto& ev : events_) {
+            for (size_t k = 1; k < ev.size(); ++k) {
+                if (ev[k].second == kWait) continue;
+                float ms = 0.f;
+                BFPN_HIP(hipEventElapsedTime(&ms, ev[k - 1].first, ev[k].first));
+                phase_seconds[ev[k].second] += ms * 1e-3;
+            }
+            for (auto& e : ev) (void)hipEventDestroy(e.first);
+            ev.clear();
         }
-        for (auto& e : events_) (void)hipEventDestroy(e.first);
-        events_.clear();
     }
 
     double array_bytes() const { return static_cast<double>(cells) * sizeof(T); }
 
-    void energy(int species, bool with_source, double* partials, bool proxy) {
+    void energy(int species, bool with_source, double* partials, bool proxy, hipStream_t s) {
         launch_energy_half<T>(cur_[2 * species], cur_[2 * species + 1], P_->coef.template as<EnergyCoef>(),
                               with_source ? q_.as<T>() : nullptr, grid.Ng, P_->lanes,
                               phys.eq15_straggling != 0, P_->yzw.template as<double>(), P_->nang,
-                              partials, proxy, stream);
-        mark(kEnergy, array_bytes() * (with_source ? 5 : 4));
+                              partials, proxy, s);
+        mark(kEnergy, array_bytes() * (with_source ? 5 : 4), s);
     }
 
-    void transport(int which, bool preserve_sign) {
+    void transport(int which, bool preserve_sign, hipStream_t s) {
         launch_transport_half<T>(cur_[which], P_->om_dev.template as<T>(), grid.Ny, grid.Nz, grid.Ng,
-                                 P_->nang, d.dy, d.dz, 0.5 * d.dt, preserve_sign, stream);
-        mark(kTransport, array_bytes() * 2);
+                                 P_->nang, d.dy, d.dz, 0.5 * d.dt, preserve_sign, s);
+        mark(kTransport, array_bytes() * 2, s);
     }
 
-    void angle(int which) {
+    void angle(int which, hipStream_t s) {
         if constexpr (std::is_same_v<T, float>) {
             if (P_->angle_plan.plan.su_h) {
                 launch_angle_mfma(cur_[which], P_->factors.template as<float>(), P_->angle_plan.plan,
-                                  grid.Ng, P_->nyz, grid.Nmu, grid.Nom, stream);
-                mark(kAngle, array_bytes() * 2);
+                                  grid.Ng, P_->nyz, grid.Nmu, grid.Nom, s);
+                mark(kAngle, array_bytes() * 2, s);
                 return;
             }
         }
         launch_angle_full<T>(cur_[which], P_->factors.template as<T>(), P_->su.template as<T>(),
-                             P_->sv.template as<T>(), grid.Ng, P_->nyz, grid.Nmu, grid.Nom, stream);
-        mark(kAngle, array_bytes() * 2);
+                             P_->sv.template as<T>(), grid.Ng, P_->nyz, grid.Nmu, grid.Nom, s);
+        mark(kAngle, array_bytes() * 2, s);
     }
 
-    void source() {
+    // Secondary source on stream s; `inputs_read` (optional) is recorded on s
+    // once the primary state has been read (after the angular GEMM).
+    void source(hipStream_t s, hipEvent_t inputs_read) {
         if constexpr (std::is_same_v<T, float>) {
             launch_source_mfma(cur_[0], cur_[1], a1_.as<float>(), a2_.as<float>(), q_.as<float>(),
-                               P_->plan.plan, grid.Ng, P_->nyz, P_->nang, stream);
+                               P_->plan.plan, grid.Ng, P_->nyz, P_->nang, s, inputs_read);
         } else {
             launch_source_angular<T>(cur_[0], cur_[1], P_->kv.template as<T>(), a1_.as<T>(),
-                                     a2_.as<T>(), grid.Ng, P_->nyz, P_->nang, stream);
+                                     a2_.as<T>(), grid.Ng, P_->nyz, P_->nang, s);
+            if (inputs_read) BFPN_HIP(hipEventRecord(inputs_read, s));
             launch_source_energy<T>(a1_.as<T>(), a2_.as<T>(), P_->e1.template as<double>(),
                                     P_->e2.template as<double>(), P_->beg.template as<int>(),
                                     P_->end.template as<int>(), q_.as<T>(), grid.Ng, P_->lanes, d.dg,
-                                    stream);
+                                    s);
         }
-        mark(kSource, array_bytes() * 7);  // 2 in, 2 out; 2 in, 1 out
+        mark(kSource, array_bytes() * 7, s);  // 2 in, 2 out; 2 in, 1 out
     }
 
     // One depth step (BFPnSolver::solve loop body, src/bfpm_solver.cu:2213-2242).
+    // Primary on `stream` (sp); source, secondary and the IDD reduction on
+    // stream2_ (ss) after the primary of the same step.  The next step's
+    // primary waits for this step's angular source GEMM (the last reader of
+    // the primary state it overwrites); the primary dose partials alternate
+    // between two slots, so step n+1 never writes the slot that step n's
+    // reduction reads (step n+2, which does, waits for the source of n+1,
+    // issued after that reduction on ss).
     void step() {
         const bool preserve_primary = phys.no_spatial_clipping != 0;
-        double* pp = partial_p_.as<double>();
+        const int eb = energy_blocks(P_->lanes, sizeof(T));
+        double* pp = partial_p_.as<double>() + (step_ & 1) * 2 * eb;
         double* ps = partial_s_.as<double>();
-        const int eb = energy_blocks(P_->lanes);
-        mark(kDose);  // opens the first interval of the step
+        hipStream_t sp = stream, ss = stream2_ && overlap ? stream2_ : stream;
+        if (src_pending_) {
+            BFPN_HIP(hipStreamWaitEvent(sp, ev_src_, 0));
+            src_pending_ = false;
+        }
+        mark(kWait, 0.0, sp);  // opens the first interval of the step
         if (phys.energy_only) {  // stepPrimaryEnergyOnly, src/bfpm_solver.cu:3313-3333
-            energy(0, false, nullptr, false);
-            energy(0, false, pp, true);
+            energy(0, false, nullptr, false, sp);
+            energy(0, false, pp, true, sp);
         } else {  // stepPrimary, src/bfpm_solver.cu:3238-3311
-            energy(0, false, nullptr, false);
+            energy(0, false, nullptr, false, sp);
             if (!phys.no_transport) {
-                transport(0, preserve_primary);
-                transport(1, true);
+                transport(0, preserve_primary, sp);
+                transport(1, true, sp);
             }
             if (!phys.no_angle) {
-                angle(0);
-                angle(1);
+                angle(0, sp);
+                angle(1, sp);
             }
             if (!phys.no_transport) {
-                transport(0, preserve_primary);
-                transport(1, true);
+                transport(0, preserve_primary, sp);
+                transport(1, true, sp);
             }
-            energy(0, false, pp, true);
+            energy(0, false, pp, true, sp);
+        }
+        if (ss != sp) {
+            BFPN_HIP(hipEventRecord(ev_prim_, sp));
+            BFPN_HIP(hipStreamWaitEvent(ss, ev_prim_, 0));
+            mark(kWait, 0.0, ss);
         }
         const bool sec = P_->secondary && step_ > 0;
         if (sec) {  // stepSecondary, src/bfpm_solver.cu:3335-3394
-            source();
-            energy(1, true, nullptr, false);
-            transport(2, false);  // secondary psi^1 is always clipped
-            transport(3, true);
-            angle(2);
-            angle(3);
-            transport(2, false);
-            transport(3, true);
-            energy(1, true, ps, false);
+            source(ss, ss != sp ? ev_src_ : nullptr);
+            src_pending_ = ss != sp;
+            energy(1, true, nullptr, false, ss);
+            transport(2, false, ss);  // secondary psi^1 is always clipped
+            transport(3, true, ss);
+            angle(2, ss);
+            angle(3, ss);
+            transport(2, false, ss);
+            transport(3, true, ss);
+            energy(1, true, ps, false, ss);
         }
         const double scale = d.du * d.dv * d.dy * d.dz / phys.density;
         double* hist = hist_.as<double>();
-        launch_finish_sum(pp, eb, sec ? ps : nullptr, eb, scale, hist + step_, stream);
-        launch_finish_sum(pp + eb, eb, nullptr, 0, d.dg, hist + hist_cap_ + step_, stream);
-        mark(kDose);
+        launch_finish_sum(pp, eb, sec ? ps : nullptr, eb, scale, hist + step_, ss);
+        launch_finish_sum(pp + eb, eb, nullptr, 0, d.dg, hist + hist_cap_ + step_, ss);
+        mark(kDose, 0.0, ss);
         ++step_;
     }
 
@@ -492,7 +554,10 @@ private:
     T* cur_[4] = {nullptr, nullptr, nullptr, nullptr};
     int step_ = 0;
     int hist_cap_ = 0;
-    std::vector<std::pair<hipEvent_t, Phase>> events_;
+    std::vector<std::pair<hipEvent_t, Phase>> events_[2];  // per stream (profiling)
+    hipStream_t stream2_ = nullptr;  // source + secondary; null without a secondary
+    hipEvent_t ev_prim_ = nullptr, ev_src_ = nullptr, ev_tail_ = nullptr;
+    bool src_pending_ = false;  // the next primary must wait for ev_src_
 };
 
 }  // namespace
@@ -906,7 +971,14 @@ BFPN_API bfpn_status bfpn_write_state(bfpn_ctx* ctx, int which, const double* ho
     });
 }
 
-BFPN_API void* bfpn_stream(bfpn_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+BFPN_API void* bfpn_stream(bfpn_ctx* ctx) { return ctx ? static_cast<void*>(ctx->done_stream) : nullptr; }
+
+BFPN_API bfpn_status bfpn_set_overlap(bfpn_ctx* ctx, int on) {
+    return guarded([&] {
+        BFPN_REQUIRE(ctx, "null context");
+        ctx->overlap = on != 0;
+    });
+}
 
 BFPN_API bfpn_status bfpn_set_profiling(bfpn_ctx* ctx, int on) {
     return guarded([&] {

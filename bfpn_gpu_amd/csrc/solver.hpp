@@ -59,7 +59,9 @@ struct bfpn_ctx {
     bfpn_phys phys{};
     bfpn_deltas d{};
     size_t cells = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;       // primary march
+    hipStream_t done_stream = nullptr;  // work issued by advance() ends on this stream
+    bool overlap = true;                // secondary on its own stream (if any)
     bool profiling = false;
     double phase_seconds[5] = {0, 0, 0, 0, 0};
     double phase_bytes[5] = {0, 0, 0, 0, 0};

@@ -464,7 +464,8 @@ void launch_split(const float* src, __bf16* hi, __bf16* lo, long long n, hipStre
 }
 
 void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, float* Q,
-                        const SourcePlan& plan, int Ng, int nyz, int nang, hipStream_t s) {
+                        const SourcePlan& plan, int Ng, int nyz, int nang, hipStream_t s,
+                        hipEvent_t inputs_read) {
     const int nap = source_nap(nang);
     const long long t1 = static_cast<long long>(Ng) * ((nyz + 63) / 64) * ((nang + 63) / 64);
     BFPN_REQUIRE(t1 < (1LL << 31) - 8, "source GEMM grid too large");
@@ -478,6 +479,7 @@ void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, 
                            plan.ktl, plan.kt_count, plan.kt_list, A1, A2, nyz, nang, nap,
                            static_cast<int>(t1));
     BFPN_HIP(hipGetLastError());
+    if (inputs_read) BFPN_HIP(hipEventRecord(inputs_read, s));
     const long long lanes = static_cast<long long>(nyz) * nang;
     const long long t2 =
         ((lanes + kBandLanes - 1) / kBandLanes) * ((Ng + kBandRows - 1) / kBandRows);

@@ -24,62 +24,94 @@
 //      a = R - C, b = C - L, ab > 0 and |a| >= 1e-10:
 //      slope = sign(a) min(max(|a|, |b|), 2 min(|a|, |b|)),   else 0,
 //    which equals forward * max(0, min(2 theta, 1), min(theta, 2)) with
-//    theta = b / a, up to the rounding of that quotient.
+//    theta = b / a, up to the rounding of that quotient; it is computed as
+//    two signed medians of three (one instruction each in FP32).
 //  * Rows and columns outside the plane are clamped, exactly like the
 //    reference's clamped stencil taps: the slope of a boundary cell is 0 and
 //    the cell outside reads the boundary value with slope 0.
 
 #include "kernels.hpp"
 
+#include <type_traits>
+
 namespace bfpn {
 namespace {
 
-constexpr int kMaxThreads = 1024;
-
+// Work-group size limit: 1024 threads in FP32; 768 in FP64, so that a
+// thread may hold up to 168 VGPRs (its windows are twice as wide).
 template <typename T>
-__device__ inline T limited_slope(T a, T b) {  // a: forward, b: backward difference
-    const T aa = fabs(a), ab = fabs(b);
-    const T mag = fmin(fmax(aa, ab), T(2) * fmin(aa, ab));
-    return (aa < T(1e-10) || !(a * b > T(0))) ? T(0) : copysign(mag, a);
+constexpr int max_threads() { return sizeof(T) == 4 ? 1024 : 768; }
+
+// Superbee slope of two differences (either order), without the 1e-10 test:
+//   ab > 0: sign(a) min(max(|a|, |b|), 2 min(|a|, |b|)),  else 0,
+// which is the median of a, b and 2 m, with m = median(a, b, 0) the minmod
+// of a and b: m = 0 when the signs differ (then the median is 0 too), and for
+// a common sign the median of a, b, 2m is the expression above.  One
+// median-of-three instruction each in FP32; FP64 uses
+// median(x, y, z) = max(min(x, y), min(max(x, y), z)).
+__device__ inline float superbee(float a, float b) {
+    const float m = __builtin_amdgcn_fmed3f(a, b, 0.0f);
+    return __builtin_amdgcn_fmed3f(a, b, 2.0f * m);
+}
+__device__ inline double superbee(double a, double b) {
+    const double lo = fmin(a, b), hi = fmax(a, b);
+    const double m = fmax(lo, fmin(hi, 0.0));
+    return fmax(lo, fmin(hi, 2.0 * m));
 }
 
-// Limited slope from two differences of one orientation with the 1e-10
-// test on `e` (the difference that is forward in the physical axis).  With
-// a*b > 0 the sign of a is the sign of b, so the value is symmetric in (a, b).
+// a: forward, b: backward difference (physical orientation); 0 when the
+// forward difference is below 1e-10.
 template <typename T>
-__device__ inline T upwind_slope(T a, T b, T e) {
-    const T aa = fabs(a), ab = fabs(b);
-    const T mag = fmin(fmax(aa, ab), T(2) * fmin(aa, ab));
-    return (fabs(e) < T(1e-10) || !(a * b > T(0))) ? T(0) : copysign(mag, a);
+__device__ inline T limited_slope(T a, T b) {
+    return fabs(a) < T(1e-10) ? T(0) : superbee(a, b);
 }
 
 // Flux divergence at one cell, per axis written in the lane's upwind frame:
-//   w (F(+1/2) - F(-1/2)) = |w| (downwind face - upwind face),
-// where the downwind face is the cell's own reconstruction and the upwind
-// face the upwind neighbour's.  y: near = upwind neighbour, far = the one
+//   w (F(+1/2) - F(-1/2)) / d = (|w| / d) (downwind face - upwind face),
+// where the downwind face is the cell's own reconstruction c + s_down/2 and
+// the upwind face the upwind neighbour's, n + s_up/2, so the difference is
+// (c - n) + (s_down - s_up)/2.  y: near = upwind neighbour, far = the one
 // beyond it, down = downwind neighbour (LDS row); with A = down - c,
 // B = c - near, C = near - far the two slopes are (A, B) and (B, C), the
 // 1e-10 test going to whichever is forward in +y.  z: physical window
 // (zm, zc, zp) and carried physical slopes of rows -1, 0, +1; hz = +-1/2
-// with the sign of wz.
+// with the sign of wz.  cy = |wy| / dy, cz = |wz| / dz.
 template <typename T>
 __device__ inline T divergence(const T* row, int o_near, int o_far, int o_down, T zm, T zc, T zp,
-                               T sz_m, T sz_0, T sz_p, T awy, T awz, T hz, bool py, bool pz,
-                               T inv_dy, T inv_dz) {
+                               T sz_m, T sz_0, T sz_p, T cy, T cz, T hz, bool py, bool pz) {
     const T near = row[o_near], far = row[o_far], down = row[o_down];
     const T A = down - zc, B = zc - near, C = near - far;
-    const T s_down = upwind_slope(A, B, py ? A : B);
-    const T s_up = upwind_slope(B, C, py ? B : C);
-    const T fy = awy * fma(T(0.5), s_down, zc) - awy * fma(T(0.5), s_up, near);
-    const T fz = awz * fma(hz, sz_0, zc) - awz * fma(hz, pz ? sz_m : sz_p, pz ? zm : zp);
-    return fy * inv_dy + fz * inv_dz;
+    const T s_down = fabs(py ? A : B) < T(1e-10) ? T(0) : superbee(A, B);
+    const T s_up = fabs(py ? B : C) < T(1e-10) ? T(0) : superbee(B, C);
+    const T fy = fma(T(0.5), s_down - s_up, B);
+    const T fz = fma(hz, sz_0 - (pz ? sz_m : sz_p), zc - (pz ? zm : zp));
+    return cy * fy + cz * fz;
+}
+
+// Element at a uniform base plus a per-lane 32-bit byte offset (the form the
+// global memory instructions take directly: scalar base, vector offset).
+template <typename T>
+__device__ inline T& at(T* base, unsigned byte_off) {
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off);
+}
+template <typename T>
+__device__ inline const T& at(const T* base, unsigned byte_off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
 }
 
 // grid: (ceil(nang / MW), Ng), block: MW * R threads (R y columns per pass,
-// P passes: the thread's columns are y = ry + j R).  LDS: 8 rows of
-// (Ny+1) * MW values (two rings of 4 rows).
+// P passes: the thread's columns are y = ry + j R).  LDS: two rings of 4
+// rows (psi, psi*), stored y-major: column y holds its 8 ring slots of MW
+// values side by side, padded to 9 MW so that the y columns of one wave fall
+// on distinct banks; a slot is then a constant offset from the column.
+//
+// The z march is unrolled by the ring period 4: the register windows are
+// indexed by row mod 4 (slopes by row mod 2), so with the step's phase k a
+// compile-time constant, sliding a window is a renaming rather than a chain
+// of moves, and the LDS ring slots are immediate offsets.  The corrector
+// needs psi and L(psi) of its row only as psi - h/2 L(psi), kept per row.
 template <typename T, int MW, int P, bool CLIP>
-__global__ __launch_bounds__(kMaxThreads) void transport_march_kernel(
+__global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
     T* __restrict__ F, const T* __restrict__ omend, int Ny, int Nz, int nang, int R, T inv_dy,
     T inv_dz, T h, int tiles) {
     extern __shared__ unsigned char lds_raw[];
@@ -92,149 +124,155 @@ __global__ __launch_bounds__(kMaxThreads) void transport_march_kernel(
     const int mchunks = (nang + MW - 1) / MW;
     const int g = L / mchunks;
     const int Ny1 = Ny + 1;
-    const int rs = Ny1 * MW;  // LDS row stride (elements)
-    T* psi_ring = reinterpret_cast<T*>(lds_raw);  // 4 rows of psi
-    T* pred_ring = psi_ring + 4 * rs;             // 4 rows of psi*
+    T* ring = reinterpret_cast<T*>(lds_raw);  // [y][slot 0-3 psi, 4-7 psi*][MW], pitch 9 MW
+    constexpr int YP = 9 * MW;
 
     const int mm = threadIdx.x % MW, ry = threadIdx.x / MW;
     const int m0 = (L - g * mchunks) * MW;
     const bool live = m0 + mm < nang;
+    // Lanes past the last direction (and columns past Ny) compute a copy of
+    // the last one from in-bounds addresses and store nothing, so loads and
+    // arithmetic need no predicate.
     const int m = live ? m0 + mm : nang - 1;
     const T wy = omend[2 * m], wz = omend[2 * m + 1];
     const bool py = wy >= T(0), pz = wz >= T(0);
-    const T awy = fabs(wy), awz = fabs(wz), hz = pz ? T(0.5) : T(-0.5);
+    const T cy = fabs(wy) * inv_dy, cz = fabs(wz) * inv_dz, hz = pz ? T(0.5) : T(-0.5);
+    const T hh = T(0.5) * h;
     const long long zstride = static_cast<long long>(Ny1) * nang;
-    T* Fg = F + static_cast<long long>(g) * zstride * (Nz + 1) + m0 + mm;
+    // work-group base (uniform); lanes address it with 32-bit offsets
+    T* Fg = F + static_cast<long long>(g) * zstride * (Nz + 1) + m0;
 
     bool on[P];
-    int goff[P], o0[P], o_near[P], o_far[P], o_down[P];
-    T wpsi[P][5];   // psi rows t-2 .. t+2 (after the commit of step t)
-    T wpred[P][4];  // psi* rows t-3 .. t (after the predictor of step t)
-    T wL[P][3];     // L(psi) rows t-2 .. t
-    T sz[P][2];     // z slopes of psi, rows t-1 and t (before step t)
-    T szp[P][2];    // z slopes of psi*, rows t-3 and t-2 (before step t)
-    T nxt[P];
+    unsigned boff[P];  // byte offset of the lane's column in a plane row
+    int o0[P], o_near[P], o_far[P], o_down[P];
+    T wpsi[P][4];   // psi, rows t-1 .. t+2 after the commit of step t
+    T wpred[P][4];  // psi*, rows t-3 .. t after the predictor of step t
+    T wbase[P][4];  // psi - h/2 L(psi), rows t-2 .. t
+    T sz[P][2];     // z slopes of psi, rows t-1 and t before step t
+    T szp[P][2];    // z slopes of psi*, rows t-3 and t-2 before step t
+    T nxt[P][4];    // psi rows t+2 .. t+5 in flight (slot: row mod 4)
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         const int y = ry + j * R;
         on[j] = y < Ny1;
         const int yc = min(y, Ny);
-        goff[j] = yc * nang;
-        auto lo = [&](int yy) { return (min(max(yy, 0), Ny)) * MW + mm; };
+        boff[j] = static_cast<unsigned>((yc * nang + (m - m0)) * sizeof(T));
+        auto lo = [&](int yy) { return (min(max(yy, 0), Ny)) * YP + mm; };
         o0[j] = lo(yc);
         o_near[j] = lo(py ? yc - 1 : yc + 1);
         o_far[j] = lo(py ? yc - 2 : yc + 2);
         o_down[j] = lo(py ? yc + 1 : yc - 1);
-        const bool ld = on[j] && live;
-        const T r0 = ld ? Fg[goff[j]] : T(0);
-        const T r1 = ld ? Fg[zstride + goff[j]] : T(0);  // Nz >= 1
+        const T r0 = at(Fg, boff[j]);
+        const T r1 = at(Fg + zstride, boff[j]);  // Nz >= 1
         if (on[j]) {
-            psi_ring[o0[j]] = r0;
-            psi_ring[rs + o0[j]] = r1;
+            ring[o0[j]] = r0;
+            ring[o0[j] + MW] = r1;
         }
-        wpsi[j][0] = wpsi[j][1] = wpsi[j][2] = wpsi[j][3] = r0;
-        wpsi[j][4] = r1;
-        nxt[j] = (ld && Nz >= 2) ? Fg[2 * zstride + goff[j]] : T(0);
-        sz[j][0] = sz[j][1] = T(0);  // boundary slopes are 0 (clamped taps)
-        szp[j][0] = szp[j][1] = T(0);
+        wpsi[j][3] = wpsi[j][0] = r0;  // row -1 clamps to row 0
+        wpsi[j][1] = r1;
+        wpsi[j][2] = T(0);
 #pragma unroll
-        for (int k = 0; k < 4; ++k) wpred[j][k] = T(0);
+        for (int r = 0; r < 4; ++r)  // rows past Nz: never used
+            nxt[j][(r + 2) & 3] = at(Fg + min(r + 2, Nz) * zstride, boff[j]);
 #pragma unroll
-        for (int k = 0; k < 3; ++k) wL[j][k] = T(0);
+        for (int k = 0; k < 2; ++k) sz[j][k] = szp[j][k] = T(0);  // boundary slopes are 0
+#pragma unroll
+        for (int k = 0; k < 4; ++k) wpred[j][k] = wbase[j][k] = T(0);
     }
 
-    for (int t = 0; t <= Nz + 2; ++t) {
-        // commit psi row t+2 (clamped past Nz)
-        const int rc = t + 2;
-        T* crow = psi_ring + (rc & 3) * rs;
+    // Step t (phase k = t mod 4): commit psi row t+2, predictor on row t,
+    // corrector on row t-2.  Every step runs the same branch-free code; near
+    // the ends of the march (EDGE) selects apply the clamped rows and the
+    // boundary slopes, and only the stores are predicated.  Past the last
+    // row the predictor works on clamped copies and its result is replaced
+    // by the clamp; steps past Nz+2 (padding the last block of 4) store
+    // nothing.
+    auto step = [&](int t, auto phase, auto edge) {
+        constexpr int k = decltype(phase)::value;
+        constexpr bool E = decltype(edge)::value;
+        const int rc = t + 2;  // commit psi row t+2 into the slot of row t-2
+        T* crow = ring + ((k + 2) & 3) * MW;
 #pragma unroll
         for (int j = 0; j < P; ++j) {
-            const T v = rc <= Nz ? nxt[j] : wpsi[j][4];
-#pragma unroll
-            for (int k = 0; k < 4; ++k) wpsi[j][k] = wpsi[j][k + 1];
-            wpsi[j][4] = v;
-            if (rc <= Nz && on[j]) crow[o0[j]] = v;
+            const T v = (!E || rc <= Nz) ? nxt[j][(k + 2) & 3] : wpsi[j][(k + 1) & 3];
+            wpsi[j][(k + 2) & 3] = v;
+            if (on[j]) crow[o0[j]] = v;
         }
         __syncthreads();
-        if (t + 3 <= Nz) {
+        {
+            // refill the slot just committed with row t+6: four rows in flight
+            const T* src = Fg + static_cast<long long>(E ? min(t + 6, Nz) : t + 6) * zstride;
 #pragma unroll
-            for (int j = 0; j < P; ++j)
-                nxt[j] = (on[j] && live) ? Fg[(t + 3) * zstride + goff[j]] : T(0);
+            for (int j = 0; j < P; ++j) nxt[j][(k + 2) & 3] = at(src, boff[j]);
         }
-        if (t <= Nz) {  // predictor on row t
-            const T* row = psi_ring + (t & 3) * rs;
-            T* prow = pred_ring + (t & 3) * rs;
+        {  // predictor on row t
+            const T* row = ring + k * MW;
+            T* prow = ring + (4 + k) * MW;
 #pragma unroll
             for (int j = 0; j < P; ++j) {
-                const T zm = wpsi[j][1], zc = wpsi[j][2], zp = wpsi[j][3];
-                const T s_next = limited_slope(wpsi[j][4] - zp, zp - zc);
-                T v = T(0), L0 = T(0);
-                if (on[j]) {
-                    L0 = divergence(row, o_near[j], o_far[j], o_down[j], zm, zc, zp, sz[j][0],
-                                    sz[j][1], s_next, awy, awz, hz, py, pz, inv_dy, inv_dz);
-                    v = zc - h * L0;
-                    if (CLIP) v = fmax(v, T(0));
-                    prow[o0[j]] = v;
-                }
-                sz[j][0] = sz[j][1];
-                sz[j][1] = s_next;
-                if (t == 0) {
-#pragma unroll
-                    for (int k = 0; k < 4; ++k) wpred[j][k] = v;  // rows < 0 clamp to row 0
-                } else {
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) wpred[j][k] = wpred[j][k + 1];
-                    wpred[j][3] = v;
-                }
-                wL[j][0] = wL[j][1];
-                wL[j][1] = wL[j][2];
-                wL[j][2] = L0;
-            }
-        } else {  // rows past Nz clamp to row Nz
-#pragma unroll
-            for (int j = 0; j < P; ++j) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) wpred[j][k] = wpred[j][k + 1];
-                wL[j][0] = wL[j][1];
-                wL[j][1] = wL[j][2];
+                const T zm = wpsi[j][(k + 3) & 3], zc = wpsi[j][k], zp = wpsi[j][(k + 1) & 3];
+                const T s_next = limited_slope(wpsi[j][(k + 2) & 3] - zp, zp - zc);
+                const T L0 = divergence(row, o_near[j], o_far[j], o_down[j], zm, zc, zp,
+                                        sz[j][(k + 1) & 1], sz[j][k & 1], s_next, cy, cz, hz, py,
+                                        pz);
+                T v = zc - h * L0;
+                if (CLIP) v = fmax(v, T(0));
+                if (on[j]) prow[o0[j]] = v;
+                sz[j][(k + 1) & 1] = s_next;
+                wbase[j][k] = zc - hh * L0;
+                wpred[j][k] = (!E || t <= Nz) ? v : wpred[j][(k + 3) & 3];  // past Nz: row Nz
+                if (E && k == 0) wpred[j][3] = t == 0 ? v : wpred[j][3];  // row -1: row 0
             }
         }
-        if (t >= 2) {  // corrector on row t-2, written in place
-            const int ro = t - 2;
-            const T* row = pred_ring + (ro & 3) * rs;
-            T* dst = Fg + ro * zstride;
+        {  // corrector on row t-2, written in place
+            const T* row = ring + (4 + ((k + 2) & 3)) * MW;
+            T* dst = Fg + static_cast<long long>(t - 2) * zstride;
+            const bool real = !E || (t >= 2 && t <= Nz + 2);
 #pragma unroll
             for (int j = 0; j < P; ++j) {
-                const T zm = wpred[j][0], zc = wpred[j][1], zp = wpred[j][2];
-                const T s_next = limited_slope(wpred[j][3] - zp, zp - zc);
-                if (on[j]) {
-                    const T L1 = divergence(row, o_near[j], o_far[j], o_down[j], zm, zc, zp,
-                                            szp[j][0], szp[j][1], s_next, awy, awz, hz, py,
-                                            pz, inv_dy, inv_dz);
-                    T v = wpsi[j][0] - T(0.5) * h * (wL[j][0] + L1);
-                    if (CLIP) v = fmax(v, T(0));
-                    if (live) dst[goff[j]] = v;
-                }
-                szp[j][0] = szp[j][1];
-                szp[j][1] = s_next;
+                const T zm = wpred[j][(k + 1) & 3], zc = wpred[j][(k + 2) & 3],
+                        zp = wpred[j][(k + 3) & 3];
+                const T s_next = limited_slope(wpred[j][k] - zp, zp - zc);
+                const T L1 = divergence(row, o_near[j], o_far[j], o_down[j], zm, zc, zp,
+                                        szp[j][(k + 1) & 1], szp[j][k & 1], s_next, cy, cz, hz,
+                                        py, pz);
+                T v = wbase[j][(k + 2) & 3] - hh * L1;
+                if (CLIP) v = fmax(v, T(0));
+                if (real && on[j] && live) at(dst, boff[j]) = v;
+                szp[j][(k + 1) & 1] = (!E || t >= 2) ? s_next : T(0);  // rows <= 0: boundary
             }
         }
-    }
+    };
+    auto block = [&](int t0, auto edge) {
+        step(t0, std::integral_constant<int, 0>{}, edge);
+        step(t0 + 1, std::integral_constant<int, 1>{}, edge);
+        step(t0 + 2, std::integral_constant<int, 2>{}, edge);
+        step(t0 + 3, std::integral_constant<int, 3>{}, edge);
+    };
+    // interior blocks: 2 <= t and t + 6 <= Nz for all four steps
+    block(0, std::true_type{});
+    int t0 = 4;
+    for (; t0 + 9 <= Nz; t0 += 4) block(t0, std::false_type{});
+    for (; t0 <= Nz + 2; t0 += 4) block(t0, std::true_type{});
 }
 
 struct MarchShape {
     int MW, P, R, threads;
 };
 
-// Widest direction chunk (coalesced rows) whose plane fits one work-group
-// with at most 2 (MW = 32) or 4 (MW = 16, 8) columns per thread and whose two
-// LDS rings fit the 160 KB of a CU.
+// Launch shape, the first of (MW, P) = (16, 1), (16, 2), (8, 2), (8, 3),
+// (8, 4) whose work-group covers the plane and whose two LDS rings fit the
+// 160 KB of a CU.  Measured on MI355X (FP32, 400 directions): 16 directions
+// per work-group beat 32 (smaller groups pack the CU's wave slots better:
+// 235 vs 248 us at 21 x 21 nodes) and 8 (64-byte row segments: 3.41 vs
+// 3.57 ms at 81 x 81 nodes with two columns per thread).
 MarchShape march_shape(int Ny1, size_t elem) {
-    for (int MW : {32, 16, 8}) {
-        const int rows = kMaxThreads / MW;
-        const int P = (Ny1 + rows - 1) / rows;
-        if (P > (MW == 32 ? 2 : 4)) continue;
-        if (8 * static_cast<size_t>(Ny1) * MW * elem > 160 * 1024) continue;
+    const int max_t = elem == 4 ? max_threads<float>() : max_threads<double>();
+    constexpr int cand[5][2] = {{16, 1}, {16, 2}, {8, 2}, {8, 3}, {8, 4}};
+    for (const auto& c : cand) {
+        const int MW = c[0], P = c[1];
+        if (static_cast<long long>(max_t / MW) * P < Ny1) continue;
+        if (9 * static_cast<size_t>(Ny1) * MW * elem > 160 * 1024) continue;
         const int R0 = (Ny1 + P - 1) / P;
         const int threads = (MW * R0 + 63) / 64 * 64;
         return MarchShape{MW, P, threads / MW, threads};
@@ -245,7 +283,7 @@ MarchShape march_shape(int Ny1, size_t elem) {
 template <typename T, int MW, int P>
 void launch_march(T* F, const T* omend, int Ny, int Nz, int Ng, int nang, const MarchShape& sh,
                   double dy, double dz, double h, bool preserve_sign, hipStream_t s) {
-    const size_t lds = sizeof(T) * 8 * static_cast<size_t>(Ny + 1) * MW;
+    const size_t lds = sizeof(T) * 9 * static_cast<size_t>(Ny + 1) * MW;
     BFPN_REQUIRE(lds <= 160 * 1024, "transport plane too wide for the LDS rings");
     const long long tiles = static_cast<long long>((nang + MW - 1) / MW) * Ng;
     BFPN_REQUIRE(tiles < (1LL << 31) - 8, "transport grid too large");
@@ -274,16 +312,13 @@ void launch_transport_half(T* F, const T* omend, int Ny, int Nz, int Ng, int nan
     BFPN_REQUIRE(Ny >= 1 && Nz >= 1, "transport needs Ny, Nz >= 1");
     const MarchShape sh = march_shape(Ny + 1, sizeof(T));
     BFPN_REQUIRE(sh.MW > 0, sizeof(T) == 4 ? "transport supports at most 512 y nodes in FP32"
-                                           : "transport supports at most 320 y nodes in FP64");
+                                           : "transport supports at most 284 y nodes in FP64");
 #define BFPN_MARCH(MW_, P_)                                                                    \
     if (sh.MW == MW_ && sh.P == P_)                                                            \
         return launch_march<T, MW_, P_>(F, omend, Ny, Nz, Ng, nang, sh, dy, dz, h, preserve_sign, \
                                         s)
-    BFPN_MARCH(32, 1);
-    BFPN_MARCH(32, 2);
+    BFPN_MARCH(16, 1);
     BFPN_MARCH(16, 2);
-    BFPN_MARCH(16, 3);
-    BFPN_MARCH(16, 4);
     BFPN_MARCH(8, 2);
     BFPN_MARCH(8, 3);
     BFPN_MARCH(8, 4);

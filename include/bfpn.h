@@ -250,8 +250,18 @@ BFPN_API bfpn_status bfpn_read_state(bfpn_ctx* ctx, int which, double* host,
 BFPN_API bfpn_status bfpn_write_state(bfpn_ctx* ctx, int which,
                                       const double* host, size_t count);
 
-/* The context's HIP stream (hipStream_t) so callers can order their own work. */
+/* The HIP stream (hipStream_t) on which the context's issued work ends, so
+ * callers can order their own work: work enqueued there runs after every
+ * step issued so far, and steps issued later start after it. */
 BFPN_API void* bfpn_stream(bfpn_ctx* ctx);
+
+/* Overlap (on by default): with the secondary enabled, the source, the
+ * secondary march and the IDD reduction of step n run on a second stream,
+ * concurrently with the primary march of step n+1.  Off: every operator in
+ * issue order on one stream (the reference's sequence; per-kernel timings
+ * without contention).  Results are identical either way.  No counterpart in
+ * the reference, whose operators synchronise the host after every call. */
+BFPN_API bfpn_status bfpn_set_overlap(bfpn_ctx* ctx, int on);
 
 /* Per-phase device timing of the bfpn_advance calls made while profiling is
  * on (replaces StepProfile, src/bfpm_solver.cuh:64-75).  Each operator launch

@@ -5,10 +5,11 @@ so tails, zeros and both signs of psi^2 occur) plus seeded random fields.
 Tolerances, relative to the largest magnitude of the reference output:
   FP64 state: 1e-12 (same algebra; only FMA contraction and summation order differ)
   FP32 state: energy 2e-6, transport 5e-5, IDD 1e-6; source and angle 1e-4
-              (both run on the matrix cores in split-BF16 with FP32
-              accumulation, ~2^-16 relative per product and stage; a
-              single-precision limiter can take the other branch of the
-              superbee kink in isolated cells).
+              (both run on the matrix cores with FP32 accumulation: the
+              source in split-BF16, hi + lo, ~2^-16 relative per product;
+              the angle in a three-term BF16 split of both operands, close
+              to FP32 products; a single-precision limiter can take the
+              other branch of the superbee kink in isolated cells).
 """
 import numpy as np
 import pytest
@@ -69,6 +70,31 @@ def test_energy_half(gpu, oracle_mod, physical, dtype, straggling, with_source):
     _close(df.numpy().astype(np.float64), f, tol)
 
 
+@pytest.mark.parametrize("straggling,with_source", [(False, False), (True, True)])
+def test_energy_half_wide_lanes(gpu, oracle_mod, physical, straggling, with_source):
+    """2^18 lanes of FP32 state: the sweep that runs four adjacent lanes per
+    thread (16-byte row accesses) on wide transverse grids."""
+    c, d, st, t = physical
+    Ng, nyz, nang = 48, 1024, 256
+    rng = np.random.default_rng(11)
+    shape = Ng * nyz * nang
+    F = rng.standard_normal(shape).astype(np.float32).astype(np.float64)
+    f = (0.3 * rng.standard_normal(shape)).astype(np.float32).astype(np.float64)
+    Q = (0.1 * rng.random(shape)).astype(np.float32).astype(np.float64) if with_source else None
+    # the first Ng groups' tables (stopping power has Ng + 1 edge values)
+    tabs = {k: np.ascontiguousarray(t[k][:Ng + (k == "S")]) for k in ("S", "sigma", "T")}
+    dF, df = _dev(gpu, F, np.float32), _dev(gpu, f, np.float32)
+    S, sig, T = (_dev(gpu, tabs[k], np.float64) for k in ("S", "sigma", "T"))
+    dQ = _dev(gpu, Q, np.float32) if with_source else None
+    h = 0.5 * d.dt
+    gpu.ops.energy_half(dF, df, S, sig, T, h, d.dg, 1.0, Ng, nyz, nang, Q1=dQ,
+                        straggling=straggling)
+    oracle_mod.energy_half(F, f, tabs["S"], tabs["sigma"], tabs["T"], h, d.dg, 1.0, Ng, nyz, nang,
+                           Q1=Q, straggling=straggling)
+    _close(dF.numpy().astype(np.float64), F, TOL32["energy"])
+    _close(df.numpy().astype(np.float64), f, TOL32["energy"])
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("which,preserve", [(0, False), (1, True), (0, True)])
 def test_transport_half(gpu, oracle_mod, physical, dtype, which, preserve):
@@ -96,11 +122,20 @@ def test_transport_random_field_large_velocity(gpu, oracle_mod, dtype):
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("Ny,Nz,nang", [(1, 1, 3), (2, 5, 40), (20, 3, 33), (31, 4, 7),
                                         (40, 2, 17), (63, 2, 5), (64, 3, 9), (80, 2, 16),
-                                        (127, 1, 3), (128, 2, 2), (300, 1, 2)])
+                                        (127, 1, 3), (128, 2, 2), (200, 1, 2), (283, 6, 3),
+                                        (400, 1, 2)])
 def test_transport_plane_shapes(gpu, oracle_mod, Ny, Nz, nang, dtype):
     """Every launch shape (32/16/8 directions per work-group, 1-4 y columns per
-    thread), plateaus with exact ties (flat-slope branches), both clip modes."""
+    thread; z marches of 2..7 rows, i.e. edge-only and mixed unrolled blocks),
+    plateaus with exact ties (flat-slope branches), both clip modes.  Planes
+    wider than the LDS rings allow (FP64 above 284 y nodes) are an error."""
     Ng = 2
+    if dtype == np.float64 and Ny + 1 > 284:
+        dF = _dev(gpu, np.zeros(Ng * (Ny + 1) * (Nz + 1) * nang), dtype)
+        om = _dev(gpu, RNG.uniform(-0.9, 0.9, 2 * nang), np.float64)
+        with pytest.raises(gpu.BfpnError, match="at most 284 y nodes"):
+            gpu.ops.transport_half(dF, om, Ny, Nz, Ng, nang, 0.2, 0.3, 0.05, True)
+        return
     F = RNG.standard_normal(Ng * (Ny + 1) * (Nz + 1) * nang)
     F[::3] = np.round(F[::3] * 2) / 2
     F = F.astype(dtype).astype(np.float64)

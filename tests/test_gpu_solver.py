@@ -91,6 +91,37 @@ def test_fp32_solver_tracks_oracle(gpu, oracle_mod):
     assert _rel(g.state(2), o.state(2)) < 5e-4
 
 
+def test_fp32_wide_grid_tracks_oracle(gpu, oracle_mod):
+    """128x128 nodes x 4x4 directions (2^18 lanes, the four-lanes-per-thread
+    energy sweep with the fused dose sum), 40 groups: IDD and states after 3
+    steps within the FP32 bounds of the small-grid test."""
+    case = small_case(Ny=128, Nz=128, Nmu=4, Nom=4, Ng=40, Ly=8, Lz=8)
+    o, g = _run_both(gpu, oracle_mod, case, 3, gpu.FP32)
+    np.testing.assert_allclose(g.history()[0], o.history()[0], rtol=2e-5)
+    assert _rel(g.state(0), o.state(0)) < 1e-4
+    assert _rel(g.state(2), o.state(2)) < 5e-4
+
+
+@pytest.mark.parametrize("precision", ["FP32", "FP64"])
+def test_overlap_gives_identical_results(gpu, precision):
+    """The secondary march on its own stream (default) and every operator on
+    one stream produce the same bits: only the schedule differs."""
+    case = small_case(flags={"eq15_straggling": 1})
+    out = []
+    for overlap in (True, False):
+        with gpu.Solver(case, getattr(gpu, precision)) as s:
+            s.set_overlap(overlap)
+            s.advance(3)
+            s.set_overlap(not overlap)  # switching between calls is allowed
+            s.advance(3)
+            out.append((s.history(), [s.state(k) for k in range(4)]))
+    (h1, st1), (h2, st2) = out
+    np.testing.assert_array_equal(h1[0], h2[0])
+    np.testing.assert_array_equal(h1[1], h2[1])
+    for a, b in zip(st1, st2):
+        np.testing.assert_array_equal(a, b)
+
+
 def test_history_and_restart_are_consistent(gpu):
     """advance(a)+advance(b) equals advance(a+b); write_state round-trips."""
     case = small_case(Ny=3, Nz=3)
