@@ -83,6 +83,47 @@ inline int lanes_per_thread(long long lanes, size_t elem) {
     return (elem == 4 && lanes % 4 == 0 && lanes >= (1LL << 18)) ? 4 : 1;
 }
 
+// The two linear forms of a group (f_new = af.x, F_new = aF.x over
+// x = (P_hi, R_hi, P, R, P_lo, Q, Delta_hi)), wave-uniform: scalar loads.
+struct Forms {
+    double af[7], aF[7];
+};
+
+__device__ inline Forms load_forms(const EnergyCoef* coef, int g) {
+    Forms r;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) {
+        r.af[i] = coef[g].af[i];
+        r.aF[i] = coef[g].aF[i];
+    }
+    return r;
+}
+
+// One group of one lane's recurrence: input (P, R) at g, carried input
+// (P_hi, R_hi) and output difference hd at g+1; returns the outputs (a: psi^2,
+// b: psi^1) and moves the carries to g.  Inputs are state values, so the
+// carries are held in the state type C (exactly).
+template <bool SRC, bool STRAG, typename C>
+__device__ inline void recur(const Forms& c, C P, C R, double P_lo, double src, C& P_hi, C& R_hi,
+                             double& hd, double& a, double& b) {
+    const double Ph = P_hi, Rh = R_hi, Pd = P, Rd = R;
+    a = c.af[0] * Ph + c.af[1] * Rh + c.af[2] * Pd + c.af[3] * Rd;
+    b = c.aF[0] * Ph + c.aF[1] * Rh + c.aF[2] * Pd + c.aF[3] * Rd;
+    if (STRAG) {
+        a += c.af[4] * P_lo;
+        b += c.aF[4] * P_lo;
+    }
+    if (SRC) {
+        a += c.af[5] * src;
+        b += c.aF[5] * src;
+    }
+    a += c.af[6] * hd;
+    b += c.aF[6] * hd;
+    P_hi = P;
+    R_hi = R;
+    hd = b - a;
+}
+
 // Everything a group needs besides the state (recurrence coefficients and the
 // dose weights) sits in one EnergyCoef read through the scalar cache, so the
 // vector loads in flight are the state ring alone: the in-order load counter
@@ -110,30 +151,22 @@ __global__ __launch_bounds__(kBlock) void energy_half_kernel(
         };
 #pragma unroll
         for (int k = 0; k < D; ++k) fetch(k, Ng - 1 - k);
-        double P_hi[LPT], R_hi[LPT], hi_delta[LPT], dose[LPT];
+        T P_hi[LPT], R_hi[LPT];
+        double hi_delta[LPT], dose[LPT];
 #pragma unroll
-        for (int j = 0; j < LPT; ++j) P_hi[j] = R_hi[j] = hi_delta[j] = dose[j] = 0.0;
+        for (int j = 0; j < LPT; ++j) {
+            P_hi[j] = R_hi[j] = T(0);
+            hi_delta[j] = dose[j] = 0.0;
+        }
         // The linear forms run one group ahead through the scalar cache:
         // group g-1's are requested before group g's arithmetic (the
         // scheduling barrier keeps the request there), so their latency is
         // covered.  (Dose weights are read at use: two sets of forms plus
         // them would not fit the scalar registers.)
-        struct Forms {
-            double af[7], aF[7];
-        };
-        auto forms = [&](int g) {
-            Forms r;
-#pragma unroll
-            for (int i = 0; i < 7; ++i) {
-                r.af[i] = coef[g].af[i];
-                r.aF[i] = coef[g].aF[i];
-            }
-            return r;
-        };
-        Forms cn = forms(Ng - 1);
+        Forms cn = load_forms(coef, Ng - 1);
         auto step = [&](int k, int g) {
             const Forms c = cn;
-            cn = forms(max(g - 1, 0));
+            cn = load_forms(coef, max(g - 1, 0));
             __builtin_amdgcn_sched_barrier(0);
             const Pack<T, LPT> cF = rF[k], cf = rf[k], cQ = rQ[k];
             // old F at g-1: next ring slot (slot 0 was refilled with g0-D)
@@ -141,29 +174,10 @@ __global__ __launch_bounds__(kBlock) void energy_half_kernel(
             fetch(k, g - D);
             double Fv[LPT], fv[LPT];
 #pragma unroll
-            for (int j = 0; j < LPT; ++j) {
-                const double P = static_cast<double>(cF.v[j]);
-                const double R = static_cast<double>(cf.v[j]);
-                double a = c.af[0] * P_hi[j] + c.af[1] * R_hi[j] + c.af[2] * P + c.af[3] * R;
-                double b = c.aF[0] * P_hi[j] + c.aF[1] * R_hi[j] + c.aF[2] * P + c.aF[3] * R;
-                if (STRAG) {
-                    const double P_lo = g > 0 ? static_cast<double>(lo.v[j]) : 0.0;
-                    a += c.af[4] * P_lo;
-                    b += c.aF[4] * P_lo;
-                }
-                if (SRC) {
-                    const double src = static_cast<double>(cQ.v[j]);
-                    a += c.af[5] * src;
-                    b += c.aF[5] * src;
-                }
-                a += c.af[6] * hi_delta[j];
-                b += c.aF[6] * hi_delta[j];
-                fv[j] = a;
-                Fv[j] = b;
-                P_hi[j] = P;
-                R_hi[j] = R;
-                hi_delta[j] = b - a;
-            }
+            for (int j = 0; j < LPT; ++j)
+                recur<SRC, STRAG>(c, cF.v[j], cf.v[j], g > 0 ? static_cast<double>(lo.v[j]) : 0.0,
+                                  SRC ? static_cast<double>(cQ.v[j]) : 0.0, P_hi[j], R_hi[j],
+                                  hi_delta[j], fv[j], Fv[j]);
             const long long base = static_cast<long long>(g) * lanes;
             store_pack<T, LPT>(F + base + lane, Fv);
             store_pack<T, LPT>(f + base + lane, fv);
@@ -198,6 +212,158 @@ __global__ __launch_bounds__(kBlock) void energy_half_kernel(
             if (threadIdx.x == 0) partials[gridDim.x + blockIdx.x] = p;
         }
     }
+}
+
+// Two consecutive half steps in one sweep: the final E-half of depth step n
+// (source Qa, with the dose and proxy of step n) and the first E-half of step
+// n+1 (source Qb), i.e. E(h) o E(h) over the same coefficients, reading and
+// writing the state once instead of twice.  The second recurrence runs one
+// group behind the first: at group g it needs the first's outputs at g+1 and
+// g+2 and, for straggling, at g, which the first has just produced.  The
+// first's outputs are rounded to the state type before the second uses them,
+// exactly as if stored and reloaded, so the result is bit-identical to two
+// launches.  XOUT also stores them (the state at the end of step n, which the
+// secondary source of step n reads).
+template <typename T, int LPT, bool SRC>
+constexpr int pair_depth() {
+    return LPT == 4 ? (SRC ? 4 : 6) : (sizeof(T) == 4 ? (SRC ? 10 : 14) : (SRC ? 4 : 6));
+}
+
+template <typename T, int LPT, bool SRC, bool STRAG, bool XOUT>
+__global__ __launch_bounds__(kBlock) void energy_pair_kernel(
+    T* __restrict__ F, T* __restrict__ f, const EnergyCoef* __restrict__ coef,
+    const T* __restrict__ Qa, const T* __restrict__ Qb, T* __restrict__ XF, T* __restrict__ Xf,
+    int Ng, int lanes, const double* __restrict__ yzw, int nang, double* __restrict__ partials,
+    int proxy) {
+    constexpr int D = pair_depth<T, LPT, SRC>();
+    __shared__ double lds[kBlock / 64];
+    const int lane = (blockIdx.x * kBlock + threadIdx.x) * LPT;
+    double acc_dose = 0.0, acc_proxy = 0.0;
+    if (lane < lanes) {
+        Pack<T, LPT> rF[D], rf[D], rQa[D], rQb[D];
+        auto fetch = [&](int k, int g) {
+            const long long base = static_cast<long long>(max(g, 0)) * lanes;
+            rF[k] = load_pack<T, LPT>(F + base + lane);
+            rf[k] = load_pack<T, LPT>(f + base + lane);
+            if (SRC) {
+                rQa[k] = load_pack<T, LPT>(Qa + base + lane);
+                rQb[k] = load_pack<T, LPT>(Qb + base + lane);
+            }
+        };
+#pragma unroll
+        for (int k = 0; k < D; ++k) fetch(k, Ng - 1 - k);
+        // first half: carried input at g+1, output difference; dose sums
+        T P_hi[LPT], R_hi[LPT];
+        double hd[LPT], dose[LPT];
+        // second half, one group behind: its input (the first's output,
+        // rounded to the state type) at g+1 and g+2, its output difference,
+        // its source at g+1
+        T P2[LPT], R2[LPT], P2_hi[LPT], R2_hi[LPT], q2[LPT];
+        double hd2[LPT];
+#pragma unroll
+        for (int j = 0; j < LPT; ++j) {
+            P_hi[j] = R_hi[j] = P2[j] = R2[j] = P2_hi[j] = R2_hi[j] = q2[j] = T(0);
+            hd[j] = dose[j] = hd2[j] = 0.0;
+        }
+        // second half at group g2, whose lower neighbour's input is P_lo2
+        auto second = [&](int g2, const T (&P_lo2)[LPT]) {
+            const Forms c2 = load_forms(coef, g2);
+            double Fv[LPT], fv[LPT];
+#pragma unroll
+            for (int j = 0; j < LPT; ++j)
+                recur<SRC, STRAG>(c2, P2[j], R2[j], static_cast<double>(P_lo2[j]),
+                                  static_cast<double>(q2[j]), P2_hi[j], R2_hi[j], hd2[j], fv[j],
+                                  Fv[j]);
+            const long long base = static_cast<long long>(g2) * lanes;
+            store_pack<T, LPT>(F + base + lane, Fv);
+            store_pack<T, LPT>(f + base + lane, fv);
+        };
+        Forms cn = load_forms(coef, Ng - 1);
+        auto step = [&](int k, int g) {
+            const Forms c = cn;
+            cn = load_forms(coef, max(g - 1, 0));
+            __builtin_amdgcn_sched_barrier(0);
+            const Pack<T, LPT> cF = rF[k], cf = rf[k], cQa = rQa[k], cQb = rQb[k];
+            const Pack<T, LPT> lo = rF[(k + 1) % D];
+            fetch(k, g - D);
+            double Fv[LPT], fv[LPT];
+            T tF[LPT];
+#pragma unroll
+            for (int j = 0; j < LPT; ++j)
+                recur<SRC, STRAG>(c, cF.v[j], cf.v[j], g > 0 ? static_cast<double>(lo.v[j]) : 0.0,
+                                  SRC ? static_cast<double>(cQa.v[j]) : 0.0, P_hi[j], R_hi[j],
+                                  hd[j], fv[j], Fv[j]);
+            {  // dose and proxy of step n from the first half's outputs
+                const double w1 = coef[g].w1, w2 = coef[g].w2;
+#pragma unroll
+                for (int j = 0; j < LPT; ++j) {
+                    dose[j] += Fv[j] * w1 + fv[j] * w2;
+                    acc_proxy += fv[j];
+                }
+            }
+            if (XOUT) {
+                const long long base = static_cast<long long>(g) * lanes;
+                store_pack<T, LPT>(XF + base + lane, Fv);
+                store_pack<T, LPT>(Xf + base + lane, fv);
+            }
+#pragma unroll
+            for (int j = 0; j < LPT; ++j) tF[j] = static_cast<T>(Fv[j]);
+            if (g + 1 < Ng) second(g + 1, tF);
+#pragma unroll
+            for (int j = 0; j < LPT; ++j) {  // the first's output at g: the second's next input
+                P2[j] = tF[j];
+                R2[j] = static_cast<T>(fv[j]);
+                q2[j] = SRC ? cQb.v[j] : T(0);
+            }
+        };
+        int g0 = Ng - 1;
+        for (; g0 >= D - 1; g0 -= D) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) step(k, g0 - k);
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+            if (k <= g0) step(k, g0 - k);
+        const T none[LPT] = {};
+        second(0, none);  // group 0 has no lower neighbour
+#pragma unroll
+        for (int j = 0; j < LPT; ++j) acc_dose += dose[j] * yzw[(lane + j) / nang];
+    }
+    const double d = block_sum(acc_dose, lds);
+    if (threadIdx.x == 0) partials[blockIdx.x] = d;
+    if (proxy) {
+        const double p = block_sum(acc_proxy, lds);
+        if (threadIdx.x == 0) partials[gridDim.x + blockIdx.x] = p;
+    }
+}
+
+template <typename T, int LPT, bool SRC, bool STRAG>
+void dispatch_pair(T* F, T* f, const EnergyCoef* coef, const T* Qa, const T* Qb, T* XF, T* Xf,
+                   int Ng, long long lanes, const double* yzw, int nang, double* partials,
+                   bool proxy, hipStream_t s) {
+    const int blocks = energy_blocks(lanes, sizeof(T));
+    BFPN_REQUIRE(lanes < (1LL << 31), "energy sweep indexes lanes with 32-bit offsets");
+    if (XF)
+        hipLaunchKernelGGL((energy_pair_kernel<T, LPT, SRC, STRAG, true>), dim3(blocks),
+                           dim3(kBlock), 0, s, F, f, coef, Qa, Qb, XF, Xf, Ng,
+                           static_cast<int>(lanes), yzw, nang, partials, proxy ? 1 : 0);
+    else
+        hipLaunchKernelGGL((energy_pair_kernel<T, LPT, SRC, STRAG, false>), dim3(blocks),
+                           dim3(kBlock), 0, s, F, f, coef, Qa, Qb, XF, Xf, Ng,
+                           static_cast<int>(lanes), yzw, nang, partials, proxy ? 1 : 0);
+}
+
+template <typename T, bool SRC, bool STRAG>
+void dispatch_pair_lpt(T* F, T* f, const EnergyCoef* coef, const T* Qa, const T* Qb, T* XF,
+                       T* Xf, int Ng, long long lanes, const double* yzw, int nang,
+                       double* partials, bool proxy, hipStream_t s) {
+    if constexpr (sizeof(T) == 4) {
+        if (lanes_per_thread(lanes, sizeof(T)) == 4)
+            return dispatch_pair<T, 4, SRC, STRAG>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yzw,
+                                                   nang, partials, proxy, s);
+    }
+    dispatch_pair<T, 1, SRC, STRAG>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yzw, nang, partials,
+                                    proxy, s);
 }
 
 template <typename T, int LPT, bool SRC, bool STRAG>
@@ -255,6 +421,39 @@ void launch_energy_half(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng,
     }
     BFPN_HIP(hipGetLastError());
 }
+
+template <typename T>
+void launch_energy_pair(T* F, T* f, const EnergyCoef* coef, const T* Qa, const T* Qb, int Ng,
+                        long long lanes, bool straggling, const double* yz_weight, int nang,
+                        double* dose_partials, bool proxy, T* XF, T* Xf, hipStream_t s) {
+    if (lanes <= 0 || Ng <= 0) return;
+    BFPN_REQUIRE(dose_partials, "the paired energy sweep needs dose partials");
+    BFPN_REQUIRE((Qa != nullptr) == (Qb != nullptr), "both half steps need a source, or neither");
+    BFPN_REQUIRE((XF != nullptr) == (Xf != nullptr), "the intermediate state needs both arrays");
+    if (Qa) {
+        if (straggling)
+            dispatch_pair_lpt<T, true, true>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yz_weight,
+                                              nang, dose_partials, proxy, s);
+        else
+            dispatch_pair_lpt<T, true, false>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yz_weight,
+                                               nang, dose_partials, proxy, s);
+    } else {
+        if (straggling)
+            dispatch_pair_lpt<T, false, true>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yz_weight,
+                                               nang, dose_partials, proxy, s);
+        else
+            dispatch_pair_lpt<T, false, false>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yz_weight,
+                                                nang, dose_partials, proxy, s);
+    }
+    BFPN_HIP(hipGetLastError());
+}
+
+template void launch_energy_pair<float>(float*, float*, const EnergyCoef*, const float*,
+                                        const float*, int, long long, bool, const double*, int,
+                                        double*, bool, float*, float*, hipStream_t);
+template void launch_energy_pair<double>(double*, double*, const EnergyCoef*, const double*,
+                                         const double*, int, long long, bool, const double*, int,
+                                         double*, bool, double*, double*, hipStream_t);
 
 template void launch_energy_half<float>(float*, float*, const EnergyCoef*, const float*, int,
                                         long long, bool, const double*, int, double*, bool,

@@ -17,6 +17,13 @@ void launch_energy_half(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng,
 // Dose partials written by a sweep over `lanes` lanes of `elem`-byte state.
 int energy_blocks(long long lanes, size_t elem);
 
+// Two consecutive energy half steps in one sweep (end of depth step n with
+// source Qa and the dose/proxy partials of step n, start of step n+1 with
+// source Qb); XF/Xf (optional) receive the state between them.
+template <typename T>
+void launch_energy_pair(T* F, T* f, const EnergyCoef* coef, const T* Qa, const T* Qb, int Ng,
+                        long long lanes, bool straggling, const double* yz_weight, int nang,
+                        double* dose_partials, bool proxy, T* XF, T* Xf, hipStream_t s);
 // Split-BF16 operands of the FP32 source (matrix cores), built once per beam.
 struct SourcePlan {
     const __bf16* kth = nullptr;  // Kv transposed [g][out][in], nap x nap, hi part

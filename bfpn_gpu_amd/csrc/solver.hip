@@ -299,11 +299,12 @@ public:
         if (P_->secondary) {
             a1_.ensure(state_bytes);
             a2_.ensure(state_bytes);
-            q_.ensure(state_bytes);
+            for (auto& q : q_) q.ensure(state_bytes);
+            for (auto& x : x_) x.ensure(state_bytes);
         }
         for (int k = 0; k < 4; ++k) cur_[k] = state_[k].as<T>();
         const int eb = energy_blocks(P_->lanes, sizeof(T));
-        partial_p_.ensure(2 * 2 * eb * sizeof(double));  // two slots: steps alternate
+        partial_p_.ensure(4 * 2 * eb * sizeof(double));  // four slots: steps rotate
         partial_s_.ensure(eb * sizeof(double));
         grow_history(std::max(16, g.Nx));
         // Eq.23 beam on the device (the reference builds a full host array)
@@ -329,7 +330,8 @@ public:
     }
 
     size_t device_bytes() const override {
-        size_t b = a1_.bytes() + a2_.bytes() + q_.bytes() + hist_.bytes() + partial_p_.bytes() +
+        size_t b = a1_.bytes() + a2_.bytes() + q_[0].bytes() + q_[1].bytes() + x_[0].bytes() +
+                   x_[1].bytes() + hist_.bytes() + partial_p_.bytes() +
                    partial_s_.bytes();
         for (const auto& s : state_) b += s.bytes();
         return b;
@@ -346,7 +348,7 @@ public:
             BFPN_HIP(hipEventRecord(ev_tail_, stream2_));
             BFPN_HIP(hipStreamWaitEvent(stream, ev_tail_, 0));
         }
-        for (int k = 0; k < nsteps; ++k) step();
+        for (int k = 0; k < nsteps; ++k) step(k == 0, k == nsteps - 1);
         if (stream2_ && !overlap) {  // all on `stream`: done_stream still ends last
             BFPN_HIP(hipEventRecord(ev_tail_, stream));
             BFPN_HIP(hipStreamWaitEvent(stream2_, ev_tail_, 0));
@@ -438,12 +440,24 @@ private:
 
     double array_bytes() const { return static_cast<double>(cells) * sizeof(T); }
 
-    void energy(int species, bool with_source, double* partials, bool proxy, hipStream_t s) {
+    // One energy half step of a species (source Q or none; dose partials or none).
+    void energy(int species, const T* Q, double* partials, bool proxy, hipStream_t s) {
         launch_energy_half<T>(cur_[2 * species], cur_[2 * species + 1], P_->coef.template as<EnergyCoef>(),
-                              with_source ? q_.as<T>() : nullptr, grid.Ng, P_->lanes,
-                              phys.eq15_straggling != 0, P_->yzw.template as<double>(), P_->nang,
-                              partials, proxy, s);
-        mark(kEnergy, array_bytes() * (with_source ? 5 : 4), s);
+                              Q, grid.Ng, P_->lanes, phys.eq15_straggling != 0,
+                              P_->yzw.template as<double>(), P_->nang, partials, proxy, s);
+        mark(kEnergy, array_bytes() * (Q ? 5 : 4), s);
+    }
+
+    // The end half of one depth step and the start half of the next in one
+    // sweep (sources Qa, Qb or none; the dose of the first); X (optional)
+    // receives the state at the end of the step.
+    void energy_pair(int species, const T* Qa, const T* Qb, double* partials, bool proxy, T* const* X,
+                     hipStream_t s) {
+        launch_energy_pair<T>(cur_[2 * species], cur_[2 * species + 1], P_->coef.template as<EnergyCoef>(),
+                              Qa, Qb, grid.Ng, P_->lanes, phys.eq15_straggling != 0,
+                              P_->yzw.template as<double>(), P_->nang, partials, proxy,
+                              X ? X[0] : nullptr, X ? X[1] : nullptr, s);
+        mark(kEnergy, array_bytes() * (4 + (Qa ? 2 : 0) + (X ? 2 : 0)), s);
     }
 
     void transport(int which, bool preserve_sign, hipStream_t s) {
@@ -466,48 +480,68 @@ private:
         mark(kAngle, array_bytes() * 2, s);
     }
 
-    // Secondary source on stream s; `inputs_read` (optional) is recorded on s
-    // once the primary state has been read (after the angular GEMM).
-    void source(hipStream_t s, hipEvent_t inputs_read) {
+    // Secondary source Q from the primary state (Fp, fp) on stream s;
+    // `inputs_read` (optional) is recorded on s once (Fp, fp) have been read
+    // (after the angular GEMM).
+    void source(const T* Fp, const T* fp, T* Q, hipStream_t s, hipEvent_t inputs_read) {
         if constexpr (std::is_same_v<T, float>) {
-            launch_source_mfma(cur_[0], cur_[1], a1_.as<float>(), a2_.as<float>(), q_.as<float>(),
-                               P_->plan.plan, grid.Ng, P_->nyz, P_->nang, s, inputs_read);
+            launch_source_mfma(Fp, fp, a1_.as<float>(), a2_.as<float>(), Q, P_->plan.plan,
+                               grid.Ng, P_->nyz, P_->nang, s, inputs_read);
         } else {
-            launch_source_angular<T>(cur_[0], cur_[1], P_->kv.template as<T>(), a1_.as<T>(),
-                                     a2_.as<T>(), grid.Ng, P_->nyz, P_->nang, s);
+            launch_source_angular<T>(Fp, fp, P_->kv.template as<T>(), a1_.as<T>(), a2_.as<T>(),
+                                     grid.Ng, P_->nyz, P_->nang, s);
             if (inputs_read) BFPN_HIP(hipEventRecord(inputs_read, s));
             launch_source_energy<T>(a1_.as<T>(), a2_.as<T>(), P_->e1.template as<double>(),
                                     P_->e2.template as<double>(), P_->beg.template as<int>(),
-                                    P_->end.template as<int>(), q_.as<T>(), grid.Ng, P_->lanes, d.dg,
-                                    s);
+                                    P_->end.template as<int>(), Q, grid.Ng, P_->lanes, d.dg, s);
         }
         mark(kSource, array_bytes() * 7, s);  // 2 in, 2 out; 2 in, 1 out
     }
 
-    // One depth step (BFPnSolver::solve loop body, src/bfpm_solver.cu:2213-2242).
-    // Primary on `stream` (sp); source, secondary and the IDD reduction on
-    // stream2_ (ss) after the primary of the same step.  The next step's
-    // primary waits for this step's angular source GEMM (the last reader of
-    // the primary state it overwrites); the primary dose partials alternate
-    // between two slots, so step n+1 never writes the slot that step n's
-    // reduction reads (step n+2, which does, waits for the source of n+1,
-    // issued after that reduction on ss).
-    void step() {
+    // One depth step (BFPnSolver::solve loop body, src/bfpm_solver.cu:2213-2242)
+    // of an advance() call (`first`, `last`: its first and last step).
+    //
+    // Energy half steps: inside a call, the end half of step n and the start
+    // half of step n+1 run as one paired sweep, so a call of k steps makes
+    // k + 1 energy sweeps per species instead of 2k; the call's first step
+    // starts and its last step ends with a single half step, so between calls
+    // the state is the reference's state after the last step.
+    //
+    // Streams: the primary on `stream` (sp); the source, the secondary and the
+    // IDD reduction on stream2_ (ss).  The primary state at the end of step n
+    // is what the source of step n reads: the paired primary sweep also
+    // writes it to x_ (inside a call), or it is the state itself (end of a
+    // call).  The primary waits for that source's angular GEMM before it next
+    // overwrites what the GEMM reads: x_ at its next paired sweep, or the state
+    // at the start of the next call.  The secondary's end half of step n needs
+    // the source of step n+1, so it runs, paired with the start half of n+1,
+    // after that source; the IDD of step n is reduced then.  Sources alternate
+    // between q_[0] and q_[1]; the primary dose partials rotate over four slots
+    // (the reduction of step n precedes, on ss, the source of step n+2, which
+    // the primary waits for before it writes slot n again at step n+4).
+    void step(bool first, bool last) {
         const bool preserve_primary = phys.no_spatial_clipping != 0;
         const int eb = energy_blocks(P_->lanes, sizeof(T));
-        double* pp = partial_p_.as<double>() + (step_ & 1) * 2 * eb;
+        const int n = step_;
+        double* pp = partial_p_.as<double>() + (n & 3) * 2 * eb;
         double* ps = partial_s_.as<double>();
-        hipStream_t sp = stream, ss = stream2_ && overlap ? stream2_ : stream;
-        if (src_pending_) {
-            BFPN_HIP(hipStreamWaitEvent(sp, ev_src_, 0));
-            src_pending_ = false;
+        const bool two = stream2_ && overlap;
+        hipStream_t sp = stream, ss = two ? stream2_ : stream;
+        const bool secondary = P_->secondary;
+        T* X[2] = {secondary ? x_[0].as<T>() : nullptr, secondary ? x_[1].as<T>() : nullptr};
+        auto wait_source = [&] {
+            if (src_pending_) {
+                BFPN_HIP(hipStreamWaitEvent(sp, ev_src_, 0));
+                src_pending_ = false;
+            }
+        };
+        // ---- primary: stepPrimary / stepPrimaryEnergyOnly (src/bfpm_solver.cu:3238-3333)
+        if (first) {
+            wait_source();
+            mark(kWait, 0.0, sp);  // opens the first interval
+            energy(0, nullptr, nullptr, false, sp);
         }
-        mark(kWait, 0.0, sp);  // opens the first interval of the step
-        if (phys.energy_only) {  // stepPrimaryEnergyOnly, src/bfpm_solver.cu:3313-3333
-            energy(0, false, nullptr, false, sp);
-            energy(0, false, pp, true, sp);
-        } else {  // stepPrimary, src/bfpm_solver.cu:3238-3311
-            energy(0, false, nullptr, false, sp);
+        if (!phys.energy_only) {
             if (!phys.no_transport) {
                 transport(0, preserve_primary, sp);
                 transport(1, true, sp);
@@ -520,36 +554,62 @@ private:
                 transport(0, preserve_primary, sp);
                 transport(1, true, sp);
             }
-            energy(0, false, pp, true, sp);
         }
-        if (ss != sp) {
+        if (last) {
+            energy(0, nullptr, pp, true, sp);
+        } else {
+            wait_source();  // x_ is about to be rewritten
+            if (!first) mark(kWait, 0.0, sp);
+            energy_pair(0, nullptr, nullptr, pp, true, secondary ? X : nullptr, sp);
+        }
+        const T* Fp = last ? cur_[0] : X[0];  // primary state at the end of step n
+        const T* fp = last ? cur_[1] : X[1];
+        if (two) {
             BFPN_HIP(hipEventRecord(ev_prim_, sp));
             BFPN_HIP(hipStreamWaitEvent(ss, ev_prim_, 0));
             mark(kWait, 0.0, ss);
         }
-        const bool sec = P_->secondary && step_ > 0;
-        if (sec) {  // stepSecondary, src/bfpm_solver.cu:3335-3394
-            source(ss, ss != sp ? ev_src_ : nullptr);
-            src_pending_ = ss != sp;
-            energy(1, true, nullptr, false, ss);
+        // ---- secondary: stepSecondary (src/bfpm_solver.cu:3335-3394), step n > 0
+        const double scale = d.du * d.dv * d.dy * d.dz / phys.density;
+        double* hist = hist_.as<double>();
+        auto finish = [&](int m, bool with_secondary) {  // IDD and proxy of step m
+            double* pm = partial_p_.as<double>() + (m & 3) * 2 * eb;
+            launch_finish_sum(pm, eb, with_secondary ? ps : nullptr, eb, scale, hist + m, ss);
+            launch_finish_sum(pm + eb, eb, nullptr, 0, d.dg, hist + hist_cap_ + m, ss);
+            mark(kDose, 0.0, ss);
+        };
+        const bool sec = secondary && n > 0;
+        if (sec) {
+            T* Q = q_[n & 1].as<T>();
+            source(Fp, fp, Q, ss, two ? ev_src_ : nullptr);
+            src_pending_ = two;
+            if (sec_open_) {  // end half of step n-1 with the start half of step n
+                energy_pair(1, q_[(n - 1) & 1].as<T>(), Q, ps, false, nullptr, ss);
+                finish(n - 1, true);
+            } else {
+                energy(1, Q, nullptr, false, ss);
+            }
             transport(2, false, ss);  // secondary psi^1 is always clipped
             transport(3, true, ss);
             angle(2, ss);
             angle(3, ss);
             transport(2, false, ss);
             transport(3, true, ss);
-            energy(1, true, ps, false, ss);
+            if (last) {
+                energy(1, Q, ps, false, ss);
+                finish(n, true);
+                sec_open_ = false;
+            } else {
+                sec_open_ = true;  // its end half waits for the source of step n+1
+            }
+        } else {
+            finish(n, false);  // no secondary in step n: its IDD is the primary's
         }
-        const double scale = d.du * d.dv * d.dy * d.dz / phys.density;
-        double* hist = hist_.as<double>();
-        launch_finish_sum(pp, eb, sec ? ps : nullptr, eb, scale, hist + step_, ss);
-        launch_finish_sum(pp + eb, eb, nullptr, 0, d.dg, hist + hist_cap_ + step_, ss);
-        mark(kDose, 0.0, ss);
         ++step_;
     }
 
     std::shared_ptr<Physics<T>> P_;
-    DeviceBuffer state_[4], a1_, a2_, q_;
+    DeviceBuffer state_[4], a1_, a2_, q_[2], x_[2];
     DeviceBuffer partial_p_, partial_s_, hist_;
     T* cur_[4] = {nullptr, nullptr, nullptr, nullptr};
     int step_ = 0;
@@ -557,7 +617,8 @@ private:
     std::vector<std::pair<hipEvent_t, Phase>> events_[2];  // per stream (profiling)
     hipStream_t stream2_ = nullptr;  // source + secondary; null without a secondary
     hipEvent_t ev_prim_ = nullptr, ev_src_ = nullptr, ev_tail_ = nullptr;
-    bool src_pending_ = false;  // the next primary must wait for ev_src_
+    bool src_pending_ = false;  // the primary must wait for ev_src_ before its next write
+    bool sec_open_ = false;     // the secondary's last step still lacks its end half
 };
 
 }  // namespace
@@ -583,7 +644,7 @@ size_t context_bytes(const bfpn_grid& g, const bfpn_phys& p, int precision) {
     const size_t cells = lanes * g.Ng;
     const size_t word = precision == BFPN_FP64 ? 8 : 4;
     const bool sec = !p.primary_only && !p.energy_only;
-    return cells * word * (sec ? 7 : 4) + 3 * ((lanes + 255) / 256) * 8 +
+    return cells * word * (sec ? 10 : 4) + 9 * ((lanes + 255) / 256) * 8 +
            2 * static_cast<size_t>(std::max(16, g.Nx)) * 8;
 }
 

@@ -122,6 +122,26 @@ def test_overlap_gives_identical_results(gpu, precision):
         np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("precision", ["FP32", "FP64"])
+@pytest.mark.parametrize("flags", [{"eq15_straggling": 1}, {}, {"energy_only": 1},
+                                   {"primary_only": 1, "eq15_straggling": 1}],
+                         ids=lambda f: "-".join(sorted(f)) or "default")
+def test_paired_energy_sweeps_are_bit_identical(gpu, precision, flags):
+    """Inside one advance() call the end half of step n and the start half of
+    step n+1 run as one paired sweep; one step per call runs every half step
+    alone.  States, IDD and proxy agree bit for bit."""
+    case = small_case(Ny=3, Nz=4, flags=flags)
+    prec = getattr(gpu, precision)
+    with gpu.Solver(case, prec) as one, gpu.Solver(case, prec) as many:
+        for _ in range(5):
+            one.advance(1)
+        many.advance(5)
+        for a, b in zip(one.history(), many.history()):
+            np.testing.assert_array_equal(a, b)
+        for k in range(4):
+            np.testing.assert_array_equal(one.state(k), many.state(k))
+
+
 def test_history_and_restart_are_consistent(gpu):
     """advance(a)+advance(b) equals advance(a+b); write_state round-trips."""
     case = small_case(Ny=3, Nz=3)
