@@ -52,6 +52,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+STEP_BYTES_PER_CELL = 176.0  # algorithmic HBM bytes per cell-update of the step model
 METRIC = "phase-space cell-updates/s; beamlet solves/s + p99 latency at IDD err ≤0.5%"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_single_beam.md")
 
@@ -172,8 +173,12 @@ def run_resident(args, case, workload, with_traffic):
         "config": {"workload": workload, "cells": cells,
                    "precision": "FP32 state; FP64 energy recurrence and IDD"
                    if prec == bg.FP32 else "FP64",
+                   # the step model: 176 B of HBM traffic per cell-update (both
+                   # species, FP32 state) at 8 TB/s is 45.5 G cell-updates/s
                    "step_fraction_of_hbm_roofline":
-                       round(step_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+                       round(cells / (ms_per_step * 1e-3) * STEP_BYTES_PER_CELL
+                             * (2 if prec == bg.FP64 else 1) / 1e9 / HBM_PEAK_GBS, 4),
+                   "moved_bytes_per_step_GB": round(step_bytes / 1e9, 3)},
         "roofline": roofline,
         "phases": phases,
     }

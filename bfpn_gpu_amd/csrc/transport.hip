@@ -76,11 +76,16 @@ __device__ inline T limited_slope(T a, T b) {
 // 1e-10 test going to whichever is forward in +y.  z: physical window
 // (zm, zc, zp) and carried physical slopes of rows -1, 0, +1; hz = +-1/2
 // with the sign of wz.  cy = |wy| / dy, cz = |wz| / dz.
+// (near, far, down): the y neighbours, read from an LDS ring row.
 template <typename T>
-__device__ inline T divergence(const T* row, int o_near, int o_far, int o_down, T zm, T zc, T zp,
-                               T sz_m, T sz_0, T sz_p, T cy, T cz, T hz, bool py, bool pz) {
-    const T near = row[o_near], far = row[o_far], down = row[o_down];
-    const T A = down - zc, B = zc - near, C = near - far;
+struct Nbrs {
+    T near, far, down;
+};
+
+template <typename T>
+__device__ inline T divergence(const Nbrs<T>& n, T zm, T zc, T zp, T sz_m, T sz_0, T sz_p, T cy,
+                               T cz, T hz, bool py, bool pz) {
+    const T A = n.down - zc, B = zc - n.near, C = n.near - n.far;
     const T s_down = fabs(py ? A : B) < T(1e-10) ? T(0) : superbee(A, B);
     const T s_up = fabs(py ? B : C) < T(1e-10) ? T(0) : superbee(B, C);
     const T fy = fma(T(0.5), s_down - s_up, B);
@@ -205,16 +210,27 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
 #pragma unroll
             for (int j = 0; j < P; ++j) nxt[j][(k + 2) & 3] = at(src, boff[j]);
         }
+        // y neighbours of both stages, read together so the corrector's LDS
+        // round trip overlaps the predictor (psi* row t-2 was written two
+        // steps ago; the predictor below writes another slot)
+        Nbrs<T> np_[P], nc_[P];
+        {
+            const T* prd = ring + k * MW;                   // psi row t
+            const T* crd = ring + (4 + ((k + 2) & 3)) * MW;  // psi* row t-2
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                np_[j] = Nbrs<T>{prd[o_near[j]], prd[o_far[j]], prd[o_down[j]]};
+                nc_[j] = Nbrs<T>{crd[o_near[j]], crd[o_far[j]], crd[o_down[j]]};
+            }
+        }
         {  // predictor on row t
-            const T* row = ring + k * MW;
             T* prow = ring + (4 + k) * MW;
 #pragma unroll
             for (int j = 0; j < P; ++j) {
                 const T zm = wpsi[j][(k + 3) & 3], zc = wpsi[j][k], zp = wpsi[j][(k + 1) & 3];
                 const T s_next = limited_slope(wpsi[j][(k + 2) & 3] - zp, zp - zc);
-                const T L0 = divergence(row, o_near[j], o_far[j], o_down[j], zm, zc, zp,
-                                        sz[j][(k + 1) & 1], sz[j][k & 1], s_next, cy, cz, hz, py,
-                                        pz);
+                const T L0 = divergence(np_[j], zm, zc, zp, sz[j][(k + 1) & 1], sz[j][k & 1],
+                                        s_next, cy, cz, hz, py, pz);
                 T v = zc - h * L0;
                 if (CLIP) v = fmax(v, T(0));
                 if (on[j]) prow[o0[j]] = v;
@@ -225,7 +241,6 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
             }
         }
         {  // corrector on row t-2, written in place
-            const T* row = ring + (4 + ((k + 2) & 3)) * MW;
             T* dst = Fg + static_cast<long long>(t - 2) * zstride;
             const bool real = !E || (t >= 2 && t <= Nz + 2);
 #pragma unroll
@@ -233,9 +248,8 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
                 const T zm = wpred[j][(k + 1) & 3], zc = wpred[j][(k + 2) & 3],
                         zp = wpred[j][(k + 3) & 3];
                 const T s_next = limited_slope(wpred[j][k] - zp, zp - zc);
-                const T L1 = divergence(row, o_near[j], o_far[j], o_down[j], zm, zc, zp,
-                                        szp[j][(k + 1) & 1], szp[j][k & 1], s_next, cy, cz, hz,
-                                        py, pz);
+                const T L1 = divergence(nc_[j], zm, zc, zp, szp[j][(k + 1) & 1], szp[j][k & 1],
+                                        s_next, cy, cz, hz, py, pz);
                 T v = wbase[j][(k + 2) & 3] - hh * L1;
                 if (CLIP) v = fmax(v, T(0));
                 if (real && on[j] && live) at(dst, boff[j]) = v;
