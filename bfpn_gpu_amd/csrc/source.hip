@@ -167,22 +167,30 @@ __global__ __launch_bounds__(256) void source_angular_mfma(const float* __restri
 
 constexpr int kBandLanes = 128;  // lanes per work-group
 constexpr int kBandRows = 64;    // output groups per work-group
-constexpr int kPitch = kBandLanes + 2;
 
 // Banded energy GEMM.  Logical tile = (128 lanes, 64 output groups), the
 // group blocks of one lane block consecutive (one XCD, one L2 for the A rows
 // their bands share).  Q[g][lane] = E1.A1 + E2.A2 over the union of the
 // 16-row tiles' bands; each 16-row tile skips K steps outside its own band.
-// The next K step's A rows are in flight during the current step's MFMAs.
-// VEC: lanes % 4 == 0 (16-byte loads).
+// Per 32-wide K step the work-group stages in LDS the 32 x 128 tiles of A1
+// and A2 (FP32) and the 64 x 32 tiles of E1 and E2 (BF16 hi/lo), the next
+// step's tiles in flight in registers meanwhile.  Each wave forms the B
+// fragments of one input at a time and reads E fragments one 16-row tile at a
+// time from LDS (rows padded to 40 values against bank conflicts), so few
+// fragments are live (three work-groups per CU).  The
+// A tiles have no padding: the 16-column blocks of a row are XOR-swizzled by
+// (row / 8) mod 4, so the fragment reads (rows 8q..8q+7 of one column, q =
+// 0..3) fall on 64 distinct banks.  VEC: lanes % 4 == 0 (16-byte loads).
+__device__ inline int band_swz(int r, int c) { return c ^ (((r >> 3) & 3) << 4); }
+
 template <bool VEC>
 __global__ __launch_bounds__(256) void source_band_mfma(
     const float* __restrict__ A1, const float* __restrict__ A2, const __bf16* __restrict__ e1h,
     const __bf16* __restrict__ e1l, const __bf16* __restrict__ e2h, const __bf16* __restrict__ e2l,
     const int2* __restrict__ tile_band, float* __restrict__ Q, int Ng, int ngp, long long lanes,
     int tiles) {
-    __shared__ float b1s[32][kPitch];
-    __shared__ float b2s[32][kPitch];
+    __shared__ __attribute__((aligned(16))) float bs[2][32][kBandLanes];        // A1, A2
+    __shared__ __attribute__((aligned(16))) __bf16 es[4][kBandRows][kKPitch];   // E1 hi/lo, E2 hi/lo
     const int L = xcd_tile(blockIdx.x, gridDim.x);
     if (L >= tiles) return;
     const int rblocks = (Ng + kBandRows - 1) / kBandRows;
@@ -205,8 +213,11 @@ __global__ __launch_bounds__(256) void source_band_mfma(
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) acc[mt][0] = acc[mt][1] = zero4();
 
-    // staging: 32 rows x 128 lanes of A1 and A2; VEC: 4 float4 per thread each
+    // staging: A, 4 float4 per thread and input; E, one 64-byte row segment
+    // (32 columns) of array ea = wave, row er = lane
     float4 r1[4], r2[4];
+    bf16x8 re[4];
+    const __bf16* esrc = wave == 0 ? e1h : wave == 1 ? e1l : wave == 2 ? e2h : e2l;
     auto load = [&](int k0) {
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -233,6 +244,10 @@ __global__ __launch_bounds__(256) void source_band_mfma(
                 r2[j] = make_float4(b[0], b[1], b[2], b[3]);
             }
         }
+        // rows g_base .. g_base+63 < erows and columns k0 .. k0+31 < ngp
+        const __bf16* src = esrc + static_cast<long long>(g_base + lane) * ngp + k0;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) re[c] = *reinterpret_cast<const bf16x8*>(src + 8 * c);
     };
     const int kstart = (kb / 32) * 32;
     if (kstart < ke) load(kstart);
@@ -240,38 +255,33 @@ __global__ __launch_bounds__(256) void source_band_mfma(
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int e4 = threadIdx.x + j * 256;
-            const int r = e4 >> 5, c = (e4 & 31) * 4;
-            b1s[r][c] = r1[j].x; b1s[r][c + 1] = r1[j].y; b1s[r][c + 2] = r1[j].z; b1s[r][c + 3] = r1[j].w;
-            b2s[r][c] = r2[j].x; b2s[r][c + 1] = r2[j].y; b2s[r][c + 2] = r2[j].z; b2s[r][c + 3] = r2[j].w;
+            const int r = e4 >> 5, c = band_swz(r, (e4 & 31) * 4);
+            *reinterpret_cast<float4*>(&bs[0][r][c]) = r1[j];
+            *reinterpret_cast<float4*>(&bs[1][r][c]) = r2[j];
         }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) *reinterpret_cast<bf16x8*>(&es[wave][lane][8 * c]) = re[c];
         __syncthreads();
         if (k0 + 32 < ke) load(k0 + 32);
-        SplitFrag bf1[2], bf2[2];
 #pragma unroll
-        for (int nt = 0; nt < 2; ++nt) {
-            const int c = wave * 32 + nt * 16 + i;
-            float v1[8], v2[8];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                v1[e] = b1s[8 * q + e][c];
-                v2[e] = b2s[8 * q + e][c];
-            }
-            bf1[nt] = split8(v1);
-            bf2[nt] = split8(v2);
-        }
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt) {
-            if (band[mt].y <= band[mt].x || k0 + 32 <= band[mt].x || k0 >= band[mt].y) continue;
-            const long long off = static_cast<long long>(g_base + mt * 16 + i) * ngp + k0 + 8 * q;
-            SplitFrag a1, a2;
-            a1.hi = *reinterpret_cast<const bf16x8*>(e1h + off);
-            a1.lo = *reinterpret_cast<const bf16x8*>(e1l + off);
-            a2.hi = *reinterpret_cast<const bf16x8*>(e2h + off);
-            a2.lo = *reinterpret_cast<const bf16x8*>(e2l + off);
+        for (int in = 0; in < 2; ++in) {  // E1.A1, then E2.A2 (the same order per accumulator)
+            SplitFrag bf[2];
 #pragma unroll
             for (int nt = 0; nt < 2; ++nt) {
-                acc[mt][nt] = mma_split(a1, bf1[nt], acc[mt][nt]);
-                acc[mt][nt] = mma_split(a2, bf2[nt], acc[mt][nt]);
+                const int c = wave * 32 + nt * 16 + i;
+                float v[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) v[e] = bs[in][8 * q + e][band_swz(8 * q + e, c)];
+                bf[nt] = split8(v);
+            }
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                if (band[mt].y <= band[mt].x || k0 + 32 <= band[mt].x || k0 >= band[mt].y) continue;
+                SplitFrag a;
+                a.hi = *reinterpret_cast<const bf16x8*>(&es[2 * in][mt * 16 + i][8 * q]);
+                a.lo = *reinterpret_cast<const bf16x8*>(&es[2 * in + 1][mt * 16 + i][8 * q]);
+#pragma unroll
+                for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = mma_split(a, bf[nt], acc[mt][nt]);
             }
         }
         __syncthreads();
