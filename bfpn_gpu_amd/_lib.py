@@ -91,6 +91,8 @@ def lib():
     L.bfpn_stream.restype = vp
     L.bfpn_set_profiling.argtypes = [vp, i]
     L.bfpn_set_overlap.argtypes = [vp, i]
+    L.bfpn_energy_moments.argtypes = [vp, vp, i]
+    L.bfpn_spot_plane.argtypes = [vp, vp, C.c_size_t]
     L.bfpn_phase_stats.argtypes = [vp, vp, vp, vp]
     L.bfpn_stopping_power.argtypes = [C.c_char_p, i, d, vp]
     L.bfpn_server_create.argtypes = [C.POINTER(ServerOpts), C.POINTER(vp)]
@@ -260,6 +262,18 @@ class Solver:
 
     def set_profiling(self, on: bool = True) -> None:
         check(lib().bfpn_set_profiling(self._h, int(on)))
+
+    def energy_moments(self) -> np.ndarray:
+        """Eq.25 moments of the current state: [Ng, 2] = (int psi^1, int psi^2) per group."""
+        out = np.zeros(2 * self.case.Ng)
+        check(lib().bfpn_energy_moments(self._h, _addr(out), out.size))
+        return out.reshape(self.case.Ng, 2)
+
+    def spot_plane(self) -> np.ndarray:
+        """Y-Z dose plane of the current state, flat: s = iz (Ny+1) + iy."""
+        out = np.zeros((self.case.Ny + 1) * (self.case.Nz + 1))
+        check(lib().bfpn_spot_plane(self._h, _addr(out), out.size))
+        return out
 
     def set_overlap(self, on: bool = True) -> None:
         """Secondary march on its own stream, overlapping the next primary step."""

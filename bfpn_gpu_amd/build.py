@@ -32,7 +32,7 @@ ARCH = os.environ.get("BFPN_OFFLOAD_ARCH", "gfx950")
 HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 
 LIB_SOURCES = ["tables.cpp", "energy.hip", "source.hip", "transport.hip", "angle.hip", "misc.hip",
-               "solver.hip", "server.hip"]
+               "diag.hip", "solver.hip", "server.hip"]
 COMMON = ["-std=c++17", "-O3", "-fPIC", "-Wall", "-Wno-unused-result",
           f"-I{os.path.join(ROOT, 'include')}"]
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -57,6 +57,11 @@ __device__ inline Pack<T, LPT> load_pack(const T* p) {
         r.v[1] = q.y;
         r.v[2] = q.z;
         r.v[3] = q.w;
+    } else if constexpr (LPT == 2) {
+        static_assert(sizeof(T) == 4, "2-lane packs are FP32");
+        const float2 q = *reinterpret_cast<const float2*>(p);
+        r.v[0] = q.x;
+        r.v[1] = q.y;
     } else {
         r.v[0] = *p;
     }
@@ -68,6 +73,8 @@ __device__ inline void store_pack(T* p, const double (&v)[LPT]) {
     if constexpr (LPT == 4) {
         *reinterpret_cast<float4*>(p) = make_float4(static_cast<float>(v[0]), static_cast<float>(v[1]),
                                                     static_cast<float>(v[2]), static_cast<float>(v[3]));
+    } else if constexpr (LPT == 2) {
+        *reinterpret_cast<float2*>(p) = make_float2(static_cast<float>(v[0]), static_cast<float>(v[1]));
     } else {
         *p = static_cast<T>(v[0]);
     }
@@ -76,7 +83,7 @@ __device__ inline void store_pack(T* p, const double (&v)[LPT]) {
 // Prefetch depth (groups in flight per thread): a register ring, slot k
 // refilled with group g - D right after group g consumed it.
 template <typename T, int LPT>
-constexpr int prefetch_depth() { return LPT == 4 ? 8 : (sizeof(T) == 4 ? 16 : 8); }
+constexpr int prefetch_depth() { return LPT == 4 ? 8 : LPT == 2 ? 12 : (sizeof(T) == 4 ? 16 : 8); }
 
 // Lanes per thread for a sweep over `lanes` lanes of `elem`-byte state.
 inline int lanes_per_thread(long long lanes, size_t elem) {
@@ -226,7 +233,8 @@ __global__ __launch_bounds__(kBlock) void energy_half_kernel(
 // secondary source of step n reads).
 template <typename T, int LPT, bool SRC>
 constexpr int pair_depth() {
-    return LPT == 4 ? (SRC ? 4 : 6) : (sizeof(T) == 4 ? (SRC ? 10 : 14) : (SRC ? 4 : 6));
+    return LPT == 4 ? (SRC ? 4 : 6)
+                    : LPT == 2 ? (SRC ? 6 : 8) : (sizeof(T) == 4 ? (SRC ? 10 : 14) : (SRC ? 4 : 6));
 }
 
 template <typename T, int LPT, bool SRC, bool STRAG, bool XOUT>
@@ -358,8 +366,12 @@ void dispatch_pair_lpt(T* F, T* f, const EnergyCoef* coef, const T* Qa, const T*
                        T* Xf, int Ng, long long lanes, const double* yzw, int nang,
                        double* partials, bool proxy, hipStream_t s) {
     if constexpr (sizeof(T) == 4) {
-        if (lanes_per_thread(lanes, sizeof(T)) == 4)
+        const int lpt = lanes_per_thread(lanes, sizeof(T));
+        if (lpt == 4)
             return dispatch_pair<T, 4, SRC, STRAG>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yzw,
+                                                   nang, partials, proxy, s);
+        if (lpt == 2)
+            return dispatch_pair<T, 2, SRC, STRAG>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yzw,
                                                    nang, partials, proxy, s);
     }
     dispatch_pair<T, 1, SRC, STRAG>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yzw, nang, partials,
@@ -385,8 +397,12 @@ template <typename T, bool SRC, bool STRAG>
 void dispatch_lpt(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng, long long lanes,
                   const double* yzw, int nang, double* partials, bool proxy, hipStream_t s) {
     if constexpr (sizeof(T) == 4) {
-        if (lanes_per_thread(lanes, sizeof(T)) == 4)
+        const int lpt = lanes_per_thread(lanes, sizeof(T));
+        if (lpt == 4)
             return dispatch_dose<T, 4, SRC, STRAG>(F, f, coef, Q1, Ng, lanes, yzw, nang, partials,
+                                                   proxy, s);
+        if (lpt == 2)
+            return dispatch_dose<T, 2, SRC, STRAG>(F, f, coef, Q1, Ng, lanes, yzw, nang, partials,
                                                    proxy, s);
     }
     dispatch_dose<T, 1, SRC, STRAG>(F, f, coef, Q1, Ng, lanes, yzw, nang, partials, proxy, s);

@@ -121,4 +121,17 @@ namespace bfpn {
 // dst[k] = (T) src[k], k < n (device to device)
 template <typename T>
 void launch_convert(const double* src, T* dst, long long n, hipStream_t s);
+// Eq.25 energy moments of psi^1 = F + F1 and psi^2 = f + f1 per group (out:
+// 2 Ng doubles, weight = du dv dy dz); partials: moment_partial_count doubles.
+size_t moment_partial_count(int Ng, long long lanes);
+template <typename T>
+void launch_energy_moments(const T* F, const T* f, const T* F1, const T* f1, int Ng, long long lanes,
+                           double weight, double* partials, double* out, hipStream_t s);
+
+// Y-Z dose plane (nyz doubles): per node, the Eq.24 dose weights of `coef`
+// applied to psi^1 and psi^2, summed over groups and directions, times scale.
+template <typename T>
+void launch_spot_plane(const T* F, const T* f, const T* F1, const T* f1, const EnergyCoef* coef,
+                       int Ng, int nyz, int nang, double scale, double* plane, hipStream_t s);
+
 }  // namespace bfpn

@@ -339,6 +339,35 @@ public:
 
     int steps_done() const override { return step_; }
 
+    // Eq.25 moments of the current state (computeEnergyMoments,
+    // src/bfpm_solver.cu:3504-3526): [g][psi^1, psi^2] x du dv dy dz.
+    void energy_moments(double* host) override {
+        sync_all();
+        const long long per = static_cast<long long>(P_->lanes);
+        diag_part_.ensure(moment_partial_count(grid.Ng, per) * sizeof(double));
+        diag_out_.ensure(2 * static_cast<size_t>(grid.Ng) * sizeof(double));
+        launch_energy_moments<T>(cur_[0], cur_[1], cur_[2], cur_[3], grid.Ng, per,
+                                 d.du * d.dv * d.dy * d.dz, diag_part_.as<double>(),
+                                 diag_out_.as<double>(), stream);
+        BFPN_HIP(hipStreamSynchronize(stream));
+        BFPN_HIP(hipMemcpy(host, diag_out_.as<double>(), 2 * static_cast<size_t>(grid.Ng) * sizeof(double),
+                           hipMemcpyDeviceToHost));
+    }
+
+    // Y-Z dose plane of the current state (computeSpotDosePlane,
+    // src/bfpm_solver.cu:3560-3585): node s = iz (Ny+1) + iy.
+    void spot_plane(double* host) override {
+        sync_all();
+        const int nyz = static_cast<int>(P_->nyz);
+        diag_out_.ensure(static_cast<size_t>(nyz) * sizeof(double));
+        launch_spot_plane<T>(cur_[0], cur_[1], cur_[2], cur_[3], P_->coef.template as<EnergyCoef>(),
+                             grid.Ng, nyz, P_->nang, d.du * d.dv / phys.density,
+                             diag_out_.as<double>(), stream);
+        BFPN_HIP(hipStreamSynchronize(stream));
+        BFPN_HIP(hipMemcpy(host, diag_out_.as<double>(), static_cast<size_t>(nyz) * sizeof(double),
+                           hipMemcpyDeviceToHost));
+    }
+
     void advance(int nsteps) override {
         BFPN_REQUIRE(nsteps >= 0, "nsteps must be >= 0");
         if (nsteps == 0) return;
@@ -610,6 +639,7 @@ private:
 
     std::shared_ptr<Physics<T>> P_;
     DeviceBuffer state_[4], a1_, a2_, q_[2], x_[2];
+    DeviceBuffer diag_part_, diag_out_;  // diagnostics scratch, allocated on first use
     DeviceBuffer partial_p_, partial_s_, hist_;
     T* cur_[4] = {nullptr, nullptr, nullptr, nullptr};
     int step_ = 0;
@@ -1022,6 +1052,23 @@ BFPN_API bfpn_status bfpn_read_state(bfpn_ctx* ctx, int which, double* host, siz
     return guarded([&] {
         BFPN_REQUIRE(ctx && host, "null argument");
         ctx->read_state(which, host, count);
+    });
+}
+
+BFPN_API bfpn_status bfpn_energy_moments(bfpn_ctx* ctx, double* moments, int capacity) {
+    return guarded([&] {
+        BFPN_REQUIRE(ctx && moments, "null argument");
+        BFPN_REQUIRE(capacity >= 2 * ctx->grid.Ng, "energy moments need 2*Ng values");
+        ctx->energy_moments(moments);
+    });
+}
+
+BFPN_API bfpn_status bfpn_spot_plane(bfpn_ctx* ctx, double* plane, size_t count) {
+    return guarded([&] {
+        BFPN_REQUIRE(ctx && plane, "null argument");
+        BFPN_REQUIRE(count == static_cast<size_t>(ctx->grid.Ny + 1) * (ctx->grid.Nz + 1),
+                     "spot plane needs (Ny+1)*(Nz+1) values");
+        ctx->spot_plane(plane);
     });
 }
 

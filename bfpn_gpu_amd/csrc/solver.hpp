@@ -54,6 +54,8 @@ struct bfpn_ctx {
     virtual void write_state(int which, const double* host, size_t count) = 0;
     virtual size_t device_bytes() const = 0;
     virtual int steps_done() const = 0;
+    virtual void energy_moments(double* host) = 0;  // 2 Ng values
+    virtual void spot_plane(double* host) = 0;      // (Ny+1)(Nz+1) values
 
     bfpn_grid grid{};
     bfpn_phys phys{};

@@ -42,6 +42,9 @@ namespace {
 template <typename T>
 constexpr int max_threads() { return sizeof(T) == 4 ? 1024 : 768; }
 
+// LDS slots per y column (14 used, one pad: an odd pitch in units of MW)
+constexpr int kSlots = 15;
+
 // Superbee slope of two differences (either order), without the 1e-10 test:
 //   ab > 0: sign(a) min(max(|a|, |b|), 2 min(|a|, |b|)),  else 0,
 // which is the median of a, b and 2 m, with m = median(a, b, 0) the minmod
@@ -70,25 +73,17 @@ __device__ inline T limited_slope(T a, T b) {
 //   w (F(+1/2) - F(-1/2)) / d = (|w| / d) (downwind face - upwind face),
 // where the downwind face is the cell's own reconstruction c + s_down/2 and
 // the upwind face the upwind neighbour's, n + s_up/2, so the difference is
-// (c - n) + (s_down - s_up)/2.  y: near = upwind neighbour, far = the one
-// beyond it, down = downwind neighbour (LDS row); with A = down - c,
-// B = c - near, C = near - far the two slopes are (A, B) and (B, C), the
-// 1e-10 test going to whichever is forward in +y.  z: physical window
-// (zm, zc, zp) and carried physical slopes of rows -1, 0, +1; hz = +-1/2
+// (c - n) + (s_down - s_up)/2.  y: from the physical limited slopes of the
+// cell (S_c) and of its upwind neighbour (S_n), the upwind frame's slopes are
+// +-S (superbee is odd and symmetric, and the 1e-10 test goes to the
+// physical forward difference either way), so the y term is
+// (c - n) + hy (S_c - S_n) with hy = +-1/2 the sign of wy.  z: physical
+// window (zm, zc, zp) and carried physical slopes of rows -1, 0, +1; hz = +-1/2
 // with the sign of wz.  cy = |wy| / dy, cz = |wz| / dz.
-// (near, far, down): the y neighbours, read from an LDS ring row.
 template <typename T>
-struct Nbrs {
-    T near, far, down;
-};
-
-template <typename T>
-__device__ inline T divergence(const Nbrs<T>& n, T zm, T zc, T zp, T sz_m, T sz_0, T sz_p, T cy,
-                               T cz, T hz, bool py, bool pz) {
-    const T A = n.down - zc, B = zc - n.near, C = n.near - n.far;
-    const T s_down = fabs(py ? A : B) < T(1e-10) ? T(0) : superbee(A, B);
-    const T s_up = fabs(py ? B : C) < T(1e-10) ? T(0) : superbee(B, C);
-    const T fy = fma(T(0.5), s_down - s_up, B);
+__device__ inline T divergence(T zc, T near, T S_c, T S_n, T zm, T zp, T sz_m, T sz_0, T sz_p,
+                               T cy, T cz, T hy, T hz, bool pz) {
+    const T fy = fma(hy, S_c - S_n, zc - near);
     const T fz = fma(hz, sz_0 - (pz ? sz_m : sz_p), zc - (pz ? zm : zp));
     return cy * fy + cz * fz;
 }
@@ -105,10 +100,16 @@ __device__ inline const T& at(const T* base, unsigned byte_off) {
 }
 
 // grid: (ceil(nang / MW), Ng), block: MW * R threads (R y columns per pass,
-// P passes: the thread's columns are y = ry + j R).  LDS: two rings of 4
-// rows (psi, psi*), stored y-major: column y holds its 8 ring slots of MW
-// values side by side, padded to 9 MW so that the y columns of one wave fall
-// on distinct banks; a slot is then a constant offset from the column.
+// P passes: the thread's columns are y = ry + j R).  LDS, stored y-major
+// (column y holds its slots of MW values side by side, pitch 15 MW, odd, so
+// the y columns of one wave fall on distinct banks; a slot is then a constant
+// offset from the column):
+//   slots 0-3   psi rows (row mod 4)        slots 8-11  y slopes of psi rows
+//   slots 4-7   psi* rows (row mod 4)       slots 12-13 y slopes of psi* rows
+// The physical y slope of a row is computed once per cell, two steps before
+// the stage that needs it (after the barrier that publishes the row), kept
+// in a register by its own lane and published in LDS for the downwind
+// neighbour, which reads it instead of recomputing it.
 //
 // The z march is unrolled by the ring period 4: the register windows are
 // indexed by row mod 4 (slopes by row mod 2), so with the step's phase k a
@@ -128,9 +129,9 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
     if (L >= tiles) return;
     const int mchunks = (nang + MW - 1) / MW;
     const int g = L / mchunks;
-    const int Ny1 = Ny + 1;
-    T* ring = reinterpret_cast<T*>(lds_raw);  // [y][slot 0-3 psi, 4-7 psi*][MW], pitch 9 MW
-    constexpr int YP = 9 * MW;
+    T* ring = reinterpret_cast<T*>(lds_raw);
+    constexpr int YP = kSlots * MW;
+    constexpr int kPsi = 0, kPred = 4, kSPsi = 8, kSPred = 12;  // first slot of each ring
 
     const int mm = threadIdx.x % MW, ry = threadIdx.x / MW;
     const int m0 = (L - g * mchunks) * MW;
@@ -141,37 +142,40 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
     const int m = live ? m0 + mm : nang - 1;
     const T wy = omend[2 * m], wz = omend[2 * m + 1];
     const bool py = wy >= T(0), pz = wz >= T(0);
-    const T cy = fabs(wy) * inv_dy, cz = fabs(wz) * inv_dz, hz = pz ? T(0.5) : T(-0.5);
+    const T cy = fabs(wy) * inv_dy, cz = fabs(wz) * inv_dz;
+    const T hy = py ? T(0.5) : T(-0.5), hz = pz ? T(0.5) : T(-0.5);
     const T hh = T(0.5) * h;
-    const long long zstride = static_cast<long long>(Ny1) * nang;
+    const long long zstride = static_cast<long long>(Ny + 1) * nang;
     // work-group base (uniform); lanes address it with 32-bit offsets
     T* Fg = F + static_cast<long long>(g) * zstride * (Nz + 1) + m0;
 
     bool on[P];
     unsigned boff[P];  // byte offset of the lane's column in a plane row
-    int o0[P], o_near[P], o_far[P], o_down[P];
+    int o0[P], o_near[P], o_m[P], o_p[P];  // LDS column: own, upwind, y-1, y+1 (clamped)
     T wpsi[P][4];   // psi, rows t-1 .. t+2 after the commit of step t
     T wpred[P][4];  // psi*, rows t-3 .. t after the predictor of step t
     T wbase[P][4];  // psi - h/2 L(psi), rows t-2 .. t
+    T wS[P][4];     // own y slope of psi, rows t .. t+2 after step t's slope pass
+    T wSp[P][2];    // own y slope of psi*, rows t-2, t-1 after step t's slope pass
     T sz[P][2];     // z slopes of psi, rows t-1 and t before step t
     T szp[P][2];    // z slopes of psi*, rows t-3 and t-2 before step t
     T nxt[P][4];    // psi rows t+2 .. t+5 in flight (slot: row mod 4)
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         const int y = ry + j * R;
-        on[j] = y < Ny1;
+        on[j] = y < Ny + 1;
         const int yc = min(y, Ny);
         boff[j] = static_cast<unsigned>((yc * nang + (m - m0)) * sizeof(T));
         auto lo = [&](int yy) { return (min(max(yy, 0), Ny)) * YP + mm; };
         o0[j] = lo(yc);
         o_near[j] = lo(py ? yc - 1 : yc + 1);
-        o_far[j] = lo(py ? yc - 2 : yc + 2);
-        o_down[j] = lo(py ? yc + 1 : yc - 1);
+        o_m[j] = lo(yc - 1);
+        o_p[j] = lo(yc + 1);
         const T r0 = at(Fg, boff[j]);
         const T r1 = at(Fg + zstride, boff[j]);  // Nz >= 1
         if (on[j]) {
-            ring[o0[j]] = r0;
-            ring[o0[j] + MW] = r1;
+            ring[o0[j] + kPsi * MW] = r0;
+            ring[o0[j] + (kPsi + 1) * MW] = r1;
         }
         wpsi[j][3] = wpsi[j][0] = r0;  // row -1 clamps to row 0
         wpsi[j][1] = r1;
@@ -180,28 +184,42 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
         for (int r = 0; r < 4; ++r)  // rows past Nz: never used
             nxt[j][(r + 2) & 3] = at(Fg + min(r + 2, Nz) * zstride, boff[j]);
 #pragma unroll
-        for (int k = 0; k < 2; ++k) sz[j][k] = szp[j][k] = T(0);  // boundary slopes are 0
+        for (int k = 0; k < 2; ++k) sz[j][k] = szp[j][k] = wSp[j][k] = T(0);  // boundary: 0
 #pragma unroll
-        for (int k = 0; k < 4; ++k) wpred[j][k] = wbase[j][k] = T(0);
+        for (int k = 0; k < 4; ++k) wpred[j][k] = wbase[j][k] = wS[j][k] = T(0);
+    }
+    __syncthreads();
+    // y slopes of psi rows 0 and 1 (later rows: two steps ahead, in the march)
+#pragma unroll
+    for (int j = 0; j < P; ++j) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const T* row = ring + (kPsi + r) * MW;
+            const T S = limited_slope(row[o_p[j]] - wpsi[j][r], wpsi[j][r] - row[o_m[j]]);
+            wS[j][r] = S;
+            if (on[j]) ring[o0[j] + (kSPsi + r) * MW] = S;
+        }
     }
 
-    // Step t (phase k = t mod 4): commit psi row t+2, predictor on row t,
-    // corrector on row t-2.  Every step runs the same branch-free code; near
-    // the ends of the march (EDGE) selects apply the clamped rows and the
-    // boundary slopes, and only the stores are predicated.  Past the last
-    // row the predictor works on clamped copies and its result is replaced
-    // by the clamp; steps past Nz+2 (padding the last block of 4) store
-    // nothing.
+    // Step t (phase k = t mod 4): commit psi row t+2; slopes of psi row t+2
+    // and psi* row t-1; predictor on row t; corrector on row t-2.  Every step
+    // runs the same branch-free code; near the ends of the march (EDGE)
+    // selects apply the clamped rows and the boundary slopes, and only the
+    // stores are predicated.  Past the last row the predictor works on
+    // clamped copies and its result is replaced by the clamp; steps past
+    // Nz+2 (padding the last block of 4) store nothing.
     auto step = [&](int t, auto phase, auto edge) {
         constexpr int k = decltype(phase)::value;
         constexpr bool E = decltype(edge)::value;
         const int rc = t + 2;  // commit psi row t+2 into the slot of row t-2
-        T* crow = ring + ((k + 2) & 3) * MW;
+        {
+            T* crow = ring + (kPsi + ((k + 2) & 3)) * MW;
 #pragma unroll
-        for (int j = 0; j < P; ++j) {
-            const T v = (!E || rc <= Nz) ? nxt[j][(k + 2) & 3] : wpsi[j][(k + 1) & 3];
-            wpsi[j][(k + 2) & 3] = v;
-            if (on[j]) crow[o0[j]] = v;
+            for (int j = 0; j < P; ++j) {
+                const T v = (!E || rc <= Nz) ? nxt[j][(k + 2) & 3] : wpsi[j][(k + 1) & 3];
+                wpsi[j][(k + 2) & 3] = v;
+                if (on[j]) crow[o0[j]] = v;
+            }
         }
         __syncthreads();
         {
@@ -210,27 +228,53 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
 #pragma unroll
             for (int j = 0; j < P; ++j) nxt[j][(k + 2) & 3] = at(src, boff[j]);
         }
-        // y neighbours of both stages, read together so the corrector's LDS
-        // round trip overlaps the predictor (psi* row t-2 was written two
-        // steps ago; the predictor below writes another slot)
-        Nbrs<T> np_[P], nc_[P];
+        // LDS reads of the step, issued together: y neighbours of psi row t+2
+        // and psi* row t-1 (their slopes), upwind value and slope of psi row t
+        // (predictor) and psi* row t-2 (corrector)
+        T a_m[P], a_p[P], b_m[P], b_p[P], pn[P], pS[P], cn_[P], cS[P];
         {
-            const T* prd = ring + k * MW;                   // psi row t
-            const T* crd = ring + (4 + ((k + 2) & 3)) * MW;  // psi* row t-2
+            const T* psi2 = ring + (kPsi + ((k + 2) & 3)) * MW;    // psi row t+2
+            const T* prd1 = ring + (kPred + ((k + 3) & 3)) * MW;   // psi* row t-1
+            const T* psi0 = ring + (kPsi + k) * MW;                // psi row t
+            const T* sps0 = ring + (kSPsi + k) * MW;               // its slopes
+            const T* prd2 = ring + (kPred + ((k + 2) & 3)) * MW;   // psi* row t-2
+            const T* spd2 = ring + (kSPred + (k & 1)) * MW;        // its slopes
 #pragma unroll
             for (int j = 0; j < P; ++j) {
-                np_[j] = Nbrs<T>{prd[o_near[j]], prd[o_far[j]], prd[o_down[j]]};
-                nc_[j] = Nbrs<T>{crd[o_near[j]], crd[o_far[j]], crd[o_down[j]]};
+                a_m[j] = psi2[o_m[j]];
+                a_p[j] = psi2[o_p[j]];
+                b_m[j] = prd1[o_m[j]];
+                b_p[j] = prd1[o_p[j]];
+                pn[j] = psi0[o_near[j]];
+                pS[j] = sps0[o_near[j]];
+                cn_[j] = prd2[o_near[j]];
+                cS[j] = spd2[o_near[j]];
+            }
+        }
+        {  // y slopes of psi row t+2 and psi* row t-1, published for step t+2
+            T* sp2 = ring + (kSPsi + ((k + 2) & 3)) * MW;
+            T* sd1 = ring + (kSPred + ((k + 1) & 1)) * MW;
+#pragma unroll
+            for (int j = 0; j < P; ++j) {
+                const T c2 = wpsi[j][(k + 2) & 3], c1 = wpred[j][(k + 3) & 3];
+                const T S2 = limited_slope(a_p[j] - c2, c2 - a_m[j]);
+                const T S1 = limited_slope(b_p[j] - c1, c1 - b_m[j]);
+                wS[j][(k + 2) & 3] = S2;
+                wSp[j][(k + 1) & 1] = S1;
+                if (on[j]) {
+                    sp2[o0[j]] = S2;
+                    sd1[o0[j]] = S1;
+                }
             }
         }
         {  // predictor on row t
-            T* prow = ring + (4 + k) * MW;
+            T* prow = ring + (kPred + k) * MW;
 #pragma unroll
             for (int j = 0; j < P; ++j) {
                 const T zm = wpsi[j][(k + 3) & 3], zc = wpsi[j][k], zp = wpsi[j][(k + 1) & 3];
                 const T s_next = limited_slope(wpsi[j][(k + 2) & 3] - zp, zp - zc);
-                const T L0 = divergence(np_[j], zm, zc, zp, sz[j][(k + 1) & 1], sz[j][k & 1],
-                                        s_next, cy, cz, hz, py, pz);
+                const T L0 = divergence(zc, pn[j], wS[j][k], pS[j], zm, zp, sz[j][(k + 1) & 1],
+                                        sz[j][k & 1], s_next, cy, cz, hy, hz, pz);
                 T v = zc - h * L0;
                 if (CLIP) v = fmax(v, T(0));
                 if (on[j]) prow[o0[j]] = v;
@@ -248,8 +292,9 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
                 const T zm = wpred[j][(k + 1) & 3], zc = wpred[j][(k + 2) & 3],
                         zp = wpred[j][(k + 3) & 3];
                 const T s_next = limited_slope(wpred[j][k] - zp, zp - zc);
-                const T L1 = divergence(nc_[j], zm, zc, zp, szp[j][(k + 1) & 1], szp[j][k & 1],
-                                        s_next, cy, cz, hz, py, pz);
+                const T L1 = divergence(zc, cn_[j], wSp[j][k & 1], cS[j], zm, zp,
+                                        szp[j][(k + 1) & 1], szp[j][k & 1], s_next, cy, cz, hy,
+                                        hz, pz);
                 T v = wbase[j][(k + 2) & 3] - hh * L1;
                 if (CLIP) v = fmax(v, T(0));
                 if (real && on[j] && live) at(dst, boff[j]) = v;
@@ -274,19 +319,19 @@ struct MarchShape {
     int MW, P, R, threads;
 };
 
-// Launch shape, the first of (MW, P) = (16, 1), (16, 2), (8, 2), (8, 3),
-// (8, 4) whose work-group covers the plane and whose two LDS rings fit the
+// Launch shape, the first of (MW, P) = (16, 1), (16, 2), (8, 2), (8, 3)
+// whose work-group covers the plane and whose LDS rings fit the
 // 160 KB of a CU.  Measured on MI355X (FP32, 400 directions): 16 directions
 // per work-group beat 32 (smaller groups pack the CU's wave slots better:
 // 235 vs 248 us at 21 x 21 nodes) and 8 (64-byte row segments: 3.41 vs
 // 3.57 ms at 81 x 81 nodes with two columns per thread).
 MarchShape march_shape(int Ny1, size_t elem) {
     const int max_t = elem == 4 ? max_threads<float>() : max_threads<double>();
-    constexpr int cand[5][2] = {{16, 1}, {16, 2}, {8, 2}, {8, 3}, {8, 4}};
+    constexpr int cand[4][2] = {{16, 1}, {16, 2}, {8, 2}, {8, 3}};
     for (const auto& c : cand) {
         const int MW = c[0], P = c[1];
         if (static_cast<long long>(max_t / MW) * P < Ny1) continue;
-        if (9 * static_cast<size_t>(Ny1) * MW * elem > 160 * 1024) continue;
+        if (static_cast<size_t>(kSlots) * Ny1 * MW * elem > 160 * 1024) continue;
         const int R0 = (Ny1 + P - 1) / P;
         const int threads = (MW * R0 + 63) / 64 * 64;
         return MarchShape{MW, P, threads / MW, threads};
@@ -297,7 +342,7 @@ MarchShape march_shape(int Ny1, size_t elem) {
 template <typename T, int MW, int P>
 void launch_march(T* F, const T* omend, int Ny, int Nz, int Ng, int nang, const MarchShape& sh,
                   double dy, double dz, double h, bool preserve_sign, hipStream_t s) {
-    const size_t lds = sizeof(T) * 9 * static_cast<size_t>(Ny + 1) * MW;
+    const size_t lds = sizeof(T) * kSlots * static_cast<size_t>(Ny + 1) * MW;
     BFPN_REQUIRE(lds <= 160 * 1024, "transport plane too wide for the LDS rings");
     const long long tiles = static_cast<long long>((nang + MW - 1) / MW) * Ng;
     BFPN_REQUIRE(tiles < (1LL << 31) - 8, "transport grid too large");
@@ -325,8 +370,9 @@ void launch_transport_half(T* F, const T* omend, int Ny, int Nz, int Ng, int nan
     if (Ng <= 0 || nang <= 0) return;
     BFPN_REQUIRE(Ny >= 1 && Nz >= 1, "transport needs Ny, Nz >= 1");
     const MarchShape sh = march_shape(Ny + 1, sizeof(T));
-    BFPN_REQUIRE(sh.MW > 0, sizeof(T) == 4 ? "transport supports at most 512 y nodes in FP32"
-                                           : "transport supports at most 284 y nodes in FP64");
+    // the LDS holds 15 slots of MW values per y column: 15 * (Ny+1) * 8 * elem <= 160 KB
+    BFPN_REQUIRE(sh.MW > 0, sizeof(T) == 4 ? "transport supports at most 341 y nodes in FP32"
+                                           : "transport supports at most 170 y nodes in FP64");
 #define BFPN_MARCH(MW_, P_)                                                                    \
     if (sh.MW == MW_ && sh.P == P_)                                                            \
         return launch_march<T, MW_, P_>(F, omend, Ny, Nz, Ng, nang, sh, dy, dz, h, preserve_sign, \
@@ -335,7 +381,6 @@ void launch_transport_half(T* F, const T* omend, int Ny, int Nz, int Ng, int nan
     BFPN_MARCH(16, 2);
     BFPN_MARCH(8, 2);
     BFPN_MARCH(8, 3);
-    BFPN_MARCH(8, 4);
 #undef BFPN_MARCH
     throw Error(BFPN_E_INVALID, "no transport launch shape for this plane");
 }

@@ -250,6 +250,19 @@ BFPN_API bfpn_status bfpn_read_state(bfpn_ctx* ctx, int which, double* host,
 BFPN_API bfpn_status bfpn_write_state(bfpn_ctx* ctx, int which,
                                       const double* host, size_t count);
 
+/* Paper Eq.25 energy moments of the current state: moments[2g] = du dv dy dz
+ * sum over lanes of (F + F1) at group g, moments[2g+1] the same for (f + f1);
+ * capacity >= 2*Ng.  Replaces BFPnSolver::computeEnergyMoments
+ * (src/bfpm_solver.cu:3504-3526, energyMomentKernel 311-360). */
+BFPN_API bfpn_status bfpn_energy_moments(bfpn_ctx* ctx, double* moments, int capacity);
+
+/* Y-Z dose plane of the current state: per transverse node s = iz*(Ny+1)+iy,
+ * the Eq.24 cell dose of psi^1 = F + F1 and psi^2 = f + f1 summed over
+ * groups and directions, times du*dv/density; count = (Ny+1)*(Nz+1).
+ * Replaces BFPnSolver::computeSpotDosePlane (src/bfpm_solver.cu:3560-3585,
+ * spotDosePlaneKernel 433-475). */
+BFPN_API bfpn_status bfpn_spot_plane(bfpn_ctx* ctx, double* plane, size_t count);
+
 /* The HIP stream (hipStream_t) on which the context's issued work ends, so
  * callers can order their own work: work enqueued there runs after every
  * step issued so far, and steps issued later start after it. */

@@ -122,18 +122,20 @@ def test_transport_random_field_large_velocity(gpu, oracle_mod, dtype):
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("Ny,Nz,nang", [(1, 1, 3), (2, 5, 40), (20, 3, 33), (31, 4, 7),
                                         (40, 2, 17), (63, 2, 5), (64, 3, 9), (80, 2, 16),
-                                        (127, 1, 3), (128, 2, 2), (200, 1, 2), (283, 6, 3),
-                                        (400, 1, 2)])
+                                        (127, 1, 3), (128, 2, 2), (169, 1, 2), (200, 1, 2),
+                                        (283, 6, 3), (340, 1, 2), (400, 1, 2)])
 def test_transport_plane_shapes(gpu, oracle_mod, Ny, Nz, nang, dtype):
-    """Every launch shape (32/16/8 directions per work-group, 1-4 y columns per
+    """Every launch shape (16/8 directions per work-group, 1-3 y columns per
     thread; z marches of 2..7 rows, i.e. edge-only and mixed unrolled blocks),
     plateaus with exact ties (flat-slope branches), both clip modes.  Planes
-    wider than the LDS rings allow (FP64 above 284 y nodes) are an error."""
+    wider than the LDS rings allow (above 341 y nodes in FP32, 170 in FP64)
+    are an error."""
     Ng = 2
-    if dtype == np.float64 and Ny + 1 > 284:
+    limit = 341 if dtype == np.float32 else 170
+    if Ny + 1 > limit:
         dF = _dev(gpu, np.zeros(Ng * (Ny + 1) * (Nz + 1) * nang), dtype)
         om = _dev(gpu, RNG.uniform(-0.9, 0.9, 2 * nang), np.float64)
-        with pytest.raises(gpu.BfpnError, match="at most 284 y nodes"):
+        with pytest.raises(gpu.BfpnError, match=f"at most {limit} y nodes"):
             gpu.ops.transport_half(dF, om, Ny, Nz, Ng, nang, 0.2, 0.3, 0.05, True)
         return
     F = RNG.standard_normal(Ng * (Ny + 1) * (Nz + 1) * nang)

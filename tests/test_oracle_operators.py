@@ -29,6 +29,11 @@ source (src/energy_solver.cu:287-333)
     non-zero only for source groups above the output group.
 IDD (src/bfpm_solver.cu:229-309)
   * one cell, one group: closed form with trapezoid weights.
+energy moments and spot dose plane (src/bfpm_solver.cu:311-360, 433-475)
+  * weighted by the per-group dose weights, the Eq.25 moments integrate to
+    the IDD without y-z weights; with the trapezoid weights, the Y-Z dose
+    plane integrates to the IDD (both on random states of all four arrays,
+    negative cross sections included); one cell of the plane in closed form.
 beam (src/bfpm_solver.cu:1815-2025)
   * --normalize-initial-mass gives unit discrete mass.
 """
@@ -256,6 +261,40 @@ def test_idd_single_cell_closed_form(oracle_mod):
                         trapezoidal=False, catastrophic=True)
     dose2 = 0.5 * (S[2] + S[3]) * dg + 0.5 * (en[2] + en[3]) * 0.3 * dg
     assert v2 == pytest.approx(dose2 / rho * du * dv * dy * dz, rel=1e-14)
+
+
+def test_moments_and_spot_plane_integrate_to_the_idd(oracle_mod):
+    Ng, Ny, Nz, nang = 5, 3, 2, 6
+    nyz = (Ny + 1) * (Nz + 1)
+    n = Ng * nyz * nang
+    F, f, F1, f1 = (RNG.standard_normal(n) for _ in range(4))
+    S = 1.0 + RNG.random(Ng + 1)
+    sig = RNG.uniform(-0.2, 0.5, Ng)
+    en = np.cumsum(0.5 + RNG.random(Ng + 1))
+    dg, du, dv, dy, dz, rho = 0.7, 0.1, 0.2, 0.3, 0.4, 1.3
+    m = oracle_mod.energy_moments(F, f, F1, f1, Ng, nyz, nang, du, dv, dy, dz)
+    sp = np.maximum(sig, 0.0)
+    w1 = (0.5 * (S[:-1] + S[1:]) + 0.5 * (en[:-1] + en[1:]) * sp) * dg
+    w2 = ((S[1:] - S[:-1]) / 6 + (en[1:] - en[:-1]) * sp / 6) * dg
+    idd_plain = oracle_mod.idd(F, f, F1, f1, S, sig, en, Ng, Ny, Nz, nang, dg, du, dv, dy, dz, rho,
+                               trapezoidal=False, catastrophic=True)
+    assert (m[:, 0] @ w1 + m[:, 1] @ w2) / rho == pytest.approx(idd_plain, rel=1e-12)
+    # moments are plain sums of psi^1 = F + F1 and psi^2 = f + f1 per group
+    per = (F + F1).reshape(Ng, -1).sum(1) * du * dv * dy * dz
+    np.testing.assert_allclose(m[:, 0], per, rtol=1e-12)
+    plane = oracle_mod.spot_plane(F, f, F1, f1, S, sig, en, Ng, nyz, nang, dg, du, dv, rho)
+    iy, iz = np.arange(nyz) % (Ny + 1), np.arange(nyz) // (Ny + 1)
+    wyz = np.where((iy == 0) | (iy == Ny), 0.5, 1.0) * np.where((iz == 0) | (iz == Nz), 0.5, 1.0)
+    idd_trap = oracle_mod.idd(F, f, F1, f1, S, sig, en, Ng, Ny, Nz, nang, dg, du, dv, dy, dz, rho,
+                              trapezoidal=True, catastrophic=True)
+    assert plane @ wyz * dy * dz == pytest.approx(idd_trap, rel=1e-12)
+    # one cell: psi^1 only, group 2, node 5, catastrophic loss off
+    G = np.zeros(n)
+    G[(2 * nyz + 5) * nang + 3] = 1.5
+    p1 = oracle_mod.spot_plane(G, None, None, None, S, sig, en, Ng, nyz, nang, dg, du, dv, rho,
+                               catastrophic=False)
+    assert np.count_nonzero(p1) == 1
+    assert p1[5] == pytest.approx(1.5 * 0.5 * (S[2] + S[3]) * dg / rho * du * dv, rel=1e-14)
 
 
 def test_beam_mass_normalisation(oracle_mod):
