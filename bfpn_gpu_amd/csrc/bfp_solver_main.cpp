@@ -6,7 +6,12 @@
 // failure, and the same output files in the working directory:
 //   idd_output.txt   "# x_cm value" then (i+1)*dt and the IDD, 17 digits
 //   dose_output.txt  same format, dg * sum psi^2 (primary)
-// (BFPnSolver::saveResults, src/bfpm_solver.cu:3808-3815).
+// (BFPnSolver::saveResults, src/bfpm_solver.cu:3808-3815), and on request
+//   energy_moments.txt       --save-energy-moments: Eq.25 moments per step and
+//                            group (saveEnergyMoments, 3817-3835)
+//   <prefix>_reqD_xX.txt     --spot-depths D1,D2,..: Y-Z dose plane after the
+//                            first step whose depth reaches D - dt/2
+//                            (saveSpotPlane, 3587-3612; solve loop 2244-2249)
 //
 // Opt-in additions: --precision fp32|fp64, --lg <MeV span>.
 // The out-of-core streaming flags are accepted and run in core (the whole
@@ -52,6 +57,56 @@ void save_series(const std::vector<double>& v, double dt, const std::string& pat
     for (size_t i = 0; i < v.size(); ++i) out << (i + 1) * dt << " " << v[i] << "\n";
 }
 
+// Comma-separated depths, empty items skipped (parseDepthList, src/main.cu:12-22).
+std::vector<double> parse_depth_list(const std::string& text) {
+    std::vector<double> values;
+    std::stringstream ss(text);
+    std::string item;
+    while (std::getline(ss, item, ','))
+        if (!item.empty()) values.push_back(std::stod(item));
+    return values;
+}
+
+// BFPnSolver::saveSpotPlane (src/bfpm_solver.cu:3587-3612): file name and
+// layout as the reference writes them (default stream precision).
+void save_spot_plane(const std::vector<double>& plane, const std::string& prefix, double requested,
+                     double actual, int ny, int nz, double ly, double lz, double dy, double dz) {
+    if (prefix.empty()) return;
+    std::ostringstream name;
+    name << prefix << "_req" << std::fixed << std::setprecision(2) << requested << "_x"
+         << std::fixed << std::setprecision(2) << actual << ".txt";
+    std::ofstream out(name.str());
+    out << "# requested_x_cm " << requested << "\n";
+    out << "# actual_x_cm " << actual << "\n";
+    out << "# y_cm z_cm dose\n";
+    for (int j = 0; j <= nz; ++j) {
+        const double z = -0.5 * lz + j * dz;
+        for (int i = 0; i <= ny; ++i) {
+            const double y = -0.5 * ly + i * dy;
+            out << y << " " << z << " " << plane[static_cast<size_t>(j) * (ny + 1) + i] << "\n";
+        }
+        out << "\n";
+    }
+    std::cout << "Spot dose plane saved to " << name.str() << std::endl;
+}
+
+// BFPnSolver::saveEnergyMoments (src/bfpm_solver.cu:3817-3835).
+void save_energy_moments(const std::vector<double>& values, int ng, double dt,
+                         const std::string& path) {
+    std::ofstream out(path);
+    out << "# step x_cm g psi1_integral psi2_integral\n";
+    const size_t per_step = static_cast<size_t>(ng) * 2;
+    if (per_step == 0) return;
+    const size_t steps = values.size() / per_step;
+    for (size_t step = 0; step < steps; ++step) {
+        const double x = (step + 1) * dt;
+        const size_t base = step * per_step;
+        for (int g = 0; g < ng; ++g)
+            out << (step + 1) << " " << x << " " << g << " " << values[base + 2 * g] << " "
+                << values[base + 2 * g + 1] << "\n";
+    }
+}
+
 void usage(const char* argv0) {
     std::cout << "Usage: " << argv0 << " [options]\n"
               << "Options:\n"
@@ -75,6 +130,9 @@ void usage(const char* argv0) {
               << "  --no-angle       Skip the angular diffusion subsystem (primary)\n"
               << "  --no-spatial-clipping Disable positivity clipping of primary psi^1\n"
               << "  --profile-steps  Print energy/transport/angle step timing summary\n"
+              << "  --save-energy-moments Save paper Eq.25 energy-moment diagnostics\n"
+              << "  --spot-depths <x,...> Save YZ dose planes near requested depths\n"
+              << "  --spot-prefix <path> Prefix for spot plane files (default: spot)\n"
               << "  --discrete-angle-delta Quadrature-normalized mono-directional beam\n"
               << "  --normalize-initial-mass Normalize the discrete beam mass\n"
               << "  --trapezoidal-yz Trapezoidal y/z weights in normalization and IDD\n"
@@ -103,6 +161,8 @@ int main(int argc, char** argv) {
     double energy = 230.0, sigma_e = 1.0, sigma_yz = 0.1, rho = 1.0, rho_e = 1.0;
     bool lite = false, streaming = false, legacy_energy = false, profile = false;
     int precision = BFPN_FP32;
+    std::vector<double> spot_depths;
+    std::string spot_prefix = "spot";
     bfpn_phys phys{};
     phys.dose_catastrophic_loss = 1;
 
@@ -169,9 +229,11 @@ int main(int argc, char** argv) {
                     std::cerr << "Unknown precision: " << p << std::endl;
                     return 1;
                 }
-            } else if ((a == "--lane-chunk" || a == "--energy-chunk" || a == "--stream-dir" ||
-                        a == "--idd-stride" || a == "--spot-depths" || a == "--spot-prefix") && has)
-                ++i;  // streaming / diagnostics knobs: accepted, no effect in core
+            } else if (a == "--spot-depths" && has) spot_depths = parse_depth_list(argv[++i]);
+            else if (a == "--spot-prefix" && has) spot_prefix = argv[++i];
+            else if ((a == "--lane-chunk" || a == "--energy-chunk" || a == "--stream-dir" ||
+                      a == "--idd-stride") && has)
+                ++i;  // streaming knobs: accepted, no effect in core
             else if (a == "--help") {
                 usage(argv[0]);
                 return 0;
@@ -248,17 +310,50 @@ int main(int argc, char** argv) {
     if (profile) bfpn_set_profiling(ctx, 1);
 
     std::vector<double> idd(steps), proxy(steps);
+    // depth after each step, accumulated like the reference's t += dt
+    std::vector<double> depth(static_cast<size_t>(steps) + 1, 0.0);
+    for (int k = 1; k <= steps; ++k) depth[k] = depth[k - 1] + d.dt;
+    // a spot plane is taken after the first step whose depth + dt/2 reaches
+    // the requested depth (src/bfpm_solver.cu:2244-2249)
+    std::vector<int> spot_step(spot_depths.size(), -1);
+    for (size_t i = 0; i < spot_depths.size(); ++i)
+        for (int k = 1; k <= steps; ++k)
+            if (depth[k] + 0.5 * d.dt >= spot_depths[i]) {
+                spot_step[i] = k;
+                break;
+            }
+    const bool moments = phys.save_energy_moments != 0;
+    std::vector<double> moment_history;
+    std::vector<double> plane(static_cast<size_t>(ny + 1) * (nz + 1));
+    auto fail = [&] {
+        std::cerr << "Error: " << bfpn_last_error() << std::endl;
+        bfpn_destroy(ctx);
+        return 1;
+    };
     double t = 0.0;
     int done = 0;
     while (done < steps) {
-        const int chunk = std::min(100 - done % 100, steps - done);
-        if (bfpn_advance(ctx, chunk) != BFPN_OK) {
-            std::cerr << "Error: " << bfpn_last_error() << std::endl;
-            bfpn_destroy(ctx);
-            return 1;
-        }
+        // advance to the next progress line, diagnostic step or the end (a
+        // call pairs the energy half steps inside it; its last step ends
+        // with the full state, which the diagnostics read)
+        int next = std::min(done + 100 - done % 100, steps);
+        if (moments) next = done + 1;
+        for (int k : spot_step)
+            if (k > done && k < next) next = k;
+        const int chunk = next - done;
+        if (bfpn_advance(ctx, chunk) != BFPN_OK) return fail();
         for (int k = 0; k < chunk; ++k) t += d.dt;
         done += chunk;
+        if (moments) {
+            std::vector<double> m(2 * static_cast<size_t>(ng));
+            if (bfpn_energy_moments(ctx, m.data(), 2 * ng) != BFPN_OK) return fail();
+            moment_history.insert(moment_history.end(), m.begin(), m.end());
+        }
+        for (size_t i = 0; i < spot_depths.size(); ++i)
+            if (spot_step[i] == done) {
+                if (bfpn_spot_plane(ctx, plane.data(), plane.size()) != BFPN_OK) return fail();
+                save_spot_plane(plane, spot_prefix, spot_depths[i], t, ny, nz, ly, lz, d.dy, d.dz);
+            }
         if (done % 100 == 0) {
             int n = 0;
             if (bfpn_history(ctx, idd.data(), proxy.data(), steps, &n) != BFPN_OK) {
@@ -278,6 +373,7 @@ int main(int argc, char** argv) {
     }
     save_series(idd, d.dt, "idd_output.txt");
     save_series(proxy, d.dt, "dose_output.txt");
+    if (moments) save_energy_moments(moment_history, ng, d.dt, "energy_moments.txt");
     std::cout << "Solver completed. Paper-style IDD saved to idd_output.txt" << std::endl;
     std::cout << "Diagnostic scalar proxy saved to dose_output.txt" << std::endl;
     if (profile) {

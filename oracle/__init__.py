@@ -66,6 +66,11 @@ def lib():
         L.orc_idd.argtypes = [_dp, C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp, _dp] + [C.c_int] * 4 + \
             [C.c_double] * 6 + [C.c_int, C.c_int]
         L.orc_idd.restype = C.c_double
+        L.orc_energy_moments.argtypes = [_dp, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                         C.c_longlong, C.c_int] + [C.c_double] * 4 + [_dp]
+        L.orc_spot_plane.argtypes = [_dp, C.c_void_p, C.c_void_p, C.c_void_p, _dp, _dp, _dp,
+                                     C.c_int, C.c_longlong, C.c_int] + [C.c_double] * 4 + \
+            [C.c_int, _dp]
         L.orc_create.argtypes = [C.POINTER(Grid), C.POINTER(Phys), C.c_char_p]
         L.orc_create.restype = C.c_void_p
         L.orc_destroy.argtypes = [C.c_void_p]
@@ -198,6 +203,21 @@ def idd(F, f, F1, f1, S, sigma, en, Ng, Ny, Nz, nang, dg, du, dv, dy, dz, densit
         trapezoidal=False, catastrophic=True) -> float:
     return lib().orc_idd(F, _ptr(f), _ptr(F1), _ptr(f1), S, sigma, en, Ng, Ny, Nz, nang, dg, du,
                          dv, dy, dz, density, int(trapezoidal), int(catastrophic))
+
+
+def energy_moments(F, f, F1, f1, Ng, nyz, nang, du, dv, dy, dz):
+    """Eq.25 moments (energyMomentKernel): array [Ng, 2] of (int psi^1, int psi^2)."""
+    out = np.zeros(2 * Ng)
+    lib().orc_energy_moments(F, _ptr(f), _ptr(F1), _ptr(f1), Ng, nyz, nang, du, dv, dy, dz, out)
+    return out.reshape(Ng, 2)
+
+
+def spot_plane(F, f, F1, f1, S, sigma, en, Ng, nyz, nang, dg, du, dv, density, catastrophic=True):
+    """Y-Z dose plane (spotDosePlaneKernel): nyz values, s = iz (Ny+1) + iy."""
+    out = np.zeros(nyz)
+    lib().orc_spot_plane(F, _ptr(f), _ptr(F1), _ptr(f1), S, sigma, en, Ng, nyz, nang, dg, du, dv,
+                         density, int(catastrophic), out)
+    return out
 
 
 # ------------------------------------------------------------------ solver ---

@@ -716,6 +716,61 @@ double idd_value(const double* F, const double* f, const double* F1, const doubl
     return sum * du * dv * dy * dz;
 }
 
+// Paper Eq.25 energy moments: per group g, the phase-space integrals of
+// psi^1 = F + F1 and psi^2 = f + f1 (energyMomentKernel,
+// src/bfpm_solver.cu:311-360, called from computeEnergyMoments, 3504-3526):
+// out[2g] = du dv dy dz sum psi^1, out[2g+1] = du dv dy dz sum psi^2.
+// Summation in lane order per group.
+void energy_moments(const double* F, const double* f, const double* F1, const double* f1, int Ng,
+                    long long nyz, int nang, double du, double dv, double dy, double dz,
+                    double* out) {
+    const long long per = nyz * nang;
+#pragma omp parallel for schedule(static)
+    for (int ek = 0; ek < Ng; ++ek) {
+        double s1 = 0.0, s2 = 0.0;
+        const long long base = static_cast<long long>(ek) * per;
+        for (long long l = 0; l < per; ++l) {
+            s1 += F[base + l] + (F1 ? F1[base + l] : 0.0);
+            s2 += (f ? f[base + l] : 0.0) + (f1 ? f1[base + l] : 0.0);
+        }
+        const double w = du * dv * dy * dz;
+        out[2 * ek] = s1 * w;
+        out[2 * ek + 1] = s2 * w;
+    }
+}
+
+// Y-Z dose plane at the current depth (spotDosePlaneKernel,
+// src/bfpm_solver.cu:433-475, called from computeSpotDosePlane, 3560-3585):
+// per transverse node s, the Eq.24 cell dose summed over groups and
+// directions, times du dv (no y-z quadrature weights).
+void spot_plane(const double* F, const double* f, const double* F1, const double* f1,
+                const double* S_s, const double* sigma_c, const double* en, int Ng, long long nyz,
+                int nang, double dg, double du, double dv, double density, bool catastrophic,
+                double* out) {
+#pragma omp parallel for schedule(static)
+    for (long long s = 0; s < nyz; ++s) {
+        double sum = 0.0;
+        for (int ek = 0; ek < Ng; ++ek) {
+            const double S_mid = 0.5 * (S_s[ek] + S_s[ek + 1]);
+            const double slope_w = (S_s[ek + 1] - S_s[ek]) / 6.0;
+            const long long base = (static_cast<long long>(ek) * nyz + s) * nang;
+            for (int a = 0; a < nang; ++a) {
+                const long long c = base + a;
+                const double psi1 = F[c] + (F1 ? F1[c] : 0.0);
+                const double psi2 = (f ? f[c] : 0.0) + (f1 ? f1[c] : 0.0);
+                double dose = psi1 * S_mid * dg + psi2 * slope_w * dg;
+                if (catastrophic) {
+                    const double sigma = std::max(sigma_c[ek], 0.0);
+                    const double E_mid = 0.5 * (en[ek] + en[ek + 1]);
+                    dose += psi1 * E_mid * sigma * dg + psi2 * (en[ek + 1] - en[ek]) * sigma * dg / 6.0;
+                }
+                sum += dose / density;
+            }
+        }
+        out[s] = sum * du * dv;
+    }
+}
+
 // ------------------------------------------------------------------ solver ---
 
 struct Solver {
@@ -914,6 +969,20 @@ BFPN_API double orc_idd(const double* F, const double* f, const double* F1, cons
                         double density, int trapezoidal, int catastrophic) {
     return idd_value(F, f, F1, f1, S, sigma, en, Ng, Ny, Nz, nang, dg, du, dv, dy, dz, density,
                      trapezoidal != 0, catastrophic != 0);
+}
+
+BFPN_API void orc_energy_moments(const double* F, const double* f, const double* F1,
+                                 const double* f1, int Ng, long long nyz, int nang, double du,
+                                 double dv, double dy, double dz, double* out) {
+    energy_moments(F, f, F1, f1, Ng, nyz, nang, du, dv, dy, dz, out);
+}
+
+BFPN_API void orc_spot_plane(const double* F, const double* f, const double* F1, const double* f1,
+                             const double* S, const double* sigma, const double* en, int Ng,
+                             long long nyz, int nang, double dg, double du, double dv,
+                             double density, int catastrophic, double* out) {
+    spot_plane(F, f, F1, f1, S, sigma, en, Ng, nyz, nang, dg, du, dv, density, catastrophic != 0,
+               out);
 }
 
 BFPN_API void* orc_create(const bfpn_grid* g, const bfpn_phys* p, const char* path) {

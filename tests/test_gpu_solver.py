@@ -226,3 +226,67 @@ def test_cli_fp64_writes_the_reference_file(tmp_path, golden, gpu):
     m = golden[:, 1] >= 1e-6 * golden[:, 1].max()
     assert np.max(np.abs(out[m, 1] - golden[m, 1]) / golden[m, 1]) < 1e-8
     assert os.path.exists(tmp_path / "dose_output.txt")
+
+
+# ----------------------------------------------------------- diagnostics ---
+@pytest.mark.parametrize("precision", ["FP64", "FP32"])
+def test_energy_moments_and_spot_plane_match_oracle(gpu, oracle_mod, precision):
+    """Eq.25 moments and the Y-Z dose plane of the GPU's own state after 3
+    steps (primary and secondary) equal the oracle's functions on that state
+    (1e-12: FP64 sums of the same values in another order); the plane
+    integrated with the trapezoid weights is the step's IDD (FP32: to the
+    rounding of the stored state, which the IDD does not see)."""
+    case = small_case(flags={"eq15_straggling": 1})
+    g = gpu.Solver(case, getattr(gpu, precision))
+    g.advance(3)
+    st = [g.state(k) for k in range(4)]
+    d = oracle_mod.deltas(case)
+    nyz = (case.Ny + 1) * (case.Nz + 1)
+    nang = case.Nmu * case.Nom
+    m_ref = oracle_mod.energy_moments(*st, case.Ng, nyz, nang, d.du, d.dv, d.dy, d.dz)
+    m = g.energy_moments()
+    assert m.shape == (case.Ng, 2)
+    np.testing.assert_allclose(m, m_ref, rtol=1e-12, atol=1e-12 * np.abs(m_ref).max())
+    S, sig, en = (g.table(k) for k in ("S", "sigma", "en"))
+    p_ref = oracle_mod.spot_plane(*st, S, sig, en, case.Ng, nyz, nang, d.dg, d.du, d.dv,
+                                  case.phys().density)
+    p = g.spot_plane()
+    np.testing.assert_allclose(p, p_ref, rtol=1e-12, atol=1e-12 * np.abs(p_ref).max())
+    iy, iz = np.arange(nyz) % (case.Ny + 1), np.arange(nyz) // (case.Ny + 1)
+    w = np.where((iy == 0) | (iy == case.Ny), 0.5, 1.0) * np.where((iz == 0) | (iz == case.Nz), 0.5, 1.0)
+    # the IDD sums the energy sweep's FP64 outputs, the plane the stored state
+    # (rounded to FP32 in the FP32 instantiation)
+    tol = 1e-11 if precision == "FP64" else 1e-6
+    assert p @ w * d.dy * d.dz == pytest.approx(g.history()[0][-1], rel=tol)
+
+
+def test_cli_writes_energy_moments_and_spot_planes(tmp_path, gpu):
+    """--save-energy-moments and --spot-depths/--spot-prefix write the
+    reference's files (names, headers, layout) with the API's values."""
+    cli = os.path.join(ROOT, "bfpn_gpu_amd", "bin", "bfp_solver")
+    data = os.path.join(ROOT, "bfpn_gpu_amd", "tables", "water")
+    cmd = [cli, "1000", "--data", data, "--material", "water", "--energy", "100", "--ny", "4",
+           "--nz", "3", "--nmu", "4", "--nom", "5", "--ly", "4", "--lz", "3", "--lu", "2",
+           "--lv", "2", "--ng", "60", "--sigma-yz", "0.3", "--trapezoidal-yz", "--time", "0.2",
+           "--precision", "fp64", "--save-energy-moments", "--spot-depths", "0.05,,0.11",
+           "--spot-prefix", "plane"]
+    r = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    idd = np.loadtxt(tmp_path / "idd_output.txt")
+    steps, dt = len(idd), idd[0, 0]
+    lines = open(tmp_path / "energy_moments.txt").read().splitlines()
+    assert lines[0] == "# step x_cm g psi1_integral psi2_integral"
+    mom = np.loadtxt(tmp_path / "energy_moments.txt")
+    assert mom.shape == (steps * 60, 5)
+    np.testing.assert_array_equal(mom[:, 0], np.repeat(np.arange(1, steps + 1), 60))
+    np.testing.assert_array_equal(mom[:, 2], np.tile(np.arange(60), steps))
+    # spot planes: after the first step with (k dt) + dt/2 >= depth
+    files = sorted(p.name for p in tmp_path.glob("plane_req*.txt"))
+    k1, k2 = int(np.ceil(0.05 / dt - 0.5 - 1e-9)), int(np.ceil(0.11 / dt - 0.5 - 1e-9))
+    assert files == sorted([f"plane_req0.05_x{k1 * dt:.2f}.txt", f"plane_req0.11_x{k2 * dt:.2f}.txt"])
+    body = open(tmp_path / files[0]).read().splitlines()
+    assert body[0].startswith("# requested_x_cm 0.05") and body[2] == "# y_cm z_cm dose"
+    blocks = [b for b in "\n".join(body[3:]).split("\n\n") if b.strip()]
+    assert len(blocks) == 4 and all(len(b.splitlines()) == 5 for b in blocks)
+    y0, z0, _ = map(float, blocks[0].splitlines()[0].split())
+    assert (y0, z0) == (-2.0, -1.5)
