@@ -271,7 +271,8 @@ struct Physics final : PhysicsBase {
     size_t device_bytes() const override {
         return coef.bytes() + dose_w.bytes() + yzw.bytes() + om_dev.bytes() + factors.bytes() +
                su.bytes() + sv.bytes() + kv.bytes() + e1.bytes() + e2.bytes() + beg.bytes() +
-               end.bytes() + plan.bytes() + angle_plan.h.bytes() + angle_plan.l.bytes();
+               end.bytes() + plan.bytes() + angle_plan.h.bytes() + angle_plan.m.bytes() +
+               angle_plan.l.bytes();
     }
 };
 

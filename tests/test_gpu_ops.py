@@ -177,6 +177,22 @@ def test_angle_shapes(gpu, oracle_mod, Nmu, Nom, dtype):
     _close(dF.numpy().astype(np.float64), F, TOL64 if dtype == np.float64 else TOL32["angle"])
 
 
+@pytest.mark.parametrize("Nmu,Nom", [(20, 20), (17, 23), (24, 24), (25, 20), (31, 17), (32, 32),
+                                     (20, 31)])
+def test_angle_two_tile_planes(gpu, oracle_mod, Nmu, Nom):
+    """FP32 planes with 16 < Nmu, Nom <= 32 (two 16-row MFMA tiles per axis):
+    several groups, a run of planes that does not divide evenly over the
+    waves, padding of 8 to 15 rows and columns, and rows that are not a
+    multiple of 4 wide."""
+    Ng, nyz = 3, 37
+    F = RNG.standard_normal(Ng * nyz * Nmu * Nom).astype(np.float32).astype(np.float64)
+    sig = RNG.random(Ng) * 1e-3
+    dF = _dev(gpu, F, np.float32)
+    gpu.ops.angle_full(dF, sig, nyz, Ng, Nmu, Nom, 0.1, 0.07, 0.02, 1.85)
+    oracle_mod.angle_full(F, sig, nyz, Ng, Nmu, Nom, 0.1, 0.07, 0.02, 1.85)
+    _close(dF.numpy().astype(np.float64), F, TOL32["angle"])
+
+
 @pytest.mark.parametrize("Nmu,Nom", [(20, 20), (7, 7)])
 def test_angle_repeated_does_not_drift(gpu, Nmu, Nom):
     """The DST runs once per depth step on the same beam, so a systematic
