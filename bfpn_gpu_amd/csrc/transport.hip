@@ -88,15 +88,31 @@ __device__ inline T divergence(T zc, T near, T S_c, T S_n, T zm, T zp, T sz_m, T
     return cy * fy + cz * fz;
 }
 
-// Element at a uniform base plus a per-lane 32-bit byte offset (the form the
-// global memory instructions take directly: scalar base, vector offset).
-template <typename T>
-__device__ inline T& at(T* base, unsigned byte_off) {
-    return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off);
+// One energy group's block of the state is addressed through a buffer
+// resource (base and size in SGPRs): the row offset is a uniform scalar, the
+// lane's column a 32-bit vector offset, so no address arithmetic runs on the
+// vector ALUs.  0x00020000: the raw 32-bit data format word of gfx9-family
+// buffer resources.
+__device__ inline __amdgpu_buffer_rsrc_t group_rsrc(const void* base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes),
+                                             0x00020000);
 }
 template <typename T>
-__device__ inline const T& at(const T* base, unsigned byte_off) {
-    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+__device__ inline T bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff);
+template <>
+__device__ inline float bload<float>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+template <>
+__device__ inline double bload<double>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ inline void bstore(float v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
+}
+__device__ inline void bstore(double v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
 }
 
 // grid: (ceil(nang / MW), Ng), block: MW * R threads (R y columns per pass,
@@ -111,11 +127,14 @@ __device__ inline const T& at(const T* base, unsigned byte_off) {
 // in a register by its own lane and published in LDS for the downwind
 // neighbour, which reads it instead of recomputing it.
 //
-// The z march is unrolled by the ring period 4: the register windows are
-// indexed by row mod 4 (slopes by row mod 2), so with the step's phase k a
-// compile-time constant, sliding a window is a renaming rather than a chain
-// of moves, and the LDS ring slots are immediate offsets.  The corrector
-// needs psi and L(psi) of its row only as psi - h/2 L(psi), kept per row.
+// The z march is unrolled by 8: the psi window and the rows in flight share
+// one register ring indexed by row mod 8 (row t+6 is loaded into the slot of
+// row t-2, which step t no longer reads), the other windows are indexed by
+// row mod 4 (slopes by row mod 2), so with the step's phase a compile-time
+// constant, sliding a window is a renaming rather than a chain of moves, a
+// loaded row is never copied (a copy would wait for the load), and the LDS
+// ring slots are immediate offsets.  The corrector needs psi and L(psi) of
+// its row only as psi - h/2 L(psi), kept per row.
 template <typename T, int MW, int P, bool CLIP>
 __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
     T* __restrict__ F, const T* __restrict__ omend, int Ny, int Nz, int nang, int R, T inv_dy,
@@ -137,8 +156,11 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
     const int m0 = (L - g * mchunks) * MW;
     const bool live = m0 + mm < nang;
     // Lanes past the last direction (and columns past Ny) compute a copy of
-    // the last one from in-bounds addresses and store nothing, so loads and
-    // arithmetic need no predicate.
+    // the last one from in-bounds addresses, so loads and arithmetic need no
+    // predicate.  Their LDS writes repeat the owner's value at the owner's
+    // slot in the same step (a harmless duplicate), and their global stores
+    // go to an offset past the end of the buffer resource, which the
+    // hardware's range check drops: no store needs a branch.
     const int m = live ? m0 + mm : nang - 1;
     const T wy = omend[2 * m], wz = omend[2 * m + 1];
     const bool py = wy >= T(0), pz = wz >= T(0);
@@ -146,43 +168,44 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
     const T hy = py ? T(0.5) : T(-0.5), hz = pz ? T(0.5) : T(-0.5);
     const T hh = T(0.5) * h;
     const long long zstride = static_cast<long long>(Ny + 1) * nang;
-    // work-group base (uniform); lanes address it with 32-bit offsets
-    T* Fg = F + static_cast<long long>(g) * zstride * (Nz + 1) + m0;
+    const unsigned zbytes = static_cast<unsigned>(zstride * sizeof(T));  // one z row
+    // the group's block from direction m0 on (uniform); lanes add 32-bit offsets
+    const __amdgpu_buffer_rsrc_t Fg =
+        group_rsrc(F + static_cast<long long>(g) * zstride * (Nz + 1) + m0,
+                   static_cast<unsigned>((zstride * (Nz + 1) - m0) * sizeof(T)));
 
-    bool on[P];
     unsigned boff[P];  // byte offset of the lane's column in a plane row
+    unsigned soff[P];  // the same for stores; 0x80000000 (out of range) for copies
     int o0[P], o_near[P], o_m[P], o_p[P];  // LDS column: own, upwind, y-1, y+1 (clamped)
-    T wpsi[P][4];   // psi, rows t-1 .. t+2 after the commit of step t
+    T win[P][8];    // psi rows t-1 .. t+2 after the commit of step t, then
+                    // rows t+3 .. t+6 in flight (slot: row mod 8)
     T wpred[P][4];  // psi*, rows t-3 .. t after the predictor of step t
     T wbase[P][4];  // psi - h/2 L(psi), rows t-2 .. t
     T wS[P][4];     // own y slope of psi, rows t .. t+2 after step t's slope pass
     T wSp[P][2];    // own y slope of psi*, rows t-2, t-1 after step t's slope pass
     T sz[P][2];     // z slopes of psi, rows t-1 and t before step t
     T szp[P][2];    // z slopes of psi*, rows t-3 and t-2 before step t
-    T nxt[P][4];    // psi rows t+2 .. t+5 in flight (slot: row mod 4)
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         const int y = ry + j * R;
-        on[j] = y < Ny + 1;
         const int yc = min(y, Ny);
         boff[j] = static_cast<unsigned>((yc * nang + (m - m0)) * sizeof(T));
+        soff[j] = (y <= Ny && live) ? boff[j] : 0x80000000u;
         auto lo = [&](int yy) { return (min(max(yy, 0), Ny)) * YP + mm; };
         o0[j] = lo(yc);
         o_near[j] = lo(py ? yc - 1 : yc + 1);
         o_m[j] = lo(yc - 1);
         o_p[j] = lo(yc + 1);
-        const T r0 = at(Fg, boff[j]);
-        const T r1 = at(Fg + zstride, boff[j]);  // Nz >= 1
-        if (on[j]) {
-            ring[o0[j] + kPsi * MW] = r0;
-            ring[o0[j] + (kPsi + 1) * MW] = r1;
-        }
-        wpsi[j][3] = wpsi[j][0] = r0;  // row -1 clamps to row 0
-        wpsi[j][1] = r1;
-        wpsi[j][2] = T(0);
+        const T r0 = bload<T>(Fg, boff[j], 0);
+        const T r1 = bload<T>(Fg, boff[j], zbytes);  // Nz >= 1
+        ring[o0[j] + kPsi * MW] = r0;
+        ring[o0[j] + (kPsi + 1) * MW] = r1;
+        win[j][7] = win[j][0] = r0;  // row -1 clamps to row 0
+        win[j][1] = r1;
+        win[j][6] = T(0);
 #pragma unroll
-        for (int r = 0; r < 4; ++r)  // rows past Nz: never used
-            nxt[j][(r + 2) & 3] = at(Fg + min(r + 2, Nz) * zstride, boff[j]);
+        for (int r = 2; r < 6; ++r)  // rows past Nz: copies of row Nz
+            win[j][r] = bload<T>(Fg, boff[j], static_cast<unsigned>(min(r, Nz)) * zbytes);
 #pragma unroll
         for (int k = 0; k < 2; ++k) sz[j][k] = szp[j][k] = wSp[j][k] = T(0);  // boundary: 0
 #pragma unroll
@@ -195,38 +218,42 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
             const T* row = ring + (kPsi + r) * MW;
-            const T S = limited_slope(row[o_p[j]] - wpsi[j][r], wpsi[j][r] - row[o_m[j]]);
+            const T S = limited_slope(row[o_p[j]] - win[j][r], win[j][r] - row[o_m[j]]);
             wS[j][r] = S;
-            if (on[j]) ring[o0[j] + (kSPsi + r) * MW] = S;
+            ring[o0[j] + (kSPsi + r) * MW] = S;
         }
     }
 
-    // Step t (phase k = t mod 4): commit psi row t+2; slopes of psi row t+2
-    // and psi* row t-1; predictor on row t; corrector on row t-2.  Every step
-    // runs the same branch-free code; near the ends of the march (EDGE)
-    // selects apply the clamped rows and the boundary slopes, and only the
-    // stores are predicated.  Past the last row the predictor works on
-    // clamped copies and its result is replaced by the clamp; steps past
-    // Nz+2 (padding the last block of 4) store nothing.
+    // Step t (phase k8 = t mod 8, k = t mod 4): commit psi row t+2; slopes of
+    // psi row t+2 and psi* row t-1; predictor on row t; corrector on row t-2.
+    // Every step runs the same branch-free code; near the ends of the march
+    // (EDGE) selects apply the clamped rows and the boundary slopes, and the
+    // global store is skipped (uniformly) before row 0 and past row Nz.  Past
+    // the last row the predictor works on clamped copies and its result is
+    // replaced by the clamp; steps past Nz+2 (padding the last block) store
+    // nothing.
     auto step = [&](int t, auto phase, auto edge) {
-        constexpr int k = decltype(phase)::value;
+        constexpr int k8 = decltype(phase)::value;  // t mod 8
+        constexpr int k = k8 & 3;
         constexpr bool E = decltype(edge)::value;
-        const int rc = t + 2;  // commit psi row t+2 into the slot of row t-2
+        const int rc = t + 2;  // commit psi row t+2 (LDS ring slot of row t-2)
         {
             T* crow = ring + (kPsi + ((k + 2) & 3)) * MW;
 #pragma unroll
             for (int j = 0; j < P; ++j) {
-                const T v = (!E || rc <= Nz) ? nxt[j][(k + 2) & 3] : wpsi[j][(k + 1) & 3];
-                wpsi[j][(k + 2) & 3] = v;
-                if (on[j]) crow[o0[j]] = v;
+                if (E && rc > Nz) win[j][(k8 + 2) & 7] = win[j][(k8 + 1) & 7];
+                crow[o0[j]] = win[j][(k8 + 2) & 7];
             }
         }
         __syncthreads();
+        // Keep each step's arithmetic after its barrier: moved earlier, the z
+        // differences of later rows would wait for rows still in flight.
+        __builtin_amdgcn_sched_barrier(0);
         {
-            // refill the slot just committed with row t+6: four rows in flight
-            const T* src = Fg + static_cast<long long>(E ? min(t + 6, Nz) : t + 6) * zstride;
+            // row t+6 into the register slot of row t-2: four rows in flight
+            const unsigned so = static_cast<unsigned>(E ? min(t + 6, Nz) : t + 6) * zbytes;
 #pragma unroll
-            for (int j = 0; j < P; ++j) nxt[j][(k + 2) & 3] = at(src, boff[j]);
+            for (int j = 0; j < P; ++j) win[j][(k8 + 6) & 7] = bload<T>(Fg, boff[j], so);
         }
         // LDS reads of the step, issued together: y neighbours of psi row t+2
         // and psi* row t-1 (their slopes), upwind value and slope of psi row t
@@ -256,28 +283,26 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
             T* sd1 = ring + (kSPred + ((k + 1) & 1)) * MW;
 #pragma unroll
             for (int j = 0; j < P; ++j) {
-                const T c2 = wpsi[j][(k + 2) & 3], c1 = wpred[j][(k + 3) & 3];
+                const T c2 = win[j][(k8 + 2) & 7], c1 = wpred[j][(k + 3) & 3];
                 const T S2 = limited_slope(a_p[j] - c2, c2 - a_m[j]);
                 const T S1 = limited_slope(b_p[j] - c1, c1 - b_m[j]);
                 wS[j][(k + 2) & 3] = S2;
                 wSp[j][(k + 1) & 1] = S1;
-                if (on[j]) {
-                    sp2[o0[j]] = S2;
-                    sd1[o0[j]] = S1;
-                }
+                sp2[o0[j]] = S2;
+                sd1[o0[j]] = S1;
             }
         }
         {  // predictor on row t
             T* prow = ring + (kPred + k) * MW;
 #pragma unroll
             for (int j = 0; j < P; ++j) {
-                const T zm = wpsi[j][(k + 3) & 3], zc = wpsi[j][k], zp = wpsi[j][(k + 1) & 3];
-                const T s_next = limited_slope(wpsi[j][(k + 2) & 3] - zp, zp - zc);
+                const T zm = win[j][(k8 + 7) & 7], zc = win[j][k8], zp = win[j][(k8 + 1) & 7];
+                const T s_next = limited_slope(win[j][(k8 + 2) & 7] - zp, zp - zc);
                 const T L0 = divergence(zc, pn[j], wS[j][k], pS[j], zm, zp, sz[j][(k + 1) & 1],
                                         sz[j][k & 1], s_next, cy, cz, hy, hz, pz);
                 T v = zc - h * L0;
                 if (CLIP) v = fmax(v, T(0));
-                if (on[j]) prow[o0[j]] = v;
+                prow[o0[j]] = v;
                 sz[j][(k + 1) & 1] = s_next;
                 wbase[j][k] = zc - hh * L0;
                 wpred[j][k] = (!E || t <= Nz) ? v : wpred[j][(k + 3) & 3];  // past Nz: row Nz
@@ -285,7 +310,7 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
             }
         }
         {  // corrector on row t-2, written in place
-            T* dst = Fg + static_cast<long long>(t - 2) * zstride;
+            const unsigned so = static_cast<unsigned>(E ? max(t - 2, 0) : t - 2) * zbytes;
             const bool real = !E || (t >= 2 && t <= Nz + 2);
 #pragma unroll
             for (int j = 0; j < P; ++j) {
@@ -297,22 +322,33 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
                                         hz, pz);
                 T v = wbase[j][(k + 2) & 3] - hh * L1;
                 if (CLIP) v = fmax(v, T(0));
-                if (real && on[j] && live) at(dst, boff[j]) = v;
+                if (real) bstore(v, Fg, soff[j], so);
                 szp[j][(k + 1) & 1] = (!E || t >= 2) ? s_next : T(0);  // rows <= 0: boundary
             }
         }
     };
-    auto block = [&](int t0, auto edge) {
-        step(t0, std::integral_constant<int, 0>{}, edge);
-        step(t0 + 1, std::integral_constant<int, 1>{}, edge);
-        step(t0 + 2, std::integral_constant<int, 2>{}, edge);
-        step(t0 + 3, std::integral_constant<int, 3>{}, edge);
+    // four steps from t0, t0 mod 8 = B
+    auto block = [&](int t0, auto base, auto edge) {
+        constexpr int B = decltype(base)::value;
+        step(t0, std::integral_constant<int, B>{}, edge);
+        step(t0 + 1, std::integral_constant<int, B + 1>{}, edge);
+        step(t0 + 2, std::integral_constant<int, B + 2>{}, edge);
+        step(t0 + 3, std::integral_constant<int, B + 3>{}, edge);
     };
-    // interior blocks: 2 <= t and t + 6 <= Nz for all four steps
-    block(0, std::true_type{});
+    using I0 = std::integral_constant<int, 0>;
+    using I4 = std::integral_constant<int, 4>;
+    block(0, I0{}, std::true_type{});
     int t0 = 4;
-    for (; t0 + 9 <= Nz; t0 += 4) block(t0, std::false_type{});
-    for (; t0 <= Nz + 2; t0 += 4) block(t0, std::true_type{});
+    // interior: 2 <= t and t + 6 <= Nz for all eight steps
+    for (; t0 + 13 <= Nz; t0 += 8) {
+        block(t0, I4{}, std::false_type{});
+        block(t0 + 4, I0{}, std::false_type{});
+    }
+    for (; t0 <= Nz + 2; t0 += 8) {
+        block(t0, I4{}, std::true_type{});
+        if (t0 + 4 > Nz + 2) break;
+        block(t0 + 4, I0{}, std::true_type{});
+    }
 }
 
 struct MarchShape {
@@ -373,6 +409,9 @@ void launch_transport_half(T* F, const T* omend, int Ny, int Nz, int Ng, int nan
     // the LDS holds 15 slots of MW values per y column: 15 * (Ny+1) * 8 * elem <= 160 KB
     BFPN_REQUIRE(sh.MW > 0, sizeof(T) == 4 ? "transport supports at most 341 y nodes in FP32"
                                            : "transport supports at most 170 y nodes in FP64");
+    // one group's block is addressed with 32-bit byte offsets
+    BFPN_REQUIRE(static_cast<long long>(Ny + 1) * (Nz + 1) * nang * sizeof(T) < (1LL << 31),
+                 "transport needs (Ny+1)(Nz+1)*Nmu*Nom values of one energy group below 2 GiB");
 #define BFPN_MARCH(MW_, P_)                                                                    \
     if (sh.MW == MW_ && sh.P == P_)                                                            \
         return launch_march<T, MW_, P_>(F, omend, Ny, Nz, Ng, nang, sh, dy, dz, h, preserve_sign, \
