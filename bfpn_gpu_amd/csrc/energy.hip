@@ -22,6 +22,8 @@
 
 #include "kernels.hpp"
 
+#include <utility>
+
 namespace bfpn {
 namespace {
 
@@ -46,6 +48,16 @@ template <typename T, int LPT>
 struct Pack {
     T v[LPT];
 };
+
+// f(integral_constant<int, 0>), ..., f(integral_constant<int, N - 1>)
+template <typename F, int... I>
+__device__ inline void unroll_seq(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, typename F>
+__device__ inline void unroll(F&& f) {
+    unroll_seq(f, std::make_integer_sequence<int, N>{});
+}
 
 template <typename T, int LPT>
 __device__ inline Pack<T, LPT> load_pack(const T* p) {
@@ -80,8 +92,13 @@ __device__ inline void store_pack(T* p, const double (&v)[LPT]) {
     }
 }
 
-// Prefetch depth (groups in flight per thread): a register ring, slot k
-// refilled with group g - D right after group g consumed it.
+// Prefetch depth D (groups in flight per thread).  The register ring has
+// D + 1 slots and the sweep is unrolled by D + 1, so slot indices are
+// compile-time constants: the step on group g (slot k) loads group g - D
+// into slot k - 1, which the previous step has finished with.  No load ever
+// targets a register whose old value is still in use, so the compiler never
+// has to copy a loaded value -- a copy waits for the load and would drain
+// the ring.
 template <typename T, int LPT>
 constexpr int prefetch_depth() { return LPT == 4 ? 8 : LPT == 2 ? 12 : (sizeof(T) == 4 ? 16 : 8); }
 
@@ -140,12 +157,12 @@ __global__ __launch_bounds__(kBlock) void energy_half_kernel(
     T* __restrict__ F, T* __restrict__ f, const EnergyCoef* __restrict__ coef,
     const T* __restrict__ Q1, int Ng, int lanes, const double* __restrict__ yzw, int nang,
     double* __restrict__ partials, int proxy) {
-    constexpr int D = prefetch_depth<T, LPT>();
+    constexpr int D = prefetch_depth<T, LPT>(), RING = D + 1;
     __shared__ double lds[kBlock / 64];
     const int lane = (blockIdx.x * kBlock + threadIdx.x) * LPT;  // first of LPT lanes
     double acc_dose = 0.0, acc_proxy = 0.0;
     if (lane < lanes) {
-        Pack<T, LPT> rF[D], rf[D], rQ[D];
+        Pack<T, LPT> rF[RING], rf[RING], rQ[RING];
         // Loads are unconditional (groups below 0 re-read group 0 and are
         // never consumed) and full chunks run without branches, so the
         // compiler can count the loads in flight instead of draining them.
@@ -171,14 +188,14 @@ __global__ __launch_bounds__(kBlock) void energy_half_kernel(
         // covered.  (Dose weights are read at use: two sets of forms plus
         // them would not fit the scalar registers.)
         Forms cn = load_forms(coef, Ng - 1);
-        auto step = [&](int k, int g) {
+        auto step = [&](auto slot, int g) {  // group g sits in ring slot `slot`
+            constexpr int k = decltype(slot)::value;
             const Forms c = cn;
             cn = load_forms(coef, max(g - 1, 0));
             __builtin_amdgcn_sched_barrier(0);
+            fetch((k + D) % RING, g - D);
             const Pack<T, LPT> cF = rF[k], cf = rf[k], cQ = rQ[k];
-            // old F at g-1: next ring slot (slot 0 was refilled with g0-D)
-            const Pack<T, LPT> lo = rF[(k + 1) % D];
-            fetch(k, g - D);
+            const Pack<T, LPT> lo = rF[(k + 1) % RING];  // old F at g-1
             double Fv[LPT], fv[LPT];
 #pragma unroll
             for (int j = 0; j < LPT; ++j)
@@ -198,13 +215,11 @@ __global__ __launch_bounds__(kBlock) void energy_half_kernel(
             }
         };
         int g0 = Ng - 1;
-        for (; g0 >= D - 1; g0 -= D) {
-#pragma unroll
-            for (int k = 0; k < D; ++k) step(k, g0 - k);
-        }
-#pragma unroll
-        for (int k = 0; k < D; ++k)  // tail: groups g0 .. 0 (fewer than D)
-            if (k <= g0) step(k, g0 - k);
+        for (; g0 >= RING - 1; g0 -= RING)
+            unroll<RING>([&](auto k) { step(k, g0 - decltype(k)::value); });
+        unroll<RING>([&](auto k) {  // tail: groups g0 .. 0 (fewer than RING)
+            if (decltype(k)::value <= g0) step(k, g0 - decltype(k)::value);
+        });
         // (loaded only now: a load issued before the ring would hold up its
         // first consumer behind every state load in flight)
         if (DOSE)
@@ -243,57 +258,68 @@ __global__ __launch_bounds__(kBlock) void energy_pair_kernel(
     const T* __restrict__ Qa, const T* __restrict__ Qb, T* __restrict__ XF, T* __restrict__ Xf,
     int Ng, int lanes, const double* __restrict__ yzw, int nang, double* __restrict__ partials,
     int proxy) {
-    constexpr int D = pair_depth<T, LPT, SRC>();
+    constexpr int D = pair_depth<T, LPT, SRC>(), RING = D + 1;
     __shared__ double lds[kBlock / 64];
     const int lane = (blockIdx.x * kBlock + threadIdx.x) * LPT;
     double acc_dose = 0.0, acc_proxy = 0.0;
     if (lane < lanes) {
-        Pack<T, LPT> rF[D], rf[D], rQa[D], rQb[D];
+        Pack<T, LPT> rF[RING], rf[RING], rQa[RING], rQb[RING];
+        // Qb (the second half's source) is refilled separately, at the end of
+        // a step: the second half reads slot k - 1's Qb in the step that
+        // refills that slot.
         auto fetch = [&](int k, int g) {
             const long long base = static_cast<long long>(max(g, 0)) * lanes;
             rF[k] = load_pack<T, LPT>(F + base + lane);
             rf[k] = load_pack<T, LPT>(f + base + lane);
-            if (SRC) {
-                rQa[k] = load_pack<T, LPT>(Qa + base + lane);
-                rQb[k] = load_pack<T, LPT>(Qb + base + lane);
-            }
+            if (SRC) rQa[k] = load_pack<T, LPT>(Qa + base + lane);
+        };
+        auto fetch_qb = [&](int k, int g) {
+            if (SRC) rQb[k] = load_pack<T, LPT>(Qb + static_cast<long long>(max(g, 0)) * lanes + lane);
         };
 #pragma unroll
-        for (int k = 0; k < D; ++k) fetch(k, Ng - 1 - k);
+        for (int k = 0; k < D; ++k) {
+            fetch(k, Ng - 1 - k);
+            fetch_qb(k, Ng - 1 - k);
+        }
         // first half: carried input at g+1, output difference; dose sums
         T P_hi[LPT], R_hi[LPT];
         double hd[LPT], dose[LPT];
         // second half, one group behind: its input (the first's output,
-        // rounded to the state type) at g+1 and g+2, its output difference,
-        // its source at g+1
-        T P2[LPT], R2[LPT], P2_hi[LPT], R2_hi[LPT], q2[LPT];
+        // rounded to the state type) at g+1 and g+2, its output difference
+        T P2[LPT], R2[LPT], P2_hi[LPT], R2_hi[LPT];
         double hd2[LPT];
 #pragma unroll
         for (int j = 0; j < LPT; ++j) {
-            P_hi[j] = R_hi[j] = P2[j] = R2[j] = P2_hi[j] = R2_hi[j] = q2[j] = T(0);
+            P_hi[j] = R_hi[j] = P2[j] = R2[j] = P2_hi[j] = R2_hi[j] = T(0);
             hd[j] = dose[j] = hd2[j] = 0.0;
         }
-        // second half at group g2, whose lower neighbour's input is P_lo2
-        auto second = [&](int g2, const T (&P_lo2)[LPT]) {
+        // second half at group g2, whose lower neighbour's input is P_lo2 and
+        // whose source is q
+        auto second = [&](int g2, const T (&P_lo2)[LPT], const Pack<T, LPT>& q) {
+            // (requested here, not hoisted: three sets of forms live at once
+            // would not fit the scalar registers)
+            __builtin_amdgcn_sched_barrier(0);
             const Forms c2 = load_forms(coef, g2);
             double Fv[LPT], fv[LPT];
 #pragma unroll
             for (int j = 0; j < LPT; ++j)
                 recur<SRC, STRAG>(c2, P2[j], R2[j], static_cast<double>(P_lo2[j]),
-                                  static_cast<double>(q2[j]), P2_hi[j], R2_hi[j], hd2[j], fv[j],
-                                  Fv[j]);
+                                  SRC ? static_cast<double>(q.v[j]) : 0.0, P2_hi[j], R2_hi[j],
+                                  hd2[j], fv[j], Fv[j]);
             const long long base = static_cast<long long>(g2) * lanes;
             store_pack<T, LPT>(F + base + lane, Fv);
             store_pack<T, LPT>(f + base + lane, fv);
         };
         Forms cn = load_forms(coef, Ng - 1);
-        auto step = [&](int k, int g) {
+        // group g in ring slot `slot`; FIRST: g = Ng - 1, no second half behind it
+        auto step = [&](auto slot, int g, auto first) {
+            constexpr int k = decltype(slot)::value;
             const Forms c = cn;
             cn = load_forms(coef, max(g - 1, 0));
             __builtin_amdgcn_sched_barrier(0);
-            const Pack<T, LPT> cF = rF[k], cf = rf[k], cQa = rQa[k], cQb = rQb[k];
-            const Pack<T, LPT> lo = rF[(k + 1) % D];
-            fetch(k, g - D);
+            fetch((k + D) % RING, g - D);
+            const Pack<T, LPT> cF = rF[k], cf = rf[k], cQa = rQa[k];
+            const Pack<T, LPT> lo = rF[(k + 1) % RING];
             double Fv[LPT], fv[LPT];
             T tF[LPT];
 #pragma unroll
@@ -316,24 +342,31 @@ __global__ __launch_bounds__(kBlock) void energy_pair_kernel(
             }
 #pragma unroll
             for (int j = 0; j < LPT; ++j) tF[j] = static_cast<T>(Fv[j]);
-            if (g + 1 < Ng) second(g + 1, tF);
+            if (!decltype(first)::value) second(g + 1, tF, rQb[(k + D) % RING]);  // Qb at g+1
+            fetch_qb((k + D) % RING, g - D);
 #pragma unroll
             for (int j = 0; j < LPT; ++j) {  // the first's output at g: the second's next input
                 P2[j] = tF[j];
                 R2[j] = static_cast<T>(fv[j]);
-                q2[j] = SRC ? cQb.v[j] : T(0);
             }
         };
-        int g0 = Ng - 1;
-        for (; g0 >= D - 1; g0 -= D) {
-#pragma unroll
-            for (int k = 0; k < D; ++k) step(k, g0 - k);
-        }
-#pragma unroll
-        for (int k = 0; k < D; ++k)
-            if (k <= g0) step(k, g0 - k);
+        // group Ng - 1 (slot 0) on its own, so the loop body has no branch
+        step(std::integral_constant<int, 0>{}, Ng - 1, std::true_type{});
+        int g0 = Ng - 2;  // in slot 1
+        for (; g0 >= RING - 1; g0 -= RING)
+            unroll<RING>([&](auto i) {
+                step(std::integral_constant<int, (decltype(i)::value + 1) % RING>{},
+                     g0 - decltype(i)::value, std::false_type{});
+            });
+        unroll<RING>([&](auto i) {
+            if (decltype(i)::value <= g0)
+                step(std::integral_constant<int, (decltype(i)::value + 1) % RING>{},
+                     g0 - decltype(i)::value, std::false_type{});
+        });
         const T none[LPT] = {};
-        second(0, none);  // group 0 has no lower neighbour
+        Pack<T, LPT> q0{};
+        if (SRC) q0 = load_pack<T, LPT>(Qb + lane);
+        second(0, none, q0);  // group 0 has no lower neighbour
 #pragma unroll
         for (int j = 0; j < LPT; ++j) acc_dose += dose[j] * yzw[(lane + j) / nang];
     }
