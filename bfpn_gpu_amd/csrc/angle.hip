@@ -22,11 +22,18 @@ namespace {
 // ---------------------------------------------------------- FP32 / MFMA ----
 // One wave per (g, s) plane (each wave walks a contiguous run of planes), the
 // plane zero padded to 32 x 32, every stage a 32x32x32 product on the matrix
-// cores (both operands split in three BF16 terms, six products kept, FP32
-// accumulation; MT / NT = 16-row tiles along v and a).  The DST runs every
-// depth step on the same beam, so its error compounds: with two-term splits
-// the FP32 IDD drifts ~1 % above the FP64 one over 1600 steps, with three
-// terms the operator is as exact as FP32 arithmetic.
+// cores in split FP16 with FP32 accumulation (MT / NT = 16-row tiles along v
+// and a): the state in two FP16 terms, the constant bases in three, four
+// products kept.  The DST runs every depth step on the same beam, so its
+// error compounds: with two-term BF16 splits the FP32 IDD drifts ~1 % above
+// the FP64 one over 1600 steps; two FP16 terms carry 22 significand bits,
+// and the operator is as exact as FP32 arithmetic.  FP16 needs a range: each
+// plane is scaled by a power of two s so that max|X s| lies in [16, 32),
+// the bases are stored times 16, so every intermediate stays below 2^14
+// (|Y|, |W| <= 21 max|X| for the sine bases, |Z| <= 1.6 max|X| with the CN
+// normalisation) and small values stay clear of the FP16 subnormals; the
+// CN factors absorb 1/256 and the result is multiplied by 1/(256 s), all
+// exact.
 // The stages chain in registers, with no LDS: a stage's output tile, lane
 // holding D[4q+r][i], is the next stage's K operand when the K axis is taken
 // in the order  slot 8q + 4u + r  <->  index 16u + 4q + r  (chain_index), and
@@ -48,14 +55,14 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
     // lo) live in LDS, shared by the work-group's four waves and read per
     // stage: held in registers they would cost 72 VGPRs and half the
     // occupancy.  Rows padded to 40 values (80 bytes) against bank conflicts.
-    __shared__ __attribute__((aligned(16))) __bf16 cst[9][32][40];
+    __shared__ __attribute__((aligned(16))) _Float16 cst[9][32][40];
     {
-        const __bf16* src[9] = {plan.su_h, plan.su_m, plan.su_l, plan.sup_h, plan.sup_m,
-                                plan.sup_l, plan.svp_h, plan.svp_m, plan.svp_l};
+        const _Float16* src[9] = {plan.su_h, plan.su_m, plan.su_l, plan.sup_h, plan.sup_m,
+                                  plan.sup_l, plan.svp_h, plan.svp_m, plan.svp_l};
         for (int e = threadIdx.x; e < 9 * 32 * 4; e += blockDim.x) {  // 16-byte chunks
             const int a = e >> 7, r = (e >> 2) & 31, c = (e & 3) * 8;
-            *reinterpret_cast<bf16x8*>(&cst[a][r][c]) =
-                *reinterpret_cast<const bf16x8*>(src[a] + r * 32 + c);
+            *reinterpret_cast<f16x8*>(&cst[a][r][c]) =
+                *reinterpret_cast<const f16x8*>(src[a] + r * 32 + c);
         }
     }
     __syncthreads();
@@ -66,11 +73,11 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
     const int na = Nmu * Nom;
     // Fragment of constant `a` (0 Su, 1 Su chained, 2 Sv chained), tile t,
     // at compile-time offsets from the lane's base (row i, columns 8q..).
-    auto frag = [&](const __bf16* base, int a, int t) {
+    auto frag = [&](const _Float16* base, int a, int t) {
         constexpr int kArr = 32 * 40, kTile = 16 * 40;
-        return SplitFrag3{*reinterpret_cast<const bf16x8*>(base + (3 * a) * kArr + t * kTile),
-                          *reinterpret_cast<const bf16x8*>(base + (3 * a + 1) * kArr + t * kTile),
-                          *reinterpret_cast<const bf16x8*>(base + (3 * a + 2) * kArr + t * kTile)};
+        return HalfFrag3{*reinterpret_cast<const f16x8*>(base + (3 * a) * kArr + t * kTile),
+                         *reinterpret_cast<const f16x8*>(base + (3 * a + 1) * kArr + t * kTile),
+                         *reinterpret_cast<const f16x8*>(base + (3 * a + 2) * kArr + t * kTile)};
     };
 
     int g_cur = -1;
@@ -79,12 +86,13 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
         // an opaque zero per plane keeps the constant reads inside the loop
         int oz;
         asm volatile("v_mov_b32 %0, 0" : "=v"(oz));
-        const __bf16* cb = &cst[0][i][8 * q] + oz;
+        const _Float16* cb = &cst[0][i][8 * q] + oz;
         float* X = F + static_cast<long long>(p) * na;
         const int g = p / nyz;
-        if (g != g_cur) {  // CN factors of this group, in the D layout
+        if (g != g_cur) {  // CN factors of this group, in the D layout, / 256
             g_cur = g;
             const float* C = factors + static_cast<long long>(g) * na;
+            constexpr float kUnscale = 1.0f / (kAngleBasisScale * kAngleBasisScale);
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -92,39 +100,55 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {
                         const int nu = mt * 16 + 4 * q + r, mu = nt * 16 + i;
-                        cf[mt][nt][r] = (nu < Nmu && mu < Nom) ? C[nu * Nom + mu] : 0.f;
+                        cf[mt][nt][r] = (nu < Nmu && mu < Nom) ? C[nu * Nom + mu] * kUnscale : 0.f;
                     }
         }
         // stage 1: Y[v][mu] = sum_a X[v][a] Su[mu][a].  The padding rows and
         // columns of the 32 x 32 tile load clamped copies of plane values:
         // the bases' zero padding multiplies them by exact zeros (columns
         // a >= Nom here, rows v >= Nmu in stage 2), so no lane is masked.
-        SplitFrag3 xa[MT];
+        float x[MT][8];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
             const float* row = X + min(mt * 16 + i, Nmu - 1) * Nom;
-            float x[8];
             if (V4) {
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
                     const float4 t =
                         *reinterpret_cast<const float4*>(row + min(8 * q + 4 * h, Nom - 4));
-                    x[4 * h] = t.x;
-                    x[4 * h + 1] = t.y;
-                    x[4 * h + 2] = t.z;
-                    x[4 * h + 3] = t.w;
+                    x[mt][4 * h] = t.x;
+                    x[mt][4 * h + 1] = t.y;
+                    x[mt][4 * h + 2] = t.z;
+                    x[mt][4 * h + 3] = t.w;
                 }
             } else {
 #pragma unroll
-                for (int e = 0; e < 8; ++e) x[e] = row[min(8 * q + e, Nom - 1)];
+                for (int e = 0; e < 8; ++e) x[mt][e] = row[min(8 * q + e, Nom - 1)];
             }
-            xa[mt] = split8_3(x);
+        }
+        // the plane's scale s = 2^(4 - floor(log2 max|X|)) (biased exponent
+        // clamped so that s and 1/s are normal numbers), uniform over the wave
+        float mx = 0.f;
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+            for (int e = 0; e < 8; ++e) mx = fmaxf(mx, fabsf(x[mt][e]));
+        const int ebits = __builtin_amdgcn_readfirstlane(__float_as_int(wave_max(mx))) >> 23;
+        const int sb = min(max(258 - ebits, 1), 253);
+        const float sc = __int_as_float(sb << 23);
+        const float unscale = __int_as_float((254 - sb) << 23) / (kAngleBasisScale * kAngleBasisScale);
+        HalfFrag2 xa[MT];
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) x[mt][e] *= sc;
+            xa[mt] = split8_h(x[mt]);
         }
         f32x4 d1[MT][NT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) d1[mt][nt] = mma_split3(xa[mt], frag(cb, 0, nt), zero4());
+            for (int nt = 0; nt < NT; ++nt) d1[mt][nt] = mma_sc(xa[mt], frag(cb, 0, nt), zero4());
         // stage 2: Z[nu][mu] = C[nu][mu] sum_v Sv[nu][v] Y[v][mu]
         f32x4 d2[MT][NT];
 #pragma unroll
@@ -135,10 +159,10 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                 y[r] = d1[0][nt][r];
                 y[4 + r] = MT > 1 ? d1[MT - 1][nt][r] : 0.f;
             }
-            const SplitFrag3 b = split8_3(y);
+            const HalfFrag2 b = split8_h(y);
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
-                d2[mt][nt] = mma_split3(frag(cb, 2, mt), b, zero4());
+                d2[mt][nt] = mma_cs(frag(cb, 2, mt), b, zero4());
 #pragma unroll
                 for (int r = 0; r < 4; ++r) d2[mt][nt][r] *= cf[mt][nt][r];
             }
@@ -153,9 +177,9 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                 z[r] = d2[0][nt][r];
                 z[4 + r] = MT > 1 ? d2[MT - 1][nt][r] : 0.f;
             }
-            const SplitFrag3 a = split8_3(z);
+            const HalfFrag2 a = split8_h(z);
 #pragma unroll
-            for (int mt = 0; mt < MT; ++mt) d3[nt][mt] = mma_split3(a, frag(cb, 2, mt), zero4());
+            for (int mt = 0; mt < MT; ++mt) d3[nt][mt] = mma_sc(a, frag(cb, 2, mt), zero4());
         }
         // stage 4, transposed: X'^T[a][v] = sum_mu Su[mu][a] W[v][mu] (operands
         // swapped), so each lane holds 4 consecutive a of one row v; in place
@@ -167,11 +191,13 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                 w[r] = d3[0][mt][r];
                 w[4 + r] = NT > 1 ? d3[NT - 1][mt][r] : 0.f;
             }
-            const SplitFrag3 b = split8_3(w);
+            const HalfFrag2 b = split8_h(w);
             const int v = mt * 16 + i;
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
-                const f32x4 d = mma_split3(frag(cb, 1, nt), b, zero4());
+                f32x4 d = mma_cs(frag(cb, 1, nt), b, zero4());
+#pragma unroll
+                for (int r = 0; r < 4; ++r) d[r] *= unscale;
                 const int a0 = nt * 16 + 4 * q;
                 if (V4) {
                     if (v < Nmu && a0 < Nom)

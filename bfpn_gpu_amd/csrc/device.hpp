@@ -1,4 +1,4 @@
-// device.hpp — device buffers and the split-BF16 operand sets (host side).
+// device.hpp — device buffers and the split operand sets of the MFMA kernels (host side).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -80,7 +80,7 @@ struct SourcePlanBuffers {
     }
 };
 
-// Split-BF16 sine bases for the MFMA DST (FP32 state, Nmu, Nom <= 32).
+// Split-FP16 sine bases for the MFMA DST (FP32 state, Nmu, Nom <= 32).
 struct AnglePlanBuffers {
     DeviceBuffer h, m, l;
     AnglePlan plan;

@@ -78,20 +78,21 @@ template <typename T>
 void launch_angle_full(T* F, const T* factors, const T* su, const T* sv, int Ng, int nyz, int Nmu,
                        int Nom, hipStream_t s);
 
-// Split-BF16 sine bases padded to 32x32 (row-major), for the MFMA DST.
-// Three-term BF16 sine bases (hi + mid + lo, split from the FP64 values),
-// 32x32 zero padded, row-major: Su as is, and Su and Sv with their columns in
-// the register-chaining order of the DST kernel.
+// Sine bases of the MFMA DST: 16 x the basis (a power of two, exact), split
+// into three FP16 terms (hi + mid + lo) from the FP64 values, 32x32 zero
+// padded, row-major: Su as is, and Su and Sv with their columns in the
+// register-chaining order of the DST kernel.
+constexpr float kAngleBasisScale = 16.0f;
 struct AnglePlan {
-    const __bf16* su_h = nullptr;
-    const __bf16* su_m = nullptr;
-    const __bf16* su_l = nullptr;
-    const __bf16* sup_h = nullptr;
-    const __bf16* sup_m = nullptr;
-    const __bf16* sup_l = nullptr;
-    const __bf16* svp_h = nullptr;
-    const __bf16* svp_m = nullptr;
-    const __bf16* svp_l = nullptr;
+    const _Float16* su_h = nullptr;
+    const _Float16* su_m = nullptr;
+    const _Float16* su_l = nullptr;
+    const _Float16* sup_h = nullptr;
+    const _Float16* sup_m = nullptr;
+    const _Float16* sup_l = nullptr;
+    const _Float16* svp_h = nullptr;
+    const _Float16* svp_m = nullptr;
+    const _Float16* svp_l = nullptr;
 };
 // K slot 8q + 4u + r of a chained MFMA operand <-> matrix index 16u + 4q + r.
 constexpr int chain_index(int slot) { return 16 * ((slot & 7) >> 2) + 4 * (slot >> 3) + (slot & 3); }

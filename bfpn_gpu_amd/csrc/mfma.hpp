@@ -1,10 +1,12 @@
-// mfma.hpp — split-BF16 matrix-core helpers for CDNA4 (gfx950).
+// mfma.hpp — split-precision matrix-core helpers for CDNA4 (gfx950).
 //
-// A product x*y of FP32 values is evaluated on the BF16 MFMA pipe as
+// Split BF16 (the source GEMMs): a product x*y of FP32 values is evaluated
+// on the BF16 MFMA pipe as
 //     x*y ~= xh*yh + xh*yl + xl*yh,   xh = bf16(x), xl = bf16(x - xh)
 // with FP32 accumulation (v_mfma_f32_16x16x32_bf16, three issues per tile).
 // xh + xl carries 16 significand bits, so each product is accurate to about
-// 2^-16 relative; the dropped xl*yl term is ~2^-18.
+// 2^-16 relative; the dropped xl*yl term is ~2^-18.  Split FP16 (the DST):
+// below.
 //
 // Operand layouts of v_mfma_f32_16x16x32_bf16 (one wave64, q = lane/16):
 //   A (16 x 32, row-major):  lane holds A[lane%16][8q .. 8q+7]
@@ -61,50 +63,59 @@ __device__ inline f32x4 mma_split(const SplitFrag& a, const SplitFrag& b, f32x4 
 
 __device__ inline f32x4 zero4() { return f32x4{0.f, 0.f, 0.f, 0.f}; }
 
-// Three-term split, x = hi + mid + lo exactly for FP32 x (8 + 8 + 8 bits).
-// Used where an operator is applied thousands of times in a row (the DST of
-// every depth step): there a two-term split's ~2^-17 representation error
-// compounds step after step -- for the constant bases, and for the slowly
-// varying state as well (a two-term state split with exact bases still
-// drifts by 2e-3 over 1000 applications) -- a three-term split's does not.
-struct SplitFrag3 {
-    bf16x8 hi, mid, lo;
+// ---- split FP16 (the DST) ----
+// Operators applied thousands of times in a row (the DST of every depth step)
+// need products close to FP32: there a split's representation error
+// compounds step after step.  A two-term BF16 split (16 significand bits)
+// drifts by ~2e-3 over 1000 applications; two FP16 terms carry 22 bits, and
+// a per-plane power-of-two scale keeps the state inside the FP16 range.
+// Operand layouts of v_mfma_f32_16x16x32_f16 are those of the BF16 form.
+typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
+
+// The state: x = hi + lo (+ ~2^-22 |x|); x already scaled into FP16 range.
+struct HalfFrag2 {
+    f16x8 hi, lo;
+};
+// A constant: hi + mid + lo from FP64 (the constant's own error ~2^-33).
+struct HalfFrag3 {
+    f16x8 hi, mid, lo;
 };
 
-__device__ inline SplitFrag3 split8_3(const float (&v)[8]) {
+__device__ inline HalfFrag2 split8_h(const float (&v)[8]) {
     typedef float f32x8 __attribute__((ext_vector_type(8)));
-    typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
-    auto widen = [](bf16x8 b) {  // bf16 -> FP32 from the packed words
-        const u32x4 w = __builtin_bit_cast(u32x4, b);
-        f32x8 h;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            h[2 * k] = __uint_as_float(w[k] << 16);
-            h[2 * k + 1] = __uint_as_float(w[k] & 0xffff0000u);
-        }
-        return h;
-    };
     f32x8 x;
 #pragma unroll
     for (int e = 0; e < 8; ++e) x[e] = v[e];
-    SplitFrag3 f;
-    f.hi = __builtin_convertvector(x, bf16x8);
-    const f32x8 r1 = x - widen(f.hi);
-    f.mid = __builtin_convertvector(r1, bf16x8);
-    f.lo = __builtin_convertvector(r1 - widen(f.mid), bf16x8);
+    HalfFrag2 f;
+    f.hi = __builtin_convertvector(x, f16x8);
+    f.lo = __builtin_convertvector(x - __builtin_convertvector(f.hi, f32x8), f16x8);
     return f;
 }
 
-// acc + A*B keeping every product term down to 2^-16 of hi*hi (six MFMA
-// issues, the smallest terms first).
-__device__ inline f32x4 mma_split3(const SplitFrag3& a, const SplitFrag3& b, f32x4 acc) {
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.mid, b.mid, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.lo, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.lo, b.hi, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.mid, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.mid, b.hi, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.hi, b.hi, acc, 0, 0, 0);
+// acc + A*B with A the state and B a constant, keeping every product term
+// down to ~2^-22 of hi*hi: c_lo x_hi, c_mid x_hi, c_hi x_lo, c_hi x_hi (four
+// MFMA issues, the smallest first; c_mid x_lo ~2^-22 is dropped).
+__device__ inline f32x4 mma_sc(const HalfFrag2& a, const HalfFrag3& b, f32x4 acc) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.hi, b.lo, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.hi, b.mid, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.lo, b.hi, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.hi, b.hi, acc, 0, 0, 0);
     return acc;
+}
+// The same with A the constant and B the state.
+__device__ inline f32x4 mma_cs(const HalfFrag3& a, const HalfFrag2& b, f32x4 acc) {
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.lo, b.hi, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.mid, b.hi, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.hi, b.lo, acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.hi, b.hi, acc, 0, 0, 0);
+    return acc;
+}
+
+// Maximum of v over the wave (every lane gets it).
+__device__ inline float wave_max(float v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+    return v;
 }
 
 // Orders one wave's LDS writes before its later LDS reads of locations other

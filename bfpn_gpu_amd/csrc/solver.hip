@@ -123,26 +123,27 @@ void SourcePlanBuffers::build(const float* kv_f32, const std::vector<double>& e1
     plan.kt_list = kt_list.as<short>();
 }
 
+
 namespace {
-// Round to the nearest BF16 (ties to even), as a bit pattern.
-uint16_t bf16_bits(double v) {
-    const float f = static_cast<float>(v);
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return static_cast<uint16_t>(u >> 16);
+// Round to the nearest FP16 (ties to even, subnormals kept), as a bit pattern.
+uint16_t half_bits(double v) {
+    const _Float16 h = static_cast<_Float16>(static_cast<float>(v));
+    uint16_t b;
+    std::memcpy(&b, &h, 2);
+    return b;
 }
-double bf16_value(uint16_t b) {
-    const uint32_t u = static_cast<uint32_t>(b) << 16;
-    float f;
-    std::memcpy(&f, &u, 4);
-    return f;
+double half_value(uint16_t b) {
+    _Float16 h;
+    std::memcpy(&h, &b, 2);
+    return static_cast<double>(h);
 }
 }  // namespace
 
 void AnglePlanBuffers::build(int Nmu, int Nom, hipStream_t) {
     // Su, Su (chained columns), Sv (chained columns); each zero padded to
-    // 32x32 and split into hi + mid + lo from the FP64 basis.
+    // 32x32, scaled by kAngleBasisScale (exact) and split into FP16
+    // hi + mid + lo from the FP64 basis.  (The scale keeps the smallest
+    // basis values' lo terms clear of the FP16 subnormal range.)
     std::vector<double> pad(3 * 32 * 32, 0.0);
     const auto su = sine_basis(Nom), sv = sine_basis(Nmu);
     for (int r = 0; r < 32; ++r)
@@ -154,17 +155,18 @@ void AnglePlanBuffers::build(int Nmu, int Nom, hipStream_t) {
         }
     std::vector<uint16_t> vh(pad.size()), vm(pad.size()), vl(pad.size());
     for (size_t k = 0; k < pad.size(); ++k) {
-        vh[k] = bf16_bits(pad[k]);
-        const double r1 = pad[k] - bf16_value(vh[k]);
-        vm[k] = bf16_bits(r1);
-        vl[k] = bf16_bits(r1 - bf16_value(vm[k]));
+        const double x = kAngleBasisScale * pad[k];
+        vh[k] = half_bits(x);
+        const double r1 = x - half_value(vh[k]);
+        vm[k] = half_bits(r1);
+        vl[k] = half_bits(r1 - half_value(vm[k]));
     }
     upload(h, vh);
     upload(m, vm);
     upload(l, vl);
-    const __bf16* H = h.as<__bf16>();
-    const __bf16* M = m.as<__bf16>();
-    const __bf16* L = l.as<__bf16>();
+    const _Float16* H = h.as<_Float16>();
+    const _Float16* M = m.as<_Float16>();
+    const _Float16* L = l.as<_Float16>();
     plan.su_h = H;
     plan.su_m = M;
     plan.su_l = L;
