@@ -7,9 +7,10 @@ Tolerances, relative to the largest magnitude of the reference output:
   FP32 state: energy 2e-6, transport 5e-5, IDD 1e-6; source and angle 1e-4
               (both run on the matrix cores with FP32 accumulation: the
               source in split-BF16, hi + lo, ~2^-16 relative per product;
-              the angle in a three-term BF16 split of both operands, close
-              to FP32 products; a single-precision limiter can take the
-              other branch of the superbee kink in isolated cells).
+              the angle in split FP16, the state in two terms under a
+              per-plane power-of-two scale and the bases in three, close to
+              FP32 products; a single-precision limiter can take the other
+              branch of the superbee kink in isolated cells).
 """
 import numpy as np
 import pytest
@@ -198,7 +199,8 @@ def test_angle_repeated_does_not_drift(gpu, Nmu, Nom):
     """The DST runs once per depth step on the same beam, so a systematic
     operator error compounds.  1000 FP32 applications to a near-delta angular
     beam with the weak diffusion of a 230 MeV proton keep the mass within 2e-4
-    of the FP64 operator (two-term BF16 splits drift by ~7e-3 here)."""
+    of the FP64 operator (two-term BF16 splits drift by ~7e-3 here; the
+    kernel's two-term FP16 state split under a per-plane scale does not)."""
     nyz, Ng = 2, 2
     F = np.zeros((Ng, nyz, Nmu, Nom))
     F[:, :, Nmu // 2 - 1:Nmu // 2 + 1, Nom // 2 - 1:Nom // 2 + 1] = 1.0
