@@ -13,6 +13,7 @@
 // FP32 state: matrix cores, stages chained in registers (below).  FP64 state
 // or planes wider than 32: vector kernel with the plane and bases in LDS.
 
+#include "buffer.hpp"
 #include "kernels.hpp"
 #include "mfma.hpp"
 
@@ -67,10 +68,33 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, i = lane & 15, q = lane >> 4;
-    int p = (blockIdx.x * 4 + (threadIdx.x >> 6)) * per_wave;
+    // (the wave's first plane, made visibly wave-uniform for the compiler)
+    int p = __builtin_amdgcn_readfirstlane((blockIdx.x * 4 + (threadIdx.x >> 6)) * per_wave);
     const int p_end = min(planes, p + per_wave);
     if (p >= p_end) return;
     const int na = Nmu * Nom;
+    // The wave's run of planes as a buffer resource: a plane is a uniform
+    // offset, the lane's elements fixed 32-bit offsets (V4: two 16-byte loads
+    // per row tile, and one 16-byte store per output tile, kOutside where
+    // the tile lies outside the plane).
+    const int p0 = p;
+    const __amdgpu_buffer_rsrc_t Fw =
+        make_rsrc(F + static_cast<long long>(p0) * na,
+                  static_cast<unsigned>(static_cast<long long>(p_end - p0) * na * sizeof(float)));
+    unsigned ld_off[MT][2], st_off[MT][NT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            ld_off[mt][h] = static_cast<unsigned>(
+                (min(mt * 16 + i, Nmu - 1) * Nom + min(8 * q + 4 * h, max(Nom - 4, 0))) * sizeof(float));
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            const int v = mt * 16 + i, a0 = nt * 16 + 4 * q;
+            st_off[mt][nt] = (v < Nmu && a0 < Nom) ? static_cast<unsigned>((v * Nom + a0) * sizeof(float))
+                                                   : kOutside;
+        }
+    }
     // Fragment of constant `a` (0 Su, 1 Su chained, 2 Sv chained), tile t,
     // at compile-time offsets from the lane's base (row i, columns 8q..).
     auto frag = [&](const _Float16* base, int a, int t) {
@@ -88,6 +112,7 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
         asm volatile("v_mov_b32 %0, 0" : "=v"(oz));
         const _Float16* cb = &cst[0][i][8 * q] + oz;
         float* X = F + static_cast<long long>(p) * na;
+        const unsigned so = static_cast<unsigned>((p - p0) * na * static_cast<int>(sizeof(float)));
         const int g = p / nyz;
         if (g != g_cur) {  // CN factors of this group, in the D layout, / 256
             g_cur = g;
@@ -114,8 +139,7 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
             if (V4) {
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
-                    const float4 t =
-                        *reinterpret_cast<const float4*>(row + min(8 * q + 4 * h, Nom - 4));
+                    const float4 t = bload4(Fw, ld_off[mt][h], so);
                     x[mt][4 * h] = t.x;
                     x[mt][4 * h + 1] = t.y;
                     x[mt][4 * h + 2] = t.z;
@@ -200,8 +224,7 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                 for (int r = 0; r < 4; ++r) d[r] *= unscale;
                 const int a0 = nt * 16 + 4 * q;
                 if (V4) {
-                    if (v < Nmu && a0 < Nom)
-                        *reinterpret_cast<float4*>(X + v * Nom + a0) = make_float4(d[0], d[1], d[2], d[3]);
+                    bstore4(make_float4(d[0], d[1], d[2], d[3]), Fw, st_off[mt][nt], so);
                 } else {
 #pragma unroll
                     for (int r = 0; r < 4; ++r)

@@ -30,6 +30,7 @@
 //    reference's clamped stencil taps: the slope of a boundary cell is 0 and
 //    the cell outside reads the boundary value with slope 0.
 
+#include "buffer.hpp"
 #include "kernels.hpp"
 
 #include <type_traits>
@@ -88,32 +89,6 @@ __device__ inline T divergence(T zc, T near, T S_c, T S_n, T zm, T zp, T sz_m, T
     return cy * fy + cz * fz;
 }
 
-// One energy group's block of the state is addressed through a buffer
-// resource (base and size in SGPRs): the row offset is a uniform scalar, the
-// lane's column a 32-bit vector offset, so no address arithmetic runs on the
-// vector ALUs.  0x00020000: the raw 32-bit data format word of gfx9-family
-// buffer resources.
-__device__ inline __amdgpu_buffer_rsrc_t group_rsrc(const void* base, unsigned bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, static_cast<int>(bytes),
-                                             0x00020000);
-}
-template <typename T>
-__device__ inline T bload(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff);
-template <>
-__device__ inline float bload<float>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
-}
-template <>
-__device__ inline double bload<double>(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-}
-__device__ inline void bstore(float v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, voff, soff, 0);
-}
-__device__ inline void bstore(double v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-    typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
-}
 
 // grid: (ceil(nang / MW), Ng), block: MW * R threads (R y columns per pass,
 // P passes: the thread's columns are y = ry + j R).  LDS, stored y-major
@@ -169,13 +144,14 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
     const T hh = T(0.5) * h;
     const long long zstride = static_cast<long long>(Ny + 1) * nang;
     const unsigned zbytes = static_cast<unsigned>(zstride * sizeof(T));  // one z row
-    // the group's block from direction m0 on (uniform); lanes add 32-bit offsets
+    // the group's block from direction m0 on, as a buffer resource: uniform
+    // row offsets, 32-bit lane offsets
     const __amdgpu_buffer_rsrc_t Fg =
-        group_rsrc(F + static_cast<long long>(g) * zstride * (Nz + 1) + m0,
+        make_rsrc(F + static_cast<long long>(g) * zstride * (Nz + 1) + m0,
                    static_cast<unsigned>((zstride * (Nz + 1) - m0) * sizeof(T)));
 
     unsigned boff[P];  // byte offset of the lane's column in a plane row
-    unsigned soff[P];  // the same for stores; 0x80000000 (out of range) for copies
+    unsigned soff[P];  // the same for stores; kOutside (dropped) for copies
     int o0[P], o_near[P], o_m[P], o_p[P];  // LDS column: own, upwind, y-1, y+1 (clamped)
     T win[P][8];    // psi rows t-1 .. t+2 after the commit of step t, then
                     // rows t+3 .. t+6 in flight (slot: row mod 8)
@@ -190,7 +166,7 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
         const int y = ry + j * R;
         const int yc = min(y, Ny);
         boff[j] = static_cast<unsigned>((yc * nang + (m - m0)) * sizeof(T));
-        soff[j] = (y <= Ny && live) ? boff[j] : 0x80000000u;
+        soff[j] = (y <= Ny && live) ? boff[j] : kOutside;
         auto lo = [&](int yy) { return (min(max(yy, 0), Ny)) * YP + mm; };
         o0[j] = lo(yc);
         o_near[j] = lo(py ? yc - 1 : yc + 1);
