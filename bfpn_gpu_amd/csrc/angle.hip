@@ -51,7 +51,7 @@ template <int MT, int NT, bool V4>
 __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                                                          const float* __restrict__ factors,
                                                          AnglePlan plan, int planes, int per_wave,
-                                                         int nyz, int Nmu, int Nom) {
+                                                         int nyz, int Nmu, int Nom, float one) {
     // The nine constant basis arrays (Su, Su chained, Sv chained; hi, mid,
     // lo) live in LDS, shared by the work-group's four waves and read per
     // stage: held in registers they would cost 72 VGPRs and half the
@@ -108,6 +108,8 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
     float cf[MT][NT][4];
     for (; p < p_end; ++p) {
         // an opaque zero per plane keeps the constant reads inside the loop
+        // (`one`, a kernel argument equal to 1, keeps the residuals of the
+        // splits in stages 2-4 single FMAs)
         int oz;
         asm volatile("v_mov_b32 %0, 0" : "=v"(oz));
         const _Float16* cb = &cst[0][i][8 * q] + oz;
@@ -163,11 +165,7 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
         const float unscale = __int_as_float((254 - sb) << 23) / (kAngleBasisScale * kAngleBasisScale);
         HalfFrag2 xa[MT];
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-#pragma unroll
-            for (int e = 0; e < 8; ++e) x[mt][e] *= sc;
-            xa[mt] = split8_h(x[mt]);
-        }
+        for (int mt = 0; mt < MT; ++mt) xa[mt] = split8_h<true>(x[mt], sc);
         f32x4 d1[MT][NT];
 #pragma unroll
         for (int mt = 0; mt < MT; ++mt)
@@ -183,7 +181,7 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                 y[r] = d1[0][nt][r];
                 y[4 + r] = MT > 1 ? d1[MT - 1][nt][r] : 0.f;
             }
-            const HalfFrag2 b = split8_h(y);
+            const HalfFrag2 b = split8_h<false>(y, one);
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) {
                 d2[mt][nt] = mma_cs(frag(cb, 2, mt), b, zero4());
@@ -201,7 +199,7 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                 z[r] = d2[0][nt][r];
                 z[4 + r] = MT > 1 ? d2[MT - 1][nt][r] : 0.f;
             }
-            const HalfFrag2 a = split8_h(z);
+            const HalfFrag2 a = split8_h<false>(z, one);
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt) d3[nt][mt] = mma_sc(a, frag(cb, 2, mt), zero4());
         }
@@ -215,7 +213,7 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                 w[r] = d3[0][mt][r];
                 w[4 + r] = NT > 1 ? d3[NT - 1][mt][r] : 0.f;
             }
-            const HalfFrag2 b = split8_h(w);
+            const HalfFrag2 b = split8_h<false>(w, one);
             const int v = mt * 16 + i;
 #pragma unroll
             for (int nt = 0; nt < NT; ++nt) {
@@ -318,7 +316,7 @@ void launch_angle_mfma(float* F, const float* factors, const AnglePlan& plan, in
     const int mt = Nmu > 16 ? 2 : 1, nt = Nom > 16 ? 2 : 1;
 #define BFPN_ANGLE(MT_, NT_, V4_)                                                              \
     hipLaunchKernelGGL((angle_mfma_kernel<MT_, NT_, V4_>), grid, block, 0, s, F, factors, plan,\
-                       static_cast<int>(planes), per_wave, nyz, Nmu, Nom)
+                       static_cast<int>(planes), per_wave, nyz, Nmu, Nom, 1.0f)
 #define BFPN_ANGLE_V(MT_, NT_)                                                                 \
     if (Nom % 4 == 0) BFPN_ANGLE(MT_, NT_, true);                                              \
     else BFPN_ANGLE(MT_, NT_, false)

@@ -81,14 +81,26 @@ struct HalfFrag3 {
     f16x8 hi, mid, lo;
 };
 
-__device__ inline HalfFrag2 split8_h(const float (&v)[8]) {
+// Splits 8 values v * s (SCALE) or v.  The residual is formed as
+// fma(v, s, -hi), one mixed-precision FMA per value (hi read as FP16).
+// Unscaled, s must be a 1.0 the compiler cannot see through (a kernel
+// argument), or it forms the residual as a conversion plus a subtraction.
+template <bool SCALE>
+__device__ inline HalfFrag2 split8_h(const float (&v)[8], float s) {
     typedef float f32x8 __attribute__((ext_vector_type(8)));
-    f32x8 x;
+    f32x8 x, xs;
 #pragma unroll
-    for (int e = 0; e < 8; ++e) x[e] = v[e];
+    for (int e = 0; e < 8; ++e) {
+        x[e] = v[e];
+        xs[e] = SCALE ? v[e] * s : v[e];
+    }
     HalfFrag2 f;
-    f.hi = __builtin_convertvector(x, f16x8);
-    f.lo = __builtin_convertvector(x - __builtin_convertvector(f.hi, f32x8), f16x8);
+    f.hi = __builtin_convertvector(xs, f16x8);
+    const f32x8 h = __builtin_convertvector(f.hi, f32x8);
+    f32x8 r;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) r[e] = __builtin_fmaf(x[e], s, -h[e]);
+    f.lo = __builtin_convertvector(r, f16x8);
     return f;
 }
 
