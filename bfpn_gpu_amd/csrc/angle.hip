@@ -106,7 +106,8 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
 
     int g_cur = -1;
     float cf[MT][NT][4];
-    for (; p < p_end; ++p) {
+    int g = p / nyz, s_in = p - g * nyz;  // group and node of plane p (uniform)
+    for (; p < p_end; ++p, (++s_in == nyz) ? (s_in = 0, ++g) : 0) {
         // an opaque zero per plane keeps the constant reads inside the loop
         // (`one`, a kernel argument equal to 1, keeps the residuals of the
         // splits in stages 2-4 single FMAs)
@@ -115,7 +116,6 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
         const _Float16* cb = &cst[0][i][8 * q] + oz;
         float* X = F + static_cast<long long>(p) * na;
         const unsigned so = static_cast<unsigned>((p - p0) * na * static_cast<int>(sizeof(float)));
-        const int g = p / nyz;
         if (g != g_cur) {  // CN factors of this group, in the D layout, / 256
             g_cur = g;
             const float* C = factors + static_cast<long long>(g) * na;
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
         for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
             for (int e = 0; e < 8; ++e) mx = fmaxf(mx, fabsf(x[mt][e]));
-        const int ebits = __builtin_amdgcn_readfirstlane(__float_as_int(wave_max(mx))) >> 23;
+        const int ebits = __float_as_int(wave_max_nonneg(mx)) >> 23;
         const int sb = min(max(258 - ebits, 1), 253);
         const float sc = __int_as_float(sb << 23);
         const float unscale = __int_as_float((254 - sb) << 23) / (kAngleBasisScale * kAngleBasisScale);

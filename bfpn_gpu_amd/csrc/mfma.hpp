@@ -123,11 +123,20 @@ __device__ inline f32x4 mma_cs(const HalfFrag3& a, const HalfFrag2& b, f32x4 acc
     return acc;
 }
 
-// Maximum of v over the wave (every lane gets it).
-__device__ inline float wave_max(float v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
-    return v;
+// Maximum of non-negative v over the wave, as a wave-uniform value: DPP
+// maxima within each row of 16 lanes (neighbour pairs, quads, half-row and
+// row mirrors), then the four rows' results read into scalar registers.  On
+// the bit patterns: for non-negative floats the integer order is the
+// numeric order.
+__device__ inline float wave_max_nonneg(float v) {
+    unsigned u = __float_as_uint(v);
+    u = max(u, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(u), 0xB1, 0xF, 0xF, false)));
+    u = max(u, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(u), 0x4E, 0xF, 0xF, false)));
+    u = max(u, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(u), 0x141, 0xF, 0xF, false)));
+    u = max(u, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(u), 0x140, 0xF, 0xF, false)));
+    const unsigned r0 = __builtin_amdgcn_readlane(u, 0), r1 = __builtin_amdgcn_readlane(u, 16);
+    const unsigned r2 = __builtin_amdgcn_readlane(u, 32), r3 = __builtin_amdgcn_readlane(u, 48);
+    return __uint_as_float(max(max(r0, r1), max(r2, r3)));
 }
 
 // Orders one wave's LDS writes before its later LDS reads of locations other
