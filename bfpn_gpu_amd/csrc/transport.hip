@@ -78,14 +78,18 @@ __device__ inline T limited_slope(T a, T b) {
 // cell (S_c) and of its upwind neighbour (S_n), the upwind frame's slopes are
 // +-S (superbee is odd and symmetric, and the 1e-10 test goes to the
 // physical forward difference either way), so the y term is
-// (c - n) + hy (S_c - S_n) with hy = +-1/2 the sign of wy.  z: physical
-// window (zm, zc, zp) and carried physical slopes of rows -1, 0, +1; hz = +-1/2
-// with the sign of wz.  cy = |wy| / dy, cz = |wz| / dz.
+// (c - n) + hy (S_c - S_n) with hy = +-1/2 the sign of wy.  z: the same with
+// the row window (zm, zc, zp) and the physical slopes of rows -1, 0, +1,
+// given as differences: dz_b = zc - zm, dz_f = zp - zc, dS_b = sz_0 - sz_m,
+// dS_f = sz_p - sz_0 (each is also the other row's difference, so the march
+// forms it once); the upwind terms zc - z_up and sz_0 - sz_up are dz_b, dS_b
+// for wz >= 0 and -dz_f, -dS_f (exact negations) otherwise.  hz = +-1/2 with
+// the sign of wz; cy = |wy| / dy, cz = |wz| / dz.
 template <typename T>
-__device__ inline T divergence(T zc, T near, T S_c, T S_n, T zm, T zp, T sz_m, T sz_0, T sz_p,
-                               T cy, T cz, T hy, T hz, bool pz) {
+__device__ inline T divergence(T zc, T near, T S_c, T S_n, T dz_b, T dz_f, T dS_b, T dS_f, T cy,
+                               T cz, T hy, T hz, bool pz) {
     const T fy = fma(hy, S_c - S_n, zc - near);
-    const T fz = fma(hz, sz_0 - (pz ? sz_m : sz_p), zc - (pz ? zm : zp));
+    const T fz = fma(hz, pz ? dS_b : -dS_f, pz ? dz_b : -dz_f);
     return cy * fy + cz * fz;
 }
 
@@ -273,9 +277,11 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
 #pragma unroll
             for (int j = 0; j < P; ++j) {
                 const T zm = win[j][(k8 + 7) & 7], zc = win[j][k8], zp = win[j][(k8 + 1) & 7];
-                const T s_next = limited_slope(win[j][(k8 + 2) & 7] - zp, zp - zc);
-                const T L0 = divergence(zc, pn[j], wS[j][k], pS[j], zm, zp, sz[j][(k + 1) & 1],
-                                        sz[j][k & 1], s_next, cy, cz, hy, hz, pz);
+                const T dz_b = zc - zm, dz_f = zp - zc;
+                const T s_next = limited_slope(win[j][(k8 + 2) & 7] - zp, dz_f);
+                const T sz_m = sz[j][(k + 1) & 1], sz_0 = sz[j][k & 1];
+                const T L0 = divergence(zc, pn[j], wS[j][k], pS[j], dz_b, dz_f, sz_0 - sz_m,
+                                        s_next - sz_0, cy, cz, hy, hz, pz);
                 T v = zc - h * L0;
                 if (CLIP) v = fmax(v, T(0));
                 prow[o0[j]] = v;
@@ -292,10 +298,11 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
             for (int j = 0; j < P; ++j) {
                 const T zm = wpred[j][(k + 1) & 3], zc = wpred[j][(k + 2) & 3],
                         zp = wpred[j][(k + 3) & 3];
-                const T s_next = limited_slope(wpred[j][k] - zp, zp - zc);
-                const T L1 = divergence(zc, cn_[j], wSp[j][k & 1], cS[j], zm, zp,
-                                        szp[j][(k + 1) & 1], szp[j][k & 1], s_next, cy, cz, hy,
-                                        hz, pz);
+                const T dz_b = zc - zm, dz_f = zp - zc;
+                const T s_next = limited_slope(wpred[j][k] - zp, dz_f);
+                const T sz_m = szp[j][(k + 1) & 1], sz_0 = szp[j][k & 1];
+                const T L1 = divergence(zc, cn_[j], wSp[j][k & 1], cS[j], dz_b, dz_f, sz_0 - sz_m,
+                                        s_next - sz_0, cy, cz, hy, hz, pz);
                 T v = wbase[j][(k + 2) & 3] - hh * L1;
                 if (CLIP) v = fmax(v, T(0));
                 if (real) bstore(v, Fg, soff[j], so);
