@@ -1,0 +1,35 @@
+"""The harness runners driving the real bfp_solver on the GPU (small case)."""
+import math
+import os
+
+import pytest
+
+from bfpn_gpu_amd import harness
+
+pytestmark = pytest.mark.gpu
+
+
+def test_convergence_runner_on_the_solver(tmp_path, gpu):
+    """Depth-step refinement of a small water case through the CLI: every
+    coarser IDD is closer to the finest as Nx grows, with a positive observed
+    order (the measured orders on the full single-beam case are stated with
+    the design's measurements)."""
+    assert os.path.exists(harness.DEFAULT_EXE)
+    rows = harness.convergence(harness.DEFAULT_EXE, tmp_path, nxs=(200, 400, 800, 1600),
+                               time=8.0, lx=8.0, material="water", energy=100, ny=4, nz=4,
+                               ng=100, nmu=4, nom=4, flags=("--eq15-straggling",),
+                               extra=("--ly", "4", "--lz", "4"))
+    errs = [r["rel_l2"] for r in rows]
+    assert all(math.isfinite(e) and e > 0 for e in errs), errs
+    assert errs[0] > errs[1] > errs[2], errs
+    assert all(r["order_to_next"] > 0.5 for r in rows[:-1]), rows
+
+
+def test_table2_runner_on_one_case(tmp_path, gpu):
+    """One Table 2 case at a coarse depth grid: the runner produces all four
+    metrics and their distance from the paper's values."""
+    rows = harness.table2(harness.DEFAULT_EXE, tmp_path, nx=500, time=10.0,
+                          cases=(("water", 100),))
+    (r,) = rows
+    assert all(math.isfinite(r["metrics_cm"][k]) for k in ("BP", "P90", "D90", "D20")), r
+    assert 6.0 < r["metrics_cm"]["BP"] < 9.0, r
