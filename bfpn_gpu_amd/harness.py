@@ -186,7 +186,10 @@ def main(argv=None) -> int:
     t.add_argument("--time", type=float, default=40.0)
     t.add_argument("--outdir", default="table2_runs")
     t.add_argument("--paper-grid", action="store_true")
+    t.add_argument("--materials", default="water,bone", help="subset of the cases, e.g. water")
     t.add_argument("--skip-run", action="store_true")
+    t.add_argument("extra", nargs=argparse.REMAINDER,
+                   help="further solver arguments after --, e.g. -- --ly 8 --lz 8 --sigma-yz 0.3")
     c = sub.add_parser("convergence", help="depth-step convergence and observed order")
     c.add_argument("--exe", default=str(DEFAULT_EXE))
     c.add_argument("--out", default="convergence")
@@ -206,7 +209,9 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     extra = [e for e in getattr(a, "extra", []) if e != "--"]
     if a.cmd == "table2":
-        rows = table2(a.exe, a.outdir, a.nx, a.time, a.paper_grid, skip_run=a.skip_run)
+        mats = set(a.materials.split(","))
+        rows = table2(a.exe, a.outdir, a.nx, a.time, a.paper_grid, extra, a.skip_run,
+                      cases=tuple(c for c in TABLE2_CASES if c[0] in mats))
         for r in rows:
             d = r["diff_mm"]
             print(f"{r['material']:5s} {r['energy']:3d} MeV: " + "  ".join(
