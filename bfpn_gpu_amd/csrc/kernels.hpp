@@ -36,7 +36,7 @@ struct SourcePlan {
     // Kv block sparsity of the angular GEMM: per (g, 64-direction tile) the
     // 32-wide K tiles holding a nonzero entry, ascending.
     const int* kt_count = nullptr;    // [Ng][mtiles]
-    const short* kt_list = nullptr;   // [Ng][mtiles][ktiles]
+    const int* kt_list = nullptr;     // [Ng][mtiles][ktiles] (32-bit: scalar loads)
 };
 int source_nap(int nang);    // nang padded to 32
 int source_ngp(int Ng);      // Ng padded to 32 (E columns)

@@ -78,10 +78,10 @@ void SourcePlanBuffers::build(const float* kv_f32, const std::vector<double>& e1
         BFPN_HIP(hipMemcpyAsync(flags.data(), scratch.as<void>(), nb, hipMemcpyDeviceToHost, s));
         BFPN_HIP(hipStreamSynchronize(s));
         std::vector<int> count(static_cast<size_t>(Ng) * mtiles, 0);
-        std::vector<short> list(nb, 0);
+        std::vector<int> list(nb, 0);
         for (size_t b = 0; b < count.size(); ++b)
             for (int k = 0; k < ktiles; ++k)
-                if (flags[b * ktiles + k]) list[b * ktiles + count[b]++] = static_cast<short>(k);
+                if (flags[b * ktiles + k]) list[b * ktiles + count[b]++] = k;
         upload(kt_count, count);
         upload(kt_list, list);
     }
@@ -120,7 +120,7 @@ void SourcePlanBuffers::build(const float* kv_f32, const std::vector<double>& e1
     plan.e2l = e2l.as<__bf16>();
     plan.tile_band = band.as<int2>();
     plan.kt_count = kt_count.as<int>();
-    plan.kt_list = kt_list.as<short>();
+    plan.kt_list = kt_list.as<int>();
 }
 
 

@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256) void source_angular_mfma(const float* __restri
                                                            const __bf16* __restrict__ kth,
                                                            const __bf16* __restrict__ ktl,
                                                            const int* __restrict__ kt_count,
-                                                           const short* __restrict__ kt_list,
+                                                           const int* __restrict__ kt_list,
                                                            float* __restrict__ A1,
                                                            float* __restrict__ A2, int nyz,
                                                            int nang, int nap, int tiles) {
@@ -78,14 +78,15 @@ __global__ __launch_bounds__(256) void source_angular_mfma(const float* __restri
     auto load = [&](int k0) {
         const long long base = plane + static_cast<long long>(srow) * nang + k0 + lk;
         if (VEC) {
-            const bool in = srow < nyz && k0 + lk < nang;
-            float4 a0 = make_float4(0.f, 0.f, 0.f, 0.f), a1 = a0, b0 = a0, b1 = a0;
-            if (in) {  // (a select of the loads would take the zero's address)
-                a0 = *reinterpret_cast<const float4*>(Fp + base);
-                a1 = *reinterpret_cast<const float4*>(Fp + base + 4);
-                b0 = *reinterpret_cast<const float4*>(fp + base);
-                b1 = *reinterpret_cast<const float4*>(fp + base + 4);
-            }
+            // Unmasked: rows past nyz read the last row (their products only
+            // reach output rows that are never stored) and columns past nang
+            // read the row's last 8 values (Kv^T is zero there).
+            const long long cb = plane + static_cast<long long>(min(srow, nyz - 1)) * nang +
+                                 min(k0 + lk, nang - 8);
+            const float4 a0 = *reinterpret_cast<const float4*>(Fp + cb);
+            const float4 a1 = *reinterpret_cast<const float4*>(Fp + cb + 4);
+            const float4 b0 = *reinterpret_cast<const float4*>(fp + cb);
+            const float4 b1 = *reinterpret_cast<const float4*>(fp + cb + 4);
             v1[0] = a0.x; v1[1] = a0.y; v1[2] = a0.z; v1[3] = a0.w;
             v1[4] = a1.x; v1[5] = a1.y; v1[6] = a1.z; v1[7] = a1.w;
             v2[0] = b0.x; v2[1] = b0.y; v2[2] = b0.z; v2[3] = b0.w;
@@ -98,20 +99,17 @@ __global__ __launch_bounds__(256) void source_angular_mfma(const float* __restri
                 v2[e] = in ? fp[base + e] : 0.f;
             }
         }
-        if (mrow < nap) {
-            const long long off = kt_plane + static_cast<long long>(mrow) * nap + k0 + lk;
+        {  // rows past nap read the last row: their columns are never stored
+            const long long off = kt_plane + static_cast<long long>(min(mrow, nap - 1)) * nap + k0 + lk;
             kh = *reinterpret_cast<const bf16x8*>(kth + off);
             kl = *reinterpret_cast<const bf16x8*>(ktl + off);
-        } else {
-#pragma unroll
-            for (int e = 0; e < 8; ++e) kh[e] = kl[e] = static_cast<__bf16>(0.f);
         }
     };
     // only the K tiles where this direction tile's Kv block is nonzero (the
     // skipped products are exact zeros)
     const int mt_idx = m0 / 64, ktiles = nap / 32;
     const int nk = kt_count[g * mtiles + mt_idx];
-    const short* kl_ = kt_list + static_cast<long long>(g * mtiles + mt_idx) * ktiles;
+    const int* kl_ = kt_list + static_cast<long long>(g * mtiles + mt_idx) * ktiles;
     if (nk > 0) load(kl_[0] * 32);
     for (int j = 0; j < nk; ++j) {
         {  // stage: rows s0+lrow / m0+lrow, columns k0+lk .. +7
