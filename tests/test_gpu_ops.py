@@ -194,6 +194,31 @@ def test_angle_two_tile_planes(gpu, oracle_mod, Nmu, Nom):
     _close(dF.numpy().astype(np.float64), F, TOL32["angle"])
 
 
+@pytest.mark.parametrize("Nmu,Nom", [(20, 20), (9, 13)])
+def test_angle_planes_across_magnitudes(gpu, oracle_mod, Nmu, Nom):
+    """The FP32 DST scales each plane by a power of two into the FP16 range of
+    its split operands: planes from 1e-30 to 1e+30 (and all-zero planes) next
+    to each other keep their own relative accuracy."""
+    scales = [1e-30, 1e-12, 1.0, 3e5, 1e30, 0.0]
+    Ng, nyz = 2, len(scales)
+    rng = np.random.default_rng(23)
+    planes = rng.standard_normal((Ng, nyz, Nmu * Nom))
+    planes *= np.array(scales)[None, :, None]
+    F = planes.astype(np.float32).astype(np.float64).ravel()
+    sig = rng.random(Ng) * 1e-3
+    dF = _dev(gpu, F, np.float32)
+    gpu.ops.angle_full(dF, sig, nyz, Ng, Nmu, Nom, 0.1, 0.07, 0.02, 1.85)
+    oracle_mod.angle_full(F, sig, nyz, Ng, Nmu, Nom, 0.1, 0.07, 0.02, 1.85)
+    got = dF.numpy().astype(np.float64).reshape(Ng, nyz, -1)
+    ref = F.reshape(Ng, nyz, -1)
+    for g in range(Ng):
+        for s, sc in enumerate(scales):
+            if sc == 0.0:
+                assert not np.any(got[g, s]), (g, s)
+                continue
+            _close(got[g, s], ref[g, s], TOL32["angle"])
+
+
 @pytest.mark.parametrize("Nmu,Nom", [(20, 20), (7, 7)])
 def test_angle_repeated_does_not_drift(gpu, Nmu, Nom):
     """The DST runs once per depth step on the same beam, so a systematic
