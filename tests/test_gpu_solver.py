@@ -7,7 +7,7 @@ Tolerances:
       summation order, compounded over the steps);
   FP64 full golden run (1000 steps):  IDD within 1e-8 relative pointwise where
       IDD >= 1e-6 of its max, metrics within 1e-4 cm;
-  FP32 state (north_star gate):       IDD within 0.5 % pointwise where
+  FP32 state (the accuracy gate):     IDD within 0.5 % pointwise where
       IDD >= 10 % of max; BP, D90, D20 within 0.01 cm (0.1 mm).
 """
 import os
