@@ -39,7 +39,7 @@ typedef enum bfpn_status {
     BFPN_E_NOMEM = 2,       /* device memory preflight or allocation failed */
     BFPN_E_HIP = 3,         /* HIP runtime error                            */
     BFPN_E_IO = 4,          /* table file missing or short                  */
-    BFPN_E_UNSUPPORTED = 5  /* option not available in this build           */
+    BFPN_E_UNSUPPORTED = 5  /* option not supported by this library         */
 } bfpn_status;
 
 typedef enum bfpn_precision {
