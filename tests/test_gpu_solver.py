@@ -142,6 +142,23 @@ def test_paired_energy_sweeps_are_bit_identical(gpu, precision, flags):
             np.testing.assert_array_equal(one.state(k), many.state(k))
 
 
+@pytest.mark.parametrize("flags", [{"eq15_straggling": 1}, {"primary_only": 1}],
+                         ids=lambda f: "-".join(sorted(f)))
+def test_paired_energy_sweeps_are_bit_identical_wide_lanes(gpu, flags):
+    """2^18 lanes (32 x 32 nodes x 16 x 16 directions): the FP32 sweeps run
+    four adjacent lanes per thread, paired and alone; bit for bit the same."""
+    case = small_case(Ny=31, Nz=31, Nmu=16, Nom=16, Ng=40, flags=flags)
+    assert (case.Ny + 1) * (case.Nz + 1) * case.Nmu * case.Nom == 1 << 18
+    with gpu.Solver(case, gpu.FP32) as one, gpu.Solver(case, gpu.FP32) as many:
+        for _ in range(3):
+            one.advance(1)
+        many.advance(3)
+        for a, b in zip(one.history(), many.history()):
+            np.testing.assert_array_equal(a, b)
+        for k in range(4):
+            np.testing.assert_array_equal(one.state(k), many.state(k))
+
+
 def test_history_and_restart_are_consistent(gpu):
     """advance(a)+advance(b) equals advance(a+b); write_state round-trips."""
     case = small_case(Ny=3, Nz=3)
