@@ -33,8 +33,9 @@ inline int xcd_grid(long long tiles) { return static_cast<int>((tiles + 7) / 8 *
 
 // Angular GEMM.  Logical tile = (g, 64 transverse nodes, 64 directions); the
 // 7 x 7 tiles of one source group g share X[g] and Kv[g] through one L2.
-// Kv is thresholded (about half of its 64 x 32 blocks are zero at 20 x 20
-// directions), so each tile visits only its nonzero K tiles.
+// Kv is thresholded (at 20 x 20 directions, water 230 MeV, 58 % of its
+// 64 x 32 blocks are zero in the groups that have a kernel, all of them in
+// the others), so each tile visits only its nonzero K tiles.
 // 4 waves, each a 32x32 output block of both inputs.  Per 32-wide K step the
 // work-group stages a 64(s) x 32(m') tile of both inputs -- split into BF16
 // hi/lo once -- and the matching 64(m) x 32(m') tile of the pre-split
