@@ -17,7 +17,8 @@ Prints ONE JSON line.  Fields beyond the common contract:
                 instrumented copy of the timed region; peak 8 TB/s.  traffic:
                 HBM bytes per launch of that operator measured with rocprofv3
                 FETCH_SIZE / WRITE_SIZE (calibrated on a plain copy), read from
-                the committed summary profiles/hbm_traffic_single_beam.md
+                the committed summaries profiles/hbm_traffic_single_beam.md and
+                profiles/hbm_traffic_paper_grid.md (FP32 state only)
   phases        per-operator seconds/calls/GB/s of that instrumented run
   cpu_baseline  the CPU FP64 oracle on this host (all cores), a bounded sample
                 of the same workload: the same physics on a 5x5-node
@@ -54,7 +55,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 STEP_BYTES_PER_CELL = 176.0  # algorithmic HBM bytes per cell-update of the step model
 METRIC = "phase-space cell-updates/s; beamlet solves/s + p99 latency at IDD err ≤0.5%"
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_single_beam.md")
+TRAFFIC_FILES = {"beam": os.path.join(ROOT, "profiles", "hbm_traffic_single_beam.md"),
+                 "paper": os.path.join(ROOT, "profiles", "hbm_traffic_paper_grid.md")}
 
 
 def log(msg):
@@ -94,10 +96,10 @@ def cpu_baseline(case, threads: int | None):
                       f"steps (primary+secondary) after an untimed one, {dt:.2f} s"}
 
 
-def load_traffic(phase: str):
-    """HBM bytes per launch of `phase` from the committed rocprofv3 summary."""
+def load_traffic(grid: str, phase: str):
+    """HBM bytes per launch of `phase` on `grid` from the committed rocprofv3 summary."""
     try:
-        text = open(TRAFFIC_FILE).read()
+        text = open(TRAFFIC_FILES[grid]).read()
     except OSError:
         return None
     m = re.search(r"```json\s*(\{.*?\})\s*```", text, re.S)
@@ -136,7 +138,7 @@ def phases_of(solver, steps):
     return phases, step_bytes
 
 
-def run_resident(args, case, workload, with_traffic):
+def run_resident(args, case, workload, grid):
     import numpy as np
 
     import bfpn_gpu_amd as bg
@@ -155,7 +157,7 @@ def run_resident(args, case, workload, with_traffic):
     ach = phases[dom]["GBps"]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
-                "traffic": load_traffic(dom) if with_traffic else None,
+                "traffic": load_traffic(grid, dom) if args.precision == "fp32" else None,
                 "measured": "HIP events around each launch, one stream, after the timed steps"}
     return {
         "metric": METRIC,
@@ -412,7 +414,7 @@ def main():
         case = beam_case()
         out = run_resident(args, case, "single beam, water 230 MeV, 21x21 nodes x 20x20 "
                            "directions x 500 groups, dt 0.02 cm, eq15 + straggling + secondary "
-                           "source", with_traffic=True)
+                           "source", grid="beam")
         if not args.no_cpu:
             log("CPU baseline (oracle, bounded sample)")
             out["cpu_baseline"] = cpu_baseline(case, args.cpu_threads)
@@ -420,7 +422,7 @@ def main():
         case = beam_case(Ny=80, Nx=4000)
         out = run_resident(args, case, "paper grid, water 230 MeV, 81x81 nodes x 20x20 "
                            "directions x 500 groups, dt 0.01 cm, eq15 + straggling + secondary "
-                           "source", with_traffic=False)
+                           "source", grid="paper")
     elif args.workload == "beamlets":
         out = run_beamlets(args)
     elif args.workload == "serve":
