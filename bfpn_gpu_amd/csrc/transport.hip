@@ -343,7 +343,10 @@ struct MarchShape {
 // 160 KB of a CU.  Measured on MI355X (FP32, 400 directions): 16 directions
 // per work-group beat 32 (smaller groups pack the CU's wave slots better:
 // 235 vs 248 us at 21 x 21 nodes) and 8 (64-byte row segments: 3.41 vs
-// 3.57 ms at 81 x 81 nodes with two columns per thread).
+// 3.57 ms at 81 x 81 nodes with two columns per thread).  At 81 x 81 nodes
+// the (16, 2) shape runs one work-group per CU; 12 or 8 directions with one
+// column per thread fit two, but their 48- and 32-byte row segments cost
+// more than the occupancy gains (3.30 and 3.78 vs 2.52 ms per launch).
 MarchShape march_shape(int Ny1, size_t elem) {
     const int max_t = elem == 4 ? max_threads<float>() : max_threads<double>();
     constexpr int cand[4][2] = {{16, 1}, {16, 2}, {8, 2}, {8, 3}};
