@@ -11,6 +11,11 @@ runs), and evaluate the IDD files on the host:
   20x20 angle grid, run_table2.py:62-71), the Bragg peak, P90, D90 and D20 of
   each IDD (:mod:`bfpn_gpu_amd.metrics`, restating scripts/analyze_idd.py) and
   their distance from the paper's deterministic column.
+* ``grid_sensitivity``: scripts/idd_grid_sensitivity.py run against the
+  binary: a baseline grid and variants that each change exactly one grid
+  family (depth Nx, transverse Ny x Nz, angular Nmu x Nom), each IDD's
+  entrance dose, Dmax, BP/P90/D90/D20, and the relative L2 / max-norm of its
+  Dmax-normalised curve against the baseline's.
 * ``convergence``: the depth-step refinement of
   scripts/convergence_order.py:127-196, 251-333: the same case at each Nx,
   every coarser IDD compared with the finest one on their common depth range
@@ -19,7 +24,8 @@ runs), and evaluate the IDD files on the host:
   between neighbouring levels, log(e0/e1) / log(h0/h1) with h = Lx / Nx.
 
 The material tables default to the ones bundled under ``bfpn_gpu_amd/tables``.
-``python -m bfpn_gpu_amd.harness table2 --help`` / ``convergence --help``.
+``python -m bfpn_gpu_amd.harness table2 --help`` / ``convergence --help`` /
+``grid-sensitivity --help``.
 """
 from __future__ import annotations
 
@@ -82,6 +88,8 @@ def _overlap(x, y, x_ref, y_ref):
     lo = max(float(np.min(x)), float(np.min(x_ref)))
     hi = min(float(np.max(x)), float(np.max(x_ref)))
     m = (x >= lo) & (x <= hi)
+    if np.count_nonzero(m) < 2:
+        raise ValueError("curves overlap on fewer than two samples")
     return x[m], y[m], np.interp(x[m], x_ref, y_ref)
 
 
@@ -94,8 +102,6 @@ def _trapezoid(f, x):
 def relative_l2(x, y, x_ref, y_ref) -> float:
     """||y - y_ref|| / ||y_ref|| in L2 over the common depth range."""
     xc, yc, yr = _overlap(x, y, x_ref, y_ref)
-    if xc.size < 2:
-        raise ValueError("curves overlap on fewer than two samples")
     den = _trapezoid(yr * yr, xc)
     if den <= 0:
         raise ValueError("reference curve has zero L2 norm")
@@ -170,6 +176,108 @@ def convergence(exe=DEFAULT_EXE, outdir="convergence", nxs=(500, 1000, 2000, 400
     return rows
 
 
+# scripts/idd_grid_sensitivity.py:44-77: a grid is (Nx, Ny, Nz, Nmu, Nom) and
+# a variant may change only the family it names
+GRID_AXES = ("depth", "transverse", "angular")
+
+
+def grid_families_changed(base, var) -> dict:
+    """Which grid families differ between two (Nx, Ny, Nz, Nmu, Nom) grids."""
+    return {"depth": var[0] != base[0],
+            "transverse": (var[1], var[2]) != (base[1], base[2]),
+            "angular": (var[3], var[4]) != (base[3], base[4])}
+
+
+def check_variant(axis: str, label: str, base, var) -> None:
+    """ValueError unless `var` changes the `axis` family of `base` and nothing else."""
+    if axis not in GRID_AXES:
+        raise ValueError(f"unknown variant axis: {axis}")
+    if len(var) != 5 or min(var) <= 0:
+        raise ValueError(f"{label}: grid sizes must be five positive integers")
+    changed = grid_families_changed(base, var)
+    if not changed[axis]:
+        raise ValueError(f"{label}: the requested {axis} grid did not change")
+    other = [a for a in GRID_AXES if a != axis and changed[a]]
+    if other:
+        raise ValueError(f"{label}: changes {axis} together with {', '.join(other)}")
+
+
+def sensitivity_rows(baseline, variants):
+    """Rows of scripts/idd_grid_sensitivity.py:119-145.  baseline = (label,
+    grid, idd path); variants = [(axis, label, grid, idd path)]."""
+    b_label, b_grid, b_path = baseline
+    for axis, label, grid, _ in variants:
+        check_variant(axis, label, b_grid, grid)
+
+    def curve(path):
+        x, dose = metrics.load_idd(path)
+        got = metrics.analyze(x, dose)
+        return got, x, dose / got["Dmax"], float(dose[0])
+
+    _, bx, bnorm, _ = curve(b_path)
+    rows = []
+    for axis, label, grid, path in [("baseline", b_label, b_grid, b_path), *variants]:
+        got, x, norm, entrance = curve(path)
+        base = axis == "baseline"
+        rows.append({"axis": axis, "label": label, "nx": grid[0], "ny": grid[1], "nz": grid[2],
+                     "nmu": grid[3], "nom": grid[4], "entrance": entrance, "dmax": got["Dmax"],
+                     "norm0": float(norm[0]), "bp": got["BP"], "p90": got["P90"],
+                     "d90": got["D90"], "d20": got["D20"],
+                     "norm_l2_vs_baseline": 0.0 if base else relative_l2(x, norm, bx, bnorm),
+                     "norm_linf_vs_baseline": 0.0 if base else relative_linf(x, norm, bx, bnorm)})
+    return rows
+
+
+def sensitivity_markdown(rows) -> str:
+    """The table of scripts/idd_grid_sensitivity.py:157-169."""
+    lines = ["| axis | label | grid | norm0 | BP cm | normalized L2 | normalized Linf |",
+             "|---|---|---|---:|---:|---:|---:|"]
+    for r in rows:
+        grid = f"{r['nx']} / {r['ny']}x{r['nz']} / {r['nmu']}x{r['nom']}"
+        lines.append(f"| {r['axis']} | {r['label']} | {grid} | {r['norm0']:.9f} | {r['bp']:.4f} "
+                     f"| {r['norm_l2_vs_baseline']:.6g} | {r['norm_linf_vs_baseline']:.6g} |")
+    return "\n".join(lines) + "\n"
+
+
+def grid_sensitivity(exe=DEFAULT_EXE, outdir="grid_sensitivity", baseline=None, variants=(),
+                     time: float = 40.0, material: str = "water", energy: float = 230.0,
+                     extra=(), skip_run: bool = False):
+    """Runs the baseline grid ("label", (Nx, Ny, Nz, Nmu, Nom)) and each variant
+    (axis, label, grid) through the solver, one directory per label (the
+    variants are checked before anything runs), and returns the rows of
+    sensitivity_rows; writes grid_sensitivity.csv and .md into outdir."""
+    if baseline is None:
+        raise ValueError("a baseline grid is required")
+    b_label, b_grid = baseline[0], tuple(int(v) for v in baseline[1])
+    variants = [(a, l, tuple(int(v) for v in g)) for a, l, g in variants]
+    labels = [b_label] + [l for _, l, _ in variants]
+    if len(set(labels)) != len(labels):
+        raise ValueError("case labels must be unique")
+    for axis, label, grid in variants:
+        check_variant(axis, label, b_grid, grid)
+    paths = {}
+    for label, grid in [(b_label, b_grid)] + [(l, g) for _, l, g in variants]:
+        case_dir = Path(outdir) / label
+        if skip_run:
+            paths[label] = case_dir / "idd_output.txt"
+            continue
+        nx, ny, nz, nmu, nom = grid
+        args = ["--time", time, "--material", material, "--energy", energy,
+                "--ny", ny, "--nz", nz, "--nmu", nmu, "--nom", nom, "--data", data_dir(material),
+                *extra]
+        paths[label] = run_solver(exe, nx, args, case_dir)
+    rows = sensitivity_rows((b_label, b_grid, paths[b_label]),
+                            [(a, l, g, paths[l]) for a, l, g in variants])
+    out = Path(outdir)
+    out.mkdir(parents=True, exist_ok=True)
+    with open(out / "grid_sensitivity.csv", "w", newline="", encoding="utf-8") as f:
+        w = csv.DictWriter(f, fieldnames=list(rows[0]), lineterminator="\n")
+        w.writeheader()
+        w.writerows(rows)
+    (out / "grid_sensitivity.md").write_text(sensitivity_markdown(rows), encoding="utf-8")
+    return rows
+
+
 def _write_csv(path: Path, rows) -> None:
     with open(path, "w", newline="", encoding="utf-8") as f:
         w = csv.DictWriter(f, fieldnames=["nx", "h", "rel_l2", "rel_linf", "order_to_next"])
@@ -190,6 +298,20 @@ def main(argv=None) -> int:
     t.add_argument("--skip-run", action="store_true")
     t.add_argument("extra", nargs=argparse.REMAINDER,
                    help="further solver arguments after --, e.g. -- --ly 8 --lz 8 --sigma-yz 0.3")
+    s = sub.add_parser("grid-sensitivity", help="one grid family changed at a time")
+    s.add_argument("--exe", default=str(DEFAULT_EXE))
+    s.add_argument("--outdir", default="grid_sensitivity")
+    s.add_argument("--baseline", nargs=6, required=True,
+                   metavar=("LABEL", "NX", "NY", "NZ", "NMU", "NOM"))
+    s.add_argument("--variant", nargs=7, action="append", required=True,
+                   metavar=("AXIS", "LABEL", "NX", "NY", "NZ", "NMU", "NOM"),
+                   help="AXIS is depth, transverse or angular")
+    s.add_argument("--time", type=float, default=40.0)
+    s.add_argument("--material", default="water", choices=["water", "bone", "air"])
+    s.add_argument("--energy", type=float, default=230.0)
+    s.add_argument("--skip-run", action="store_true")
+    s.add_argument("extra", nargs=argparse.REMAINDER,
+                   help="further solver arguments after --, e.g. -- --ly 8 --lz 8 --ng 300")
     c = sub.add_parser("convergence", help="depth-step convergence and observed order")
     c.add_argument("--exe", default=str(DEFAULT_EXE))
     c.add_argument("--out", default="convergence")
@@ -208,6 +330,20 @@ def main(argv=None) -> int:
                    help="further solver arguments after --, e.g. -- --sigma-yz 0.3")
     a = ap.parse_args(argv)
     extra = [e for e in getattr(a, "extra", []) if e != "--"]
+    if a.cmd == "grid-sensitivity":
+        def grid(vals):
+            try:
+                return tuple(int(v) for v in vals)
+            except ValueError:
+                ap.error(f"grid sizes must be integers: {' '.join(vals)}")
+        variants = [(v[0], v[1], grid(v[2:])) for v in a.variant]
+        try:
+            rows = grid_sensitivity(a.exe, a.outdir, (a.baseline[0], grid(a.baseline[1:])),
+                                    variants, a.time, a.material, a.energy, extra, a.skip_run)
+        except ValueError as exc:
+            ap.error(str(exc))
+        print(sensitivity_markdown(rows), end="")
+        return 0
     if a.cmd == "table2":
         mats = set(a.materials.split(","))
         rows = table2(a.exe, a.outdir, a.nx, a.time, a.paper_grid, extra, a.skip_run,

@@ -33,3 +33,24 @@ def test_table2_runner_on_one_case(tmp_path, gpu):
     (r,) = rows
     assert all(math.isfinite(r["metrics_cm"][k]) for k in ("BP", "P90", "D90", "D20")), r
     assert 6.0 < r["metrics_cm"]["BP"] < 9.0, r
+
+
+def test_grid_sensitivity_runner_on_the_solver(tmp_path, gpu):
+    """A small water case with each grid family refined on its own through
+    the CLI: every variant's normalised curve stays close to the baseline's
+    and keeps its Bragg peak, and the baseline row is exactly zero."""
+    rows = harness.grid_sensitivity(
+        harness.DEFAULT_EXE, tmp_path, ("base", (400, 4, 4, 4, 4)),
+        [("depth", "fine_dx", (800, 4, 4, 4, 4)),
+         ("transverse", "fine_yz", (400, 8, 8, 4, 4)),
+         ("angular", "fine_angle", (400, 4, 4, 8, 8))],
+        time=8.0, material="water", energy=100,
+        extra=("--ng", "100", "--ly", "4", "--lz", "4", "--eq15-straggling"))
+    base = rows[0]
+    assert base["norm_l2_vs_baseline"] == 0.0
+    for r in rows:
+        assert all(math.isfinite(r[k]) for k in ("entrance", "dmax", "bp", "d90", "d20")), r
+    for r in rows[1:]:
+        assert 0.0 < r["norm_l2_vs_baseline"] < 0.2, r
+        assert abs(r["bp"] - base["bp"]) < 0.5, r
+    assert (tmp_path / "grid_sensitivity.md").exists()

@@ -6,6 +6,7 @@ runners' plumbing, error norms and observed order are checked against the
 closed form.  The same runners drive the real binary in
 tests/test_gpu_harness.py.
 """
+import csv
 import math
 import os
 import stat
@@ -32,6 +33,9 @@ with open("idd_output.txt", "w") as f:
         x = (i + 1) * dt
         v = math.exp(-((x - r) / (0.02 * r + 0.05)) ** 2) + 0.3 * math.exp(-x / (2 * r + 1e-9))
         v += {C} * dt ** {ORDER} * math.sin(x)
+        # transverse size scales the dose, angular size adds a small wave
+        v *= 1 + 0.01 * (int(opt.get("--ny", 20)) - 20)
+        v += 1e-3 * (int(opt.get("--nmu", 20)) - 20) * math.sin(3 * x)
         f.write(f"{{x:.17g}} {{v:.17g}}\n")
 print(f"Step {{nx}}, t = {{time}}")
 '''
@@ -116,3 +120,81 @@ def test_solver_failure_is_reported(tmp_path):
     bad.chmod(bad.stat().st_mode | stat.S_IEXEC)
     with pytest.raises(RuntimeError, match="exited with 1"):
         harness.run_solver(bad, 10, [], tmp_path / "case")
+
+
+def test_grid_variants_must_change_exactly_one_family():
+    base = (2000, 20, 20, 20, 20)
+    harness.check_variant("depth", "d", base, (1000, 20, 20, 20, 20))
+    harness.check_variant("transverse", "t", base, (2000, 40, 40, 20, 20))
+    harness.check_variant("angular", "a", base, (2000, 20, 20, 16, 16))
+    with pytest.raises(ValueError, match="did not change"):
+        harness.check_variant("angular", "a", base, (1000, 20, 20, 20, 20))
+    with pytest.raises(ValueError, match="together with transverse"):
+        harness.check_variant("depth", "d", base, (1000, 40, 40, 20, 20))
+    with pytest.raises(ValueError, match="unknown variant axis"):
+        harness.check_variant("energy", "e", base, (2000, 20, 20, 20, 20))
+    with pytest.raises(ValueError, match="positive"):
+        harness.check_variant("depth", "d", base, (0, 20, 20, 20, 20))
+
+
+def test_grid_sensitivity_runner(tmp_path):
+    exe = _stub(tmp_path)
+    out = tmp_path / "gs"
+    rows = harness.grid_sensitivity(
+        exe, out, ("base", (2000, 20, 20, 20, 20)),
+        [("depth", "half_dx", (1000, 20, 20, 20, 20)),
+         ("transverse", "wide", (2000, 40, 40, 20, 20)),
+         ("angular", "fine_angle", (2000, 20, 20, 30, 30))], time=40.0, energy=150)
+    assert [r["label"] for r in rows] == ["base", "half_dx", "wide", "fine_angle"]
+    base, half, wide, fine = rows
+    assert base["norm_l2_vs_baseline"] == 0.0 and base["norm_linf_vs_baseline"] == 0.0
+    # a pure scale of the dose leaves the Dmax-normalised curve unchanged
+    assert wide["dmax"] == pytest.approx(1.2 * base["dmax"], rel=1e-12)
+    assert wide["entrance"] == pytest.approx(1.2 * base["entrance"], rel=1e-12)
+    assert wide["norm0"] == pytest.approx(base["norm0"], rel=1e-12)
+    assert wide["norm_l2_vs_baseline"] < 1e-12 and wide["bp"] == base["bp"]
+    # the added wave is seen; the depth variant samples the same curve at every
+    # other baseline depth, so interpolation is exact there
+    assert 1e-4 < fine["norm_l2_vs_baseline"] < 0.1
+    assert half["norm_l2_vs_baseline"] < 1e-12 and abs(half["bp"] - base["bp"]) <= 2 * 40.0 / 1000
+    x, y = harness.metrics.load_idd(out / "fine_angle" / "idd_output.txt")
+    assert fine["entrance"] == y[0] and fine["dmax"] == y.max()
+    cmd = open(out / "wide" / "command.txt").read().split()
+    assert cmd[1] == "2000" and cmd[cmd.index("--ny") + 1] == "40" and cmd[cmd.index("--nz") + 1] == "40"
+    csv_rows = list(csv.DictReader(open(out / "grid_sensitivity.csv")))
+    assert [r["label"] for r in csv_rows] == ["base", "half_dx", "wide", "fine_angle"]
+    md = (out / "grid_sensitivity.md").read_text().splitlines()
+    assert md[0].startswith("| axis | label | grid |") and len(md) == 6
+    assert md[4].startswith("| transverse | wide | 2000 / 40x40 / 20x20 |")
+    # re-reading the written files gives the same rows
+    again = harness.grid_sensitivity(
+        exe, out, ("base", (2000, 20, 20, 20, 20)),
+        [("depth", "half_dx", (1000, 20, 20, 20, 20)),
+         ("transverse", "wide", (2000, 40, 40, 20, 20)),
+         ("angular", "fine_angle", (2000, 20, 20, 30, 30))], skip_run=True)
+    assert again == rows
+
+
+def test_grid_sensitivity_checks_before_running(tmp_path):
+    exe = _stub(tmp_path)
+    with pytest.raises(ValueError, match="together with"):
+        harness.grid_sensitivity(exe, tmp_path / "gs", ("base", (2000, 20, 20, 20, 20)),
+                                 [("depth", "bad", (1000, 20, 20, 16, 16))])
+    assert not (tmp_path / "gs").exists()
+    with pytest.raises(ValueError, match="unique"):
+        harness.grid_sensitivity(exe, tmp_path / "gs", ("base", (2000, 20, 20, 20, 20)),
+                                 [("depth", "base", (1000, 20, 20, 20, 20))])
+
+
+def test_grid_sensitivity_cli(tmp_path, capsys):
+    exe = _stub(tmp_path)
+    rc = harness.main(["grid-sensitivity", "--exe", str(exe), "--outdir", str(tmp_path / "cli"),
+                       "--baseline", "base", "1000", "20", "20", "20", "20",
+                       "--variant", "angular", "coarse", "1000", "20", "20", "16", "16"])
+    assert rc == 0
+    text = capsys.readouterr().out
+    assert "| angular | coarse | 1000 / 20x20 / 16x16 |" in text
+    with pytest.raises(SystemExit):
+        harness.main(["grid-sensitivity", "--exe", str(exe), "--outdir", str(tmp_path / "cli2"),
+                      "--baseline", "base", "1000", "20", "20", "20", "20",
+                      "--variant", "depth", "same", "1000", "20", "20", "20", "20"])
