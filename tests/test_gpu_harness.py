@@ -44,7 +44,7 @@ def test_grid_sensitivity_runner_on_the_solver(tmp_path, gpu):
         [("depth", "fine_dx", (800, 4, 4, 4, 4)),
          ("transverse", "fine_yz", (400, 8, 8, 4, 4)),
          ("angular", "fine_angle", (400, 4, 4, 8, 8))],
-        time=8.0, material="water", energy=100,
+        time=10.0, material="water", energy=100,  # 10 cm: past the distal 20 % point
         extra=("--ng", "100", "--ly", "4", "--lz", "4", "--eq15-straggling"))
     base = rows[0]
     assert base["norm_l2_vs_baseline"] == 0.0
