@@ -76,13 +76,8 @@ __device__ inline Pack<T, LPT> load_pack(const T* p) {
         r.v[1] = q[1];
         r.v[2] = q[2];
         r.v[3] = q[3];
-    } else if constexpr (LPT == 2) {
-        static_assert(sizeof(T) == 4, "2-lane packs are FP32");
-        typedef float f32x2v __attribute__((ext_vector_type(2)));
-        const f32x2v q = __builtin_nontemporal_load(reinterpret_cast<const f32x2v*>(p));
-        r.v[0] = q[0];
-        r.v[1] = q[1];
     } else {
+        static_assert(LPT == 1, "one or four lanes per thread");
         r.v[0] = __builtin_nontemporal_load(p);
     }
     return r;
@@ -95,10 +90,6 @@ __device__ inline void store_pack(T* p, const double (&v)[LPT]) {
         const f32x4v q = {static_cast<float>(v[0]), static_cast<float>(v[1]),
                           static_cast<float>(v[2]), static_cast<float>(v[3])};
         __builtin_nontemporal_store(q, reinterpret_cast<f32x4v*>(p));
-    } else if constexpr (LPT == 2) {
-        typedef float f32x2v __attribute__((ext_vector_type(2)));
-        const f32x2v q = {static_cast<float>(v[0]), static_cast<float>(v[1])};
-        __builtin_nontemporal_store(q, reinterpret_cast<f32x2v*>(p));
     } else {
         __builtin_nontemporal_store(static_cast<T>(v[0]), p);
     }
@@ -112,7 +103,7 @@ __device__ inline void store_pack(T* p, const double (&v)[LPT]) {
 // has to copy a loaded value -- a copy waits for the load and would drain
 // the ring.
 template <typename T, int LPT>
-constexpr int prefetch_depth() { return LPT == 4 ? 8 : LPT == 2 ? 12 : (sizeof(T) == 4 ? 16 : 8); }
+constexpr int prefetch_depth() { return LPT == 4 ? 8 : (sizeof(T) == 4 ? 16 : 8); }
 
 // Lanes per thread for a sweep over `lanes` lanes of `elem`-byte state.
 inline int lanes_per_thread(long long lanes, size_t elem) {
@@ -260,8 +251,7 @@ __global__ __launch_bounds__(kBlock) void energy_half_kernel(
 // secondary source of step n reads).
 template <typename T, int LPT, bool SRC>
 constexpr int pair_depth() {
-    return LPT == 4 ? (SRC ? 4 : 6)
-                    : LPT == 2 ? (SRC ? 6 : 8) : (sizeof(T) == 4 ? (SRC ? 10 : 14) : (SRC ? 4 : 6));
+    return LPT == 4 ? (SRC ? 4 : 6) : (sizeof(T) == 4 ? (SRC ? 10 : 14) : (SRC ? 4 : 6));
 }
 
 template <typename T, int LPT, bool SRC, bool STRAG, bool XOUT>
@@ -415,9 +405,6 @@ void dispatch_pair_lpt(T* F, T* f, const EnergyCoef* coef, const T* Qa, const T*
         if (lpt == 4)
             return dispatch_pair<T, 4, SRC, STRAG>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yzw,
                                                    nang, partials, proxy, s);
-        if (lpt == 2)
-            return dispatch_pair<T, 2, SRC, STRAG>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yzw,
-                                                   nang, partials, proxy, s);
     }
     dispatch_pair<T, 1, SRC, STRAG>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yzw, nang, partials,
                                     proxy, s);
@@ -445,9 +432,6 @@ void dispatch_lpt(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng, long 
         const int lpt = lanes_per_thread(lanes, sizeof(T));
         if (lpt == 4)
             return dispatch_dose<T, 4, SRC, STRAG>(F, f, coef, Q1, Ng, lanes, yzw, nang, partials,
-                                                   proxy, s);
-        if (lpt == 2)
-            return dispatch_dose<T, 2, SRC, STRAG>(F, f, coef, Q1, Ng, lanes, yzw, nang, partials,
                                                    proxy, s);
     }
     dispatch_dose<T, 1, SRC, STRAG>(F, f, coef, Q1, Ng, lanes, yzw, nang, partials, proxy, s);
