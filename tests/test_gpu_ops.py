@@ -124,10 +124,13 @@ def test_transport_random_field_large_velocity(gpu, oracle_mod, dtype):
 @pytest.mark.parametrize("Ny,Nz,nang", [(1, 1, 3), (2, 5, 40), (20, 3, 33), (31, 4, 7),
                                         (40, 2, 17), (63, 2, 5), (64, 3, 9), (80, 2, 16),
                                         (127, 1, 3), (128, 2, 2), (169, 1, 2), (200, 1, 2),
-                                        (283, 6, 3), (340, 1, 2), (400, 1, 2)])
+                                        (283, 6, 3), (340, 1, 2), (400, 1, 2),
+                                        (20, 20, 33), (80, 21, 16), (128, 14, 3)])
 def test_transport_plane_shapes(gpu, oracle_mod, Ny, Nz, nang, dtype):
     """Every launch shape (16/8 directions per work-group, 1-3 y columns per
-    thread; z marches of 2..7 rows, i.e. edge-only and mixed unrolled blocks),
+    thread; z marches of 2..7 rows, i.e. edge-only and mixed unrolled blocks,
+    and of 15..22 rows, which run the steady eight-row loop: 21 x 21 nodes as
+    on the single beam, 81 x 81 as on the paper grid, 129 columns),
     plateaus with exact ties (flat-slope branches), both clip modes.  Planes
     wider than the LDS rings allow (above 341 y nodes in FP32, 170 in FP64)
     are an error."""
