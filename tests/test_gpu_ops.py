@@ -261,6 +261,41 @@ def test_secondary_source(gpu, oracle_mod, physical, dtype):
     _close(dQ.numpy().astype(np.float64), Q, TOL64 if dtype == np.float64 else TOL32["source"])
 
 
+@pytest.fixture(scope="module")
+def full_directions(oracle_mod):
+    """Source operands at 20 x 20 directions, 9 x 9 nodes, 500 groups."""
+    c = small_case(Ny=8, Nz=8, Nmu=20, Nom=20)
+    s = oracle_mod.Solver(c)
+    t = {k: s.table(k) for k in ("ker_e1", "ker_e2", "ker_v")}
+    t["beg"] = s.table("ker_beg").astype(np.int32)
+    t["end"] = s.table("ker_end").astype(np.int32)
+    s.close()
+    return c, oracle_mod.deltas(c), t
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_secondary_source_full_direction_grid(gpu, oracle_mod, full_directions, dtype):
+    """400 directions take the 16-byte-load angular GEMM over 13 K tiles
+    (the zero Kv blocks skipped) and 7 direction tiles; 81 node rows make one
+    full and one clamped 64-row tile; 500 groups give bands across several
+    64-group blocks.  Checked against the oracle's factorised sum (itself
+    pinned to the reference-order loop on the CPU) with signed random inputs,
+    so the max(F, 0) of the first input matters."""
+    c, d, t = full_directions
+    rng = np.random.default_rng(7)
+    Fp = rng.standard_normal(c.cells).astype(dtype).astype(np.float64)
+    fp = rng.standard_normal(c.cells).astype(dtype).astype(np.float64)
+    Q = oracle_mod.secondary_source_factorized(Fp, fp, t["ker_e1"], t["ker_e2"], t["ker_v"],
+                                               t["beg"], t["end"], c.Ng, c.nyz, c.nang, d.dg)
+    assert np.abs(Q).max() > 0
+    dQ = gpu.DeviceArray(shape=c.cells, dtype=dtype)
+    gpu.ops.secondary_source(_dev(gpu, Fp, dtype), _dev(gpu, fp, dtype), dQ,
+                             _dev(gpu, t["ker_e1"], np.float64), _dev(gpu, t["ker_e2"], np.float64),
+                             _dev(gpu, t["ker_v"], np.float64), _dev(gpu, t["beg"], np.int32),
+                             _dev(gpu, t["end"], np.int32), c.Ng, c.nyz, c.nang, d.dg)
+    _close(dQ.numpy().astype(np.float64), Q, TOL64 if dtype == np.float64 else TOL32["source"])
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("trap,cat", [(True, True), (False, False)])
 def test_idd(gpu, oracle_mod, physical, dtype, trap, cat):
