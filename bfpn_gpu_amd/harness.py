@@ -249,6 +249,8 @@ def grid_sensitivity(exe=DEFAULT_EXE, outdir="grid_sensitivity", baseline=None, 
     if baseline is None:
         raise ValueError("a baseline grid is required")
     b_label, b_grid = baseline[0], tuple(int(v) for v in baseline[1])
+    if len(b_grid) != 5 or min(b_grid) <= 0:
+        raise ValueError(f"{b_label}: grid sizes must be five positive integers")
     variants = [(a, l, tuple(int(v) for v in g)) for a, l, g in variants]
     labels = [b_label] + [l for _, l, _ in variants]
     if len(set(labels)) != len(labels):

@@ -181,6 +181,9 @@ def test_grid_sensitivity_checks_before_running(tmp_path):
         harness.grid_sensitivity(exe, tmp_path / "gs", ("base", (2000, 20, 20, 20, 20)),
                                  [("depth", "bad", (1000, 20, 20, 16, 16))])
     assert not (tmp_path / "gs").exists()
+    with pytest.raises(ValueError, match="five positive"):
+        harness.grid_sensitivity(exe, tmp_path / "gs", ("base", (2000, 0, 20, 20, 20)),
+                                 [("depth", "d", (1000, 0, 20, 20, 20))])
     with pytest.raises(ValueError, match="unique"):
         harness.grid_sensitivity(exe, tmp_path / "gs", ("base", (2000, 20, 20, 20, 20)),
                                  [("depth", "base", (1000, 20, 20, 20, 20))])
