@@ -1,10 +1,12 @@
 """Throughput benchmark of the in-core eq15 depth march on one MI355X.
 
-Default workload ("beam", one beam resident in HBM): water, 230 MeV Gaussian
-pencil (sigma_yz 0.3 cm), 21x21 transverse nodes over 8x8 cm, 20x20
-directions, 500 energy groups on [1, 260] MeV, dt = 0.02 cm (Nx = 2000,
-40 cm), eq15 with straggling, primary + secondary with the catastrophic
-source.  FP32 state, FP64 energy recurrence and IDD accumulation.
+Default workload ("paper", the paper grid with its whole state resident in
+HBM): water, 230 MeV Gaussian pencil (sigma_yz 0.3 cm), 81x81 transverse nodes
+over 8x8 cm, 20x20 directions, 500 energy groups on [1, 260] MeV, dt = 0.01 cm
+(Nx = 4000, 40 cm), eq15 with straggling, primary + secondary with the
+catastrophic source.  FP32 state, FP64 energy recurrence and IDD accumulation.
+The beam is synthetic (the Eq.23 Gaussian pencil); the material tables are the
+reference's water tables.
 
 A "step" is one Strang depth step of both species (E/2, T/2, A, T/2, E/2 and
 the source), i.e. Ng*(Ny+1)*(Nz+1)*Nmu*Nom phase-space cell-updates.  The
@@ -20,14 +22,18 @@ Prints ONE JSON line.  Fields beyond the common contract:
                 the committed summaries profiles/hbm_traffic_single_beam.md and
                 profiles/hbm_traffic_paper_grid.md (FP32 state only)
   phases        per-operator seconds/calls/GB/s of that instrumented run
-  cpu_baseline  the CPU FP64 oracle on this host (all cores), a bounded sample
-                of the same workload: the same physics on a 5x5-node
-                transverse slab at the same 0.4 cm spacing, two full depth
+  accuracy      the same steps with the FP64 instantiation: max relative IDD
+                error of the timed FP32 run over all its steps
+  cpu_baseline  (kind "port") the CPU FP64 oracle on this host (all cores), a
+                bounded sample of the same workload: the same physics on a
+                7x7-node transverse slab at the same node spacing, full depth
                 steps after an untimed one (cell-updates/s does not depend on
-                the slab width)
+                the slab width); "reference": the reference's own CPU
+                prototype serial_bfpn (single-threaded, not eq15: its grid and
+                physics are fixed in its source), one depth step
 
 Other workloads (--workload), each one JSON line:
-  paper     81x81 nodes (the paper grid), Nx = 4000, FP32 state resident
+  beam      21x21 nodes, Nx = 2000 (the single-beam configuration)
   beamlets  a seeded sample of the 64-beam batch (E ~ U[70,230] MeV, sigma
             ~ U[0.2,0.5] cm, 21x21 or 41x41 nodes, Ng = 300, water/bone, depth
             1.2 x CSDA range): one-at-a-time solver vs the batching server
@@ -45,6 +51,7 @@ import dataclasses
 import json
 import os
 import re
+import subprocess
 import sys
 import threading
 import time
@@ -72,28 +79,60 @@ def beam_case(Ny=20, Nx=2000, time_cm=40.0, material="water"):
                 time=time_cm, flags=dict(eq15_straggling=1))
 
 
-def cpu_baseline(case, threads: int | None):
-    """The CPU FP64 oracle on the same physics over a 5x5-node transverse slab
-    with the workload's node spacing: one untimed step, two timed full steps."""
+def cpu_baseline(case, threads: int | None, nodes: int = 7, steps: int = 3):
+    """The CPU FP64 oracle on the same physics over a nodes x nodes transverse
+    slab with the workload's node spacing: one untimed step, `steps` timed
+    full steps."""
     if threads:
         os.environ["OMP_NUM_THREADS"] = str(threads)
     import oracle
 
     oracle.build()
     dy = case.Ly / case.Ny
-    sample = dataclasses.replace(case, Ny=4, Nz=4, Ly=4 * dy, Lz=4 * dy)
+    w = nodes - 1
+    sample = dataclasses.replace(case, Ny=w, Nz=w, Ly=w * dy, Lz=w * dy)
     s = oracle.Solver(sample)
     s.advance(1)
     t0 = time.perf_counter()
-    s.advance(2)
+    s.advance(steps)
     dt = time.perf_counter() - t0
     cells = s.cells
     n = oracle.threads()
     s.close()
-    return {"value": 2 * cells / dt, "unit": "cell-updates/s", "cores": n, "kind": "port",
-            "sample": f"CPU FP64 oracle (C++/OpenMP, {n} threads): same physics on a 5x5-node "
-                      f"transverse slab at {dy:.2f} cm spacing ({cells} cells), 2 full depth "
-                      f"steps (primary+secondary) after an untimed one, {dt:.2f} s"}
+    return {"value": steps * cells / dt, "unit": "cell-updates/s", "cores": n, "kind": "port",
+            "sample": f"CPU FP64 oracle (C++/OpenMP, {n} threads): same physics on a "
+                      f"{nodes}x{nodes}-node transverse slab at {dy:.2f} cm spacing ({cells} "
+                      f"cells), {steps} full depth steps (primary+secondary) after an untimed "
+                      f"one, {dt:.2f} s"}
+
+
+def reference_cpu_baseline():
+    """The reference's CPU prototype serial_bfpn (BFPn_CPU_Solver/serial_bfpn.cpp,
+    built into oracle/_ref by oracle/Makefile), one depth step.  It is single
+    threaded and not eq15 (periodic DFT angles, first-order upwind transport,
+    exponential attenuation; serial_bfpn.cpp:92-219); its grid is fixed in its
+    source: 11x11 nodes, 11x11 directions, 500 groups, 230 MeV
+    (serial_bfpn.cpp:15-43, 289).  Nx = 100 with --time 0.4 is one step."""
+    from bfpn_gpu_amd.tables import material_dir
+
+    exe = os.path.join(ROOT, "oracle", "_ref", "serial_bfpn")
+    if not os.path.exists(exe):
+        return {"skipped": "oracle/_ref/serial_bfpn not built"}
+    # the synthetic table has the 500 rows the prototype reads (the shipped
+    # water table has 499 and is refused by it)
+    proc = subprocess.run([exe, "100", "--data", material_dir("synthetic"), "--time", "0.4"],
+                          capture_output=True, text=True, timeout=900)
+    if proc.returncode != 0:
+        return {"skipped": f"serial_bfpn exited {proc.returncode}: {proc.stderr.strip()[:200]}"}
+    sec = float(re.search(r"Elapsed seconds:\s*([0-9.eE+-]+)", proc.stdout).group(1))
+    steps = int(re.search(r"Steps:\s*(\d+)", proc.stdout).group(1))
+    cells = 500 * 11 * 11 * 11 * 11
+    host = os.cpu_count() or 0
+    return {"value": steps * cells / sec, "unit": "cell-updates/s", "cores": 1,
+            "kind": "reference", "seconds_per_step": sec / steps,
+            "sample": f"reference CPU prototype serial_bfpn (non-eq15; primary only), 1 of "
+                      f"{host} host cores, its fixed grid of 11x11 nodes x 11x11 directions x "
+                      f"500 groups ({cells} cells), {steps} depth step(s) in {sec:.1f} s"}
 
 
 def load_traffic(grid: str, phase: str):
@@ -153,6 +192,18 @@ def run_resident(args, case, workload, grid):
     assert np.all(np.isfinite(idd)) and idd[-1] > 0
     phases, step_bytes = phases_of(solver, args.steps)
     solver.close()
+    accuracy = None
+    if prec == bg.FP32 and not args.no_fp64_check:
+        # the same steps with the FP64 instantiation (after the timed region)
+        n = len(idd)
+        t0 = time.perf_counter()
+        with bg.Solver(case, bg.FP64) as ref:
+            ref.advance(n)
+            ref_idd, _ = ref.history()
+        rel = float(np.max(np.abs(idd - ref_idd[:n]) / np.abs(ref_idd[:n])))
+        accuracy = {"vs": "FP64 instantiation, same case and steps", "steps": n,
+                    "max_rel_idd_err": rel, "seconds": round(time.perf_counter() - t0, 1)}
+        log(f"FP64 check over {n} steps: max rel IDD error {rel:.2e}")
     dom = max(phases, key=lambda k: phases[k]["seconds"])
     ach = phases[dom]["GBps"]
     roofline = {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
@@ -171,7 +222,7 @@ def run_resident(args, case, workload, grid):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if prec == bg.FP32 else "f64",
-        "data": "synthetic",
+        "data": "synthetic Gaussian pencil beam; reference material tables",
         "config": {"workload": workload, "cells": cells,
                    "precision": "FP32 state; FP64 energy recurrence and IDD"
                    if prec == bg.FP32 else "FP64",
@@ -183,6 +234,7 @@ def run_resident(args, case, workload, grid):
                    "moved_bytes_per_step_GB": round(step_bytes / 1e9, 3)},
         "roofline": roofline,
         "phases": phases,
+        "accuracy": accuracy,
     }
 
 
@@ -382,11 +434,11 @@ def run_copy(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", choices=("fp32", "fp64"), default="fp32")
     ap.add_argument("--workload", choices=("beam", "paper", "beamlets", "serve", "accuracy",
-                                           "copy"), default="beam")
+                                           "copy"), default="paper")
     ap.add_argument("--grid", choices=("beam", "paper"), default="beam")
     ap.add_argument("--material", choices=("water", "bone"), default="water")
     ap.add_argument("--save", default=None, help="directory for the accuracy IDD arrays")
@@ -394,7 +446,9 @@ def main():
                     help="accuracy: run one precision and save its IDD (needs --save)")
     ap.add_argument("--compare", default=None,
                     help="accuracy: compare the fp32/fp64 IDDs saved in this directory")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--no-fp64-check", action="store_true",
+                    help="skip the FP64 rerun of the timed steps")
     ap.add_argument("--cpu-threads", type=int, default=None)
     ap.add_argument("--beams", type=int, default=8)
     ap.add_argument("--fp64-check", action="store_true")
@@ -410,19 +464,21 @@ def main():
     if args.warmup < 1:
         args.warmup = 1
 
-    if args.workload == "beam":
-        case = beam_case()
-        out = run_resident(args, case, "single beam, water 230 MeV, 21x21 nodes x 20x20 "
-                           "directions x 500 groups, dt 0.02 cm, eq15 + straggling + secondary "
-                           "source", grid="beam")
+    if args.workload in ("beam", "paper"):
+        if args.workload == "beam":
+            case = beam_case()
+            name = ("single beam, water 230 MeV, 21x21 nodes x 20x20 directions x 500 groups, "
+                    "dt 0.02 cm, eq15 + straggling + secondary source")
+        else:
+            case = beam_case(Ny=80, Nx=4000)
+            name = ("paper grid, water 230 MeV, 81x81 nodes x 20x20 directions x 500 groups, "
+                    "dt 0.01 cm, eq15 + straggling + secondary source, state resident in HBM")
+        out = run_resident(args, case, name, grid=args.workload)
         if not args.no_cpu:
-            log("CPU baseline (oracle, bounded sample)")
+            log("CPU baseline: oracle, bounded sample")
             out["cpu_baseline"] = cpu_baseline(case, args.cpu_threads)
-    elif args.workload == "paper":
-        case = beam_case(Ny=80, Nx=4000)
-        out = run_resident(args, case, "paper grid, water 230 MeV, 81x81 nodes x 20x20 "
-                           "directions x 500 groups, dt 0.01 cm, eq15 + straggling + secondary "
-                           "source", grid="paper")
+            log("CPU baseline: reference prototype serial_bfpn, one step")
+            out["cpu_baseline"]["reference"] = reference_cpu_baseline()
     elif args.workload == "beamlets":
         out = run_beamlets(args)
     elif args.workload == "serve":

@@ -64,11 +64,14 @@ def lib():
     L.bfpn_mem_info.argtypes = [C.POINTER(sz), C.POINTER(sz)]
     L.bfpn_workspace_create.argtypes = [C.POINTER(vp)]
     L.bfpn_workspace_destroy.argtypes = [vp]
+    L.bfpn_workspace_stats.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(sz)]
     for suffix in ("f32", "f64"):
         getattr(L, f"bfpn_energy_half_{suffix}").argtypes = [vp, vp, vp, vp, vp, vp, d, d, d, i, i, i,
                                                              vp, vp, i, vp]
         getattr(L, f"bfpn_secondary_source_{suffix}").argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                                                   i, i, i, d, vp]
+        getattr(L, f"bfpn_secondary_source_prepare_{suffix}").argtypes = [vp, vp, vp, vp, vp, vp, i,
+                                                                          i, d, vp]
         getattr(L, f"bfpn_transport_half_{suffix}").argtypes = [vp, vp, vp, i, i, i, i, d, d, d, i, vp]
         getattr(L, f"bfpn_angle_full_{suffix}").argtypes = [vp, vp, vp, i, i, i, i, d, d, d, d, vp]
         getattr(L, f"bfpn_idd_{suffix}").argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i, i, i, i, d, d,

@@ -107,6 +107,14 @@ void launch_dose(const T* F, const T* f, const T* F1, const T* f1, const DoseCoe
                  hipStream_t s);
 int dose_blocks(int Ng, long long lanes);
 
+// Coefficient tables built on the device from device tables (same values as
+// the host builders energy_coefs, dose_coefs and yz_weights).
+void launch_energy_coefs(const double* S, const double* sigma, const double* Tc, bool straggling,
+                         double h, double dg, double rho, int Ng, EnergyCoef* out, hipStream_t s);
+void launch_dose_coefs(const double* S, const double* sigma, const double* en, double dg,
+                       bool catastrophic, int Ng, DoseCoef* out, hipStream_t s);
+void launch_yz_weights(int Ny, int Nz, bool trapezoidal, double* out, hipStream_t s);
+
 // out[slot] = scale * sum(a[0:na]) + scale * sum(b[0:nb])  (b may be null), one block.
 void launch_finish_sum(const double* a, int na, const double* b, int nb, double scale, double* out,
                        hipStream_t s);

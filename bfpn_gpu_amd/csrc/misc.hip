@@ -98,6 +98,95 @@ __global__ __launch_bounds__(kBlock) void copy16_kernel(const float4* __restrict
         dst[i] = src[i];
 }
 
+// Per-group coefficient tables of the operator-level API, built on the device
+// from device tables so that an operator call needs no host round trip.  Each
+// mirrors the host builder in tables.cpp operation for operation, without
+// contraction, so both give the same bits.
+__device__ inline double max_like_std(double a, double b) { return (a < b) ? b : a; }  // std::max
+
+__global__ __launch_bounds__(kBlock) void energy_coef_kernel(const double* __restrict__ S,
+                                                             const double* __restrict__ sigma,
+                                                             const double* __restrict__ Tc,
+                                                             int straggling, double h, double dg,
+                                                             double rho, int Ng,
+                                                             EnergyCoef* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= Ng) return;
+    const double dg2 = max_like_std(dg * dg, kEps);
+    const double sig = max_like_std(sigma[g], 0.0);
+    const double Slo = max_like_std(S[g], 0.0);
+    const double Shi = max_like_std(S[g + 1], 0.0);
+    const bool st = straggling && Tc != nullptr;
+    const double Tg = st ? max_like_std(Tc[g], 0.0) : 0.0;
+    const double Thi = st && g + 1 < Ng ? max_like_std(Tc[g + 1], 0.0) : 0.0;
+    const double Tlo = st && g > 0 ? max_like_std(Tc[g - 1], 0.0) : 0.0;
+    const double inv_h = 1.0 / h, half_sig = 0.5 * sig;
+    const double c_up = 0.5 * rho * Shi / dg, c_cur = 0.5 * rho * Slo / dg;
+    const double inv_F = 1.0 / max_like_std(1.0 / h + 0.5 * rho * Slo / dg + 0.5 * sig, kEps);
+    const double A = (1.5 * rho * Shi / dg) * inv_F;
+    const double B = (0.5 * rho * Slo / dg) * inv_F;
+    const double Cc = (0.5 * rho * Shi / dg) * inv_F;
+    const double k_up = 1.5 * rho * Shi / dg, k_cur = 1.5 * rho * Slo / dg;
+    const double k_F = 1.5 * rho * (Slo + Shi) / dg, k_f = 0.5 * rho * (Shi - Slo) / dg;
+    const double H = k_up - A * 0.5 * rho * Shi / dg;
+    const double coe = 1.0 / h + 0.5 * sig + 1.5 * rho * Slo / dg + 0.5 * rho * (Shi - Slo) / dg +
+                       A * 0.5 * rho * Slo / dg;
+    const double inv_coe = 1.0 / max_like_std(coe, kEps);
+    const double a_hi = 0.5 * rho * Thi / dg2, a_lo = 0.5 * rho * Tlo / dg2, a_g = rho * Tg / dg2;
+    const double cm[6] = {c_up + a_hi, -c_up, inv_h - half_sig - c_cur - a_g, c_cur, a_lo, 1.0};
+    const double r[6] = {k_up - A * cm[0],
+                         -k_up - A * cm[1],
+                         (k_cur - k_F) - A * cm[2],
+                         (inv_h - half_sig - k_cur - k_f) - A * cm[3],
+                         -A * cm[4],
+                         -A * cm[5]};
+    EnergyCoef c;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        c.af[i] = inv_coe * r[i];
+        c.aF[i] = B * c.af[i] + inv_F * cm[i];
+    }
+    c.af[6] = inv_coe * H;
+    c.aF[6] = B * c.af[6] + Cc;
+    c.w1 = c.w2 = 0.0;
+    out[g] = c;
+}
+
+__global__ __launch_bounds__(kBlock) void dose_coef_kernel(const double* __restrict__ S,
+                                                           const double* __restrict__ sigma,
+                                                           const double* __restrict__ en,
+                                                           double dg, int catastrophic, int Ng,
+                                                           DoseCoef* __restrict__ out) {
+#pragma clang fp contract(off)
+    const int g = blockIdx.x * kBlock + threadIdx.x;
+    if (g >= Ng) return;
+    DoseCoef w;
+    w.w1 = 0.5 * (S[g] + S[g + 1]) * dg;
+    w.w2 = (S[g + 1] - S[g]) / 6.0 * dg;
+    if (catastrophic) {
+        const double sig = max_like_std(sigma[g], 0.0);
+        w.w1 += 0.5 * (en[g] + en[g + 1]) * sig * dg;
+        w.w2 += (en[g + 1] - en[g]) * sig * dg / 6.0;
+    }
+    out[g] = w;
+}
+
+// Transverse quadrature weight per node s = iz (Ny+1) + iy: 1, or the
+// trapezoid's 1/2 per boundary axis.
+__global__ __launch_bounds__(kBlock) void yz_weight_kernel(int Ny, int Nz, int trapezoidal,
+                                                           double* __restrict__ out) {
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= (Ny + 1) * (Nz + 1)) return;
+    const int iz = s / (Ny + 1), iy = s - iz * (Ny + 1);
+    double v = 1.0;
+    if (trapezoidal) {
+        if (iy == 0 || iy == Ny) v *= 0.5;
+        if (iz == 0 || iz == Nz) v *= 0.5;
+    }
+    out[s] = v;
+}
+
 }  // namespace
 
 void launch_copy16(const void* src, void* dst, size_t bytes, hipStream_t s) {
@@ -118,6 +207,30 @@ void launch_convert(const double* src, T* dst, long long n, hipStream_t s) {
 }
 template void launch_convert<float>(const double*, float*, long long, hipStream_t);
 template void launch_convert<double>(const double*, double*, long long, hipStream_t);
+
+void launch_energy_coefs(const double* S, const double* sigma, const double* Tc, bool straggling,
+                         double h, double dg, double rho, int Ng, EnergyCoef* out, hipStream_t s) {
+    if (Ng <= 0) return;
+    hipLaunchKernelGGL(energy_coef_kernel, dim3((Ng + kBlock - 1) / kBlock), dim3(kBlock), 0, s, S,
+                       sigma, Tc, straggling ? 1 : 0, h, dg, rho, Ng, out);
+    BFPN_HIP(hipGetLastError());
+}
+
+void launch_dose_coefs(const double* S, const double* sigma, const double* en, double dg,
+                       bool catastrophic, int Ng, DoseCoef* out, hipStream_t s) {
+    if (Ng <= 0) return;
+    hipLaunchKernelGGL(dose_coef_kernel, dim3((Ng + kBlock - 1) / kBlock), dim3(kBlock), 0, s, S,
+                       sigma, en, dg, catastrophic ? 1 : 0, Ng, out);
+    BFPN_HIP(hipGetLastError());
+}
+
+void launch_yz_weights(int Ny, int Nz, bool trapezoidal, double* out, hipStream_t s) {
+    const int n = (Ny + 1) * (Nz + 1);
+    if (n <= 0) return;
+    hipLaunchKernelGGL(yz_weight_kernel, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, Ny, Nz,
+                       trapezoidal ? 1 : 0, out);
+    BFPN_HIP(hipGetLastError());
+}
 
 int dose_blocks(int Ng, long long lanes) {
     (void)Ng;

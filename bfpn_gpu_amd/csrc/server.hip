@@ -103,11 +103,12 @@ struct bfpn_server {
     }
 
     // Frees cached physics tables that no admitted beam uses, oldest first,
-    // until `need` more bytes fit.  Called with the lock held.
-    void evict_for(size_t need) {
+    // until `need` more bytes fit; the entry `keep` (the tables of the request
+    // being admitted) is never freed.  Called with the lock held.
+    void evict_for(size_t need, const std::string& keep) {
         for (auto it = lru.rbegin(); it != lru.rend() && used + need > opts.hbm_budget_bytes;) {
             auto c = cache.find(*it);
-            if (c != cache.end() && c->second.physics.use_count() == 1) {
+            if (c != cache.end() && *it != keep && c->second.physics.use_count() == 1) {
                 used -= c->second.bytes;
                 cache.erase(c);
                 it = std::list<std::string>::reverse_iterator(lru.erase(std::next(it).base()));
@@ -148,10 +149,11 @@ struct bfpn_server {
                                          });
             auto req = *pick;
             const std::string key = physics_key(req->grid, req->phys, req->data, opts.precision);
-            const bool cached = cache.count(key) != 0;
+            const auto hit = cache.find(key);
+            const bool cached = hit != cache.end();
             const size_t phys_need = cached ? 0 : physics_bytes(req->grid, req->phys, opts.precision);
             const size_t need = req->ctx_bytes + phys_need;
-            if (used + need > opts.hbm_budget_bytes) evict_for(need);
+            if (used + need > opts.hbm_budget_bytes) evict_for(need, key);
             if (used + need > opts.hbm_budget_bytes) {
                 if (admitted() > 0) {  // wait for a running beam to finish
                     cv_prep.wait(lk);
@@ -168,7 +170,7 @@ struct bfpn_server {
             used += need;
             ++preparing;
             std::shared_ptr<PhysicsBase> physics;
-            if (cached) physics = cache[key].physics;
+            if (cached) physics = hit->second.physics;  // evict_for kept this entry
             lk.unlock();
             size_t physics_real = 0;
             std::string err;

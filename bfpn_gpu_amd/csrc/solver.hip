@@ -697,20 +697,48 @@ size_t physics_bytes(const bfpn_grid& g, const bfpn_phys& p, int precision) {
 using namespace bfpn;
 
 // ============================================================ workspace ===
+// Scratch and cached operands of the operator-level API.  Per call, the
+// coefficient tables are rebuilt on the device from the caller's device
+// tables (stream-ordered, no host round trip).  Two operand sets are built on
+// the host and kept: the secondary source's operands (keyed by the kernel
+// table pointers and sizes; bfpn_secondary_source_prepare_* builds them) and
+// the CN factors of the angular step (keyed by the values of sig_tr and the
+// scalars).  A workspace orders its scratch on the stream of each call.
 struct bfpn_workspace {
+    struct SourceKey {
+        const void *e1 = nullptr, *e2 = nullptr, *kv = nullptr, *beg = nullptr, *end = nullptr;
+        int Ng = 0, nang = 0, precision = -1;
+        double dg = 0.0;
+        bool operator==(const SourceKey& o) const {
+            return e1 == o.e1 && e2 == o.e2 && kv == o.kv && beg == o.beg && end == o.end &&
+                   Ng == o.Ng && nang == o.nang && precision == o.precision && dg == o.dg;
+        }
+    };
     DeviceBuffer coef, partial, a1, a2, factors, su, sv, om, dose_w, yzw, kvT;
     SourcePlanBuffers plan;
     AnglePlanBuffers angle_plan;
+    SourceKey source_key;
+    bool source_ready = false;
+    std::vector<double> angle_key;  // precision, grid scalars and sig_tr of the cached factors
+    int builds = 0;                 // host-side builds of cached operands
+
+    // Grows `b` to at least `bytes`.  Earlier calls on `s` may still read the
+    // old buffer, so it is freed only after they have finished.
+    static void grow(DeviceBuffer& b, size_t bytes, hipStream_t s) {
+        if (bytes <= b.bytes()) return;
+        BFPN_HIP(hipStreamSynchronize(s));
+        b.ensure(bytes);
+    }
+    size_t device_bytes() const {
+        size_t n = plan.bytes() + angle_plan.h.bytes() + angle_plan.m.bytes() + angle_plan.l.bytes();
+        for (const DeviceBuffer* b : {&coef, &partial, &a1, &a2, &factors, &su, &sv, &om, &dose_w,
+                                      &yzw, &kvT})
+            n += b->bytes();
+        return n;
+    }
 };
 
 namespace {
-
-template <typename T>
-std::vector<double> to_host(const T* dptr, size_t n) {
-    std::vector<T> t(n);
-    if (n) BFPN_HIP(hipMemcpy(t.data(), dptr, n * sizeof(T), hipMemcpyDeviceToHost));
-    return std::vector<double>(t.begin(), t.end());
-}
 
 template <typename T>
 void op_energy(bfpn_workspace* ws, T* F, T* f, const double* S, const double* sigma,
@@ -719,15 +747,49 @@ void op_energy(bfpn_workspace* ws, T* F, T* f, const double* S, const double* si
     BFPN_REQUIRE(ws && F && f && S && sigma, "null argument");
     BFPN_REQUIRE(Ng > 0 && nyz > 0 && nang > 0 && h > 0, "invalid sizes");
     if (Q2) throw Error(BFPN_E_UNSUPPORTED, "a psi^2 source (Q2) is not supported");
-    BFPN_HIP(hipStreamSynchronize(s));  // the workspace may still be in use
-    const auto hS = to_host(S, Ng + 1);
-    const auto hsig = to_host(sigma, Ng);
-    std::vector<double> hT;
-    if (Tc) hT = to_host(Tc, Ng);
-    upload(ws->coef, energy_coefs(hS, hsig, Tc ? hT.data() : nullptr, straggling != 0, h, dg, rho, Ng));
+    const bool st = straggling != 0 && Tc != nullptr;
+    bfpn_workspace::grow(ws->coef, static_cast<size_t>(Ng) * sizeof(EnergyCoef), s);
+    launch_energy_coefs(S, sigma, Tc, st, h, dg, rho, Ng, ws->coef.as<EnergyCoef>(), s);
     launch_energy_half<T>(F, f, ws->coef.as<EnergyCoef>(), Q1, Ng,
-                          static_cast<long long>(nyz) * nang, straggling != 0 && Tc, nullptr, nang,
-                          nullptr, false, s);
+                          static_cast<long long>(nyz) * nang, st, nullptr, nang, nullptr, false, s);
+}
+
+// Builds the source operands of (e1, e2, kv, beg, end) into the workspace;
+// synchronizes the stream (host copies of the tables are needed).
+template <typename T>
+void prepare_source(bfpn_workspace* ws, const double* e1, const double* e2, const double* kv,
+                    const int* beg, const int* end, int Ng, int nang, double dg, hipStream_t s) {
+    BFPN_REQUIRE(ws && e1 && e2 && kv && beg && end, "null argument");
+    BFPN_REQUIRE(Ng > 0 && nang > 0, "invalid sizes");
+    bfpn_workspace::SourceKey key;
+    key.e1 = e1;
+    key.e2 = e2;
+    key.kv = kv;
+    key.beg = beg;
+    key.end = end;
+    key.Ng = Ng;
+    key.nang = nang;
+    key.dg = dg;
+    key.precision = std::is_same_v<T, float> ? BFPN_FP32 : BFPN_FP64;
+    BFPN_HIP(hipStreamSynchronize(s));
+    ws->source_ready = false;
+    if constexpr (std::is_same_v<T, float>) {
+        const long long nk = static_cast<long long>(Ng) * nang * nang;
+        ws->kvT.ensure(nk * sizeof(float));
+        launch_convert<float>(kv, ws->kvT.as<float>(), nk, s);
+        std::vector<double> h1(static_cast<size_t>(Ng) * Ng), h2(h1.size());
+        std::vector<int> hb(Ng), he(Ng);
+        BFPN_HIP(hipMemcpy(h1.data(), e1, h1.size() * sizeof(double), hipMemcpyDeviceToHost));
+        BFPN_HIP(hipMemcpy(h2.data(), e2, h2.size() * sizeof(double), hipMemcpyDeviceToHost));
+        BFPN_HIP(hipMemcpy(hb.data(), beg, Ng * sizeof(int), hipMemcpyDeviceToHost));
+        BFPN_HIP(hipMemcpy(he.data(), end, Ng * sizeof(int), hipMemcpyDeviceToHost));
+        BFPN_HIP(hipStreamSynchronize(s));
+        ws->plan.build(ws->kvT.as<float>(), h1, h2, hb, he, Ng, nang, dg, s);
+        ws->kvT.release();
+    }
+    ws->source_key = key;
+    ws->source_ready = true;
+    ++ws->builds;
 }
 
 template <typename T>
@@ -736,26 +798,26 @@ void op_source(bfpn_workspace* ws, const T* Fp, const T* fp, T* Q1, const double
                int nang, double dg, hipStream_t s) {
     BFPN_REQUIRE(ws && Fp && fp && Q1 && e1 && e2 && kv && beg && end, "null argument");
     BFPN_REQUIRE(Ng > 0 && nyz > 0 && nang > 0, "invalid sizes");
-    BFPN_HIP(hipStreamSynchronize(s));
+    bfpn_workspace::SourceKey key;
+    key.e1 = e1;
+    key.e2 = e2;
+    key.kv = kv;
+    key.beg = beg;
+    key.end = end;
+    key.Ng = Ng;
+    key.nang = nang;
+    key.dg = dg;
+    key.precision = std::is_same_v<T, float> ? BFPN_FP32 : BFPN_FP64;
+    if (!ws->source_ready || !(ws->source_key == key))
+        prepare_source<T>(ws, e1, e2, kv, beg, end, Ng, nang, dg, s);
     const size_t n = static_cast<size_t>(Ng) * nyz * nang;
-    const long long nk = static_cast<long long>(Ng) * nang * nang;
-    ws->a1.ensure(n * sizeof(T));
-    ws->a2.ensure(n * sizeof(T));
-    ws->kvT.ensure(nk * sizeof(T));
-    launch_convert<T>(kv, ws->kvT.as<T>(), nk, s);
+    bfpn_workspace::grow(ws->a1, n * sizeof(T), s);
+    bfpn_workspace::grow(ws->a2, n * sizeof(T), s);
     if constexpr (std::is_same_v<T, float>) {
-        std::vector<double> h1 = to_host(e1, static_cast<size_t>(Ng) * Ng);
-        std::vector<double> h2 = to_host(e2, static_cast<size_t>(Ng) * Ng);
-        std::vector<int> hb(Ng), he(Ng);
-        BFPN_HIP(hipMemcpy(hb.data(), beg, Ng * sizeof(int), hipMemcpyDeviceToHost));
-        BFPN_HIP(hipMemcpy(he.data(), end, Ng * sizeof(int), hipMemcpyDeviceToHost));
-        BFPN_HIP(hipStreamSynchronize(s));
-        ws->plan.build(ws->kvT.as<float>(), h1, h2, hb, he, Ng, nang, dg, s);
         launch_source_mfma(Fp, fp, ws->a1.as<float>(), ws->a2.as<float>(), Q1, ws->plan.plan, Ng,
                            nyz, nang, s);
     } else {
-        launch_source_angular<T>(Fp, fp, ws->kvT.as<T>(), ws->a1.as<T>(), ws->a2.as<T>(), Ng, nyz,
-                                 nang, s);
+        launch_source_angular<T>(Fp, fp, kv, ws->a1.as<T>(), ws->a2.as<T>(), Ng, nyz, nang, s);
         launch_source_energy<T>(ws->a1.as<T>(), ws->a2.as<T>(), e1, e2, beg, end, Q1, Ng,
                                 static_cast<long long>(nyz) * nang, dg, s);
     }
@@ -766,8 +828,7 @@ void op_transport(bfpn_workspace* ws, T* F, const double* omend, int Ny, int Nz,
                   double dy, double dz, double h, int preserve, hipStream_t s) {
     BFPN_REQUIRE(ws && F && omend, "null argument");
     BFPN_REQUIRE(Ny > 0 && Nz > 0 && Ng > 0 && nang > 0, "invalid sizes");
-    BFPN_HIP(hipStreamSynchronize(s));
-    ws->om.ensure(2 * nang * sizeof(T));
+    bfpn_workspace::grow(ws->om, 2 * nang * sizeof(T), s);
     launch_convert<T>(omend, ws->om.as<T>(), 2 * nang, s);
     launch_transport_half<T>(F, ws->om.as<T>(), Ny, Nz, Ng, nang, dy, dz, h, preserve != 0, s);
 }
@@ -777,17 +838,29 @@ void op_angle(bfpn_workspace* ws, T* F, const double* sig_tr_host, int nyz, int 
               int Nom, double du, double dv, double dt, double rho, hipStream_t s) {
     BFPN_REQUIRE(ws && F && sig_tr_host, "null argument");
     BFPN_REQUIRE(nyz > 0 && Ng > 0 && Nmu > 0 && Nom > 0 && Nmu * Nom <= 1024, "invalid sizes");
-    BFPN_HIP(hipStreamSynchronize(s));
-    upload(ws->factors, cast_vector<T>(angle_factors(sig_tr_host, Ng, Nmu, Nom, du, dv, dt, rho)));
-    if constexpr (std::is_same_v<T, float>) {
-        if (AnglePlanBuffers::usable(Nmu, Nom)) {
+    const bool mfma = std::is_same_v<T, float> && AnglePlanBuffers::usable(Nmu, Nom);
+    std::vector<double> key = {std::is_same_v<T, float> ? 0.0 : 1.0, double(Ng), double(Nmu),
+                               double(Nom), du, dv, dt, rho};
+    key.insert(key.end(), sig_tr_host, sig_tr_host + Ng);
+    if (key != ws->angle_key) {  // new tables: the buffers are rewritten after earlier calls
+        BFPN_HIP(hipStreamSynchronize(s));
+        ws->angle_key.clear();
+        upload(ws->factors, cast_vector<T>(angle_factors(sig_tr_host, Ng, Nmu, Nom, du, dv, dt, rho)));
+        if (mfma) {
             ws->angle_plan.build(Nmu, Nom, s);
+        } else {
+            upload(ws->su, cast_vector<T>(sine_basis(Nom)));
+            upload(ws->sv, cast_vector<T>(sine_basis(Nmu)));
+        }
+        ws->angle_key.swap(key);
+        ++ws->builds;
+    }
+    if constexpr (std::is_same_v<T, float>) {
+        if (mfma) {
             launch_angle_mfma(F, ws->factors.as<float>(), ws->angle_plan.plan, Ng, nyz, Nmu, Nom, s);
             return;
         }
     }
-    upload(ws->su, cast_vector<T>(sine_basis(Nom)));
-    upload(ws->sv, cast_vector<T>(sine_basis(Nmu)));
     launch_angle_full<T>(F, ws->factors.as<T>(), ws->su.as<T>(), ws->sv.as<T>(), Ng, nyz, Nmu, Nom, s);
 }
 
@@ -798,15 +871,13 @@ void op_idd(bfpn_workspace* ws, const T* F, const T* f, const T* F1, const T* f1
             hipStream_t s) {
     BFPN_REQUIRE(ws && F && S && sigma && en && out, "null argument");
     BFPN_REQUIRE(Ng > 0 && Ny > 0 && Nz > 0 && nang > 0 && rho > 0, "invalid sizes");
-    BFPN_HIP(hipStreamSynchronize(s));
-    const auto hS = to_host(S, Ng + 1);
-    const auto hsig = to_host(sigma, Ng);
-    const auto hen = to_host(en, Ng + 1);
-    upload(ws->dose_w, dose_coefs(hS, hsig, hen, dg, cat != 0, Ng));
-    upload(ws->yzw, yz_weights(Ny, Nz, trap != 0));
     const long long lanes = static_cast<long long>(Ny + 1) * (Nz + 1) * nang;
     const int nb = dose_blocks(Ng, lanes);
-    ws->partial.ensure((nb + 1) * sizeof(double));
+    bfpn_workspace::grow(ws->dose_w, static_cast<size_t>(Ng) * sizeof(DoseCoef), s);
+    bfpn_workspace::grow(ws->yzw, static_cast<size_t>(Ny + 1) * (Nz + 1) * sizeof(double), s);
+    bfpn_workspace::grow(ws->partial, (nb + 1) * sizeof(double), s);
+    launch_dose_coefs(S, sigma, en, dg, cat != 0, Ng, ws->dose_w.as<DoseCoef>(), s);
+    launch_yz_weights(Ny, Nz, trap != 0, ws->yzw.as<double>(), s);
     double* part = ws->partial.as<double>();
     launch_dose<T>(F, f, F1, f1, ws->dose_w.as<DoseCoef>(), ws->yzw.as<double>(), Ng, lanes, nang, part, s);
     launch_finish_sum(part, nb, nullptr, 0, du * dv * dy * dz / rho, part + nb, s);
@@ -915,6 +986,27 @@ BFPN_API bfpn_status bfpn_secondary_source_f64(bfpn_workspace* ws, const double*
                                                const double* kv, const int* beg, const int* end,
                                                int Ng, int nyz, int nang, double dg, void* stream) {
     return guarded([&] { op_source<double>(ws, Fp, fp, Q1, e1, e2, kv, beg, end, Ng, nyz, nang, dg, as_stream(stream)); });
+}
+
+BFPN_API bfpn_status bfpn_secondary_source_prepare_f32(bfpn_workspace* ws, const double* e1,
+                                                       const double* e2, const double* kv,
+                                                       const int* beg, const int* end, int Ng,
+                                                       int nang, double dg, void* stream) {
+    return guarded([&] { prepare_source<float>(ws, e1, e2, kv, beg, end, Ng, nang, dg, as_stream(stream)); });
+}
+BFPN_API bfpn_status bfpn_secondary_source_prepare_f64(bfpn_workspace* ws, const double* e1,
+                                                       const double* e2, const double* kv,
+                                                       const int* beg, const int* end, int Ng,
+                                                       int nang, double dg, void* stream) {
+    return guarded([&] { prepare_source<double>(ws, e1, e2, kv, beg, end, Ng, nang, dg, as_stream(stream)); });
+}
+
+BFPN_API bfpn_status bfpn_workspace_stats(const bfpn_workspace* ws, int* builds, size_t* bytes) {
+    return guarded([&] {
+        BFPN_REQUIRE(ws, "null workspace");
+        if (builds) *builds = ws->builds;
+        if (bytes) *bytes = ws->device_bytes();
+    });
 }
 
 BFPN_API bfpn_status bfpn_transport_half_f32(bfpn_workspace* ws, float* F, const double* omend,

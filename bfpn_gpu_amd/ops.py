@@ -45,11 +45,35 @@ def energy_half(F, f, S, sigma, T, h, dg, density, Ng, nyz, nang, Q1=None, strag
              _p(Q1), None, int(straggling), None))
 
 
-def secondary_source(Fp, fp, Q, ke1, ke2, kv, beg, end, Ng, nyz, nang, dg, ws=None):
-    """secondarySourceFromPrimaryKernel (src/energy_solver.cu:287-333) into Q."""
+def secondary_source(Fp, fp, Q, ke1, ke2, kv, beg, end, Ng, nyz, nang, dg, ws=None,
+                     prepared=False):
+    """secondarySourceFromPrimaryKernel (src/energy_solver.cu:287-333) into Q.
+
+    The library caches the source operands by table pointer; unless
+    `prepared` says that secondary_source_prepare ran on these very tables
+    (which must not have changed since), they are rebuilt first, because a
+    freed table's address can come back with other contents."""
+    if not prepared:
+        secondary_source_prepare(ke1, ke2, kv, beg, end, Ng, nang, dg, Fp.dtype, ws)
     fn = getattr(lib(), f"bfpn_secondary_source_{_suffix(Fp)}")
     check(fn(_ws(ws).ptr, Fp.ptr, fp.ptr, Q.ptr, ke1.ptr, ke2.ptr, kv.ptr, beg.ptr, end.ptr, Ng,
              nyz, nang, dg, None))
+
+
+def secondary_source_prepare(ke1, ke2, kv, beg, end, Ng, nang, dg, dtype=np.float32, ws=None):
+    """Builds the source operands once (the kernel upload of BFPnSolver::initialize,
+    src/bfpm_solver.cu:1638-1813); later secondary_source calls with the same
+    tables reuse them."""
+    suffix = "f32" if np.dtype(dtype) == np.float32 else "f64"
+    fn = getattr(lib(), f"bfpn_secondary_source_prepare_{suffix}")
+    check(fn(_ws(ws).ptr, ke1.ptr, ke2.ptr, kv.ptr, beg.ptr, end.ptr, Ng, nang, dg, None))
+
+
+def workspace_stats(ws=None):
+    """(host-side operand builds so far, device bytes held) of a workspace."""
+    builds, nbytes = C.c_int(0), C.c_size_t(0)
+    check(lib().bfpn_workspace_stats(_ws(ws).ptr, C.byref(builds), C.byref(nbytes)))
+    return builds.value, nbytes.value
 
 
 def transport_half(F, omend, Ny, Nz, Ng, nang, dy, dz, h, preserve_sign, ws=None):

@@ -111,15 +111,30 @@ BFPN_API bfpn_status bfpn_mem_info(size_t* free_bytes, size_t* total_bytes);
 
 /* ---------------------------------------------------------------------------
  * Operator-level API (the seam of the reference's sub-solver classes).
- * All pointers are device pointers; `stream` is a hipStream_t (NULL = the
- * default stream).  Calls are stream-ordered and never synchronize the host
- * unless they return a host value.  Scratch comes from a workspace that grows
- * on demand, like the sub-solvers' lazily grown buffers
- * (src/energy_solver.cu:461-464, src/spatial_transport.cu:211-215,
- *  src/angle_diffusion.cu:429-440).
+ * All pointers are device pointers unless stated; `stream` is a hipStream_t
+ * (NULL = the default stream).  Calls are stream-ordered: each call builds
+ * its per-group coefficient tables on the device from the caller's device
+ * tables and enqueues its kernels, without waiting on the host.  The host
+ * waits (hipStreamSynchronize on `stream`) only when a call
+ *   - returns a host value (bfpn_idd_*),
+ *   - grows a workspace buffer (the first call at a larger size),
+ *   - builds the source operands: bfpn_secondary_source_prepare_*, or a
+ *     source call whose table pointers or sizes differ from the prepared ones,
+ *   - is an angular call whose sig_tr values or grid scalars differ from the
+ *     previous angular call's (the CN factors are built on the host).
+ * Scratch comes from a workspace that grows on demand, like the sub-solvers'
+ * lazily grown buffers (src/energy_solver.cu:461-464,
+ * src/spatial_transport.cu:211-215, src/angle_diffusion.cu:429-440).  A
+ * workspace orders its scratch on the stream of each call: calls on
+ * different streams that share a workspace must be ordered by the caller.
  * ------------------------------------------------------------------------- */
 BFPN_API bfpn_status bfpn_workspace_create(bfpn_workspace** ws);
 BFPN_API void        bfpn_workspace_destroy(bfpn_workspace* ws);
+/* Host-side builds of cached operands the workspace has made so far (source
+ * operands, CN factors) and the device bytes it holds.  No counterpart in the
+ * reference; it makes the caching contract above observable. */
+BFPN_API bfpn_status bfpn_workspace_stats(const bfpn_workspace* ws, int* builds,
+                                          size_t* device_bytes);
 
 /* Eq.15 DG/CN energy half step, in place on (F, f).
  * Replaces EnergyDepositionSolver::solve with the eq15 model
@@ -153,6 +168,19 @@ BFPN_API bfpn_status bfpn_secondary_source_f64(
     const double* ker_e1, const double* ker_e2, const double* ker_v,
     const int* ker_beg, const int* ker_end,
     int Ng, int nyz, int nang, double dg, void* stream);
+/* Builds the source operands of these tables into the workspace (FP32: the
+ * split-BF16 transposed Kv with its block-sparsity lists and the banded energy
+ * matrices); later bfpn_secondary_source_* calls with the same table pointers,
+ * Ng, nang and dg reuse them.  Call it again after changing table contents in
+ * place.  Synchronizes `stream`.  Stands for the one-time kernel upload of
+ * BFPnSolver::initialize (src/bfpm_solver.cu:1638-1813), which the
+ * reference's per-step solveSecondaryFromPrimary relies on. */
+BFPN_API bfpn_status bfpn_secondary_source_prepare_f32(
+    bfpn_workspace* ws, const double* ker_e1, const double* ker_e2, const double* ker_v,
+    const int* ker_beg, const int* ker_end, int Ng, int nang, double dg, void* stream);
+BFPN_API bfpn_status bfpn_secondary_source_prepare_f64(
+    bfpn_workspace* ws, const double* ker_e1, const double* ker_e2, const double* ker_v,
+    const int* ker_beg, const int* ker_end, int Ng, int nang, double dg, void* stream);
 
 /* Explicit MUSCL predictor-corrector transport over h, in place.
  * Replaces SpatialTransportSolver::solveEq18
@@ -180,8 +208,8 @@ BFPN_API bfpn_status bfpn_angle_full_f64(
     int nyz, int Ng, int Nmu, int Nom, double du, double dv, double dt,
     double density, void* stream);
 
-/* Eq.24 integrated depth dose of (F+F1, f+f1).  Synchronizes and returns the
- * value in *idd.  Replaces computeIntegratedDepthDose + integratedDepthDoseKernel
+/* Eq.24 integrated depth dose of (F+F1, f+f1).  Synchronizes `stream` and
+ * returns the value in *idd (host).  Replaces computeIntegratedDepthDose + integratedDepthDoseKernel
  * + reduceDevicePartials (src/bfpm_solver.cu:229-309, 2132-2164, 3473-3502).
  * S/sigma/en are device pointers (Ng+1, Ng, Ng+1). F1/f1 may be NULL. */
 BFPN_API bfpn_status bfpn_idd_f32(

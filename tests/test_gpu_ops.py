@@ -308,3 +308,54 @@ def test_idd(gpu, oracle_mod, physical, dtype, trap, cat):
                     c.Nz, c.nang, d.dg, d.du, d.dv, d.dy, d.dz, 1.3, trapezoidal=trap,
                     catastrophic=cat)
     assert v == pytest.approx(ref, rel=1e-13 if dtype == np.float64 else 1e-6)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_operator_calls_reuse_cached_operands(gpu, oracle_mod, physical, dtype):
+    """Per-step operator calls on unchanged tables build nothing on the host:
+    the source operands once per prepare (the second and third source calls
+    reuse them and give the same bits), the CN factors once per sig_tr;
+    energy, transport and IDD coefficients come from device kernels.  Every
+    result still matches the oracle."""
+    c, d, st, t = physical
+    ws = gpu.Workspace()
+    tab = {k: _dev(gpu, t[k], np.float64) for k in ("ker_e1", "ker_e2", "ker_v", "S", "sigma", "T",
+                                                    "en", "omend")}
+    beg, end = _dev(gpu, t["beg"], np.int32), _dev(gpu, t["end"], np.int32)
+    Fp = st[0].astype(dtype).astype(np.float64)
+    fp = st[1].astype(dtype).astype(np.float64)
+    dFp, dfp = _dev(gpu, Fp, dtype), _dev(gpu, fp, dtype)
+    src = (tab["ker_e1"], tab["ker_e2"], tab["ker_v"], beg, end, c.Ng)
+    gpu.ops.secondary_source_prepare(*src, c.nang, d.dg, dtype=dtype, ws=ws)
+    assert gpu.ops.workspace_stats(ws)[0] == 1
+    outs = []
+    for _ in range(3):
+        dQ = gpu.DeviceArray(shape=c.cells, dtype=dtype)
+        gpu.ops.secondary_source(dFp, dfp, dQ, *src, c.nyz, c.nang, d.dg, ws=ws, prepared=True)
+        outs.append(dQ.numpy())
+    assert gpu.ops.workspace_stats(ws)[0] == 1, "the source operands were rebuilt"
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    Q = oracle_mod.secondary_source(Fp, fp, t["ker_e1"], t["ker_e2"], t["ker_v"], t["beg"],
+                                    t["end"], c.Ng, c.nyz, c.nang, d.dg)
+    _close(outs[0].astype(np.float64), Q, TOL64 if dtype == np.float64 else TOL32["source"])
+
+    F, f = Fp.copy(), fp.copy()
+    dF, df = _dev(gpu, F, dtype), _dev(gpu, f, dtype)
+    for _ in range(2):
+        gpu.ops.energy_half(dF, df, tab["S"], tab["sigma"], tab["T"], d.dt / 2, d.dg, 1.0, c.Ng,
+                            c.nyz, c.nang, straggling=True, ws=ws)
+        oracle_mod.energy_half(F, f, t["S"], t["sigma"], t["T"], d.dt / 2, d.dg, 1.0, c.Ng, c.nyz,
+                               c.nang, straggling=True)
+        gpu.ops.transport_half(dF, tab["omend"], c.Ny, c.Nz, c.Ng, c.nang, d.dy, d.dz, d.dt / 2,
+                               False, ws=ws)
+        oracle_mod.transport_half(F, t["omend"], c.Ny, c.Nz, c.Ng, c.nang, d.dy, d.dz, d.dt / 2,
+                                  False)
+        gpu.ops.angle_full(dF, t["sig_tr"], c.nyz, c.Ng, c.Nmu, c.Nom, d.du, d.dv, d.dt, 1.0, ws=ws)
+        oracle_mod.angle_full(F, t["sig_tr"], c.nyz, c.Ng, c.Nmu, c.Nom, d.du, d.dv, d.dt, 1.0)
+    assert gpu.ops.workspace_stats(ws)[0] == 2, "only the first angular call builds its factors"
+    gpu.ops.angle_full(dF, 2 * t["sig_tr"], c.nyz, c.Ng, c.Nmu, c.Nom, d.du, d.dv, d.dt, 1.0, ws=ws)
+    oracle_mod.angle_full(F, 2 * t["sig_tr"], c.nyz, c.Ng, c.Nmu, c.Nom, d.du, d.dv, d.dt, 1.0)
+    assert gpu.ops.workspace_stats(ws)[0] == 3, "a changed sig_tr rebuilds the factors"
+    tol = TOL64 if dtype == np.float64 else 3e-4
+    _close(dF.numpy().astype(np.float64), F, tol)
+    _close(df.numpy().astype(np.float64), f, tol)
