@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <type_traits>
@@ -889,6 +890,72 @@ hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
 }  // namespace
 
+// ---------------------------------------------------- allocation-free IDD ---
+// bfpn_idd_f32_noalloc keeps the scratch of the IDD reduction (per-group dose
+// coefficients, y/z weights, per-block partial sums and the final sum) in
+// this file-scope device array instead of a workspace.  Each GPU holds its
+// own copy until the process ends; every call on a GPU shares it, so calls
+// that overlap on one GPU race.  Every byte a call reads from it is written
+// earlier in the same call (no clearing is needed).
+constexpr size_t kNoAllocScratchBytes = size_t(2) << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch_ptr;  // device address of `scratch`, per device
+
+// The device address of `scratch` on the current device, looked up once per
+// device (with that device current).  The array's host-side name is not a
+// device address and is never passed to a kernel or copy.
+unsigned char* noalloc_scratch() {
+    int dev = 0;
+    BFPN_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr.size() <= static_cast<size_t>(dev)) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        BFPN_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        BFPN_REQUIRE(p != nullptr, "no device address for the IDD scratch array");
+        g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    return g_scratch_ptr[dev];
+}
+
+size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
+
+void op_idd_noalloc(const float* F, const float* f, const float* F1, const float* f1,
+                    const double* S, const double* sigma, const double* en, int Ng, int Ny,
+                    int Nz, int nang, double dg, double du, double dv, double dy, double dz,
+                    double rho, int trap, int cat, double* out, hipStream_t s) {
+    BFPN_REQUIRE(F && S && sigma && en && out, "null argument");
+    BFPN_REQUIRE(Ng > 0 && Ny > 0 && Nz > 0 && nang > 0 && rho > 0, "invalid sizes");
+    const long long nyz = static_cast<long long>(Ny + 1) * (Nz + 1);
+    const long long lanes = nyz * nang;
+    const int nb = dose_blocks(Ng, lanes);
+    const size_t w_bytes = round256(static_cast<size_t>(Ng) * sizeof(DoseCoef));
+    const size_t yz_bytes = round256(static_cast<size_t>(nyz) * sizeof(double));
+    const size_t part_bytes = (static_cast<size_t>(nb) + 1) * sizeof(double);
+    const size_t need = w_bytes + yz_bytes + part_bytes;
+    if (need > kNoAllocScratchBytes)
+        throw Error(BFPN_E_INVALID,
+                    "bfpn_idd_f32_noalloc: this grid needs " + std::to_string(need) +
+                        " bytes of scratch, more than its fixed " +
+                        std::to_string(kNoAllocScratchBytes) + " (use bfpn_idd_f32)");
+    unsigned char* base = noalloc_scratch();
+    auto* w = reinterpret_cast<DoseCoef*>(base);
+    auto* yzw = reinterpret_cast<double*>(base + w_bytes);
+    auto* part = reinterpret_cast<double*>(base + w_bytes + yz_bytes);
+    launch_dose_coefs(S, sigma, en, dg, cat != 0, Ng, w, s);
+    launch_yz_weights(Ny, Nz, trap != 0, yzw, s);
+    launch_dose<float>(F, f, F1, f1, w, yzw, Ng, lanes, nang, part, s);
+    launch_finish_sum(part, nb, nullptr, 0, du * dv * dy * dz / rho, part + nb, s);
+    BFPN_HIP(hipMemcpyAsync(out, part + nb, sizeof(double), hipMemcpyDeviceToHost, s));
+    BFPN_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
 // ================================================================ C ABI ===
 extern "C" {
 
@@ -1044,6 +1111,14 @@ BFPN_API bfpn_status bfpn_idd_f64(bfpn_workspace* ws, const double* F, const dou
                                   int nang, double dg, double du, double dv, double dy, double dz,
                                   double rho, int trap, int cat, double* idd, void* stream) {
     return guarded([&] { op_idd<double>(ws, F, f, F1, f1, S, sigma, en, Ng, Ny, Nz, nang, dg, du, dv, dy, dz, rho, trap, cat, idd, as_stream(stream)); });
+}
+BFPN_API bfpn_status bfpn_idd_f32_noalloc(const float* F, const float* f, const float* F1,
+                                          const float* f1, const double* S, const double* sigma,
+                                          const double* en, int Ng, int Ny, int Nz, int nang,
+                                          double dg, double du, double dv, double dy, double dz,
+                                          double rho, int trap, int cat, double* idd,
+                                          void* stream) {
+    return guarded([&] { op_idd_noalloc(F, f, F1, f1, S, sigma, en, Ng, Ny, Nz, nang, dg, du, dv, dy, dz, rho, trap, cat, idd, as_stream(stream)); });
 }
 
 BFPN_API bfpn_status bfpn_create(const bfpn_grid* grid, const bfpn_phys* phys, const char* data_path,

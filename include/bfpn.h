@@ -225,6 +225,23 @@ BFPN_API bfpn_status bfpn_idd_f64(
     double dy, double dz, double density, int trapezoidal_yz,
     int dose_catastrophic_loss, double* idd, void* stream);
 
+/* bfpn_idd_f32 without a workspace and without any device allocation.  No
+ * counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc).  Its scratch (dose coefficients, y/z weights and
+ * per-block partial sums) is a fixed 2 MiB device array in the library, one
+ * per GPU, shared by every call on that GPU: calls that overlap on one GPU
+ * race and must be ordered by the caller.  Grids whose scratch exceeds 2 MiB
+ * (Ng*16 + (Ny+1)(Nz+1)*8 + (Ny+1)(Nz+1)*nang/256*8 bytes, each part rounded
+ * up to 256) are rejected with BFPN_E_INVALID.  Same arguments, result and
+ * synchronization as bfpn_idd_f32. */
+BFPN_API bfpn_status bfpn_idd_f32_noalloc(
+    const float* F, const float* f, const float* F1, const float* f1,
+    const double* S, const double* sigma, const double* en,
+    int Ng, int Ny, int Nz, int nang, double dg, double du, double dv,
+    double dy, double dz, double density, int trapezoidal_yz,
+    int dose_catastrophic_loss, double* idd, void* stream);
+
 /* ---------------------------------------------------------------------------
  * Context-level API: one beam, state resident in HBM.
  * Replaces BFPnSolver(GridParams, PhysicsParams), initialize(path) and
