@@ -94,6 +94,8 @@ def lib():
     L.bfpn_stream.restype = vp
     L.bfpn_set_profiling.argtypes = [vp, i]
     L.bfpn_set_overlap.argtypes = [vp, i]
+    L.bfpn_set_group_skip.argtypes = [vp, i]
+    L.bfpn_active_groups.argtypes = [vp, C.POINTER(i), C.POINTER(i)]
     L.bfpn_energy_moments.argtypes = [vp, vp, i]
     L.bfpn_spot_plane.argtypes = [vp, vp, C.c_size_t]
     L.bfpn_phase_stats.argtypes = [vp, vp, vp, vp]
@@ -281,6 +283,17 @@ class Solver:
     def set_overlap(self, on: bool = True) -> None:
         """Secondary march on its own stream, overlapping the next primary step."""
         check(lib().bfpn_set_overlap(self._h, int(on)))
+
+    def set_group_skip(self, on: bool = True) -> None:
+        """Skip the energy groups above each species' top nonzero group
+        (bit-identical results; on by default)."""
+        check(lib().bfpn_set_group_skip(self._h, int(on)))
+
+    def active_groups(self) -> tuple[int, int]:
+        """(primary top group, secondary top group); -1 when all zero."""
+        a, b = C.c_int(0), C.c_int(0)
+        check(lib().bfpn_active_groups(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     PHASES = ("energy", "transport", "angle", "source", "dose")
 

@@ -41,6 +41,8 @@ class Phys(C.Structure):
             "dose_catastrophic_loss",
             "calibrate_cross_sections",
             "save_energy_moments",
+            "legacy_energy",
+            "lite_memory",
         )
     ]
 

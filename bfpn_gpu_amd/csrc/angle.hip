@@ -51,7 +51,14 @@ template <int MT, int NT, bool V4>
 __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
                                                          const float* __restrict__ factors,
                                                          AnglePlan plan, int planes, int per_wave,
-                                                         int nyz, int Nmu, int Nom, float one) {
+                                                         int nyz, int Nmu, int Nom, float one,
+                                                         const int* __restrict__ top) {
+    // Only the planes of groups up to the state's top are nonzero (the
+    // others stay zero): the first `active` planes, shared out evenly over
+    // the grid's waves; blocks without any exit before touching LDS.
+    const int active = top ? min(planes, (min(*top, planes / nyz - 1) + 1) * nyz) : planes;
+    const int pw = top ? (active + gridDim.x * 4 - 1) / (gridDim.x * 4) : per_wave;
+    if (static_cast<int>(blockIdx.x) * 4 * pw >= active) return;
     // The nine constant basis arrays (Su, Su chained, Sv chained; hi, mid,
     // lo) live in LDS, shared by the work-group's four waves and read per
     // stage: held in registers they would cost 72 VGPRs and half the
@@ -69,8 +76,8 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
     __syncthreads();
     const int lane = threadIdx.x & 63, i = lane & 15, q = lane >> 4;
     // (the wave's first plane, made visibly wave-uniform for the compiler)
-    int p = __builtin_amdgcn_readfirstlane((blockIdx.x * 4 + (threadIdx.x >> 6)) * per_wave);
-    const int p_end = min(planes, p + per_wave);
+    int p = __builtin_amdgcn_readfirstlane((blockIdx.x * 4 + (threadIdx.x >> 6)) * pw);
+    const int p_end = min(active, p + pw);
     if (p >= p_end) return;
     const int na = Nmu * Nom;
     // The wave's run of planes as a buffer resource: a plane is a uniform
@@ -243,8 +250,15 @@ __global__ __launch_bounds__(kThreads) void angle_kernel(T* __restrict__ F,
                                                          const T* __restrict__ factors,
                                                          const T* __restrict__ su,
                                                          const T* __restrict__ sv, long long planes,
-                                                         int nyz, int Nmu, int Nom, int per_block) {
+                                                         int nyz, int Nmu, int Nom, int per_block,
+                                                         const int* __restrict__ top) {
     extern __shared__ unsigned char lds_raw[];
+    // planes of groups above the state's top are zero and stay zero
+    const long long active =
+        top ? min(planes, static_cast<long long>(min(*top, static_cast<int>(planes / nyz) - 1) + 1) * nyz)
+            : planes;
+    const long long p0 = static_cast<long long>(blockIdx.x) * per_block;
+    if (p0 >= active) return;
     T* lds = reinterpret_cast<T*>(lds_raw);
     const int na = Nmu * Nom;
     T* Su = lds;                    // Nom*Nom, Su[mu][a]
@@ -253,8 +267,7 @@ __global__ __launch_bounds__(kThreads) void angle_kernel(T* __restrict__ F,
     T* Y = X + per_block * na;      // per_block * na
     for (int e = threadIdx.x; e < Nom * Nom; e += kThreads) Su[e] = su[e];
     for (int e = threadIdx.x; e < Nmu * Nmu; e += kThreads) Sv[e] = sv[e];
-    const long long p0 = static_cast<long long>(blockIdx.x) * per_block;
-    const int np = static_cast<int>(min(static_cast<long long>(per_block), planes - p0));
+    const int np = static_cast<int>(min(static_cast<long long>(per_block), active - p0));
     const int n = np * na;
     T* base = F + p0 * na;
     for (int e = threadIdx.x; e < n; e += kThreads) X[e] = base[e];
@@ -302,7 +315,7 @@ __global__ __launch_bounds__(kThreads) void angle_kernel(T* __restrict__ F,
 }  // namespace
 
 void launch_angle_mfma(float* F, const float* factors, const AnglePlan& plan, int Ng, int nyz,
-                       int Nmu, int Nom, hipStream_t s) {
+                       int Nmu, int Nom, hipStream_t s, const int* top) {
     BFPN_REQUIRE(Nmu >= 1 && Nom >= 1 && Nmu <= 32 && Nom <= 32, "MFMA DST needs Nmu, Nom <= 32");
     const long long planes = static_cast<long long>(Ng) * nyz;
     if (planes == 0) return;
@@ -316,7 +329,7 @@ void launch_angle_mfma(float* F, const float* factors, const AnglePlan& plan, in
     const int mt = Nmu > 16 ? 2 : 1, nt = Nom > 16 ? 2 : 1;
 #define BFPN_ANGLE(MT_, NT_, V4_)                                                              \
     hipLaunchKernelGGL((angle_mfma_kernel<MT_, NT_, V4_>), grid, block, 0, s, F, factors, plan,\
-                       static_cast<int>(planes), per_wave, nyz, Nmu, Nom, 1.0f)
+                       static_cast<int>(planes), per_wave, nyz, Nmu, Nom, 1.0f, top)
 #define BFPN_ANGLE_V(MT_, NT_)                                                                 \
     if (Nom % 4 == 0) BFPN_ANGLE(MT_, NT_, true);                                              \
     else BFPN_ANGLE(MT_, NT_, false)
@@ -331,7 +344,7 @@ void launch_angle_mfma(float* F, const float* factors, const AnglePlan& plan, in
 
 template <typename T>
 void launch_angle_full(T* F, const T* factors, const T* su, const T* sv, int Ng, int nyz, int Nmu,
-                       int Nom, hipStream_t s) {
+                       int Nom, hipStream_t s, const int* top) {
     const int na = Nmu * Nom;
     BFPN_REQUIRE(na > 0 && na <= kMaxAngles, "Nmu*Nom must be in [1, 1024]");
     const long long planes = static_cast<long long>(Ng) * nyz;
@@ -342,13 +355,13 @@ void launch_angle_full(T* F, const T* factors, const T* su, const T* sv, int Ng,
     const long long blocks = (planes + per_block - 1) / per_block;
     BFPN_REQUIRE(blocks < (1LL << 31), "too many angular planes");
     hipLaunchKernelGGL(angle_kernel<T>, dim3(static_cast<unsigned>(blocks)), dim3(kThreads), lds, s,
-                       F, factors, su, sv, planes, nyz, Nmu, Nom, per_block);
+                       F, factors, su, sv, planes, nyz, Nmu, Nom, per_block, top);
     BFPN_HIP(hipGetLastError());
 }
 
 template void launch_angle_full<float>(float*, const float*, const float*, const float*, int, int,
-                                       int, int, hipStream_t);
+                                       int, int, hipStream_t, const int*);
 template void launch_angle_full<double>(double*, const double*, const double*, const double*, int,
-                                        int, int, int, hipStream_t);
+                                        int, int, int, hipStream_t, const int*);
 
 }  // namespace bfpn

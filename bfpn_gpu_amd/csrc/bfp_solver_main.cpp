@@ -122,8 +122,9 @@ void usage(const char* argv0) {
               << "  --lu/--lv <v>    U/V angular domain width (default: 1)\n"
               << "  --ng <n>         Energy groups (default: 500)\n"
               << "  --nmu/--nom <n>  Angular grid points (default: 11)\n"
+              << "  --lite-memory    Primary psi^1 only, energy step only (low memory)\n"
               << "  --primary-only   Skip secondary proton evolution/source\n"
-              << "  --energy-model eq15   Energy update model (eq15 only)\n"
+              << "  --energy-model <eq15|legacy> Energy update model (default: eq15)\n"
               << "  --eq15-straggling     Enable Eq.15 straggling terms\n"
               << "  --energy-only    Only advance the primary energy subsystem\n"
               << "  --no-transport   Skip the spatial transport subsystem (primary)\n"
@@ -255,16 +256,8 @@ int main(int argc, char** argv) {
         std::cout << "Running in test mode..." << std::endl;
         return 0;
     }
-    if (lite) {
-        std::cerr << "Error: --lite-memory (primary-only energy toy) is not provided; "
-                     "the full eq15 state fits in HBM"
-                  << std::endl;
-        return 1;
-    }
-    if (legacy_energy) {
-        std::cerr << "Error: --energy-model legacy is not provided; use eq15" << std::endl;
-        return 1;
-    }
+    phys.legacy_energy = legacy_energy ? 1 : 0;
+    phys.lite_memory = lite ? 1 : 0;
 
     bfpn_grid grid{nx, ny, nz, ng, nmu, nom, 40.0, ly, lz, lu, lv, lg};
     phys.sig_E = sigma_e;

@@ -66,6 +66,16 @@ struct EnergyCoef {
     double w1, w2;  // Eq.24 dose weights of the group (when the sweep accumulates the IDD)
 };
 
+// Per-group coefficients of the first-order upwind energy step (the legacy
+// energy model and the lite-memory step): for the half step h,
+//   F' = max((max(F,0)/h + c_hi F'_{g+1} + Q) / denom + strag * lap(F), 0)
+//   f' = max((f/h + c_hi f'_{g+1}) / denom, 0)
+// with c_hi = rho S_hi/dg, denom = max(1/h + rho S_lo/dg + sigma, eps) and
+// strag = h/2 T_g/dg^2 (0 without straggling).
+struct UpwindCoef {
+    double h, c_hi, denom, strag;
+};
+
 // Per-group dose weights: IDD = scale * sum w_yz(s) (psi1*w1[g] + psi2*w2[g]).
 struct DoseCoef {
     double w1;

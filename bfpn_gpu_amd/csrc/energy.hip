@@ -41,6 +41,54 @@ __device__ inline double block_sum(double v, double* lds) {
     return t;
 }
 
+__device__ inline int block_max(int v, int* lds) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) lds[wave] = v;
+    __syncthreads();
+    int t = -1;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kBlock / 64; ++w) t = max(t, lds[w]);
+    __syncthreads();
+    return t;
+}
+
+// Results below the smallest normal number of the state type are flushed to
+// zero (what a flush-to-zero mode does).  Without it the upper energy tail of
+// a beam never vanishes: a value at the smallest denormal, scaled by a decay
+// factor above 1/2 each half step, rounds back to itself, so the top nonzero
+// group could never fall as the beam slows.  The change is below 2^-126 of
+// the state's scale in FP32 (2^-1022 in FP64).
+template <typename T>
+__device__ inline double flush_tiny(double v) {
+    constexpr double tiny = sizeof(T) == 4 ? 1.1754943508222875e-38 : 2.2250738585072014e-308;
+    return fabs(v) < tiny ? 0.0 : v;
+}
+
+// Top group of the state a sweep reads, and of a source made from a primary
+// state with top *x (all groups when the tops are not tracked).
+__device__ inline int state_top(const SweepTops& t, int Ng) { return t.in ? *t.in : Ng - 1; }
+__device__ inline int source_top(const SweepTops& t, const int* x, int Ng) {
+    return (t.qmap && x) ? t.qmap[max(-1, min(Ng - 1, *x)) + 1] : Ng - 1;
+}
+
+// Presets the slots a later sweep writes (one thread of the grid).
+__device__ inline void reset_tops(const SweepTops& t) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (t.reset) *t.reset = -1;
+        if (t.reset_x) *t.reset_x = -1;
+    }
+}
+
+// Publishes a block's highest nonzero output group (-1: none).
+__device__ inline void publish_top(int top, int* out, int* out2, int* lds) {
+    const int b = block_max(top, lds);
+    if (threadIdx.x == 0 && b >= 0) {
+        if (out) atomicMax(out, b);
+        if (out2) atomicMax(out2, b);
+    }
+}
+
 // LPT adjacent lanes per thread (4 with FP32 state on large lane counts):
 // 16-byte row loads and stores, four independent recurrences in flight per
 // thread, one set of coefficient loads for four lanes.
@@ -134,21 +182,30 @@ template <bool SRC, bool STRAG, typename C>
 __device__ inline void recur(const Forms& c, C P, C R, double P_lo, double src, C& P_hi, C& R_hi,
                              double& hd, double& a, double& b) {
     const double Ph = P_hi, Rh = R_hi, Pd = P, Rd = R;
-    a = c.af[0] * Ph + c.af[1] * Rh + c.af[2] * Pd + c.af[3] * Rd;
-    b = c.aF[0] * Ph + c.aF[1] * Rh + c.aF[2] * Pd + c.aF[3] * Rd;
+    // explicit FMAs: every kernel that inlines this evaluates it identically
+    // (left to the compiler, contraction could differ between kernels and
+    // break the paired sweep's bit-identity with two single sweeps)
+    a = fma(c.af[3], Rd, fma(c.af[2], Pd, fma(c.af[1], Rh, c.af[0] * Ph)));
+    b = fma(c.aF[3], Rd, fma(c.aF[2], Pd, fma(c.aF[1], Rh, c.aF[0] * Ph)));
     if (STRAG) {
-        a += c.af[4] * P_lo;
-        b += c.aF[4] * P_lo;
+        a = fma(c.af[4], P_lo, a);
+        b = fma(c.aF[4], P_lo, b);
     }
     if (SRC) {
-        a += c.af[5] * src;
-        b += c.aF[5] * src;
+        a = fma(c.af[5], src, a);
+        b = fma(c.aF[5], src, b);
     }
-    a += c.af[6] * hd;
-    b += c.aF[6] * hd;
+    a = flush_tiny<C>(fma(c.af[6], hd, a));
+    b = flush_tiny<C>(fma(c.aF[6], hd, b));
     P_hi = P;
     R_hi = R;
     hd = b - a;
+}
+
+// Eq.24 dose of one group's result, accumulated with explicit FMAs (the same
+// operations in every kernel, see recur).
+__device__ inline double add_dose(double acc, double F, double f, double w1, double w2) {
+    return fma(f, w2, fma(F, w1, acc));
 }
 
 // Everything a group needs besides the state (recurrence coefficients and the
@@ -159,10 +216,18 @@ template <typename T, int LPT, bool SRC, bool STRAG, bool DOSE>
 __global__ __launch_bounds__(kBlock) void energy_half_kernel(
     T* __restrict__ F, T* __restrict__ f, const EnergyCoef* __restrict__ coef,
     const T* __restrict__ Q1, int Ng, int lanes, const double* __restrict__ yzw, int nang,
-    double* __restrict__ partials, int proxy) {
+    double* __restrict__ partials, int proxy, SweepTops tops) {
     constexpr int D = prefetch_depth<T, LPT>(), RING = D + 1;
     __shared__ double lds[kBlock / 64];
+    __shared__ int ldi[kBlock / 64];
     const int lane = (blockIdx.x * kBlock + threadIdx.x) * LPT;  // first of LPT lanes
+    reset_tops(tops);
+    // Groups above G0 are zero before and after the sweep: the state above
+    // its top, the source above its own, and straggling lifts the result at
+    // most one group.  Source values above the source's top are not read.
+    const int q_top = SRC ? source_top(tops, tops.xa, Ng) : -1;
+    const int G0 = min(Ng - 1, max(state_top(tops, Ng), q_top) + (STRAG ? 1 : 0));
+    int top = -1;  // highest group with a nonzero result in this thread's lanes
     double acc_dose = 0.0, acc_proxy = 0.0;
     if (lane < lanes) {
         Pack<T, LPT> rF[RING], rf[RING], rQ[RING];
@@ -177,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void energy_half_kernel(
             if (SRC) rQ[k] = load_pack<T, LPT>(Q1 + base + lane);
         };
 #pragma unroll
-        for (int k = 0; k < D; ++k) fetch(k, Ng - 1 - k);
+        for (int k = 0; k < D; ++k) fetch(k, G0 - k);
         T P_hi[LPT], R_hi[LPT];
         double hi_delta[LPT], dose[LPT];
 #pragma unroll
@@ -190,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void energy_half_kernel(
         // scheduling barrier keeps the request there), so their latency is
         // covered.  (Dose weights are read at use: two sets of forms plus
         // them would not fit the scalar registers.)
-        Forms cn = load_forms(coef, Ng - 1);
+        Forms cn = load_forms(coef, max(G0, 0));
         auto step = [&](auto slot, int g) {  // group g sits in ring slot `slot`
             constexpr int k = decltype(slot)::value;
             const Forms c = cn;
@@ -199,25 +264,31 @@ __global__ __launch_bounds__(kBlock) void energy_half_kernel(
             fetch((k + D) % RING, g - D);
             const Pack<T, LPT> cF = rF[k], cf = rf[k], cQ = rQ[k];
             const Pack<T, LPT> lo = rF[(k + 1) % RING];  // old F at g-1
+            const bool q_live = g <= q_top;
             double Fv[LPT], fv[LPT];
 #pragma unroll
             for (int j = 0; j < LPT; ++j)
                 recur<SRC, STRAG>(c, cF.v[j], cf.v[j], g > 0 ? static_cast<double>(lo.v[j]) : 0.0,
-                                  SRC ? static_cast<double>(cQ.v[j]) : 0.0, P_hi[j], R_hi[j],
-                                  hi_delta[j], fv[j], Fv[j]);
+                                  SRC && q_live ? static_cast<double>(cQ.v[j]) : 0.0, P_hi[j],
+                                  R_hi[j], hi_delta[j], fv[j], Fv[j]);
             const long long base = static_cast<long long>(g) * lanes;
             store_pack<T, LPT>(F + base + lane, Fv);
             store_pack<T, LPT>(f + base + lane, fv);
+            bool nz = false;
+#pragma unroll
+            for (int j = 0; j < LPT; ++j)
+                nz |= (static_cast<T>(Fv[j]) != T(0)) | (static_cast<T>(fv[j]) != T(0));
+            top = max(top, nz ? g : -1);
             if (DOSE) {  // the lanes' transverse weights are applied after the sweep
                 const double w1 = coef[g].w1, w2 = coef[g].w2;
 #pragma unroll
                 for (int j = 0; j < LPT; ++j) {
-                    dose[j] += Fv[j] * w1 + fv[j] * w2;
+                    dose[j] = add_dose(dose[j], Fv[j], fv[j], w1, w2);
                     acc_proxy += fv[j];
                 }
             }
         };
-        int g0 = Ng - 1;
+        int g0 = G0;
         for (; g0 >= RING - 1; g0 -= RING)
             unroll<RING>([&](auto k) { step(k, g0 - decltype(k)::value); });
         unroll<RING>([&](auto k) {  // tail: groups g0 .. 0 (fewer than RING)
@@ -227,7 +298,7 @@ __global__ __launch_bounds__(kBlock) void energy_half_kernel(
         // first consumer behind every state load in flight)
         if (DOSE)
 #pragma unroll
-            for (int j = 0; j < LPT; ++j) acc_dose += dose[j] * yzw[(lane + j) / nang];
+            for (int j = 0; j < LPT; ++j) acc_dose = fma(dose[j], yzw[(lane + j) / nang], acc_dose);
     }
     if (DOSE) {
         const double d = block_sum(acc_dose, lds);
@@ -237,6 +308,7 @@ __global__ __launch_bounds__(kBlock) void energy_half_kernel(
             if (threadIdx.x == 0) partials[gridDim.x + blockIdx.x] = p;
         }
     }
+    if (tops.out) publish_top(top, tops.out, tops.out_x, ldi);
 }
 
 // Two consecutive half steps in one sweep: the final E-half of depth step n
@@ -259,12 +331,21 @@ __global__ __launch_bounds__(kBlock) void energy_pair_kernel(
     T* __restrict__ F, T* __restrict__ f, const EnergyCoef* __restrict__ coef,
     const T* __restrict__ Qa, const T* __restrict__ Qb, T* __restrict__ XF, T* __restrict__ Xf,
     int Ng, int lanes, const double* __restrict__ yzw, int nang, double* __restrict__ partials,
-    int proxy) {
+    int proxy, SweepTops tops) {
     constexpr int D = pair_depth<T, LPT, SRC>(), RING = D + 1;
     __shared__ double lds[kBlock / 64];
+    __shared__ int ldi[kBlock / 64];
     const int lane = (blockIdx.x * kBlock + threadIdx.x) * LPT;
+    reset_tops(tops);
+    // The first half's result is zero above t1, the second's above G0 (see
+    // energy_half_kernel); both recurrences start at G0.
+    const int qa_top = SRC ? source_top(tops, tops.xa, Ng) : -1;
+    const int qb_top = SRC ? source_top(tops, tops.xb, Ng) : -1;
+    const int t1 = max(state_top(tops, Ng), qa_top) + (STRAG ? 1 : 0);
+    const int G0 = min(Ng - 1, max(t1, qb_top) + (STRAG ? 1 : 0));
+    int top_a = -1, top_b = -1;  // highest nonzero group of each half's result
     double acc_dose = 0.0, acc_proxy = 0.0;
-    if (lane < lanes) {
+    if (lane < lanes && G0 >= 0) {
         Pack<T, LPT> rF[RING], rf[RING], rQa[RING], rQb[RING];
         // Qb (the second half's source) is refilled separately, at the end of
         // a step: the second half reads slot k - 1's Qb in the step that
@@ -280,8 +361,8 @@ __global__ __launch_bounds__(kBlock) void energy_pair_kernel(
         };
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-            fetch(k, Ng - 1 - k);
-            fetch_qb(k, Ng - 1 - k);
+            fetch(k, G0 - k);
+            fetch_qb(k, G0 - k);
         }
         // first half: carried input at g+1, output difference; dose sums
         T P_hi[LPT], R_hi[LPT];
@@ -302,17 +383,23 @@ __global__ __launch_bounds__(kBlock) void energy_pair_kernel(
             // would not fit the scalar registers)
             __builtin_amdgcn_sched_barrier(0);
             const Forms c2 = load_forms(coef, g2);
+            const bool q_live = g2 <= qb_top;
             double Fv[LPT], fv[LPT];
 #pragma unroll
             for (int j = 0; j < LPT; ++j)
                 recur<SRC, STRAG>(c2, P2[j], R2[j], static_cast<double>(P_lo2[j]),
-                                  SRC ? static_cast<double>(q.v[j]) : 0.0, P2_hi[j], R2_hi[j],
-                                  hd2[j], fv[j], Fv[j]);
+                                  SRC && q_live ? static_cast<double>(q.v[j]) : 0.0, P2_hi[j],
+                                  R2_hi[j], hd2[j], fv[j], Fv[j]);
             const long long base = static_cast<long long>(g2) * lanes;
             store_pack<T, LPT>(F + base + lane, Fv);
             store_pack<T, LPT>(f + base + lane, fv);
+            bool nz = false;
+#pragma unroll
+            for (int j = 0; j < LPT; ++j)
+                nz |= (static_cast<T>(Fv[j]) != T(0)) | (static_cast<T>(fv[j]) != T(0));
+            top_b = max(top_b, nz ? g2 : -1);
         };
-        Forms cn = load_forms(coef, Ng - 1);
+        Forms cn = load_forms(coef, G0);
         // group g in ring slot `slot`; FIRST: g = Ng - 1, no second half behind it
         auto step = [&](auto slot, int g, auto first) {
             constexpr int k = decltype(slot)::value;
@@ -322,18 +409,19 @@ __global__ __launch_bounds__(kBlock) void energy_pair_kernel(
             fetch((k + D) % RING, g - D);
             const Pack<T, LPT> cF = rF[k], cf = rf[k], cQa = rQa[k];
             const Pack<T, LPT> lo = rF[(k + 1) % RING];
+            const bool q_live = g <= qa_top;
             double Fv[LPT], fv[LPT];
             T tF[LPT];
 #pragma unroll
             for (int j = 0; j < LPT; ++j)
                 recur<SRC, STRAG>(c, cF.v[j], cf.v[j], g > 0 ? static_cast<double>(lo.v[j]) : 0.0,
-                                  SRC ? static_cast<double>(cQa.v[j]) : 0.0, P_hi[j], R_hi[j],
-                                  hd[j], fv[j], Fv[j]);
+                                  SRC && q_live ? static_cast<double>(cQa.v[j]) : 0.0, P_hi[j],
+                                  R_hi[j], hd[j], fv[j], Fv[j]);
             {  // dose and proxy of step n from the first half's outputs
                 const double w1 = coef[g].w1, w2 = coef[g].w2;
 #pragma unroll
                 for (int j = 0; j < LPT; ++j) {
-                    dose[j] += Fv[j] * w1 + fv[j] * w2;
+                    dose[j] = add_dose(dose[j], Fv[j], fv[j], w1, w2);
                     acc_proxy += fv[j];
                 }
             }
@@ -342,8 +430,13 @@ __global__ __launch_bounds__(kBlock) void energy_pair_kernel(
                 store_pack<T, LPT>(XF + base + lane, Fv);
                 store_pack<T, LPT>(Xf + base + lane, fv);
             }
+            bool nz = false;
 #pragma unroll
-            for (int j = 0; j < LPT; ++j) tF[j] = static_cast<T>(Fv[j]);
+            for (int j = 0; j < LPT; ++j) {
+                tF[j] = static_cast<T>(Fv[j]);
+                nz |= (tF[j] != T(0)) | (static_cast<T>(fv[j]) != T(0));
+            }
+            top_a = max(top_a, nz ? g : -1);
             if (!decltype(first)::value) second(g + 1, tF, rQb[(k + D) % RING]);  // Qb at g+1
             fetch_qb((k + D) % RING, g - D);
 #pragma unroll
@@ -352,9 +445,9 @@ __global__ __launch_bounds__(kBlock) void energy_pair_kernel(
                 R2[j] = static_cast<T>(fv[j]);
             }
         };
-        // group Ng - 1 (slot 0) on its own, so the loop body has no branch
-        step(std::integral_constant<int, 0>{}, Ng - 1, std::true_type{});
-        int g0 = Ng - 2;  // in slot 1
+        // group G0 (slot 0) on its own, so the loop body has no branch
+        step(std::integral_constant<int, 0>{}, G0, std::true_type{});
+        int g0 = G0 - 1;  // in slot 1
         for (; g0 >= RING - 1; g0 -= RING)
             unroll<RING>([&](auto i) {
                 step(std::integral_constant<int, (decltype(i)::value + 1) % RING>{},
@@ -370,7 +463,7 @@ __global__ __launch_bounds__(kBlock) void energy_pair_kernel(
         if (SRC) q0 = load_pack<T, LPT>(Qb + lane);
         second(0, none, q0);  // group 0 has no lower neighbour
 #pragma unroll
-        for (int j = 0; j < LPT; ++j) acc_dose += dose[j] * yzw[(lane + j) / nang];
+        for (int j = 0; j < LPT; ++j) acc_dose = fma(dose[j], yzw[(lane + j) / nang], acc_dose);
     }
     const double d = block_sum(acc_dose, lds);
     if (threadIdx.x == 0) partials[blockIdx.x] = d;
@@ -378,63 +471,131 @@ __global__ __launch_bounds__(kBlock) void energy_pair_kernel(
         const double p = block_sum(acc_proxy, lds);
         if (threadIdx.x == 0) partials[gridDim.x + blockIdx.x] = p;
     }
+    if (tops.out) publish_top(top_b, tops.out, nullptr, ldi);
+    if (tops.out_x) publish_top(top_a, tops.out_x, nullptr, ldi);
+}
+
+// First-order implicit upwind energy step: the reference's legacy energy
+// model (energyDgCnKernel, use_legacy branch, src/energy_solver.cu:88-106)
+// and its lite-memory primary step (litePrimaryEnergyStepKernel,
+// src/bfpm_solver.cu:600-622, psi^1 only: f == null).  One thread per lane,
+// high to low energy, in place: the old psi^1 of g+1 (straggling) is carried
+// in a register and the old psi^1 of g-1 is read before it is overwritten.
+// Divisions as in the reference; results flushed like the eq15 sweeps.
+template <typename T, bool SRC, bool DOSE>
+__global__ __launch_bounds__(kBlock) void energy_upwind_kernel(
+    T* __restrict__ F, T* __restrict__ f, const UpwindCoef* __restrict__ uc,
+    const EnergyCoef* __restrict__ coef, const T* __restrict__ Q1, int Ng, int lanes,
+    const double* __restrict__ yzw, int nang, double* __restrict__ partials, int proxy,
+    int straggling, SweepTops tops) {
+    __shared__ double lds[kBlock / 64];
+    __shared__ int ldi[kBlock / 64];
+    const int lane = blockIdx.x * kBlock + threadIdx.x;
+    reset_tops(tops);
+    const int q_top = SRC ? source_top(tops, tops.xa, Ng) : -1;
+    const int G0 = min(Ng - 1, max(state_top(tops, Ng), q_top) + (straggling ? 1 : 0));
+    int top = -1;
+    double acc_dose = 0.0, acc_proxy = 0.0;
+    if (lane < lanes && G0 >= 0) {
+        double next_F = 0.0, next_f = 0.0, dose = 0.0;
+        double F_hi = 0.0;  // old psi^1 at g+1 (zero above G0)
+        double Fg = static_cast<double>(F[static_cast<long long>(G0) * lanes + lane]);
+        for (int g = G0; g >= 0; --g) {
+            const long long idx = static_cast<long long>(g) * lanes + lane;
+            const double F_lo = g > 0 ? static_cast<double>(F[idx - lanes]) : 0.0;
+            const double fg = f ? static_cast<double>(f[idx]) : 0.0;
+            const UpwindCoef c = uc[g];
+            const double src = (SRC && g <= q_top) ? static_cast<double>(Q1[idx]) : 0.0;
+            double Fv = (fmax(Fg, 0.0) / c.h + c.c_hi * next_F + src) / c.denom;
+            double fv = (fg / c.h + c.c_hi * next_f) / c.denom;
+            if (c.strag > 0.0) {
+                const double lap = fmax(F_hi, 0.0) - 2.0 * fmax(Fg, 0.0) + fmax(F_lo, 0.0);
+                Fv += c.strag * lap;
+            }
+            const T sF = static_cast<T>(flush_tiny<T>(fmax(Fv, 0.0)));
+            const T sf = static_cast<T>(flush_tiny<T>(fmax(fv, 0.0)));
+            F[idx] = sF;
+            if (f) f[idx] = sf;
+            next_F = static_cast<double>(sF);
+            next_f = static_cast<double>(sf);
+            top = max(top, (sF != T(0) || sf != T(0)) ? g : -1);
+            if (DOSE) {
+                dose = add_dose(dose, next_F, next_f, coef[g].w1, coef[g].w2);
+                acc_proxy += next_f;
+            }
+            F_hi = Fg;
+            Fg = F_lo;
+        }
+        if (DOSE) acc_dose = dose * yzw[lane / nang];
+    }
+    if (DOSE) {
+        const double d = block_sum(acc_dose, lds);
+        if (threadIdx.x == 0) partials[blockIdx.x] = d;
+        if (proxy) {
+            const double p = block_sum(acc_proxy, lds);
+            if (threadIdx.x == 0) partials[gridDim.x + blockIdx.x] = p;
+        }
+    }
+    if (tops.out) publish_top(top, tops.out, tops.out_x, ldi);
 }
 
 template <typename T, int LPT, bool SRC, bool STRAG>
 void dispatch_pair(T* F, T* f, const EnergyCoef* coef, const T* Qa, const T* Qb, T* XF, T* Xf,
                    int Ng, long long lanes, const double* yzw, int nang, double* partials,
-                   bool proxy, hipStream_t s) {
+                   bool proxy, hipStream_t s, const SweepTops& tops) {
     const int blocks = energy_blocks(lanes, sizeof(T));
     BFPN_REQUIRE(lanes < (1LL << 31), "energy sweep indexes lanes with 32-bit offsets");
     if (XF)
         hipLaunchKernelGGL((energy_pair_kernel<T, LPT, SRC, STRAG, true>), dim3(blocks),
                            dim3(kBlock), 0, s, F, f, coef, Qa, Qb, XF, Xf, Ng,
-                           static_cast<int>(lanes), yzw, nang, partials, proxy ? 1 : 0);
+                           static_cast<int>(lanes), yzw, nang, partials, proxy ? 1 : 0, tops);
     else
         hipLaunchKernelGGL((energy_pair_kernel<T, LPT, SRC, STRAG, false>), dim3(blocks),
                            dim3(kBlock), 0, s, F, f, coef, Qa, Qb, XF, Xf, Ng,
-                           static_cast<int>(lanes), yzw, nang, partials, proxy ? 1 : 0);
+                           static_cast<int>(lanes), yzw, nang, partials, proxy ? 1 : 0, tops);
 }
 
 template <typename T, bool SRC, bool STRAG>
 void dispatch_pair_lpt(T* F, T* f, const EnergyCoef* coef, const T* Qa, const T* Qb, T* XF,
                        T* Xf, int Ng, long long lanes, const double* yzw, int nang,
-                       double* partials, bool proxy, hipStream_t s) {
+                       double* partials, bool proxy, hipStream_t s, const SweepTops& tops) {
     if constexpr (sizeof(T) == 4) {
         const int lpt = lanes_per_thread(lanes, sizeof(T));
         if (lpt == 4)
             return dispatch_pair<T, 4, SRC, STRAG>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yzw,
-                                                   nang, partials, proxy, s);
+                                                   nang, partials, proxy, s, tops);
     }
     dispatch_pair<T, 1, SRC, STRAG>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yzw, nang, partials,
-                                    proxy, s);
+                                    proxy, s, tops);
 }
 
 template <typename T, int LPT, bool SRC, bool STRAG>
 void dispatch_dose(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng, long long lanes,
-                   const double* yzw, int nang, double* partials, bool proxy, hipStream_t s) {
+                   const double* yzw, int nang, double* partials, bool proxy, hipStream_t s,
+                   const SweepTops& tops) {
     const int blocks = energy_blocks(lanes, sizeof(T));
     BFPN_REQUIRE(lanes < (1LL << 31), "energy sweep indexes lanes with 32-bit offsets");
     if (partials)
         hipLaunchKernelGGL((energy_half_kernel<T, LPT, SRC, STRAG, true>), dim3(blocks),
                            dim3(kBlock), 0, s, F, f, coef, Q1, Ng, static_cast<int>(lanes), yzw,
-                           nang, partials, proxy ? 1 : 0);
+                           nang, partials, proxy ? 1 : 0, tops);
     else
         hipLaunchKernelGGL((energy_half_kernel<T, LPT, SRC, STRAG, false>), dim3(blocks),
                            dim3(kBlock), 0, s, F, f, coef, Q1, Ng, static_cast<int>(lanes), yzw,
-                           nang, partials, 0);
+                           nang, partials, 0, tops);
 }
 
 template <typename T, bool SRC, bool STRAG>
 void dispatch_lpt(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng, long long lanes,
-                  const double* yzw, int nang, double* partials, bool proxy, hipStream_t s) {
+                  const double* yzw, int nang, double* partials, bool proxy, hipStream_t s,
+                  const SweepTops& tops) {
     if constexpr (sizeof(T) == 4) {
         const int lpt = lanes_per_thread(lanes, sizeof(T));
         if (lpt == 4)
             return dispatch_dose<T, 4, SRC, STRAG>(F, f, coef, Q1, Ng, lanes, yzw, nang, partials,
-                                                   proxy, s);
+                                                   proxy, s, tops);
     }
-    dispatch_dose<T, 1, SRC, STRAG>(F, f, coef, Q1, Ng, lanes, yzw, nang, partials, proxy, s);
+    dispatch_dose<T, 1, SRC, STRAG>(F, f, coef, Q1, Ng, lanes, yzw, nang, partials, proxy, s, tops);
 }
 
 }  // namespace
@@ -447,22 +608,22 @@ int energy_blocks(long long lanes, size_t elem) {
 template <typename T>
 void launch_energy_half(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng, long long lanes,
                         bool straggling, const double* yz_weight, int nang, double* dose_partials,
-                        bool proxy, hipStream_t s) {
+                        bool proxy, hipStream_t s, const SweepTops& tops) {
     if (lanes <= 0 || Ng <= 0) return;
     if (Q1) {
         if (straggling)
             dispatch_lpt<T, true, true>(F, f, coef, Q1, Ng, lanes, yz_weight, nang, dose_partials,
-                                         proxy, s);
+                                         proxy, s, tops);
         else
             dispatch_lpt<T, true, false>(F, f, coef, Q1, Ng, lanes, yz_weight, nang, dose_partials,
-                                          proxy, s);
+                                          proxy, s, tops);
     } else {
         if (straggling)
             dispatch_lpt<T, false, true>(F, f, coef, Q1, Ng, lanes, yz_weight, nang, dose_partials,
-                                          proxy, s);
+                                          proxy, s, tops);
         else
             dispatch_lpt<T, false, false>(F, f, coef, Q1, Ng, lanes, yz_weight, nang,
-                                           dose_partials, proxy, s);
+                                           dose_partials, proxy, s, tops);
     }
     BFPN_HIP(hipGetLastError());
 }
@@ -470,7 +631,8 @@ void launch_energy_half(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng,
 template <typename T>
 void launch_energy_pair(T* F, T* f, const EnergyCoef* coef, const T* Qa, const T* Qb, int Ng,
                         long long lanes, bool straggling, const double* yz_weight, int nang,
-                        double* dose_partials, bool proxy, T* XF, T* Xf, hipStream_t s) {
+                        double* dose_partials, bool proxy, T* XF, T* Xf, hipStream_t s,
+                        const SweepTops& tops) {
     if (lanes <= 0 || Ng <= 0) return;
     BFPN_REQUIRE(dose_partials, "the paired energy sweep needs dose partials");
     BFPN_REQUIRE((Qa != nullptr) == (Qb != nullptr), "both half steps need a source, or neither");
@@ -478,33 +640,68 @@ void launch_energy_pair(T* F, T* f, const EnergyCoef* coef, const T* Qa, const T
     if (Qa) {
         if (straggling)
             dispatch_pair_lpt<T, true, true>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yz_weight,
-                                              nang, dose_partials, proxy, s);
+                                              nang, dose_partials, proxy, s, tops);
         else
             dispatch_pair_lpt<T, true, false>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yz_weight,
-                                               nang, dose_partials, proxy, s);
+                                               nang, dose_partials, proxy, s, tops);
     } else {
         if (straggling)
             dispatch_pair_lpt<T, false, true>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yz_weight,
-                                               nang, dose_partials, proxy, s);
+                                               nang, dose_partials, proxy, s, tops);
         else
             dispatch_pair_lpt<T, false, false>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yz_weight,
-                                                nang, dose_partials, proxy, s);
+                                                nang, dose_partials, proxy, s, tops);
     }
     BFPN_HIP(hipGetLastError());
 }
 
+template <typename T>
+void launch_energy_upwind(T* F, T* f, const UpwindCoef* uc, const EnergyCoef* coef, const T* Q1,
+                          int Ng, long long lanes, bool straggling, const double* yz_weight,
+                          int nang, double* dose_partials, bool proxy, hipStream_t s,
+                          const SweepTops& tops) {
+    if (lanes <= 0 || Ng <= 0) return;
+    BFPN_REQUIRE(lanes < (1LL << 31), "energy sweep indexes lanes with 32-bit offsets");
+    const int blocks = static_cast<int>((lanes + kBlock - 1) / kBlock);
+    const int st = straggling ? 1 : 0, px = proxy ? 1 : 0;
+    const int n = static_cast<int>(lanes);
+    if (Q1 && dose_partials)
+        hipLaunchKernelGGL((energy_upwind_kernel<T, true, true>), dim3(blocks), dim3(kBlock), 0, s,
+                           F, f, uc, coef, Q1, Ng, n, yz_weight, nang, dose_partials, px, st, tops);
+    else if (Q1)
+        hipLaunchKernelGGL((energy_upwind_kernel<T, true, false>), dim3(blocks), dim3(kBlock), 0, s,
+                           F, f, uc, coef, Q1, Ng, n, yz_weight, nang, dose_partials, 0, st, tops);
+    else if (dose_partials)
+        hipLaunchKernelGGL((energy_upwind_kernel<T, false, true>), dim3(blocks), dim3(kBlock), 0, s,
+                           F, f, uc, coef, Q1, Ng, n, yz_weight, nang, dose_partials, px, st, tops);
+    else
+        hipLaunchKernelGGL((energy_upwind_kernel<T, false, false>), dim3(blocks), dim3(kBlock), 0,
+                           s, F, f, uc, coef, Q1, Ng, n, yz_weight, nang, dose_partials, 0, st,
+                           tops);
+    BFPN_HIP(hipGetLastError());
+}
+
+template void launch_energy_upwind<float>(float*, float*, const UpwindCoef*, const EnergyCoef*,
+                                          const float*, int, long long, bool, const double*, int,
+                                          double*, bool, hipStream_t, const SweepTops&);
+template void launch_energy_upwind<double>(double*, double*, const UpwindCoef*, const EnergyCoef*,
+                                           const double*, int, long long, bool, const double*, int,
+                                           double*, bool, hipStream_t, const SweepTops&);
+
 template void launch_energy_pair<float>(float*, float*, const EnergyCoef*, const float*,
                                         const float*, int, long long, bool, const double*, int,
-                                        double*, bool, float*, float*, hipStream_t);
+                                        double*, bool, float*, float*, hipStream_t,
+                                        const SweepTops&);
 template void launch_energy_pair<double>(double*, double*, const EnergyCoef*, const double*,
                                          const double*, int, long long, bool, const double*, int,
-                                         double*, bool, double*, double*, hipStream_t);
+                                         double*, bool, double*, double*, hipStream_t,
+                                         const SweepTops&);
 
 template void launch_energy_half<float>(float*, float*, const EnergyCoef*, const float*, int,
                                         long long, bool, const double*, int, double*, bool,
-                                        hipStream_t);
+                                        hipStream_t, const SweepTops&);
 template void launch_energy_half<double>(double*, double*, const EnergyCoef*, const double*, int,
                                          long long, bool, const double*, int, double*, bool,
-                                         hipStream_t);
+                                         hipStream_t, const SweepTops&);
 
 }  // namespace bfpn

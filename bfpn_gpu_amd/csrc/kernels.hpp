@@ -3,7 +3,30 @@
 
 #include "common.hpp"
 
+#include <vector>
+
 namespace bfpn {
+
+// Active energy groups.  Above the highest nonzero group ("top") of a beam
+// every group of the state is exactly zero: the energy sweep moves particles
+// down in energy (straggling spreads them at most one group up per half
+// step), the secondary source emits at or below its source group, and
+// transport and angular diffusion never mix groups.  The operators therefore
+// skip the groups above the top -- each result they would compute there is
+// an exact zero -- and the energy sweep measures the exact top of what it
+// writes.  Tops are device ints (-1: no nonzero group), so no host round
+// trip is needed; a null pointer means all Ng groups.  A sweep presets (to
+// -1) the slots the next sweep writes, so tops rotate over small rings.
+struct SweepTops {
+    const int* in = nullptr;    // top of the state before the sweep
+    const int* xa = nullptr;    // top of the primary state source Qa was made from
+    const int* xb = nullptr;    // the same for Qb (paired sweeps)
+    const int* qmap = nullptr;  // top of a source from its primary's top t: qmap[t + 1]
+    int* out = nullptr;         // receives the top of the result (preset to -1)
+    int* out_x = nullptr;       // paired: top of the state between the halves; single: copy of out
+    int* reset = nullptr;       // set to -1 by the sweep (the slot a later sweep writes)
+    int* reset_x = nullptr;
+};
 
 // Eq.15 energy half step, in place.  coef: Ng device EnergyCoef.  Q1 may be
 // null.  When dose != null each block writes FP64 partials:
@@ -13,7 +36,8 @@ namespace bfpn {
 template <typename T>
 void launch_energy_half(T* F, T* f, const EnergyCoef* coef, const T* Q1, int Ng, long long lanes,
                         bool straggling, const double* yz_weight,
-                        int nang, double* dose_partials, bool proxy, hipStream_t s);
+                        int nang, double* dose_partials, bool proxy, hipStream_t s,
+                        const SweepTops& tops = SweepTops{});
 // Dose partials written by a sweep over `lanes` lanes of `elem`-byte state.
 int energy_blocks(long long lanes, size_t elem);
 
@@ -23,7 +47,19 @@ int energy_blocks(long long lanes, size_t elem);
 template <typename T>
 void launch_energy_pair(T* F, T* f, const EnergyCoef* coef, const T* Qa, const T* Qb, int Ng,
                         long long lanes, bool straggling, const double* yz_weight, int nang,
-                        double* dose_partials, bool proxy, T* XF, T* Xf, hipStream_t s);
+                        double* dose_partials, bool proxy, T* XF, T* Xf, hipStream_t s,
+                        const SweepTops& tops = SweepTops{});
+// First-order upwind energy step, in place (legacy energy model; the
+// lite-memory step with f == null).  uc: Ng UpwindCoef; coef supplies the
+// dose weights w1/w2 when dose_partials != null (partials as above).
+template <typename T>
+void launch_energy_upwind(T* F, T* f, const UpwindCoef* uc, const EnergyCoef* coef, const T* Q1,
+                          int Ng, long long lanes, bool straggling, const double* yz_weight,
+                          int nang, double* dose_partials, bool proxy, hipStream_t s,
+                          const SweepTops& tops = SweepTops{});
+// Block count of launch_energy_upwind's partials.
+inline int upwind_blocks(long long lanes) { return static_cast<int>((lanes + 255) / 256); }
+
 // Split-BF16 operands of the FP32 source (matrix cores), built once per beam.
 struct SourcePlan {
     const __bf16* kth = nullptr;  // Kv transposed [g][out][in], nap x nap, hi part
@@ -53,30 +89,41 @@ int source_mtiles(int nang);  // 64-direction output tiles of the angular GEMM
 int source_ktiles(int nang);  // 32-wide K tiles
 // FP32 source on the matrix cores: A1/A2 scratch, Q output.
 // `inputs_read` (optional) is recorded on s once the angular GEMM, the only
-// reader of Fp and fp, has been issued.
+// reader of Fp and fp, has been issued.  xtop/qmap (optional, device; both or
+// neither): the primary's top group and the source-top map of SweepTops;
+// rows of Q above the source's top are then left unwritten.
 void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, float* Q,
                         const SourcePlan& plan, int Ng, int nyz, int nang, hipStream_t s,
-                        hipEvent_t inputs_read = nullptr);
+                        hipEvent_t inputs_read = nullptr, const int* xtop = nullptr,
+                        const int* qmap = nullptr);
 
 // Secondary source (FP64 vector path): A1/A2 angular contraction, banded sum.
+// kcount/klist (optional, device; from source_angular_blocks): per (g,
+// 64-direction tile) the 16-deep K blocks of Kv with a nonzero entry.
 template <typename T>
 void launch_source_angular(const T* Fp, const T* fp, const T* kv, T* A1, T* A2, int Ng, int nyz,
-                           int nang, hipStream_t s);
+                           int nang, hipStream_t s, const int* xtop = nullptr,
+                           const int* kcount = nullptr, const int* klist = nullptr);
+// Host: the nonzero K blocks of Kv ([g][in][out], FP64) for launch_source_angular.
+void source_angular_blocks(const std::vector<double>& kv, int Ng, int nang, std::vector<int>& count,
+                           std::vector<int>& list);
 template <typename T>
 void launch_source_energy(const T* A1, const T* A2, const double* e1, const double* e2,
                           const int* beg, const int* end, T* Q, int Ng, long long lanes, double dg,
-                          hipStream_t s);
+                          hipStream_t s, const int* xtop = nullptr, const int* qmap = nullptr);
 
 // MUSCL predictor-corrector transport half step, in place (no scratch).
+// top (optional, device): groups above it are zero and are skipped.
 template <typename T>
 void launch_transport_half(T* F, const T* omend, int Ny, int Nz, int Ng, int nang, double dy,
-                           double dz, double h, bool preserve_sign, hipStream_t s);
+                           double dz, double h, bool preserve_sign, hipStream_t s,
+                           const int* top = nullptr);
 
 // CN angular diffusion on the DST-I basis, in place.  factors: Ng*nang (T),
 // su: Nom*Nom, sv: Nmu*Nmu (T).
 template <typename T>
 void launch_angle_full(T* F, const T* factors, const T* su, const T* sv, int Ng, int nyz, int Nmu,
-                       int Nom, hipStream_t s);
+                       int Nom, hipStream_t s, const int* top = nullptr);
 
 // Sine bases of the MFMA DST: 16 x the basis (a power of two, exact), split
 // into three FP16 terms (hi + mid + lo) from the FP64 values, 32x32 zero
@@ -98,7 +145,7 @@ struct AnglePlan {
 constexpr int chain_index(int slot) { return 16 * ((slot & 7) >> 2) + 4 * (slot >> 3) + (slot & 3); }
 // FP32 DST on the matrix cores (Nmu, Nom <= 32), in place.
 void launch_angle_mfma(float* F, const float* factors, const AnglePlan& plan, int Ng, int nyz,
-                       int Nmu, int Nom, hipStream_t s);
+                       int Nmu, int Nom, hipStream_t s, const int* top = nullptr);
 
 // Eq.24 dose of (F+F1, f+f1) into per-block partials (count = dose_blocks()).
 template <typename T>

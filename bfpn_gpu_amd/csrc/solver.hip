@@ -197,7 +197,8 @@ std::string physics_key(const bfpn_grid& g, const bfpn_phys& p, const std::strin
       << ',' << g.Nmu << ',' << g.Nom << '|' << g.Lx << ',' << g.Ly << ',' << g.Lz << ',' << g.Lu
       << ',' << g.Lv << ',' << g.Lg << '|' << p.density << ',' << p.energy_density << '|'
       << p.material_name << '|' << p.primary_only << p.eq15_straggling << p.energy_only
-      << p.trapezoidal_yz << p.dose_catastrophic_loss << p.calibrate_cross_sections;
+      << p.trapezoidal_yz << p.dose_catastrophic_loss << p.calibrate_cross_sections
+      << p.legacy_energy << p.lite_memory;
     return k.str();
 }
 
@@ -205,9 +206,31 @@ namespace {
 
 enum Phase { kEnergy = 0, kTransport = 1, kAngle = 2, kSource = 3, kDose = 4, kWait = 5 };
 
+// Top group of the secondary source made from a primary state whose top is
+// t: entry t + 1 (t = -1 .. Ng-1) is the highest output group whose kernel
+// row has a nonzero entry at a source group <= t (-1: none).
+std::vector<int> source_top_map(const SourceKernels& ker, int Ng) {
+    std::vector<int> first(Ng, -1), map(static_cast<size_t>(Ng) + 1, -1);
+    for (int g = 0; g < Ng; ++g) {
+        const int b = std::max(0, ker.beg[g]), e = std::min(Ng, ker.end[g]);
+        if (b < e) first[b] = std::max(first[b], g);
+    }
+    for (int t = 0; t < Ng; ++t) map[t + 1] = std::max(map[t], first[t]);
+    return map;
+}
+
+// Highest group with a nonzero beam value (-1: none).
+int beam_top(const BeamFactors& b) {
+    for (int g = static_cast<int>(b.energy.size()) - 1; g >= 0; --g)
+        if (b.energy[g] != 0.0) return g;
+    return -1;
+}
+
 template <typename T>
 struct Physics final : PhysicsBase {
-    DeviceBuffer coef, dose_w, yzw, om_dev, factors, su, sv, kv, e1, e2, beg, end;
+    DeviceBuffer coef, dose_w, yzw, om_dev, factors, su, sv, kv, e1, e2, beg, end, qmap, ucoef;
+    DeviceBuffer kcount, klist;  // FP64 source: nonzero Kv blocks
+    bool legacy = false, lite = false;  // upwind energy model; lite-memory primary step
     SourcePlanBuffers plan;
     AnglePlanBuffers angle_plan;
 
@@ -219,7 +242,9 @@ struct Physics final : PhysicsBase {
         nang = g.Nmu * g.Nom;
         lanes = static_cast<long long>(nyz) * nang;
         cells = static_cast<size_t>(g.Ng) * lanes;
-        secondary = !p.primary_only && !p.energy_only;
+        secondary = !p.primary_only && !p.energy_only && !p.lite_memory;
+        legacy = p.legacy_energy != 0;
+        lite = p.lite_memory != 0;
         key = physics_key(g, p, data_path, prec);
 
         // host tables (initialize, src/bfpm_solver.cu:1435-1568)
@@ -243,6 +268,11 @@ struct Physics final : PhysicsBase {
             }
             upload(coef, ec);
             upload(dose_w, dc);
+            if (legacy || lite)  // lite: one full step on psi^1 without density or straggling
+                upload(ucoef, upwind_coefs(S, mat.sigma, Tc.data(),
+                                           !lite && p.eq15_straggling != 0,
+                                           lite ? d.dt : 0.5 * d.dt, d.dg,
+                                           lite ? 1.0 : p.energy_density, g.Ng));
             upload(yzw, yz_weights(g.Ny, g.Nz, p.trapezoidal_yz != 0));
             upload(om_dev, cast_vector<T>(om));
             upload(factors, cast_vector<T>(angle_factors(sig_tr.data(), g.Ng, g.Nmu, g.Nom, d.du,
@@ -252,6 +282,7 @@ struct Physics final : PhysicsBase {
             if (std::is_same_v<T, float> && AnglePlanBuffers::usable(g.Nmu, g.Nom))
                 angle_plan.build(g.Nmu, g.Nom, s);
             if (secondary) {
+                upload(qmap, source_top_map(ker, g.Ng));
                 upload(kv, cast_vector<T>(ker.v));
                 if constexpr (std::is_same_v<T, float>) {
                     plan.build(kv.as<float>(), ker.e1, ker.e2, ker.beg, ker.end, g.Ng, nang, d.dg, s);
@@ -261,6 +292,10 @@ struct Physics final : PhysicsBase {
                     upload(e2, ker.e2);
                     upload(beg, ker.beg);
                     upload(end, ker.end);
+                    std::vector<int> cnt, lst;
+                    source_angular_blocks(ker.v, g.Ng, nang, cnt, lst);
+                    upload(kcount, cnt);
+                    upload(klist, lst);
                 }
             }
             BFPN_HIP(hipStreamSynchronize(s));
@@ -272,10 +307,12 @@ struct Physics final : PhysicsBase {
     }
 
     size_t device_bytes() const override {
-        return coef.bytes() + dose_w.bytes() + yzw.bytes() + om_dev.bytes() + factors.bytes() +
+        return coef.bytes() + dose_w.bytes() + yzw.bytes() + ucoef.bytes() + om_dev.bytes() +
+               factors.bytes() +
                su.bytes() + sv.bytes() + kv.bytes() + e1.bytes() + e2.bytes() + beg.bytes() +
-               end.bytes() + plan.bytes() + angle_plan.h.bytes() + angle_plan.m.bytes() +
-               angle_plan.l.bytes();
+               end.bytes() + qmap.bytes() + kcount.bytes() + klist.bytes() + plan.bytes() +
+               angle_plan.h.bytes() +
+               angle_plan.m.bytes() + angle_plan.l.bytes();
     }
 };
 
@@ -299,7 +336,8 @@ public:
             BFPN_HIP(hipEventCreateWithFlags(&ev_tail_, hipEventDisableTiming));
         }
         const size_t state_bytes = cells * sizeof(T);
-        for (auto& b : state_) b.ensure(state_bytes);
+        // lite-memory keeps psi^1 of the primary only
+        for (int k = 0; k < (P_->lite ? 1 : 4); ++k) state_[k].ensure(state_bytes);
         if (P_->secondary) {
             a1_.ensure(state_bytes);
             a2_.ensure(state_bytes);
@@ -307,16 +345,21 @@ public:
             for (auto& x : x_) x.ensure(state_bytes);
         }
         for (int k = 0; k < 4; ++k) cur_[k] = state_[k].as<T>();
-        const int eb = energy_blocks(P_->lanes, sizeof(T));
-        partial_p_.ensure(4 * 2 * eb * sizeof(double));  // four slots: steps rotate
-        partial_s_.ensure(eb * sizeof(double));
+        eb_ = (P_->legacy || P_->lite) ? upwind_blocks(P_->lanes)
+                                       : energy_blocks(P_->lanes, sizeof(T));
+        partial_p_.ensure(4 * 2 * eb_ * sizeof(double));  // four slots: steps rotate
+        partial_s_.ensure(eb_ * sizeof(double));
         grow_history(std::max(16, g.Nx));
+        // active-group tops: primary ring, secondary ring, end-of-step primary ring
+        tops_.ensure(kTopSlots * sizeof(int));
+        init_tops(beam_top(beam), -1);
         // Eq.23 beam on the device (the reference builds a full host array)
         DeviceBuffer sa, ee;
         upload(sa, beam.space_angle);
         upload(ee, beam.energy);
         launch_beam<T>(cur_[0], sa.as<double>(), ee.as<double>(), g.Ng, P_->lanes, stream);
-        for (int k = 1; k < 4; ++k) BFPN_HIP(hipMemsetAsync(cur_[k], 0, state_bytes, stream));
+        for (int k = 1; k < 4; ++k)
+            if (cur_[k]) BFPN_HIP(hipMemsetAsync(cur_[k], 0, state_bytes, stream));
         BFPN_HIP(hipStreamSynchronize(stream));
     }
 
@@ -346,6 +389,9 @@ public:
     // Eq.25 moments of the current state (computeEnergyMoments,
     // src/bfpm_solver.cu:3504-3526): [g][psi^1, psi^2] x du dv dy dz.
     void energy_moments(double* host) override {
+        if (P_->lite)
+            throw Error(BFPN_E_UNSUPPORTED, "energy moments need psi^2 and the secondary "
+                                            "(not kept with --lite-memory)");
         sync_all();
         const long long per = static_cast<long long>(P_->lanes);
         diag_part_.ensure(moment_partial_count(grid.Ng, per) * sizeof(double));
@@ -361,6 +407,9 @@ public:
     // Y-Z dose plane of the current state (computeSpotDosePlane,
     // src/bfpm_solver.cu:3560-3585): node s = iz (Ny+1) + iy.
     void spot_plane(double* host) override {
+        if (P_->lite)
+            throw Error(BFPN_E_UNSUPPORTED, "spot planes need psi^2 and the secondary "
+                                            "(not kept with --lite-memory)");
         sync_all();
         const int nyz = static_cast<int>(P_->nyz);
         diag_out_.ensure(static_cast<size_t>(nyz) * sizeof(double));
@@ -381,7 +430,12 @@ public:
             BFPN_HIP(hipEventRecord(ev_tail_, stream2_));
             BFPN_HIP(hipStreamWaitEvent(stream, ev_tail_, 0));
         }
-        for (int k = 0; k < nsteps; ++k) step(k == 0, k == nsteps - 1);
+        for (int k = 0; k < nsteps; ++k) {
+            if (P_->lite)
+                lite_step();
+            else  // the upwind model has no paired sweep: every step stands alone
+                step(k == 0 || P_->legacy, k == nsteps - 1 || P_->legacy);
+        }
         if (stream2_ && !overlap) {  // all on `stream`: done_stream still ends last
             BFPN_HIP(hipEventRecord(ev_tail_, stream));
             BFPN_HIP(hipStreamWaitEvent(stream2_, ev_tail_, 0));
@@ -406,6 +460,10 @@ public:
         BFPN_REQUIRE(which >= 0 && which < 4, "state index must be 0..3");
         BFPN_REQUIRE(count == cells, "state count mismatch");
         sync_all();
+        if (!cur_[which]) {  // not kept (lite-memory): identically zero
+            std::fill(host, host + cells, 0.0);
+            return;
+        }
         std::vector<T> tmp(cells);
         BFPN_HIP(hipMemcpy(tmp.data(), cur_[which], cells * sizeof(T), hipMemcpyDeviceToHost));
         for (size_t k = 0; k < cells; ++k) host[k] = static_cast<double>(tmp[k]);
@@ -414,12 +472,87 @@ public:
     void write_state(int which, const double* host, size_t count) override {
         BFPN_REQUIRE(which >= 0 && which < 4, "state index must be 0..3");
         BFPN_REQUIRE(count == cells, "state count mismatch");
+        if (!cur_[which])
+            throw Error(BFPN_E_UNSUPPORTED, "this state array is not kept with --lite-memory");
         std::vector<T> tmp(host, host + cells);
         sync_all();
         BFPN_HIP(hipMemcpy(cur_[which], tmp.data(), cells * sizeof(T), hipMemcpyHostToDevice));
+        // the written values may reach any group: that species' top is Ng-1
+        const int all = grid.Ng - 1;
+        BFPN_HIP(hipMemcpy(top_slot(which < 2 ? p_cur_ : 3 + s_cur_), &all, sizeof(int),
+                           hipMemcpyHostToDevice));
+    }
+
+    // Active-group skipping (on by default): bit-identical either way.
+    // Turning it on makes every top conservative (Ng-1) until the next sweeps
+    // have measured them.
+    void set_group_skip(bool on) override {
+        if (on && !skip_) {
+            sync_all();
+            init_tops(grid.Ng - 1, grid.Ng - 1);
+        }
+        skip_ = on;
+    }
+
+    void active_tops(int* primary, int* secondary) override {
+        sync_all();
+        int h[kTopSlots];
+        BFPN_HIP(hipMemcpy(h, tops_.as<int>(), sizeof(h), hipMemcpyDeviceToHost));
+        if (primary) *primary = skip_ ? h[p_cur_] : grid.Ng - 1;
+        if (secondary) *secondary = skip_ ? h[3 + s_cur_] : grid.Ng - 1;
     }
 
 private:
+    // Tops (see SweepTops): slots 0-2 the primary's ring, 3-5 the
+    // secondary's, 6-13 the primary state at the end of step n (slot
+    // 6 + n%8), which the source of step n and the secondary sweeps using that
+    // source (steps n and n+1) read.  A sweep reads its species' current
+    // slot, writes the next and presets the one after; the closing primary
+    // sweep of step n writes slot 6 + n%8 and presets 6 + (n+1)%8.  The last
+    // readers of that preset slot, step n-7's, are long done: the primary of
+    // step n-1 has waited for the source of step n-2, which the secondary
+    // stream runs after the sweeps of step n-3.
+    static constexpr int kTopSlots = 14;
+    int* top_slot(int k) { return tops_.as<int>() + k; }
+    int* x_top(int n) { return top_slot(6 + (n & 7)); }
+    const int* cur_top(int species) {
+        return skip_ ? top_slot(species == 0 ? p_cur_ : 3 + s_cur_) : nullptr;
+    }
+    // Host-side reset: current tops of the two species, the rest preset
+    // (the slot the next closing sweep writes: -1; the others conservative).
+    void init_tops(int primary, int secondary) {
+        std::vector<int> h(kTopSlots, -1);
+        h[p_cur_] = primary;
+        h[3 + s_cur_] = secondary;
+        for (int k = 0; k < 8; ++k) h[6 + k] = grid.Ng - 1;
+        h[6 + (step_ & 7)] = -1;
+        BFPN_HIP(hipMemcpy(tops_.as<int>(), h.data(), h.size() * sizeof(int),
+                           hipMemcpyHostToDevice));
+    }
+    // Tops of the next energy sweep of `species`, which advances its ring:
+    // sources made at steps xa / xb (-1: none), end-of-step top written for
+    // step x_out (-1: none).  Empty when skipping is off.
+    SweepTops sweep_tops(int species, int xa, int xb, int x_out) {
+        SweepTops t;
+        if (!skip_) return t;
+        int& cur = species == 0 ? p_cur_ : s_cur_;
+        int* ring = top_slot(3 * species);
+        t.in = ring + cur;
+        t.out = ring + (cur + 1) % 3;
+        t.reset = ring + (cur + 2) % 3;
+        cur = (cur + 1) % 3;
+        if (xa >= 0) {
+            t.xa = x_top(xa);
+            t.qmap = P_->qmap.template as<int>();
+        }
+        if (xb >= 0) t.xb = x_top(xb);
+        if (x_out >= 0) {
+            t.out_x = x_top(x_out);
+            t.reset_x = x_top(x_out + 1);
+        }
+        return t;
+    }
+
     void sync_all() {
         BFPN_HIP(hipStreamSynchronize(stream));
         if (stream2_) BFPN_HIP(hipStreamSynchronize(stream2_));
@@ -474,10 +607,20 @@ private:
     double array_bytes() const { return static_cast<double>(cells) * sizeof(T); }
 
     // One energy half step of a species (source Q or none; dose partials or none).
-    void energy(int species, const T* Q, double* partials, bool proxy, hipStream_t s) {
+    void energy(int species, const T* Q, double* partials, bool proxy, hipStream_t s,
+                const SweepTops& tops) {
+        if (P_->legacy) {
+            launch_energy_upwind<T>(cur_[2 * species], cur_[2 * species + 1],
+                                    P_->ucoef.template as<UpwindCoef>(),
+                                    P_->coef.template as<EnergyCoef>(), Q, grid.Ng, P_->lanes,
+                                    phys.eq15_straggling != 0, P_->yzw.template as<double>(),
+                                    P_->nang, partials, proxy, s, tops);
+            mark(kEnergy, array_bytes() * (Q ? 5 : 4), s);
+            return;
+        }
         launch_energy_half<T>(cur_[2 * species], cur_[2 * species + 1], P_->coef.template as<EnergyCoef>(),
                               Q, grid.Ng, P_->lanes, phys.eq15_straggling != 0,
-                              P_->yzw.template as<double>(), P_->nang, partials, proxy, s);
+                              P_->yzw.template as<double>(), P_->nang, partials, proxy, s, tops);
         mark(kEnergy, array_bytes() * (Q ? 5 : 4), s);
     }
 
@@ -485,17 +628,18 @@ private:
     // sweep (sources Qa, Qb or none; the dose of the first); X (optional)
     // receives the state at the end of the step.
     void energy_pair(int species, const T* Qa, const T* Qb, double* partials, bool proxy, T* const* X,
-                     hipStream_t s) {
+                     hipStream_t s, const SweepTops& tops) {
         launch_energy_pair<T>(cur_[2 * species], cur_[2 * species + 1], P_->coef.template as<EnergyCoef>(),
                               Qa, Qb, grid.Ng, P_->lanes, phys.eq15_straggling != 0,
                               P_->yzw.template as<double>(), P_->nang, partials, proxy,
-                              X ? X[0] : nullptr, X ? X[1] : nullptr, s);
+                              X ? X[0] : nullptr, X ? X[1] : nullptr, s, tops);
         mark(kEnergy, array_bytes() * (4 + (Qa ? 2 : 0) + (X ? 2 : 0)), s);
     }
 
     void transport(int which, bool preserve_sign, hipStream_t s) {
         launch_transport_half<T>(cur_[which], P_->om_dev.template as<T>(), grid.Ny, grid.Nz, grid.Ng,
-                                 P_->nang, d.dy, d.dz, 0.5 * d.dt, preserve_sign, s);
+                                 P_->nang, d.dy, d.dz, 0.5 * d.dt, preserve_sign, s,
+                                 cur_top(which / 2));
         mark(kTransport, array_bytes() * 2, s);
     }
 
@@ -503,32 +647,58 @@ private:
         if constexpr (std::is_same_v<T, float>) {
             if (P_->angle_plan.plan.su_h) {
                 launch_angle_mfma(cur_[which], P_->factors.template as<float>(), P_->angle_plan.plan,
-                                  grid.Ng, P_->nyz, grid.Nmu, grid.Nom, s);
+                                  grid.Ng, P_->nyz, grid.Nmu, grid.Nom, s, cur_top(which / 2));
                 mark(kAngle, array_bytes() * 2, s);
                 return;
             }
         }
         launch_angle_full<T>(cur_[which], P_->factors.template as<T>(), P_->su.template as<T>(),
-                             P_->sv.template as<T>(), grid.Ng, P_->nyz, grid.Nmu, grid.Nom, s);
+                             P_->sv.template as<T>(), grid.Ng, P_->nyz, grid.Nmu, grid.Nom, s,
+                             cur_top(which / 2));
         mark(kAngle, array_bytes() * 2, s);
     }
 
     // Secondary source Q from the primary state (Fp, fp) on stream s;
     // `inputs_read` (optional) is recorded on s once (Fp, fp) have been read
     // (after the angular GEMM).
-    void source(const T* Fp, const T* fp, T* Q, hipStream_t s, hipEvent_t inputs_read) {
+    // With skipping on, `n` names the step whose end-of-step primary top
+    // bounds the source groups.
+    void source(const T* Fp, const T* fp, T* Q, hipStream_t s, hipEvent_t inputs_read, int n) {
+        const int* xt = skip_ ? x_top(n) : nullptr;
+        const int* qm = skip_ ? P_->qmap.template as<int>() : nullptr;
         if constexpr (std::is_same_v<T, float>) {
             launch_source_mfma(Fp, fp, a1_.as<float>(), a2_.as<float>(), Q, P_->plan.plan,
-                               grid.Ng, P_->nyz, P_->nang, s, inputs_read);
+                               grid.Ng, P_->nyz, P_->nang, s, inputs_read, xt, qm);
         } else {
             launch_source_angular<T>(Fp, fp, P_->kv.template as<T>(), a1_.as<T>(), a2_.as<T>(),
-                                     grid.Ng, P_->nyz, P_->nang, s);
+                                     grid.Ng, P_->nyz, P_->nang, s, xt,
+                                     P_->kcount.template as<int>(), P_->klist.template as<int>());
             if (inputs_read) BFPN_HIP(hipEventRecord(inputs_read, s));
             launch_source_energy<T>(a1_.as<T>(), a2_.as<T>(), P_->e1.template as<double>(),
                                     P_->e2.template as<double>(), P_->beg.template as<int>(),
-                                    P_->end.template as<int>(), Q, grid.Ng, P_->lanes, d.dg, s);
+                                    P_->end.template as<int>(), Q, grid.Ng, P_->lanes, d.dg, s,
+                                    xt, qm);
         }
         mark(kSource, array_bytes() * 7, s);  // 2 in, 2 out; 2 in, 1 out
+    }
+
+    // One lite-memory depth step (src/bfpm_solver.cu:2191-2212): the upwind
+    // energy step over dt on psi^1 (stepLitePrimary, 3396-3416) and the IDD of
+    // psi^1 alone (computeIntegratedDepthDoseLite, 3528-3549); the proxy
+    // history stays zero as in the reference.
+    void lite_step() {
+        const int n = step_;
+        double* pp = partial_p_.as<double>() + (n & 3) * 2 * eb_;
+        mark(kWait, 0.0, stream);  // opens the step's first interval
+        launch_energy_upwind<T>(cur_[0], nullptr, P_->ucoef.template as<UpwindCoef>(),
+                                P_->coef.template as<EnergyCoef>(), nullptr, grid.Ng, P_->lanes,
+                                false, P_->yzw.template as<double>(), P_->nang, pp, false, stream,
+                                sweep_tops(0, -1, -1, n));
+        mark(kEnergy, array_bytes() * 2, stream);
+        launch_finish_sum(pp, eb_, nullptr, 0, d.du * d.dv * d.dy * d.dz / phys.density,
+                          hist_.as<double>() + n, stream);
+        mark(kDose, 0.0, stream);
+        ++step_;
     }
 
     // One depth step (BFPnSolver::solve loop body, src/bfpm_solver.cu:2213-2242)
@@ -554,7 +724,7 @@ private:
     // the primary waits for before it writes slot n again at step n+4).
     void step(bool first, bool last) {
         const bool preserve_primary = phys.no_spatial_clipping != 0;
-        const int eb = energy_blocks(P_->lanes, sizeof(T));
+        const int eb = eb_;
         const int n = step_;
         double* pp = partial_p_.as<double>() + (n & 3) * 2 * eb;
         double* ps = partial_s_.as<double>();
@@ -572,7 +742,7 @@ private:
         if (first) {
             wait_source();
             mark(kWait, 0.0, sp);  // opens the first interval
-            energy(0, nullptr, nullptr, false, sp);
+            energy(0, nullptr, nullptr, false, sp, sweep_tops(0, -1, -1, -1));
         }
         if (!phys.energy_only) {
             if (!phys.no_transport) {
@@ -589,11 +759,12 @@ private:
             }
         }
         if (last) {
-            energy(0, nullptr, pp, true, sp);
+            energy(0, nullptr, pp, true, sp, sweep_tops(0, -1, -1, n));
         } else {
             wait_source();  // x_ is about to be rewritten
             if (!first) mark(kWait, 0.0, sp);
-            energy_pair(0, nullptr, nullptr, pp, true, secondary ? X : nullptr, sp);
+            energy_pair(0, nullptr, nullptr, pp, true, secondary ? X : nullptr, sp,
+                        sweep_tops(0, -1, -1, n));
         }
         const T* Fp = last ? cur_[0] : X[0];  // primary state at the end of step n
         const T* fp = last ? cur_[1] : X[1];
@@ -614,13 +785,14 @@ private:
         const bool sec = secondary && n > 0;
         if (sec) {
             T* Q = q_[n & 1].as<T>();
-            source(Fp, fp, Q, ss, two ? ev_src_ : nullptr);
+            source(Fp, fp, Q, ss, two ? ev_src_ : nullptr, n);
             src_pending_ = two;
             if (sec_open_) {  // end half of step n-1 with the start half of step n
-                energy_pair(1, q_[(n - 1) & 1].as<T>(), Q, ps, false, nullptr, ss);
+                energy_pair(1, q_[(n - 1) & 1].as<T>(), Q, ps, false, nullptr, ss,
+                            sweep_tops(1, n - 1, n, -1));
                 finish(n - 1, true);
             } else {
-                energy(1, Q, nullptr, false, ss);
+                energy(1, Q, nullptr, false, ss, sweep_tops(1, n, -1, -1));
             }
             transport(2, false, ss);  // secondary psi^1 is always clipped
             transport(3, true, ss);
@@ -629,7 +801,7 @@ private:
             transport(2, false, ss);
             transport(3, true, ss);
             if (last) {
-                energy(1, Q, ps, false, ss);
+                energy(1, Q, ps, false, ss, sweep_tops(1, n, -1, -1));
                 finish(n, true);
                 sec_open_ = false;
             } else {
@@ -644,9 +816,12 @@ private:
     std::shared_ptr<Physics<T>> P_;
     DeviceBuffer state_[4], a1_, a2_, q_[2], x_[2];
     DeviceBuffer diag_part_, diag_out_;  // diagnostics scratch, allocated on first use
-    DeviceBuffer partial_p_, partial_s_, hist_;
+    DeviceBuffer partial_p_, partial_s_, hist_, tops_;
     T* cur_[4] = {nullptr, nullptr, nullptr, nullptr};
     int step_ = 0;
+    int p_cur_ = 0, s_cur_ = 0;  // current slots of the primary / secondary top rings
+    int eb_ = 0;                 // blocks (dose partials) of a closing energy sweep
+    bool skip_ = true;           // active-group skipping
     int hist_cap_ = 0;
     std::vector<std::pair<hipEvent_t, Phase>> events_[2];  // per stream (profiling)
     hipStream_t stream2_ = nullptr;  // source + secondary; null without a secondary
@@ -677,14 +852,14 @@ size_t context_bytes(const bfpn_grid& g, const bfpn_phys& p, int precision) {
     const size_t lanes = static_cast<size_t>(g.Ny + 1) * (g.Nz + 1) * g.Nmu * g.Nom;
     const size_t cells = lanes * g.Ng;
     const size_t word = precision == BFPN_FP64 ? 8 : 4;
-    const bool sec = !p.primary_only && !p.energy_only;
-    return cells * word * (sec ? 10 : 4) + 9 * ((lanes + 255) / 256) * 8 +
+    const bool sec = !p.primary_only && !p.energy_only && !p.lite_memory;
+    return cells * word * (p.lite_memory ? 1 : sec ? 10 : 4) + 9 * ((lanes + 255) / 256) * 8 +
            2 * static_cast<size_t>(std::max(16, g.Nx)) * 8;
 }
 
 size_t physics_bytes(const bfpn_grid& g, const bfpn_phys& p, int precision) {
     const size_t nang = static_cast<size_t>(g.Nmu) * g.Nom;
-    const bool sec = !p.primary_only && !p.energy_only;
+    const bool sec = !p.primary_only && !p.energy_only && !p.lite_memory;
     if (!sec) return static_cast<size_t>(g.Ng) * nang * 8 + (1 << 20);
     if (precision == BFPN_FP64) return static_cast<size_t>(g.Ng) * nang * nang * 8 + (1 << 22);
     const size_t nap = source_nap(static_cast<int>(nang));
@@ -1255,6 +1430,20 @@ BFPN_API bfpn_status bfpn_set_overlap(bfpn_ctx* ctx, int on) {
     return guarded([&] {
         BFPN_REQUIRE(ctx, "null context");
         ctx->overlap = on != 0;
+    });
+}
+
+BFPN_API bfpn_status bfpn_set_group_skip(bfpn_ctx* ctx, int on) {
+    return guarded([&] {
+        BFPN_REQUIRE(ctx, "null context");
+        ctx->set_group_skip(on != 0);
+    });
+}
+
+BFPN_API bfpn_status bfpn_active_groups(bfpn_ctx* ctx, int* primary_top, int* secondary_top) {
+    return guarded([&] {
+        BFPN_REQUIRE(ctx, "null context");
+        ctx->active_tops(primary_top, secondary_top);
     });
 }
 

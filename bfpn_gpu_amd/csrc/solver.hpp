@@ -56,6 +56,8 @@ struct bfpn_ctx {
     virtual int steps_done() const = 0;
     virtual void energy_moments(double* host) = 0;  // 2 Ng values
     virtual void spot_plane(double* host) = 0;      // (Ny+1)(Nz+1) values
+    virtual void set_group_skip(bool on) = 0;
+    virtual void active_tops(int* primary, int* secondary) = 0;
 
     bfpn_grid grid{};
     bfpn_phys phys{};

@@ -17,6 +17,9 @@
 // sum regrouped; only rounding order differs.
 
 #include "kernels.hpp"
+
+#include <algorithm>
+#include <vector>
 #include "mfma.hpp"
 
 namespace bfpn {
@@ -52,12 +55,19 @@ __global__ __launch_bounds__(256) void source_angular_mfma(const float* __restri
                                                            const int* __restrict__ kt_list,
                                                            float* __restrict__ A1,
                                                            float* __restrict__ A2, int nyz,
-                                                           int nang, int nap, int tiles) {
+                                                           int nang, int nap, int tiles,
+                                                           const int* __restrict__ xtop) {
     __shared__ __attribute__((aligned(16))) __bf16 xs[4][64][kKPitch];  // X1 hi/lo, X2 hi/lo
     __shared__ __attribute__((aligned(16))) __bf16 ks[2][64][kKPitch];  // Kv^T hi/lo
-    const int L = xcd_tile(blockIdx.x, gridDim.x);
-    if (L >= tiles) return;
     const int mtiles = (nang + 63) / 64, stiles = (nyz + 63) / 64;
+    // Source groups above the primary's top are zero: their tiles (the last
+    // ones) are skipped, and their A rows are never read.
+    const int per_g = mtiles * stiles;
+    const int active = xtop ? min(tiles, (min(*xtop, tiles / per_g - 1) + 1) * per_g) : tiles;
+    const int n8 = (active + 7) & ~7;
+    if (static_cast<int>(blockIdx.x) >= n8) return;
+    const int L = xcd_tile(blockIdx.x, n8);
+    if (L >= active) return;
     const int g = L / (mtiles * stiles);
     const int rest = L - g * mtiles * stiles;
     const int s0 = (rest / mtiles) * 64, m0 = (rest % mtiles) * 64;
@@ -187,12 +197,24 @@ __global__ __launch_bounds__(256) void source_band_mfma(
     const float* __restrict__ A1, const float* __restrict__ A2, const __bf16* __restrict__ e1h,
     const __bf16* __restrict__ e1l, const __bf16* __restrict__ e2h, const __bf16* __restrict__ e2l,
     const int2* __restrict__ tile_band, float* __restrict__ Q, int Ng, int ngp, long long lanes,
-    int tiles) {
+    int tiles, const int* __restrict__ xtop, const int* __restrict__ qmap) {
     __shared__ __attribute__((aligned(16))) float bs[2][32][kBandLanes];        // A1, A2
     __shared__ __attribute__((aligned(16))) __bf16 es[4][kBandRows][kKPitch];   // E1 hi/lo, E2 hi/lo
-    const int L = xcd_tile(blockIdx.x, gridDim.x);
-    if (L >= tiles) return;
-    const int rblocks = (Ng + kBandRows - 1) / kBandRows;
+    int rblocks = (Ng + kBandRows - 1) / kBandRows;
+    // A rows above the primary's top (kmax) hold no data of this step and
+    // read as zero; output group blocks above the source's top are skipped
+    // (their values are never read).
+    int kmax = Ng - 1, active = tiles;
+    if (xtop) {
+        kmax = max(-1, min(Ng - 1, *xtop));
+        const int lblocks = tiles / rblocks;
+        rblocks = (qmap[kmax + 1] + kBandRows) / kBandRows;  // blocks up to the source's top
+        active = rblocks * lblocks;
+    }
+    const int n8 = (active + 7) & ~7;
+    if (static_cast<int>(blockIdx.x) >= n8) return;
+    const int L = xcd_tile(blockIdx.x, n8);
+    if (L >= active) return;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = lane & 15, q = lane >> 4;
     const int g_base = (L % rblocks) * kBandRows;
@@ -208,6 +230,7 @@ __global__ __launch_bounds__(256) void source_band_mfma(
             ke = max(ke, band[mt].y);
         }
     }
+    ke = min(ke, kmax + 1);  // zero A rows contribute exact zeros
     f32x4 acc[4][2];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) acc[mt][0] = acc[mt][1] = zero4();
@@ -227,7 +250,7 @@ __global__ __launch_bounds__(256) void source_band_mfma(
             const long long idx = static_cast<long long>(gs) * lanes + ln;
             if (VEC) {
                 r1[j] = r2[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (gs < Ng && ln < lanes) {
+                if (gs <= kmax && ln < lanes) {
                     r1[j] = *reinterpret_cast<const float4*>(A1 + idx);
                     r2[j] = *reinterpret_cast<const float4*>(A2 + idx);
                 }
@@ -235,7 +258,7 @@ __global__ __launch_bounds__(256) void source_band_mfma(
                 float a[4], b[4];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const bool in = gs < Ng && ln + e < lanes;
+                    const bool in = gs <= kmax && ln + e < lanes;
                     a[e] = in ? A1[idx + e] : 0.f;
                     b[e] = in ? A2[idx + e] : 0.f;
                 }
@@ -348,11 +371,15 @@ __global__ __launch_bounds__(256) void source_angular_kernel(const T* __restrict
                                                              const T* __restrict__ fp,
                                                              const T* __restrict__ kv,
                                                              T* __restrict__ A1, T* __restrict__ A2,
-                                                             int nyz, int nang) {
+                                                             int nyz, int nang,
+                                                             const int* __restrict__ xtop,
+                                                             const int* __restrict__ kcount,
+                                                             const int* __restrict__ klist) {
     __shared__ T x1s[kDepth][kTile + 1];
     __shared__ T x2s[kDepth][kTile + 1];
     __shared__ T ks[kDepth][kTile + 1];
     const int g = blockIdx.z;
+    if (xtop && g > *xtop) return;  // a zero source group: its A rows are never read
     const int m0 = blockIdx.x * kTile;
     const int s0 = blockIdx.y * kTile;
     const long long plane = static_cast<long long>(g) * nyz * nang;
@@ -364,7 +391,18 @@ __global__ __launch_bounds__(256) void source_angular_kernel(const T* __restrict
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc1[i][j] = acc2[i][j] = T(0);
-    for (int k0 = 0; k0 < nang; k0 += kDepth) {
+    // With block lists, only the kDepth-deep K blocks where this direction
+    // tile's Kv block has a nonzero entry (the skipped products are zeros).
+    const int ktiles = (nang + kDepth - 1) / kDepth;
+    int nk = ktiles;
+    const int* list = nullptr;
+    if (kcount) {
+        const long long t = static_cast<long long>(g) * gridDim.x + blockIdx.x;
+        nk = kcount[t];
+        list = klist + t * ktiles;
+    }
+    for (int kj = 0; kj < nk; ++kj) {
+        const int k0 = (list ? list[kj] : kj) * kDepth;
         for (int e = threadIdx.x; e < kTile * kDepth; e += 256) {
             const int kk = e % kDepth, r = e / kDepth;
             const int s = s0 + r, k = k0 + kk;
@@ -419,32 +457,103 @@ __global__ __launch_bounds__(256) void source_angular_kernel(const T* __restrict
     }
 }
 
+// Banded energy sum, FP64 vector path.  A work-group takes 256 lanes (one
+// per thread) and 16 consecutive output groups, walks the union of their
+// bands once and keeps the 32 sums (E1.A1, E2.A2 per group) in registers, so
+// each A row is read once per 16 output groups rather than once per group.
+// The E rows of the 16 groups are staged through LDS, 128 source groups at a
+// time, and read as broadcasts.  Every sum runs over ascending source groups
+// (a zero weight adds an exact zero), as in the reference's loop.
+constexpr int kEnergyRows = 16;
+constexpr int kEnergyChunk = 128;
+
 template <typename T>
 __global__ __launch_bounds__(256) void source_energy_kernel(
     const T* __restrict__ A1, const T* __restrict__ A2, const double* __restrict__ e1,
     const double* __restrict__ e2, const int* __restrict__ beg, const int* __restrict__ end,
-    T* __restrict__ Q, int Ng, long long lanes, double dg) {
-    const long long lane = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
-    const int g = blockIdx.y;
-    if (lane >= lanes) return;
-    const int b = max(0, beg[g]);
-    const int e = min(Ng, end[g]);
-    double q1 = 0.0, q2 = 0.0;
-    const double* r1 = e1 + static_cast<long long>(g) * Ng;
-    const double* r2 = e2 + static_cast<long long>(g) * Ng;
-    for (int src = b; src < e; ++src) {
-        const double w1 = r1[src], w2 = r2[src];
-        if (w1 == 0.0 && w2 == 0.0) continue;
-        const long long idx = static_cast<long long>(src) * lanes + lane;
-        q1 += w1 * static_cast<double>(A1[idx]);
-        q2 += w2 * static_cast<double>(A2[idx]);
+    T* __restrict__ Q, int Ng, long long lanes, double dg, const int* __restrict__ xtop,
+    const int* __restrict__ qmap) {
+    __shared__ double w1s[kEnergyRows][kEnergyChunk];
+    __shared__ double w2s[kEnergyRows][kEnergyChunk];
+    const int g0 = blockIdx.y * kEnergyRows;
+    int kmax = Ng - 1, gmax = Ng - 1;
+    if (xtop) {  // groups above the source's top are never read; A rows above kmax are zero
+        kmax = max(-1, min(Ng - 1, *xtop));
+        gmax = qmap[kmax + 1];
     }
-    Q[static_cast<long long>(g) * lanes + lane] = static_cast<T>(dg * (q1 + q2 / 3.0));
+    if (g0 > gmax) return;
+    const int rows = min(kEnergyRows, gmax - g0 + 1);
+    int kb = Ng, ke = 0;  // union of the rows' bands, below kmax + 1
+    for (int j = 0; j < rows; ++j) {
+        const int b = max(0, beg[g0 + j]), e = min(kmax + 1, end[g0 + j]);
+        if (b < e) {
+            kb = min(kb, b);
+            ke = max(ke, e);
+        }
+    }
+    const long long lane = static_cast<long long>(blockIdx.x) * 256 + threadIdx.x;
+    const bool live = lane < lanes;
+    double q1[kEnergyRows], q2[kEnergyRows];
+#pragma unroll
+    for (int j = 0; j < kEnergyRows; ++j) q1[j] = q2[j] = 0.0;
+    for (int c0 = kb; c0 < ke; c0 += kEnergyChunk) {
+        const int cn = min(kEnergyChunk, ke - c0);
+        for (int e = threadIdx.x; e < kEnergyRows * kEnergyChunk; e += 256) {
+            const int j = e / kEnergyChunk, c = e % kEnergyChunk;
+            const bool in = j < rows && c < cn;
+            const long long off = static_cast<long long>(g0 + j) * Ng + c0 + c;
+            w1s[j][c] = in ? e1[off] : 0.0;
+            w2s[j][c] = in ? e2[off] : 0.0;
+        }
+        __syncthreads();
+        if (live) {
+            for (int c = 0; c < cn; ++c) {
+                const long long idx = static_cast<long long>(c0 + c) * lanes + lane;
+                const double a1 = static_cast<double>(A1[idx]);
+                const double a2 = static_cast<double>(A2[idx]);
+#pragma unroll
+                for (int j = 0; j < kEnergyRows; ++j) {
+                    q1[j] += w1s[j][c] * a1;
+                    q2[j] += w2s[j][c] * a2;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    if (!live) return;
+#pragma unroll
+    for (int j = 0; j < kEnergyRows; ++j)
+        if (j < rows)
+            Q[static_cast<long long>(g0 + j) * lanes + lane] =
+                static_cast<T>(dg * (q1[j] + q2[j] / 3.0));
 }
 
 }  // namespace
 
 int source_nap(int nang) { return (nang + 31) / 32 * 32; }
+
+void source_angular_blocks(const std::vector<double>& kv, int Ng, int nang, std::vector<int>& count,
+                           std::vector<int>& list) {
+    const int mtiles = (nang + kTile - 1) / kTile, ktiles = (nang + kDepth - 1) / kDepth;
+    count.assign(static_cast<size_t>(Ng) * mtiles, 0);
+    list.assign(static_cast<size_t>(Ng) * mtiles * ktiles, 0);
+    for (int g = 0; g < Ng; ++g)
+        for (int mt = 0; mt < mtiles; ++mt) {
+            const size_t t = static_cast<size_t>(g) * mtiles + mt;
+            for (int kt = 0; kt < ktiles; ++kt) {
+                bool nz = false;
+                for (int k = kt * kDepth; k < std::min(nang, (kt + 1) * kDepth) && !nz; ++k) {
+                    const double* row = kv.data() + (static_cast<size_t>(g) * nang + k) * nang;
+                    for (int m = mt * kTile; m < std::min(nang, (mt + 1) * kTile); ++m)
+                        if (row[m] != 0.0) {
+                            nz = true;
+                            break;
+                        }
+                }
+                if (nz) list[t * ktiles + count[t]++] = kt;
+            }
+        }
+}
 int source_mtiles(int nang) { return (nang + 63) / 64; }
 int source_ktiles(int nang) { return source_nap(nang) / 32; }
 
@@ -474,7 +583,8 @@ void launch_split(const float* src, __bf16* hi, __bf16* lo, long long n, hipStre
 
 void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, float* Q,
                         const SourcePlan& plan, int Ng, int nyz, int nang, hipStream_t s,
-                        hipEvent_t inputs_read) {
+                        hipEvent_t inputs_read, const int* xtop, const int* qmap) {
+    BFPN_REQUIRE((xtop == nullptr) == (qmap == nullptr), "source tops need both xtop and qmap");
     const int nap = source_nap(nang);
     const long long t1 = static_cast<long long>(Ng) * ((nyz + 63) / 64) * ((nang + 63) / 64);
     BFPN_REQUIRE(t1 < (1LL << 31) - 8, "source GEMM grid too large");
@@ -482,11 +592,11 @@ void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, 
     if (nang % 8 == 0)
         hipLaunchKernelGGL(source_angular_mfma<true>, dim3(g1), dim3(256), 0, s, Fp, fp, plan.kth,
                            plan.ktl, plan.kt_count, plan.kt_list, A1, A2, nyz, nang, nap,
-                           static_cast<int>(t1));
+                           static_cast<int>(t1), xtop);
     else
         hipLaunchKernelGGL(source_angular_mfma<false>, dim3(g1), dim3(256), 0, s, Fp, fp, plan.kth,
                            plan.ktl, plan.kt_count, plan.kt_list, A1, A2, nyz, nang, nap,
-                           static_cast<int>(t1));
+                           static_cast<int>(t1), xtop);
     BFPN_HIP(hipGetLastError());
     if (inputs_read) BFPN_HIP(hipEventRecord(inputs_read, s));
     const long long lanes = static_cast<long long>(nyz) * nang;
@@ -497,44 +607,48 @@ void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, 
     if (lanes % 4 == 0)
         hipLaunchKernelGGL(source_band_mfma<true>, dim3(g2), dim3(256), 0, s, A1, A2, plan.e1h,
                            plan.e1l, plan.e2h, plan.e2l, plan.tile_band, Q, Ng, source_ngp(Ng),
-                           lanes, static_cast<int>(t2));
+                           lanes, static_cast<int>(t2), xtop, qmap);
     else
         hipLaunchKernelGGL(source_band_mfma<false>, dim3(g2), dim3(256), 0, s, A1, A2, plan.e1h,
                            plan.e1l, plan.e2h, plan.e2l, plan.tile_band, Q, Ng, source_ngp(Ng),
-                           lanes, static_cast<int>(t2));
+                           lanes, static_cast<int>(t2), xtop, qmap);
     BFPN_HIP(hipGetLastError());
 }
 
 template <typename T>
 void launch_source_angular(const T* Fp, const T* fp, const T* kv, T* A1, T* A2, int Ng, int nyz,
-                           int nang, hipStream_t s) {
+                           int nang, hipStream_t s, const int* xtop, const int* kcount,
+                           const int* klist) {
     if (Ng <= 0 || nyz <= 0 || nang <= 0) return;
     dim3 grid((nang + kTile - 1) / kTile, (nyz + kTile - 1) / kTile, Ng);
     hipLaunchKernelGGL(source_angular_kernel<T>, grid, dim3(256), 0, s, Fp, fp, kv, A1, A2, nyz,
-                       nang);
+                       nang, xtop, kcount, klist);
     BFPN_HIP(hipGetLastError());
 }
 
 template <typename T>
 void launch_source_energy(const T* A1, const T* A2, const double* e1, const double* e2,
                           const int* beg, const int* end, T* Q, int Ng, long long lanes, double dg,
-                          hipStream_t s) {
+                          hipStream_t s, const int* xtop, const int* qmap) {
     if (Ng <= 0 || lanes <= 0) return;
-    dim3 grid(static_cast<unsigned>((lanes + 255) / 256), Ng);
+    BFPN_REQUIRE((xtop == nullptr) == (qmap == nullptr), "source tops need both xtop and qmap");
+    dim3 grid(static_cast<unsigned>((lanes + 255) / 256), (Ng + kEnergyRows - 1) / kEnergyRows);
     hipLaunchKernelGGL(source_energy_kernel<T>, grid, dim3(256), 0, s, A1, A2, e1, e2, beg, end, Q,
-                       Ng, lanes, dg);
+                       Ng, lanes, dg, xtop, qmap);
     BFPN_HIP(hipGetLastError());
 }
 
 template void launch_source_angular<float>(const float*, const float*, const float*, float*, float*,
-                                           int, int, int, hipStream_t);
+                                           int, int, int, hipStream_t, const int*, const int*,
+                                           const int*);
 template void launch_source_angular<double>(const double*, const double*, const double*, double*,
-                                            double*, int, int, int, hipStream_t);
+                                            double*, int, int, int, hipStream_t, const int*,
+                                            const int*, const int*);
 template void launch_source_energy<float>(const float*, const float*, const double*, const double*,
                                           const int*, const int*, float*, int, long long, double,
-                                          hipStream_t);
+                                          hipStream_t, const int*, const int*);
 template void launch_source_energy<double>(const double*, const double*, const double*,
                                            const double*, const int*, const int*, double*, int,
-                                           long long, double, hipStream_t);
+                                           long long, double, hipStream_t, const int*, const int*);
 
 }  // namespace bfpn

@@ -434,6 +434,24 @@ std::vector<EnergyCoef> energy_coefs(const std::vector<double>& S, const std::ve
 
 // Eq.24 integrand per group (src/bfpm_solver.cu:279-287): S unclamped,
 // sigma clamped at 0.
+std::vector<UpwindCoef> upwind_coefs(const std::vector<double>& S, const std::vector<double>& sigma,
+                                     const double* T, bool straggling, double h, double dg,
+                                     double rho, int Ng) {
+    constexpr double kEps = 1e-10;  // Numerics::EPSILON, include/config.hpp:39
+    const double dg2 = std::max(dg * dg, kEps);
+    std::vector<UpwindCoef> c(Ng);
+    for (int g = 0; g < Ng; ++g) {
+        const double sig = std::max(sigma[g], 0.0);
+        const double S_lo = std::max(S[g], 0.0), S_hi = std::max(S[g + 1], 0.0);
+        const double T_g = (straggling && T) ? std::max(T[g], 0.0) : 0.0;
+        c[g].h = h;
+        c[g].c_hi = rho * S_hi / dg;
+        c[g].denom = std::max(1.0 / h + rho * S_lo / dg + sig, kEps);
+        c[g].strag = T_g > 0.0 ? 0.5 * h * T_g / dg2 : 0.0;
+    }
+    return c;
+}
+
 std::vector<DoseCoef> dose_coefs(const std::vector<double>& S, const std::vector<double>& sigma,
                                  const std::vector<double>& en, double dg, bool catastrophic, int Ng) {
     std::vector<DoseCoef> w(Ng);

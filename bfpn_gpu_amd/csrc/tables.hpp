@@ -45,6 +45,12 @@ std::vector<EnergyCoef> energy_coefs(const std::vector<double>& S, const std::ve
                                      const double* T, bool straggling, double h, double dg,
                                      double rho, int Ng);
 
+// Upwind energy step coefficients (src/energy_solver.cu:88-106 with the
+// straggling enabled by `straggling`; src/bfpm_solver.cu:600-622 with rho = 1).
+std::vector<UpwindCoef> upwind_coefs(const std::vector<double>& S, const std::vector<double>& sigma,
+                                     const double* T, bool straggling, double h, double dg,
+                                     double rho, int Ng);
+
 std::vector<DoseCoef> dose_coefs(const std::vector<double>& S, const std::vector<double>& sigma,
                                  const std::vector<double>& en, double dg, bool catastrophic, int Ng);
 

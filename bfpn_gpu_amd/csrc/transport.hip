@@ -117,15 +117,21 @@ __device__ inline T divergence(T zc, T near, T S_c, T S_n, T dz_b, T dz_f, T dS_
 template <typename T, int MW, int P, bool CLIP>
 __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
     T* __restrict__ F, const T* __restrict__ omend, int Ny, int Nz, int nang, int R, T inv_dy,
-    T inv_dz, T h, int tiles) {
+    T inv_dz, T h, int tiles, const int* __restrict__ top) {
     extern __shared__ unsigned char lds_raw[];
-    // Blocks are dispatched round-robin over the 8 XCDs.  Direction chunks of
-    // one group share 128-byte lines (a plane row is nang floats), so the
-    // chunks of a group run consecutively on one XCD and share its L2:
-    // block b -> logical tile (b mod 8) * (grid / 8) + b / 8.
-    const int L = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
-    if (L >= tiles) return;
     const int mchunks = (nang + MW - 1) / MW;
+    // Only the groups up to the state's top hold nonzero values (the others
+    // stay zero); their tiles are the first `active` and the grid's later
+    // blocks exit at once.  Blocks are dispatched round-robin over the 8
+    // XCDs.  Direction chunks of one group share 128-byte lines (a plane row
+    // is nang floats), so the chunks of a group run consecutively on one XCD
+    // and share its L2: block b -> logical tile (b mod 8) * (n8 / 8) + b / 8,
+    // n8 = active rounded up to a multiple of 8.
+    const int active = top ? min(tiles, (min(*top, tiles / mchunks - 1) + 1) * mchunks) : tiles;
+    const int n8 = (active + 7) & ~7;
+    if (static_cast<int>(blockIdx.x) >= n8) return;
+    const int L = (blockIdx.x & 7) * (n8 >> 3) + (blockIdx.x >> 3);
+    if (L >= active) return;
     const int g = L / mchunks;
     T* ring = reinterpret_cast<T*>(lds_raw);
     constexpr int YP = kSlots * MW;
@@ -363,7 +369,8 @@ MarchShape march_shape(int Ny1, size_t elem) {
 
 template <typename T, int MW, int P>
 void launch_march(T* F, const T* omend, int Ny, int Nz, int Ng, int nang, const MarchShape& sh,
-                  double dy, double dz, double h, bool preserve_sign, hipStream_t s) {
+                  double dy, double dz, double h, bool preserve_sign, hipStream_t s,
+                  const int* top) {
     const size_t lds = sizeof(T) * kSlots * static_cast<size_t>(Ny + 1) * MW;
     BFPN_REQUIRE(lds <= 160 * 1024, "transport plane too wide for the LDS rings");
     const long long tiles = static_cast<long long>((nang + MW - 1) / MW) * Ng;
@@ -375,7 +382,7 @@ void launch_march(T* F, const T* omend, int Ny, int Nz, int Ng, int nang, const 
                                      static_cast<int>(lds)));
         hipLaunchKernelGGL(kernel, grid, dim3(sh.threads), lds, s, F, omend, Ny, Nz, nang, sh.R,
                            static_cast<T>(1.0 / dy), static_cast<T>(1.0 / dz), static_cast<T>(h),
-                           static_cast<int>(tiles));
+                           static_cast<int>(tiles), top);
     };
     if (preserve_sign)
         go(&transport_march_kernel<T, MW, P, false>);
@@ -388,7 +395,8 @@ void launch_march(T* F, const T* omend, int Ny, int Nz, int Ng, int nang, const 
 
 template <typename T>
 void launch_transport_half(T* F, const T* omend, int Ny, int Nz, int Ng, int nang, double dy,
-                           double dz, double h, bool preserve_sign, hipStream_t s) {
+                           double dz, double h, bool preserve_sign, hipStream_t s,
+                           const int* top) {
     if (Ng <= 0 || nang <= 0) return;
     BFPN_REQUIRE(Ny >= 1 && Nz >= 1, "transport needs Ny, Nz >= 1");
     const MarchShape sh = march_shape(Ny + 1, sizeof(T));
@@ -401,7 +409,7 @@ void launch_transport_half(T* F, const T* omend, int Ny, int Nz, int Ng, int nan
 #define BFPN_MARCH(MW_, P_)                                                                    \
     if (sh.MW == MW_ && sh.P == P_)                                                            \
         return launch_march<T, MW_, P_>(F, omend, Ny, Nz, Ng, nang, sh, dy, dz, h, preserve_sign, \
-                                        s)
+                                        s, top)
     BFPN_MARCH(16, 1);
     BFPN_MARCH(16, 2);
     BFPN_MARCH(8, 2);
@@ -411,8 +419,8 @@ void launch_transport_half(T* F, const T* omend, int Ny, int Nz, int Ng, int nan
 }
 
 template void launch_transport_half<float>(float*, const float*, int, int, int, int, double, double,
-                                           double, bool, hipStream_t);
+                                           double, bool, hipStream_t, const int*);
 template void launch_transport_half<double>(double*, const double*, int, int, int, int, double,
-                                            double, double, bool, hipStream_t);
+                                            double, double, bool, hipStream_t, const int*);
 
 }  // namespace bfpn

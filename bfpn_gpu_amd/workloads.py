@@ -33,7 +33,7 @@ def csda_range(energy: float, material: str, rho: float, Ng: int = 2000, Lg: flo
     E = np.concatenate([[1.0], (np.arange(1, Ng) - 0.5) * dg + 1.0, [Lg + 1.0]])
     m = E <= energy
     Em, Sm = E[m], S[m]
-    return float(np.trapz(1.0 / (rho * Sm), Em) + (energy - Em[-1]) / (rho * Sm[-1]))
+    return float(np.trapezoid(1.0 / (rho * Sm), Em) + (energy - Em[-1]) / (rho * Sm[-1]))
 
 
 def beam_case(energy, sigma_yz, ny, material, depth, Ng=300, Nmu=20, Nom=20):

@@ -75,6 +75,11 @@ typedef struct bfpn_phys {
     int dose_catastrophic_loss;
     int calibrate_cross_sections;
     int save_energy_moments;
+    int legacy_energy;               /* --energy-model legacy: first-order
+                                        upwind energy step (src/energy_solver.cu:88-106) */
+    int lite_memory;                 /* --lite-memory: primary psi^1 only, one
+                                        upwind energy step per depth step
+                                        (src/bfpm_solver.cu:2191-2212, 3396-3416) */
 } bfpn_phys;
 
 /* Derived deltas (GridParams::computeDeltas, src/bfpm_solver.cu:1116-1126). */
@@ -320,6 +325,22 @@ BFPN_API void* bfpn_stream(bfpn_ctx* ctx);
  * without contention).  Results are identical either way.  No counterpart in
  * the reference, whose operators synchronise the host after every call. */
 BFPN_API bfpn_status bfpn_set_overlap(bfpn_ctx* ctx, int on);
+
+/* Active-group skipping (on by default).  Above the highest nonzero energy
+ * group of a species every group of its state is exactly zero (energy only
+ * decreases, straggling lifts it by at most one group per half step, the
+ * secondary source emits at or below its source group, transport and
+ * angular diffusion keep groups apart).  Each energy sweep measures that top
+ * group on the device, and the operators of the next sweeps skip the groups
+ * above it, whose results would be exact zeros: results are bit-identical
+ * with skipping off.  The reference's streaming mode skips zero chunks the
+ * same way from a conservative bound (src/bfpm_solver.cu:2325-2342,
+ * --no-zero-chunk-skip); here the bound is exact and also falls as the beam
+ * slows.  Call between bfpn_advance calls. */
+BFPN_API bfpn_status bfpn_set_group_skip(bfpn_ctx* ctx, int on);
+/* The current top groups of the primary and secondary state (-1: all zero;
+ * Ng-1 when skipping is off).  Synchronizes. */
+BFPN_API bfpn_status bfpn_active_groups(bfpn_ctx* ctx, int* primary_top, int* secondary_top);
 
 /* Per-phase device timing of the bfpn_advance calls made while profiling is
  * on (replaces StepProfile, src/bfpm_solver.cuh:64-75).  Each operator launch

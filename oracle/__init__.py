@@ -55,6 +55,9 @@ def lib():
         L.orc_energy_half.argtypes = [_dp, C.c_void_p, _dp, _dp, C.c_void_p, C.c_double,
                                       C.c_double, C.c_double, C.c_int, C.c_longlong, C.c_int,
                                       C.c_void_p, C.c_void_p, C.c_int]
+        L.orc_energy_half_legacy.argtypes = [_dp, C.c_void_p, _dp, _dp, C.c_void_p, C.c_double,
+                                             C.c_double, C.c_double, C.c_int, C.c_longlong,
+                                             C.c_int, C.c_void_p, C.c_int]
         L.orc_secondary_source.argtypes = [_dp, C.c_void_p, _dp, _dp, _dp, _dp, _ip, _ip, C.c_int,
                                            C.c_longlong, C.c_int, C.c_double]
         L.orc_secondary_source_factorized.argtypes = L.orc_secondary_source.argtypes
@@ -165,6 +168,13 @@ def energy_half(F, f, S, sigma, T, h, dg, density, Ng, nyz, nang, Q1=None, Q2=No
     """In place on float64 arrays F, f (f may be None)."""
     lib().orc_energy_half(F, _ptr(f), S, sigma, _ptr(T), h, dg, density, Ng, nyz, nang,
                           _ptr(Q1), _ptr(Q2), int(straggling))
+
+
+def energy_half_legacy(F, f, S, sigma, T, h, dg, density, Ng, nyz, nang, Q1=None,
+                       straggling=False):
+    """Legacy (first-order upwind) energy step, in place; f may be None (lite)."""
+    lib().orc_energy_half_legacy(F, _ptr(f), S, sigma, _ptr(T), h, dg, density, Ng, nyz, nang,
+                                 _ptr(Q1), int(straggling))
 
 
 def secondary_source(Fp, fp, ke1, ke2, kv, beg, end, Ng, nyz, nang, dg):

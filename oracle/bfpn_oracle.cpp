@@ -435,6 +435,55 @@ void energy_half(double* F, double* f, const double* S_s, const double* sigma_c,
     if (f) std::memcpy(f, fn.data(), total * sizeof(double));
 }
 
+// energyDgCnKernel, legacy branch (src/energy_solver.cu:88-106): first-order
+// implicit upwind sweep from high to low energy on (F, f), psi^1 clipped at
+// zero on input and both clipped on output, explicit straggling on the
+// clipped old psi^1 (with T_g only, and without the density), the source on
+// psi^1 only.  The lite-memory step (litePrimaryEnergyStepKernel,
+// src/bfpm_solver.cu:600-622) is this sweep on psi^1 alone with density 1,
+// no source and no straggling.
+void energy_half_legacy(double* F, double* f, const double* S_s, const double* sigma_c,
+                        const double* T_c, double dt, double dg, double density, int Ng,
+                        long long nyz, int nang, const double* Q1, bool straggling) {
+    const long long n_per_E = nyz * nang;
+    const size_t total = static_cast<size_t>(Ng) * n_per_E;
+    std::vector<double> Fn(total), fn(f ? total : 0);
+    const double dg2 = std::max(dg * dg, kEps);
+#pragma omp parallel for schedule(static)
+    for (long long lane = 0; lane < n_per_E; ++lane) {
+        auto old = [&](int ek) -> double {
+            if (ek < 0 || ek >= Ng) return 0.0;
+            return F[static_cast<long long>(ek) * n_per_E + lane];
+        };
+        double next_F = 0.0, next_f = 0.0;
+        for (int ek = Ng - 1; ek >= 0; --ek) {
+            const long long idx = static_cast<long long>(ek) * n_per_E + lane;
+            const double Fg = F[idx];
+            const double fg = f ? f[idx] : 0.0;
+            const double sig = std::max(sigma_c[ek], 0.0);
+            const double S_lo = std::max(S_s[ek], 0.0);
+            const double S_hi = std::max(S_s[ek + 1], 0.0);
+            const double source = Q1 ? Q1[idx] : 0.0;
+            const double T_g = (straggling && T_c) ? std::max(T_c[ek], 0.0) : 0.0;
+            const double denom = 1.0 / dt + density * S_lo / dg + sig;
+            double F_val = (std::max(Fg, 0.0) / dt + density * S_hi / dg * next_F + source) /
+                           std::max(denom, kEps);
+            const double f_val = (fg / dt + density * S_hi / dg * next_f) / std::max(denom, kEps);
+            if (T_g > 0.0) {
+                const double lap =
+                    std::max(old(ek + 1), 0.0) - 2.0 * std::max(Fg, 0.0) + std::max(old(ek - 1), 0.0);
+                F_val += 0.5 * dt * T_g / dg2 * lap;
+            }
+            Fn[idx] = std::max(F_val, 0.0);
+            if (f) fn[idx] = std::max(f_val, 0.0);
+            next_F = Fn[idx];
+            next_f = f ? fn[idx] : 0.0;
+        }
+    }
+    std::memcpy(F, Fn.data(), total * sizeof(double));
+    if (f) std::memcpy(f, fn.data(), total * sizeof(double));
+}
+
 // secondarySourceFromPrimaryKernel, src/energy_solver.cu:287-333.  The dense
 // ker_v is walked through its nonzeros per output column in increasing input
 // order, which is the reference's order with its zero skips.
@@ -837,9 +886,24 @@ void solver_step(Solver& s) {
     const auto& d = s.d;
     const double half = 0.5 * d.dt;
     const bool strag = p.eq15_straggling != 0;
+    if (p.lite_memory) {  // stepLitePrimary + computeIntegratedDepthDoseLite
+        energy_half_legacy(s.F.data(), nullptr, s.S.data(), s.sigma.data(), nullptr, d.dt, d.dg,
+                           1.0, g.Ng, s.nyz, s.nang, nullptr, false);
+        s.idd.push_back(idd_value(s.F.data(), nullptr, nullptr, nullptr, s.S.data(),
+                                  s.sigma.data(), s.en.data(), g.Ng, g.Ny, g.Nz, s.nang, d.dg,
+                                  d.du, d.dv, d.dy, d.dz, p.density, p.trapezoidal_yz != 0,
+                                  p.dose_catastrophic_loss != 0));
+        s.proxy.push_back(0.0);
+        s.step++;
+        return;
+    }
     auto E = [&](std::vector<double>& a, std::vector<double>& b, const double* Q) {
-        energy_half(a.data(), b.data(), s.S.data(), s.sigma.data(), s.T.data(), half, d.dg,
-                    p.energy_density, g.Ng, s.nyz, s.nang, Q, nullptr, strag);
+        if (p.legacy_energy)
+            energy_half_legacy(a.data(), b.data(), s.S.data(), s.sigma.data(), s.T.data(), half,
+                               d.dg, p.energy_density, g.Ng, s.nyz, s.nang, Q, strag);
+        else
+            energy_half(a.data(), b.data(), s.S.data(), s.sigma.data(), s.T.data(), half, d.dg,
+                        p.energy_density, g.Ng, s.nyz, s.nang, Q, nullptr, strag);
     };
     auto Tr = [&](std::vector<double>& a, bool preserve) {
         transport_half(a.data(), s.om.data(), g.Ny, g.Nz, g.Ng, s.nang, d.dy, d.dz, half, preserve);
@@ -928,6 +992,12 @@ BFPN_API int orc_init_beam(const bfpn_grid* g, const bfpn_phys* p, double* F, do
     std::vector<double> om(2 * static_cast<size_t>(g->Nmu) * g->Nom);
     omend_table(*g, d, om.data());
     return init_beam(*g, *p, d, om.data(), F, masses) ? 0 : 1;
+}
+
+BFPN_API void orc_energy_half_legacy(double* F, double* f, const double* S, const double* sigma,
+                                     const double* T, double h, double dg, double density, int Ng,
+                                     long long nyz, int nang, const double* Q1, int straggling) {
+    energy_half_legacy(F, f, S, sigma, T, h, dg, density, Ng, nyz, nang, Q1, straggling != 0);
 }
 
 BFPN_API void orc_energy_half(double* F, double* f, const double* S, const double* sigma,

@@ -6,9 +6,12 @@
   fit fails with a message and does not disturb the others.
 * Requests submitted while others are running are admitted and completed.
 """
+import os
+
 import numpy as np
 import pytest
 
+from conftest import TABLES
 from gpu_cases import small_case
 
 pytestmark = pytest.mark.gpu
@@ -93,3 +96,26 @@ def test_requests_join_while_others_run(gpu):
     np.testing.assert_array_equal(rb, ref_short)
     assert len(ra) == long_case.num_steps()
     assert st["peak_active"] == 2
+
+
+def test_alternating_table_keys_under_a_tight_budget(gpu):
+    """Water and bone requests (two table keys) alternate under a budget
+    that holds one set of tables and one beam: admitting one key evicts the
+    other's unused tables but never its own, every byte stays charged, and
+    the results are those of plain solves."""
+    w = small_case(Ny=3, Nz=3, Nmu=4, Nom=4, time=0.08)
+    b = small_case(Ny=3, Nz=3, Nmu=4, Nom=4, time=0.08, material="bone", density=1.85,
+                   data=os.path.join(TABLES, "bone"))
+    cases = [w, b, w, b, w, b]
+    ref = {id(c): _alone(gpu, c) for c in (w, b)}
+    (beam_w, shared_w), (beam_b, shared_b) = gpu.Server.estimate(w), gpu.Server.estimate(b)
+    budget = max(shared_w, shared_b) + max(beam_w, beam_b) + min(shared_w, shared_b) // 2
+    with gpu.Server(max_active=1, slice_steps=1, hbm_budget_bytes=budget) as srv:
+        ids = [srv.submit(c) for c in cases]
+        out = [srv.wait(i, timeout=120)[0] for i in ids]
+        st = srv.stats()
+    for c, o in zip(cases, out):
+        np.testing.assert_array_equal(o, ref[id(c)])
+    assert st["completed"] == len(cases) and st["failed"] == 0
+    assert st["cached_tables"] == 1
+    assert 0 < st["bytes_in_use"] <= budget

@@ -69,6 +69,33 @@ def test_fp64_materials(gpu, oracle_mod, material, rho):
         assert _rel(g.state(k), o.state(k)) < 1e-10 or np.abs(o.state(k)).max() == 0
 
 
+@pytest.mark.parametrize("precision,tol", [("FP64", 1e-10), ("FP32", 2e-4)])
+@pytest.mark.parametrize("flags", [
+    {"legacy_energy": 1, "eq15_straggling": 1},
+    {"legacy_energy": 1},
+    {"legacy_energy": 1, "primary_only": 1, "eq15_straggling": 1},
+    {"lite_memory": 1},
+    {"lite_memory": 1, "eq15_straggling": 1, "trapezoidal_yz": 0},
+], ids=lambda f: "-".join(sorted(f)))
+def test_legacy_energy_model_and_lite_memory_match_oracle(gpu, oracle_mod, flags, precision, tol):
+    """--energy-model legacy (src/energy_solver.cu:88-106) and --lite-memory
+    (src/bfpm_solver.cu:2191-2212): FP64 to 1e-10, FP32 state to 2e-4 (its
+    DST and source run on the matrix cores); lite keeps psi1 only, reports
+    zero proxies and ignores straggling."""
+    case = small_case(flags=flags)
+    o, g = _run_both(gpu, oracle_mod, case, 5, getattr(gpu, precision))
+    scale = np.abs(o.state(0)).max()
+    for k in range(4):
+        assert np.abs(g.state(k) - o.state(k)).max() / scale < tol, k
+    gi, gp = g.history()
+    oi, op = o.history()
+    assert len(gi) == 5
+    np.testing.assert_allclose(gi, oi, rtol=tol)
+    np.testing.assert_allclose(gp, op, rtol=tol, atol=tol * np.abs(op).max())
+    if flags.get("lite_memory"):
+        assert np.all(gp == 0) and all(np.all(g.state(k) == 0) for k in (1, 2, 3))
+
+
 def test_tables_bit_identical_to_oracle(gpu, oracle_mod):
     case = small_case()
     o = oracle_mod.Solver(case)
@@ -243,6 +270,33 @@ def test_cli_fp64_writes_the_reference_file(tmp_path, golden, gpu):
     m = golden[:, 1] >= 1e-6 * golden[:, 1].max()
     assert np.max(np.abs(out[m, 1] - golden[m, 1]) / golden[m, 1]) < 1e-8
     assert os.path.exists(tmp_path / "dose_output.txt")
+
+
+@pytest.mark.parametrize("extra", [["--lite-memory"], ["--energy-model", "legacy",
+                                                      "--eq15-straggling"]])
+def test_cli_legacy_and_lite_runs(tmp_path, gpu, oracle_mod, extra):
+    """The CLI accepts --lite-memory and --energy-model legacy and writes the
+    reference's files with the oracle's values."""
+    cli = os.path.join(ROOT, "bfpn_gpu_amd", "bin", "bfp_solver")
+    case = small_case(Nx=200, time=1.0)
+    cmd = [cli, "200", "--data", case.data, "--material", "water", "--energy", "230",
+           "--ny", str(case.Ny), "--nz", str(case.Nz), "--nmu", str(case.Nmu), "--nom",
+           str(case.Nom), "--ly", str(case.Ly), "--lz", str(case.Lz), "--lu", "2", "--lv", "2",
+           "--sigma-yz", str(case.sigma_yz), "--discrete-angle-delta", "--normalize-initial-mass",
+           "--trapezoidal-yz", "--time", "1.0", "--precision", "fp64", *extra]
+    r = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    out = np.loadtxt(tmp_path / "idd_output.txt")
+    proxy = np.loadtxt(tmp_path / "dose_output.txt")
+    assert out.shape == (5, 2)
+    flags = {"lite_memory": 1} if extra[0] == "--lite-memory" else {"legacy_energy": 1,
+                                                                     "eq15_straggling": 1}
+    case.flags.update(flags)
+    o = oracle_mod.Solver(case)
+    o.advance(5)
+    np.testing.assert_allclose(out[:, 1], o.history()[0], rtol=1e-10)
+    if extra[0] == "--lite-memory":
+        assert np.all(proxy[:, 1] == 0)
 
 
 # ----------------------------------------------------------- diagnostics ---

@@ -9,6 +9,15 @@ energy (src/energy_solver.cu:108-160)
   * S = sigma = 0, T > 0 is one explicit Euler step of energy diffusion
     psi1 += h rho T/2 (psi1[g+1] - 2 psi1[g] + psi1[g-1]) / dg^2, psi2 fixed;
   * the full operator (S > 0) is linear: E(a x + b y) = a E(x) + b E(y).
+legacy energy model and lite-memory step (src/energy_solver.cu:88-106,
+src/bfpm_solver.cu:600-622)
+  * positive states without straggling: the result solves the implicit
+    upwind system (1/h + rho S_lo/dg + sigma) x_g - rho S_hi/dg x_{g+1}
+    = psi_g/h (+ Q_g for psi1), assembled and solved densely with numpy;
+  * S = sigma = 0, T > 0: psi1 + h T/2 lap(psi1)/dg^2 -- without the density,
+    unlike eq15;
+  * negative inputs of psi1 and negative results are clipped to zero;
+  * without psi2 (lite) psi1 is the same as with it.
 transport (src/spatial_transport.cu:23-262)
   * a constant field is unchanged; omega = 0 is the identity;
   * away from the boundary the fluxes telescope: without clipping the sum of
@@ -92,6 +101,45 @@ def test_energy_straggling_is_explicit_diffusion(oracle_mod):
     lap = pad[2:] - 2 * pad[1:-1] + pad[:-2]
     np.testing.assert_allclose(F.reshape(Ng, lanes), F0 + h * rho * Tv / 2 * lap / dg**2, rtol=1e-12)
     np.testing.assert_allclose(f, f0, rtol=1e-15)
+
+
+def test_legacy_energy_solves_the_implicit_upwind_system(oracle_mod):
+    Ng, nyz, nang = 8, 2, 3
+    lanes = nyz * nang
+    h, dg, rho = 0.01, 0.6, 1.2
+    S = 1 + RNG.random(Ng + 1)
+    sig = RNG.random(Ng) * 0.3
+    F = RNG.random(Ng * lanes) + 0.1
+    f = RNG.random(Ng * lanes) + 0.1
+    Q = RNG.random(Ng * lanes)
+    F0, f0 = F.reshape(Ng, lanes).copy(), f.reshape(Ng, lanes).copy()
+    oracle_mod.energy_half_legacy(F, f, S, sig, None, h, dg, rho, Ng, nyz, nang, Q1=Q)
+    A = np.diag(1 / h + rho * S[:-1] / dg + sig) - np.diag(rho * S[1:-1] / dg, k=1)
+    np.testing.assert_allclose(F.reshape(Ng, lanes),
+                               np.linalg.solve(A, F0 / h + Q.reshape(Ng, lanes)), rtol=1e-12)
+    np.testing.assert_allclose(f.reshape(Ng, lanes), np.linalg.solve(A, f0 / h), rtol=1e-12)
+    # lite: psi1 alone gives the same psi1
+    F2 = F0.ravel().copy()
+    oracle_mod.energy_half_legacy(F2, None, S, sig, None, h, dg, rho, Ng, nyz, nang, Q1=Q)
+    np.testing.assert_array_equal(F2, F)
+
+
+def test_legacy_energy_straggling_and_clipping(oracle_mod):
+    Ng, lanes = 9, 4
+    h, dg, rho, Tv = 0.01, 0.7, 1.3, 0.4
+    F = RNG.random(Ng * lanes) + 0.5
+    f = RNG.standard_normal(Ng * lanes)
+    F0, f0 = F.reshape(Ng, lanes).copy(), f.copy()
+    S, sig, _ = _tables(Ng)
+    oracle_mod.energy_half_legacy(F, f, S, sig, np.full(Ng, Tv), h, dg, rho, Ng, 1, lanes,
+                                  straggling=True)
+    pad = np.vstack([np.zeros(lanes), F0, np.zeros(lanes)])
+    lap = pad[2:] - 2 * pad[1:-1] + pad[:-2]
+    np.testing.assert_allclose(F.reshape(Ng, lanes), F0 + h * Tv / 2 * lap / dg**2, rtol=1e-12)
+    np.testing.assert_allclose(f, np.maximum(f0, 0.0), rtol=1e-15)
+    G = -np.abs(RNG.random(Ng * lanes))
+    oracle_mod.energy_half_legacy(G, None, S, sig, None, h, dg, rho, Ng, 1, lanes)
+    assert np.all(G == 0.0)
 
 
 def test_energy_is_linear(oracle_mod):
