@@ -54,3 +54,37 @@ def test_grid_sensitivity_runner_on_the_solver(tmp_path, gpu):
         assert 0.0 < r["norm_l2_vs_baseline"] < 0.2, r
         assert abs(r["bp"] - base["bp"]) < 0.5, r
     assert (tmp_path / "grid_sensitivity.md").exists()
+
+
+def test_grid_sensitivity_reproduces_the_reference_stage3_csv(tmp_path, gpu):
+    """The reference's grid-sensitivity study (results/stage3_grid_sensitivity:
+    water 230 MeV, baseline Nx 500 / 8x8 cells / 7x7 directions, one family
+    refined per variant) through the CLI in FP64 with the settings that
+    reproduce its entrance values (tests/test_oracle_golden.py): every column
+    of every row equals the committed CSV -- entrance, Dmax and norm0 to 1e-12,
+    BP/P90/D90/D20 to 1e-10 cm, the normalised L2/Linf differences to 1e-9."""
+    import csv
+
+    from conftest import GOLDEN
+
+    rows = harness.grid_sensitivity(
+        harness.DEFAULT_EXE, tmp_path, ("base", (500, 8, 8, 7, 7)),
+        [("depth", "nx1000", (1000, 8, 8, 7, 7)),
+         ("transverse", "yz12", (500, 12, 12, 7, 7)),
+         ("angular", "uv9", (500, 8, 8, 9, 9))],
+        time=40.0, material="water", energy=230,
+        extra=("--ly", "8", "--lz", "8", "--lu", "2", "--lv", "2", "--sigma-yz", "0.3",
+               "--normalize-initial-mass", "--legacy-cross-section-calibration",
+               "--stopping-power-dose-only", "--precision", "fp64"))
+    with open(os.path.join(GOLDEN, "grid_sensitivity.csv"), newline="") as f:
+        gold = list(csv.DictReader(f))
+    assert [r["label"] for r in rows] == [g["label"] for g in gold]
+    for r, g in zip(rows, gold):
+        for k in ("nx", "ny", "nz", "nmu", "nom"):
+            assert r[k] == int(g[k])
+        for k in ("entrance", "dmax", "norm0"):
+            assert abs(r[k] - float(g[k])) <= 1e-12 * abs(float(g[k])), (r["label"], k, r[k], g[k])
+        for k in ("bp", "p90", "d90", "d20"):
+            assert abs(r[k] - float(g[k])) <= 1e-10, (r["label"], k, r[k], g[k])
+        for k in ("norm_l2_vs_baseline", "norm_linf_vs_baseline"):
+            assert abs(r[k] - float(g[k])) <= 1e-9, (r["label"], k, r[k], g[k])

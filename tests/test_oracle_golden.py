@@ -12,7 +12,19 @@ reproduce the file to ~1e-15 (straggling ON misses by ~1e-5).
 The first steps exercise every operator of the depth march: energy, MUSCL
 transport, DST angular diffusion (step 0), and from step 1 on the secondary
 source and the secondary energy/transport/angle chain, plus the IDD.
+
+  tests/golden/grid_sensitivity.csv  <- results/stage3_grid_sensitivity/grid_sensitivity.csv
+Four water 230 MeV runs (Nx 500/1000, 8x8/12x12 cells, 7x7/9x9 directions)
+whose settings the reference does not state.  The absolute entrance value
+(the IDD after the first depth step) of every row is reproduced to 1e-14 by
+Ly=Lz=8, Lu=Lv=2, sigma_yz=0.3, normalised initial mass, the legacy
+cross-section calibration and the stopping-power-only dose, straggling off;
+flipping either of the last two, or straggling on, moves it by 1e-8 or more.  This pins the
+oracle's calibration (src/bfpm_solver.cu:1488-1493) and stopping-power-only
+dose branches; the full runs (Dmax, BP, P90, D90, D20, normalised L2/Linf)
+are reproduced on the GPU in tests/test_gpu_harness.py.
 """
+import csv
 import os
 
 import numpy as np
@@ -68,3 +80,40 @@ def test_oracle_tables_load_like_reference(oracle_mod):
         oracle_mod.load_tables(os.path.join(TABLES, "water"), 502)
     with pytest.raises(OSError, match="Cannot open"):
         oracle_mod.load_tables("/nonexistent/dir", 10)
+
+
+def stage3_rows():
+    with open(os.path.join(GOLDEN, "grid_sensitivity.csv"), newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def stage3_case(nx, ny, nmu, **flags):
+    f = dict(normalize_initial_mass=1, calibrate_cross_sections=1, dose_catastrophic_loss=0)
+    f.update(flags)
+    return Case(Nx=nx, Ny=ny, Nz=ny, Ng=500, Nmu=nmu, Nom=nmu, Ly=8, Lz=8, Lu=2, Lv=2,
+                energy=230, sigma_yz=0.3, material="water", data=os.path.join(TABLES, "water"),
+                time=40, flags=f)
+
+
+def _entrance(oracle_mod, case):
+    s = oracle_mod.Solver(case)
+    s.advance(1)
+    return s.history()[0][0]
+
+
+@pytest.mark.parametrize("row", range(4))
+def test_oracle_reproduces_stage3_entrance(oracle_mod, row):
+    r = stage3_rows()[row]
+    case = stage3_case(int(r["nx"]), int(r["ny"]), int(r["nmu"]))
+    assert int(r["nz"]) == case.Nz and int(r["nom"]) == case.Nom
+    got = _entrance(oracle_mod, case)
+    assert abs(got - float(r["entrance"])) / float(r["entrance"]) < 1e-14, (got, r["entrance"])
+
+
+@pytest.mark.parametrize("flip", [dict(calibrate_cross_sections=0),
+                                  dict(dose_catastrophic_loss=1),
+                                  dict(eq15_straggling=1)], ids=lambda f: next(iter(f)))
+def test_stage3_flags_are_the_ones_that_reproduce_it(oracle_mod, flip):
+    r = stage3_rows()[0]
+    got = _entrance(oracle_mod, stage3_case(500, 8, 7, **flip))
+    assert abs(got - float(r["entrance"])) / float(r["entrance"]) > 1e-9
