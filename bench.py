@@ -195,16 +195,18 @@ def run_resident(args, case, workload, grid):
     assert np.all(np.isfinite(idd)) and idd[-1] > 0
     phases, step_bytes = phases_of(solver, args.steps)
     solver.close()
-    with bg.Solver(case, prec) as s2:  # skipping on (the library default)
-        e2, idd2 = timed_steps(s2, args.steps, args.warmup)
-        top = s2.active_groups()
-    skip = {"ms_per_step": round(e2 / args.steps * 1e3, 4),
-            "value": cells * args.steps / e2,
-            "bit_identical_idd": bool(np.array_equal(idd2, idd[:len(idd2)])),
-            "primary_top_group": top[0], "secondary_top_group": top[1], "Ng": case.Ng,
-            "note": "the groups above each species' top nonzero group are exact zeros and are "
-                    "skipped; same steps, same results"}
-    log(f"with active-group skipping: {skip['ms_per_step']:.3f} ms/step, top {top}")
+    skip = None
+    if not args.no_skip_figure:
+        with bg.Solver(case, prec) as s2:  # skipping on (the library default)
+            e2, idd2 = timed_steps(s2, args.steps, args.warmup)
+            top = s2.active_groups()
+        skip = {"ms_per_step": round(e2 / args.steps * 1e3, 4),
+                "value": cells * args.steps / e2,
+                "bit_identical_idd": bool(np.array_equal(idd2, idd[:len(idd2)])),
+                "primary_top_group": top[0], "secondary_top_group": top[1], "Ng": case.Ng,
+                "note": "the groups above each species' top nonzero group are exact zeros and "
+                        "are skipped; same steps, same results"}
+        log(f"with active-group skipping: {skip['ms_per_step']:.3f} ms/step, top {top}")
     accuracy = None
     if prec == bg.FP32 and not args.no_fp64_check:
         # the same steps with the FP64 instantiation (after the timed region)
@@ -298,7 +300,11 @@ def run_beamlets(args):
     t0 = time.perf_counter()
     with bg.Server(max_active=args.max_active, slice_steps=args.slice_steps) as srv:
         ids = [srv.submit(c) for c in cases]
-        served = [srv.wait(i, timeout=3600)[0] for i in ids]
+        served = []
+        for k, i in enumerate(ids):
+            served.append(srv.wait(i, timeout=3600)[0])
+            if (k + 1) % 8 == 0:
+                log(f"  server: {k + 1}/{len(ids)} results, {time.perf_counter() - t0:.1f} s")
         st = srv.stats()
     t_srv = time.perf_counter() - t0
     log(f"server: {t_srv:.2f} s, peak active {st['peak_active']}")
@@ -494,7 +500,7 @@ def run_copy(args):
 
     import bfpn_gpu_amd as bg
 
-    n = 1 << 28  # 1 GiB of float32
+    n = int(args.copy_gib * (1 << 28))  # GiB of float32 per buffer
     a = bg.DeviceArray(shape=(n,), dtype=np.float32)
     b = bg.DeviceArray(shape=(n,), dtype=np.float32)
     bg.lib().bfpn_memset(a.ptr, 0, n * 4)
@@ -528,7 +534,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
     ap.add_argument("--no-fp64-check", action="store_true",
                     help="skip the FP64 rerun of the timed steps")
+    ap.add_argument("--no-skip-figure", action="store_true",
+                    help="skip the rerun of the timed steps with active-group skipping")
     ap.add_argument("--cpu-threads", type=int, default=None)
+    ap.add_argument("--copy-gib", type=float, default=1.0, help="copy: GiB per buffer")
     ap.add_argument("--beams", type=int, default=64)
     ap.add_argument("--fp64-beams", type=int, default=0,
                     help="beamlets: FP64 reruns of the first N served beams")

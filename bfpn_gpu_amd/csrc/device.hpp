@@ -84,7 +84,13 @@ struct SourcePlanBuffers {
 struct AnglePlanBuffers {
     DeviceBuffer h, m, l;
     AnglePlan plan;
-    static bool usable(int Nmu, int Nom) { return Nmu <= 32 && Nom <= 32; }
+    // The MFMA DST holds planes of at most 32 x 32 directions and addresses a
+    // launch's planes with 32-bit offsets (launch_angle_mfma); other grids
+    // take the vector path.
+    static bool usable(int Nmu, int Nom, long long planes) {
+        return Nmu <= 32 && Nom <= 32 && planes < (1LL << 30) &&
+               planes * Nmu * Nom < (1LL << 31);
+    }
     void build(int Nmu, int Nom, hipStream_t s);
 };
 

@@ -279,7 +279,8 @@ struct Physics final : PhysicsBase {
                                                          d.dv, d.dt, p.density)));
             upload(su, cast_vector<T>(sine_basis(g.Nom)));
             upload(sv, cast_vector<T>(sine_basis(g.Nmu)));
-            if (std::is_same_v<T, float> && AnglePlanBuffers::usable(g.Nmu, g.Nom))
+            if (std::is_same_v<T, float> &&
+                AnglePlanBuffers::usable(g.Nmu, g.Nom, static_cast<long long>(g.Ng) * nyz))
                 angle_plan.build(g.Nmu, g.Nom, s);
             if (secondary) {
                 upload(qmap, source_top_map(ker, g.Ng));
@@ -1014,7 +1015,8 @@ void op_angle(bfpn_workspace* ws, T* F, const double* sig_tr_host, int nyz, int 
               int Nom, double du, double dv, double dt, double rho, hipStream_t s) {
     BFPN_REQUIRE(ws && F && sig_tr_host, "null argument");
     BFPN_REQUIRE(nyz > 0 && Ng > 0 && Nmu > 0 && Nom > 0 && Nmu * Nom <= 1024, "invalid sizes");
-    const bool mfma = std::is_same_v<T, float> && AnglePlanBuffers::usable(Nmu, Nom);
+    const bool mfma = std::is_same_v<T, float> &&
+                      AnglePlanBuffers::usable(Nmu, Nom, static_cast<long long>(Ng) * nyz);
     std::vector<double> key = {std::is_same_v<T, float> ? 0.0 : 1.0, double(Ng), double(Nmu),
                                double(Nom), du, dv, dt, rho};
     key.insert(key.end(), sig_tr_host, sig_tr_host + Ng);

@@ -157,21 +157,43 @@ __global__ __launch_bounds__(256) void source_angular_mfma(const float* __restri
             }
         __syncthreads();
     }
+    // Epilogue: each output tile passes through LDS (the staging buffers are
+    // free after the loop's last barrier) so that 16 threads store a row's
+    // 64 directions as one 256-byte run of 16-byte vectors, instead of each
+    // MFMA fragment's four 64-byte pieces.
+    constexpr int kSP = 68;  // floats per staged row (64 + 4: rows start on different banks)
+    static_assert(64 * kSP * sizeof(float) <= sizeof(xs), "output tile must fit the X staging");
+    float* stage = reinterpret_cast<float*>(&xs[0][0][0]);
+    const int orow = threadIdx.x >> 4, ocol = (threadIdx.x & 15) * 4;  // store: 16 rows per pass
 #pragma unroll
-    for (int mt = 0; mt < 2; ++mt)
+    for (int in = 0; in < 2; ++in) {
+        const f32x4 (&acc)[2][2] = in == 0 ? acc1 : acc2;
+        float* A = in == 0 ? A1 : A2;
 #pragma unroll
-        for (int nt = 0; nt < 2; ++nt) {
-            const int m = m0 + wm + nt * 16 + i;
+        for (int mt = 0; mt < 2; ++mt)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int sidx = s0 + ws + mt * 16 + 4 * q + r;
-                if (sidx < nyz && m < nang) {
-                    const long long idx = plane + static_cast<long long>(sidx) * nang + m;
-                    A1[idx] = acc1[mt][nt][r];
-                    A2[idx] = acc2[mt][nt][r];
+            for (int nt = 0; nt < 2; ++nt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    stage[(ws + mt * 16 + 4 * q + r) * kSP + wm + nt * 16 + i] = acc[mt][nt][r];
+        __syncthreads();
+#pragma unroll
+        for (int pass = 0; pass < 4; ++pass) {
+            const int row = pass * 16 + orow, sidx = s0 + row, m = m0 + ocol;
+            if (sidx < nyz && m < nang) {
+                const float* v = stage + row * kSP + ocol;
+                const long long idx = plane + static_cast<long long>(sidx) * nang + m;
+                if (VEC) {  // nang % 8 == 0: m + 3 < nang and 16-byte aligned
+                    *reinterpret_cast<float4*>(A + idx) = make_float4(v[0], v[1], v[2], v[3]);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (m + e < nang) A[idx + e] = v[e];
                 }
             }
         }
+        __syncthreads();
+    }
 }
 
 constexpr int kBandLanes = 128;  // lanes per work-group

@@ -361,3 +361,24 @@ def test_cli_writes_energy_moments_and_spot_planes(tmp_path, gpu):
     assert len(blocks) == 4 and all(len(b.splitlines()) == 5 for b in blocks)
     y0, z0, _ = map(float, blocks[0].splitlines()[0].split())
     assert (y0, z0) == (-2.0, -1.5)
+
+
+def test_dst_takes_the_vector_path_beyond_32_bit_plane_offsets(gpu):
+    """104x104 nodes x 20x20 directions x 500 groups: 2.16e9 cells, past the
+    32-bit plane offsets of the MFMA DST, so the FP32 solver takes the vector
+    DST (FP32) and still agrees with the FP64 instantiation
+    after one full depth step (86 GB of FP32 state, 173 GB of FP64)."""
+    from bfpn_gpu_amd.abi import Case
+    from bfpn_gpu_amd.tables import material_dir
+
+    case = Case(Nx=4000, Ny=103, Nz=103, Ng=500, Nmu=20, Nom=20, Ly=8, Lz=8, Lu=2, Lv=2,
+                energy=230.0, sigma_yz=0.3, material="water", data=material_dir("water"),
+                time=40.0, flags=dict(eq15_straggling=1))
+    assert case.cells * 1 >= 2**31
+    with gpu.Solver(case, gpu.FP32) as a:
+        a.advance(2)
+        ia = a.history()[0]
+    with gpu.Solver(case, gpu.FP64) as b:
+        b.advance(2)
+        ib = b.history()[0]
+    np.testing.assert_allclose(ia, ib, rtol=1e-4)

@@ -361,3 +361,24 @@ def test_beam_mass_normalisation(oracle_mod):
     # the discrete delta puts all angular mass on the 4 central directions
     nz = np.nonzero(F.reshape(c.Ng, c.nyz, c.nang).sum(axis=(0, 1)))[0]
     assert len(nz) == 4
+
+
+# ------------------------------------------------------ physics tables -----
+def test_straggling_and_transport_cross_section_known_values(oracle_mod):
+    """The straggling coefficient is one constant for every group (the
+    reference's simplified formula, src/bfpm_solver.cu:1599-1615), and
+    sigma_tr at the 230 MeV group follows the screened-Rutherford form of
+    src/bfpm_solver.cu:1617-1636; both checked against the values evaluated
+    independently in the survey of the reference (0.011591435793073675 and
+    3.7446e-05 /cm at E = 230 MeV on the 500-group, 259 MeV grid).  No
+    reference run output exercises straggling, so this is the only check of
+    the constant's value."""
+    T = oracle_mod.straggling(500)
+    np.testing.assert_allclose(T, 0.011591435793073675, rtol=1e-15)
+    dg = 259.0 / 500
+    sig = oracle_mod.transport_xs(500, dg)
+    g = int((230.0 - 1.0) / dg)  # the group whose centre (g + 1/2) dg + 1 is nearest 230 MeV
+    E = (g + 0.5) * dg + 1.0
+    assert abs(E - 230.0) < dg
+    assert sig[g] == pytest.approx(3.7446e-05, rel=2e-3)
+    assert np.all(np.diff(sig) < 0)  # falls with energy (~1/E^2 with a slowly varying log)
