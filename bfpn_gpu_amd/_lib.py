@@ -6,7 +6,7 @@ import os
 
 import numpy as np
 
-from .abi import Case, Deltas, Grid, Phys
+from .abi import Case, Deltas, Grid, Layer, Phys
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 _LIB_FILE = os.path.join(PKG, "libbfpn_hip.so")
@@ -77,6 +77,8 @@ def lib():
         getattr(L, f"bfpn_idd_{suffix}").argtypes = [vp, vp, vp, vp, vp, vp, vp, vp, i, i, i, i, d, d,
                                                      d, d, d, d, i, i, C.POINTER(d), vp]
     L.bfpn_create.argtypes = [C.POINTER(Grid), C.POINTER(Phys), C.c_char_p, i, C.POINTER(vp)]
+    L.bfpn_create_layered.argtypes = [C.POINTER(Grid), C.POINTER(Phys), C.POINTER(Layer), i, i,
+                                      C.POINTER(vp)]
     L.bfpn_destroy.argtypes = [vp]
     L.bfpn_get_deltas.argtypes = [vp, C.POINTER(Deltas)]
     L.bfpn_cells.argtypes = [vp]
@@ -192,12 +194,24 @@ TABLES = ("S", "sigma", "T", "sig_tr", "en", "ker_e1", "ker_e2", "ker_v", "omend
 class Solver:
     """One beam: tables, state and depth march on the GPU (bfpn_ctx)."""
 
-    def __init__(self, case: Case, precision: int = FP32):
+    def __init__(self, case: Case, precision: int = FP32, layers=None):
+        """layers: optional depth layers [(x_end_cm, material, density, table_dir), ...]
+        in increasing depth, the last extending to any depth (bfpn_create_layered);
+        the beam and flags come from `case`."""
         self.case = case
         self.precision = precision
+        self.layers = list(layers) if layers else None
         g, p = case.grid(), case.phys()
         h = C.c_void_p(0)
-        check(lib().bfpn_create(C.byref(g), C.byref(p), case.data.encode(), precision, C.byref(h)))
+        if self.layers:
+            n = len(self.layers)
+            spec = (Layer * n)(*[Layer(float(x), m.encode(), float(rho), path.encode())
+                                 for x, m, rho, path in self.layers])
+            check(lib().bfpn_create_layered(C.byref(g), C.byref(p), spec, n, precision,
+                                            C.byref(h)))
+        else:
+            check(lib().bfpn_create(C.byref(g), C.byref(p), case.data.encode(), precision,
+                                    C.byref(h)))
         self._h = h
         d = Deltas()
         check(lib().bfpn_get_deltas(self._h, C.byref(d)))

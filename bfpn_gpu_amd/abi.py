@@ -47,6 +47,12 @@ class Phys(C.Structure):
     ]
 
 
+class Layer(C.Structure):
+    """bfpn_layer: material tables and density up to depth x_end (cm)."""
+    _fields_ = [("x_end", C.c_double), ("material", C.c_char_p), ("density", C.c_double),
+                ("data_path", C.c_char_p)]
+
+
 class Deltas(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("dt", "dy", "dz", "dg", "du", "dv")]
 

@@ -24,6 +24,7 @@
 #include <iomanip>
 #include <iostream>
 #include <sstream>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -107,6 +108,32 @@ void save_energy_moments(const std::vector<double>& values, int ng, double dt,
     }
 }
 
+double preset_density(const std::string& material) {
+    if (material == "water") return 1.0;
+    if (material == "bone") return 1.757;
+    if (material == "air") return 0.001205;
+    throw std::runtime_error("Unknown material: " + material);
+}
+
+// --layer x_end:material[:rho[:data]]
+struct LayerArg {
+    double x_end;
+    std::string material, data;
+    double rho;
+};
+
+LayerArg parse_layer(const std::string& text) {
+    std::vector<std::string> f;
+    std::stringstream ss(text);
+    for (std::string item; std::getline(ss, item, ':');) f.push_back(item);
+    if (f.size() < 2 || f.size() > 4)
+        throw std::runtime_error("--layer expects x_end:material[:rho[:data]], got " + text);
+    LayerArg l{std::stod(f[0]), f[1], "", 0.0};
+    l.rho = f.size() > 2 && !f[2].empty() ? std::stod(f[2]) : preset_density(l.material);
+    l.data = f.size() > 3 ? f[3] : "BFPn_CPU_Solver/" + l.material;
+    return l;
+}
+
 void usage(const char* argv0) {
     std::cout << "Usage: " << argv0 << " [options]\n"
               << "Options:\n"
@@ -142,6 +169,10 @@ void usage(const char* argv0) {
               << "  --no-cross-section-calibration Use fitted sigma_c (default)\n"
               << "  --legacy-cross-section-calibration Apply the legacy sigma_c correction\n"
               << "  --sigma-yz <cm>  Initial transverse Gaussian sigma (default: 0.1)\n"
+              << "  --layer <x_end>:<material>[:<rho>[:<data>]] Depth layer up to x_end cm\n"
+              << "                   (repeat in increasing depth; the last extends to any depth;\n"
+              << "                   rho defaults to the material preset, data to\n"
+              << "                   BFPn_CPU_Solver/<material>)\n"
               << "  --streaming-full Accepted; the state always stays resident in HBM\n"
               << "  --precision <fp32|fp64> State precision (default: fp32)\n"
               << "  --lg <MeV>       Energy span of the group grid [1, 1+Lg] (default: 259)\n"
@@ -164,6 +195,7 @@ int main(int argc, char** argv) {
     int precision = BFPN_FP32;
     std::vector<double> spot_depths;
     std::string spot_prefix = "spot";
+    std::vector<LayerArg> layers;
     bfpn_phys phys{};
     phys.dose_catastrophic_loss = 1;
 
@@ -231,6 +263,7 @@ int main(int argc, char** argv) {
                     return 1;
                 }
             } else if (a == "--spot-depths" && has) spot_depths = parse_depth_list(argv[++i]);
+            else if (a == "--layer" && has) layers.push_back(parse_layer(argv[++i]));
             else if (a == "--spot-prefix" && has) spot_prefix = argv[++i];
             else if ((a == "--lane-chunk" || a == "--energy-chunk" || a == "--stream-dir" ||
                       a == "--idd-stride") && has)
@@ -287,7 +320,17 @@ int main(int argc, char** argv) {
               << (streaming ? " (streaming flags ignored: in-core)" : "") << std::endl;
 
     bfpn_ctx* ctx = nullptr;
-    if (bfpn_create(&grid, &phys, data_path.c_str(), precision, &ctx) != BFPN_OK) {
+    std::vector<bfpn_layer> spec;
+    for (const auto& l : layers) {
+        spec.push_back({l.x_end, l.material.c_str(), l.rho, l.data.c_str()});
+        std::cout << "Layer: to x=" << l.x_end << " cm, " << l.material << ", rho=" << l.rho
+                  << ", data " << l.data << std::endl;
+    }
+    const bfpn_status created =
+        spec.empty() ? bfpn_create(&grid, &phys, data_path.c_str(), precision, &ctx)
+                     : bfpn_create_layered(&grid, &phys, spec.data(), static_cast<int>(spec.size()),
+                                           precision, &ctx);
+    if (created != BFPN_OK) {
         std::cerr << "Error: " << bfpn_last_error() << std::endl;
         return 1;
     }

@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -231,6 +232,7 @@ struct Physics final : PhysicsBase {
     DeviceBuffer coef, dose_w, yzw, om_dev, factors, su, sv, kv, e1, e2, beg, end, qmap, ucoef;
     DeviceBuffer kcount, klist;  // FP64 source: nonzero Kv blocks
     bool legacy = false, lite = false;  // upwind energy model; lite-memory primary step
+    double density = 1.0;               // g/cm^3: the IDD and spot-plane scale
     SourcePlanBuffers plan;
     AnglePlanBuffers angle_plan;
 
@@ -245,6 +247,7 @@ struct Physics final : PhysicsBase {
         secondary = !p.primary_only && !p.energy_only && !p.lite_memory;
         legacy = p.legacy_energy != 0;
         lite = p.lite_memory != 0;
+        density = p.density;
         key = physics_key(g, p, data_path, prec);
 
         // host tables (initialize, src/bfpm_solver.cu:1435-1568)
@@ -382,7 +385,18 @@ public:
                    x_[1].bytes() + hist_.bytes() + partial_p_.bytes() +
                    partial_s_.bytes();
         for (const auto& s : state_) b += s.bytes();
+        for (size_t k = 1; k < layers_.size(); ++k) b += layers_[k]->device_bytes();
         return b;
+    }
+
+    // Depth layers: layer k's tables serve the steps whose midpoint depth
+    // lies below x_end[k] (the last layer: any depth).  The tables must share
+    // this context's grid and flags; the first layer is the one it was built on.
+    void set_layers(std::vector<std::shared_ptr<Physics<T>>> layers, std::vector<double> x_end) {
+        BFPN_REQUIRE(!layers.empty() && layers.front() == P_ && x_end.size() == layers.size(),
+                     "layer tables do not match the context");
+        layers_ = std::move(layers);
+        x_end_ = std::move(x_end);
     }
 
     int steps_done() const override { return step_; }
@@ -415,7 +429,7 @@ public:
         const int nyz = static_cast<int>(P_->nyz);
         diag_out_.ensure(static_cast<size_t>(nyz) * sizeof(double));
         launch_spot_plane<T>(cur_[0], cur_[1], cur_[2], cur_[3], P_->coef.template as<EnergyCoef>(),
-                             grid.Ng, nyz, P_->nang, d.du * d.dv / phys.density,
+                             grid.Ng, nyz, P_->nang, d.du * d.dv / P_->density,
                              diag_out_.as<double>(), stream);
         BFPN_HIP(hipStreamSynchronize(stream));
         BFPN_HIP(hipMemcpy(host, diag_out_.as<double>(), static_cast<size_t>(nyz) * sizeof(double),
@@ -432,10 +446,17 @@ public:
             BFPN_HIP(hipStreamWaitEvent(stream, ev_tail_, 0));
         }
         for (int k = 0; k < nsteps; ++k) {
+            bool first = k == 0, last = k == nsteps - 1;
+            if (!layers_.empty()) {  // a paired sweep never spans two materials
+                const int L = layer_of(step_);
+                P_ = layers_[L];
+                first = first || L != layer_of(step_ - 1);
+                last = last || L != layer_of(step_ + 1);
+            }
             if (P_->lite)
                 lite_step();
             else  // the upwind model has no paired sweep: every step stands alone
-                step(k == 0 || P_->legacy, k == nsteps - 1 || P_->legacy);
+                step(first || P_->legacy, last || P_->legacy);
         }
         if (stream2_ && !overlap) {  // all on `stream`: done_stream still ends last
             BFPN_HIP(hipEventRecord(ev_tail_, stream));
@@ -514,6 +535,13 @@ private:
     // step n-1 has waited for the source of step n-2, which the secondary
     // stream runs after the sweeps of step n-3.
     static constexpr int kTopSlots = 14;
+    // Layer of depth step n: the one holding the step's midpoint (n + 1/2) dt.
+    int layer_of(int n) const {
+        const double x = (n + 0.5) * d.dt;
+        for (size_t k = 0; k + 1 < layers_.size(); ++k)
+            if (x < x_end_[k]) return static_cast<int>(k);
+        return static_cast<int>(layers_.size()) - 1;
+    }
     int* top_slot(int k) { return tops_.as<int>() + k; }
     int* x_top(int n) { return top_slot(6 + (n & 7)); }
     const int* cur_top(int species) {
@@ -696,7 +724,7 @@ private:
                                 false, P_->yzw.template as<double>(), P_->nang, pp, false, stream,
                                 sweep_tops(0, -1, -1, n));
         mark(kEnergy, array_bytes() * 2, stream);
-        launch_finish_sum(pp, eb_, nullptr, 0, d.du * d.dv * d.dy * d.dz / phys.density,
+        launch_finish_sum(pp, eb_, nullptr, 0, d.du * d.dv * d.dy * d.dz / P_->density,
                           hist_.as<double>() + n, stream);
         mark(kDose, 0.0, stream);
         ++step_;
@@ -775,7 +803,7 @@ private:
             mark(kWait, 0.0, ss);
         }
         // ---- secondary: stepSecondary (src/bfpm_solver.cu:3335-3394), step n > 0
-        const double scale = d.du * d.dv * d.dy * d.dz / phys.density;
+        const double scale = d.du * d.dv * d.dy * d.dz / P_->density;
         double* hist = hist_.as<double>();
         auto finish = [&](int m, bool with_secondary) {  // IDD and proxy of step m
             double* pm = partial_p_.as<double>() + (m & 3) * 2 * eb;
@@ -814,7 +842,9 @@ private:
         ++step_;
     }
 
-    std::shared_ptr<Physics<T>> P_;
+    std::shared_ptr<Physics<T>> P_;  // tables of the current (or last) step
+    std::vector<std::shared_ptr<Physics<T>>> layers_;  // empty: one material
+    std::vector<double> x_end_;
     DeviceBuffer state_[4], a1_, a2_, q_[2], x_[2];
     DeviceBuffer diag_part_, diag_out_;  // diagnostics scratch, allocated on first use
     DeviceBuffer partial_p_, partial_s_, hist_, tops_;
@@ -847,6 +877,44 @@ bfpn_ctx* make_context(std::shared_ptr<PhysicsBase> physics, const bfpn_grid& g,
     if (physics->precision == BFPN_FP32)
         return new Context<float>(std::static_pointer_cast<Physics<float>>(physics), g, p);
     return new Context<double>(std::static_pointer_cast<Physics<double>>(physics), g, p);
+}
+
+void validate_layers(const std::vector<LayerSpec>& layers) {
+    BFPN_REQUIRE(!layers.empty(), "at least one layer is needed");
+    for (size_t k = 0; k < layers.size(); ++k) {
+        const LayerSpec& l = layers[k];
+        BFPN_REQUIRE(l.density > 0, "layer density must be positive");
+        BFPN_REQUIRE(l.material.size() < sizeof(bfpn_phys{}.material_name),
+                     "material name too long");
+        if (k + 1 == layers.size()) break;  // the last layer's depth is not used
+        BFPN_REQUIRE(l.x_end > 0, "layer depths must be positive");
+        BFPN_REQUIRE(k == 0 || l.x_end > layers[k - 1].x_end, "layer depths must increase");
+    }
+}
+
+bfpn_ctx* make_layered_context(const bfpn_grid& g, const bfpn_phys& p,
+                               const std::vector<LayerSpec>& layers, int precision) {
+    validate_layers(layers);
+    std::vector<std::shared_ptr<PhysicsBase>> tables;
+    std::vector<double> x_end;
+    for (const LayerSpec& l : layers) {
+        bfpn_phys q = p;
+        std::snprintf(q.material_name, sizeof(q.material_name), "%s", l.material.c_str());
+        q.density = q.energy_density = l.density;
+        tables.push_back(make_physics(g, q, l.data_path, precision));
+        x_end.push_back(l.x_end);
+    }
+    std::unique_ptr<bfpn_ctx> ctx(make_context(tables.front(), g, p));
+    if (precision == BFPN_FP32) {
+        std::vector<std::shared_ptr<Physics<float>>> t;
+        for (auto& b : tables) t.push_back(std::static_pointer_cast<Physics<float>>(b));
+        static_cast<Context<float>*>(ctx.get())->set_layers(std::move(t), std::move(x_end));
+    } else {
+        std::vector<std::shared_ptr<Physics<double>>> t;
+        for (auto& b : tables) t.push_back(std::static_pointer_cast<Physics<double>>(b));
+        static_cast<Context<double>*>(ctx.get())->set_layers(std::move(t), std::move(x_end));
+    }
+    return ctx.release();
 }
 
 size_t context_bytes(const bfpn_grid& g, const bfpn_phys& p, int precision) {
@@ -1318,6 +1386,33 @@ BFPN_API bfpn_status bfpn_create(const bfpn_grid* grid, const bfpn_phys* phys, c
                             ", Nmu=" + std::to_string(grid->Nmu) + ", Nom=" + std::to_string(grid->Nom) + ")");
         auto physics = make_physics(*grid, *phys, data_path ? data_path : "./data", precision);
         *out = make_context(physics, *grid, *phys);
+    });
+}
+
+BFPN_API bfpn_status bfpn_create_layered(const bfpn_grid* grid, const bfpn_phys* phys,
+                                         const bfpn_layer* layers, int nlayers, int precision,
+                                         bfpn_ctx** out) {
+    return guarded([&] {
+        BFPN_REQUIRE(grid && phys && layers && out && nlayers > 0, "null argument");
+        *out = nullptr;
+        validate_params(*grid, *phys);
+        std::vector<LayerSpec> spec;
+        for (int k = 0; k < nlayers; ++k) {
+            BFPN_REQUIRE(layers[k].material && layers[k].data_path, "null layer field");
+            spec.push_back({layers[k].x_end, layers[k].material, layers[k].density,
+                            layers[k].data_path});
+        }
+        validate_layers(spec);  // before any device call
+        size_t free_b = 0, total_b = 0;
+        BFPN_HIP(hipMemGetInfo(&free_b, &total_b));
+        const size_t need = context_bytes(*grid, *phys, precision) +
+                            nlayers * physics_bytes(*grid, *phys, precision) + (64ull << 20);
+        if (need > free_b)
+            throw Error(BFPN_E_NOMEM, "Requested grid and " + std::to_string(nlayers) +
+                                          " layers need about " + std::to_string(need >> 20) +
+                                          " MiB of device memory, but only " +
+                                          std::to_string(free_b >> 20) + " MiB is free");
+        *out = make_layered_context(*grid, *phys, spec, precision);
     });
 }
 

@@ -43,6 +43,16 @@ size_t physics_bytes(const bfpn_grid& g, const bfpn_phys& p, int precision);
 
 void validate_params(const bfpn_grid& g, const bfpn_phys& p);
 
+// One depth layer of a layered medium: material tables and density up to
+// depth x_end (cm).
+struct LayerSpec {
+    double x_end;
+    std::string material;
+    double density;
+    std::string data_path;
+};
+void validate_layers(const std::vector<LayerSpec>& layers);
+
 }  // namespace bfpn
 
 // One beam: its state in HBM, history, stream; the opaque handle of the C ABI.
@@ -76,4 +86,8 @@ struct bfpn_ctx {
 namespace bfpn {
 bfpn_ctx* make_context(std::shared_ptr<PhysicsBase> physics, const bfpn_grid& g,
                        const bfpn_phys& p);
+// A context marching through depth layers (tables built per layer; the beam
+// from `p`).
+bfpn_ctx* make_layered_context(const bfpn_grid& g, const bfpn_phys& p,
+                               const std::vector<LayerSpec>& layers, int precision);
 }  // namespace bfpn

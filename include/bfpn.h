@@ -261,6 +261,29 @@ BFPN_API bfpn_status bfpn_idd_f32_noalloc(
 BFPN_API bfpn_status bfpn_create(const bfpn_grid* grid, const bfpn_phys* phys,
                                  const char* data_path, int precision,
                                  bfpn_ctx** out);
+
+/* One layer of a medium stratified in depth: the material tables
+ * (material_name as in bfpn_phys, data_path as in bfpn_create) and density
+ * (g/cm^3) serve the depth steps whose midpoint (n + 1/2) dt lies below x_end
+ * (cm); the last layer extends to any depth. */
+typedef struct bfpn_layer {
+    double x_end;
+    const char* material;
+    double density;
+    const char* data_path;
+} bfpn_layer;
+
+/* bfpn_create for a layered medium.  No counterpart in the reference, whose
+ * solver is homogeneous (the material-map item of TODO.md:114-118 and the
+ * validation gap of README.md:30-37).  Each layer builds its own tables from
+ * `phys` with the layer's material and density (both `density` and
+ * `energy_density`); the Eq.23 beam comes from `phys`.  Depth steps use the
+ * tables of their layer: the energy sweeps, the angular-diffusion factors,
+ * the secondary source and the IDD scale 1/density.  Layer depths must
+ * increase.  bfpn_get_table returns the first layer's tables. */
+BFPN_API bfpn_status bfpn_create_layered(const bfpn_grid* grid, const bfpn_phys* phys,
+                                         const bfpn_layer* layers, int nlayers,
+                                         int precision, bfpn_ctx** out);
 BFPN_API void bfpn_destroy(bfpn_ctx* ctx);
 
 /* Derived deltas and array sizes. */

@@ -76,6 +76,9 @@ def lib():
             [C.c_int, _dp]
         L.orc_create.argtypes = [C.POINTER(Grid), C.POINTER(Phys), C.c_char_p]
         L.orc_create.restype = C.c_void_p
+        L.orc_create_layered.argtypes = [C.POINTER(Grid), C.POINTER(Phys), C.c_int, _dp,
+                                         C.POINTER(C.c_char_p), _dp, C.POINTER(C.c_char_p)]
+        L.orc_create_layered.restype = C.c_void_p
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_num_steps.argtypes = [C.c_void_p, C.c_double]
         L.orc_advance.argtypes = [C.c_void_p, C.c_int]
@@ -238,10 +241,20 @@ TABLE_NAMES = ("S", "sigma", "T", "sig_tr", "en", "ker_e1", "ker_e2", "ker_v", "
 class Solver:
     """Whole in-core eq15 solve on the CPU (BFPnSolver, FP64)."""
 
-    def __init__(self, case: Case):
+    def __init__(self, case: Case, layers=None):
+        """layers: optional [(x_end_cm, material, density, table_dir), ...] in
+        increasing depth; the last layer extends to any depth."""
         self.case = case
         g, p = case.grid(), case.phys()
-        self._h = lib().orc_create(C.byref(g), C.byref(p), case.data.encode())
+        if layers:
+            n = len(layers)
+            x_end = np.array([float(l[0]) for l in layers])
+            mats = (C.c_char_p * n)(*[l[1].encode() for l in layers])
+            dens = np.array([float(l[2]) for l in layers])
+            paths = (C.c_char_p * n)(*[l[3].encode() for l in layers])
+            self._h = lib().orc_create_layered(C.byref(g), C.byref(p), n, x_end, mats, dens, paths)
+        else:
+            self._h = lib().orc_create(C.byref(g), C.byref(p), case.data.encode())
         if not self._h:
             raise OSError(lib().orc_last_error().decode())
 

@@ -23,6 +23,7 @@
 #include <fstream>
 #include <limits>
 #include <string>
+#include <memory>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -836,7 +837,21 @@ struct Solver {
     std::vector<double> F, f, F1, f1, Q;
     std::vector<double> idd, proxy;
     int step = 0;
+    // Depth layers (none: homogeneous): layer k spans depths up to x_end[k]
+    // cm, the last one to any depth; each holds its own material tables and
+    // density, the state stays in this object.
+    std::vector<std::unique_ptr<Solver>> layers;
+    std::vector<double> x_end;
 };
+
+// The material of depth step n: the layer holding the step's midpoint.
+const Solver& tables_of_step(const Solver& s) {
+    if (s.layers.empty()) return s;
+    const double x = (s.step + 0.5) * s.d.dt;
+    for (size_t k = 0; k + 1 < s.layers.size(); ++k)
+        if (x < s.x_end[k]) return *s.layers[k];
+    return *s.layers.back();
+}
 
 bool solver_init(Solver& s, const bfpn_grid& g, const bfpn_phys& p, const std::string& path) {
     s.g = g;
@@ -882,15 +897,16 @@ bool solver_init(Solver& s, const bfpn_grid& g, const bfpn_phys& p, const std::s
 // computeScalarDoseProxy (3418-3444).
 void solver_step(Solver& s) {
     const auto& g = s.g;
-    const auto& p = s.p;
+    const Solver& t = tables_of_step(s);  // material tables and density of this step
+    const auto& p = t.p;
     const auto& d = s.d;
     const double half = 0.5 * d.dt;
     const bool strag = p.eq15_straggling != 0;
     if (p.lite_memory) {  // stepLitePrimary + computeIntegratedDepthDoseLite
-        energy_half_legacy(s.F.data(), nullptr, s.S.data(), s.sigma.data(), nullptr, d.dt, d.dg,
+        energy_half_legacy(s.F.data(), nullptr, t.S.data(), t.sigma.data(), nullptr, d.dt, d.dg,
                            1.0, g.Ng, s.nyz, s.nang, nullptr, false);
-        s.idd.push_back(idd_value(s.F.data(), nullptr, nullptr, nullptr, s.S.data(),
-                                  s.sigma.data(), s.en.data(), g.Ng, g.Ny, g.Nz, s.nang, d.dg,
+        s.idd.push_back(idd_value(s.F.data(), nullptr, nullptr, nullptr, t.S.data(),
+                                  t.sigma.data(), t.en.data(), g.Ng, g.Ny, g.Nz, s.nang, d.dg,
                                   d.du, d.dv, d.dy, d.dz, p.density, p.trapezoidal_yz != 0,
                                   p.dose_catastrophic_loss != 0));
         s.proxy.push_back(0.0);
@@ -899,17 +915,17 @@ void solver_step(Solver& s) {
     }
     auto E = [&](std::vector<double>& a, std::vector<double>& b, const double* Q) {
         if (p.legacy_energy)
-            energy_half_legacy(a.data(), b.data(), s.S.data(), s.sigma.data(), s.T.data(), half,
+            energy_half_legacy(a.data(), b.data(), t.S.data(), t.sigma.data(), t.T.data(), half,
                                d.dg, p.energy_density, g.Ng, s.nyz, s.nang, Q, strag);
         else
-            energy_half(a.data(), b.data(), s.S.data(), s.sigma.data(), s.T.data(), half, d.dg,
+            energy_half(a.data(), b.data(), t.S.data(), t.sigma.data(), t.T.data(), half, d.dg,
                         p.energy_density, g.Ng, s.nyz, s.nang, Q, nullptr, strag);
     };
     auto Tr = [&](std::vector<double>& a, bool preserve) {
         transport_half(a.data(), s.om.data(), g.Ny, g.Nz, g.Ng, s.nang, d.dy, d.dz, half, preserve);
     };
     auto A = [&](std::vector<double>& a) {
-        angle_full(a.data(), s.sig_tr.data(), s.nyz, g.Ng, g.Nmu, g.Nom, d.du, d.dv, d.dt, p.density);
+        angle_full(a.data(), t.sig_tr.data(), s.nyz, g.Ng, g.Nmu, g.Nom, d.du, d.dv, d.dt, p.density);
     };
     if (p.energy_only) {
         E(s.F, s.f, nullptr);
@@ -926,8 +942,8 @@ void solver_step(Solver& s) {
         // from the same primary state, so one evaluation serves both.
         s.Q.assign(s.N, 0.0);
         auto src = g_direct_source ? secondary_source : secondary_source_factorized;
-        src(s.F.data(), s.f.data(), s.Q.data(), s.ke1.data(), s.ke2.data(),
-                         s.kv.data(), s.beg.data(), s.end.data(), g.Ng, s.nyz, s.nang, d.dg);
+        src(s.F.data(), s.f.data(), s.Q.data(), t.ke1.data(), t.ke2.data(),
+                         t.kv.data(), t.beg.data(), t.end.data(), g.Ng, s.nyz, s.nang, d.dg);
         E(s.F1, s.f1, s.Q.data());
         Tr(s.F1, false);
         Tr(s.f1, true);
@@ -937,8 +953,8 @@ void solver_step(Solver& s) {
         Tr(s.f1, true);
         E(s.F1, s.f1, s.Q.data());
     }
-    double v = idd_value(s.F.data(), s.f.data(), s.F1.data(), s.f1.data(), s.S.data(),
-                         s.sigma.data(), s.en.data(), g.Ng, g.Ny, g.Nz, s.nang, d.dg, d.du, d.dv,
+    double v = idd_value(s.F.data(), s.f.data(), s.F1.data(), s.f1.data(), t.S.data(),
+                         t.sigma.data(), t.en.data(), g.Ng, g.Ny, g.Nz, s.nang, d.dg, d.du, d.dv,
                          d.dy, d.dz, p.density, p.trapezoidal_yz != 0, p.dose_catastrophic_loss != 0);
     double px = 0.0;
     for (size_t k = 0; k < s.N; ++k) px += s.f[k];
@@ -1060,6 +1076,39 @@ BFPN_API void* orc_create(const bfpn_grid* g, const bfpn_phys* p, const char* pa
     if (!solver_init(*s, *g, *p, path ? path : "./data")) {
         delete s;
         return nullptr;
+    }
+    return s;
+}
+
+// A solve through depth layers: layer k (material_k, density_k, table
+// directory path_k) spans depths up to x_end[k] cm, the last one to any depth.
+BFPN_API void* orc_create_layered(const bfpn_grid* g, const bfpn_phys* p, int n,
+                                  const double* x_end, const char* const* materials,
+                                  const double* densities, const char* const* paths) {
+    if (n < 1) {
+        g_err = "at least one layer is needed";
+        return nullptr;
+    }
+    auto* s = new Solver();
+    if (!solver_init(*s, *g, *p, paths[0])) {
+        delete s;
+        return nullptr;
+    }
+    for (int k = 0; k < n; ++k) {
+        bfpn_phys q = *p;
+        std::snprintf(q.material_name, sizeof(q.material_name), "%s", materials[k]);
+        q.density = q.energy_density = densities[k];
+        auto layer = std::make_unique<Solver>();
+        if (!solver_init(*layer, *g, q, paths[k])) {
+            delete s;
+            return nullptr;
+        }
+        layer->F.clear();  // tables only
+        layer->f.clear();
+        layer->F1.clear();
+        layer->f1.clear();
+        s->layers.push_back(std::move(layer));
+        s->x_end.push_back(x_end[k]);
     }
     return s;
 }

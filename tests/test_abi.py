@@ -76,3 +76,32 @@ def test_cli_help_and_test_mode():
 def test_cli_rejects_like_reference(args, msg):
     r = _cli(*args)
     assert r.returncode == 1 and msg in r.stderr
+
+
+@pytest.mark.parametrize("stack,msg", [
+    ([], b"null argument"),
+    ([(2.0, b"water", 0.0)], b"density must be positive"),
+    ([(2.0, b"water", 1.0), (1.0, b"bone", 1.85), (9.0, b"water", 1.0)], b"must increase"),
+    ([(-1.0, b"water", 1.0), (9.0, b"bone", 1.85)], b"must be positive"),
+    ([(2.0, b"x" * 40, 1.0)], b"too long"),
+])
+def test_layered_create_rejects_bad_stacks_before_any_device_call(stack, msg):
+    import ctypes as C
+
+    import bfpn_gpu_amd as bg
+    from bfpn_gpu_amd.abi import Case, Layer
+
+    L = bg.lib()
+    g, p = Case().grid(), Case().phys()
+    arr = (Layer * max(1, len(stack)))(*[Layer(x, m, r, b"./data") for x, m, r in stack])
+    h = C.c_void_p(0)
+    st = L.bfpn_create_layered(C.byref(g), C.byref(p), arr, len(stack), 0, C.byref(h))
+    assert st == 1 and not h.value
+    assert msg in L.bfpn_last_error()
+
+
+@pytest.mark.parametrize("arg,msg", [("2:steel", "Unknown material: steel"),
+                                     ("2", "--layer expects"), ("x:water", "stod")])
+def test_cli_rejects_bad_layers(arg, msg):
+    r = _cli("--layer", arg)
+    assert r.returncode == 1 and msg in r.stderr
