@@ -73,6 +73,13 @@ struct SourcePlan {
     // 32-wide K tiles holding a nonzero entry, ascending.
     const int* kt_count = nullptr;    // [Ng][mtiles]
     const int* kt_list = nullptr;     // [Ng][mtiles][ktiles] (32-bit: scalar loads)
+    // Source groups whose A rows the band GEMM reads: inside some output
+    // group's band and with a nonzero Kv (the others' A rows are zero or never
+    // read, so the angular GEMM skips them).
+    const int* glist = nullptr;        // [ngroups] ascending
+    const int* gcount = nullptr;       // [Ng + 1]: entries of glist below g
+    const unsigned* amask = nullptr;   // [ngp / 32]: bit g % 32 of word g / 32 set for glist
+    int ngroups = 0;
 };
 int source_nap(int nang);    // nang padded to 32
 int source_ngp(int Ng);      // Ng padded to 32 (E columns)

@@ -86,6 +86,26 @@ void SourcePlanBuffers::build(const float* kv_f32, const std::vector<double>& e1
                 if (flags[b * ktiles + k]) list[b * ktiles + count[b]++] = k;
         upload(kt_count, count);
         upload(kt_list, list);
+        // source groups the band GEMM reads with a nonzero Kv
+        std::vector<char> in_band(Ng, 0);
+        for (int r = 0; r < Ng; ++r)
+            for (int g = std::max(0, beg[r]); g < std::min(Ng, end[r]); ++g) in_band[g] = 1;
+        std::vector<int> gl, gc(static_cast<size_t>(Ng) + 1, 0);
+        std::vector<unsigned> am(static_cast<size_t>(ngp) / 32, 0u);
+        for (int g = 0; g < Ng; ++g) {
+            bool nz = false;
+            for (int mt = 0; mt < mtiles && !nz; ++mt) nz = count[static_cast<size_t>(g) * mtiles + mt] > 0;
+            gc[g + 1] = gc[g];
+            if (nz && in_band[g]) {
+                gl.push_back(g);
+                ++gc[g + 1];
+                am[g / 32] |= 1u << (g % 32);
+            }
+        }
+        upload(glist, gl.empty() ? std::vector<int>{0} : gl);
+        upload(gcount, gc);
+        upload(amask, am);
+        plan.ngroups = static_cast<int>(gl.size());
     }
     const size_t ne = static_cast<size_t>(erows) * ngp;
     std::vector<float> E1(ne, 0.f), E2(ne, 0.f);
@@ -123,6 +143,9 @@ void SourcePlanBuffers::build(const float* kv_f32, const std::vector<double>& e1
     plan.tile_band = band.as<int2>();
     plan.kt_count = kt_count.as<int>();
     plan.kt_list = kt_list.as<int>();
+    plan.glist = glist.as<int>();
+    plan.gcount = gcount.as<int>();
+    plan.amask = amask.as<unsigned>();
 }
 
 

@@ -53,23 +53,27 @@ __global__ __launch_bounds__(256) void source_angular_mfma(const float* __restri
                                                            const __bf16* __restrict__ ktl,
                                                            const int* __restrict__ kt_count,
                                                            const int* __restrict__ kt_list,
+                                                           const int* __restrict__ glist,
+                                                           const int* __restrict__ gcount,
                                                            float* __restrict__ A1,
-                                                           float* __restrict__ A2, int nyz,
-                                                           int nang, int nap, int tiles,
+                                                           float* __restrict__ A2, int Ng,
+                                                           int nyz, int nang, int nap, int tiles,
                                                            const int* __restrict__ xtop) {
     __shared__ __attribute__((aligned(16))) __bf16 xs[4][64][kKPitch];  // X1 hi/lo, X2 hi/lo
     __shared__ __attribute__((aligned(16))) __bf16 ks[2][64][kKPitch];  // Kv^T hi/lo
     const int mtiles = (nang + 63) / 64, stiles = (nyz + 63) / 64;
-    // Source groups above the primary's top are zero: their tiles (the last
-    // ones) are skipped, and their A rows are never read.
+    // Tiles run over the source groups of glist (tiles = its length x per_g).
+    // Those above the primary's top are zero: their tiles (the last ones) are
+    // skipped, and their A rows are never read.
     const int per_g = mtiles * stiles;
-    const int active = xtop ? min(tiles, (min(*xtop, tiles / per_g - 1) + 1) * per_g) : tiles;
+    const int active =
+        xtop ? min(tiles, gcount[min(max(-1, *xtop), Ng - 1) + 1] * per_g) : tiles;
     const int n8 = (active + 7) & ~7;
     if (static_cast<int>(blockIdx.x) >= n8) return;
     const int L = xcd_tile(blockIdx.x, n8);
     if (L >= active) return;
-    const int g = L / (mtiles * stiles);
-    const int rest = L - g * mtiles * stiles;
+    const int gi = L / per_g, g = glist[gi];
+    const int rest = L - gi * per_g;
     const int s0 = (rest / mtiles) * 64, m0 = (rest % mtiles) * 64;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = lane & 15, q = lane >> 4;
@@ -218,8 +222,9 @@ template <bool VEC>
 __global__ __launch_bounds__(256) void source_band_mfma(
     const float* __restrict__ A1, const float* __restrict__ A2, const __bf16* __restrict__ e1h,
     const __bf16* __restrict__ e1l, const __bf16* __restrict__ e2h, const __bf16* __restrict__ e2l,
-    const int2* __restrict__ tile_band, float* __restrict__ Q, int Ng, int ngp, long long lanes,
-    int tiles, const int* __restrict__ xtop, const int* __restrict__ qmap) {
+    const int2* __restrict__ tile_band, const unsigned* __restrict__ amask, float* __restrict__ Q,
+    int Ng, int ngp, long long lanes, int tiles, const int* __restrict__ xtop,
+    const int* __restrict__ qmap) {
     __shared__ __attribute__((aligned(16))) float bs[2][32][kBandLanes];        // A1, A2
     __shared__ __attribute__((aligned(16))) __bf16 es[4][kBandRows][kKPitch];   // E1 hi/lo, E2 hi/lo
     int rblocks = (Ng + kBandRows - 1) / kBandRows;
@@ -262,17 +267,21 @@ __global__ __launch_bounds__(256) void source_band_mfma(
     float4 r1[4], r2[4];
     bf16x8 re[4];
     const __bf16* esrc = wave == 0 ? e1h : wave == 1 ? e1l : wave == 2 ? e2h : e2l;
+    // A rows of source groups outside amask (zero Kv, or in no band) are not
+    // written by the angular GEMM and read as zero.
     auto load = [&](int k0) {
+        const unsigned rows = amask[k0 >> 5];  // k0 % 32 == 0
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int e4 = threadIdx.x + j * 256;
             const int r = e4 >> 5, c = (e4 & 31) * 4;
             const int gs = k0 + r;
+            const bool has = gs <= kmax && ((rows >> r) & 1u);
             const long long ln = n_base + c;
             const long long idx = static_cast<long long>(gs) * lanes + ln;
             if (VEC) {
                 r1[j] = r2[j] = make_float4(0.f, 0.f, 0.f, 0.f);
-                if (gs <= kmax && ln < lanes) {
+                if (has && ln < lanes) {
                     r1[j] = *reinterpret_cast<const float4*>(A1 + idx);
                     r2[j] = *reinterpret_cast<const float4*>(A2 + idx);
                 }
@@ -280,7 +289,7 @@ __global__ __launch_bounds__(256) void source_band_mfma(
                 float a[4], b[4];
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
-                    const bool in = gs <= kmax && ln + e < lanes;
+                    const bool in = has && ln + e < lanes;
                     a[e] = in ? A1[idx + e] : 0.f;
                     b[e] = in ? A2[idx + e] : 0.f;
                 }
@@ -608,18 +617,24 @@ void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, 
                         hipEvent_t inputs_read, const int* xtop, const int* qmap) {
     BFPN_REQUIRE((xtop == nullptr) == (qmap == nullptr), "source tops need both xtop and qmap");
     const int nap = source_nap(nang);
-    const long long t1 = static_cast<long long>(Ng) * ((nyz + 63) / 64) * ((nang + 63) / 64);
+    // tiles over the source groups whose A rows the band GEMM reads
+    const long long t1 =
+        static_cast<long long>(plan.ngroups) * ((nyz + 63) / 64) * ((nang + 63) / 64);
     BFPN_REQUIRE(t1 < (1LL << 31) - 8, "source GEMM grid too large");
     const int g1 = xcd_grid(t1);
-    if (nang % 8 == 0)
-        hipLaunchKernelGGL(source_angular_mfma<true>, dim3(g1), dim3(256), 0, s, Fp, fp, plan.kth,
-                           plan.ktl, plan.kt_count, plan.kt_list, A1, A2, nyz, nang, nap,
-                           static_cast<int>(t1), xtop);
-    else
-        hipLaunchKernelGGL(source_angular_mfma<false>, dim3(g1), dim3(256), 0, s, Fp, fp, plan.kth,
-                           plan.ktl, plan.kt_count, plan.kt_list, A1, A2, nyz, nang, nap,
-                           static_cast<int>(t1), xtop);
-    BFPN_HIP(hipGetLastError());
+    if (g1 > 0) {
+        if (nang % 8 == 0)
+            hipLaunchKernelGGL(source_angular_mfma<true>, dim3(g1), dim3(256), 0, s, Fp, fp,
+                               plan.kth, plan.ktl, plan.kt_count, plan.kt_list, plan.glist,
+                               plan.gcount, A1, A2, Ng, nyz, nang, nap, static_cast<int>(t1),
+                               xtop);
+        else
+            hipLaunchKernelGGL(source_angular_mfma<false>, dim3(g1), dim3(256), 0, s, Fp, fp,
+                               plan.kth, plan.ktl, plan.kt_count, plan.kt_list, plan.glist,
+                               plan.gcount, A1, A2, Ng, nyz, nang, nap, static_cast<int>(t1),
+                               xtop);
+        BFPN_HIP(hipGetLastError());
+    }
     if (inputs_read) BFPN_HIP(hipEventRecord(inputs_read, s));
     const long long lanes = static_cast<long long>(nyz) * nang;
     const long long t2 =
@@ -628,12 +643,12 @@ void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, 
     const int g2 = xcd_grid(t2);
     if (lanes % 4 == 0)
         hipLaunchKernelGGL(source_band_mfma<true>, dim3(g2), dim3(256), 0, s, A1, A2, plan.e1h,
-                           plan.e1l, plan.e2h, plan.e2l, plan.tile_band, Q, Ng, source_ngp(Ng),
-                           lanes, static_cast<int>(t2), xtop, qmap);
+                           plan.e1l, plan.e2h, plan.e2l, plan.tile_band, plan.amask, Q, Ng,
+                           source_ngp(Ng), lanes, static_cast<int>(t2), xtop, qmap);
     else
         hipLaunchKernelGGL(source_band_mfma<false>, dim3(g2), dim3(256), 0, s, A1, A2, plan.e1h,
-                           plan.e1l, plan.e2h, plan.e2l, plan.tile_band, Q, Ng, source_ngp(Ng),
-                           lanes, static_cast<int>(t2), xtop, qmap);
+                           plan.e1l, plan.e2h, plan.e2l, plan.tile_band, plan.amask, Q, Ng,
+                           source_ngp(Ng), lanes, static_cast<int>(t2), xtop, qmap);
     BFPN_HIP(hipGetLastError());
 }
 

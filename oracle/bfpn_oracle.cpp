@@ -13,6 +13,15 @@
 // Parallelism: OpenMP over independent lanes / planes / cells only; every
 // reduction is done in a fixed order, so results do not depend on the thread
 // count.
+//
+// Parity unpinned: the value of straggling() (T_c).  No run output of the
+// reference has straggling on; the value is checked only through its role in
+// the energy operator (one explicit diffusion step) and against the constant
+// evaluated from the reference's formula.  Every other function is checked
+// against the reference's run outputs, closed forms or independent data.
+// Depth layers (orc_create_layered, tables_of_step) have no counterpart in
+// the reference; they are checked against the homogeneous run and against a
+// CSDA integration of each layer's stopping-power table.
 
 #include "../include/bfpn.h"
 

@@ -382,3 +382,35 @@ def test_straggling_and_transport_cross_section_known_values(oracle_mod):
     assert abs(E - 230.0) < dg
     assert sig[g] == pytest.approx(3.7446e-05, rel=2e-3)
     assert np.all(np.diff(sig) < 0)  # falls with energy (~1/E^2 with a slowly varying log)
+
+
+def test_air_stopping_power_is_nist_dry_air_by_bragg_additivity(oracle_mod):
+    """Air (the material no reference output uses): the stopping-power table
+    is the water curve scaled by the Bragg-additivity ratio of the material's
+    mass fractions to water's (src/bfpm_solver.cu:150-196, 1570-1597), so
+    S_air / S_water cancels the water curve and the constant.  That ratio
+    must equal the one computed from NIST's published dry-air composition
+    (C 0.000124, N 0.755267, O 0.231781, Ar 0.012827 by mass) with the
+    per-element Bethe term whose water and bone instances the reference's
+    own outputs already pin; a wrong element, weight or atomic mass (Ar's
+    A = 40 is used by no other material) moves it by >1e-3."""
+    def element(E, Z, A):  # Bethe term per element, I(Z) of the reference's fit
+        I = 19.0 if Z == 1 else (11.2 + 11.7 * Z if Z <= 13 else 52.8 + 8.71 * Z)
+        b2 = E * (E + 2 * 938.3) / (E + 938.3) ** 2
+        return Z / A / b2 * (np.log(2 * 0.51099895e6 * b2 / (1 - b2) / I) - b2)
+
+    def table(material):
+        c = Case(Nx=10, Ny=1, Nz=1, Ng=500, Nmu=1, Nom=1, material=material,
+                 data=os.path.join(TABLES, material), time=0.1)
+        return oracle_mod.Solver(c).table("S")
+
+    Sw, Sa = table("water"), table("air")
+    nist_air = [(6, 12.0, 0.000124), (7, 14.0, 0.755267), (8, 16.0, 0.231781),
+                (18, 40.0, 0.012827)]
+    water = [(1, 1.0, 0.1111), (8, 16.0, 0.8889)]  # the reference's water fractions
+    dg = 259.0 / 500
+    E = (np.arange(1, 500) - 0.5) * dg + 1.0  # the table's interior points
+    want = (sum(w * element(E, Z, A) for Z, A, w in nist_air) /
+            sum(w * element(E, Z, A) for Z, A, w in water))
+    np.testing.assert_allclose(Sa[1:-1] / Sw[1:-1], want, rtol=2e-4)
+    assert 0.85 < (Sa / Sw).mean() < 0.88
