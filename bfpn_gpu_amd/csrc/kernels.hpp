@@ -107,17 +107,26 @@ void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, 
 // Secondary source (FP64 vector path): A1/A2 angular contraction, banded sum.
 // kcount/klist (optional, device; from source_angular_blocks): per (g,
 // 64-direction tile) the 16-deep K blocks of Kv with a nonzero entry.
+// need (optional, device, [Ng]; from source_needed_groups): the source groups
+// whose A rows the banded sum reads; the others are neither written nor read
+// (their contributions are exact zeros).
 template <typename T>
 void launch_source_angular(const T* Fp, const T* fp, const T* kv, T* A1, T* A2, int Ng, int nyz,
                            int nang, hipStream_t s, const int* xtop = nullptr,
-                           const int* kcount = nullptr, const int* klist = nullptr);
+                           const int* kcount = nullptr, const int* klist = nullptr,
+                           const int* need = nullptr);
 // Host: the nonzero K blocks of Kv ([g][in][out], FP64) for launch_source_angular.
 void source_angular_blocks(const std::vector<double>& kv, int Ng, int nang, std::vector<int>& count,
                            std::vector<int>& list);
+// Host: need[g] = 1 if source group g lies in some output group's band
+// [beg, end) and its Kv (count from source_angular_blocks) is not all zero.
+std::vector<int> source_needed_groups(const std::vector<int>& count, const std::vector<int>& beg,
+                                      const std::vector<int>& end, int Ng, int nang);
 template <typename T>
 void launch_source_energy(const T* A1, const T* A2, const double* e1, const double* e2,
                           const int* beg, const int* end, T* Q, int Ng, long long lanes, double dg,
-                          hipStream_t s, const int* xtop = nullptr, const int* qmap = nullptr);
+                          hipStream_t s, const int* xtop = nullptr, const int* qmap = nullptr,
+                          const int* need = nullptr);
 
 // MUSCL predictor-corrector transport half step, in place (no scratch).
 // top (optional, device): groups above it are zero and are skipped.

@@ -253,7 +253,7 @@ int beam_top(const BeamFactors& b) {
 template <typename T>
 struct Physics final : PhysicsBase {
     DeviceBuffer coef, dose_w, yzw, om_dev, factors, su, sv, kv, e1, e2, beg, end, qmap, ucoef;
-    DeviceBuffer kcount, klist;  // FP64 source: nonzero Kv blocks
+    DeviceBuffer kcount, klist, need;  // FP64 source: nonzero Kv blocks, groups read
     bool legacy = false, lite = false;  // upwind energy model; lite-memory primary step
     double density = 1.0;               // g/cm^3: the IDD and spot-plane scale
     SourcePlanBuffers plan;
@@ -323,6 +323,7 @@ struct Physics final : PhysicsBase {
                     source_angular_blocks(ker.v, g.Ng, nang, cnt, lst);
                     upload(kcount, cnt);
                     upload(klist, lst);
+                    upload(need, source_needed_groups(cnt, ker.beg, ker.end, g.Ng, nang));
                 }
             }
             BFPN_HIP(hipStreamSynchronize(s));
@@ -337,7 +338,8 @@ struct Physics final : PhysicsBase {
         return coef.bytes() + dose_w.bytes() + yzw.bytes() + ucoef.bytes() + om_dev.bytes() +
                factors.bytes() +
                su.bytes() + sv.bytes() + kv.bytes() + e1.bytes() + e2.bytes() + beg.bytes() +
-               end.bytes() + qmap.bytes() + kcount.bytes() + klist.bytes() + plan.bytes() +
+               end.bytes() + qmap.bytes() + kcount.bytes() + klist.bytes() + need.bytes() +
+               plan.bytes() +
                angle_plan.h.bytes() +
                angle_plan.m.bytes() + angle_plan.l.bytes();
     }
@@ -722,14 +724,16 @@ private:
             launch_source_mfma(Fp, fp, a1_.as<float>(), a2_.as<float>(), Q, P_->plan.plan,
                                grid.Ng, P_->nyz, P_->nang, s, inputs_read, xt, qm);
         } else {
+            const int* need = P_->need.template as<int>();
             launch_source_angular<T>(Fp, fp, P_->kv.template as<T>(), a1_.as<T>(), a2_.as<T>(),
                                      grid.Ng, P_->nyz, P_->nang, s, xt,
-                                     P_->kcount.template as<int>(), P_->klist.template as<int>());
+                                     P_->kcount.template as<int>(), P_->klist.template as<int>(),
+                                     need);
             if (inputs_read) BFPN_HIP(hipEventRecord(inputs_read, s));
             launch_source_energy<T>(a1_.as<T>(), a2_.as<T>(), P_->e1.template as<double>(),
                                     P_->e2.template as<double>(), P_->beg.template as<int>(),
                                     P_->end.template as<int>(), Q, grid.Ng, P_->lanes, d.dg, s,
-                                    xt, qm);
+                                    xt, qm, need);
         }
         mark(kSource, array_bytes() * 7, s);  // 2 in, 2 out; 2 in, 1 out
     }

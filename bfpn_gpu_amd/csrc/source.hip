@@ -405,12 +405,14 @@ __global__ __launch_bounds__(256) void source_angular_kernel(const T* __restrict
                                                              int nyz, int nang,
                                                              const int* __restrict__ xtop,
                                                              const int* __restrict__ kcount,
-                                                             const int* __restrict__ klist) {
+                                                             const int* __restrict__ klist,
+                                                             const int* __restrict__ need) {
     __shared__ T x1s[kDepth][kTile + 1];
     __shared__ T x2s[kDepth][kTile + 1];
     __shared__ T ks[kDepth][kTile + 1];
     const int g = blockIdx.z;
     if (xtop && g > *xtop) return;  // a zero source group: its A rows are never read
+    if (need && !need[g]) return;   // A rows the banded sum does not read
     const int m0 = blockIdx.x * kTile;
     const int s0 = blockIdx.y * kTile;
     const long long plane = static_cast<long long>(g) * nyz * nang;
@@ -503,7 +505,7 @@ __global__ __launch_bounds__(256) void source_energy_kernel(
     const T* __restrict__ A1, const T* __restrict__ A2, const double* __restrict__ e1,
     const double* __restrict__ e2, const int* __restrict__ beg, const int* __restrict__ end,
     T* __restrict__ Q, int Ng, long long lanes, double dg, const int* __restrict__ xtop,
-    const int* __restrict__ qmap) {
+    const int* __restrict__ qmap, const int* __restrict__ need) {
     __shared__ double w1s[kEnergyRows][kEnergyChunk];
     __shared__ double w2s[kEnergyRows][kEnergyChunk];
     const int g0 = blockIdx.y * kEnergyRows;
@@ -539,6 +541,7 @@ __global__ __launch_bounds__(256) void source_energy_kernel(
         __syncthreads();
         if (live) {
             for (int c = 0; c < cn; ++c) {
+                if (need && !need[c0 + c]) continue;  // A row unwritten; its terms are zeros
                 const long long idx = static_cast<long long>(c0 + c) * lanes + lane;
                 const double a1 = static_cast<double>(A1[idx]);
                 const double a2 = static_cast<double>(A2[idx]);
@@ -584,6 +587,19 @@ void source_angular_blocks(const std::vector<double>& kv, int Ng, int nang, std:
                 if (nz) list[t * ktiles + count[t]++] = kt;
             }
         }
+}
+std::vector<int> source_needed_groups(const std::vector<int>& count, const std::vector<int>& beg,
+                                      const std::vector<int>& end, int Ng, int nang) {
+    const int mtiles = (nang + kTile - 1) / kTile;
+    std::vector<int> need(Ng, 0);
+    for (int r = 0; r < Ng; ++r)
+        for (int g = std::max(0, beg[r]); g < std::min(Ng, end[r]); ++g) need[g] = 1;
+    for (int g = 0; g < Ng; ++g) {
+        bool nz = false;
+        for (int mt = 0; mt < mtiles && !nz; ++mt) nz = count[static_cast<size_t>(g) * mtiles + mt] > 0;
+        need[g] = need[g] && nz;
+    }
+    return need;
 }
 int source_mtiles(int nang) { return (nang + 63) / 64; }
 int source_ktiles(int nang) { return source_nap(nang) / 32; }
@@ -655,37 +671,38 @@ void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, 
 template <typename T>
 void launch_source_angular(const T* Fp, const T* fp, const T* kv, T* A1, T* A2, int Ng, int nyz,
                            int nang, hipStream_t s, const int* xtop, const int* kcount,
-                           const int* klist) {
+                           const int* klist, const int* need) {
     if (Ng <= 0 || nyz <= 0 || nang <= 0) return;
     dim3 grid((nang + kTile - 1) / kTile, (nyz + kTile - 1) / kTile, Ng);
     hipLaunchKernelGGL(source_angular_kernel<T>, grid, dim3(256), 0, s, Fp, fp, kv, A1, A2, nyz,
-                       nang, xtop, kcount, klist);
+                       nang, xtop, kcount, klist, need);
     BFPN_HIP(hipGetLastError());
 }
 
 template <typename T>
 void launch_source_energy(const T* A1, const T* A2, const double* e1, const double* e2,
                           const int* beg, const int* end, T* Q, int Ng, long long lanes, double dg,
-                          hipStream_t s, const int* xtop, const int* qmap) {
+                          hipStream_t s, const int* xtop, const int* qmap, const int* need) {
     if (Ng <= 0 || lanes <= 0) return;
     BFPN_REQUIRE((xtop == nullptr) == (qmap == nullptr), "source tops need both xtop and qmap");
     dim3 grid(static_cast<unsigned>((lanes + 255) / 256), (Ng + kEnergyRows - 1) / kEnergyRows);
     hipLaunchKernelGGL(source_energy_kernel<T>, grid, dim3(256), 0, s, A1, A2, e1, e2, beg, end, Q,
-                       Ng, lanes, dg, xtop, qmap);
+                       Ng, lanes, dg, xtop, qmap, need);
     BFPN_HIP(hipGetLastError());
 }
 
 template void launch_source_angular<float>(const float*, const float*, const float*, float*, float*,
                                            int, int, int, hipStream_t, const int*, const int*,
-                                           const int*);
+                                           const int*, const int*);
 template void launch_source_angular<double>(const double*, const double*, const double*, double*,
                                             double*, int, int, int, hipStream_t, const int*,
-                                            const int*, const int*);
+                                            const int*, const int*, const int*);
 template void launch_source_energy<float>(const float*, const float*, const double*, const double*,
                                           const int*, const int*, float*, int, long long, double,
-                                          hipStream_t, const int*, const int*);
+                                          hipStream_t, const int*, const int*, const int*);
 template void launch_source_energy<double>(const double*, const double*, const double*,
                                            const double*, const int*, const int*, double*, int,
-                                           long long, double, hipStream_t, const int*, const int*);
+                                           long long, double, hipStream_t, const int*, const int*,
+                                           const int*);
 
 }  // namespace bfpn
