@@ -69,7 +69,7 @@ std::vector<double> yz_weights(int Ny, int Nz, bool trapezoidal);
 // Device storage of a SourcePlan (FP32 source on the matrix cores).
 struct SourcePlanBuffers {
     DeviceBuffer kth, ktl, e1h, e1l, e2h, e2l, band, kt_count, kt_list, glist, gcount, amask,
-        scratch;
+        steps, scratch;
     SourcePlan plan;
     // kv_f32: device FP32 Kv [Ng][nang][nang]; e1/e2/beg/end: host tables.
     void build(const float* kv_f32, const std::vector<double>& e1, const std::vector<double>& e2,
@@ -78,7 +78,7 @@ struct SourcePlanBuffers {
     size_t bytes() const {
         return kth.bytes() + ktl.bytes() + e1h.bytes() + e1l.bytes() + e2h.bytes() + e2l.bytes() +
                band.bytes() + kt_count.bytes() + kt_list.bytes() + glist.bytes() +
-               gcount.bytes() + amask.bytes();
+               gcount.bytes() + amask.bytes() + steps.bytes();
     }
 };
 

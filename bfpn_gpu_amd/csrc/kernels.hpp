@@ -80,7 +80,15 @@ struct SourcePlan {
     const int* gcount = nullptr;       // [Ng + 1]: entries of glist below g
     const unsigned* amask = nullptr;   // [ngp / 32]: bit g % 32 of word g / 32 set for glist
     int ngroups = 0;
+    // Band GEMM as one march over the source groups (when every 32-wide step's
+    // window of live 16-group output tiles fits source_march_slots()): per
+    // step the first and last tile whose band holds it (empty: x > y), and
+    // the steps [march_first, march_last) that hold any.
+    const int2* march_steps = nullptr;  // [ngp / 32]
+    int march_first = 0, march_last = 0;
+    bool march = false;
 };
+int source_march_slots();  // output tiles a march step may hold at once
 int source_nap(int nang);    // nang padded to 32
 int source_ngp(int Ng);      // Ng padded to 32 (E columns)
 int source_erows(int Ng);    // Ng padded to 64 (E rows); tile_band has erows/16 entries

@@ -132,6 +132,30 @@ void SourcePlanBuffers::build(const float* kv_f32, const std::vector<double>& e1
         tiles[t] = hi > 0 ? make_int2(lo, hi) : make_int2(0, 0);
     }
     upload(band, tiles);
+    {  // per 32-wide source step: the window of output tiles whose band holds it
+        const int nsteps = ngp / 32;
+        std::vector<int2> st(nsteps, make_int2(1, 0));
+        int first = nsteps, last = 0, widest = 0;
+        for (int k = 0; k < nsteps; ++k) {
+            for (int t = 0; t < static_cast<int>(tiles.size()); ++t) {
+                const int2 b = tiles[t];
+                if (b.y > b.x && b.x < 32 * k + 32 && b.y > 32 * k) {
+                    if (st[k].x > st[k].y) st[k] = make_int2(t, t);
+                    st[k].x = std::min(st[k].x, t);
+                    st[k].y = std::max(st[k].y, t);
+                }
+            }
+            if (st[k].x <= st[k].y) {
+                first = std::min(first, k);
+                last = k + 1;
+                widest = std::max(widest, st[k].y - st[k].x + 1);
+            }
+        }
+        upload(steps, st);
+        plan.march_first = std::min(first, last);
+        plan.march_last = last;
+        plan.march = widest <= source_march_slots();
+    }
     BFPN_HIP(hipStreamSynchronize(s));
     scratch.release();
     plan.kth = kth.as<__bf16>();
@@ -146,6 +170,7 @@ void SourcePlanBuffers::build(const float* kv_f32, const std::vector<double>& e1
     plan.glist = glist.as<int>();
     plan.gcount = gcount.as<int>();
     plan.amask = amask.as<unsigned>();
+    plan.march_steps = steps.as<int2>();
 }
 
 

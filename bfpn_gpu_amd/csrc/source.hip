@@ -352,6 +352,161 @@ __global__ __launch_bounds__(256) void source_band_mfma(
         }
 }
 
+// Banded energy GEMM as one march over the source groups.  A work-group
+// owns 128 lanes and walks the 32-wide source-group steps once, so every A
+// row it needs is read once; the output tiles (16 groups) whose bands hold
+// the current step -- at most kMarchSlots consecutive tiles, a host-checked
+// window -- keep their accumulators in a register ring (tile t in slot
+// t mod kMarchSlots), and a tile is stored when the march passes its band.
+// Per step the work-group stages the 32 x 128 tiles of A1 and A2 (FP32,
+// XOR-swizzled as in source_band_mfma) and the window's E rows (BF16 hi/lo of
+// E1 and E2) in LDS, the latter in MFMA fragment order (16-byte chunk
+// [slot][array][q][row]); the next step's tiles are in flight in registers.
+// 8 waves, 16 lanes each.  Per output tile the sums run over the same steps
+// in the same order, with the same operands, as in source_band_mfma.
+constexpr int kMarchLanes = 128;
+constexpr int kMarchSlots = 10;
+
+template <bool VEC>
+__global__ __launch_bounds__(512, 4) void source_band_march(
+    const float* __restrict__ A1, const float* __restrict__ A2, const __bf16* __restrict__ e1h,
+    const __bf16* __restrict__ e1l, const __bf16* __restrict__ e2h, const __bf16* __restrict__ e2l,
+    const int2* __restrict__ tile_band, const int2* __restrict__ steps, int s_first, int s_last,
+    const unsigned* __restrict__ amask, float* __restrict__ Q, int Ng, int ngp, long long lanes,
+    int lblocks, const int* __restrict__ xtop, const int* __restrict__ qmap) {
+    constexpr int W = kMarchSlots;
+    __shared__ __attribute__((aligned(16))) float as[2][32][kMarchLanes];  // A1, A2
+    __shared__ bf16x8 es[W * 256];  // [slot][array][q][row] 16-byte chunks
+    const int n8 = (lblocks + 7) & ~7;
+    if (static_cast<int>(blockIdx.x) >= n8) return;
+    const int L = xcd_tile(blockIdx.x, n8);
+    if (L >= lblocks) return;
+    const long long n_base = static_cast<long long>(L) * kMarchLanes;
+    // A rows above the primary's top (kmax) read as zero; output rows above
+    // the source's top (gmax) are never read.
+    int kmax = Ng - 1, gmax = Ng - 1;
+    if (xtop) {
+        kmax = max(-1, min(Ng - 1, *xtop));
+        gmax = qmap[kmax + 1];
+    }
+    const int s_end = min(s_last, kmax < 0 ? 0 : kmax / 32 + 1);  // exclusive
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = lane & 15, q = lane >> 4;
+    const long long ln_out = n_base + wave * 16 + i;  // this lane's output column
+    f32x4 acc[W];
+#pragma unroll
+    for (int s = 0; s < W; ++s) acc[s] = zero4();
+
+    float4 ra[4];   // next step's A: 2 float4 per input
+    bf16x8 re[5];   // next step's E window: W * 256 / 512 chunks per thread
+    auto load = [&](int k) {
+        const int k0 = 32 * k;
+        const unsigned rows = amask[k];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int in = j >> 1, e4 = threadIdx.x + (j & 1) * 512;
+            const int r = e4 >> 5, c = (e4 & 31) * 4;
+            const int gs = k0 + r;
+            const bool has = gs <= kmax && ((rows >> r) & 1u);
+            const long long ln = n_base + c;
+            const float* A = in == 0 ? A1 : A2;
+            const long long idx = static_cast<long long>(gs) * lanes + ln;
+            if (VEC) {
+                ra[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (has && ln < lanes) ra[j] = *reinterpret_cast<const float4*>(A + idx);
+            } else {
+                float v[4];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v[e] = has && ln + e < lanes ? A[idx + e] : 0.f;
+                ra[j] = make_float4(v[0], v[1], v[2], v[3]);
+            }
+        }
+        const int2 w = steps[k];  // live tiles [x, y]
+        const int r0 = w.x % W;
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            const int c = threadIdx.x + 512 * j;
+            const int slot = c >> 8, rem = c & 255;
+            const int a = rem >> 6, cq = (rem >> 4) & 3, row = rem & 15;
+            const int t = w.x + (slot - r0 + W) % W;
+            re[j] = bf16x8{};
+            if (t <= w.y) {
+                const __bf16* E = a == 0 ? e1h : a == 1 ? e1l : a == 2 ? e2h : e2l;
+                re[j] = *reinterpret_cast<const bf16x8*>(
+                    E + static_cast<long long>(16 * t + row) * ngp + k0 + 8 * cq);
+            }
+        }
+    };
+    auto store_tile = [&](int t, const f32x4& v) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int g = 16 * t + 4 * q + r;
+            if (g < Ng && ln_out < lanes) Q[static_cast<long long>(g) * lanes + ln_out] = v[r];
+        }
+    };
+
+    if (s_first < s_end) load(s_first);
+    for (int k = s_first; k < s_end; ++k) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int in = j >> 1, e4 = threadIdx.x + (j & 1) * 512;
+            const int r = e4 >> 5, c = band_swz(r, (e4 & 31) * 4);
+            *reinterpret_cast<float4*>(&as[in][r][c]) = ra[j];
+        }
+#pragma unroll
+        for (int j = 0; j < 5; ++j) es[threadIdx.x + 512 * j] = re[j];
+        __syncthreads();
+        if (k + 1 < s_end) load(k + 1);
+        const int2 w = steps[k];
+        const int r0 = w.x % W;
+        int tl[W];
+        bool on[W], done[W];
+#pragma unroll
+        for (int s = 0; s < W; ++s) {  // the tile in each slot, and whether it works / retires
+            tl[s] = w.x + (s - r0 + W) % W;
+            on[s] = done[s] = false;
+            if (tl[s] <= w.y) {
+                const int2 b = tile_band[tl[s]];
+                on[s] = b.y > b.x && b.x < 32 * k + 32 && b.y > 32 * k;
+                done[s] = on[s] && min((b.y - 1) / 32, s_end - 1) == k;
+            }
+        }
+#pragma unroll
+        for (int in = 0; in < 2; ++in) {  // E1.A1, then E2.A2 (the same order per accumulator)
+            SplitFrag bf;
+            {
+                const int c = wave * 16 + i;
+                float v[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) v[e] = as[in][8 * q + e][band_swz(8 * q + e, c)];
+                bf = split8(v);
+            }
+#pragma unroll
+            for (int s = 0; s < W; ++s) {
+                if (!on[s]) continue;
+                SplitFrag a;
+                a.hi = es[s * 256 + (2 * in) * 64 + q * 16 + i];
+                a.lo = es[s * 256 + (2 * in + 1) * 64 + q * 16 + i];
+                acc[s] = mma_split(a, bf, acc[s]);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < W; ++s)
+            if (done[s]) {
+                store_tile(tl[s], acc[s]);
+                acc[s] = zero4();
+            }
+        __syncthreads();
+    }
+    // tiles no step touched: zero rows up to the source's top
+    const int tiles = min((gmax + 16) / 16, (Ng + 15) / 16);
+    for (int t = 0; t < tiles; ++t) {
+        const int2 b = tile_band[t];
+        const bool marched = b.y > b.x && b.x < 32 * s_end && b.y > 32 * s_first && s_first < s_end;
+        if (!marched) store_tile(t, zero4());
+    }
+}
+
 // grid: (ktiles, mtiles, Ng)
 __global__ __launch_bounds__(256) void kv_block_flags_kernel(const __bf16* __restrict__ kth,
                                                              unsigned char* __restrict__ flags,
@@ -602,6 +757,7 @@ std::vector<int> source_needed_groups(const std::vector<int>& count, const std::
     return need;
 }
 int source_mtiles(int nang) { return (nang + 63) / 64; }
+int source_march_slots() { return kMarchSlots; }
 int source_ktiles(int nang) { return source_nap(nang) / 32; }
 
 void launch_kv_block_flags(const __bf16* kth, unsigned char* flags, int Ng, int nang,
@@ -653,6 +809,23 @@ void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, 
     }
     if (inputs_read) BFPN_HIP(hipEventRecord(inputs_read, s));
     const long long lanes = static_cast<long long>(nyz) * nang;
+    if (plan.march) {
+        const long long lb = (lanes + kMarchLanes - 1) / kMarchLanes;
+        BFPN_REQUIRE(lb < (1LL << 31) - 8, "source band grid too large");
+        const int gm = xcd_grid(lb);
+        if (lanes % 4 == 0)
+            hipLaunchKernelGGL(source_band_march<true>, dim3(gm), dim3(512), 0, s, A1, A2,
+                               plan.e1h, plan.e1l, plan.e2h, plan.e2l, plan.tile_band,
+                               plan.march_steps, plan.march_first, plan.march_last, plan.amask, Q,
+                               Ng, source_ngp(Ng), lanes, static_cast<int>(lb), xtop, qmap);
+        else
+            hipLaunchKernelGGL(source_band_march<false>, dim3(gm), dim3(512), 0, s, A1, A2,
+                               plan.e1h, plan.e1l, plan.e2h, plan.e2l, plan.tile_band,
+                               plan.march_steps, plan.march_first, plan.march_last, plan.amask, Q,
+                               Ng, source_ngp(Ng), lanes, static_cast<int>(lb), xtop, qmap);
+        BFPN_HIP(hipGetLastError());
+        return;
+    }
     const long long t2 =
         ((lanes + kBandLanes - 1) / kBandLanes) * ((Ng + kBandRows - 1) / kBandRows);
     BFPN_REQUIRE(t2 < (1LL << 31) - 8, "source band grid too large");
