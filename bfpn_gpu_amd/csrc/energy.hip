@@ -22,6 +22,8 @@
 
 #include "kernels.hpp"
 
+#include <type_traits>
+
 #include <utility>
 
 namespace bfpn {
@@ -330,8 +332,8 @@ template <typename T, int LPT, bool SRC, bool STRAG, bool XOUT>
 __global__ __launch_bounds__(kBlock) void energy_pair_kernel(
     T* __restrict__ F, T* __restrict__ f, const EnergyCoef* __restrict__ coef,
     const T* __restrict__ Qa, const T* __restrict__ Qb, T* __restrict__ XF, T* __restrict__ Xf,
-    int Ng, int lanes, const double* __restrict__ yzw, int nang, double* __restrict__ partials,
-    int proxy, SweepTops tops) {
+    const unsigned* __restrict__ xmask, int Ng, int lanes, const double* __restrict__ yzw,
+    int nang, double* __restrict__ partials, int proxy, SweepTops tops) {
     constexpr int D = pair_depth<T, LPT, SRC>(), RING = D + 1;
     __shared__ double lds[kBlock / 64];
     __shared__ int ldi[kBlock / 64];
@@ -425,7 +427,8 @@ __global__ __launch_bounds__(kBlock) void energy_pair_kernel(
                     acc_proxy += fv[j];
                 }
             }
-            if (XOUT) {
+            // (only the groups the source reads, when xmask is given)
+            if (XOUT && (!xmask || ((xmask[g >> 5] >> (g & 31)) & 1u))) {
                 const long long base = static_cast<long long>(g) * lanes;
                 store_pack<T, LPT>(XF + base + lane, Fv);
                 store_pack<T, LPT>(Xf + base + lane, fv);
@@ -541,32 +544,33 @@ __global__ __launch_bounds__(kBlock) void energy_upwind_kernel(
 
 template <typename T, int LPT, bool SRC, bool STRAG>
 void dispatch_pair(T* F, T* f, const EnergyCoef* coef, const T* Qa, const T* Qb, T* XF, T* Xf,
-                   int Ng, long long lanes, const double* yzw, int nang, double* partials,
-                   bool proxy, hipStream_t s, const SweepTops& tops) {
+                   const unsigned* xmask, int Ng, long long lanes, const double* yzw, int nang,
+                   double* partials, bool proxy, hipStream_t s, const SweepTops& tops) {
     const int blocks = energy_blocks(lanes, sizeof(T));
     BFPN_REQUIRE(lanes < (1LL << 31), "energy sweep indexes lanes with 32-bit offsets");
     if (XF)
         hipLaunchKernelGGL((energy_pair_kernel<T, LPT, SRC, STRAG, true>), dim3(blocks),
-                           dim3(kBlock), 0, s, F, f, coef, Qa, Qb, XF, Xf, Ng,
+                           dim3(kBlock), 0, s, F, f, coef, Qa, Qb, XF, Xf, xmask, Ng,
                            static_cast<int>(lanes), yzw, nang, partials, proxy ? 1 : 0, tops);
     else
         hipLaunchKernelGGL((energy_pair_kernel<T, LPT, SRC, STRAG, false>), dim3(blocks),
-                           dim3(kBlock), 0, s, F, f, coef, Qa, Qb, XF, Xf, Ng,
+                           dim3(kBlock), 0, s, F, f, coef, Qa, Qb, XF, Xf, xmask, Ng,
                            static_cast<int>(lanes), yzw, nang, partials, proxy ? 1 : 0, tops);
 }
 
 template <typename T, bool SRC, bool STRAG>
 void dispatch_pair_lpt(T* F, T* f, const EnergyCoef* coef, const T* Qa, const T* Qb, T* XF,
-                       T* Xf, int Ng, long long lanes, const double* yzw, int nang,
-                       double* partials, bool proxy, hipStream_t s, const SweepTops& tops) {
+                       T* Xf, const unsigned* xmask, int Ng, long long lanes, const double* yzw,
+                       int nang, double* partials, bool proxy, hipStream_t s,
+                       const SweepTops& tops) {
     if constexpr (sizeof(T) == 4) {
         const int lpt = lanes_per_thread(lanes, sizeof(T));
         if (lpt == 4)
-            return dispatch_pair<T, 4, SRC, STRAG>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yzw,
-                                                   nang, partials, proxy, s, tops);
+            return dispatch_pair<T, 4, SRC, STRAG>(F, f, coef, Qa, Qb, XF, Xf, xmask, Ng, lanes,
+                                                   yzw, nang, partials, proxy, s, tops);
     }
-    dispatch_pair<T, 1, SRC, STRAG>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yzw, nang, partials,
-                                    proxy, s, tops);
+    dispatch_pair<T, 1, SRC, STRAG>(F, f, coef, Qa, Qb, XF, Xf, xmask, Ng, lanes, yzw, nang,
+                                    partials, proxy, s, tops);
 }
 
 template <typename T, int LPT, bool SRC, bool STRAG>
@@ -632,25 +636,24 @@ template <typename T>
 void launch_energy_pair(T* F, T* f, const EnergyCoef* coef, const T* Qa, const T* Qb, int Ng,
                         long long lanes, bool straggling, const double* yz_weight, int nang,
                         double* dose_partials, bool proxy, T* XF, T* Xf, hipStream_t s,
-                        const SweepTops& tops) {
+                        const SweepTops& tops, const unsigned* xmask) {
     if (lanes <= 0 || Ng <= 0) return;
     BFPN_REQUIRE(dose_partials, "the paired energy sweep needs dose partials");
     BFPN_REQUIRE((Qa != nullptr) == (Qb != nullptr), "both half steps need a source, or neither");
     BFPN_REQUIRE((XF != nullptr) == (Xf != nullptr), "the intermediate state needs both arrays");
+    const auto run = [&](auto src, auto strag) {
+        dispatch_pair_lpt<T, decltype(src)::value, decltype(strag)::value>(
+            F, f, coef, Qa, Qb, XF, Xf, xmask, Ng, lanes, yz_weight, nang, dose_partials, proxy,
+            s, tops);
+    };
+    using Y = std::true_type;
+    using N = std::false_type;
     if (Qa) {
-        if (straggling)
-            dispatch_pair_lpt<T, true, true>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yz_weight,
-                                              nang, dose_partials, proxy, s, tops);
-        else
-            dispatch_pair_lpt<T, true, false>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yz_weight,
-                                               nang, dose_partials, proxy, s, tops);
+        if (straggling) run(Y{}, Y{});
+        else run(Y{}, N{});
     } else {
-        if (straggling)
-            dispatch_pair_lpt<T, false, true>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yz_weight,
-                                               nang, dose_partials, proxy, s, tops);
-        else
-            dispatch_pair_lpt<T, false, false>(F, f, coef, Qa, Qb, XF, Xf, Ng, lanes, yz_weight,
-                                                nang, dose_partials, proxy, s, tops);
+        if (straggling) run(N{}, Y{});
+        else run(N{}, N{});
     }
     BFPN_HIP(hipGetLastError());
 }
@@ -691,11 +694,11 @@ template void launch_energy_upwind<double>(double*, double*, const UpwindCoef*, 
 template void launch_energy_pair<float>(float*, float*, const EnergyCoef*, const float*,
                                         const float*, int, long long, bool, const double*, int,
                                         double*, bool, float*, float*, hipStream_t,
-                                        const SweepTops&);
+                                        const SweepTops&, const unsigned*);
 template void launch_energy_pair<double>(double*, double*, const EnergyCoef*, const double*,
                                          const double*, int, long long, bool, const double*, int,
                                          double*, bool, double*, double*, hipStream_t,
-                                         const SweepTops&);
+                                         const SweepTops&, const unsigned*);
 
 template void launch_energy_half<float>(float*, float*, const EnergyCoef*, const float*, int,
                                         long long, bool, const double*, int, double*, bool,

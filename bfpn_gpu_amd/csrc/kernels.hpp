@@ -43,12 +43,13 @@ int energy_blocks(long long lanes, size_t elem);
 
 // Two consecutive energy half steps in one sweep (end of depth step n with
 // source Qa and the dose/proxy partials of step n, start of step n+1 with
-// source Qb); XF/Xf (optional) receive the state between them.
+// source Qb); XF/Xf (optional) receive the state between them, at every group
+// or, with xmask (device, bit g % 32 of word g / 32), at the groups it sets.
 template <typename T>
 void launch_energy_pair(T* F, T* f, const EnergyCoef* coef, const T* Qa, const T* Qb, int Ng,
                         long long lanes, bool straggling, const double* yz_weight, int nang,
                         double* dose_partials, bool proxy, T* XF, T* Xf, hipStream_t s,
-                        const SweepTops& tops = SweepTops{});
+                        const SweepTops& tops = SweepTops{}, const unsigned* xmask = nullptr);
 // First-order upwind energy step, in place (legacy energy model; the
 // lite-memory step with f == null).  uc: Ng UpwindCoef; coef supplies the
 // dose weights w1/w2 when dose_partials != null (partials as above).

@@ -279,6 +279,7 @@ template <typename T>
 struct Physics final : PhysicsBase {
     DeviceBuffer coef, dose_w, yzw, om_dev, factors, su, sv, kv, e1, e2, beg, end, qmap, ucoef;
     DeviceBuffer kcount, klist, need;  // FP64 source: nonzero Kv blocks, groups read
+    DeviceBuffer xmask64;              // FP64: the groups read, as bits (FP32: plan.amask)
     bool legacy = false, lite = false;  // upwind energy model; lite-memory primary step
     double density = 1.0;               // g/cm^3: the IDD and spot-plane scale
     SourcePlanBuffers plan;
@@ -348,7 +349,13 @@ struct Physics final : PhysicsBase {
                     source_angular_blocks(ker.v, g.Ng, nang, cnt, lst);
                     upload(kcount, cnt);
                     upload(klist, lst);
-                    upload(need, source_needed_groups(cnt, ker.beg, ker.end, g.Ng, nang));
+                    const std::vector<int> nd = source_needed_groups(cnt, ker.beg, ker.end, g.Ng,
+                                                                     nang);
+                    upload(need, nd);
+                    std::vector<unsigned> bits((g.Ng + 31) / 32, 0u);
+                    for (int k = 0; k < g.Ng; ++k)
+                        if (nd[k]) bits[k / 32] |= 1u << (k % 32);
+                    upload(xmask64, bits);
                 }
             }
             BFPN_HIP(hipStreamSynchronize(s));
@@ -364,6 +371,7 @@ struct Physics final : PhysicsBase {
                factors.bytes() +
                su.bytes() + sv.bytes() + kv.bytes() + e1.bytes() + e2.bytes() + beg.bytes() +
                end.bytes() + qmap.bytes() + kcount.bytes() + klist.bytes() + need.bytes() +
+               xmask64.bytes() +
                plan.bytes() +
                angle_plan.h.bytes() +
                angle_plan.m.bytes() + angle_plan.l.bytes();
@@ -711,8 +719,17 @@ private:
         launch_energy_pair<T>(cur_[2 * species], cur_[2 * species + 1], P_->coef.template as<EnergyCoef>(),
                               Qa, Qb, grid.Ng, P_->lanes, phys.eq15_straggling != 0,
                               P_->yzw.template as<double>(), P_->nang, partials, proxy,
-                              X ? X[0] : nullptr, X ? X[1] : nullptr, s, tops);
+                              X ? X[0] : nullptr, X ? X[1] : nullptr, s, tops, x_rows());
         mark(kEnergy, array_bytes() * (4 + (Qa ? 2 : 0) + (X ? 2 : 0)), s);
+    }
+
+    // The groups of the end-of-step copy that the source reads (the others are
+    // never written): the source plan's list as bits.
+    const unsigned* x_rows() const {
+        if constexpr (std::is_same_v<T, float>)
+            return P_->plan.plan.amask;
+        else
+            return P_->xmask64.template as<unsigned>();
     }
 
     void transport(int which, bool preserve_sign, hipStream_t s) {
