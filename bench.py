@@ -41,7 +41,10 @@ Other workloads (--workload), each one JSON line:
             solves/s and arrival->IDD latency p50/p99
   accuracy  one case (--grid beam|paper, --material water|bone) to full depth
             with FP32 state and with the FP64 instantiation: IDD error above 10 %
-            of the maximum and BP/D90/D20 shifts (gate 0.5 %, 0.1 mm)
+            of the maximum and BP/D90/D20 shifts (gate 0.5 %, 0.1 mm); with
+            --slab A:B:MATERIAL[:RHO] the beam crosses that slab in water
+            (depth layers), the metrics are taken behind it and its D90 shift
+            against the homogeneous water run is reported
   copy      the plain device copy used to calibrate the traffic counters
 """
 from __future__ import annotations
@@ -421,18 +424,28 @@ def run_accuracy(args):
 
     import bfpn_gpu_amd as bg
 
+    from bfpn_gpu_amd.tables import material_dir
+
     ny, nx = (20, 2000) if args.grid == "beam" else (80, 4000)
     depth = args.depth or (40.0 if args.material == "water" else 22.0)
     case = beam_case(Ny=ny, Nx=nx, time_cm=depth, material=args.material)
     if args.material == "bone":
         case = dataclasses.replace(case, density=1.85)
+    layers, slab = None, None
+    if args.slab:  # A:B:MATERIAL[:RHO], in water
+        f = args.slab.split(":")
+        presets = {"water": 1.0, "bone": 1.85, "air": 0.001205}
+        slab = (float(f[0]), float(f[1]), f[2], float(f[3]) if len(f) > 3 else presets[f[2]])
+        water = ("water", 1.0, material_dir("water"))
+        layers = [(slab[0], *water), (slab[1], slab[2], slab[3], material_dir(slab[2])),
+                  (depth + 100.0, *water)]
     res = {}
     runs = (("fp32", bg.FP32), ("fp64", bg.FP64))
     if args.only:
         runs = tuple(r for r in runs if r[0] == args.only)
     for name, prec in runs:
         t0 = time.perf_counter()
-        with bg.Solver(case, prec) as s:
+        with bg.Solver(case, prec, layers=layers) as s:
             total = case.num_steps()
             done = 0
             while done < total:  # in chunks, with a progress line per chunk
@@ -444,8 +457,8 @@ def run_accuracy(args):
                     f"top groups {s.active_groups()}")
                 if args.save:  # the IDD so far, rewritten after every chunk
                     os.makedirs(args.save, exist_ok=True)
-                    np.save(os.path.join(args.save, f"idd_{args.grid}_{args.material}_{name}.npy"),
-                            idd)
+                    tag = f"{args.grid}_{args.material}" + ("_slab" if slab else "")
+                    np.save(os.path.join(args.save, f"idd_{tag}_{name}.npy"), idd)
         res[name] = (idd, time.perf_counter() - t0)
         log(f"{name}: {len(idd)} steps in {res[name][1]:.1f} s")
     from bfpn_gpu_amd import metrics
@@ -457,6 +470,8 @@ def run_accuracy(args):
                 "config": {"workload": f"{args.grid} grid ({ny + 1}x{ny + 1} nodes), "
                                        f"{args.material}, depth {depth} cm", "cells": case.cells},
                 "steps": len(idd), "seconds": sec, "metrics_cm": metrics.analyze(x, idd)}
+    if slab:
+        return slab_result(args, case, slab, res, ny, nx, depth)
     err = idd_error(res["fp32"][0], res["fp64"][0], case.dt)
 
     x = (np.arange(len(res["fp64"][0])) + 1) * case.dt
@@ -470,6 +485,63 @@ def run_accuracy(args):
             "seconds": {"fp32": res["fp32"][1], "fp64": res["fp64"][1]},
             "gate": {"max_rel_err": 0.005, "max_shift_mm": 0.1,
                      "pass": bool(err["max_rel"] <= 0.005 and err["max_shift_mm"] <= 0.1)}}
+
+
+def slab_result(args, case, slab, res, ny, nx, depth):
+    """FP32 vs FP64 through a slab, with the metrics behind it; the D90 shift
+    against the same beam in homogeneous water (an FP32 run)."""
+    import numpy as np
+
+    import bfpn_gpu_amd as bg
+    from bfpn_gpu_amd import metrics
+
+    a, b, mat, rho = slab
+    idd32, idd64 = res["fp32"][0], res["fp64"][0]
+    x = (np.arange(len(idd64)) + 1) * case.dt
+    behind = x > b  # the IDD scales by 1/density: the slab's own values are not comparable
+    c = metrics.compare(idd32[behind], idd64[behind])
+    m32, m64 = metrics.analyze(x[behind], idd32[behind]), metrics.analyze(x[behind], idd64[behind])
+    shift = 10 * max(abs(m32[k] - m64[k]) for k in ("BP", "D90", "D20"))
+    t0 = time.perf_counter()
+    with bg.Solver(case, bg.FP32) as s:
+        s.advance(case.num_steps())
+        water, _ = s.history()
+    mw = metrics.analyze(x, water)
+    log(f"homogeneous water: {time.perf_counter() - t0:.1f} s, D90 {mw['D90']:.4f} cm")
+    # CSDA prediction from the solver's own stopping powers: (b + R_w(E_out)) - R_w(E0)
+    Ng, Lg = case.Ng, case.Lg
+    E = np.concatenate(([1.0], (np.arange(1, Ng) - 0.5) * Lg / Ng + 1.0, [1.0 + Lg]))
+    Sw, Sm = bg.stopping_power("water", Ng, Lg), bg.stopping_power(mat, Ng, Lg)
+    e_fine = np.linspace(1.0, case.energy, 200001)
+    inv_w = 1.0 / np.interp(e_fine, E, Sw)
+    range_w = np.concatenate(([0.0], np.cumsum(0.5 * (inv_w[1:] + inv_w[:-1]) * np.diff(e_fine))))
+
+    def slow(e0, t, S, dens):  # energy after t cm (RK4 on dE/dx = -dens S(E))
+        h = t / 2000
+        e = e0
+        for _ in range(2000):
+            k1 = -dens * np.interp(e, E, S)
+            k2 = -dens * np.interp(e + 0.5 * h * k1, E, S)
+            k3 = -dens * np.interp(e + 0.5 * h * k2, E, S)
+            k4 = -dens * np.interp(e + h * k3, E, S)
+            e += h * (k1 + 2 * k2 + 2 * k3 + k4) / 6
+        return e
+
+    e_out = slow(slow(case.energy, a, Sw, 1.0), b - a, Sm, rho)
+    predicted = b + float(np.interp(e_out, e_fine, range_w)) - float(range_w[-1])
+    return {"metric": "IDD error of the FP32-state solver vs the FP64 instantiation through a slab",
+            "config": {"workload": f"{args.grid} grid ({ny + 1}x{ny + 1} nodes), water 230 MeV "
+                                   f"with a {mat} slab (rho {rho}) from {a} to {b} cm, depth "
+                                   f"{depth} cm, Nx {nx}, Ng 500, 20x20 directions, eq15 + "
+                                   f"straggling + secondary source", "cells": case.cells},
+            "max_rel_err_above_10pct_behind_slab": float(c["max_rel"]),
+            "max_shift_mm_BP_D90_D20": float(shift),
+            "fp64_metrics_cm_behind_slab": m64,
+            "d90_shift_vs_homogeneous_water_cm": float(m64["D90"] - mw["D90"]),
+            "d90_shift_csda_prediction_cm": predicted,
+            "seconds": {"fp32": res["fp32"][1], "fp64": res["fp64"][1]},
+            "gate": {"max_rel_err": 0.005, "max_shift_mm": 0.1,
+                     "pass": bool(c["max_rel"] <= 0.005 and shift <= 0.1)}}
 
 
 def compare_saved(args):
@@ -527,6 +599,8 @@ def main():
     ap.add_argument("--depth", type=float, default=None,
                     help="accuracy: depth in cm (default 40 water, 22 bone)")
     ap.add_argument("--save", default=None, help="directory for the accuracy IDD arrays")
+    ap.add_argument("--slab", default=None,
+                    help="accuracy: A:B:MATERIAL[:RHO], a slab from A to B cm in water")
     ap.add_argument("--only", choices=("fp32", "fp64"), default=None,
                     help="accuracy: run one precision and save its IDD (needs --save)")
     ap.add_argument("--compare", default=None,

@@ -25,5 +25,6 @@ from ._lib import (  # noqa: F401
     device_count,
     lib,
     lib_path,
+    stopping_power,
 )
 from . import metrics, ops, tables  # noqa: F401

@@ -120,6 +120,14 @@ def check(status: int) -> None:
         raise BfpnError(status, lib().bfpn_last_error().decode())
 
 
+def stopping_power(material: str, Ng: int, Lg: float) -> np.ndarray:
+    """The solver's stopping-power table S (MeV cm^2/g, Ng + 1 points:
+    E = 1, (i - 1/2) Lg/Ng + 1, 1 + Lg), computed on the host."""
+    S = np.zeros(Ng + 1)
+    check(lib().bfpn_stopping_power(material.encode(), Ng, Lg, _addr(S)))
+    return S
+
+
 def device_count() -> int:
     n = C.c_int(0)
     try:

@@ -296,6 +296,50 @@ def test_secondary_source_full_direction_grid(gpu, oracle_mod, full_directions, 
     _close(dQ.numpy().astype(np.float64), Q, TOL64 if dtype == np.float64 else TOL32["source"])
 
 
+def widest_source_window(beg, end, Ng):
+    """Most 16-group output tiles whose bands hold one 32-wide source step
+    (the band GEMM marches the source groups when this is at most 10)."""
+    tiles = []
+    for t in range((Ng + 63) // 64 * 4):
+        rows = [r for r in range(16 * t, min(16 * t + 16, Ng)) if end[r] > beg[r]]
+        tiles.append((min(beg[r] for r in rows), max(end[r] for r in rows)) if rows else None)
+    widest = 0
+    for k in range((Ng + 31) // 32):
+        live = [t for t, b in enumerate(tiles) if b and b[0] < 32 * k + 32 and b[1] > 32 * k]
+        if live:
+            widest = max(widest, max(live) - min(live) + 1)
+    return widest
+
+
+@pytest.mark.parametrize("Lg,energy,march", [(259.0, 230.0, True), (100.0, 90.0, False)],
+                         ids=["march", "per-tile"])
+def test_secondary_source_band_paths(gpu, oracle_mod, Lg, energy, march):
+    """Both FP32 band GEMMs against the oracle: on the default 500-group grid
+    every source step's window of output tiles fits the march (<= 10 tiles);
+    500 groups on [1, 101] MeV widen the bands in groups so far that it does
+    not, and the per-tile kernel runs.  Signed random inputs, 9 x 9 nodes."""
+    c = small_case(Ny=8, Nz=8, Nmu=8, Nom=8, Lg=Lg, energy=energy)
+    s = oracle_mod.Solver(c)
+    t = {k: s.table(k) for k in ("ker_e1", "ker_e2", "ker_v")}
+    beg = s.table("ker_beg").astype(np.int32)
+    end = s.table("ker_end").astype(np.int32)
+    s.close()
+    assert (widest_source_window(beg, end, c.Ng) <= 10) == march
+    d = oracle_mod.deltas(c)
+    rng = np.random.default_rng(11)
+    Fp = rng.standard_normal(c.cells).astype(np.float32).astype(np.float64)
+    fp = rng.standard_normal(c.cells).astype(np.float32).astype(np.float64)
+    Q = oracle_mod.secondary_source_factorized(Fp, fp, t["ker_e1"], t["ker_e2"], t["ker_v"],
+                                               beg, end, c.Ng, c.nyz, c.nang, d.dg)
+    assert np.abs(Q).max() > 0
+    dQ = gpu.DeviceArray(shape=c.cells, dtype=np.float32)
+    gpu.ops.secondary_source(_dev(gpu, Fp, np.float32), _dev(gpu, fp, np.float32), dQ,
+                             _dev(gpu, t["ker_e1"], np.float64), _dev(gpu, t["ker_e2"], np.float64),
+                             _dev(gpu, t["ker_v"], np.float64), _dev(gpu, beg, np.int32),
+                             _dev(gpu, end, np.int32), c.Ng, c.nyz, c.nang, d.dg)
+    _close(dQ.numpy().astype(np.float64), Q, TOL32["source"])
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("trap,cat", [(True, True), (False, False)])
 def test_idd(gpu, oracle_mod, physical, dtype, trap, cat):

@@ -79,7 +79,9 @@ def csda_shift(oracle_mod, case, slab, material, density):
     (b + R_w(E_out)) - R_w(E0), with E_out the energy leaving the slab."""
     w = oracle_mod.Solver(case)
     m = oracle_mod.Solver(line_case(material=material, data=material_dir(material)))
-    en, Sw, Sm = w.table("en"), w.table("S"), m.table("S")
+    Sw, Sm = w.table("S"), m.table("S")
+    Ng, Lg = case.Ng, case.Lg  # the table's points: 1, (i - 1/2) dg + 1, 1 + Lg
+    en = np.concatenate(([1.0], (np.arange(1, Ng) - 0.5) * Lg / Ng + 1.0, [1.0 + Lg]))
 
     def slow(E, t, S, rho):
         f = lambda x, e: [-rho * np.interp(e[0], en, S)]  # noqa: E731
