@@ -70,6 +70,14 @@ __device__ inline T limited_slope(T a, T b) {
     return fabs(a) < T(1e-10) ? T(0) : superbee(a, b);
 }
 
+// z flux term of a cell in the lane's upwind frame (see divergence):
+// hz (S_c - S_up) + (c - z_up) from the backward differences for wz >= 0 and
+// the negated forward ones otherwise.
+template <typename T>
+__device__ inline T z_term(T dz_b, T dz_f, T dS_b, T dS_f, T hz, bool pz) {
+    return fma(hz, pz ? dS_b : -dS_f, pz ? dz_b : -dz_f);
+}
+
 // Flux divergence at one cell, per axis written in the lane's upwind frame:
 //   w (F(+1/2) - F(-1/2)) / d = (|w| / d) (downwind face - upwind face),
 // where the downwind face is the cell's own reconstruction c + s_down/2 and
@@ -85,11 +93,14 @@ __device__ inline T limited_slope(T a, T b) {
 // forms it once); the upwind terms zc - z_up and sz_0 - sz_up are dz_b, dS_b
 // for wz >= 0 and -dz_f, -dS_f (exact negations) otherwise.  hz = +-1/2 with
 // the sign of wz; cy = |wy| / dy, cz = |wz| / dz.
+// The z term of a row for wz < 0, fma(hz, -dS_f, -dz_f), is the negation of
+// fma(hz, dS_b, dz_b) of the next row (its backward differences are this
+// row's forward ones; rounding to nearest is odd), so the march forms
+// U = fma(hz, dS_f, dz_f) once per row, for the next row's wz >= 0 lanes and
+// this row's other lanes, and selects between U(row) and -U(row + 1).
 template <typename T>
-__device__ inline T divergence(T zc, T near, T S_c, T S_n, T dz_b, T dz_f, T dS_b, T dS_f, T cy,
-                               T cz, T hy, T hz, bool pz) {
+__device__ inline T divergence(T zc, T near, T S_c, T S_n, T fz, T cy, T cz, T hy) {
     const T fy = fma(hy, S_c - S_n, zc - near);
-    const T fz = fma(hz, pz ? dS_b : -dS_f, pz ? dz_b : -dz_f);
     return cy * fy + cz * fz;
 }
 
@@ -171,6 +182,7 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
     T wSp[P][2];    // own y slope of psi*, rows t-2, t-1 after step t's slope pass
     T sz[P][2];     // z slopes of psi, rows t-1 and t before step t
     T szp[P][2];    // z slopes of psi*, rows t-3 and t-2 before step t
+    T uz[P], uzp[P];  // z term U of the row the predictor / corrector works on next
 #pragma unroll
     for (int j = 0; j < P; ++j) {
         const int y = ry + j * R;
@@ -194,6 +206,7 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
             win[j][r] = bload<T>(Fg, boff[j], static_cast<unsigned>(min(r, Nz)) * zbytes);
 #pragma unroll
         for (int k = 0; k < 2; ++k) sz[j][k] = szp[j][k] = wSp[j][k] = T(0);  // boundary: 0
+        uz[j] = uzp[j] = T(0);  // (edge steps form their own z terms)
 #pragma unroll
         for (int k = 0; k < 4; ++k) wpred[j][k] = wbase[j][k] = wS[j][k] = T(0);
     }
@@ -286,8 +299,11 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
                 const T dz_b = zc - zm, dz_f = zp - zc;
                 const T s_next = limited_slope(win[j][(k8 + 2) & 7] - zp, dz_f);
                 const T sz_m = sz[j][(k + 1) & 1], sz_0 = sz[j][k & 1];
-                const T L0 = divergence(zc, pn[j], wS[j][k], pS[j], dz_b, dz_f, sz_0 - sz_m,
-                                        s_next - sz_0, cy, cz, hy, hz, pz);
+                const T dS_f = s_next - sz_0;
+                const T un = fma(hz, dS_f, dz_f);  // U of row t+1
+                const T fz = E ? z_term(dz_b, dz_f, sz_0 - sz_m, dS_f, hz, pz) : (pz ? uz[j] : -un);
+                uz[j] = un;
+                const T L0 = divergence(zc, pn[j], wS[j][k], pS[j], fz, cy, cz, hy);
                 T v = zc - h * L0;
                 if (CLIP) v = fmax(v, T(0));
                 prow[o0[j]] = v;
@@ -307,12 +323,18 @@ __global__ __launch_bounds__(max_threads<T>()) void transport_march_kernel(
                 const T dz_b = zc - zm, dz_f = zp - zc;
                 const T s_next = limited_slope(wpred[j][k] - zp, dz_f);
                 const T sz_m = szp[j][(k + 1) & 1], sz_0 = szp[j][k & 1];
-                const T L1 = divergence(zc, cn_[j], wSp[j][k & 1], cS[j], dz_b, dz_f, sz_0 - sz_m,
-                                        s_next - sz_0, cy, cz, hy, hz, pz);
+                const T dS_f = s_next - sz_0;
+                const T un = fma(hz, dS_f, dz_f);  // U of row t-1
+                const T fz = E ? z_term(dz_b, dz_f, sz_0 - sz_m, dS_f, hz, pz) : (pz ? uzp[j] : -un);
+                const T L1 = divergence(zc, cn_[j], wSp[j][k & 1], cS[j], fz, cy, cz, hy);
                 T v = wbase[j][(k + 2) & 3] - hh * L1;
                 if (CLIP) v = fmax(v, T(0));
                 if (real) bstore(v, Fg, soff[j], so);
-                szp[j][(k + 1) & 1] = (!E || t >= 2) ? s_next : T(0);  // rows <= 0: boundary
+                const T s_kept = (!E || t >= 2) ? s_next : T(0);  // rows <= 0: boundary
+                szp[j][(k + 1) & 1] = s_kept;
+                // the next row's U from the slope kept for it (at edge steps the
+                // kept slope may be the boundary's 0)
+                uzp[j] = E ? fma(hz, s_kept - sz_0, dz_f) : un;
             }
         }
     };
