@@ -71,7 +71,13 @@ __device__ inline double flush_tiny(double v) {
 // state with top *x (all groups when the tops are not tracked).
 __device__ inline int state_top(const SweepTops& t, int Ng) { return t.in ? *t.in : Ng - 1; }
 __device__ inline int source_top(const SweepTops& t, const int* x, int Ng) {
-    return (t.qmap && x) ? t.qmap[max(-1, min(Ng - 1, *x)) + 1] : Ng - 1;
+    const int top = (t.qmap && x) ? t.qmap[max(-1, min(Ng - 1, *x)) + 1] : Ng - 1;
+    return max(-1, min(top, t.q_cap));
+}
+// Source row read at group g: rows above the source's top are masked where
+// used, so their loads re-read the top row (an L2 hit) instead of HBM.
+__device__ inline long long source_row(int g, int q_top) {
+    return static_cast<long long>(max(min(g, q_top), 0));
 }
 
 // Presets the slots a later sweep writes (one thread of the grid).
@@ -241,7 +247,7 @@ __global__ __launch_bounds__(kBlock) void energy_half_kernel(
             const long long base = static_cast<long long>(max(g, 0)) * lanes;
             rF[k] = load_pack<T, LPT>(F + base + lane);
             rf[k] = load_pack<T, LPT>(f + base + lane);
-            if (SRC) rQ[k] = load_pack<T, LPT>(Q1 + base + lane);
+            if (SRC) rQ[k] = load_pack<T, LPT>(Q1 + source_row(g, q_top) * lanes + lane);
         };
 #pragma unroll
         for (int k = 0; k < D; ++k) fetch(k, G0 - k);
@@ -356,10 +362,10 @@ __global__ __launch_bounds__(kBlock) void energy_pair_kernel(
             const long long base = static_cast<long long>(max(g, 0)) * lanes;
             rF[k] = load_pack<T, LPT>(F + base + lane);
             rf[k] = load_pack<T, LPT>(f + base + lane);
-            if (SRC) rQa[k] = load_pack<T, LPT>(Qa + base + lane);
+            if (SRC) rQa[k] = load_pack<T, LPT>(Qa + source_row(g, qa_top) * lanes + lane);
         };
         auto fetch_qb = [&](int k, int g) {
-            if (SRC) rQb[k] = load_pack<T, LPT>(Qb + static_cast<long long>(max(g, 0)) * lanes + lane);
+            if (SRC) rQb[k] = load_pack<T, LPT>(Qb + source_row(g, qb_top) * lanes + lane);
         };
 #pragma unroll
         for (int k = 0; k < D; ++k) {

@@ -26,6 +26,9 @@ struct SweepTops {
     int* out_x = nullptr;       // paired: top of the state between the halves; single: copy of out
     int* reset = nullptr;       // set to -1 by the sweep (the slot a later sweep writes)
     int* reset_x = nullptr;
+    // Source rows above q_cap are zero whatever the state (the output groups
+    // with no band): they are neither read nor, by the context, written.
+    int q_cap = 1 << 30;
 };
 
 // Eq.15 energy half step, in place.  coef: Ng device EnergyCoef.  Q1 may be
@@ -108,10 +111,12 @@ int source_ktiles(int nang);  // 32-wide K tiles
 // reader of Fp and fp, has been issued.  xtop/qmap (optional, device; both or
 // neither): the primary's top group and the source-top map of SweepTops;
 // rows of Q above the source's top are then left unwritten.
+// q_rows (optional, >= 0): rows of Q above it need not be written (the
+// caller never reads them: they hold no source).
 void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, float* Q,
                         const SourcePlan& plan, int Ng, int nyz, int nang, hipStream_t s,
                         hipEvent_t inputs_read = nullptr, const int* xtop = nullptr,
-                        const int* qmap = nullptr);
+                        const int* qmap = nullptr, int q_rows = -1);
 
 // Secondary source (FP64 vector path): A1/A2 angular contraction, banded sum.
 // kcount/klist (optional, device; from source_angular_blocks): per (g,

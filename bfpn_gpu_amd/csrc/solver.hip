@@ -282,6 +282,7 @@ struct Physics final : PhysicsBase {
     DeviceBuffer xmask64;              // FP64: the groups read, as bits (FP32: plan.amask)
     bool legacy = false, lite = false;  // upwind energy model; lite-memory primary step
     double density = 1.0;               // g/cm^3: the IDD and spot-plane scale
+    int qcap = -1;                      // highest output group with a source band (-1: none)
     SourcePlanBuffers plan;
     AnglePlanBuffers angle_plan;
 
@@ -335,6 +336,8 @@ struct Physics final : PhysicsBase {
                 AnglePlanBuffers::usable(g.Nmu, g.Nom, static_cast<long long>(g.Ng) * nyz))
                 angle_plan.build(g.Nmu, g.Nom, s);
             if (secondary) {
+                for (int r = 0; r < g.Ng; ++r)
+                    if (ker.end[r] > ker.beg[r]) qcap = r;
                 upload(qmap, source_top_map(ker, g.Ng));
                 upload(kv, cast_vector<T>(ker.v));
                 if constexpr (std::is_same_v<T, float>) {
@@ -621,6 +624,7 @@ private:
     // step x_out (-1: none).  Empty when skipping is off.
     SweepTops sweep_tops(int species, int xa, int xb, int x_out) {
         SweepTops t;
+        t.q_cap = P_->qcap;  // static: rows above hold no source
         if (!skip_) return t;
         int& cur = species == 0 ? p_cur_ : s_cur_;
         int* ring = top_slot(3 * species);
@@ -764,7 +768,7 @@ private:
         const int* qm = skip_ ? P_->qmap.template as<int>() : nullptr;
         if constexpr (std::is_same_v<T, float>) {
             launch_source_mfma(Fp, fp, a1_.as<float>(), a2_.as<float>(), Q, P_->plan.plan,
-                               grid.Ng, P_->nyz, P_->nang, s, inputs_read, xt, qm);
+                               grid.Ng, P_->nyz, P_->nang, s, inputs_read, xt, qm, P_->qcap);
         } else {
             const int* need = P_->need.template as<int>();
             launch_source_angular<T>(Fp, fp, P_->kv.template as<T>(), a1_.as<T>(), a2_.as<T>(),

@@ -373,7 +373,7 @@ __global__ __launch_bounds__(512, 4) void source_band_march(
     const __bf16* __restrict__ e1l, const __bf16* __restrict__ e2h, const __bf16* __restrict__ e2l,
     const int2* __restrict__ tile_band, const int2* __restrict__ steps, int s_first, int s_last,
     const unsigned* __restrict__ amask, float* __restrict__ Q, int Ng, int ngp, long long lanes,
-    int lblocks, const int* __restrict__ xtop, const int* __restrict__ qmap) {
+    int lblocks, const int* __restrict__ xtop, const int* __restrict__ qmap, int q_rows) {
     constexpr int W = kMarchSlots;
     __shared__ __attribute__((aligned(16))) float as[2][32][kMarchLanes];  // A1, A2
     __shared__ bf16x8 es[W * 256];  // [slot][array][q][row] 16-byte chunks
@@ -389,6 +389,7 @@ __global__ __launch_bounds__(512, 4) void source_band_march(
         kmax = max(-1, min(Ng - 1, *xtop));
         gmax = qmap[kmax + 1];
     }
+    if (q_rows >= 0) gmax = min(gmax, q_rows);  // rows above are never read
     const int s_end = min(s_last, kmax < 0 ? 0 : kmax / 32 + 1);  // exclusive
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = lane & 15, q = lane >> 4;
@@ -786,7 +787,7 @@ void launch_split(const float* src, __bf16* hi, __bf16* lo, long long n, hipStre
 
 void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, float* Q,
                         const SourcePlan& plan, int Ng, int nyz, int nang, hipStream_t s,
-                        hipEvent_t inputs_read, const int* xtop, const int* qmap) {
+                        hipEvent_t inputs_read, const int* xtop, const int* qmap, int q_rows) {
     BFPN_REQUIRE((xtop == nullptr) == (qmap == nullptr), "source tops need both xtop and qmap");
     const int nap = source_nap(nang);
     // tiles over the source groups whose A rows the band GEMM reads
@@ -817,12 +818,14 @@ void launch_source_mfma(const float* Fp, const float* fp, float* A1, float* A2, 
             hipLaunchKernelGGL(source_band_march<true>, dim3(gm), dim3(512), 0, s, A1, A2,
                                plan.e1h, plan.e1l, plan.e2h, plan.e2l, plan.tile_band,
                                plan.march_steps, plan.march_first, plan.march_last, plan.amask, Q,
-                               Ng, source_ngp(Ng), lanes, static_cast<int>(lb), xtop, qmap);
+                               Ng, source_ngp(Ng), lanes, static_cast<int>(lb), xtop, qmap,
+                               q_rows);
         else
             hipLaunchKernelGGL(source_band_march<false>, dim3(gm), dim3(512), 0, s, A1, A2,
                                plan.e1h, plan.e1l, plan.e2h, plan.e2l, plan.tile_band,
                                plan.march_steps, plan.march_first, plan.march_last, plan.amask, Q,
-                               Ng, source_ngp(Ng), lanes, static_cast<int>(lb), xtop, qmap);
+                               Ng, source_ngp(Ng), lanes, static_cast<int>(lb), xtop, qmap,
+                               q_rows);
         BFPN_HIP(hipGetLastError());
         return;
     }
