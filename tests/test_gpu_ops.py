@@ -311,14 +311,17 @@ def widest_source_window(beg, end, Ng):
     return widest
 
 
+@pytest.mark.parametrize("dirs", [(8, 8), (5, 3)], ids=["16-byte", "per-element"])
 @pytest.mark.parametrize("Lg,energy,march", [(259.0, 230.0, True), (100.0, 90.0, False)],
                          ids=["march", "per-tile"])
-def test_secondary_source_band_paths(gpu, oracle_mod, Lg, energy, march):
+def test_secondary_source_band_paths(gpu, oracle_mod, Lg, energy, march, dirs):
     """Both FP32 band GEMMs against the oracle: on the default 500-group grid
     every source step's window of output tiles fits the march (<= 10 tiles);
     500 groups on [1, 101] MeV widen the bands in groups so far that it does
-    not, and the per-tile kernel runs.  Signed random inputs, 9 x 9 nodes."""
-    c = small_case(Ny=8, Nz=8, Nmu=8, Nom=8, Lg=Lg, energy=energy)
+    not, and the per-tile kernel runs.  Signed random inputs, 9 x 9 nodes;
+    8 x 8 directions give 5184 lanes (16-byte loads), 5 x 3 give 1215 (an
+    odd lane count: per-element loads in both GEMMs)."""
+    c = small_case(Ny=8, Nz=8, Nmu=dirs[0], Nom=dirs[1], Lg=Lg, energy=energy)
     s = oracle_mod.Solver(c)
     t = {k: s.table(k) for k in ("ker_e1", "ker_e2", "ker_v")}
     beg = s.table("ker_beg").astype(np.int32)
