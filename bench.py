@@ -301,7 +301,7 @@ def run_beamlets(args):
     t_skip, plain_skip = solve_each(cases, skip=True)
     log(f"one at a time, active groups only: {t_skip:.2f} s")
     t0 = time.perf_counter()
-    with bg.Server(max_active=args.max_active, slice_steps=args.slice_steps) as srv:
+    with bg.Server(max_active=args.max_active, slice_steps=args.slice_steps or 32) as srv:
         ids = [srv.submit(c) for c in cases]
         served = []
         for k, i in enumerate(ids):
@@ -323,7 +323,7 @@ def run_beamlets(args):
                                   f"20x20 directions, Ng 300, water/bone, depth 1.2 x CSDA range, "
                                   f"dt 0.02 cm)", "depth_steps": steps,
                       "cell_steps": cell_steps, "max_active": args.max_active,
-                      "slice_steps": args.slice_steps},
+                      "slice_steps": args.slice_steps or 32},
            "one_at_a_time_every_group": {"value": len(cases) / t_full, "seconds": t_full},
            "one_at_a_time_active_groups": {"value": len(cases) / t_skip, "seconds": t_skip},
            "server": {"seconds": t_srv, "peak_active": st["peak_active"],
@@ -356,7 +356,7 @@ def run_serve(args):
         ", ".join(f"{c.energy:.0f}MeV/{c.Ny}/{c.material}" for _, c in reqs))
     lat = [None] * len(reqs)
     served = [None] * len(reqs)
-    with bg.Server(max_active=args.max_active, slice_steps=args.slice_steps) as srv:
+    with bg.Server(max_active=args.max_active, slice_steps=args.slice_steps or 8) as srv:
         t0 = time.perf_counter()
         ids = [None] * len(reqs)
         late = [0.0] * len(reqs)  # submit time past the scheduled arrival
@@ -391,7 +391,7 @@ def run_serve(args):
                                   f"arrivals at {args.rate}/s over 60 s (E ~ U[70,230] MeV, "
                                   f"21x21/41x41/81x81 nodes, 20x20 directions, Ng 300, water/bone,"
                                   f" full depth 1.2 x CSDA range)",
-                      "max_active": args.max_active, "slice_steps": args.slice_steps,
+                      "max_active": args.max_active, "slice_steps": args.slice_steps or 8,
                       "requests": len(reqs), "offered_requests": len(stream)},
            "latency_s": {"p50": float(np.percentile(lat, 50)),
                          "p99": float(np.percentile(lat, 99)), "max": float(lat.max()),
@@ -620,7 +620,9 @@ def main():
     ap.add_argument("--fp64-max-ny", type=int, default=40,
                     help="serve: FP64 reruns of the served results up to this Ny")
     ap.add_argument("--max-active", type=int, default=8)
-    ap.add_argument("--slice-steps", type=int, default=8)
+    ap.add_argument("--slice-steps", type=int, default=None,
+                    help="server: depth steps per beam per turn (default 32 for beamlets, where "
+                         "throughput counts, 8 for serve, where a new arrival's first turn does)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
