@@ -105,6 +105,8 @@ def lib():
     L.bfpn_server_create.argtypes = [C.POINTER(ServerOpts), C.POINTER(vp)]
     L.bfpn_server_submit.argtypes = [vp, C.POINTER(Grid), C.POINTER(Phys), C.c_char_p, d,
                                      C.POINTER(C.c_uint64)]
+    L.bfpn_server_submit_layered.argtypes = [vp, C.POINTER(Grid), C.POINTER(Phys),
+                                            C.POINTER(Layer), i, d, C.POINTER(C.c_uint64)]
     L.bfpn_server_poll.argtypes = [vp, C.c_uint64]
     L.bfpn_server_wait.argtypes = [vp, C.c_uint64, d, vp, i, C.POINTER(C.c_int), vp]
     L.bfpn_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
@@ -342,11 +344,20 @@ class Server:
         check(lib().bfpn_server_create(C.byref(opts), C.byref(h)))
         self._h = h
 
-    def submit(self, case: Case, t_final: float | None = None) -> int:
+    def submit(self, case: Case, t_final: float | None = None, layers=None) -> int:
+        """Queues one beam; layers as in Solver (bfpn_server_submit_layered)."""
         g, p = case.grid(), case.phys()
         rid = C.c_uint64(0)
-        check(lib().bfpn_server_submit(self._h, C.byref(g), C.byref(p), case.data.encode(),
-                                       case.time if t_final is None else t_final, C.byref(rid)))
+        t = case.time if t_final is None else t_final
+        if layers:
+            n = len(layers)
+            spec = (Layer * n)(*[Layer(float(x), m.encode(), float(rho), path.encode())
+                                 for x, m, rho, path in layers])
+            check(lib().bfpn_server_submit_layered(self._h, C.byref(g), C.byref(p), spec, n, t,
+                                                   C.byref(rid)))
+        else:
+            check(lib().bfpn_server_submit(self._h, C.byref(g), C.byref(p), case.data.encode(),
+                                           t, C.byref(rid)))
         return rid.value
 
     def poll(self, rid: int) -> int:

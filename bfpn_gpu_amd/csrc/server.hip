@@ -56,8 +56,9 @@ struct Request {
     int steps = 0;
     size_t ctx_bytes = 0;
     Clock::time_point t_submit, t_admit, t_done;
+    std::vector<LayerSpec> layers;  // empty: one material (data)
     std::unique_ptr<bfpn_ctx> ctx;
-    std::shared_ptr<PhysicsBase> physics;
+    std::vector<std::shared_ptr<PhysicsBase>> physics;  // one table set per layer
     int issued = 0;
     hipEvent_t slice_done = nullptr;
     bool in_flight = false;
@@ -103,12 +104,13 @@ struct bfpn_server {
     }
 
     // Frees cached physics tables that no admitted beam uses, oldest first,
-    // until `need` more bytes fit; the entry `keep` (the tables of the request
-    // being admitted) is never freed.  Called with the lock held.
-    void evict_for(size_t need, const std::string& keep) {
+    // until `need` more bytes fit; the entries in `keep` (the tables of the
+    // request being admitted) are never freed.  Called with the lock held.
+    void evict_for(size_t need, const std::vector<std::string>& keep) {
         for (auto it = lru.rbegin(); it != lru.rend() && used + need > opts.hbm_budget_bytes;) {
             auto c = cache.find(*it);
-            if (c != cache.end() && *it != keep && c->second.physics.use_count() == 1) {
+            const bool kept = std::find(keep.begin(), keep.end(), *it) != keep.end();
+            if (c != cache.end() && !kept && c->second.physics.use_count() == 1) {
                 used -= c->second.bytes;
                 cache.erase(c);
                 it = std::list<std::string>::reverse_iterator(lru.erase(std::next(it).base()));
@@ -148,12 +150,37 @@ struct bfpn_server {
                                              return total_cells(*a) < total_cells(*b);
                                          });
             auto req = *pick;
-            const std::string key = physics_key(req->grid, req->phys, req->data, opts.precision);
-            const auto hit = cache.find(key);
-            const bool cached = hit != cache.end();
-            const size_t phys_need = cached ? 0 : physics_bytes(req->grid, req->phys, opts.precision);
+            // the request's table sets: one per layer (one for a single material)
+            struct Tables {
+                std::string key, data;
+                bfpn_phys phys;
+                std::shared_ptr<PhysicsBase> physics;  // cached, or built below
+                size_t estimate = 0;                   // bytes charged when not cached
+            };
+            std::vector<Tables> tables;
+            if (req->layers.empty()) {
+                tables.push_back({physics_key(req->grid, req->phys, req->data, opts.precision),
+                                  req->data, req->phys, nullptr, 0});
+            } else {
+                for (const LayerSpec& l : req->layers) {
+                    const bfpn_phys q = layer_phys(req->phys, l);
+                    tables.push_back({physics_key(req->grid, q, l.data_path, opts.precision),
+                                      l.data_path, q, nullptr, 0});
+                }
+            }
+            std::vector<std::string> keys;
+            size_t phys_need = 0;
+            for (Tables& t : tables) {
+                keys.push_back(t.key);
+                const bool repeat = std::count(keys.begin(), keys.end(), t.key) > 1;
+                const auto hit = cache.find(t.key);
+                if (hit == cache.end() && !repeat) {
+                    t.estimate = physics_bytes(req->grid, t.phys, opts.precision);
+                    phys_need += t.estimate;
+                }
+            }
             const size_t need = req->ctx_bytes + phys_need;
-            if (used + need > opts.hbm_budget_bytes) evict_for(need, key);
+            if (used + need > opts.hbm_budget_bytes) evict_for(need, keys);
             if (used + need > opts.hbm_budget_bytes) {
                 if (admitted() > 0) {  // wait for a running beam to finish
                     cv_prep.wait(lk);
@@ -169,18 +196,25 @@ struct bfpn_server {
             pending.erase(pick);
             used += need;
             ++preparing;
-            std::shared_ptr<PhysicsBase> physics;
-            if (cached) physics = hit->second.physics;  // evict_for kept this entry
+            for (Tables& t : tables) {  // evict_for kept these entries
+                const auto hit = cache.find(t.key);
+                if (hit != cache.end()) t.physics = hit->second.physics;
+            }
             lk.unlock();
-            size_t physics_real = 0;
             std::string err;
             try {
-                if (!physics) {
-                    physics = make_physics(req->grid, req->phys, req->data, opts.precision);
-                    physics_real = physics->device_bytes();
+                for (size_t k = 0; k < tables.size(); ++k) {
+                    Tables& t = tables[k];
+                    for (size_t j = 0; j < k && !t.physics; ++j)  // a material repeated in depth
+                        if (tables[j].key == t.key) t.physics = tables[j].physics;
+                    if (!t.physics) t.physics = make_physics(req->grid, t.phys, t.data, opts.precision);
+                    req->physics.push_back(t.physics);
                 }
-                req->ctx.reset(make_context(physics, req->grid, req->phys));
-                req->physics = physics;
+                if (req->layers.empty())
+                    req->ctx.reset(make_context(req->physics.front(), req->grid, req->phys));
+                else
+                    req->ctx.reset(make_layered_context(req->grid, req->phys, req->layers,
+                                                        req->physics));
                 BFPN_HIP(hipEventCreateWithFlags(&req->slice_done, hipEventDisableTiming));
             } catch (const std::exception& e) {
                 err = e.what();
@@ -190,17 +224,19 @@ struct bfpn_server {
             if (!err.empty()) {
                 used -= need;
                 req->ctx.reset();
-                req->physics.reset();
+                req->physics.clear();
                 fail_locked(req, err);
                 cv_work.notify_one();
                 continue;
             }
-            if (phys_need) {  // keep the larger of estimate and real size
-                const size_t charged = std::max(phys_need, physics_real);
-                used = used - phys_need + charged;
-                cache[key] = CacheEntry{physics, charged};
+            for (Tables& t : tables) {
+                if (t.estimate) {  // newly built: keep the larger of estimate and real size
+                    const size_t charged = std::max(t.estimate, t.physics->device_bytes());
+                    used = used - t.estimate + charged;
+                    cache[t.key] = CacheEntry{t.physics, charged};
+                }
+                touch(t.key);
             }
-            touch(key);
             req->status = 1;
             req->t_admit = Clock::now();
             ready.push_back(req);
@@ -233,7 +269,7 @@ struct bfpn_server {
         (void)hipEventDestroy(r->slice_done);
         r->slice_done = nullptr;
         r->ctx.reset();
-        r->physics.reset();
+        r->physics.clear();
         std::lock_guard<std::mutex> g(mu);
         r->idd.swap(idd);
         r->status = 2;
@@ -249,7 +285,7 @@ struct bfpn_server {
         if (r->slice_done) (void)hipEventDestroy(r->slice_done);
         r->slice_done = nullptr;
         r->ctx.reset();
-        r->physics.reset();
+        r->physics.clear();
         std::lock_guard<std::mutex> g(mu);
         used -= r->ctx_bytes;
         fail_locked(r, msg);
@@ -359,28 +395,52 @@ BFPN_API bfpn_status bfpn_server_create(const bfpn_server_opts* opts, bfpn_serve
     });
 }
 
+namespace {
+// Queues a request (layers empty: one material, tables from data_path).
+void submit_request(bfpn_server* s, const bfpn_grid* grid, const bfpn_phys* phys,
+                    const char* data_path, std::vector<LayerSpec> layers, double t_final,
+                    uint64_t* id) {
+    BFPN_REQUIRE(s && grid && phys && id, "null argument");
+    validate_params(*grid, *phys);
+    if (!layers.empty()) validate_layers(layers);
+    auto r = std::make_shared<Request>();
+    r->grid = *grid;
+    r->phys = *phys;
+    r->data = data_path ? data_path : "./data";
+    r->layers = std::move(layers);
+    const double dt = grid->Lx / grid->Nx;
+    r->steps = static_cast<int>(std::ceil(t_final / dt - 1e-12));
+    BFPN_REQUIRE(r->steps >= 1, "t_final must cover at least one depth step");
+    r->ctx_bytes = context_bytes(*grid, *phys, s->opts.precision);
+    r->t_submit = Clock::now();
+    std::lock_guard<std::mutex> g(s->mu);
+    BFPN_REQUIRE(!s->stopping, "server is shutting down");
+    r->id = s->next_id++;
+    s->pending.push_back(r);
+    ++s->stats.submitted;
+    *id = r->id;
+    s->cv_prep.notify_one();
+}
+}  // namespace
+
+BFPN_API bfpn_status bfpn_server_submit_layered(bfpn_server* s, const bfpn_grid* grid,
+                                                const bfpn_phys* phys, const bfpn_layer* layers,
+                                                int nlayers, double t_final, uint64_t* id) {
+    return guarded([&] {
+        BFPN_REQUIRE(layers && nlayers > 0, "null argument");
+        std::vector<LayerSpec> spec;
+        for (int k = 0; k < nlayers; ++k) {
+            BFPN_REQUIRE(layers[k].material && layers[k].data_path, "null layer field");
+            spec.push_back({layers[k].x_end, layers[k].material, layers[k].density,
+                            layers[k].data_path});
+        }
+        submit_request(s, grid, phys, nullptr, std::move(spec), t_final, id);
+    });
+}
+
 BFPN_API bfpn_status bfpn_server_submit(bfpn_server* s, const bfpn_grid* grid, const bfpn_phys* phys,
                                         const char* data_path, double t_final, uint64_t* id) {
-    return guarded([&] {
-        BFPN_REQUIRE(s && grid && phys && id, "null argument");
-        validate_params(*grid, *phys);
-        auto r = std::make_shared<Request>();
-        r->grid = *grid;
-        r->phys = *phys;
-        r->data = data_path ? data_path : "./data";
-        const double dt = grid->Lx / grid->Nx;
-        r->steps = static_cast<int>(std::ceil(t_final / dt - 1e-12));
-        BFPN_REQUIRE(r->steps >= 1, "t_final must cover at least one depth step");
-        r->ctx_bytes = context_bytes(*grid, *phys, s->opts.precision);
-        r->t_submit = Clock::now();
-        std::lock_guard<std::mutex> g(s->mu);
-        BFPN_REQUIRE(!s->stopping, "server is shutting down");
-        r->id = s->next_id++;
-        s->pending.push_back(r);
-        ++s->stats.submitted;
-        *id = r->id;
-        s->cv_prep.notify_one();
-    });
+    return guarded([&] { submit_request(s, grid, phys, data_path, {}, t_final, id); });
 }
 
 BFPN_API int bfpn_server_poll(bfpn_server* s, uint64_t id) {

@@ -965,17 +965,32 @@ void validate_layers(const std::vector<LayerSpec>& layers) {
     }
 }
 
+bfpn_phys layer_phys(const bfpn_phys& p, const LayerSpec& l) {
+    bfpn_phys q = p;
+    std::snprintf(q.material_name, sizeof(q.material_name), "%s", l.material.c_str());
+    q.density = q.energy_density = l.density;
+    return q;
+}
+
 bfpn_ctx* make_layered_context(const bfpn_grid& g, const bfpn_phys& p,
                                const std::vector<LayerSpec>& layers, int precision) {
     validate_layers(layers);
     std::vector<std::shared_ptr<PhysicsBase>> tables;
+    for (const LayerSpec& l : layers)
+        tables.push_back(make_physics(g, layer_phys(p, l), l.data_path, precision));
+    return make_layered_context(g, p, layers, tables);
+}
+
+bfpn_ctx* make_layered_context(const bfpn_grid& g, const bfpn_phys& p,
+                               const std::vector<LayerSpec>& layers,
+                               const std::vector<std::shared_ptr<PhysicsBase>>& tables) {
+    validate_layers(layers);
+    BFPN_REQUIRE(tables.size() == layers.size() && !tables.empty(), "one table set per layer");
+    const int precision = tables.front()->precision;
     std::vector<double> x_end;
-    for (const LayerSpec& l : layers) {
-        bfpn_phys q = p;
-        std::snprintf(q.material_name, sizeof(q.material_name), "%s", l.material.c_str());
-        q.density = q.energy_density = l.density;
-        tables.push_back(make_physics(g, q, l.data_path, precision));
-        x_end.push_back(l.x_end);
+    for (size_t k = 0; k < layers.size(); ++k) {
+        BFPN_REQUIRE(tables[k]->precision == precision, "layer tables of one precision");
+        x_end.push_back(layers[k].x_end);
     }
     std::unique_ptr<bfpn_ctx> ctx(make_context(tables.front(), g, p));
     if (precision == BFPN_FP32) {

@@ -87,7 +87,14 @@ namespace bfpn {
 bfpn_ctx* make_context(std::shared_ptr<PhysicsBase> physics, const bfpn_grid& g,
                        const bfpn_phys& p);
 // A context marching through depth layers (tables built per layer; the beam
-// from `p`).
+// from `p`), or over table sets built already (one per layer, from
+// layer_phys(p, layer) and the layer's data path).
 bfpn_ctx* make_layered_context(const bfpn_grid& g, const bfpn_phys& p,
                                const std::vector<LayerSpec>& layers, int precision);
+bfpn_ctx* make_layered_context(const bfpn_grid& g, const bfpn_phys& p,
+                               const std::vector<LayerSpec>& layers,
+                               const std::vector<std::shared_ptr<PhysicsBase>>& tables);
+// The physics parameters of one layer: `p` with the layer's material and
+// density (both density and energy_density).
+bfpn_phys layer_phys(const bfpn_phys& p, const LayerSpec& l);
 }  // namespace bfpn

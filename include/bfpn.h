@@ -414,6 +414,13 @@ BFPN_API bfpn_status bfpn_server_create(const bfpn_server_opts* opts, bfpn_serve
 BFPN_API bfpn_status bfpn_server_submit(bfpn_server* s, const bfpn_grid* grid,
                                         const bfpn_phys* phys, const char* data_path,
                                         double t_final, uint64_t* id);
+/* The same for a medium layered in depth (layers as in bfpn_create_layered).
+ * No counterpart in the reference.  Each layer's tables are cached under
+ * their own key, so a layer of the same material, density and tables as a
+ * homogeneous request shares that request's tables. */
+BFPN_API bfpn_status bfpn_server_submit_layered(bfpn_server* s, const bfpn_grid* grid,
+                                                const bfpn_phys* phys, const bfpn_layer* layers,
+                                                int nlayers, double t_final, uint64_t* id);
 /* 0 queued, 1 running, 2 done, 3 failed, -1 unknown (or already collected). */
 BFPN_API int bfpn_server_poll(bfpn_server* s, uint64_t id);
 /* Blocks until request `id` finished (timeout_s <= 0: no limit), copies its IDD

@@ -119,3 +119,40 @@ def test_alternating_table_keys_under_a_tight_budget(gpu):
     assert st["completed"] == len(cases) and st["failed"] == 0
     assert st["cached_tables"] == 1
     assert 0 < st["bytes_in_use"] <= budget
+
+
+def _layered_mix():
+    """Two plain water beams and two slab beams (water/bone/water and
+    water/air/water) on one grid: the water layers' tables are the plain
+    beams' (same key), so the cache holds three table sets."""
+    def tables(m):
+        return os.path.join(TABLES, m)
+
+    water = ("water", 1.0, tables("water"))
+    base = dict(Ny=3, Nz=3, Nmu=4, Nom=4, time=1.2, flags={"eq15_straggling": 1})
+    wbw = [(0.31, *water), (0.71, "bone", 1.85, tables("bone")), (9.0, *water)]
+    waw = [(0.51, *water), (0.93, "air", 0.001205, tables("air")), (9.0, *water)]
+    return [(small_case(energy=100.0, **base), None), (small_case(energy=100.0, **base), wbw),
+            (small_case(energy=120.0, **base), None), (small_case(energy=100.0, **base), waw)]
+
+
+@pytest.mark.parametrize("max_active,slice_steps", [(4, 1), (2, 7)])
+def test_layered_requests_match_their_solo_runs(gpu, max_active, slice_steps):
+    """Slab beams served beside plain beams (slices ending on both sides of
+    the layer boundaries, mid-pair included) give the solo layered Solver's
+    IDD bit for bit; each material's tables are built once and shared."""
+    mix = _layered_mix()
+    ref = []
+    for case, layers in mix:
+        with gpu.Solver(case, gpu.FP32, layers=layers) as s:
+            ref.append(s.run()[0])
+    with gpu.Server(max_active=max_active, slice_steps=slice_steps) as srv:
+        ids = [srv.submit(case, layers=layers) for case, layers in mix]
+        out = [srv.wait(i, timeout=120)[0] for i in ids]
+        st = srv.stats()
+    for (case, layers), a, b in zip(mix, out, ref):
+        assert len(a) == len(b) == case.num_steps()
+        np.testing.assert_array_equal(a, b, err_msg=str(layers))
+    assert not np.array_equal(ref[0], ref[1])  # the slab changes the dose
+    assert st["completed"] == len(mix) and st["failed"] == 0
+    assert st["cached_tables"] == 3
