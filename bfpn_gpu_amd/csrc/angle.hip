@@ -24,11 +24,11 @@ namespace {
 // One wave per (g, s) plane (each wave walks a contiguous run of planes), the
 // plane zero padded to 32 x 32, every stage a 32x32x32 product on the matrix
 // cores in split FP16 with FP32 accumulation (MT / NT = 16-row tiles along v
-// and a): the state in two FP16 terms, the constant bases in three, four
+// and a): the state and the constant bases in two FP16 terms each, three
 // products kept.  The DST runs every depth step on the same beam, so its
 // error compounds: with two-term BF16 splits the FP32 IDD drifts ~1 % above
 // the FP64 one over 1600 steps; two FP16 terms carry 22 significand bits,
-// and the operator is as exact as FP32 arithmetic.  FP16 needs a range: each
+// and the operator is about as exact as FP32 arithmetic.  FP16 needs a range: each
 // plane is scaled by a power of two s so that max|X s| lies in [16, 32),
 // the bases are stored times 16, so every intermediate stays below 2^14
 // (|Y|, |W| <= 21 max|X| for the sine bases, |Z| <= 1.6 max|X| with the CN
@@ -59,15 +59,15 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
     const int active = top ? min(planes, (min(*top, planes / nyz - 1) + 1) * nyz) : planes;
     const int pw = top ? (active + gridDim.x * 4 - 1) / (gridDim.x * 4) : per_wave;
     if (static_cast<int>(blockIdx.x) * 4 * pw >= active) return;
-    // The nine constant basis arrays (Su, Su chained, Sv chained; hi, mid,
-    // lo) live in LDS, shared by the work-group's four waves and read per
-    // stage: held in registers they would cost 72 VGPRs and half the
-    // occupancy.  Rows padded to 40 values (80 bytes) against bank conflicts.
-    __shared__ __attribute__((aligned(16))) _Float16 cst[9][32][40];
+    // The six constant basis arrays (Su, Su chained, Sv chained; hi, lo)
+    // live in LDS, shared by the work-group's four waves and read per stage:
+    // held in registers they would cost 48 VGPRs and one of the four waves
+    // per SIMD.  Rows padded to 40 values (80 bytes) against bank conflicts.
+    __shared__ __attribute__((aligned(16))) _Float16 cst[6][32][40];
     {
-        const _Float16* src[9] = {plan.su_h, plan.su_m, plan.su_l, plan.sup_h, plan.sup_m,
-                                  plan.sup_l, plan.svp_h, plan.svp_m, plan.svp_l};
-        for (int e = threadIdx.x; e < 9 * 32 * 4; e += blockDim.x) {  // 16-byte chunks
+        const _Float16* src[6] = {plan.su_h, plan.su_l, plan.sup_h, plan.sup_l, plan.svp_h,
+                                  plan.svp_l};
+        for (int e = threadIdx.x; e < 6 * 32 * 4; e += blockDim.x) {  // 16-byte chunks
             const int a = e >> 7, r = (e >> 2) & 31, c = (e & 3) * 8;
             *reinterpret_cast<f16x8*>(&cst[a][r][c]) =
                 *reinterpret_cast<const f16x8*>(src[a] + r * 32 + c);
@@ -106,9 +106,8 @@ __global__ __launch_bounds__(256) void angle_mfma_kernel(float* __restrict__ F,
     // at compile-time offsets from the lane's base (row i, columns 8q..).
     auto frag = [&](const _Float16* base, int a, int t) {
         constexpr int kArr = 32 * 40, kTile = 16 * 40;
-        return HalfFrag3{*reinterpret_cast<const f16x8*>(base + (3 * a) * kArr + t * kTile),
-                         *reinterpret_cast<const f16x8*>(base + (3 * a + 1) * kArr + t * kTile),
-                         *reinterpret_cast<const f16x8*>(base + (3 * a + 2) * kArr + t * kTile)};
+        return HalfFrag2{*reinterpret_cast<const f16x8*>(base + (2 * a) * kArr + t * kTile),
+                         *reinterpret_cast<const f16x8*>(base + (2 * a + 1) * kArr + t * kTile)};
     };
 
     int g_cur = -1;

@@ -84,7 +84,7 @@ struct SourcePlanBuffers {
 
 // Split-FP16 sine bases for the MFMA DST (FP32 state, Nmu, Nom <= 32).
 struct AnglePlanBuffers {
-    DeviceBuffer h, m, l;
+    DeviceBuffer h, l;
     AnglePlan plan;
     // The MFMA DST holds planes of at most 32 x 32 directions and addresses a
     // launch's planes with 32-bit offsets (launch_angle_mfma); other grids

@@ -156,19 +156,16 @@ void launch_angle_full(T* F, const T* factors, const T* su, const T* sv, int Ng,
                        int Nom, hipStream_t s, const int* top = nullptr);
 
 // Sine bases of the MFMA DST: 16 x the basis (a power of two, exact), split
-// into three FP16 terms (hi + mid + lo) from the FP64 values, 32x32 zero
-// padded, row-major: Su as is, and Su and Sv with their columns in the
+// into two FP16 terms (hi + lo) from the FP64 values, 32x32 zero padded,
+// row-major: Su as is, and Su and Sv with their columns in the
 // register-chaining order of the DST kernel.
 constexpr float kAngleBasisScale = 16.0f;
 struct AnglePlan {
     const _Float16* su_h = nullptr;
-    const _Float16* su_m = nullptr;
     const _Float16* su_l = nullptr;
     const _Float16* sup_h = nullptr;
-    const _Float16* sup_m = nullptr;
     const _Float16* sup_l = nullptr;
     const _Float16* svp_h = nullptr;
-    const _Float16* svp_m = nullptr;
     const _Float16* svp_l = nullptr;
 };
 // K slot 8q + 4u + r of a chained MFMA operand <-> matrix index 16u + 4q + r.

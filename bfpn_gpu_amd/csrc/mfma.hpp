@@ -72,13 +72,10 @@ __device__ inline f32x4 zero4() { return f32x4{0.f, 0.f, 0.f, 0.f}; }
 // Operand layouts of v_mfma_f32_16x16x32_f16 are those of the BF16 form.
 typedef __attribute__((ext_vector_type(8))) _Float16 f16x8;
 
-// The state: x = hi + lo (+ ~2^-22 |x|); x already scaled into FP16 range.
+// Two FP16 terms, x = hi + lo (+ ~2^-22 |x|): the state (already scaled into
+// FP16 range) and the constant bases (split from FP64).
 struct HalfFrag2 {
     f16x8 hi, lo;
-};
-// A constant: hi + mid + lo from FP64 (the constant's own error ~2^-33).
-struct HalfFrag3 {
-    f16x8 hi, mid, lo;
 };
 
 // Splits 8 values v * s (SCALE) or v.  The residual is formed as
@@ -104,20 +101,21 @@ __device__ inline HalfFrag2 split8_h(const float (&v)[8], float s) {
     return f;
 }
 
-// acc + A*B with A the state and B a constant, keeping every product term
-// down to ~2^-22 of hi*hi: c_lo x_hi, c_mid x_hi, c_hi x_lo, c_hi x_hi (four
-// MFMA issues, the smallest first; c_mid x_lo ~2^-22 is dropped).
-__device__ inline f32x4 mma_sc(const HalfFrag2& a, const HalfFrag3& b, f32x4 acc) {
+// acc + A*B with A the state and B a constant: c_lo x_hi, c_hi x_lo, c_hi x_hi
+// (three MFMA issues, the small terms first).  The dropped c_lo x_lo is
+// ~2^-22 of hi*hi, the size of the two terms' own representation error.  A
+// third constant term (one more product, also ~2^-22) lowered the DST's error
+// on random 20 x 20 planes only from 2.3e-7 to 1.9e-7 of their maximum, for
+// 6 % of the kernel's time.
+__device__ inline f32x4 mma_sc(const HalfFrag2& a, const HalfFrag2& b, f32x4 acc) {
     acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.hi, b.lo, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.hi, b.mid, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.lo, b.hi, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.hi, b.hi, acc, 0, 0, 0);
     return acc;
 }
 // The same with A the constant and B the state.
-__device__ inline f32x4 mma_cs(const HalfFrag3& a, const HalfFrag2& b, f32x4 acc) {
+__device__ inline f32x4 mma_cs(const HalfFrag2& a, const HalfFrag2& b, f32x4 acc) {
     acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.lo, b.hi, acc, 0, 0, 0);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.mid, b.hi, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.hi, b.lo, acc, 0, 0, 0);
     acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(a.hi, b.hi, acc, 0, 0, 0);
     return acc;

@@ -191,9 +191,9 @@ double half_value(uint16_t b) {
 
 void AnglePlanBuffers::build(int Nmu, int Nom, hipStream_t) {
     // Su, Su (chained columns), Sv (chained columns); each zero padded to
-    // 32x32, scaled by kAngleBasisScale (exact) and split into FP16
-    // hi + mid + lo from the FP64 basis.  (The scale keeps the smallest
-    // basis values' lo terms clear of the FP16 subnormal range.)
+    // 32x32, scaled by kAngleBasisScale (exact) and split into FP16 hi + lo
+    // from the FP64 basis.  (The scale keeps the smallest basis values' lo
+    // terms clear of the FP16 subnormal range.)
     std::vector<double> pad(3 * 32 * 32, 0.0);
     const auto su = sine_basis(Nom), sv = sine_basis(Nmu);
     for (int r = 0; r < 32; ++r)
@@ -203,28 +203,21 @@ void AnglePlanBuffers::build(int Nmu, int Nom, hipStream_t) {
             if (r < Nom && cc < Nom) pad[1024 + r * 32 + c] = su[r * Nom + cc];
             if (r < Nmu && cc < Nmu) pad[2048 + r * 32 + c] = sv[r * Nmu + cc];
         }
-    std::vector<uint16_t> vh(pad.size()), vm(pad.size()), vl(pad.size());
+    std::vector<uint16_t> vh(pad.size()), vl(pad.size());
     for (size_t k = 0; k < pad.size(); ++k) {
         const double x = kAngleBasisScale * pad[k];
         vh[k] = half_bits(x);
-        const double r1 = x - half_value(vh[k]);
-        vm[k] = half_bits(r1);
-        vl[k] = half_bits(r1 - half_value(vm[k]));
+        vl[k] = half_bits(x - half_value(vh[k]));
     }
     upload(h, vh);
-    upload(m, vm);
     upload(l, vl);
     const _Float16* H = h.as<_Float16>();
-    const _Float16* M = m.as<_Float16>();
     const _Float16* L = l.as<_Float16>();
     plan.su_h = H;
-    plan.su_m = M;
     plan.su_l = L;
     plan.sup_h = H + 1024;
-    plan.sup_m = M + 1024;
     plan.sup_l = L + 1024;
     plan.svp_h = H + 2048;
-    plan.svp_m = M + 2048;
     plan.svp_l = L + 2048;
 }
 
@@ -377,7 +370,7 @@ struct Physics final : PhysicsBase {
                xmask64.bytes() +
                plan.bytes() +
                angle_plan.h.bytes() +
-               angle_plan.m.bytes() + angle_plan.l.bytes();
+               angle_plan.l.bytes();
     }
 };
 
@@ -1063,7 +1056,7 @@ struct bfpn_workspace {
         b.ensure(bytes);
     }
     size_t device_bytes() const {
-        size_t n = plan.bytes() + angle_plan.h.bytes() + angle_plan.m.bytes() + angle_plan.l.bytes();
+        size_t n = plan.bytes() + angle_plan.h.bytes() + angle_plan.l.bytes();
         for (const DeviceBuffer* b : {&coef, &partial, &a1, &a2, &factors, &su, &sv, &om, &dose_w,
                                       &yzw, &kvT})
             n += b->bytes();

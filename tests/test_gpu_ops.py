@@ -8,8 +8,8 @@ Tolerances, relative to the largest magnitude of the reference output:
               (both run on the matrix cores with FP32 accumulation: the
               source in split-BF16, hi + lo, ~2^-16 relative per product;
               the angle in split FP16, the state in two terms under a
-              per-plane power-of-two scale and the bases in three, close to
-              FP32 products; a single-precision limiter can take the other
+              per-plane power-of-two scale and the bases in two, three
+              products per tile, close to FP32 products; a single-precision limiter can take the other
               branch of the superbee kink in isolated cells).
 """
 import numpy as np
