@@ -156,3 +156,29 @@ def test_layered_requests_match_their_solo_runs(gpu, max_active, slice_steps):
     assert not np.array_equal(ref[0], ref[1])  # the slab changes the dose
     assert st["completed"] == len(mix) and st["failed"] == 0
     assert st["cached_tables"] == 3
+
+
+def test_layered_request_with_missing_tables_fails_alone(gpu):
+    """A slab beam whose middle layer names a table directory that does not
+    exist fails with the loader's message; the beams around it, a good slab
+    beam among them, complete with their solo results."""
+    mix = _layered_mix()
+    good_case, good_layers = mix[1]
+    bad_layers = [good_layers[0], (0.71, "bone", 1.85, os.path.join(TABLES, "no_such_material")),
+                  good_layers[2]]
+    water_case = mix[0][0]
+    with gpu.Solver(good_case, gpu.FP32, layers=good_layers) as s:
+        ref_slab = s.run()[0]
+    ref_water = _alone(gpu, water_case)
+    with gpu.Server(max_active=2, slice_steps=4) as srv:
+        a = srv.submit(water_case)
+        b = srv.submit(good_case, layers=bad_layers)
+        c = srv.submit(good_case, layers=good_layers)
+        with pytest.raises(gpu.BfpnError, match="no_such_material"):
+            srv.wait(b, timeout=120)
+        ra = srv.wait(a, timeout=120)[0]
+        rc = srv.wait(c, timeout=120)[0]
+        st = srv.stats()
+    np.testing.assert_array_equal(ra, ref_water)
+    np.testing.assert_array_equal(rc, ref_slab)
+    assert st["failed"] == 1 and st["completed"] == 2
