@@ -79,6 +79,8 @@ def lib():
     L.bfpn_create.argtypes = [C.POINTER(Grid), C.POINTER(Phys), C.c_char_p, i, C.POINTER(vp)]
     L.bfpn_create_layered.argtypes = [C.POINTER(Grid), C.POINTER(Phys), C.POINTER(Layer), i, i,
                                       C.POINTER(vp)]
+    L.bfpn_create_mapped.argtypes = [C.POINTER(Grid), C.POINTER(Phys), C.POINTER(Layer), i, vp,
+                                     i, vp, i, C.POINTER(vp)]
     L.bfpn_destroy.argtypes = [vp]
     L.bfpn_get_deltas.argtypes = [vp, C.POINTER(Deltas)]
     L.bfpn_cells.argtypes = [vp]
@@ -204,16 +206,31 @@ TABLES = ("S", "sigma", "T", "sig_tr", "en", "ker_e1", "ker_e2", "ker_v", "omend
 class Solver:
     """One beam: tables, state and depth march on the GPU (bfpn_ctx)."""
 
-    def __init__(self, case: Case, precision: int = FP32, layers=None):
+    def __init__(self, case: Case, precision: int = FP32, layers=None, mapped=None):
         """layers: optional depth layers [(x_end_cm, material, density, table_dir), ...]
-        in increasing depth, the last extending to any depth (bfpn_create_layered);
-        the beam and flags come from `case`."""
+        in increasing depth, the last extending to any depth (bfpn_create_layered).
+        mapped: optional material map (materials, slab_end, map) (bfpn_create_mapped):
+        materials [(material, density, table_dir), ...], slab_end the depths (cm)
+        ending each slab (the last: any depth), map an integer array
+        (slabs, (Ny+1)(Nz+1)) of material indices, node s = iz (Ny+1) + iy.
+        The beam and flags come from `case`."""
         self.case = case
         self.precision = precision
         self.layers = list(layers) if layers else None
         g, p = case.grid(), case.phys()
         h = C.c_void_p(0)
-        if self.layers:
+        if mapped is not None:
+            materials, slab_end, mp = mapped
+            n = len(materials)
+            spec = (Layer * n)(*[Layer(0.0, m.encode(), float(rho), path.encode())
+                                 for m, rho, path in materials])
+            ends = np.ascontiguousarray(slab_end, dtype=np.float64)
+            mp = np.ascontiguousarray(mp, dtype=np.uint8)
+            if mp.size != len(ends) * case.nyz:
+                raise ValueError("the map needs (Ny+1)(Nz+1) material indices per slab")
+            check(lib().bfpn_create_mapped(C.byref(g), C.byref(p), spec, n, _addr(ends), len(ends),
+                                           _addr(mp), precision, C.byref(h)))
+        elif self.layers:
             n = len(self.layers)
             spec = (Layer * n)(*[Layer(float(x), m.encode(), float(rho), path.encode())
                                  for x, m, rho, path in self.layers])

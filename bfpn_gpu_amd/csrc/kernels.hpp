@@ -192,6 +192,18 @@ void launch_yz_weights(int Ny, int Nz, bool trapezoidal, double* out, hipStream_
 // out[slot] = scale * sum(a[0:na]) + scale * sum(b[0:nb])  (b may be null), one block.
 void launch_finish_sum(const double* a, int na, const double* b, int nb, double scale, double* out,
                        hipStream_t s);
+// *out = ((0 + v[0]) + v[1]) + ... + v[n-1], in that order.
+void launch_sum_in_order(const double* v, int n, double* out, hipStream_t s);
+
+// Node subsets of a state [g][node][direction] (Ng x nyz x nang): gather
+// packs the nodes listed in `nodes` (n of them) into dst [g][k][direction];
+// scatter writes them back.
+template <typename T>
+void launch_gather_nodes(T* dst, const T* src, const int* nodes, int n, long long nyz, int nang,
+                         int Ng, hipStream_t s);
+template <typename T>
+void launch_scatter_nodes(T* dst, const T* src, const int* nodes, int n, long long nyz, int nang,
+                          int Ng, hipStream_t s);
 
 // F[g][s][m] = space_angle[s*nang+m] * energy[g]
 template <typename T>

@@ -71,6 +71,30 @@ __global__ __launch_bounds__(1024) void finish_sum_kernel(const double* __restri
     if (threadIdx.x == 0) *out = lds[0] * scale;
 }
 
+__global__ void sum_in_order_kernel(const double* __restrict__ v, int n, double* __restrict__ out) {
+    double s = 0.0;
+    for (int k = 0; k < n; ++k) s += v[k];
+    *out = s;
+}
+
+// One node's directions are contiguous, so each block copies runs of nang
+// values; grid (ceil(n nang / kBlock), Ng).
+template <typename T, bool GATHER>
+__global__ __launch_bounds__(kBlock) void node_copy_kernel(T* __restrict__ dst, const T* __restrict__ src,
+                                                           const int* __restrict__ nodes, int n,
+                                                           long long nyz, int nang) {
+    const long long idx = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    const long long per = static_cast<long long>(n) * nang;
+    if (idx >= per) return;
+    const long long k = idx / nang, a = idx - k * nang, g = blockIdx.y;
+    const long long packed = g * per + idx;
+    const long long full = (g * nyz + nodes[k]) * nang + a;
+    if (GATHER)
+        dst[packed] = src[full];
+    else
+        dst[full] = src[packed];
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void beam_kernel(T* __restrict__ F,
                                                       const double* __restrict__ space_angle,
@@ -272,6 +296,44 @@ void launch_finish_sum(const double* a, int na, const double* b, int nb, double 
     hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(1024), 0, s, a, na, b, nb, scale, out);
     BFPN_HIP(hipGetLastError());
 }
+
+void launch_sum_in_order(const double* v, int n, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(sum_in_order_kernel, dim3(1), dim3(1), 0, s, v, n, out);
+    BFPN_HIP(hipGetLastError());
+}
+
+template <typename T, bool GATHER>
+static void node_copy(T* dst, const T* src, const int* nodes, int n, long long nyz, int nang, int Ng,
+                      hipStream_t s) {
+    const long long per = static_cast<long long>(n) * nang;
+    if (per <= 0 || Ng <= 0) return;
+    BFPN_REQUIRE(Ng <= 65535 && (per + kBlock - 1) / kBlock < (1LL << 31), "node copy grid too large");
+    const dim3 grid(static_cast<unsigned>((per + kBlock - 1) / kBlock), static_cast<unsigned>(Ng));
+    hipLaunchKernelGGL((node_copy_kernel<T, GATHER>), grid, dim3(kBlock), 0, s, dst, src, nodes, n,
+                       nyz, nang);
+    BFPN_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_gather_nodes(T* dst, const T* src, const int* nodes, int n, long long nyz, int nang,
+                         int Ng, hipStream_t s) {
+    node_copy<T, true>(dst, src, nodes, n, nyz, nang, Ng, s);
+}
+
+template <typename T>
+void launch_scatter_nodes(T* dst, const T* src, const int* nodes, int n, long long nyz, int nang,
+                          int Ng, hipStream_t s) {
+    node_copy<T, false>(dst, src, nodes, n, nyz, nang, Ng, s);
+}
+
+template void launch_gather_nodes<float>(float*, const float*, const int*, int, long long, int, int,
+                                         hipStream_t);
+template void launch_gather_nodes<double>(double*, const double*, const int*, int, long long, int,
+                                          int, hipStream_t);
+template void launch_scatter_nodes<float>(float*, const float*, const int*, int, long long, int,
+                                          int, hipStream_t);
+template void launch_scatter_nodes<double>(double*, const double*, const int*, int, long long, int,
+                                           int, hipStream_t);
 
 template <typename T>
 void launch_beam(T* F, const double* space_angle, const double* energy, int Ng, long long lanes,

@@ -440,6 +440,10 @@ public:
                    partial_s_.bytes();
         for (const auto& s : state_) b += s.bytes();
         for (size_t k = 1; k < layers_.size(); ++k) b += layers_[k]->device_bytes();
+        for (const auto& c : cbuf_) b += c.bytes();
+        b += map_nodes_.bytes() + map_yzw_.bytes() + map_tmp_.bytes();
+        for (const auto& m : mats_)
+            if (m != P_) b += m->device_bytes();
         return b;
     }
 
@@ -453,11 +457,78 @@ public:
         x_end_ = std::move(x_end);
     }
 
+    // Material map: the node-local operators (energy, angular diffusion,
+    // source, dose) of node s in depth slab j use the tables of material
+    // map[j nyz + s]; slab j covers the depth steps whose midpoint lies below
+    // slab_end[j] (the last: any depth).  Each operator runs per material on
+    // the material's nodes packed together (the same kernels as a
+    // homogeneous run, on fewer nodes); transport, which does not depend on
+    // the material, runs on the whole state.  One stream, every group
+    // computed.
+    void set_map(std::vector<std::shared_ptr<Physics<T>>> mats, std::vector<double> slab_end,
+                 const std::vector<unsigned char>& map) {
+        BFPN_REQUIRE(!mats.empty() && !slab_end.empty(), "a material map needs materials and slabs");
+        for (const auto& m : mats)
+            BFPN_REQUIRE(!m->legacy && !m->lite,
+                         "material maps support the eq15 energy model with the full state only");
+        const long long nyz = P_->nyz;
+        const int nang = P_->nang, Ng = grid.Ng;
+        BFPN_REQUIRE(map.size() == slab_end.size() * static_cast<size_t>(nyz),
+                     "the map needs one material index per node and slab");
+        const auto w = yz_weights(grid.Ny, grid.Nz, phys.trapezoidal_yz != 0);
+        std::vector<int> lists;
+        std::vector<double> weights;
+        long long max_nodes = 0, pslot = 0;
+        for (size_t j = 0; j < slab_end.size(); ++j) {
+            std::vector<std::vector<int>> by(mats.size());
+            for (long long s = 0; s < nyz; ++s) {
+                const unsigned m = map[j * nyz + s];
+                BFPN_REQUIRE(m < mats.size(), "map entry names no material");
+                by[m].push_back(static_cast<int>(s));
+            }
+            MapSlab sl;
+            long long off = 0;
+            int poff = 0;
+            for (size_t m = 0; m < by.size(); ++m) {
+                if (by[m].empty()) continue;
+                const int n = static_cast<int>(by[m].size());
+                sl.mat.push_back(static_cast<int>(m));
+                sl.count.push_back(n);
+                sl.first.push_back(static_cast<int>(lists.size() - j * nyz));
+                sl.off.push_back(off);
+                off += static_cast<long long>(Ng) * n * nang;
+                const int eb = energy_blocks(static_cast<long long>(n) * nang, sizeof(T));
+                sl.eb.push_back(eb);
+                sl.poff.push_back(poff);
+                poff += 2 * eb;
+                for (int s : by[m]) {
+                    lists.push_back(s);
+                    weights.push_back(w[s]);
+                }
+                max_nodes = std::max<long long>(max_nodes, n);
+            }
+            pslot = std::max<long long>(pslot, poff);
+            slabs_.push_back(std::move(sl));
+        }
+        upload(map_nodes_, lists);
+        upload(map_yzw_, weights);
+        for (auto& b : cbuf_) b.ensure(static_cast<size_t>(Ng) * max_nodes * nang * sizeof(T));
+        pslot_ = static_cast<int>(pslot);
+        partial_p_.ensure(4 * static_cast<size_t>(pslot) * sizeof(double));
+        partial_s_.ensure(static_cast<size_t>(pslot) * sizeof(double));
+        map_tmp_.ensure(2 * mats.size() * sizeof(double));
+        mats_ = std::move(mats);
+        slab_end_ = std::move(slab_end);
+        skip_ = false;
+    }
+
     int steps_done() const override { return step_; }
 
     // Eq.25 moments of the current state (computeEnergyMoments,
     // src/bfpm_solver.cu:3504-3526): [g][psi^1, psi^2] x du dv dy dz.
     void energy_moments(double* host) override {
+        if (!mats_.empty())
+            throw Error(BFPN_E_UNSUPPORTED, "energy moments are not available with a material map");
         if (P_->lite)
             throw Error(BFPN_E_UNSUPPORTED, "energy moments need psi^2 and the secondary "
                                             "(not kept with --lite-memory)");
@@ -476,6 +547,8 @@ public:
     // Y-Z dose plane of the current state (computeSpotDosePlane,
     // src/bfpm_solver.cu:3560-3585): node s = iz (Ny+1) + iy.
     void spot_plane(double* host) override {
+        if (!mats_.empty())
+            throw Error(BFPN_E_UNSUPPORTED, "spot planes are not available with a material map");
         if (P_->lite)
             throw Error(BFPN_E_UNSUPPORTED, "spot planes need psi^2 and the secondary "
                                             "(not kept with --lite-memory)");
@@ -501,6 +574,11 @@ public:
         }
         for (int k = 0; k < nsteps; ++k) {
             bool first = k == 0, last = k == nsteps - 1;
+            if (!mats_.empty()) {  // a paired sweep never spans two slabs
+                const int j = slab_of(step_);
+                step_mapped(first || j != slab_of(step_ - 1), last || j != slab_of(step_ + 1));
+                continue;
+            }
             if (!layers_.empty()) {  // a paired sweep never spans two materials
                 const int L = layer_of(step_);
                 P_ = layers_[L];
@@ -512,7 +590,7 @@ public:
             else  // the upwind model has no paired sweep: every step stands alone
                 step(first || P_->legacy, last || P_->legacy);
         }
-        if (stream2_ && !overlap) {  // all on `stream`: done_stream still ends last
+        if (stream2_ && (!overlap || !mats_.empty())) {  // all on `stream`: done_stream still ends last
             BFPN_HIP(hipEventRecord(ev_tail_, stream));
             BFPN_HIP(hipStreamWaitEvent(stream2_, ev_tail_, 0));
         }
@@ -563,6 +641,7 @@ public:
     // Turning it on makes every top conservative (Ng-1) until the next sweeps
     // have measured them.
     void set_group_skip(bool on) override {
+        if (!mats_.empty()) return;  // a material map computes every group
         if (on && !skip_) {
             sync_all();
             init_tops(grid.Ng - 1, grid.Ng - 1);
@@ -908,9 +987,214 @@ private:
         ++step_;
     }
 
+    // Slab of depth step n under a material map (the one holding its midpoint).
+    int slab_of(int n) const {
+        const double x = (n + 0.5) * d.dt;
+        for (size_t k = 0; k + 1 < slab_end_.size(); ++k)
+            if (x < slab_end_[k]) return static_cast<int>(k);
+        return static_cast<int>(slab_end_.size()) - 1;
+    }
+
+    // One depth step under a material map: step() with every node-local
+    // operator run per material on its packed nodes, on one stream.  The
+    // end-of-step primary copy and the sources live packed (material k of
+    // the slab at element offset off[k]), as the source and the secondary
+    // sweeps read them; the dose partials of material k sit at poff[k] of a
+    // slot ([dose: eb][proxy: eb]), summed per material and then over the
+    // materials in order.
+    void step_mapped(bool first, bool last) {
+        const int n = step_, j = slab_of(n);
+        const MapSlab& sl = slabs_[j];
+        const int nk = static_cast<int>(sl.mat.size());
+        const int Ng = grid.Ng;
+        hipStream_t s = stream;
+        const bool secondary = P_->secondary;
+        const bool preserve_primary = phys.no_spatial_clipping != 0;
+        const bool strag = phys.eq15_straggling != 0;
+        T* cA = cbuf_[0].as<T>();
+        T* cB = cbuf_[1].as<T>();
+        auto use = [&](int k) { P_ = mats_[sl.mat[k]]; };
+        auto nodes = [&](int k) { return map_nodes_.as<int>() + static_cast<long long>(j) * P_->nyz + sl.first[k]; };
+        auto yzw = [&](int k) { return map_yzw_.as<double>() + static_cast<long long>(j) * P_->nyz + sl.first[k]; };
+        auto lanes = [&](int k) { return static_cast<long long>(sl.count[k]) * P_->nang; };
+        auto packed = [&](const DeviceBuffer& b, int k) { return b.as<T>() + sl.off[k]; };
+        auto part = [&](double* base, int k) { return base ? base + sl.poff[k] : nullptr; };
+        auto gather = [&](int k, T* c, int w) {
+            launch_gather_nodes<T>(c, cur_[w], nodes(k), sl.count[k], P_->nyz, P_->nang, Ng, s);
+        };
+        auto scatter = [&](int k, const T* c, int w) {
+            launch_scatter_nodes<T>(cur_[w], c, nodes(k), sl.count[k], P_->nyz, P_->nang, Ng, s);
+        };
+        auto tops = [&] {
+            SweepTops t;
+            t.q_cap = P_->qcap;
+            return t;
+        };
+        // single energy half of a species; Q: packed sources or none
+        auto half = [&](int sp, const DeviceBuffer* Q, double* partials, bool proxy) {
+            for (int k = 0; k < nk; ++k) {
+                use(k);
+                gather(k, cA, 2 * sp);
+                gather(k, cB, 2 * sp + 1);
+                launch_energy_half<T>(cA, cB, P_->coef.template as<EnergyCoef>(), Q ? packed(*Q, k) : nullptr,
+                                      Ng, lanes(k), strag, yzw(k), P_->nang, part(partials, k), proxy, s,
+                                      tops());
+                scatter(k, cA, 2 * sp);
+                scatter(k, cB, 2 * sp + 1);
+            }
+            mark(kEnergy, array_bytes() * (Q ? 5 : 4), s);
+        };
+        // paired energy sweep; X: packed end-of-step copies (primary) or none
+        auto pair = [&](int sp, const DeviceBuffer* Qa, const DeviceBuffer* Qb, double* partials,
+                        bool proxy, bool xout) {
+            for (int k = 0; k < nk; ++k) {
+                use(k);
+                gather(k, cA, 2 * sp);
+                gather(k, cB, 2 * sp + 1);
+                launch_energy_pair<T>(cA, cB, P_->coef.template as<EnergyCoef>(),
+                                      Qa ? packed(*Qa, k) : nullptr, Qb ? packed(*Qb, k) : nullptr, Ng,
+                                      lanes(k), strag, yzw(k), P_->nang, part(partials, k), proxy,
+                                      xout ? packed(x_[0], k) : nullptr, xout ? packed(x_[1], k) : nullptr,
+                                      s, tops(), x_rows());
+                scatter(k, cA, 2 * sp);
+                scatter(k, cB, 2 * sp + 1);
+            }
+            mark(kEnergy, array_bytes() * (4 + (Qa ? 2 : 0) + (xout ? 2 : 0)), s);
+        };
+        auto tr = [&](int w, bool preserve) {
+            launch_transport_half<T>(cur_[w], P_->om_dev.template as<T>(), grid.Ny, grid.Nz, Ng,
+                                     P_->nang, d.dy, d.dz, 0.5 * d.dt, preserve, s, nullptr);
+            mark(kTransport, array_bytes() * 2, s);
+        };
+        auto ang = [&](int w) {
+            for (int k = 0; k < nk; ++k) {
+                use(k);
+                gather(k, cA, w);
+                bool done = false;
+                if constexpr (std::is_same_v<T, float>) {
+                    if (P_->angle_plan.plan.su_h) {
+                        launch_angle_mfma(cA, P_->factors.template as<float>(), P_->angle_plan.plan, Ng,
+                                          sl.count[k], grid.Nmu, grid.Nom, s, nullptr);
+                        done = true;
+                    }
+                }
+                if (!done)
+                    launch_angle_full<T>(cA, P_->factors.template as<T>(), P_->su.template as<T>(),
+                                         P_->sv.template as<T>(), Ng, sl.count[k], grid.Nmu, grid.Nom, s,
+                                         nullptr);
+                scatter(k, cA, w);
+            }
+            mark(kAngle, array_bytes() * 2, s);
+        };
+        // the source of material k from its packed primary (Fp, fp) into Q
+        auto src = [&](int k, const T* Fp, const T* fp, T* Q) {
+            if constexpr (std::is_same_v<T, float>) {
+                launch_source_mfma(Fp, fp, a1_.as<float>(), a2_.as<float>(), Q, P_->plan.plan, Ng,
+                                   sl.count[k], P_->nang, s, nullptr, nullptr, nullptr, P_->qcap);
+            } else {
+                const int* need = P_->need.template as<int>();
+                launch_source_angular<T>(Fp, fp, P_->kv.template as<T>(), a1_.as<T>(), a2_.as<T>(), Ng,
+                                         sl.count[k], P_->nang, s, nullptr,
+                                         P_->kcount.template as<int>(), P_->klist.template as<int>(),
+                                         need);
+                launch_source_energy<T>(a1_.as<T>(), a2_.as<T>(), P_->e1.template as<double>(),
+                                        P_->e2.template as<double>(), P_->beg.template as<int>(),
+                                        P_->end.template as<int>(), Q, Ng, lanes(k), d.dg, s, nullptr,
+                                        nullptr, need);
+            }
+        };
+        double* hist = hist_.as<double>();
+        double* tmp = map_tmp_.as<double>();
+        const int nm = static_cast<int>(mats_.size());
+        auto finish = [&](int m, bool with_secondary) {  // IDD and proxy of step m (same slab)
+            double* pm = partial_p_.as<double>() + static_cast<size_t>(m & 3) * pslot_;
+            double* ps = partial_s_.as<double>();
+            for (int k = 0; k < nk; ++k) {
+                const double scale = d.du * d.dv * d.dy * d.dz / mats_[sl.mat[k]]->density;
+                launch_finish_sum(pm + sl.poff[k], sl.eb[k], with_secondary ? ps + sl.poff[k] : nullptr,
+                                  sl.eb[k], scale, tmp + k, s);
+                launch_finish_sum(pm + sl.poff[k] + sl.eb[k], sl.eb[k], nullptr, 0, d.dg, tmp + nm + k, s);
+            }
+            launch_sum_in_order(tmp, nk, hist + m, s);
+            launch_sum_in_order(tmp + nm, nk, hist + hist_cap_ + m, s);
+            mark(kDose, 0.0, s);
+        };
+        double* pp = partial_p_.as<double>() + static_cast<size_t>(n & 3) * pslot_;
+        if (first) {
+            mark(kWait, 0.0, s);
+            half(0, nullptr, nullptr, false);
+        }
+        if (!phys.energy_only) {
+            if (!phys.no_transport) {
+                tr(0, preserve_primary);
+                tr(1, true);
+            }
+            if (!phys.no_angle) {
+                ang(0);
+                ang(1);
+            }
+            if (!phys.no_transport) {
+                tr(0, preserve_primary);
+                tr(1, true);
+            }
+        }
+        if (last)
+            half(0, nullptr, pp, true);
+        else
+            pair(0, nullptr, nullptr, pp, true, secondary);
+        const bool sec = secondary && n > 0;
+        if (sec) {
+            DeviceBuffer& Q = q_[n & 1];
+            for (int k = 0; k < nk; ++k) {  // the source of step n
+                use(k);
+                if (last) {  // from the state; otherwise from the end-of-step copy
+                    gather(k, cA, 0);
+                    gather(k, cB, 1);
+                    src(k, cA, cB, packed(Q, k));
+                } else {
+                    src(k, packed(x_[0], k), packed(x_[1], k), packed(Q, k));
+                }
+            }
+            mark(kSource, array_bytes() * 7, s);
+            if (sec_open_) {
+                pair(1, &q_[(n - 1) & 1], &Q, partial_s_.as<double>(), false, false);
+                finish(n - 1, true);
+            } else {
+                half(1, &Q, nullptr, false);
+            }
+            tr(2, false);
+            tr(3, true);
+            ang(2);
+            ang(3);
+            tr(2, false);
+            tr(3, true);
+            if (last) {
+                half(1, &Q, partial_s_.as<double>(), false);
+                finish(n, true);
+                sec_open_ = false;
+            } else {
+                sec_open_ = true;
+            }
+        } else {
+            finish(n, false);
+        }
+        ++step_;
+    }
+
+    struct MapSlab {                     // one depth slab of a material map
+        std::vector<int> mat, count, first;  // per material present: index, nodes, first list entry
+        std::vector<long long> off;      // element offset of its packed arrays
+        std::vector<int> eb, poff;       // dose blocks of its sweeps, offset of its partials
+    };
+
     std::shared_ptr<Physics<T>> P_;  // tables of the current (or last) step
     std::vector<std::shared_ptr<Physics<T>>> layers_;  // empty: one material
     std::vector<double> x_end_;
+    std::vector<std::shared_ptr<Physics<T>>> mats_;  // material map (empty: none)
+    std::vector<double> slab_end_;
+    std::vector<MapSlab> slabs_;
+    DeviceBuffer map_nodes_, map_yzw_, map_tmp_, cbuf_[2];  // node lists and weights, slab-major
+    int pslot_ = 0;                                           // doubles per partial slot
     DeviceBuffer state_[4], a1_, a2_, q_[2], x_[2];
     DeviceBuffer diag_part_, diag_out_;  // diagnostics scratch, allocated on first use
     DeviceBuffer partial_p_, partial_s_, hist_, tops_;
@@ -995,6 +1279,48 @@ bfpn_ctx* make_layered_context(const bfpn_grid& g, const bfpn_phys& p,
         for (auto& b : tables) t.push_back(std::static_pointer_cast<Physics<double>>(b));
         static_cast<Context<double>*>(ctx.get())->set_layers(std::move(t), std::move(x_end));
     }
+    return ctx.release();
+}
+
+void validate_map(const std::vector<LayerSpec>& materials, const std::vector<double>& slab_end,
+                  const std::vector<unsigned char>& map, long long nyz) {
+    BFPN_REQUIRE(!materials.empty() && materials.size() <= 256, "a map needs 1 to 256 materials");
+    for (const LayerSpec& m : materials) {
+        BFPN_REQUIRE(m.density > 0, "material density must be positive");
+        BFPN_REQUIRE(m.material.size() < sizeof(bfpn_phys{}.material_name), "material name too long");
+    }
+    BFPN_REQUIRE(!slab_end.empty(), "a map needs at least one depth slab");
+    for (size_t k = 0; k + 1 < slab_end.size(); ++k) {
+        BFPN_REQUIRE(slab_end[k] > 0, "slab depths must be positive");
+        BFPN_REQUIRE(k == 0 || slab_end[k] > slab_end[k - 1], "slab depths must increase");
+    }
+    BFPN_REQUIRE(map.size() == slab_end.size() * static_cast<size_t>(nyz),
+                 "the map needs (Ny+1)(Nz+1) material indices per slab");
+    for (unsigned char m : map) BFPN_REQUIRE(m < materials.size(), "map entry names no material");
+}
+
+bfpn_ctx* make_mapped_context(const bfpn_grid& g, const bfpn_phys& p,
+                              const std::vector<LayerSpec>& materials,
+                              const std::vector<double>& slab_end,
+                              const std::vector<unsigned char>& map, int precision) {
+    validate_params(g, p);
+    validate_map(materials, slab_end, map, static_cast<long long>(g.Ny + 1) * (g.Nz + 1));
+    BFPN_REQUIRE(!p.legacy_energy && !p.lite_memory,
+                 "material maps support the eq15 energy model with the full state only");
+    std::vector<std::shared_ptr<PhysicsBase>> tables;
+    for (const LayerSpec& m : materials)
+        tables.push_back(make_physics(g, layer_phys(p, m), m.data_path, precision));
+    std::unique_ptr<bfpn_ctx> ctx(make_context(tables.front(), g, p));
+    if (precision == BFPN_FP32) {
+        std::vector<std::shared_ptr<Physics<float>>> t;
+        for (auto& b : tables) t.push_back(std::static_pointer_cast<Physics<float>>(b));
+        static_cast<Context<float>*>(ctx.get())->set_map(std::move(t), slab_end, map);
+    } else {
+        std::vector<std::shared_ptr<Physics<double>>> t;
+        for (auto& b : tables) t.push_back(std::static_pointer_cast<Physics<double>>(b));
+        static_cast<Context<double>*>(ctx.get())->set_map(std::move(t), slab_end, map);
+    }
+    ctx->overlap = false;
     return ctx.release();
 }
 
@@ -1494,6 +1820,41 @@ BFPN_API bfpn_status bfpn_create_layered(const bfpn_grid* grid, const bfpn_phys*
                                           " MiB of device memory, but only " +
                                           std::to_string(free_b >> 20) + " MiB is free");
         *out = make_layered_context(*grid, *phys, spec, precision);
+    });
+}
+
+BFPN_API bfpn_status bfpn_create_mapped(const bfpn_grid* grid, const bfpn_phys* phys,
+                                        const bfpn_layer* materials, int nmat,
+                                        const double* slab_end, int nslabs,
+                                        const unsigned char* map, int precision, bfpn_ctx** out) {
+    return guarded([&] {
+        BFPN_REQUIRE(grid && phys && materials && slab_end && map && out && nmat > 0 && nslabs > 0,
+                     "null argument");
+        *out = nullptr;
+        validate_params(*grid, *phys);
+        std::vector<LayerSpec> mats;
+        for (int k = 0; k < nmat; ++k) {
+            BFPN_REQUIRE(materials[k].material && materials[k].data_path, "null material field");
+            mats.push_back({0.0, materials[k].material, materials[k].density, materials[k].data_path});
+        }
+        const long long nyz = static_cast<long long>(grid->Ny + 1) * (grid->Nz + 1);
+        std::vector<double> ends(slab_end, slab_end + nslabs);
+        std::vector<unsigned char> m(map, map + nslabs * nyz);
+        validate_map(mats, ends, m, nyz);  // before any device call
+        BFPN_REQUIRE(!phys->legacy_energy && !phys->lite_memory,
+                     "material maps support the eq15 energy model with the full state only");
+        size_t free_b = 0, total_b = 0;
+        BFPN_HIP(hipMemGetInfo(&free_b, &total_b));
+        const size_t word = precision == BFPN_FP64 ? 8 : 4;
+        const size_t need = context_bytes(*grid, *phys, precision) +
+                            2 * static_cast<size_t>(grid->Ng) * nyz * grid->Nmu * grid->Nom * word +
+                            nmat * physics_bytes(*grid, *phys, precision) + (64ull << 20);
+        if (need > free_b)
+            throw Error(BFPN_E_NOMEM, "Requested grid and " + std::to_string(nmat) +
+                                          " materials need about " + std::to_string(need >> 20) +
+                                          " MiB of device memory, but only " +
+                                          std::to_string(free_b >> 20) + " MiB is free");
+        *out = make_mapped_context(*grid, *phys, mats, ends, m, precision);
     });
 }
 

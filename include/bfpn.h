@@ -284,6 +284,22 @@ typedef struct bfpn_layer {
 BFPN_API bfpn_status bfpn_create_layered(const bfpn_grid* grid, const bfpn_phys* phys,
                                          const bfpn_layer* layers, int nlayers,
                                          int precision, bfpn_ctx** out);
+
+/* bfpn_create for a material map: per depth slab, the material of every
+ * y-z node.  No counterpart in the reference (its material-map item,
+ * TODO.md:114-118, is open).  materials[k] (x_end unused) builds a table set
+ * from `phys` with that material and density, as a layer does.  Slab j covers
+ * the depth steps whose midpoint lies below slab_end[j] (the last slab: any
+ * depth); map[j (Ny+1)(Nz+1) + s] is the material index of node
+ * s = iz (Ny+1) + iy in slab j.  The energy sweeps, angular diffusion,
+ * secondary source and dose of a node use its material's tables and density;
+ * transport is material-independent.  Runs on one stream and computes every
+ * energy group (bfpn_set_group_skip has no effect); the legacy energy model,
+ * lite memory, energy moments and spot planes are not available. */
+BFPN_API bfpn_status bfpn_create_mapped(const bfpn_grid* grid, const bfpn_phys* phys,
+                                        const bfpn_layer* materials, int nmat,
+                                        const double* slab_end, int nslabs,
+                                        const unsigned char* map, int precision, bfpn_ctx** ctx);
 BFPN_API void bfpn_destroy(bfpn_ctx* ctx);
 
 /* Derived deltas and array sizes. */

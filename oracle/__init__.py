@@ -79,6 +79,10 @@ def lib():
         L.orc_create_layered.argtypes = [C.POINTER(Grid), C.POINTER(Phys), C.c_int, _dp,
                                          C.POINTER(C.c_char_p), _dp, C.POINTER(C.c_char_p)]
         L.orc_create_layered.restype = C.c_void_p
+        L.orc_create_mapped.argtypes = [C.POINTER(Grid), C.POINTER(Phys), C.c_int,
+                                        C.POINTER(C.c_char_p), _dp, C.POINTER(C.c_char_p), C.c_int,
+                                        _dp, np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")]
+        L.orc_create_mapped.restype = C.c_void_p
         L.orc_destroy.argtypes = [C.c_void_p]
         L.orc_num_steps.argtypes = [C.c_void_p, C.c_double]
         L.orc_advance.argtypes = [C.c_void_p, C.c_int]
@@ -241,12 +245,26 @@ TABLE_NAMES = ("S", "sigma", "T", "sig_tr", "en", "ker_e1", "ker_e2", "ker_v", "
 class Solver:
     """Whole in-core eq15 solve on the CPU (BFPnSolver, FP64)."""
 
-    def __init__(self, case: Case, layers=None):
+    def __init__(self, case: Case, layers=None, mapped=None):
         """layers: optional [(x_end_cm, material, density, table_dir), ...] in
-        increasing depth; the last layer extends to any depth."""
+        increasing depth; the last layer extends to any depth.
+        mapped: optional material map (materials, slab_end, map):
+        materials [(material, density, table_dir), ...], slab_end the depths
+        (cm) ending each slab (the last: any depth), map an integer array
+        (slabs, (Ny+1)(Nz+1)) of material indices, node s = iz (Ny+1) + iy."""
         self.case = case
         g, p = case.grid(), case.phys()
-        if layers:
+        if mapped is not None:
+            materials, slab_end, mp = mapped
+            n = len(materials)
+            names = (C.c_char_p * n)(*[m[0].encode() for m in materials])
+            dens = np.array([float(m[1]) for m in materials])
+            paths = (C.c_char_p * n)(*[m[2].encode() for m in materials])
+            ends = np.ascontiguousarray(slab_end, dtype=np.float64)
+            mp = np.ascontiguousarray(mp, dtype=np.uint8)
+            self._h = lib().orc_create_mapped(C.byref(g), C.byref(p), n, names, dens, paths,
+                                              len(ends), ends, mp.ravel())
+        elif layers:
             n = len(layers)
             x_end = np.array([float(l[0]) for l in layers])
             mats = (C.c_char_p * n)(*[l[1].encode() for l in layers])

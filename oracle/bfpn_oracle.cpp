@@ -739,8 +739,9 @@ void angle_full(double* F, const double* sig_trg, long long nyz, int Ng, int Nmu
 double idd_value(const double* F, const double* f, const double* F1, const double* f1,
                  const double* S_s, const double* sigma_c, const double* en, int Ng, int Ny, int Nz,
                  int nang, double dg, double du, double dv, double dy, double dz, double density,
-                 bool trapezoidal, bool catastrophic) {
+                 bool trapezoidal, bool catastrophic, const std::vector<long long>* nodes = nullptr) {
     const long long nyz = static_cast<long long>(Ny + 1) * (Nz + 1);
+    const long long count = nodes ? static_cast<long long>(nodes->size()) : nyz;
     std::vector<double> part(Ng, 0.0);
 #pragma omp parallel for schedule(static)
     for (int ek = 0; ek < Ng; ++ek) {
@@ -749,7 +750,8 @@ double idd_value(const double* F, const double* f, const double* F1, const doubl
         const double slope_w = (S_s[ek + 1] - S_s[ek]) / 6.0;
         const double sigma = std::max(sigma_c[ek], 0.0);
         const double E_mid = 0.5 * (en[ek] + en[ek + 1]);
-        for (long long s = 0; s < nyz; ++s) {
+        for (long long idx = 0; idx < count; ++idx) {  // the listed nodes (all: every node)
+            const long long s = nodes ? (*nodes)[idx] : idx;
             const int iy = static_cast<int>(s % (Ny + 1));
             const int iz = static_cast<int>(s / (Ny + 1));
             double w = 1.0;
@@ -851,6 +853,11 @@ struct Solver {
     // density, the state stays in this object.
     std::vector<std::unique_ptr<Solver>> layers;
     std::vector<double> x_end;
+    // Material map (none: empty): materials (tables only), depth slabs
+    // ending at slab_end, map[j nyz + s] the material of node s in slab j.
+    std::vector<std::unique_ptr<Solver>> mats;
+    std::vector<double> slab_end;
+    std::vector<unsigned char> map;
 };
 
 // The material of depth step n: the layer holding the step's midpoint.
@@ -900,11 +907,125 @@ bool solver_init(Solver& s, const bfpn_grid& g, const bfpn_phys& p, const std::s
     return true;
 }
 
+// Node subsets of a state [g][node][direction]: packed[g][k][a] = full[g][nodes[k]][a].
+void pack_nodes(const std::vector<double>& full, std::vector<double>& packed,
+                const std::vector<long long>& nodes, int Ng, long long nyz, int nang) {
+    const long long n = static_cast<long long>(nodes.size());
+    packed.assign(static_cast<size_t>(Ng) * n * nang, 0.0);
+    for (int g = 0; g < Ng; ++g)
+        for (long long k = 0; k < n; ++k)
+            for (int a = 0; a < nang; ++a)
+                packed[(g * n + k) * nang + a] = full[(g * nyz + nodes[k]) * nang + a];
+}
+void unpack_nodes(std::vector<double>& full, const std::vector<double>& packed,
+                  const std::vector<long long>& nodes, int Ng, long long nyz, int nang) {
+    const long long n = static_cast<long long>(nodes.size());
+    for (int g = 0; g < Ng; ++g)
+        for (long long k = 0; k < n; ++k)
+            for (int a = 0; a < nang; ++a)
+                full[(g * nyz + nodes[k]) * nang + a] = packed[(g * n + k) * nang + a];
+}
+
+// One depth step under a material map (no counterpart in the reference):
+// the step of solver_step with each node-local operator (energy half steps,
+// angular diffusion, secondary source, dose) applied to every node with the
+// tables and density of its material, here by applying it per material to
+// that material's nodes packed together; transport acts on the whole state.
+void solver_step_mapped(Solver& s) {
+    const auto& g = s.g;
+    const auto& d = s.d;
+    const int Ng = g.Ng;
+    const double half = 0.5 * d.dt;
+    const bool strag = s.p.eq15_straggling != 0;
+    int j = static_cast<int>(s.slab_end.size()) - 1;
+    const double x = (s.step + 0.5) * d.dt;
+    for (size_t k = 0; k + 1 < s.slab_end.size(); ++k)
+        if (x < s.slab_end[k]) {
+            j = static_cast<int>(k);
+            break;
+        }
+    std::vector<std::vector<long long>> nodes(s.mats.size());
+    for (long long n = 0; n < s.nyz; ++n) nodes[s.map[j * s.nyz + n]].push_back(n);
+    std::vector<double> pa, pb;
+    std::vector<std::vector<double>> Q(s.mats.size());
+    auto E = [&](std::vector<double>& a, std::vector<double>& b, bool with_q) {
+        for (size_t m = 0; m < s.mats.size(); ++m) {
+            if (nodes[m].empty()) continue;
+            const Solver& t = *s.mats[m];
+            pack_nodes(a, pa, nodes[m], Ng, s.nyz, s.nang);
+            pack_nodes(b, pb, nodes[m], Ng, s.nyz, s.nang);
+            energy_half(pa.data(), pb.data(), t.S.data(), t.sigma.data(), t.T.data(), half, d.dg,
+                        t.p.energy_density, Ng, static_cast<long long>(nodes[m].size()), s.nang,
+                        with_q ? Q[m].data() : nullptr, nullptr, strag);
+            unpack_nodes(a, pa, nodes[m], Ng, s.nyz, s.nang);
+            unpack_nodes(b, pb, nodes[m], Ng, s.nyz, s.nang);
+        }
+    };
+    auto Tr = [&](std::vector<double>& a, bool preserve) {
+        transport_half(a.data(), s.om.data(), g.Ny, g.Nz, Ng, s.nang, d.dy, d.dz, half, preserve);
+    };
+    auto A = [&](std::vector<double>& a) {
+        for (size_t m = 0; m < s.mats.size(); ++m) {
+            if (nodes[m].empty()) continue;
+            const Solver& t = *s.mats[m];
+            pack_nodes(a, pa, nodes[m], Ng, s.nyz, s.nang);
+            angle_full(pa.data(), t.sig_tr.data(), static_cast<long long>(nodes[m].size()), Ng, g.Nmu,
+                       g.Nom, d.du, d.dv, d.dt, t.p.density);
+            unpack_nodes(a, pa, nodes[m], Ng, s.nyz, s.nang);
+        }
+    };
+    const auto& p = s.p;
+    if (p.energy_only) {
+        E(s.F, s.f, false);
+        E(s.F, s.f, false);
+    } else {
+        E(s.F, s.f, false);
+        if (!p.no_transport) { Tr(s.F, p.no_spatial_clipping != 0); Tr(s.f, true); }
+        if (!p.no_angle) { A(s.F); A(s.f); }
+        if (!p.no_transport) { Tr(s.F, p.no_spatial_clipping != 0); Tr(s.f, true); }
+        E(s.F, s.f, false);
+    }
+    if (!p.energy_only && !p.primary_only && s.step > 0) {
+        auto src = g_direct_source ? secondary_source : secondary_source_factorized;
+        for (size_t m = 0; m < s.mats.size(); ++m) {
+            if (nodes[m].empty()) continue;
+            const Solver& t = *s.mats[m];
+            pack_nodes(s.F, pa, nodes[m], Ng, s.nyz, s.nang);
+            pack_nodes(s.f, pb, nodes[m], Ng, s.nyz, s.nang);
+            Q[m].assign(pa.size(), 0.0);
+            src(pa.data(), pb.data(), Q[m].data(), t.ke1.data(), t.ke2.data(), t.kv.data(),
+                t.beg.data(), t.end.data(), Ng, static_cast<long long>(nodes[m].size()), s.nang, d.dg);
+        }
+        E(s.F1, s.f1, true);
+        Tr(s.F1, false);
+        Tr(s.f1, true);
+        A(s.F1);
+        A(s.f1);
+        Tr(s.F1, false);
+        Tr(s.f1, true);
+        E(s.F1, s.f1, true);
+    }
+    double v = 0.0;
+    for (size_t m = 0; m < s.mats.size(); ++m) {
+        if (nodes[m].empty()) continue;
+        const Solver& t = *s.mats[m];
+        v += idd_value(s.F.data(), s.f.data(), s.F1.data(), s.f1.data(), t.S.data(), t.sigma.data(),
+                       t.en.data(), Ng, g.Ny, g.Nz, s.nang, d.dg, d.du, d.dv, d.dy, d.dz, t.p.density,
+                       p.trapezoidal_yz != 0, p.dose_catastrophic_loss != 0, &nodes[m]);
+    }
+    double px = 0.0;
+    for (size_t k = 0; k < s.N; ++k) px += s.f[k];
+    s.idd.push_back(v);
+    s.proxy.push_back(px * d.dg);
+    s.step++;
+}
+
 // One depth step: BFPnSolver::solve loop body (src/bfpm_solver.cu:2189-2255)
 // with stepPrimary (3238-3311), stepPrimaryEnergyOnly (3313-3333),
 // stepSecondary (3335-3394), computeIntegratedDepthDose (3473-3502) and
 // computeScalarDoseProxy (3418-3444).
 void solver_step(Solver& s) {
+    if (!s.mats.empty()) return solver_step_mapped(s);
     const auto& g = s.g;
     const Solver& t = tables_of_step(s);  // material tables and density of this step
     const auto& p = t.p;
@@ -1119,6 +1240,48 @@ BFPN_API void* orc_create_layered(const bfpn_grid* g, const bfpn_phys* p, int n,
         s->layers.push_back(std::move(layer));
         s->x_end.push_back(x_end[k]);
     }
+    return s;
+}
+
+// A solve under a material map: materials k (material_k, density_k, table
+// directory path_k); depth slab j up to slab_end[j] cm (the last: any
+// depth); map[j (Ny+1)(Nz+1) + s] the material index of node s in slab j.
+BFPN_API void* orc_create_mapped(const bfpn_grid* g, const bfpn_phys* p, int nmat,
+                                 const char* const* materials, const double* densities,
+                                 const char* const* paths, int nslabs, const double* slab_end,
+                                 const unsigned char* map) {
+    if (nmat < 1 || nslabs < 1) {
+        g_err = "a map needs materials and slabs";
+        return nullptr;
+    }
+    auto* s = new Solver();
+    if (!solver_init(*s, *g, *p, paths[0])) {
+        delete s;
+        return nullptr;
+    }
+    for (int k = 0; k < nmat; ++k) {
+        bfpn_phys q = *p;
+        std::snprintf(q.material_name, sizeof(q.material_name), "%s", materials[k]);
+        q.density = q.energy_density = densities[k];
+        auto mat = std::make_unique<Solver>();
+        if (!solver_init(*mat, *g, q, paths[k])) {
+            delete s;
+            return nullptr;
+        }
+        mat->F.clear();  // tables only
+        mat->f.clear();
+        mat->F1.clear();
+        mat->f1.clear();
+        s->mats.push_back(std::move(mat));
+    }
+    s->slab_end.assign(slab_end, slab_end + nslabs);
+    s->map.assign(map, map + static_cast<size_t>(nslabs) * s->nyz);
+    for (unsigned char m : s->map)
+        if (m >= nmat) {
+            g_err = "map entry names no material";
+            delete s;
+            return nullptr;
+        }
     return s;
 }
 
