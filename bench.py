@@ -44,7 +44,10 @@ Other workloads (--workload), each one JSON line:
             of the maximum and BP/D90/D20 shifts (gate 0.5 %, 0.1 mm); with
             --slab A:B:MATERIAL[:RHO] the beam crosses that slab in water
             (depth layers), the metrics are taken behind it and its D90 shift
-            against the homogeneous water run is reported
+            against the homogeneous water run is reported; with
+            --insert Y0:Y1:Z0:Z1:X0:X1:MATERIAL[:RHO] a box of that material
+            sits in water (a material map), and the mapped step's time is
+            reported beside the homogeneous one
   copy      the plain device copy used to calibrate the traffic counters
 """
 from __future__ import annotations
@@ -439,13 +442,28 @@ def run_accuracy(args):
         water = ("water", 1.0, material_dir("water"))
         layers = [(slab[0], *water), (slab[1], slab[2], slab[3], material_dir(slab[2])),
                   (depth + 100.0, *water)]
+    mapped, insert = None, None
+    if args.insert:  # Y0:Y1:Z0:Z1:X0:X1:MATERIAL[:RHO], a box of MATERIAL in water
+        f = args.insert.split(":")
+        presets = {"water": 1.0, "bone": 1.85, "air": 0.001205}
+        insert = tuple(float(v) for v in f[:6]) + (f[6], float(f[7]) if len(f) > 7 else presets[f[6]])
+        y0, y1, z0, z1, x0, x1, mat, rho = insert
+        yv = -0.5 * case.Ly + np.arange(ny + 1) * case.Ly / ny  # node coordinates
+        zv = -0.5 * case.Lz + np.arange(ny + 1) * case.Lz / ny
+        inside = ((yv[None, :] >= y0) & (yv[None, :] < y1) & (zv[:, None] >= z0) &
+                  (zv[:, None] < z1)).ravel()  # node s = iz (Ny+1) + iy
+        mp = np.zeros((3, inside.size), dtype=np.uint8)
+        mp[1][inside] = 1
+        water = ("water", 1.0, material_dir("water"))
+        mapped = ([water, (mat, rho, material_dir(mat))], [x0, x1, depth + 100.0], mp)
+        log(f"insert: {int(inside.sum())} of {inside.size} nodes {mat} from {x0} to {x1} cm")
     res = {}
     runs = (("fp32", bg.FP32), ("fp64", bg.FP64))
     if args.only:
         runs = tuple(r for r in runs if r[0] == args.only)
     for name, prec in runs:
         t0 = time.perf_counter()
-        with bg.Solver(case, prec, layers=layers) as s:
+        with bg.Solver(case, prec, layers=layers, mapped=mapped) as s:
             total = case.num_steps()
             done = 0
             while done < total:  # in chunks, with a progress line per chunk
@@ -472,6 +490,8 @@ def run_accuracy(args):
                 "steps": len(idd), "seconds": sec, "metrics_cm": metrics.analyze(x, idd)}
     if slab:
         return slab_result(args, case, slab, res, ny, nx, depth)
+    if insert:
+        return insert_result(args, case, insert, res, ny, nx, depth)
     err = idd_error(res["fp32"][0], res["fp64"][0], case.dt)
 
     x = (np.arange(len(res["fp64"][0])) + 1) * case.dt
@@ -483,6 +503,48 @@ def run_accuracy(args):
             "max_rel_err_above_10pct": err["max_rel"], "max_shift_mm_BP_D90_D20": err["max_shift_mm"],
             "fp64_metrics_cm": metrics.analyze(x, res["fp64"][0]),
             "seconds": {"fp32": res["fp32"][1], "fp64": res["fp64"][1]},
+            "gate": {"max_rel_err": 0.005, "max_shift_mm": 0.1,
+                     "pass": bool(err["max_rel"] <= 0.005 and err["max_shift_mm"] <= 0.1)}}
+
+
+def insert_result(args, case, insert, res, ny, nx, depth):
+    """FP32 vs FP64 with a material insert (a material map), the cost of a
+    mapped step against the homogeneous one, and the D90 shift the insert
+    causes."""
+    import numpy as np
+
+    import bfpn_gpu_amd as bg
+    from bfpn_gpu_amd import metrics
+
+    y0, y1, z0, z1, x0, x1, mat, rho = insert
+    err = idd_error(res["fp32"][0], res["fp64"][0], case.dt)
+    n = case.num_steps()
+    x = (np.arange(n) + 1) * case.dt
+    times = {}
+    for label, skip in (("default", True), ("every group, one stream", False)):
+        with bg.Solver(case, bg.FP32) as s:
+            s.set_group_skip(skip)
+            s.set_overlap(skip)
+            t0 = time.perf_counter()
+            s.advance(n)
+            water, _ = s.history()
+            times[label] = time.perf_counter() - t0
+    log(f"homogeneous water: {times}")
+    mw, m64 = metrics.analyze(x, water), metrics.analyze(x, res["fp64"][0])
+    return {"metric": "IDD error of the FP32-state solver vs the FP64 instantiation with a "
+                      "material insert",
+            "config": {"workload": f"{args.grid} grid ({ny + 1}x{ny + 1} nodes), water 230 MeV "
+                                   f"with a {mat} box (rho {rho}) over y [{y0}, {y1}), "
+                                   f"z [{z0}, {z1}) cm from depth {x0} to {x1} cm, depth {depth} "
+                                   f"cm, Nx {nx}, Ng 500, 20x20 directions, eq15 + straggling + "
+                                   f"secondary source", "cells": case.cells},
+            "max_rel_err_above_10pct": err["max_rel"], "max_shift_mm_BP_D90_D20": err["max_shift_mm"],
+            "fp64_metrics_cm": m64,
+            "d90_shift_vs_homogeneous_water_cm": float(m64["D90"] - mw["D90"]),
+            "seconds": {"fp32 mapped": res["fp32"][1], "fp64 mapped": res["fp64"][1],
+                        **{f"fp32 homogeneous, {k}": v for k, v in times.items()}},
+            "ms_per_step": {"fp32 mapped": 1e3 * res["fp32"][1] / n,
+                            **{f"fp32 homogeneous, {k}": 1e3 * v / n for k, v in times.items()}},
             "gate": {"max_rel_err": 0.005, "max_shift_mm": 0.1,
                      "pass": bool(err["max_rel"] <= 0.005 and err["max_shift_mm"] <= 0.1)}}
 
@@ -599,6 +661,10 @@ def main():
     ap.add_argument("--depth", type=float, default=None,
                     help="accuracy: depth in cm (default 40 water, 22 bone)")
     ap.add_argument("--save", default=None, help="directory for the accuracy IDD arrays")
+    ap.add_argument("--insert", default=None,
+                    help="accuracy: Y0:Y1:Z0:Z1:X0:X1:MATERIAL[:RHO], a box of MATERIAL in water "
+                         "(transverse node coordinates in cm, centred on the beam; depth X0 to "
+                         "X1; write --insert=... when Y0 is negative)")
     ap.add_argument("--slab", default=None,
                     help="accuracy: A:B:MATERIAL[:RHO], a slab from A to B cm in water")
     ap.add_argument("--only", choices=("fp32", "fp64"), default=None,

@@ -78,21 +78,26 @@ __global__ void sum_in_order_kernel(const double* __restrict__ v, int n, double*
 }
 
 // One node's directions are contiguous, so each block copies runs of nang
-// values; grid (ceil(n nang / kBlock), Ng).
-template <typename T, bool GATHER>
+// values, in units of U (a 16-byte vector when nang allows, else one value);
+// grid (ceil(n nang / (U kBlock)), Ng).
+template <typename T, typename U, bool GATHER>
 __global__ __launch_bounds__(kBlock) void node_copy_kernel(T* __restrict__ dst, const T* __restrict__ src,
                                                            const int* __restrict__ nodes, int n,
                                                            long long nyz, int nang) {
-    const long long idx = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
-    const long long per = static_cast<long long>(n) * nang;
+    constexpr int V = sizeof(U) / sizeof(T);
+    const long long idx = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;  // unit
+    const int nu = nang / V;                                                          // units per node
+    const long long per = static_cast<long long>(n) * nu;
     if (idx >= per) return;
-    const long long k = idx / nang, a = idx - k * nang, g = blockIdx.y;
+    const long long k = idx / nu, a = idx - k * nu, g = blockIdx.y;
+    U* d = reinterpret_cast<U*>(dst);
+    const U* s = reinterpret_cast<const U*>(src);
     const long long packed = g * per + idx;
-    const long long full = (g * nyz + nodes[k]) * nang + a;
+    const long long full = (g * nyz + nodes[k]) * nu + a;
     if (GATHER)
-        dst[packed] = src[full];
+        d[packed] = s[full];
     else
-        dst[full] = src[packed];
+        d[full] = s[packed];
 }
 
 template <typename T>
@@ -305,12 +310,20 @@ void launch_sum_in_order(const double* v, int n, double* out, hipStream_t s) {
 template <typename T, bool GATHER>
 static void node_copy(T* dst, const T* src, const int* nodes, int n, long long nyz, int nang, int Ng,
                       hipStream_t s) {
-    const long long per = static_cast<long long>(n) * nang;
+    typedef __attribute__((ext_vector_type(16 / sizeof(T)))) T vec_t;
+    constexpr int V = 16 / sizeof(T);
+    const bool vec = nang % V == 0 && reinterpret_cast<uintptr_t>(dst) % 16 == 0 &&
+                     reinterpret_cast<uintptr_t>(src) % 16 == 0;
+    const long long per = static_cast<long long>(n) * (vec ? nang / V : nang);
     if (per <= 0 || Ng <= 0) return;
     BFPN_REQUIRE(Ng <= 65535 && (per + kBlock - 1) / kBlock < (1LL << 31), "node copy grid too large");
     const dim3 grid(static_cast<unsigned>((per + kBlock - 1) / kBlock), static_cast<unsigned>(Ng));
-    hipLaunchKernelGGL((node_copy_kernel<T, GATHER>), grid, dim3(kBlock), 0, s, dst, src, nodes, n,
-                       nyz, nang);
+    if (vec)
+        hipLaunchKernelGGL((node_copy_kernel<T, vec_t, GATHER>), grid, dim3(kBlock), 0, s, dst, src,
+                           nodes, n, nyz, nang);
+    else
+        hipLaunchKernelGGL((node_copy_kernel<T, T, GATHER>), grid, dim3(kBlock), 0, s, dst, src, nodes,
+                           n, nyz, nang);
     BFPN_HIP(hipGetLastError());
 }
 

@@ -505,8 +505,9 @@ public:
                     lists.push_back(s);
                     weights.push_back(w[s]);
                 }
-                max_nodes = std::max<long long>(max_nodes, n);
             }
+            if (sl.mat.size() > 1)  // packed copies only where materials mix
+                for (int n : sl.count) max_nodes = std::max<long long>(max_nodes, n);
             pslot = std::max<long long>(pslot, poff);
             slabs_.push_back(std::move(sl));
         }
@@ -1011,6 +1012,9 @@ private:
         const bool secondary = P_->secondary;
         const bool preserve_primary = phys.no_spatial_clipping != 0;
         const bool strag = phys.eq15_straggling != 0;
+        // a slab of one material lists every node in order: its operators
+        // work on the state itself, as in a homogeneous run
+        const bool whole = nk == 1;
         T* cA = cbuf_[0].as<T>();
         T* cB = cbuf_[1].as<T>();
         auto use = [&](int k) { P_ = mats_[sl.mat[k]]; };
@@ -1019,11 +1023,15 @@ private:
         auto lanes = [&](int k) { return static_cast<long long>(sl.count[k]) * P_->nang; };
         auto packed = [&](const DeviceBuffer& b, int k) { return b.as<T>() + sl.off[k]; };
         auto part = [&](double* base, int k) { return base ? base + sl.poff[k] : nullptr; };
+        // the packed arrays of material k: gathered from / scattered to state w
         auto gather = [&](int k, T* c, int w) {
+            if (whole) return cur_[w];
             launch_gather_nodes<T>(c, cur_[w], nodes(k), sl.count[k], P_->nyz, P_->nang, Ng, s);
+            return c;
         };
         auto scatter = [&](int k, const T* c, int w) {
-            launch_scatter_nodes<T>(cur_[w], c, nodes(k), sl.count[k], P_->nyz, P_->nang, Ng, s);
+            if (!whole)
+                launch_scatter_nodes<T>(cur_[w], c, nodes(k), sl.count[k], P_->nyz, P_->nang, Ng, s);
         };
         auto tops = [&] {
             SweepTops t;
@@ -1034,13 +1042,13 @@ private:
         auto half = [&](int sp, const DeviceBuffer* Q, double* partials, bool proxy) {
             for (int k = 0; k < nk; ++k) {
                 use(k);
-                gather(k, cA, 2 * sp);
-                gather(k, cB, 2 * sp + 1);
-                launch_energy_half<T>(cA, cB, P_->coef.template as<EnergyCoef>(), Q ? packed(*Q, k) : nullptr,
+                T* a = gather(k, cA, 2 * sp);
+                T* b = gather(k, cB, 2 * sp + 1);
+                launch_energy_half<T>(a, b, P_->coef.template as<EnergyCoef>(), Q ? packed(*Q, k) : nullptr,
                                       Ng, lanes(k), strag, yzw(k), P_->nang, part(partials, k), proxy, s,
                                       tops());
-                scatter(k, cA, 2 * sp);
-                scatter(k, cB, 2 * sp + 1);
+                scatter(k, a, 2 * sp);
+                scatter(k, b, 2 * sp + 1);
             }
             mark(kEnergy, array_bytes() * (Q ? 5 : 4), s);
         };
@@ -1049,15 +1057,15 @@ private:
                         bool proxy, bool xout) {
             for (int k = 0; k < nk; ++k) {
                 use(k);
-                gather(k, cA, 2 * sp);
-                gather(k, cB, 2 * sp + 1);
-                launch_energy_pair<T>(cA, cB, P_->coef.template as<EnergyCoef>(),
+                T* a = gather(k, cA, 2 * sp);
+                T* b = gather(k, cB, 2 * sp + 1);
+                launch_energy_pair<T>(a, b, P_->coef.template as<EnergyCoef>(),
                                       Qa ? packed(*Qa, k) : nullptr, Qb ? packed(*Qb, k) : nullptr, Ng,
                                       lanes(k), strag, yzw(k), P_->nang, part(partials, k), proxy,
                                       xout ? packed(x_[0], k) : nullptr, xout ? packed(x_[1], k) : nullptr,
                                       s, tops(), x_rows());
-                scatter(k, cA, 2 * sp);
-                scatter(k, cB, 2 * sp + 1);
+                scatter(k, a, 2 * sp);
+                scatter(k, b, 2 * sp + 1);
             }
             mark(kEnergy, array_bytes() * (4 + (Qa ? 2 : 0) + (xout ? 2 : 0)), s);
         };
@@ -1069,20 +1077,20 @@ private:
         auto ang = [&](int w) {
             for (int k = 0; k < nk; ++k) {
                 use(k);
-                gather(k, cA, w);
+                T* a = gather(k, cA, w);
                 bool done = false;
                 if constexpr (std::is_same_v<T, float>) {
                     if (P_->angle_plan.plan.su_h) {
-                        launch_angle_mfma(cA, P_->factors.template as<float>(), P_->angle_plan.plan, Ng,
+                        launch_angle_mfma(a, P_->factors.template as<float>(), P_->angle_plan.plan, Ng,
                                           sl.count[k], grid.Nmu, grid.Nom, s, nullptr);
                         done = true;
                     }
                 }
                 if (!done)
-                    launch_angle_full<T>(cA, P_->factors.template as<T>(), P_->su.template as<T>(),
+                    launch_angle_full<T>(a, P_->factors.template as<T>(), P_->su.template as<T>(),
                                          P_->sv.template as<T>(), Ng, sl.count[k], grid.Nmu, grid.Nom, s,
                                          nullptr);
-                scatter(k, cA, w);
+                scatter(k, a, w);
             }
             mark(kAngle, array_bytes() * 2, s);
         };
@@ -1148,9 +1156,9 @@ private:
             for (int k = 0; k < nk; ++k) {  // the source of step n
                 use(k);
                 if (last) {  // from the state; otherwise from the end-of-step copy
-                    gather(k, cA, 0);
-                    gather(k, cB, 1);
-                    src(k, cA, cB, packed(Q, k));
+                    const T* a = gather(k, cA, 0);
+                    const T* b = gather(k, cB, 1);
+                    src(k, a, b, packed(Q, k));
                 } else {
                     src(k, packed(x_[0], k), packed(x_[1], k), packed(Q, k));
                 }
