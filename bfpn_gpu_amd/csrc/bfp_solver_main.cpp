@@ -134,6 +134,61 @@ LayerArg parse_layer(const std::string& text) {
     return l;
 }
 
+// --material-map FILE (no counterpart in the reference): a text file of
+//   material <name> [<rho> [<data>]]   one line per material, index = order
+//   slab <x_end | inf>                 then Nz+1 rows of Ny+1 material indices
+//                                      (row iz, column iy)
+// '#' starts a comment.  Slabs in increasing depth; the last covers any depth.
+struct MaterialMap {
+    std::vector<LayerArg> materials;
+    std::vector<double> slab_end;
+    std::vector<unsigned char> map;  // slab-major, node s = iz (Ny+1) + iy
+};
+
+MaterialMap read_material_map(const std::string& path, int ny, int nz) {
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("Cannot open material map " + path);
+    MaterialMap m;
+    const size_t nyz = static_cast<size_t>(ny + 1) * (nz + 1);
+    size_t row_values = 0;  // indices read for the current slab
+    for (std::string line; std::getline(in, line);) {
+        line = line.substr(0, line.find('#'));
+        std::istringstream ls(line);
+        std::string head;
+        if (!(ls >> head)) continue;
+        if (head == "material") {
+            LayerArg l{0.0, "", "", 0.0};
+            if (!(ls >> l.material)) throw std::runtime_error("material line without a name in " + path);
+            std::string rho;
+            l.rho = (ls >> rho) ? std::stod(rho) : preset_density(l.material);
+            if (!(ls >> l.data)) l.data = "BFPn_CPU_Solver/" + l.material;
+            m.materials.push_back(l);
+        } else if (head == "slab") {
+            if (!m.slab_end.empty() && row_values != nyz)
+                throw std::runtime_error("slab " + std::to_string(m.slab_end.size()) + " of " + path +
+                                         " has " + std::to_string(row_values) + " indices, not " +
+                                         std::to_string(nyz));
+            std::string x;
+            if (!(ls >> x)) throw std::runtime_error("slab line without a depth in " + path);
+            m.slab_end.push_back(x == "inf" ? 1e300 : std::stod(x));
+            row_values = 0;
+        } else {
+            if (m.slab_end.empty()) throw std::runtime_error("map values before a slab line in " + path);
+            std::istringstream vs(line);
+            for (long v; vs >> v;) {
+                if (v < 0 || v >= static_cast<long>(m.materials.size()) || row_values == nyz)
+                    throw std::runtime_error("bad map entry " + std::to_string(v) + " in " + path);
+                m.map.push_back(static_cast<unsigned char>(v));
+                ++row_values;
+            }
+        }
+    }
+    if (m.slab_end.empty() || row_values != nyz || m.materials.empty())
+        throw std::runtime_error("material map " + path + " needs materials and complete slabs of " +
+                                 std::to_string(nyz) + " indices");
+    return m;
+}
+
 void usage(const char* argv0) {
     std::cout << "Usage: " << argv0 << " [options]\n"
               << "Options:\n"
@@ -173,6 +228,9 @@ void usage(const char* argv0) {
               << "                   (repeat in increasing depth; the last extends to any depth;\n"
               << "                   rho defaults to the material preset, data to\n"
               << "                   BFPn_CPU_Solver/<material>)\n"
+              << "  --material-map <file> Materials per y-z node and depth slab: lines\n"
+              << "                   'material <name> [<rho> [<data>]]', then per slab\n"
+              << "                   'slab <x_end|inf>' and Nz+1 rows of Ny+1 indices\n"
               << "  --streaming-full Accepted; the state always stays resident in HBM\n"
               << "  --precision <fp32|fp64> State precision (default: fp32)\n"
               << "  --lg <MeV>       Energy span of the group grid [1, 1+Lg] (default: 259)\n"
@@ -196,6 +254,7 @@ int main(int argc, char** argv) {
     std::vector<double> spot_depths;
     std::string spot_prefix = "spot";
     std::vector<LayerArg> layers;
+    std::string map_path;
     bfpn_phys phys{};
     phys.dose_catastrophic_loss = 1;
 
@@ -264,6 +323,7 @@ int main(int argc, char** argv) {
                 }
             } else if (a == "--spot-depths" && has) spot_depths = parse_depth_list(argv[++i]);
             else if (a == "--layer" && has) layers.push_back(parse_layer(argv[++i]));
+            else if (a == "--material-map" && has) map_path = argv[++i];
             else if (a == "--spot-prefix" && has) spot_prefix = argv[++i];
             else if ((a == "--lane-chunk" || a == "--energy-chunk" || a == "--stream-dir" ||
                       a == "--idd-stride") && has)
@@ -326,10 +386,36 @@ int main(int argc, char** argv) {
         std::cout << "Layer: to x=" << l.x_end << " cm, " << l.material << ", rho=" << l.rho
                   << ", data " << l.data << std::endl;
     }
-    const bfpn_status created =
-        spec.empty() ? bfpn_create(&grid, &phys, data_path.c_str(), precision, &ctx)
-                     : bfpn_create_layered(&grid, &phys, spec.data(), static_cast<int>(spec.size()),
-                                           precision, &ctx);
+    bfpn_status created;
+    if (!map_path.empty()) {
+        if (!spec.empty()) {
+            std::cerr << "Error: --layer and --material-map exclude each other" << std::endl;
+            return 1;
+        }
+        MaterialMap mm;
+        try {
+            mm = read_material_map(map_path, ny, nz);
+        } catch (const std::exception& e) {
+            std::cerr << "Error: " << e.what() << std::endl;
+            return 1;
+        }
+        std::vector<bfpn_layer> mats;
+        for (const auto& l : mm.materials) {
+            mats.push_back({0.0, l.material.c_str(), l.rho, l.data.c_str()});
+            std::cout << "Map material " << mats.size() - 1 << ": " << l.material << ", rho=" << l.rho
+                      << ", data " << l.data << std::endl;
+        }
+        std::cout << "Material map: " << mm.slab_end.size() << " depth slab(s) from " << map_path
+                  << std::endl;
+        created = bfpn_create_mapped(&grid, &phys, mats.data(), static_cast<int>(mats.size()),
+                                     mm.slab_end.data(), static_cast<int>(mm.slab_end.size()),
+                                     mm.map.data(), precision, &ctx);
+    } else {
+        created = spec.empty()
+                      ? bfpn_create(&grid, &phys, data_path.c_str(), precision, &ctx)
+                      : bfpn_create_layered(&grid, &phys, spec.data(), static_cast<int>(spec.size()),
+                                            precision, &ctx);
+    }
     if (created != BFPN_OK) {
         std::cerr << "Error: " << bfpn_last_error() << std::endl;
         return 1;

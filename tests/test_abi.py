@@ -105,3 +105,55 @@ def test_layered_create_rejects_bad_stacks_before_any_device_call(stack, msg):
 def test_cli_rejects_bad_layers(arg, msg):
     r = _cli("--layer", arg)
     assert r.returncode == 1 and msg in r.stderr
+
+
+def _map_file(tmp_path, text):
+    p = tmp_path / "map.txt"
+    p.write_text(text)
+    return str(p)
+
+
+def test_cli_rejects_bad_material_maps(tmp_path):
+    """--material-map files are read before any device call (default grid:
+    11 x 11 nodes, so a slab needs 121 indices)."""
+    full = "material water\nslab inf\n" + ("0 " * 11 + "\n") * 11
+    cases = [
+        ("no such file", None, "Cannot open material map"),
+        ("index beyond the materials", full.replace("0 ", "1 ", 1), "bad map entry 1"),
+        ("short slab", "material water\nslab inf\n0 0 0\n", "complete slabs of 121"),
+        ("values before a slab", "material water\n0 0\n", "before a slab line"),
+        ("unknown material", "material steel\nslab inf\n0\n", "Unknown material: steel"),
+    ]
+    for what, text, msg in cases:
+        path = str(tmp_path / "missing.txt") if text is None else _map_file(tmp_path, text)
+        r = _cli("--material-map", path)
+        assert r.returncode == 1 and msg in r.stderr, (what, r.stderr)
+    r = _cli("--material-map", _map_file(tmp_path, full), "--layer", "2:water")
+    assert r.returncode == 1 and "exclude each other" in r.stderr
+
+
+@pytest.mark.parametrize("nmat,ends,entry,msg", [
+    (0, [9.0], 0, b"null argument"),
+    (1, [9.0], 1, b"names no material"),
+    (2, [2.0, 1.0, 9.0], 0, b"must increase"),
+    (2, [-1.0, 9.0], 0, b"must be positive"),
+])
+def test_mapped_create_rejects_bad_maps_before_any_device_call(nmat, ends, entry, msg):
+    import ctypes as C
+
+    import numpy as np
+
+    import bfpn_gpu_amd as bg
+    from bfpn_gpu_amd.abi import Case, Layer
+
+    L = bg.lib()
+    c = Case()
+    g, p = c.grid(), c.phys()
+    mats = (Layer * 2)(Layer(0.0, b"water", 1.0, b"./data"), Layer(0.0, b"bone", 1.85, b"./data"))
+    e = np.array(ends, dtype=np.float64)
+    mp = np.full(len(ends) * c.nyz, entry, dtype=np.uint8)
+    h = C.c_void_p(0)
+    st = L.bfpn_create_mapped(C.byref(g), C.byref(p), mats, nmat, e.ctypes.data, len(ends),
+                              mp.ctypes.data, 0, C.byref(h))
+    assert st == 1 and not h.value
+    assert msg in L.bfpn_last_error()

@@ -143,3 +143,36 @@ def test_bad_maps_are_refused(gpu):
                    mapped=([WATER], [100.0], np.zeros((1, case.nyz))))
     with pytest.raises(gpu.BfpnError, match="slab depths must increase"):
         gpu.Solver(case, gpu.FP32, mapped=([WATER], [0.5, 0.4, 9.0], np.zeros((3, case.nyz))))
+
+
+def test_cli_material_map_writes_the_oracle_values(tmp_path, gpu, oracle_mod):
+    import os
+    import subprocess
+
+    from conftest import ROOT
+
+    case = case_of(time=2.4)
+    _, mp = split_map(case)
+    slabs = np.vstack([np.zeros_like(mp), mp])
+    w, b = material_dir("water"), material_dir("bone")
+
+    def rows(slab):
+        return "".join(" ".join(str(int(v)) for v in slab[iz * (case.Ny + 1):(iz + 1) * (case.Ny + 1)])
+                       + "\n" for iz in range(case.Nz + 1))
+
+    path = tmp_path / "phantom.txt"
+    path.write_text(f"# water, then a bone half from 0.5 cm\nmaterial water 1.0 {w}\n"
+                    f"material bone 1.85 {b}\nslab 0.5\n{rows(slabs[0])}slab inf\n{rows(slabs[1])}")
+    cli = os.path.join(ROOT, "bfpn_gpu_amd", "bin", "bfp_solver")
+    cmd = [cli, str(case.Nx), "--data", case.data, "--material", "water", "--energy", "100",
+           "--ny", str(case.Ny), "--nz", str(case.Nz), "--nmu", str(case.Nmu), "--nom",
+           str(case.Nom), "--ly", "4", "--lz", "4", "--lu", "2", "--lv", "2", "--ng", "200",
+           "--sigma-yz", "0.5", "--eq15-straggling", "--trapezoidal-yz", "--time", "2.4",
+           "--precision", "fp64", "--material-map", str(path)]
+    r = subprocess.run(cmd, cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "Material map: 2 depth slab(s)" in r.stdout
+    out = np.loadtxt(tmp_path / "idd_output.txt")
+    o = oracle_mod.Solver(case, mapped=([WATER, BONE], [0.5, 100.0], slabs))
+    o.advance(case.num_steps())
+    np.testing.assert_allclose(out[:, 1], o.history()[0], rtol=1e-10)
