@@ -109,6 +109,9 @@ def lib():
                                      C.POINTER(C.c_uint64)]
     L.bfpn_server_submit_layered.argtypes = [vp, C.POINTER(Grid), C.POINTER(Phys),
                                             C.POINTER(Layer), i, d, C.POINTER(C.c_uint64)]
+    L.bfpn_server_submit_mapped.argtypes = [vp, C.POINTER(Grid), C.POINTER(Phys),
+                                            C.POINTER(Layer), i, vp, i, vp, d,
+                                            C.POINTER(C.c_uint64)]
     L.bfpn_server_poll.argtypes = [vp, C.c_uint64]
     L.bfpn_server_wait.argtypes = [vp, C.c_uint64, d, vp, i, C.POINTER(C.c_int), vp]
     L.bfpn_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
@@ -361,12 +364,25 @@ class Server:
         check(lib().bfpn_server_create(C.byref(opts), C.byref(h)))
         self._h = h
 
-    def submit(self, case: Case, t_final: float | None = None, layers=None) -> int:
-        """Queues one beam; layers as in Solver (bfpn_server_submit_layered)."""
+    def submit(self, case: Case, t_final: float | None = None, layers=None, mapped=None) -> int:
+        """Queues one beam; layers and mapped as in Solver
+        (bfpn_server_submit_layered, bfpn_server_submit_mapped)."""
         g, p = case.grid(), case.phys()
         rid = C.c_uint64(0)
         t = case.time if t_final is None else t_final
-        if layers:
+        if mapped is not None:
+            materials, slab_end, mp = mapped
+            n = len(materials)
+            spec = (Layer * n)(*[Layer(0.0, m.encode(), float(rho), path.encode())
+                                 for m, rho, path in materials])
+            ends = np.ascontiguousarray(slab_end, dtype=np.float64)
+            mp = np.ascontiguousarray(mp, dtype=np.uint8)
+            if mp.size != len(ends) * case.nyz:
+                raise ValueError("the map needs (Ny+1)(Nz+1) material indices per slab")
+            check(lib().bfpn_server_submit_mapped(self._h, C.byref(g), C.byref(p), spec, n,
+                                                  _addr(ends), len(ends), _addr(mp), t,
+                                                  C.byref(rid)))
+        elif layers:
             n = len(layers)
             spec = (Layer * n)(*[Layer(float(x), m.encode(), float(rho), path.encode())
                                  for x, m, rho, path in layers])

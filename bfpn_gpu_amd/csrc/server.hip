@@ -56,7 +56,9 @@ struct Request {
     int steps = 0;
     size_t ctx_bytes = 0;
     Clock::time_point t_submit, t_admit, t_done;
-    std::vector<LayerSpec> layers;  // empty: one material (data)
+    std::vector<LayerSpec> layers;  // empty: one material (data); with a map: its materials
+    std::vector<double> slab_end;   // material map (empty: none), as in bfpn_create_mapped
+    std::vector<unsigned char> map;
     std::unique_ptr<bfpn_ctx> ctx;
     std::vector<std::shared_ptr<PhysicsBase>> physics;  // one table set per layer
     int issued = 0;
@@ -212,6 +214,9 @@ struct bfpn_server {
                 }
                 if (req->layers.empty())
                     req->ctx.reset(make_context(req->physics.front(), req->grid, req->phys));
+                else if (!req->map.empty())
+                    req->ctx.reset(make_mapped_context(req->grid, req->phys, req->layers,
+                                                       req->slab_end, req->map, req->physics));
                 else
                     req->ctx.reset(make_layered_context(req->grid, req->phys, req->layers,
                                                         req->physics));
@@ -396,22 +401,36 @@ BFPN_API bfpn_status bfpn_server_create(const bfpn_server_opts* opts, bfpn_serve
 }
 
 namespace {
-// Queues a request (layers empty: one material, tables from data_path).
+// Queues a request (layers empty: one material, tables from data_path; with
+// a map, layers holds its materials).
 void submit_request(bfpn_server* s, const bfpn_grid* grid, const bfpn_phys* phys,
                     const char* data_path, std::vector<LayerSpec> layers, double t_final,
-                    uint64_t* id) {
+                    uint64_t* id, std::vector<double> slab_end = {},
+                    std::vector<unsigned char> map = {}) {
     BFPN_REQUIRE(s && grid && phys && id, "null argument");
     validate_params(*grid, *phys);
-    if (!layers.empty()) validate_layers(layers);
+    const long long nyz = static_cast<long long>(grid->Ny + 1) * (grid->Nz + 1);
+    if (!map.empty()) {
+        validate_map(layers, slab_end, map, nyz);
+        BFPN_REQUIRE(!phys->legacy_energy && !phys->lite_memory,
+                     "material maps support the eq15 energy model with the full state only");
+    } else if (!layers.empty()) {
+        validate_layers(layers);
+    }
     auto r = std::make_shared<Request>();
     r->grid = *grid;
     r->phys = *phys;
     r->data = data_path ? data_path : "./data";
     r->layers = std::move(layers);
+    r->slab_end = std::move(slab_end);
+    r->map = std::move(map);
     const double dt = grid->Lx / grid->Nx;
     r->steps = static_cast<int>(std::ceil(t_final / dt - 1e-12));
     BFPN_REQUIRE(r->steps >= 1, "t_final must cover at least one depth step");
     r->ctx_bytes = context_bytes(*grid, *phys, s->opts.precision);
+    if (!r->map.empty())  // two packed arrays for the mixed slabs, at most the state's size
+        r->ctx_bytes += 2 * static_cast<size_t>(grid->Ng) * nyz * grid->Nmu * grid->Nom *
+                        (s->opts.precision == BFPN_FP64 ? 8 : 4);
     r->t_submit = Clock::now();
     std::lock_guard<std::mutex> g(s->mu);
     BFPN_REQUIRE(!s->stopping, "server is shutting down");
@@ -435,6 +454,25 @@ BFPN_API bfpn_status bfpn_server_submit_layered(bfpn_server* s, const bfpn_grid*
                             layers[k].data_path});
         }
         submit_request(s, grid, phys, nullptr, std::move(spec), t_final, id);
+    });
+}
+
+BFPN_API bfpn_status bfpn_server_submit_mapped(bfpn_server* s, const bfpn_grid* grid,
+                                               const bfpn_phys* phys, const bfpn_layer* materials,
+                                               int nmat, const double* slab_end, int nslabs,
+                                               const unsigned char* map, double t_final,
+                                               uint64_t* id) {
+    return guarded([&] {
+        BFPN_REQUIRE(grid && materials && slab_end && map && nmat > 0 && nslabs > 0, "null argument");
+        std::vector<LayerSpec> mats;
+        for (int k = 0; k < nmat; ++k) {
+            BFPN_REQUIRE(materials[k].material && materials[k].data_path, "null material field");
+            mats.push_back({0.0, materials[k].material, materials[k].density, materials[k].data_path});
+        }
+        const long long nyz = static_cast<long long>(grid->Ny + 1) * (grid->Nz + 1);
+        submit_request(s, grid, phys, nullptr, std::move(mats), t_final, id,
+                       std::vector<double>(slab_end, slab_end + nslabs),
+                       std::vector<unsigned char>(map, map + nslabs * nyz));
     });
 }
 

@@ -1318,6 +1318,21 @@ bfpn_ctx* make_mapped_context(const bfpn_grid& g, const bfpn_phys& p,
     std::vector<std::shared_ptr<PhysicsBase>> tables;
     for (const LayerSpec& m : materials)
         tables.push_back(make_physics(g, layer_phys(p, m), m.data_path, precision));
+    return make_mapped_context(g, p, materials, slab_end, map, tables);
+}
+
+bfpn_ctx* make_mapped_context(const bfpn_grid& g, const bfpn_phys& p,
+                              const std::vector<LayerSpec>& materials,
+                              const std::vector<double>& slab_end,
+                              const std::vector<unsigned char>& map,
+                              const std::vector<std::shared_ptr<PhysicsBase>>& tables) {
+    validate_params(g, p);
+    validate_map(materials, slab_end, map, static_cast<long long>(g.Ny + 1) * (g.Nz + 1));
+    BFPN_REQUIRE(!p.legacy_energy && !p.lite_memory,
+                 "material maps support the eq15 energy model with the full state only");
+    BFPN_REQUIRE(tables.size() == materials.size(), "one table set per material");
+    const int precision = tables.front()->precision;
+    for (const auto& t : tables) BFPN_REQUIRE(t->precision == precision, "map tables of one precision");
     std::unique_ptr<bfpn_ctx> ctx(make_context(tables.front(), g, p));
     if (precision == BFPN_FP32) {
         std::vector<std::shared_ptr<Physics<float>>> t;

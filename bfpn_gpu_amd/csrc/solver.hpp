@@ -97,4 +97,19 @@ bfpn_ctx* make_layered_context(const bfpn_grid& g, const bfpn_phys& p,
 // The physics parameters of one layer: `p` with the layer's material and
 // density (both density and energy_density).
 bfpn_phys layer_phys(const bfpn_phys& p, const LayerSpec& l);
+// A context under a material map (materials as layers, x_end unused; depth
+// slabs ending at slab_end; map[j (Ny+1)(Nz+1) + s] the material of node s in
+// slab j), with tables built here or given (one set per material, from
+// layer_phys(p, material) and its data path).
+void validate_map(const std::vector<LayerSpec>& materials, const std::vector<double>& slab_end,
+                  const std::vector<unsigned char>& map, long long nyz);
+bfpn_ctx* make_mapped_context(const bfpn_grid& g, const bfpn_phys& p,
+                              const std::vector<LayerSpec>& materials,
+                              const std::vector<double>& slab_end,
+                              const std::vector<unsigned char>& map, int precision);
+bfpn_ctx* make_mapped_context(const bfpn_grid& g, const bfpn_phys& p,
+                              const std::vector<LayerSpec>& materials,
+                              const std::vector<double>& slab_end,
+                              const std::vector<unsigned char>& map,
+                              const std::vector<std::shared_ptr<PhysicsBase>>& tables);
 }  // namespace bfpn

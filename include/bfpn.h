@@ -437,6 +437,14 @@ BFPN_API bfpn_status bfpn_server_submit(bfpn_server* s, const bfpn_grid* grid,
 BFPN_API bfpn_status bfpn_server_submit_layered(bfpn_server* s, const bfpn_grid* grid,
                                                 const bfpn_phys* phys, const bfpn_layer* layers,
                                                 int nlayers, double t_final, uint64_t* id);
+/* The same under a material map (materials, slabs and map as in
+ * bfpn_create_mapped; no counterpart in the reference).  Each material's
+ * tables are cached as a layer's are. */
+BFPN_API bfpn_status bfpn_server_submit_mapped(bfpn_server* s, const bfpn_grid* grid,
+                                               const bfpn_phys* phys, const bfpn_layer* materials,
+                                               int nmat, const double* slab_end, int nslabs,
+                                               const unsigned char* map, double t_final,
+                                               uint64_t* id);
 /* 0 queued, 1 running, 2 done, 3 failed, -1 unknown (or already collected). */
 BFPN_API int bfpn_server_poll(bfpn_server* s, uint64_t id);
 /* Blocks until request `id` finished (timeout_s <= 0: no limit), copies its IDD

@@ -182,3 +182,33 @@ def test_layered_request_with_missing_tables_fails_alone(gpu):
     np.testing.assert_array_equal(ra, ref_water)
     np.testing.assert_array_equal(rc, ref_slab)
     assert st["failed"] == 1 and st["completed"] == 2
+
+
+def test_mapped_requests_match_their_solo_runs(gpu):
+    """A beam under a material map (water, then bone for the nodes with iy < 2
+    from 0.31 cm on) served beside a plain and a slab beam gives its solo
+    mapped run bit for bit; its materials' tables are the cached water and
+    bone sets the other beams use."""
+    mix = _layered_mix()
+    water_case, slab_case, slab_layers = mix[0][0], mix[1][0], mix[1][1]
+    nyz = water_case.nyz
+    split = (np.arange(nyz) % (water_case.Ny + 1) < 2).astype(np.uint8)
+    mapped = ([("water", 1.0, os.path.join(TABLES, "water")),
+               ("bone", 1.85, os.path.join(TABLES, "bone"))], [0.31, 100.0],
+              np.vstack([np.zeros(nyz, np.uint8), split]))
+    with gpu.Solver(slab_case, gpu.FP32, mapped=mapped) as s:
+        ref_map = s.run()[0]
+    with gpu.Solver(slab_case, gpu.FP32, layers=slab_layers) as s:
+        ref_slab = s.run()[0]
+    ref_water = _alone(gpu, water_case)
+    with gpu.Server(max_active=3, slice_steps=5) as srv:
+        a = srv.submit(water_case)
+        b = srv.submit(slab_case, mapped=mapped)
+        c = srv.submit(slab_case, layers=slab_layers)
+        out = [srv.wait(i, timeout=120)[0] for i in (a, b, c)]
+        st = srv.stats()
+    np.testing.assert_array_equal(out[0], ref_water)
+    np.testing.assert_array_equal(out[1], ref_map)
+    np.testing.assert_array_equal(out[2], ref_slab)
+    assert not np.array_equal(ref_map, ref_slab)
+    assert st["completed"] == 3 and st["failed"] == 0 and st["cached_tables"] == 2
