@@ -463,8 +463,9 @@ public:
     // slab_end[j] (the last: any depth).  Each operator runs per material on
     // the material's nodes packed together (the same kernels as a
     // homogeneous run, on fewer nodes); transport, which does not depend on
-    // the material, runs on the whole state.  One stream, every group
-    // computed.
+    // the material, runs on the whole state.  One stream; active-group
+    // skipping as in a homogeneous run (each material's sweep publishes into
+    // the same top slots).
     void set_map(std::vector<std::shared_ptr<Physics<T>>> mats, std::vector<double> slab_end,
                  const std::vector<unsigned char>& map) {
         BFPN_REQUIRE(!mats.empty() && !slab_end.empty(), "a material map needs materials and slabs");
@@ -520,7 +521,6 @@ public:
         map_tmp_.ensure(2 * mats.size() * sizeof(double));
         mats_ = std::move(mats);
         slab_end_ = std::move(slab_end);
-        skip_ = false;
     }
 
     int steps_done() const override { return step_; }
@@ -642,7 +642,6 @@ public:
     // Turning it on makes every top conservative (Ng-1) until the next sweeps
     // have measured them.
     void set_group_skip(bool on) override {
-        if (!mats_.empty()) return;  // a material map computes every group
         if (on && !skip_) {
             sync_all();
             init_tops(grid.Ng - 1, grid.Ng - 1);
@@ -1033,20 +1032,24 @@ private:
             if (!whole)
                 launch_scatter_nodes<T>(cur_[w], c, nodes(k), sl.count[k], P_->nyz, P_->nang, Ng, s);
         };
-        auto tops = [&] {
-            SweepTops t;
+        // The tops of one sweep (sweep_tops advances the species' ring once);
+        // each material's launch publishes into the same slots (atomic
+        // maxima), with its own source-top map and cap.
+        auto own = [&](SweepTops t) {
             t.q_cap = P_->qcap;
+            if (t.qmap) t.qmap = P_->qmap.template as<int>();
             return t;
         };
         // single energy half of a species; Q: packed sources or none
-        auto half = [&](int sp, const DeviceBuffer* Q, double* partials, bool proxy) {
+        auto half = [&](int sp, const DeviceBuffer* Q, double* partials, bool proxy,
+                        const SweepTops& tops) {
             for (int k = 0; k < nk; ++k) {
                 use(k);
                 T* a = gather(k, cA, 2 * sp);
                 T* b = gather(k, cB, 2 * sp + 1);
                 launch_energy_half<T>(a, b, P_->coef.template as<EnergyCoef>(), Q ? packed(*Q, k) : nullptr,
                                       Ng, lanes(k), strag, yzw(k), P_->nang, part(partials, k), proxy, s,
-                                      tops());
+                                      own(tops));
                 scatter(k, a, 2 * sp);
                 scatter(k, b, 2 * sp + 1);
             }
@@ -1054,7 +1057,7 @@ private:
         };
         // paired energy sweep; X: packed end-of-step copies (primary) or none
         auto pair = [&](int sp, const DeviceBuffer* Qa, const DeviceBuffer* Qb, double* partials,
-                        bool proxy, bool xout) {
+                        bool proxy, bool xout, const SweepTops& tops) {
             for (int k = 0; k < nk; ++k) {
                 use(k);
                 T* a = gather(k, cA, 2 * sp);
@@ -1063,7 +1066,7 @@ private:
                                       Qa ? packed(*Qa, k) : nullptr, Qb ? packed(*Qb, k) : nullptr, Ng,
                                       lanes(k), strag, yzw(k), P_->nang, part(partials, k), proxy,
                                       xout ? packed(x_[0], k) : nullptr, xout ? packed(x_[1], k) : nullptr,
-                                      s, tops(), x_rows());
+                                      s, own(tops), x_rows());
                 scatter(k, a, 2 * sp);
                 scatter(k, b, 2 * sp + 1);
             }
@@ -1071,7 +1074,7 @@ private:
         };
         auto tr = [&](int w, bool preserve) {
             launch_transport_half<T>(cur_[w], P_->om_dev.template as<T>(), grid.Ny, grid.Nz, Ng,
-                                     P_->nang, d.dy, d.dz, 0.5 * d.dt, preserve, s, nullptr);
+                                     P_->nang, d.dy, d.dz, 0.5 * d.dt, preserve, s, cur_top(w / 2));
             mark(kTransport, array_bytes() * 2, s);
         };
         auto ang = [&](int w) {
@@ -1082,33 +1085,35 @@ private:
                 if constexpr (std::is_same_v<T, float>) {
                     if (P_->angle_plan.plan.su_h) {
                         launch_angle_mfma(a, P_->factors.template as<float>(), P_->angle_plan.plan, Ng,
-                                          sl.count[k], grid.Nmu, grid.Nom, s, nullptr);
+                                          sl.count[k], grid.Nmu, grid.Nom, s, cur_top(w / 2));
                         done = true;
                     }
                 }
                 if (!done)
                     launch_angle_full<T>(a, P_->factors.template as<T>(), P_->su.template as<T>(),
                                          P_->sv.template as<T>(), Ng, sl.count[k], grid.Nmu, grid.Nom, s,
-                                         nullptr);
+                                         cur_top(w / 2));
                 scatter(k, a, w);
             }
             mark(kAngle, array_bytes() * 2, s);
         };
         // the source of material k from its packed primary (Fp, fp) into Q
-        auto src = [&](int k, const T* Fp, const T* fp, T* Q) {
+        auto src = [&](int k, const T* Fp, const T* fp, T* Q) {  // the source of step n
+            const int* xt = skip_ ? x_top(n) : nullptr;
+            const int* qm = skip_ ? P_->qmap.template as<int>() : nullptr;
             if constexpr (std::is_same_v<T, float>) {
                 launch_source_mfma(Fp, fp, a1_.as<float>(), a2_.as<float>(), Q, P_->plan.plan, Ng,
-                                   sl.count[k], P_->nang, s, nullptr, nullptr, nullptr, P_->qcap);
+                                   sl.count[k], P_->nang, s, nullptr, xt, qm, P_->qcap);
             } else {
                 const int* need = P_->need.template as<int>();
                 launch_source_angular<T>(Fp, fp, P_->kv.template as<T>(), a1_.as<T>(), a2_.as<T>(), Ng,
-                                         sl.count[k], P_->nang, s, nullptr,
+                                         sl.count[k], P_->nang, s, xt,
                                          P_->kcount.template as<int>(), P_->klist.template as<int>(),
                                          need);
                 launch_source_energy<T>(a1_.as<T>(), a2_.as<T>(), P_->e1.template as<double>(),
                                         P_->e2.template as<double>(), P_->beg.template as<int>(),
-                                        P_->end.template as<int>(), Q, Ng, lanes(k), d.dg, s, nullptr,
-                                        nullptr, need);
+                                        P_->end.template as<int>(), Q, Ng, lanes(k), d.dg, s, xt, qm,
+                                        need);
             }
         };
         double* hist = hist_.as<double>();
@@ -1130,7 +1135,7 @@ private:
         double* pp = partial_p_.as<double>() + static_cast<size_t>(n & 3) * pslot_;
         if (first) {
             mark(kWait, 0.0, s);
-            half(0, nullptr, nullptr, false);
+            half(0, nullptr, nullptr, false, sweep_tops(0, -1, -1, -1));
         }
         if (!phys.energy_only) {
             if (!phys.no_transport) {
@@ -1147,9 +1152,9 @@ private:
             }
         }
         if (last)
-            half(0, nullptr, pp, true);
+            half(0, nullptr, pp, true, sweep_tops(0, -1, -1, n));
         else
-            pair(0, nullptr, nullptr, pp, true, secondary);
+            pair(0, nullptr, nullptr, pp, true, secondary, sweep_tops(0, -1, -1, n));
         const bool sec = secondary && n > 0;
         if (sec) {
             DeviceBuffer& Q = q_[n & 1];
@@ -1165,10 +1170,11 @@ private:
             }
             mark(kSource, array_bytes() * 7, s);
             if (sec_open_) {
-                pair(1, &q_[(n - 1) & 1], &Q, partial_s_.as<double>(), false, false);
+                pair(1, &q_[(n - 1) & 1], &Q, partial_s_.as<double>(), false, false,
+                     sweep_tops(1, n - 1, n, -1));
                 finish(n - 1, true);
             } else {
-                half(1, &Q, nullptr, false);
+                half(1, &Q, nullptr, false, sweep_tops(1, n, -1, -1));
             }
             tr(2, false);
             tr(3, true);
@@ -1177,7 +1183,7 @@ private:
             tr(2, false);
             tr(3, true);
             if (last) {
-                half(1, &Q, partial_s_.as<double>(), false);
+                half(1, &Q, partial_s_.as<double>(), false, sweep_tops(1, n, -1, -1));
                 finish(n, true);
                 sec_open_ = false;
             } else {

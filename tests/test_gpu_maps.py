@@ -11,8 +11,10 @@ secondary source, dose); transport is material-independent.  Checked against:
 - the mapped oracle (pinned on the CPU by tests/test_oracle_maps.py) for a
   lateral water/bone split: FP64 to 1e-10, FP32 to 2e-4 (the proxy, which
   changes sign, to 1e-3 of its maximum).
-Runs under a map compute every group on one stream, so the homogeneous and
-layered references run with group skipping and overlap off.
+Runs under a map use one stream, so the homogeneous and layered references
+run with overlap off; group skipping (on by default, maps included) is off in
+the reference comparisons and checked separately: on or off, a mapped run is
+the same bit for bit.
 """
 import numpy as np
 import pytest
@@ -40,9 +42,9 @@ def case_of(**over):
     return c
 
 
-def run_gpu(gpu, case, precision, chunks=None, **kw):
+def run_gpu(gpu, case, precision, chunks=None, skip=False, **kw):
     with gpu.Solver(case, precision, **kw) as s:
-        s.set_group_skip(False)
+        s.set_group_skip(skip)
         s.set_overlap(False)
         for n in chunks or [case.num_steps()]:
             s.advance(n)
@@ -132,6 +134,16 @@ def test_lateral_map_matches_oracle(gpu, oracle_mod, precision, tol):
     bone_only = run_gpu(gpu, case_of(time=2.4, material="bone", density=1.85,
                                      data=material_dir("bone")), getattr(gpu, precision))
     assert not np.allclose(g[0], bone_only[0], rtol=1e-3)  # the water half counts
+
+
+@pytest.mark.parametrize("precision", ["FP32", "FP64"])
+def test_group_skipping_under_a_map_changes_nothing(gpu, precision):
+    case = case_of(time=2.4)
+    _, mp = split_map(case)
+    mapped = ([WATER, BONE], [0.5, 100.0], np.vstack([np.zeros_like(mp), mp]))
+    p = getattr(gpu, precision)
+    assert_identical(run_gpu(gpu, case, p, mapped=mapped, skip=True, chunks=[9, case.num_steps() - 9]),
+                     run_gpu(gpu, case, p, mapped=mapped, skip=False))
 
 
 def test_bad_maps_are_refused(gpu):
