@@ -441,6 +441,7 @@ public:
         for (const auto& s : state_) b += s.bytes();
         for (size_t k = 1; k < layers_.size(); ++k) b += layers_[k]->device_bytes();
         for (const auto& c : cbuf_) b += c.bytes();
+        for (const auto& c : cbuf2_) b += c.bytes();
         b += map_nodes_.bytes() + map_yzw_.bytes() + map_tmp_.bytes();
         for (const auto& m : mats_)
             if (m != P_) b += m->device_bytes();
@@ -463,7 +464,7 @@ public:
     // slab_end[j] (the last: any depth).  Each operator runs per material on
     // the material's nodes packed together (the same kernels as a
     // homogeneous run, on fewer nodes); transport, which does not depend on
-    // the material, runs on the whole state.  One stream; active-group
+    // the material, runs on the whole state.  Streams and active-group
     // skipping as in a homogeneous run (each material's sweep publishes into
     // the same top slots).
     void set_map(std::vector<std::shared_ptr<Physics<T>>> mats, std::vector<double> slab_end,
@@ -515,6 +516,8 @@ public:
         upload(map_nodes_, lists);
         upload(map_yzw_, weights);
         for (auto& b : cbuf_) b.ensure(static_cast<size_t>(Ng) * max_nodes * nang * sizeof(T));
+        if (P_->secondary)  // the secondary's stream packs into its own pair
+            for (auto& b : cbuf2_) b.ensure(static_cast<size_t>(Ng) * max_nodes * nang * sizeof(T));
         pslot_ = static_cast<int>(pslot);
         partial_p_.ensure(4 * static_cast<size_t>(pslot) * sizeof(double));
         partial_s_.ensure(static_cast<size_t>(pslot) * sizeof(double));
@@ -591,7 +594,7 @@ public:
             else  // the upwind model has no paired sweep: every step stands alone
                 step(first || P_->legacy, last || P_->legacy);
         }
-        if (stream2_ && (!overlap || !mats_.empty())) {  // all on `stream`: done_stream still ends last
+        if (stream2_ && !overlap) {  // all on `stream`: done_stream still ends last
             BFPN_HIP(hipEventRecord(ev_tail_, stream));
             BFPN_HIP(hipStreamWaitEvent(stream2_, ev_tail_, 0));
         }
@@ -996,7 +999,8 @@ private:
     }
 
     // One depth step under a material map: step() with every node-local
-    // operator run per material on its packed nodes, on one stream.  The
+    // operator run per material on its packed nodes, on the same two streams
+    // and events (each stream packs into its own scratch).  The
     // end-of-step primary copy and the sources live packed (material k of
     // the slab at element offset off[k]), as the source and the secondary
     // sweeps read them; the dose partials of material k sit at poff[k] of a
@@ -1007,28 +1011,38 @@ private:
         const MapSlab& sl = slabs_[j];
         const int nk = static_cast<int>(sl.mat.size());
         const int Ng = grid.Ng;
-        hipStream_t s = stream;
+        // streams as in step(): the primary on `stream`, the source and the
+        // secondary on stream2_; each species packs into its own scratch
+        const bool two = stream2_ && overlap;
+        hipStream_t sp = stream, ss = two ? stream2_ : stream;
+        hipStream_t st[2] = {sp, ss};
+        T* scratch[2][2] = {{cbuf_[0].as<T>(), cbuf_[1].as<T>()},
+                            {cbuf2_[0].as<T>(), cbuf2_[1].as<T>()}};
         const bool secondary = P_->secondary;
         const bool preserve_primary = phys.no_spatial_clipping != 0;
         const bool strag = phys.eq15_straggling != 0;
         // a slab of one material lists every node in order: its operators
         // work on the state itself, as in a homogeneous run
         const bool whole = nk == 1;
-        T* cA = cbuf_[0].as<T>();
-        T* cB = cbuf_[1].as<T>();
         auto use = [&](int k) { P_ = mats_[sl.mat[k]]; };
+        auto wait_source = [&] {
+            if (src_pending_) {
+                BFPN_HIP(hipStreamWaitEvent(sp, ev_src_, 0));
+                src_pending_ = false;
+            }
+        };
         auto nodes = [&](int k) { return map_nodes_.as<int>() + static_cast<long long>(j) * P_->nyz + sl.first[k]; };
         auto yzw = [&](int k) { return map_yzw_.as<double>() + static_cast<long long>(j) * P_->nyz + sl.first[k]; };
         auto lanes = [&](int k) { return static_cast<long long>(sl.count[k]) * P_->nang; };
         auto packed = [&](const DeviceBuffer& b, int k) { return b.as<T>() + sl.off[k]; };
         auto part = [&](double* base, int k) { return base ? base + sl.poff[k] : nullptr; };
-        // the packed arrays of material k: gathered from / scattered to state w
-        auto gather = [&](int k, T* c, int w) {
+        // the packed arrays of material k: gathered from / scattered to state w on stream s
+        auto gather = [&](int k, T* c, int w, hipStream_t s) {
             if (whole) return cur_[w];
             launch_gather_nodes<T>(c, cur_[w], nodes(k), sl.count[k], P_->nyz, P_->nang, Ng, s);
             return c;
         };
-        auto scatter = [&](int k, const T* c, int w) {
+        auto scatter = [&](int k, const T* c, int w, hipStream_t s) {
             if (!whole)
                 launch_scatter_nodes<T>(cur_[w], c, nodes(k), sl.count[k], P_->nyz, P_->nang, Ng, s);
         };
@@ -1041,46 +1055,50 @@ private:
             return t;
         };
         // single energy half of a species; Q: packed sources or none
-        auto half = [&](int sp, const DeviceBuffer* Q, double* partials, bool proxy,
+        auto half = [&](int sc, const DeviceBuffer* Q, double* partials, bool proxy,
                         const SweepTops& tops) {
+            hipStream_t s = st[sc];
             for (int k = 0; k < nk; ++k) {
                 use(k);
-                T* a = gather(k, cA, 2 * sp);
-                T* b = gather(k, cB, 2 * sp + 1);
+                T* a = gather(k, scratch[sc][0], 2 * sc, s);
+                T* b = gather(k, scratch[sc][1], 2 * sc + 1, s);
                 launch_energy_half<T>(a, b, P_->coef.template as<EnergyCoef>(), Q ? packed(*Q, k) : nullptr,
                                       Ng, lanes(k), strag, yzw(k), P_->nang, part(partials, k), proxy, s,
                                       own(tops));
-                scatter(k, a, 2 * sp);
-                scatter(k, b, 2 * sp + 1);
+                scatter(k, a, 2 * sc, s);
+                scatter(k, b, 2 * sc + 1, s);
             }
             mark(kEnergy, array_bytes() * (Q ? 5 : 4), s);
         };
         // paired energy sweep; X: packed end-of-step copies (primary) or none
-        auto pair = [&](int sp, const DeviceBuffer* Qa, const DeviceBuffer* Qb, double* partials,
+        auto pair = [&](int sc, const DeviceBuffer* Qa, const DeviceBuffer* Qb, double* partials,
                         bool proxy, bool xout, const SweepTops& tops) {
+            hipStream_t s = st[sc];
             for (int k = 0; k < nk; ++k) {
                 use(k);
-                T* a = gather(k, cA, 2 * sp);
-                T* b = gather(k, cB, 2 * sp + 1);
+                T* a = gather(k, scratch[sc][0], 2 * sc, s);
+                T* b = gather(k, scratch[sc][1], 2 * sc + 1, s);
                 launch_energy_pair<T>(a, b, P_->coef.template as<EnergyCoef>(),
                                       Qa ? packed(*Qa, k) : nullptr, Qb ? packed(*Qb, k) : nullptr, Ng,
                                       lanes(k), strag, yzw(k), P_->nang, part(partials, k), proxy,
                                       xout ? packed(x_[0], k) : nullptr, xout ? packed(x_[1], k) : nullptr,
                                       s, own(tops), x_rows());
-                scatter(k, a, 2 * sp);
-                scatter(k, b, 2 * sp + 1);
+                scatter(k, a, 2 * sc, s);
+                scatter(k, b, 2 * sc + 1, s);
             }
             mark(kEnergy, array_bytes() * (4 + (Qa ? 2 : 0) + (xout ? 2 : 0)), s);
         };
         auto tr = [&](int w, bool preserve) {
+            hipStream_t s = st[w / 2];
             launch_transport_half<T>(cur_[w], P_->om_dev.template as<T>(), grid.Ny, grid.Nz, Ng,
                                      P_->nang, d.dy, d.dz, 0.5 * d.dt, preserve, s, cur_top(w / 2));
             mark(kTransport, array_bytes() * 2, s);
         };
         auto ang = [&](int w) {
+            hipStream_t s = st[w / 2];
             for (int k = 0; k < nk; ++k) {
                 use(k);
-                T* a = gather(k, cA, w);
+                T* a = gather(k, scratch[w / 2][0], w, s);
                 bool done = false;
                 if constexpr (std::is_same_v<T, float>) {
                     if (P_->angle_plan.plan.su_h) {
@@ -1093,7 +1111,7 @@ private:
                     launch_angle_full<T>(a, P_->factors.template as<T>(), P_->su.template as<T>(),
                                          P_->sv.template as<T>(), Ng, sl.count[k], grid.Nmu, grid.Nom, s,
                                          cur_top(w / 2));
-                scatter(k, a, w);
+                scatter(k, a, w, s);
             }
             mark(kAngle, array_bytes() * 2, s);
         };
@@ -1101,6 +1119,7 @@ private:
         auto src = [&](int k, const T* Fp, const T* fp, T* Q) {  // the source of step n
             const int* xt = skip_ ? x_top(n) : nullptr;
             const int* qm = skip_ ? P_->qmap.template as<int>() : nullptr;
+            hipStream_t s = ss;
             if constexpr (std::is_same_v<T, float>) {
                 launch_source_mfma(Fp, fp, a1_.as<float>(), a2_.as<float>(), Q, P_->plan.plan, Ng,
                                    sl.count[k], P_->nang, s, nullptr, xt, qm, P_->qcap);
@@ -1120,6 +1139,7 @@ private:
         double* tmp = map_tmp_.as<double>();
         const int nm = static_cast<int>(mats_.size());
         auto finish = [&](int m, bool with_secondary) {  // IDD and proxy of step m (same slab)
+            hipStream_t s = ss;
             double* pm = partial_p_.as<double>() + static_cast<size_t>(m & 3) * pslot_;
             double* ps = partial_s_.as<double>();
             for (int k = 0; k < nk; ++k) {
@@ -1134,7 +1154,8 @@ private:
         };
         double* pp = partial_p_.as<double>() + static_cast<size_t>(n & 3) * pslot_;
         if (first) {
-            mark(kWait, 0.0, s);
+            wait_source();
+            mark(kWait, 0.0, sp);
             half(0, nullptr, nullptr, false, sweep_tops(0, -1, -1, -1));
         }
         if (!phys.energy_only) {
@@ -1151,24 +1172,34 @@ private:
                 tr(1, true);
             }
         }
-        if (last)
+        if (last) {
             half(0, nullptr, pp, true, sweep_tops(0, -1, -1, n));
-        else
+        } else {
+            wait_source();  // the end-of-step copies are about to be rewritten
+            if (!first) mark(kWait, 0.0, sp);
             pair(0, nullptr, nullptr, pp, true, secondary, sweep_tops(0, -1, -1, n));
+        }
+        if (two) {
+            BFPN_HIP(hipEventRecord(ev_prim_, sp));
+            BFPN_HIP(hipStreamWaitEvent(ss, ev_prim_, 0));
+            mark(kWait, 0.0, ss);
+        }
         const bool sec = secondary && n > 0;
         if (sec) {
             DeviceBuffer& Q = q_[n & 1];
-            for (int k = 0; k < nk; ++k) {  // the source of step n
+            for (int k = 0; k < nk; ++k) {  // the source of step n, on ss
                 use(k);
                 if (last) {  // from the state; otherwise from the end-of-step copy
-                    const T* a = gather(k, cA, 0);
-                    const T* b = gather(k, cB, 1);
+                    const T* a = gather(k, scratch[1][0], 0, ss);
+                    const T* b = gather(k, scratch[1][1], 1, ss);
                     src(k, a, b, packed(Q, k));
                 } else {
                     src(k, packed(x_[0], k), packed(x_[1], k), packed(Q, k));
                 }
             }
-            mark(kSource, array_bytes() * 7, s);
+            if (two) BFPN_HIP(hipEventRecord(ev_src_, ss));  // its inputs are read
+            src_pending_ = two;
+            mark(kSource, array_bytes() * 7, ss);
             if (sec_open_) {
                 pair(1, &q_[(n - 1) & 1], &Q, partial_s_.as<double>(), false, false,
                      sweep_tops(1, n - 1, n, -1));
@@ -1207,7 +1238,7 @@ private:
     std::vector<std::shared_ptr<Physics<T>>> mats_;  // material map (empty: none)
     std::vector<double> slab_end_;
     std::vector<MapSlab> slabs_;
-    DeviceBuffer map_nodes_, map_yzw_, map_tmp_, cbuf_[2];  // node lists and weights, slab-major
+    DeviceBuffer map_nodes_, map_yzw_, map_tmp_, cbuf_[2], cbuf2_[2];  // node lists and weights, slab-major
     int pslot_ = 0;                                           // doubles per partial slot
     DeviceBuffer state_[4], a1_, a2_, q_[2], x_[2];
     DeviceBuffer diag_part_, diag_out_;  // diagnostics scratch, allocated on first use
@@ -1349,7 +1380,6 @@ bfpn_ctx* make_mapped_context(const bfpn_grid& g, const bfpn_phys& p,
         for (auto& b : tables) t.push_back(std::static_pointer_cast<Physics<double>>(b));
         static_cast<Context<double>*>(ctx.get())->set_map(std::move(t), slab_end, map);
     }
-    ctx->overlap = false;
     return ctx.release();
 }
 
@@ -1876,7 +1906,7 @@ BFPN_API bfpn_status bfpn_create_mapped(const bfpn_grid* grid, const bfpn_phys* 
         BFPN_HIP(hipMemGetInfo(&free_b, &total_b));
         const size_t word = precision == BFPN_FP64 ? 8 : 4;
         const size_t need = context_bytes(*grid, *phys, precision) +
-                            2 * static_cast<size_t>(grid->Ng) * nyz * grid->Nmu * grid->Nom * word +
+                            4 * static_cast<size_t>(grid->Ng) * nyz * grid->Nmu * grid->Nom * word +
                             nmat * physics_bytes(*grid, *phys, precision) + (64ull << 20);
         if (need > free_b)
             throw Error(BFPN_E_NOMEM, "Requested grid and " + std::to_string(nmat) +

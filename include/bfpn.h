@@ -293,10 +293,9 @@ BFPN_API bfpn_status bfpn_create_layered(const bfpn_grid* grid, const bfpn_phys*
  * depth); map[j (Ny+1)(Nz+1) + s] is the material index of node
  * s = iz (Ny+1) + iy in slab j.  The energy sweeps, angular diffusion,
  * secondary source and dose of a node use its material's tables and density;
- * transport is material-independent.  Runs on one stream (bfpn_set_overlap
- * has no effect) and skips inactive groups as bfpn_create does; the legacy
- * energy model, lite memory, energy moments and spot planes are not
- * available. */
+ * transport is material-independent.  Streams (bfpn_set_overlap) and group
+ * skipping (bfpn_set_group_skip) work as for bfpn_create; the legacy energy
+ * model, lite memory, energy moments and spot planes are not available. */
 BFPN_API bfpn_status bfpn_create_mapped(const bfpn_grid* grid, const bfpn_phys* phys,
                                         const bfpn_layer* materials, int nmat,
                                         const double* slab_end, int nslabs,

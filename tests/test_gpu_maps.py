@@ -11,10 +11,9 @@ secondary source, dose); transport is material-independent.  Checked against:
 - the mapped oracle (pinned on the CPU by tests/test_oracle_maps.py) for a
   lateral water/bone split: FP64 to 1e-10, FP32 to 2e-4 (the proxy, which
   changes sign, to 1e-3 of its maximum).
-Runs under a map use one stream, so the homogeneous and layered references
-run with overlap off; group skipping (on by default, maps included) is off in
-the reference comparisons and checked separately: on or off, a mapped run is
-the same bit for bit.
+Group skipping and the secondary's second stream (both on by default, maps
+included) are off in the reference comparisons and checked separately: on or
+off, a mapped run is the same bit for bit.
 """
 import numpy as np
 import pytest
@@ -42,10 +41,10 @@ def case_of(**over):
     return c
 
 
-def run_gpu(gpu, case, precision, chunks=None, skip=False, **kw):
+def run_gpu(gpu, case, precision, chunks=None, skip=False, overlap=False, **kw):
     with gpu.Solver(case, precision, **kw) as s:
         s.set_group_skip(skip)
-        s.set_overlap(False)
+        s.set_overlap(overlap)
         for n in chunks or [case.num_steps()]:
             s.advance(n)
         idd, proxy = s.history()
@@ -144,6 +143,17 @@ def test_group_skipping_under_a_map_changes_nothing(gpu, precision):
     p = getattr(gpu, precision)
     assert_identical(run_gpu(gpu, case, p, mapped=mapped, skip=True, chunks=[9, case.num_steps() - 9]),
                      run_gpu(gpu, case, p, mapped=mapped, skip=False))
+
+
+@pytest.mark.parametrize("precision", ["FP32", "FP64"])
+def test_second_stream_under_a_map_changes_nothing(gpu, precision):
+    case = case_of(time=2.4)
+    _, mp = split_map(case)
+    mapped = ([WATER, BONE], [0.5, 100.0], np.vstack([np.zeros_like(mp), mp]))
+    p = getattr(gpu, precision)
+    n = case.num_steps()
+    assert_identical(run_gpu(gpu, case, p, mapped=mapped, skip=True, overlap=True, chunks=[11, n - 11]),
+                     run_gpu(gpu, case, p, mapped=mapped, skip=True, overlap=False))
 
 
 def test_bad_maps_are_refused(gpu):
