@@ -29,8 +29,11 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 
 def build(force: bool = False) -> str:
-    """Compile libbfpn_oracle.so with the recipe in oracle/Makefile."""
-    if force or not os.path.exists(LIB_PATH):
+    """Compile libbfpn_oracle.so with the recipe in oracle/Makefile (when it is
+    missing or older than its source, or when forced)."""
+    src = os.path.join(HERE, "bfpn_oracle.cpp")
+    stale = os.path.exists(LIB_PATH) and os.path.getmtime(src) > os.path.getmtime(LIB_PATH)
+    if force or stale or not os.path.exists(LIB_PATH):
         subprocess.run(["make", "-B" if force else "-s", "libbfpn_oracle.so"], cwd=HERE, check=True)
     return LIB_PATH
 
