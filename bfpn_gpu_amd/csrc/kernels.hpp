@@ -204,6 +204,10 @@ void launch_gather_nodes(T* dst, const T* src, const int* nodes, int n, long lon
 template <typename T>
 void launch_scatter_nodes(T* dst, const T* src, const int* nodes, int n, long long nyz, int nang,
                           int Ng, hipStream_t s);
+// Sets the listed nodes of a state to zero.
+template <typename T>
+void launch_zero_nodes(T* dst, const int* nodes, int n, long long nyz, int nang, int Ng,
+                       hipStream_t s);
 
 // F[g][s][m] = space_angle[s*nang+m] * energy[g]
 template <typename T>

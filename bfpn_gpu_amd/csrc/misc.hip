@@ -80,7 +80,8 @@ __global__ void sum_in_order_kernel(const double* __restrict__ v, int n, double*
 // One node's directions are contiguous, so each block copies runs of nang
 // values, in units of U (a 16-byte vector when nang allows, else one value);
 // grid (ceil(n nang / (U kBlock)), Ng).
-template <typename T, typename U, bool GATHER>
+// MODE 0: gather, 1: scatter, 2: zero the listed nodes of dst.
+template <typename T, typename U, int MODE>
 __global__ __launch_bounds__(kBlock) void node_copy_kernel(T* __restrict__ dst, const T* __restrict__ src,
                                                            const int* __restrict__ nodes, int n,
                                                            long long nyz, int nang) {
@@ -94,10 +95,12 @@ __global__ __launch_bounds__(kBlock) void node_copy_kernel(T* __restrict__ dst, 
     const U* s = reinterpret_cast<const U*>(src);
     const long long packed = g * per + idx;
     const long long full = (g * nyz + nodes[k]) * nu + a;
-    if (GATHER)
+    if (MODE == 0)
         d[packed] = s[full];
-    else
+    else if (MODE == 1)
         d[full] = s[packed];
+    else
+        d[full] = U{};
 }
 
 template <typename T>
@@ -307,22 +310,22 @@ void launch_sum_in_order(const double* v, int n, double* out, hipStream_t s) {
     BFPN_HIP(hipGetLastError());
 }
 
-template <typename T, bool GATHER>
+template <typename T, int MODE>
 static void node_copy(T* dst, const T* src, const int* nodes, int n, long long nyz, int nang, int Ng,
                       hipStream_t s) {
     typedef __attribute__((ext_vector_type(16 / sizeof(T)))) T vec_t;
     constexpr int V = 16 / sizeof(T);
     const bool vec = nang % V == 0 && reinterpret_cast<uintptr_t>(dst) % 16 == 0 &&
-                     reinterpret_cast<uintptr_t>(src) % 16 == 0;
+                     reinterpret_cast<uintptr_t>(src) % 16 == 0;  // (src is null when zeroing)
     const long long per = static_cast<long long>(n) * (vec ? nang / V : nang);
     if (per <= 0 || Ng <= 0) return;
     BFPN_REQUIRE(Ng <= 65535 && (per + kBlock - 1) / kBlock < (1LL << 31), "node copy grid too large");
     const dim3 grid(static_cast<unsigned>((per + kBlock - 1) / kBlock), static_cast<unsigned>(Ng));
     if (vec)
-        hipLaunchKernelGGL((node_copy_kernel<T, vec_t, GATHER>), grid, dim3(kBlock), 0, s, dst, src,
+        hipLaunchKernelGGL((node_copy_kernel<T, vec_t, MODE>), grid, dim3(kBlock), 0, s, dst, src,
                            nodes, n, nyz, nang);
     else
-        hipLaunchKernelGGL((node_copy_kernel<T, T, GATHER>), grid, dim3(kBlock), 0, s, dst, src, nodes,
+        hipLaunchKernelGGL((node_copy_kernel<T, T, MODE>), grid, dim3(kBlock), 0, s, dst, src, nodes,
                            n, nyz, nang);
     BFPN_HIP(hipGetLastError());
 }
@@ -330,13 +333,19 @@ static void node_copy(T* dst, const T* src, const int* nodes, int n, long long n
 template <typename T>
 void launch_gather_nodes(T* dst, const T* src, const int* nodes, int n, long long nyz, int nang,
                          int Ng, hipStream_t s) {
-    node_copy<T, true>(dst, src, nodes, n, nyz, nang, Ng, s);
+    node_copy<T, 0>(dst, src, nodes, n, nyz, nang, Ng, s);
 }
 
 template <typename T>
 void launch_scatter_nodes(T* dst, const T* src, const int* nodes, int n, long long nyz, int nang,
                           int Ng, hipStream_t s) {
-    node_copy<T, false>(dst, src, nodes, n, nyz, nang, Ng, s);
+    node_copy<T, 1>(dst, src, nodes, n, nyz, nang, Ng, s);
+}
+
+template <typename T>
+void launch_zero_nodes(T* dst, const int* nodes, int n, long long nyz, int nang, int Ng,
+                       hipStream_t s) {
+    node_copy<T, 2>(dst, nullptr, nodes, n, nyz, nang, Ng, s);
 }
 
 template void launch_gather_nodes<float>(float*, const float*, const int*, int, long long, int, int,
@@ -347,6 +356,8 @@ template void launch_scatter_nodes<float>(float*, const float*, const int*, int,
                                           int, hipStream_t);
 template void launch_scatter_nodes<double>(double*, const double*, const int*, int, long long, int,
                                            int, hipStream_t);
+template void launch_zero_nodes<float>(float*, const int*, int, long long, int, int, hipStream_t);
+template void launch_zero_nodes<double>(double*, const int*, int, long long, int, int, hipStream_t);
 
 template <typename T>
 void launch_beam(T* F, const double* space_angle, const double* energy, int Ng, long long lanes,

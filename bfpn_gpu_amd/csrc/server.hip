@@ -428,9 +428,9 @@ void submit_request(bfpn_server* s, const bfpn_grid* grid, const bfpn_phys* phys
     r->steps = static_cast<int>(std::ceil(t_final / dt - 1e-12));
     BFPN_REQUIRE(r->steps >= 1, "t_final must cover at least one depth step");
     r->ctx_bytes = context_bytes(*grid, *phys, s->opts.precision);
-    if (!r->map.empty())  // packed arrays for the mixed slabs (two per stream), at most the state's size
-        r->ctx_bytes += 4 * static_cast<size_t>(grid->Ng) * nyz * grid->Nmu * grid->Nom *
-                        (s->opts.precision == BFPN_FP64 ? 8 : 4);
+    if (!r->map.empty())  // packed copies of the smaller materials' nodes
+        r->ctx_bytes += map_bytes(*grid, *phys, r->map, static_cast<int>(r->layers.size()),
+                                  s->opts.precision);
     r->t_submit = Clock::now();
     std::lock_guard<std::mutex> g(s->mu);
     BFPN_REQUIRE(!s->stopping, "server is shutting down");

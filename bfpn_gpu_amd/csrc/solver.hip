@@ -442,6 +442,8 @@ public:
         for (size_t k = 1; k < layers_.size(); ++k) b += layers_[k]->device_bytes();
         for (const auto& c : cbuf_) b += c.bytes();
         for (const auto& c : cbuf2_) b += c.bytes();
+        for (const auto& c : xm_) b += c.bytes();
+        for (const auto& c : qm_) b += c.bytes();
         b += map_nodes_.bytes() + map_yzw_.bytes() + map_tmp_.bytes();
         for (const auto& m : mats_)
             if (m != P_) b += m->device_bytes();
@@ -461,10 +463,12 @@ public:
     // Material map: the node-local operators (energy, angular diffusion,
     // source, dose) of node s in depth slab j use the tables of material
     // map[j nyz + s]; slab j covers the depth steps whose midpoint lies below
-    // slab_end[j] (the last: any depth).  Each operator runs per material on
-    // the material's nodes packed together (the same kernels as a
-    // homogeneous run, on fewer nodes); transport, which does not depend on
-    // the material, runs on the whole state.  Streams and active-group
+    // slab_end[j] (the last: any depth).  Each operator runs per material with
+    // the same kernels as a homogeneous run: the slab's largest material in
+    // place over every node (the others' nodes copied out and zeroed first,
+    // so it gives them nothing), the others on their nodes packed together,
+    // then written back; transport, which does not depend on the material,
+    // runs on the whole state.  Streams and active-group
     // skipping as in a homogeneous run (each material's sweep publishes into
     // the same top slots).
     void set_map(std::vector<std::shared_ptr<Physics<T>>> mats, std::vector<double> slab_end,
@@ -489,35 +493,45 @@ public:
                 by[m].push_back(static_cast<int>(s));
             }
             MapSlab sl;
-            long long off = 0;
-            int poff = 0;
             for (size_t m = 0; m < by.size(); ++m) {
                 if (by[m].empty()) continue;
-                const int n = static_cast<int>(by[m].size());
                 sl.mat.push_back(static_cast<int>(m));
-                sl.count.push_back(n);
+                sl.count.push_back(static_cast<int>(by[m].size()));
                 sl.first.push_back(static_cast<int>(lists.size() - j * nyz));
-                sl.off.push_back(off);
-                off += static_cast<long long>(Ng) * n * nang;
-                const int eb = energy_blocks(static_cast<long long>(n) * nang, sizeof(T));
-                sl.eb.push_back(eb);
-                sl.poff.push_back(poff);
-                poff += 2 * eb;
                 for (int s : by[m]) {
                     lists.push_back(s);
                     weights.push_back(w[s]);
                 }
             }
-            if (sl.mat.size() > 1)  // packed copies only where materials mix
-                for (int n : sl.count) max_nodes = std::max<long long>(max_nodes, n);
+            // the material with the most nodes runs in place over every node
+            // (its sweeps cover all lanes); the others run packed
+            for (size_t k = 1; k < sl.count.size(); ++k)
+                if (sl.count[k] > sl.count[sl.major]) sl.major = static_cast<int>(k);
+            long long moff = 0;
+            int poff = 0;
+            for (size_t k = 0; k < sl.mat.size(); ++k) {
+                const bool maj = static_cast<int>(k) == sl.major;
+                const long long nodes_k = maj ? nyz : sl.count[k];
+                const int eb = energy_blocks(nodes_k * nang, sizeof(T));
+                sl.eb.push_back(eb);
+                sl.poff.push_back(poff);
+                poff += 2 * eb;
+                sl.moff.push_back(maj ? -1 : moff);
+                if (!maj) moff += static_cast<long long>(Ng) * sl.count[k] * nang;
+            }
+            max_nodes = std::max<long long>(max_nodes, moff);  // packed values of the minority
             pslot = std::max<long long>(pslot, poff);
             slabs_.push_back(std::move(sl));
         }
         upload(map_nodes_, lists);
         upload(map_yzw_, weights);
-        for (auto& b : cbuf_) b.ensure(static_cast<size_t>(Ng) * max_nodes * nang * sizeof(T));
-        if (P_->secondary)  // the secondary's stream packs into its own pair
-            for (auto& b : cbuf2_) b.ensure(static_cast<size_t>(Ng) * max_nodes * nang * sizeof(T));
+        const size_t packed_bytes = static_cast<size_t>(max_nodes) * sizeof(T);
+        for (auto& b : cbuf_) b.ensure(packed_bytes);
+        if (P_->secondary) {  // the secondary's stream packs into its own pair
+            for (auto& b : cbuf2_) b.ensure(packed_bytes);
+            for (auto& b : xm_) b.ensure(packed_bytes);  // packed end-of-step copies and sources
+            for (auto& b : qm_) b.ensure(packed_bytes);
+        }
         pslot_ = static_cast<int>(pslot);
         partial_p_.ensure(4 * static_cast<size_t>(pslot) * sizeof(double));
         partial_s_.ensure(static_cast<size_t>(pslot) * sizeof(double));
@@ -999,12 +1013,12 @@ private:
     }
 
     // One depth step under a material map: step() with every node-local
-    // operator run per material on its packed nodes, on the same two streams
-    // and events (each stream packs into its own scratch).  The
-    // end-of-step primary copy and the sources live packed (material k of
-    // the slab at element offset off[k]), as the source and the secondary
-    // sweeps read them; the dose partials of material k sit at poff[k] of a
-    // slot ([dose: eb][proxy: eb]), summed per material and then over the
+    // operator run per material, on the same two streams and events (each
+    // stream packs into its own scratch).  The slab's largest material runs
+    // in place over the whole state, the others on their packed nodes (end-
+    // of-step copies and sources too: packed at moff[k], the largest's full
+    // size); the dose partials of material k sit at poff[k] of a slot
+    // ([dose: eb][proxy: eb]), summed per material and then over the
     // materials in order.
     void step_mapped(bool first, bool last) {
         const int n = step_, j = slab_of(n);
@@ -1021,10 +1035,14 @@ private:
         const bool secondary = P_->secondary;
         const bool preserve_primary = phys.no_spatial_clipping != 0;
         const bool strag = phys.eq15_straggling != 0;
-        // a slab of one material lists every node in order: its operators
-        // work on the state itself, as in a homogeneous run
-        const bool whole = nk == 1;
+        // The material with the most nodes (the only one, in a slab of one
+        // material) runs in place over the whole state; before it does, the
+        // other materials' nodes are copied out packed and set to zero, so
+        // its sweeps give them nothing (no dose, no proxy, no source, no top),
+        // and afterwards their own packed results are written back.
+        const int km = sl.major;
         auto use = [&](int k) { P_ = mats_[sl.mat[k]]; };
+        auto minor = [&](int k) { return k != km; };
         auto wait_source = [&] {
             if (src_pending_) {
                 BFPN_HIP(hipStreamWaitEvent(sp, ev_src_, 0));
@@ -1032,19 +1050,40 @@ private:
             }
         };
         auto nodes = [&](int k) { return map_nodes_.as<int>() + static_cast<long long>(j) * P_->nyz + sl.first[k]; };
-        auto yzw = [&](int k) { return map_yzw_.as<double>() + static_cast<long long>(j) * P_->nyz + sl.first[k]; };
-        auto lanes = [&](int k) { return static_cast<long long>(sl.count[k]) * P_->nang; };
-        auto packed = [&](const DeviceBuffer& b, int k) { return b.as<T>() + sl.off[k]; };
-        auto part = [&](double* base, int k) { return base ? base + sl.poff[k] : nullptr; };
-        // the packed arrays of material k: gathered from / scattered to state w on stream s
-        auto gather = [&](int k, T* c, int w, hipStream_t s) {
-            if (whole) return cur_[w];
-            launch_gather_nodes<T>(c, cur_[w], nodes(k), sl.count[k], P_->nyz, P_->nang, Ng, s);
-            return c;
+        auto nodes_of = [&](int k) { return static_cast<long long>(minor(k) ? sl.count[k] : P_->nyz); };
+        auto lanes = [&](int k) { return nodes_of(k) * P_->nang; };
+        auto weights = [&](int k) {
+            return minor(k) ? map_yzw_.as<double>() + static_cast<long long>(j) * P_->nyz + sl.first[k]
+                            : P_->yzw.template as<double>();
         };
-        auto scatter = [&](int k, const T* c, int w, hipStream_t s) {
-            if (!whole)
-                launch_scatter_nodes<T>(cur_[w], c, nodes(k), sl.count[k], P_->nyz, P_->nang, Ng, s);
+        auto part = [&](double* base, int k) { return base ? base + sl.poff[k] : nullptr; };
+        auto xbuf = [&](int i, int k) { return minor(k) ? xm_[i].as<T>() + sl.moff[k] : x_[i].as<T>(); };
+        auto qbuf = [&](int slot, int k) {
+            return minor(k) ? qm_[slot].as<T>() + sl.moff[k] : q_[slot].as<T>();
+        };
+        // operand i of material k on stream set sc: its packed copy, or the state itself
+        auto arr = [&](int sc, int i, int k, int w) {
+            return minor(k) ? scratch[sc][i] + sl.moff[k] : cur_[w];
+        };
+        auto split = [&](int sc, const int* ws, int nw) {  // copy out, then zero, the minority nodes
+            for (int k = 0; k < nk; ++k) {
+                if (!minor(k)) continue;
+                use(k);
+                for (int i = 0; i < nw; ++i)
+                    launch_gather_nodes<T>(arr(sc, i, k, ws[i]), cur_[ws[i]], nodes(k), sl.count[k],
+                                           P_->nyz, P_->nang, Ng, st[sc]);
+                for (int i = 0; i < nw; ++i)
+                    launch_zero_nodes<T>(cur_[ws[i]], nodes(k), sl.count[k], P_->nyz, P_->nang, Ng, st[sc]);
+            }
+        };
+        auto merge = [&](int sc, const int* ws, int nw) {
+            for (int k = 0; k < nk; ++k) {
+                if (!minor(k)) continue;
+                use(k);
+                for (int i = 0; i < nw; ++i)
+                    launch_scatter_nodes<T>(cur_[ws[i]], arr(sc, i, k, ws[i]), nodes(k), sl.count[k],
+                                            P_->nyz, P_->nang, Ng, st[sc]);
+            }
         };
         // The tops of one sweep (sweep_tops advances the species' ring once);
         // each material's launch publishes into the same slots (atomic
@@ -1054,39 +1093,35 @@ private:
             if (t.qmap) t.qmap = P_->qmap.template as<int>();
             return t;
         };
-        // single energy half of a species; Q: packed sources or none
-        auto half = [&](int sc, const DeviceBuffer* Q, double* partials, bool proxy,
-                        const SweepTops& tops) {
-            hipStream_t s = st[sc];
+        // single energy half of species sc; qs: the source slot or -1
+        auto half = [&](int sc, int qs, double* partials, bool proxy, const SweepTops& tops) {
+            const int ws[2] = {2 * sc, 2 * sc + 1};
+            split(sc, ws, 2);
             for (int k = 0; k < nk; ++k) {
                 use(k);
-                T* a = gather(k, scratch[sc][0], 2 * sc, s);
-                T* b = gather(k, scratch[sc][1], 2 * sc + 1, s);
-                launch_energy_half<T>(a, b, P_->coef.template as<EnergyCoef>(), Q ? packed(*Q, k) : nullptr,
-                                      Ng, lanes(k), strag, yzw(k), P_->nang, part(partials, k), proxy, s,
-                                      own(tops));
-                scatter(k, a, 2 * sc, s);
-                scatter(k, b, 2 * sc + 1, s);
+                launch_energy_half<T>(arr(sc, 0, k, ws[0]), arr(sc, 1, k, ws[1]),
+                                      P_->coef.template as<EnergyCoef>(), qs >= 0 ? qbuf(qs, k) : nullptr,
+                                      Ng, lanes(k), strag, weights(k), P_->nang, part(partials, k), proxy,
+                                      st[sc], own(tops));
             }
-            mark(kEnergy, array_bytes() * (Q ? 5 : 4), s);
+            merge(sc, ws, 2);
+            mark(kEnergy, array_bytes() * (qs >= 0 ? 5 : 4), st[sc]);
         };
-        // paired energy sweep; X: packed end-of-step copies (primary) or none
-        auto pair = [&](int sc, const DeviceBuffer* Qa, const DeviceBuffer* Qb, double* partials,
-                        bool proxy, bool xout, const SweepTops& tops) {
-            hipStream_t s = st[sc];
+        // paired energy sweep; xout: the end-of-step copies (primary)
+        auto pair = [&](int sc, int qa, int qb, double* partials, bool proxy, bool xout,
+                        const SweepTops& tops) {
+            const int ws[2] = {2 * sc, 2 * sc + 1};
+            split(sc, ws, 2);
             for (int k = 0; k < nk; ++k) {
                 use(k);
-                T* a = gather(k, scratch[sc][0], 2 * sc, s);
-                T* b = gather(k, scratch[sc][1], 2 * sc + 1, s);
-                launch_energy_pair<T>(a, b, P_->coef.template as<EnergyCoef>(),
-                                      Qa ? packed(*Qa, k) : nullptr, Qb ? packed(*Qb, k) : nullptr, Ng,
-                                      lanes(k), strag, yzw(k), P_->nang, part(partials, k), proxy,
-                                      xout ? packed(x_[0], k) : nullptr, xout ? packed(x_[1], k) : nullptr,
-                                      s, own(tops), x_rows());
-                scatter(k, a, 2 * sc, s);
-                scatter(k, b, 2 * sc + 1, s);
+                launch_energy_pair<T>(arr(sc, 0, k, ws[0]), arr(sc, 1, k, ws[1]),
+                                      P_->coef.template as<EnergyCoef>(), qa >= 0 ? qbuf(qa, k) : nullptr,
+                                      qb >= 0 ? qbuf(qb, k) : nullptr, Ng, lanes(k), strag, weights(k),
+                                      P_->nang, part(partials, k), proxy, xout ? xbuf(0, k) : nullptr,
+                                      xout ? xbuf(1, k) : nullptr, st[sc], own(tops), x_rows());
             }
-            mark(kEnergy, array_bytes() * (4 + (Qa ? 2 : 0) + (xout ? 2 : 0)), s);
+            merge(sc, ws, 2);
+            mark(kEnergy, array_bytes() * (4 + (qa >= 0 ? 2 : 0) + (xout ? 2 : 0)), st[sc]);
         };
         auto tr = [&](int w, bool preserve) {
             hipStream_t s = st[w / 2];
@@ -1095,43 +1130,44 @@ private:
             mark(kTransport, array_bytes() * 2, s);
         };
         auto ang = [&](int w) {
-            hipStream_t s = st[w / 2];
+            const int sc = w / 2;
+            split(sc, &w, 1);
             for (int k = 0; k < nk; ++k) {
                 use(k);
-                T* a = gather(k, scratch[w / 2][0], w, s);
+                T* a = arr(sc, 0, k, w);
+                const int planes_nodes = static_cast<int>(nodes_of(k));
                 bool done = false;
                 if constexpr (std::is_same_v<T, float>) {
                     if (P_->angle_plan.plan.su_h) {
                         launch_angle_mfma(a, P_->factors.template as<float>(), P_->angle_plan.plan, Ng,
-                                          sl.count[k], grid.Nmu, grid.Nom, s, cur_top(w / 2));
+                                          planes_nodes, grid.Nmu, grid.Nom, st[sc], cur_top(sc));
                         done = true;
                     }
                 }
                 if (!done)
                     launch_angle_full<T>(a, P_->factors.template as<T>(), P_->su.template as<T>(),
-                                         P_->sv.template as<T>(), Ng, sl.count[k], grid.Nmu, grid.Nom, s,
-                                         cur_top(w / 2));
-                scatter(k, a, w, s);
+                                         P_->sv.template as<T>(), Ng, planes_nodes, grid.Nmu, grid.Nom,
+                                         st[sc], cur_top(sc));
             }
-            mark(kAngle, array_bytes() * 2, s);
+            merge(sc, &w, 1);
+            mark(kAngle, array_bytes() * 2, st[sc]);
         };
-        // the source of material k from its packed primary (Fp, fp) into Q
-        auto src = [&](int k, const T* Fp, const T* fp, T* Q) {  // the source of step n
+        // the source of step n for material k from (Fp, fp) into Q, on ss
+        auto src = [&](int k, const T* Fp, const T* fp, T* Q) {
             const int* xt = skip_ ? x_top(n) : nullptr;
             const int* qm = skip_ ? P_->qmap.template as<int>() : nullptr;
-            hipStream_t s = ss;
+            const int nn = static_cast<int>(nodes_of(k));
             if constexpr (std::is_same_v<T, float>) {
-                launch_source_mfma(Fp, fp, a1_.as<float>(), a2_.as<float>(), Q, P_->plan.plan, Ng,
-                                   sl.count[k], P_->nang, s, nullptr, xt, qm, P_->qcap);
+                launch_source_mfma(Fp, fp, a1_.as<float>(), a2_.as<float>(), Q, P_->plan.plan, Ng, nn,
+                                   P_->nang, ss, nullptr, xt, qm, P_->qcap);
             } else {
                 const int* need = P_->need.template as<int>();
                 launch_source_angular<T>(Fp, fp, P_->kv.template as<T>(), a1_.as<T>(), a2_.as<T>(), Ng,
-                                         sl.count[k], P_->nang, s, xt,
-                                         P_->kcount.template as<int>(), P_->klist.template as<int>(),
-                                         need);
+                                         nn, P_->nang, ss, xt, P_->kcount.template as<int>(),
+                                         P_->klist.template as<int>(), need);
                 launch_source_energy<T>(a1_.as<T>(), a2_.as<T>(), P_->e1.template as<double>(),
                                         P_->e2.template as<double>(), P_->beg.template as<int>(),
-                                        P_->end.template as<int>(), Q, Ng, lanes(k), d.dg, s, xt, qm,
+                                        P_->end.template as<int>(), Q, Ng, lanes(k), d.dg, ss, xt, qm,
                                         need);
             }
         };
@@ -1156,7 +1192,7 @@ private:
         if (first) {
             wait_source();
             mark(kWait, 0.0, sp);
-            half(0, nullptr, nullptr, false, sweep_tops(0, -1, -1, -1));
+            half(0, -1, nullptr, false, sweep_tops(0, -1, -1, -1));
         }
         if (!phys.energy_only) {
             if (!phys.no_transport) {
@@ -1173,11 +1209,11 @@ private:
             }
         }
         if (last) {
-            half(0, nullptr, pp, true, sweep_tops(0, -1, -1, n));
+            half(0, -1, pp, true, sweep_tops(0, -1, -1, n));
         } else {
             wait_source();  // the end-of-step copies are about to be rewritten
             if (!first) mark(kWait, 0.0, sp);
-            pair(0, nullptr, nullptr, pp, true, secondary, sweep_tops(0, -1, -1, n));
+            pair(0, -1, -1, pp, true, secondary, sweep_tops(0, -1, -1, n));
         }
         if (two) {
             BFPN_HIP(hipEventRecord(ev_prim_, sp));
@@ -1186,26 +1222,37 @@ private:
         }
         const bool sec = secondary && n > 0;
         if (sec) {
-            DeviceBuffer& Q = q_[n & 1];
+            const int qs = n & 1;
             for (int k = 0; k < nk; ++k) {  // the source of step n, on ss
                 use(k);
+                const T* a = xbuf(0, k);
+                const T* b = xbuf(1, k);
                 if (last) {  // from the state; otherwise from the end-of-step copy
-                    const T* a = gather(k, scratch[1][0], 0, ss);
-                    const T* b = gather(k, scratch[1][1], 1, ss);
-                    src(k, a, b, packed(Q, k));
-                } else {
-                    src(k, packed(x_[0], k), packed(x_[1], k), packed(Q, k));
+                    a = arr(1, 0, k, 0);
+                    b = arr(1, 1, k, 1);
+                    if (minor(k)) {
+                        launch_gather_nodes<T>(arr(1, 0, k, 0), cur_[0], nodes(k), sl.count[k], P_->nyz,
+                                               P_->nang, Ng, ss);
+                        launch_gather_nodes<T>(arr(1, 1, k, 1), cur_[1], nodes(k), sl.count[k], P_->nyz,
+                                               P_->nang, Ng, ss);
+                    }
                 }
+                src(k, a, b, qbuf(qs, k));
             }
+            for (int k = 0; k < nk; ++k)  // the in-place source holds nothing at the other nodes
+                if (minor(k)) {
+                    use(k);
+                    launch_zero_nodes<T>(q_[qs].as<T>(), nodes(k), sl.count[k], P_->nyz, P_->nang, Ng, ss);
+                }
             if (two) BFPN_HIP(hipEventRecord(ev_src_, ss));  // its inputs are read
             src_pending_ = two;
             mark(kSource, array_bytes() * 7, ss);
             if (sec_open_) {
-                pair(1, &q_[(n - 1) & 1], &Q, partial_s_.as<double>(), false, false,
+                pair(1, (n - 1) & 1, qs, partial_s_.as<double>(), false, false,
                      sweep_tops(1, n - 1, n, -1));
                 finish(n - 1, true);
             } else {
-                half(1, &Q, nullptr, false, sweep_tops(1, n, -1, -1));
+                half(1, qs, nullptr, false, sweep_tops(1, n, -1, -1));
             }
             tr(2, false);
             tr(3, true);
@@ -1214,7 +1261,7 @@ private:
             tr(2, false);
             tr(3, true);
             if (last) {
-                half(1, &Q, partial_s_.as<double>(), false, sweep_tops(1, n, -1, -1));
+                half(1, qs, partial_s_.as<double>(), false, sweep_tops(1, n, -1, -1));
                 finish(n, true);
                 sec_open_ = false;
             } else {
@@ -1228,7 +1275,8 @@ private:
 
     struct MapSlab {                     // one depth slab of a material map
         std::vector<int> mat, count, first;  // per material present: index, nodes, first list entry
-        std::vector<long long> off;      // element offset of its packed arrays
+        int major = 0;                   // the material run in place over every node
+        std::vector<long long> moff;     // the others: element offset of their packed arrays
         std::vector<int> eb, poff;       // dose blocks of its sweeps, offset of its partials
     };
 
@@ -1239,6 +1287,7 @@ private:
     std::vector<double> slab_end_;
     std::vector<MapSlab> slabs_;
     DeviceBuffer map_nodes_, map_yzw_, map_tmp_, cbuf_[2], cbuf2_[2];  // node lists and weights, slab-major
+    DeviceBuffer xm_[2], qm_[2];  // packed end-of-step copies and sources of the minority materials
     int pslot_ = 0;                                           // doubles per partial slot
     DeviceBuffer state_[4], a1_, a2_, q_[2], x_[2];
     DeviceBuffer diag_part_, diag_out_;  // diagnostics scratch, allocated on first use
@@ -1342,6 +1391,20 @@ void validate_map(const std::vector<LayerSpec>& materials, const std::vector<dou
     BFPN_REQUIRE(map.size() == slab_end.size() * static_cast<size_t>(nyz),
                  "the map needs (Ny+1)(Nz+1) material indices per slab");
     for (unsigned char m : map) BFPN_REQUIRE(m < materials.size(), "map entry names no material");
+}
+
+size_t map_bytes(const bfpn_grid& g, const bfpn_phys& p, const std::vector<unsigned char>& map,
+                 int nmat, int precision) {
+    const size_t nyz = static_cast<size_t>(g.Ny + 1) * (g.Nz + 1);
+    size_t minority = 0;  // the most nodes outside the largest material, over the slabs
+    for (size_t j = 0; nyz && j + 1 <= map.size() / nyz; ++j) {
+        std::vector<size_t> count(static_cast<size_t>(std::max(nmat, 1)), 0);
+        for (size_t s = 0; s < nyz; ++s) ++count[std::min<size_t>(map[j * nyz + s], count.size() - 1)];
+        minority = std::max(minority, nyz - *std::max_element(count.begin(), count.end()));
+    }
+    const bool sec = !p.primary_only && !p.energy_only && !p.lite_memory;
+    const size_t word = precision == BFPN_FP64 ? 8 : 4;
+    return (sec ? 8 : 2) * minority * g.Ng * g.Nmu * g.Nom * word;
 }
 
 bfpn_ctx* make_mapped_context(const bfpn_grid& g, const bfpn_phys& p,
@@ -1904,9 +1967,8 @@ BFPN_API bfpn_status bfpn_create_mapped(const bfpn_grid* grid, const bfpn_phys* 
                      "material maps support the eq15 energy model with the full state only");
         size_t free_b = 0, total_b = 0;
         BFPN_HIP(hipMemGetInfo(&free_b, &total_b));
-        const size_t word = precision == BFPN_FP64 ? 8 : 4;
         const size_t need = context_bytes(*grid, *phys, precision) +
-                            4 * static_cast<size_t>(grid->Ng) * nyz * grid->Nmu * grid->Nom * word +
+                            map_bytes(*grid, *phys, m, nmat, precision) +
                             nmat * physics_bytes(*grid, *phys, precision) + (64ull << 20);
         if (need > free_b)
             throw Error(BFPN_E_NOMEM, "Requested grid and " + std::to_string(nmat) +

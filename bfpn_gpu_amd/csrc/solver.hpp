@@ -103,6 +103,11 @@ bfpn_phys layer_phys(const bfpn_phys& p, const LayerSpec& l);
 // layer_phys(p, material) and its data path).
 void validate_map(const std::vector<LayerSpec>& materials, const std::vector<double>& slab_end,
                   const std::vector<unsigned char>& map, long long nyz);
+// Device bytes a material map adds to a context: packed copies of the nodes
+// of every material but the largest one of each slab (two per stream, and the
+// end-of-step copies and sources with a secondary).
+size_t map_bytes(const bfpn_grid& g, const bfpn_phys& p, const std::vector<unsigned char>& map,
+                 int nmat, int precision);
 bfpn_ctx* make_mapped_context(const bfpn_grid& g, const bfpn_phys& p,
                               const std::vector<LayerSpec>& materials,
                               const std::vector<double>& slab_end,
